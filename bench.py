@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Benchmark of the cloth substep on one MI355X.
+
+Workload (default): 256 independent cloth instances, grid side round(32*6^u) (log-uniform
+in [32, 192]), per-instance stretch / bend compliance log-uniform in [1e-9, 1e-6] /
+[1e-5, 1e-3], 10 substeps x 10 iterations per frame at 60 fps, ground plane, FP32,
+synthetic from seed.  A step is one frame (10 substeps) of all instances.
+
+value        batched vertex-substeps/s (one persistent launch per substep, one hipGraph
+             per frame), inputs resident in HBM
+plain_value  the same workload one instance at a time through per-color launches
+roofline     dominant kernel = the batched substep kernel; algorithmic bytes per launch
+             = sum_i (64 N_i + 16 E_i + 24 B_i + 12 K_i); HBM peak 8 TB/s
+cpu_baseline the serial FP64 oracle on one core, one substep per instance over a bounded
+             sample; the same leg reports the single-substep relative error of the GPU's
+             batched substep against it (parity at the measured configuration)
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload batched|hanging]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(solver, insts) -> tuple[int, int]:
+    """(bytes per substep launch, vertices): state read+write (64 B/vertex), stretch
+    16 B, bend 24 B and LRA 12 B per constraint."""
+    total, nv = 0, 0
+    for i in insts:
+        c = solver.counts(i)
+        total += 64 * c.nv + 16 * c.n_edges + 24 * c.n_bends + 12 * c.lra_constraints
+        nv += c.nv
+    return total, nv
+
+
+def cpu_baseline_and_parity(specs, params, seed, budget_s=12.0):
+    """CPU baseline leg: the serial FP64 oracle (1 core) advances each instance by one
+    substep from the initial state until the time budget is spent; the GPU's batched
+    substep from the same state is compared with it (relative error)."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from gpu_helpers import oracle_of, rel_err
+    from oracle import oracle as orc
+    from xrtailor_amd.solver import build_solver
+
+    solver, insts = build_solver(specs, params, seed=seed)
+    solver.SubstepBatched()
+    solver.sync()
+    op, dt = orc.params_from(params)
+    done_vs, t_used, n_inst, worst = 0, 0.0, 0, 0.0
+    for i, sp in zip(insts, specs):
+        oc = oracle_of(solver, i, sp.stretch_compliance, sp.bend_compliance,
+                       x0=sp.x0, v=np.zeros_like(sp.x0))
+        t0 = time.perf_counter()
+        oc.substep(op, dt)
+        t_used += time.perf_counter() - t0
+        worst = max(worst, rel_err(solver.GetState(i)[0], oc.state()[0]))
+        done_vs += oc.nv
+        n_inst += 1
+        if t_used >= budget_s:
+            break
+    solver.close()
+    base = {"value": done_vs / t_used, "unit": "vertex-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"one substep of the first {n_inst} of {len(specs)} instances "
+                      f"({done_vs} vertex-substeps, {t_used:.1f} s), serial FP64 oracle; "
+                      f"{os.cpu_count()} host cores present"}
+    return base, worst, n_inst
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="batched", choices=["batched", "hanging"])
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--plain-frames", type=int, default=1)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build measures one GPU only (instances are independent; "
+                                   "N GPUs would each run their own share): use --gpus 1"}))
+        return 2
+
+    from xrtailor_amd import scenes
+    from xrtailor_amd.solver import build_solver, make_params
+
+    if a.workload == "batched":
+        specs = scenes.batched(seed=a.seed, n_instances=a.instances)
+        wname = f"batched {a.instances} cloths, side round(32*6^u) in [32,192], seed {a.seed}"
+    else:
+        specs = [scenes.hanging_cloth(seed=a.seed)]
+        wname = "hanging cloth 64x64, 1 m, two corners pinned"
+    params = make_params(bend_model=1, ground_height=-1.0)
+    print(f"[bench] building {len(specs)} instances", file=sys.stderr, flush=True)
+    solver, insts = build_solver(specs, params, seed=a.seed)
+    bytes_sub, n_verts = algorithmic_bytes(solver, insts)
+    info = solver.device_info()
+
+    # batched: warmup, then K timed frames (graph replays), HIP events on the solver stream
+    solver.FramesBatched(a.warmup)
+    solver.sync()
+    t0 = time.perf_counter()
+    ms = solver.time_frames(True, a.steps)
+    wall = time.perf_counter() - t0
+    value = n_verts * params.num_substeps * a.steps / (ms * 1e-3)
+    print(f"[bench] batched {ms:.2f} ms / {a.steps} frames (wall {wall:.3f} s)",
+          file=sys.stderr, flush=True)
+
+    # dominant kernel: per-launch device time from HIP events around each launch
+    kern_ms = solver.time_batched_substep(max(10, a.steps))
+    achieved = bytes_sub / (kern_ms * 1e-3) / 1e9
+
+    # plain path on the same workload
+    solver.FramesPlain(1)
+    solver.sync()
+    pms = solver.time_frames(False, a.plain_frames)
+    plain = n_verts * params.num_substeps * a.plain_frames / (pms * 1e-3)
+    print(f"[bench] plain {pms:.1f} ms / {a.plain_frames} frame(s)", file=sys.stderr, flush=True)
+    solver.close()
+
+    cpu, parity, parity_n = None, None, 0
+    if not a.no_cpu:
+        cpu, parity, parity_n = cpu_baseline_and_parity(specs, params, a.seed, a.cpu_budget)
+        print(f"[bench] cpu {cpu['value']:.3e} vs/s, parity {parity:.3e}", file=sys.stderr,
+              flush=True)
+
+    out = {
+        "metric": "vertex-substeps/s batched vs plain on 1 GPU at fixed err; adjoint:forward ratio",
+        "value": value,
+        "unit": "vertex-substeps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wname, "instances": len(specs), "vertices": n_verts,
+                   "substeps": params.num_substeps, "iterations": params.num_iterations,
+                   "bend_model": "consistent", "step": "one frame of all instances"},
+        "plain_value": plain,
+        "batched_over_plain": value / plain,
+        "parity_rel_err": parity,
+        "parity_instances": parity_n,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "BatchedSubstepKernel", "kernel_ms": kern_ms,
+                     "bytes_per_launch": bytes_sub},
+        "cpu_baseline": cpu,
+        "device": info,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

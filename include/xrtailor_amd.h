@@ -1,0 +1,175 @@
+/*
+ * xrtailor_amd.h -- C ABI of the MI355X (gfx950) cloth-substep solver.
+ *
+ * The drop-in boundary for XRTailor's per-substep solver.  Each entry point names the
+ * reference interface it replaces (paths relative to the XRTailor repository root).
+ * Data handles change from the reference's `Node**` / `Edge**` / `Face**` device
+ * pointer arrays to an opaque context that owns the device state of one or more cloth
+ * instances (an instance = one cloth mesh plus its optional obstacle mesh).
+ *
+ * Conventions
+ *  - Every function returns an int status: XRT_OK (0) or a negative XRT_ERR_* code.
+ *    Nothing calls exit(); the last error text is available from xrt_last_error().
+ *  - Inputs are host arrays copied in; outputs go to caller-owned host buffers.
+ *    Vectors are packed float[3*n] (x,y,z), triangles int32[3*nt].
+ *  - All device work is asynchronous on the context's own HIP stream (default
+ *    priority).  Getters synchronise that stream.  One context per host thread; there
+ *    is no internal locking.
+ */
+#ifndef XRTAILOR_AMD_H
+#define XRTAILOR_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XRT_ABI_VERSION 1
+
+enum {
+  XRT_OK = 0,
+  XRT_ERR_INVALID_ARG = -1,  /* bad pointer / size / index                        */
+  XRT_ERR_HIP = -2,          /* a HIP runtime call failed                         */
+  XRT_ERR_STATE = -3,        /* call out of order (e.g. solve before xrt_start)   */
+  XRT_ERR_CAPACITY = -4,     /* a fixed-capacity buffer overflowed                */
+  XRT_ERR_NOT_FINITE = -5,   /* NaN/Inf detected in the state                     */
+  XRT_ERR_UNSUPPORTED = -6   /* requested feature not available on this device    */
+};
+
+/* Simulation parameters; mirrors the fields of SimParams the substep reads
+ * (xrtailor/include/xrtailor/core/Common.hpp:31-168) plus the scene constants the
+ * reference keeps in its config (frame time: Timer::FixedDeltaTime, utils/Timer.hpp:129-138). */
+typedef struct xrt_params {
+  float gravity[3];              /* SimParams::gravity                         */
+  float damping;                 /* SimParams::damping (0.98)                  */
+  float relaxation_factor;       /* SimParams::relaxation_factor (0.25)        */
+  float long_range_stretchiness; /* SimParams::long_range_stretchiness (1.0)   */
+  float bvh_tolerance;           /* SimParams::bvh_tolerance (swift 0.003)     */
+  float ground_height;           /* plane y (SDFCollider type 1)               */
+  float ground_friction;         /* SimParams::sdf_friction on the plane       */
+  float pbd_thickness;           /* self-contact thickness h                   */
+  float repulsion_relaxation;    /* PBD repulsion relaxation_rate              */
+  float frame_time;              /* seconds per frame (1/60)                   */
+  int32_t num_substeps;          /* SimParams::num_substeps                    */
+  int32_t num_iterations;        /* SimParams::num_iterations                  */
+  int32_t num_collision_passes;  /* SimParams::num_collision_passes            */
+  int32_t solve_bending;         /* SimParams::solve_bending                   */
+  int32_t enable_ground;
+  int32_t enable_obstacle_contact;
+  int32_t enable_self_contact;
+  int32_t pair_update_interval;  /* iterations between pair refreshes (10)     */
+  /* XRT_BEND_REFERENCE: SolveBend_Kernel as written (BasicSolverHelper.cu:8-72): d clamped to
+   * [0,1], rest angle from the other normal convention, step along +q.
+   * XRT_BEND_CONSISTENT: rest angle pi - theta0 (the solver's own normals), d clamped to
+   * [-1,1], PBD step along -q (Mueller 2007).  Same cost; stable for small compliance. */
+  int32_t bend_model;
+} xrt_params;
+
+enum { XRT_BEND_REFERENCE = 0, XRT_BEND_CONSISTENT = 1 };
+
+typedef struct xrt_ctx xrt_ctx;
+
+/* ------------------------------------------------------------------ context */
+int xrt_abi_version(void);
+const char* xrt_last_error(void);
+void xrt_default_params(xrt_params* p);
+/* creates a context on HIP device `device` with its own stream */
+int xrt_create(xrt_ctx** out, int device);
+int xrt_destroy(xrt_ctx* ctx);
+/* SetSimulationParams (xrtailor/src/physics/ClothSolverHelper.cu:16-23) */
+int xrt_set_params(xrt_ctx* ctx, const xrt_params* p);
+int xrt_get_params(xrt_ctx* ctx, xrt_params* p);
+int xrt_sync(xrt_ctx* ctx);
+/* device properties: [0]=compute units, [1]=max LDS bytes per workgroup,
+ * [2]=wavefront size, [3]=L2 bytes */
+int xrt_device_info(xrt_ctx* ctx, int64_t info[4]);
+
+/* ------------------------------------------------- host-only setup helpers */
+/* These run on the host and need no device; xrt_add_cloth / xrt_start use the same code. */
+/* stretch edges (PrepareBasicStretchData, xrtailor/src/physics/ClothSolverHelper.cpp:90-109);
+ * *ne_out receives the edge count, XRT_ERR_CAPACITY if it exceeds max_edges */
+int xrt_build_edges(const int32_t* tris, int32_t nt, int32_t* edges_out, int32_t max_edges,
+                    int32_t* ne_out);
+/* bend elements (GenerateBendingElements, ClothSolverHelper.cpp:4-88) */
+int xrt_build_bends(const int32_t* tris, int32_t nt, const int32_t* edges, int32_t ne,
+                    int32_t* bends_out, int32_t max_bends, int32_t* nb_out);
+/* constraint coloring (GraphColoring::Paint, xrtailor/src/physics/graph_coloring/Base.cu:163-317) */
+int xrt_color_constraints(const int32_t* cons, int32_t arity, int32_t n, int32_t nv,
+                          uint64_t seed, int32_t* colors_out, int32_t* ncolors_out);
+
+/* -------------------------------------------------------------- scene setup */
+/* ClothSolverUniversal::AddCloth (xrtailor/src/pipeline/impl/universal/ClothSolver.cpp:247-350):
+ * registers a cloth instance, builds its stretch edges (PrepareBasicStretchData,
+ * xrtailor/src/physics/ClothSolverHelper.cpp:90-109) and bend elements
+ * (GenerateBendingElements, ClothSolverHelper.cpp:4-88) with rest lengths / angles from
+ * `pos`, and sets uniform stretch and bend compliance (BasicSolver.cu:42-70,109-125). */
+int xrt_add_cloth(xrt_ctx* ctx, const float* pos, int32_t nv, const int32_t* tris, int32_t nt,
+                  float stretch_compliance, float bend_compliance, int32_t* instance_out);
+/* pinning: PhysicsMesh::SetInvMass(0) + SetFree(false) (universal/ClothSolver.cpp:304-307) */
+int xrt_pin_vertices(xrt_ctx* ctx, int32_t inst, const int32_t* idx, int32_t n);
+/* per-edge stretch compliance, in the edge order xrt_get_edges reports */
+int xrt_set_stretch_compliance(xrt_ctx* ctx, int32_t inst, const float* compliance);
+/* geodesic long-range attachments from slot vertices: Geodesic::ComputeGeodesicDistance +
+ * ClothObjectBase::GenerateGeodesicLongRangeConstraints
+ * (xrtailor/src/utils/Geodesic.cpp:6-79; xrtailor/src/pipeline/base/ClothObjectBase.cpp:80-94) */
+int xrt_add_lra_slots(xrt_ctx* ctx, int32_t inst, const int32_t* slots, int32_t n);
+/* ClothSolverBase::Start / SetupInternalDynamics (ClothSolverBase.cpp:453-487): colors the
+ * stretch and bend constraints of every instance from `seed` and uploads the scene. */
+int xrt_start(xrt_ctx* ctx, uint64_t seed);
+
+/* ---------------------------------------------------------------- state I/O */
+int xrt_num_instances(xrt_ctx* ctx, int32_t* n);
+/* counts[0..7] = nv, nt, n_edges, n_bends, stretch colors, bend colors, LRA slots, LRA constraints */
+int xrt_get_counts(xrt_ctx* ctx, int32_t inst, int32_t counts[8]);
+/* edges (2*ne), rest lengths, stretch colors -- any pointer may be NULL */
+int xrt_get_edges(xrt_ctx* ctx, int32_t inst, int32_t* edges, float* rest, int32_t* colors);
+/* bend elements (4*nb), rest angles, bend colors -- any pointer may be NULL */
+int xrt_get_bends(xrt_ctx* ctx, int32_t inst, int32_t* bends, float* theta, int32_t* colors);
+/* LRA: slot vertices (nslots), CSR over targets (nv+1), slot ordinals and rest lengths (K) */
+int xrt_get_lra(xrt_ctx* ctx, int32_t inst, int32_t* slot_vert, int32_t* ptr, int32_t* slot,
+                float* rest);
+int xrt_get_inv_mass(xrt_ctx* ctx, int32_t inst, float* w);
+int xrt_set_state(xrt_ctx* ctx, int32_t inst, const float* x0, const float* v);
+int xrt_get_state(xrt_ctx* ctx, int32_t inst, float* x0, float* v);
+/* predicted / iterate positions x (PhysicsMesh nodes' x) */
+int xrt_get_predicted(xrt_ctx* ctx, int32_t inst, float* x);
+
+/* ------------------------------------------ reference-shaped kernels (plain) */
+/* PredictPositions (xrtailor/include/xrtailor/physics/ClothSolverHelper.cuh:20-41) */
+int xrt_predict_positions(xrt_ctx* ctx, int32_t inst, float dt);
+/* BasicConstraint::StretchSolver::Solve(nodes, prev, color, dt, iter)
+ * (xrtailor/include/xrtailor/physics/dynamics/BasicSolver.cuh:35-54) */
+int xrt_stretch_solve(xrt_ctx* ctx, int32_t inst, int32_t color, float dt, int32_t iter);
+/* BasicConstraint::BendSolver::Solve(nodes, color, dt, iter) (BasicSolver.cuh:15-33) */
+int xrt_bend_solve(xrt_ctx* ctx, int32_t inst, int32_t color, float dt, int32_t iter);
+/* SolveGeodesicLRA (xrtailor/include/xrtailor/physics/dynamics/LongRangeAttachments.cuh:11-13) */
+int xrt_solve_lra(xrt_ctx* ctx, int32_t inst);
+/* ground plane projection (SDFCollider type 1, xrtailor/include/xrtailor/physics/sdf/SDFCollider.cuh:36-43) */
+int xrt_solve_ground(xrt_ctx* ctx, int32_t inst);
+/* ApplyDeltas (ClothSolverHelper.cuh:20-41) */
+int xrt_apply_deltas(xrt_ctx* ctx, int32_t inst);
+/* Finalize (ClothSolverHelper.cuh:20-41) */
+int xrt_finalize(xrt_ctx* ctx, int32_t inst, float dt);
+
+/* ------------------------------------------------------------------ drivers */
+/* one substep of one instance through the plain per-color launches, in the order of
+ * ClothSolverBase::SimulateSwift (xrtailor/src/pipeline/base/ClothSolverBase.cpp:190-240) */
+int xrt_substep_plain(xrt_ctx* ctx, int32_t inst);
+/* `nframes` frames of every instance, one instance after another (one cloth per launch
+ * sequence) -- the plain path of the batched workload */
+int xrt_frames_plain(xrt_ctx* ctx, int32_t nframes);
+/* one substep of all instances in one persistent launch (CSR instance table) */
+int xrt_substep_batched(xrt_ctx* ctx);
+/* `nframes` frames of all instances; the frame's substep launches are captured once in
+ * a hipGraph and replayed */
+int xrt_frames_batched(xrt_ctx* ctx, int32_t nframes);
+/* timing helper: average device time (ms) of the batched substep kernel over `n`
+ * launches, bracketed by HIP events on the context stream */
+int xrt_time_batched_substep(xrt_ctx* ctx, int32_t n, float* avg_ms);
+/* wall time (ms, HIP events on the context stream) of `nframes` plain or batched frames */
+int xrt_time_frames(xrt_ctx* ctx, int32_t batched, int32_t nframes, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

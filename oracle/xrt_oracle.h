@@ -1,0 +1,128 @@
+/*
+ * xrt_oracle.h -- serial CPU restatement of XRTailor's per-substep cloth solver.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load this library, and only as the checker (or as the timed
+ * CPU baseline).  The product path (xrtailor_amd/) never calls into it.
+ *
+ * Arithmetic: FP64 for the dynamics (the reference computes in FP32, `Scalar=float`,
+ * xrtailor/include/xrtailor/core/Scalar.hpp:26); FP32 with contraction off for the
+ * integer-producing LBVH stages so that Morton codes, sort order, topology and pair
+ * sets can be compared bit-exactly with the GPU.
+ *
+ * Every function cites the reference file:line it restates (paths relative to the
+ * reference repository root).
+ */
+#ifndef XRT_ORACLE_H
+#define XRT_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- topology */
+/* Unique edges in order of first appearance over faces f=0..F-1, local edges
+ * (0,1),(1,2),(2,0); (v0,v1) oriented as in the first face that holds the edge.
+ * Returns E, or -1 if max_edges is too small. */
+int orc_build_edges(const int32_t* tris, int nt, int32_t* edges_out, int max_edges);
+
+/* Dihedral bend elements, one per pair of faces sharing an interior edge, in edge
+ * order (restates GenerateBendingElements, xrtailor/src/physics/ClothSolverHelper.cpp:4-88).
+ * Each element is (v0, v1, v2, v3): (v0,v1) the shared edge in the first face's winding,
+ * v2 opposite in the first face, v3 opposite in the second. Returns B or -1. */
+int orc_build_bends(const int32_t* tris, int nt, const int32_t* edges, int ne,
+                    int32_t* bends_out, int max_bends);
+
+/* rest length |x_i - x_j| (ClothSolverHelper.cpp:90-109) and rest dihedral angle
+ * acos(clamp(N((v0-v2)x(v1-v2)) . N((v1-v3)x(v0-v3)), -1, 1)) (ClothSolverHelper.cpp:63-76). */
+void orc_rest_lengths(const double* x, const int32_t* edges, int ne, double* out);
+void orc_rest_angles(const double* x, const int32_t* bends, int nb, double* out);
+
+/* ---------------------------------------------------------------- coloring */
+/* Palette graph coloring of constraints (conflict = shared vertex), restating
+ * GraphColoring::Paint (xrtailor/src/physics/graph_coloring/Base.cu:12-146,163-317) and
+ * the adjacency builders (impl/StretchColoring.cu:6-58, GraphColoringHelper.cu:6-37),
+ * made deterministic: SplitMix64 per-constraint streams replace curand, and each
+ * round's conflict resolution reads the start-of-round state.
+ * cons: n x arity vertex ids.  Returns the number of colors used. */
+int orc_color_constraints(const int32_t* cons, int arity, int n, int nv, uint64_t seed,
+                          int32_t* colors_out);
+
+/* --------------------------------------------------------------- dynamics */
+typedef struct orc_params {
+  double gravity[3];
+  double damping;                 /* SimParams::damping (Common.hpp:43) */
+  double relaxation_factor;       /* ApplyDeltas relaxation (Common.hpp:45) */
+  double long_range_stretchiness; /* LRA stretchiness (Common.hpp:46) */
+  double ground_height;
+  double ground_friction;
+  int32_t num_iterations;
+  int32_t solve_bending;
+  int32_t enable_ground;
+  /* 0: the reference's bend formula (d clamped to [0,1], rest angle from the other normal
+   *    convention); 1: consistent convention (rest angle pi - theta0, d clamped to [-1,1]) */
+  int32_t bend_model;
+} orc_params;
+
+/* A cloth instance in the layout both sides share: constraints are listed in color
+ * order; color k of the stretch set is [s_color_off[k], s_color_off[k+1]). */
+typedef struct orc_cloth {
+  int32_t nv;
+  double* x0;        /* 3*nv  start-of-substep positions (in/out) */
+  double* v;         /* 3*nv  velocities (in/out) */
+  double* x;         /* 3*nv  predicted / iterate positions (scratch, out) */
+  const double* inv_mass; /* nv */
+  int32_t ns;                   /* stretch constraints */
+  const int32_t* s_idx;         /* 2*ns */
+  const double* s_rest;         /* ns */
+  const double* s_compliance;   /* ns */
+  double* s_lambda;             /* ns  XPBD multipliers (scratch) */
+  int32_t s_ncolors;
+  const int32_t* s_color_off;   /* s_ncolors+1 */
+  int32_t nb;                   /* bend elements */
+  const int32_t* b_idx;         /* 4*nb */
+  const double* b_theta;        /* nb */
+  const double* b_compliance;   /* nb */
+  int32_t b_ncolors;
+  const int32_t* b_color_off;   /* b_ncolors+1 */
+  int32_t nslots;               /* LRA slot vertices */
+  const int32_t* lra_slot_vert; /* nslots */
+  const int32_t* lra_ptr;       /* nv+1 CSR over target vertices */
+  const int32_t* lra_slot;      /* K  slot ordinal per constraint */
+  const double* lra_rest;       /* K  geodesic rest length */
+} orc_cloth;
+
+/* PredictPositions_Kernel (xrtailor/src/physics/ClothSolverHelper.cu:125-131) */
+void orc_predict(orc_cloth* c, const orc_params* p, double dt);
+/* SolveStretch_Kernel over color k (xrtailor/src/physics/dynamics/BasicSolverHelper.cu:80-133) */
+void orc_stretch_color(orc_cloth* c, int k, double dt, int iter);
+/* SolveBend_Kernel over color k (BasicSolverHelper.cu:8-72) */
+void orc_bend_color(orc_cloth* c, const orc_params* p, int k, double dt);
+/* The bend element's measured quantities for a 4-vertex hinge x[12]: d = clamp(n1.n2,0,1)
+ * before clamping in *d_raw, the Mueller gradients q[12] = dd/dx_k (BasicSolverHelper.cu:40-49).
+ * Returns 0 when the element is skipped (degenerate or angle < EPS). */
+int orc_bend_probe(const double* x12, double* d_raw, double* q12);
+/* SolveGeodesicLRA_Kernel (xrtailor/src/physics/dynamics/LongRangeAttachments.cu:5-26),
+ * per target vertex with slot positions read from the pre-pass state. */
+void orc_lra(orc_cloth* c, const orc_params* p);
+/* Ground plane y = h: SDFCollider plane correction (xrtailor/include/xrtailor/physics/sdf/
+ * SDFCollider.cuh:36-43, margin 0) with ComputeFriction (ClothSolverHelper.cu:156-169). */
+void orc_ground(orc_cloth* c, const orc_params* p);
+/* Finalize_Kernel (ClothSolverHelper.cu:222-229) */
+void orc_finalize(orc_cloth* c, const orc_params* p, double dt);
+/* One substep without contacts, in the order of ClothSolverBase::SimulateSwift
+ * (xrtailor/src/pipeline/base/ClothSolverBase.cpp:190-240). */
+void orc_substep(orc_cloth* c, const orc_params* p, double dt);
+
+/* Geodesic distances from one source over the mesh edge graph (Dijkstra, restating
+ * Geodesic::ComputeGeodesicDistance, xrtailor/src/utils/Geodesic.cpp:6-79). */
+void orc_geodesic(const double* x, int nv, const int32_t* edges, int ne, int src, double* dist_out);
+
+/* 64-bit SplitMix64 mixer shared with the library (test of the coloring streams). */
+uint64_t orc_splitmix64(uint64_t* state);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

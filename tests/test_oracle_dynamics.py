@@ -1,0 +1,280 @@
+"""Pins of the FP64 oracle's dynamics against closed forms and brute force (CPU only).
+
+Each test is chosen so that a plausible slip in the oracle (a dropped damping term, a
+wrong sign on a correction, a transposed cross product in the bend gradients, a missed
+epsilon) fails it, without restating the oracle's own formula.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from xrtailor_amd import scenes
+
+
+def _params(iters=10, bending=1, ground=0, g=(0.0, -9.81, 0.0), damping=0.98, h=-1.0, mu=0.0):
+    p = orc.OrcParams()
+    for k in range(3):
+        p.gravity[k] = g[k]
+    p.damping = damping
+    p.relaxation_factor = 0.25
+    p.long_range_stretchiness = 1.0
+    p.ground_height = h
+    p.ground_friction = mu
+    p.num_iterations = iters
+    p.solve_bending = bending
+    p.enable_ground = ground
+    return p
+
+
+def _cloth_from_mesh(pos, tris, x0=None, v=None, pinned=(), sc=1e-8, bc=1e-4, seed=0, slots=()):
+    edges = orc.build_edges(tris)
+    bends = orc.build_bends(tris, edges)
+    rest = orc.rest_lengths(pos, edges)
+    theta = orc.rest_angles(pos, bends)
+    scol, sn = orc.color(edges, len(pos), seed)
+    bcol, bn = orc.color(bends, len(pos), seed)
+    w = np.ones(len(pos))
+    w[list(pinned)] = 0.0
+    kw = {}
+    if len(slots):
+        dist = [orc.geodesic(pos, edges, s) for s in slots]
+        ptr = [0]
+        sl, rs = [], []
+        for t in range(len(pos)):
+            for j in range(len(slots)):
+                sl.append(j)
+                rs.append(dist[j][t])
+            ptr.append(len(sl))
+        kw = dict(slot_vert=np.array(slots, np.int32), lra_ptr=np.array(ptr, np.int32),
+                  lra_slot=np.array(sl, np.int32), lra_rest=np.array(rs))
+    x0 = pos if x0 is None else x0
+    v = np.zeros_like(pos) if v is None else v
+    return orc.Cloth(x0, v, w, edges, rest, np.full(len(edges), sc), scol, sn,
+                     bends, theta, np.full(len(bends), bc), bcol, bn, **kw)
+
+
+def test_free_fall_closed_form():
+    """Unconstrained-by-rest-state cloth falls rigidly: x_n - x_0 =
+    g dt^2/(1-f) [n - f(1-f^n)/(1-f)], f = 1 - damping*dt (predict + finalize)."""
+    pos, tris = scenes.grid(6, 1.0)
+    c = _cloth_from_mesh(pos, tris)
+    p = _params(iters=0, bending=0)
+    dt = (1.0 / 60.0) / 10
+    n = 25
+    for _ in range(n):
+        c.substep(p, dt)
+    f = 1.0 - 0.98 * dt
+    g = -9.81
+    disp = g * dt * dt / (1 - f) * (n - f * (1 - f ** n) / (1 - f))
+    vel = g * dt * f * (1 - f ** n) / (1 - f)
+    x0, v = c.state()
+    np.testing.assert_allclose(x0[:, 1], pos[:, 1] + disp, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(v[:, 1], vel, rtol=1e-12)
+    np.testing.assert_allclose(x0[:, [0, 2]], pos[:, [0, 2]], atol=1e-15)
+
+
+def test_pinned_vertices_do_not_move():
+    spec = scenes.hanging_cloth(seed=3, n=8)
+    c = _cloth_from_mesh(spec.pos, spec.tris, x0=spec.x0.astype(float), pinned=spec.pinned)
+    p = _params(iters=5, ground=1)
+    for _ in range(10):
+        c.substep(p, 1 / 600)
+    x0, v = c.state()
+    np.testing.assert_array_equal(x0[spec.pinned], spec.x0[spec.pinned].astype(float))
+    assert np.all(v[spec.pinned] == 0)
+    assert np.all(np.isfinite(x0))
+
+
+def _two_particle(d0, rest, comp, w=(1.0, 1.0)):
+    x0 = np.array([[0.0, 0, 0], [d0, 0, 0]])
+    c = orc.Cloth(x0, np.zeros_like(x0), np.array(w), np.array([[0, 1]], np.int32),
+                  np.array([rest]), np.array([comp]), np.array([0], np.int32), 1,
+                  np.zeros((0, 4), np.int32), np.zeros(0), np.zeros(0), np.zeros(0, np.int32), 0)
+    c.x[:] = c.x0
+    return c
+
+
+@pytest.mark.parametrize("comp", [0.0, 1e-8, 1e-5])
+@pytest.mark.parametrize("w", [(1.0, 1.0), (1.0, 0.0), (0.25, 1.0)])
+def test_two_particle_xpbd_step(comp, w):
+    """One XPBD iteration: dl = -(d0-r)/(W + a), a = c/dt^2; the particles move along the
+    unit-ish gradient diff/(|diff| + 1e-6) by w_i*dl.  A second iteration leaves the
+    compliant fixed point C = a/(W+a) C0 (up to the 1e-6 gradient epsilon)."""
+    d0, rest, dt = 1.2, 1.0, 1 / 600
+    c = _two_particle(d0, rest, comp, w)
+    lib = orc.load()
+    lib.orc_stretch_color(C.byref(c.s), 0, dt, 0)
+    a = comp / dt ** 2
+    W = w[0] + w[1]
+    dl = -(d0 - rest) / (W + a)
+    gx = -d0 / (d0 + 1e-6)  # (x0 - x1)/(|x0-x1| + eps)
+    x = c.x.reshape(-1, 3)
+    assert x[0, 0] == pytest.approx(w[0] * dl * gx, rel=1e-13, abs=1e-15)
+    assert x[1, 0] == pytest.approx(d0 - w[1] * dl * gx, rel=1e-13)
+    assert c.s_lam[0] == pytest.approx(dl, rel=1e-13)
+    lib.orc_stretch_color(C.byref(c.s), 0, dt, 1)
+    C1 = (x[1, 0] - x[0, 0]) - rest
+    assert C1 == pytest.approx(a / (W + a) * (d0 - rest), abs=5e-6 * (d0 - rest) + 1e-15)
+
+
+def test_stretch_conserves_momentum_equal_mass():
+    rng = scenes.Rng(11)
+    pos, tris = scenes.grid(7, 1.0)
+    x0 = pos + 0.05 * rng.normal(pos.size).reshape(-1, 3)
+    c = _cloth_from_mesh(pos, tris, x0=x0, sc=1e-6)
+    c.x[:] = c.x0
+    before = c.x.reshape(-1, 3).sum(0)
+    lib = orc.load()
+    for k in range(c.s_ncol):
+        lib.orc_stretch_color(C.byref(c.s), k, 1 / 600, 0)
+    np.testing.assert_allclose(c.x.reshape(-1, 3).sum(0), before, atol=1e-12)
+    # and the constraint residual decreases
+    e = c.s_idx.reshape(-1, 2)
+    xx = c.x.reshape(-1, 3)
+    r1 = np.abs(np.linalg.norm(xx[e[:, 0]] - xx[e[:, 1]], axis=1) - c.s_rest).sum()
+    r0 = np.abs(np.linalg.norm(x0[e[:, 0]] - x0[e[:, 1]], axis=1) - c.s_rest).sum()
+    assert r1 < 0.5 * r0
+
+
+def _hinge(alpha):
+    """Edge (0,0,0)-(1,0,0); wing v2 at (0.5, sin a, -cos a)... rotated by alpha about x."""
+    v0 = np.array([0.0, 0, 0])
+    v1 = np.array([1.0, 0, 0])
+    v2 = np.array([0.5, 0.0, -1.0])
+    v3 = np.array([0.5, np.sin(alpha), np.cos(alpha)])
+    return np.stack([v0, v1, v2, v3])
+
+
+def test_bend_probe_quirks():
+    # flat hinge: n1.n2 = -1 -> clamped to 0 -> measured angle pi/2 (reference quirk)
+    d, q, ok = orc.bend_probe(_hinge(0.0))
+    assert d == pytest.approx(-1.0)
+    assert ok
+    # wings folded so that n1.n2 = cos(pi - (pi - beta)) ... n1.n2 = -cos(alpha)
+    for alpha in [0.3, 1.0, 2.0, 2.8]:
+        d, q, ok = orc.bend_probe(_hinge(alpha))
+        assert d == pytest.approx(-np.cos(alpha), abs=1e-12)
+    # fully folded (alpha = pi): d = 1 -> angle 0 < eps -> skipped
+    d, q, ok = orc.bend_probe(_hinge(np.pi))
+    assert not ok
+
+
+@pytest.mark.parametrize("alpha", [1.9, 2.3, 2.7])
+def test_bend_gradients_match_finite_differences(alpha):
+    """q_k = -d(n1.n2)/dx_k where the clamp is inactive (0 < d < 1): with the reference's
+    normals n1 = N(p2 x p3), n2 = N(p2 x p4) its Mueller vectors (BasicSolverHelper.cu:40-49)
+    are the negated gradient of d, so x += w*lambda*q raises the measured angle acos(d)."""
+    rng = scenes.Rng(5)
+    x = _hinge(alpha) + 0.01 * rng.normal(12).reshape(4, 3)
+    d, q, ok = orc.bend_probe(x)
+    assert ok and 0.0 < d < 1.0
+    h = 1e-6
+    for k in range(4):
+        for a in range(3):
+            xp = x.copy(); xp[k, a] += h
+            xm = x.copy(); xm[k, a] -= h
+            fd = (orc.bend_probe(xp)[0] - orc.bend_probe(xm)[0]) / (2 * h)
+            assert q[k, a] == pytest.approx(-fd, abs=2e-5 * (1 + abs(fd))), (k, a)
+    np.testing.assert_allclose(q.sum(0), 0.0, atol=1e-12)  # translation invariance
+
+
+def test_rest_angles_analytic():
+    # rest angle = acos(N((v0-v2)x(v1-v2)) . N((v1-v3)x(v0-v3))) = fold between face normals
+    for alpha in [0.0, 0.7, 1.5]:
+        x = _hinge(alpha)
+        th = orc.rest_angles(x, np.array([[0, 1, 2, 3]], np.int32))[0]
+        assert th == pytest.approx(alpha, abs=1e-12)
+
+
+def test_geodesic_matches_brute_force():
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import shortest_path
+    rng = scenes.Rng(9)
+    pos, tris = scenes.grid(6, 1.0)
+    pos = pos + 0.02 * rng.normal(pos.size).reshape(-1, 3)
+    edges = orc.build_edges(tris)
+    w = np.linalg.norm(pos[edges[:, 0]] - pos[edges[:, 1]], axis=1)
+    n = len(pos)
+    g = csr_matrix((np.r_[w, w], (np.r_[edges[:, 0], edges[:, 1]], np.r_[edges[:, 1], edges[:, 0]])),
+                   shape=(n, n))
+    ref = shortest_path(g, directed=False, indices=[0, 17])
+    for row, src in zip(ref, [0, 17]):
+        np.testing.assert_allclose(orc.geodesic(pos, edges, src), row, rtol=1e-12)
+
+
+def test_lra_projection_idempotent_and_bounded():
+    pos, tris = scenes.grid(6, 1.0)
+    rng = scenes.Rng(4)
+    x0 = pos + 0.3 * rng.normal(pos.size).reshape(-1, 3)
+    c = _cloth_from_mesh(pos, tris, x0=x0, slots=(0,), pinned=(0,))
+    c.x[:] = c.x0
+    p = _params()
+    orc.load().orc_lra(C.byref(c.s), C.byref(p))
+    once = c.x.copy()
+    orc.load().orc_lra(C.byref(c.s), C.byref(p))
+    np.testing.assert_allclose(c.x, once, atol=1e-14)
+    xx = once.reshape(-1, 3)
+    dist = np.linalg.norm(xx - xx[0], axis=1)
+    assert np.all(dist <= c.lra_rest + 1e-12)
+
+
+def test_ground_plane_projection_and_friction():
+    pos, tris = scenes.grid(3, 1.0, height=0.0)
+    c = _cloth_from_mesh(pos, tris)
+    c.x[:] = c.x0
+    x = c.x.reshape(-1, 3)
+    x[:, 1] = -1.3
+    x[:, 0] += 0.05  # tangential motion during the substep
+    p = _params(ground=1, h=-1.0, mu=0.0)
+    orc.load().orc_ground(C.byref(c.s), C.byref(p))
+    np.testing.assert_allclose(x[:, 1], -1.0)
+    np.testing.assert_allclose(x[:, 0], pos[:, 0] + 0.05)
+    # friction: |tangential| 0.05 vs mu*|corr| = 0.1*0.3 = 0.03 -> reduced by 0.03
+    x[:, 1] = -1.3
+    x[:, 0] = pos[:, 0] + 0.05
+    p = _params(ground=1, h=-1.0, mu=0.1)
+    orc.load().orc_ground(C.byref(c.s), C.byref(p))
+    np.testing.assert_allclose(x[:, 0], pos[:, 0] + 0.02, atol=1e-12)
+    # strong friction stops the tangential motion
+    x[:, 1] = -1.3
+    x[:, 0] = pos[:, 0] + 0.05
+    p = _params(ground=1, h=-1.0, mu=1.0)
+    orc.load().orc_ground(C.byref(c.s), C.byref(p))
+    np.testing.assert_allclose(x[:, 0], pos[:, 0], atol=1e-12)
+
+
+def test_reference_bend_model_gains_energy_at_small_compliance():
+    """The reference's bend formula (clamp to [0,1], rest angle from the other normal
+    convention, step along +q) pushes a flat cloth in-plane: at compliance 1e-4 the
+    hanging cloth overstretches, while the consistent model stays near rest length."""
+    out = {}
+    for model in (0, 1):
+        spec = scenes.hanging_cloth(seed=1, n=16)
+        c = _cloth_from_mesh(spec.pos, spec.tris, x0=spec.x0.astype(float), pinned=spec.pinned)
+        p = _params(iters=10, ground=1)
+        p.bend_model = model
+        for _ in range(60):
+            c.substep(p, 1 / 600)
+        x0, _ = c.state()
+        e = c.s_idx.reshape(-1, 2)
+        out[model] = np.abs(np.linalg.norm(x0[e[:, 0]] - x0[e[:, 1]], axis=1) / c.s_rest - 1).max()
+    assert out[0] > 0.5
+    assert out[1] < 0.05
+
+
+def test_hanging_cloth_rollout_is_stable():
+    spec = scenes.hanging_cloth(seed=1, n=16)
+    c = _cloth_from_mesh(spec.pos, spec.tris, x0=spec.x0.astype(float), pinned=spec.pinned)
+    p = _params(iters=10, ground=1)
+    p.bend_model = 1
+    for _ in range(100):
+        c.substep(p, 1 / 600)
+    x0, v = c.state()
+    assert np.all(np.isfinite(x0))
+    assert x0[:, 1].min() >= -1.0 - 1e-12       # ground
+    assert x0[:, 1].min() < -0.1                  # it fell (~0.14 m of free fall in 1/6 s)
+    e = c.s_idx.reshape(-1, 2)
+    strain = np.linalg.norm(x0[e[:, 0]] - x0[e[:, 1]], axis=1) / c.s_rest - 1
+    assert np.abs(strain).max() < 0.2

@@ -1,0 +1,143 @@
+"""ctypes binding of the C ABI declared in include/xrtailor_amd.h.
+
+The shared library is built in-tree by ``python -m xrtailor_amd.build``.  Loading fails
+loudly when it is missing: there is no CPU fallback for the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libxrtailor_amd.so"
+
+XRT_OK = 0
+ERRORS = {
+    -1: "XRT_ERR_INVALID_ARG",
+    -2: "XRT_ERR_HIP",
+    -3: "XRT_ERR_STATE",
+    -4: "XRT_ERR_CAPACITY",
+    -5: "XRT_ERR_NOT_FINITE",
+    -6: "XRT_ERR_UNSUPPORTED",
+}
+
+
+class XrtError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class XrtParams(C.Structure):
+    _fields_ = [
+        ("gravity", C.c_float * 3),
+        ("damping", C.c_float),
+        ("relaxation_factor", C.c_float),
+        ("long_range_stretchiness", C.c_float),
+        ("bvh_tolerance", C.c_float),
+        ("ground_height", C.c_float),
+        ("ground_friction", C.c_float),
+        ("pbd_thickness", C.c_float),
+        ("repulsion_relaxation", C.c_float),
+        ("frame_time", C.c_float),
+        ("num_substeps", C.c_int32),
+        ("num_iterations", C.c_int32),
+        ("num_collision_passes", C.c_int32),
+        ("solve_bending", C.c_int32),
+        ("enable_ground", C.c_int32),
+        ("enable_obstacle_contact", C.c_int32),
+        ("enable_self_contact", C.c_int32),
+        ("pair_update_interval", C.c_int32),
+        ("bend_model", C.c_int32),
+    ]
+
+
+_P = C.c_void_p
+_I = C.c_int32
+_F = C.c_float
+_FP = C.POINTER(C.c_float)
+_IP = C.POINTER(C.c_int32)
+
+# name -> argtypes (all functions return int status unless listed in _RESTYPES)
+SIGNATURES = {
+    "xrt_abi_version": [],
+    "xrt_last_error": [],
+    "xrt_default_params": [C.POINTER(XrtParams)],
+    "xrt_create": [C.POINTER(_P), C.c_int],
+    "xrt_destroy": [_P],
+    "xrt_set_params": [_P, C.POINTER(XrtParams)],
+    "xrt_get_params": [_P, C.POINTER(XrtParams)],
+    "xrt_sync": [_P],
+    "xrt_device_info": [_P, C.POINTER(C.c_int64)],
+    "xrt_build_edges": [_IP, _I, _IP, _I, _IP],
+    "xrt_build_bends": [_IP, _I, _IP, _I, _IP, _I, _IP],
+    "xrt_color_constraints": [_IP, _I, _I, _I, C.c_uint64, _IP, _IP],
+    "xrt_add_cloth": [_P, _FP, _I, _IP, _I, _F, _F, _IP],
+    "xrt_pin_vertices": [_P, _I, _IP, _I],
+    "xrt_set_stretch_compliance": [_P, _I, _FP],
+    "xrt_add_lra_slots": [_P, _I, _IP, _I],
+    "xrt_start": [_P, C.c_uint64],
+    "xrt_num_instances": [_P, _IP],
+    "xrt_get_counts": [_P, _I, _IP],
+    "xrt_get_edges": [_P, _I, _IP, _FP, _IP],
+    "xrt_get_bends": [_P, _I, _IP, _FP, _IP],
+    "xrt_get_lra": [_P, _I, _IP, _IP, _IP, _FP],
+    "xrt_get_inv_mass": [_P, _I, _FP],
+    "xrt_set_state": [_P, _I, _FP, _FP],
+    "xrt_get_state": [_P, _I, _FP, _FP],
+    "xrt_get_predicted": [_P, _I, _FP],
+    "xrt_predict_positions": [_P, _I, _F],
+    "xrt_stretch_solve": [_P, _I, _I, _F, _I],
+    "xrt_bend_solve": [_P, _I, _I, _F, _I],
+    "xrt_solve_lra": [_P, _I],
+    "xrt_solve_ground": [_P, _I],
+    "xrt_apply_deltas": [_P, _I],
+    "xrt_finalize": [_P, _I, _F],
+    "xrt_substep_plain": [_P, _I],
+    "xrt_frames_plain": [_P, _I],
+    "xrt_substep_batched": [_P],
+    "xrt_frames_batched": [_P, _I],
+    "xrt_time_batched_substep": [_P, _I, _FP],
+    "xrt_time_frames": [_P, _I, _I, _FP],
+}
+_RESTYPES = {"xrt_last_error": C.c_char_p, "xrt_default_params": None}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise FileNotFoundError(
+            f"{LIB_PATH} is missing: build it with `python -m xrtailor_amd.build` "
+            "(the solver has no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, C.c_int)
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != XRT_OK:
+        raise XrtError(rc, load().xrt_last_error().decode(errors="replace"))
+
+
+def fptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_FP)
+
+
+def iptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_IP)

@@ -1,0 +1,34 @@
+// Host-side setup of a cloth instance: stretch edges, bend elements, rest state,
+// constraint coloring and geodesic rest lengths.  Runs once per scene (the reference
+// runs the same steps in AddCloth / SetupInternalDynamics).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace xrt {
+
+// Unique edges in order of first appearance over faces, local edges (0,1),(1,2),(2,0),
+// oriented as in the first face holding the edge (edge order of
+// PrepareBasicStretchData, xrtailor/src/physics/ClothSolverHelper.cpp:90-109, with the
+// vcglib enumeration replaced by this deterministic one).
+std::vector<int32_t> BuildEdges(const int32_t* tris, int nt);
+
+// Dihedral elements (v0,v1,v2,v3) per pair of faces on an edge
+// (GenerateBendingElements, xrtailor/src/physics/ClothSolverHelper.cpp:4-88).
+std::vector<int32_t> BuildBends(const int32_t* tris, int nt, const std::vector<int32_t>& edges);
+
+std::vector<float> RestLengths(const float* pos, const std::vector<int32_t>& edges);
+std::vector<float> RestAngles(const float* pos, const std::vector<int32_t>& bends);
+
+// Deterministic palette coloring (GraphColoring::Paint,
+// xrtailor/src/physics/graph_coloring/Base.cu:12-317): returns compact colors 0..k-1.
+std::vector<int32_t> ColorConstraints(const int32_t* cons, int arity, int n, int nv,
+                                      uint64_t seed, int* n_colors);
+
+// Dijkstra over the edge graph (Geodesic::ComputeGeodesicDistance,
+// xrtailor/src/utils/Geodesic.cpp:6-79).
+std::vector<double> Geodesic(const float* pos, int nv, const std::vector<int32_t>& edges, int src);
+
+uint64_t SplitMix64(uint64_t* state);
+
+}  // namespace xrt

@@ -1,0 +1,159 @@
+"""Procedural scenes of the benchmark / test workloads and their seeded RNG.
+
+RNG: SplitMix64 streams with explicit Box-Muller normals, so that every workload is
+reproducible from its seed on any host (numpy's generators are not used).
+
+Workloads (sizes follow the solver's benchmark configurations):
+  hanging_cloth   64x64 grid, 1 m, two top corners pinned, stretch 1e-8 / bend 1e-4
+  drop_cloth      256x256 grid, 2 m, 0.5 m above an icosphere (s5, r 0.5)
+  batched         256 instances, side round(32*6^u), log-uniform compliances
+  diff_rollout    128x128 cloth over a sphere
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _mix(z: np.ndarray) -> np.ndarray:
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+class Rng:
+    """SplitMix64 stream: value k is mix(seed + (k+1) * golden)."""
+
+    def __init__(self, seed: int):
+        self.state = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+        self.count = 0
+
+    def uint64(self, n: int) -> np.ndarray:
+        k = np.arange(self.count + 1, self.count + n + 1, dtype=np.uint64)
+        self.count += n
+        with np.errstate(over="ignore"):
+            return _mix(self.state + k * _GOLDEN)
+
+    def uniform(self, n: int) -> np.ndarray:
+        """[0, 1) doubles from the top 53 bits."""
+        return (self.uint64(n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+    def normal(self, n: int) -> np.ndarray:
+        """Box-Muller: u1 in (0,1], u2 in [0,1)."""
+        m = (n + 1) // 2
+        u1 = 1.0 - self.uniform(m)
+        u2 = self.uniform(m)
+        r = np.sqrt(-2.0 * np.log(u1))
+        z = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])
+        return z[:n]
+
+
+# --------------------------------------------------------------------- meshes
+def grid(n: int, side: float, height: float = 0.0):
+    """n x n vertex grid in the y=height plane, x,z in [-side/2, side/2].
+    Vertex (i, j) has index j*n + i (i along x, j along z).  Quads are split on the
+    (i,j)-(i+1,j+1) diagonal, wound counter-clockwise seen from +y."""
+    t = np.linspace(-0.5 * side, 0.5 * side, n)
+    xs, zs = np.meshgrid(t, t, indexing="xy")
+    pos = np.stack([xs.ravel(), np.full(n * n, height), zs.ravel()], axis=1)
+    j, i = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    a = (j * n + i).ravel()
+    b = a + 1
+    c = a + n + 1
+    d = a + n
+    tris = np.empty((2 * a.size, 3), np.int32)
+    tris[0::2] = np.stack([a, c, b], 1)
+    tris[1::2] = np.stack([a, d, c], 1)
+    return pos, tris
+
+
+def icosphere(subdiv: int, radius: float, center=(0.0, 0.0, 0.0)):
+    """Subdivided icosahedron (10*4^s + 2 vertices), outward CCW winding."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+         (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(p, float) / np.linalg.norm(p) for p in v]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache: dict = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    pos = np.array(verts) * radius + np.asarray(center, float)
+    return pos, np.array(faces, np.int32)
+
+
+def capsule(radius: float, half_length: float, subdiv: int = 3, center=(0.0, 0.0, 0.0)):
+    """Capsule along x: an icosphere whose +x / -x halves are moved apart."""
+    pos, tris = icosphere(subdiv, radius)
+    pos[:, 0] += np.sign(pos[:, 0]) * half_length * (np.abs(pos[:, 0]) > 1e-9)
+    return pos + np.asarray(center, float), tris
+
+
+# ------------------------------------------------------------------ workloads
+@dataclass
+class ClothSpec:
+    pos: np.ndarray            # rest positions (nv, 3) float64
+    tris: np.ndarray           # (nt, 3) int32
+    pinned: np.ndarray         # pinned vertex ids
+    stretch_compliance: float
+    bend_compliance: float
+    x0: np.ndarray             # initial (jittered) positions (nv, 3) float32
+    obstacle: tuple | None = None   # (pos, tris) or None
+    lra_slots: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+
+
+def hanging_cloth(seed: int = 1, n: int = 64, side: float = 1.0, jitter: float = 1e-4,
+                  stretch: float = 1e-8, bend: float = 1e-4) -> ClothSpec:
+    pos, tris = grid(n, side, 0.0)
+    pinned = np.array([0, n - 1], np.int32)  # the two corners of row z = -side/2
+    x0 = (pos + jitter * Rng(seed).normal(pos.size).reshape(-1, 3)).astype(np.float32)
+    return ClothSpec(pos, tris, pinned, stretch, bend, x0)
+
+
+def drop_cloth(seed: int = 2, n: int = 256, side: float = 2.0, jitter: float = 1e-3) -> ClothSpec:
+    pos, tris = grid(n, side, 1.0)
+    x0 = (pos + jitter * Rng(seed).normal(pos.size).reshape(-1, 3)).astype(np.float32)
+    obstacle = icosphere(5, 0.5)
+    return ClothSpec(pos, tris, np.zeros(0, np.int32), 1e-8, 1e-4, x0, obstacle)
+
+
+def batched(seed: int = 4, n_instances: int = 256, side_lo: int = 32, side_hi: int = 192,
+            cloth_side: float = 1.0, jitter: float = 1e-3) -> list[ClothSpec]:
+    """Independent instances: grid side round(lo*(hi/lo)^u) (log-uniform), stretch
+    compliance 10^U(-9,-6), bend 10^U(-5,-3), sphere or capsule obstacle r~U(0.2,0.6)
+    below the cloth."""
+    rng = Rng(seed)
+    u = rng.uniform(n_instances)
+    sides = np.rint(side_lo * (side_hi / side_lo) ** u).astype(int)
+    sc = 10.0 ** (-9.0 + 3.0 * rng.uniform(n_instances))
+    bc = 10.0 ** (-5.0 + 2.0 * rng.uniform(n_instances))
+    kind = rng.uniform(n_instances) < 0.5
+    radius = 0.2 + 0.4 * rng.uniform(n_instances)
+    out = []
+    for i in range(n_instances):
+        pos, tris = grid(int(sides[i]), cloth_side, 0.0)
+        x0 = (pos + jitter * Rng(seed * 1000003 + i).normal(pos.size).reshape(-1, 3)).astype(np.float32)
+        r = float(radius[i])
+        ctr = (0.0, -r - 0.05, 0.0)
+        obst = icosphere(3, r, ctr) if kind[i] else capsule(r, 0.5 * r, 3, ctr)
+        out.append(ClothSpec(pos, tris, np.zeros(0, np.int32), float(sc[i]), float(bc[i]), x0, obst))
+    return out

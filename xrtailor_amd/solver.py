@@ -1,0 +1,266 @@
+"""Python mirror of the solver's C ABI (used by the tests and the benchmark).
+
+Class and method names follow the reference's C++ interfaces for this path:
+``ClothSolver`` ~ ``ClothSolverBase`` (xrtailor/src/pipeline/base/ClothSolverBase.cpp),
+``StretchSolver`` / ``BendSolver`` ~ ``BasicConstraint::{Stretch,Bend}Solver``
+(xrtailor/include/xrtailor/physics/dynamics/BasicSolver.cuh:15-54).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import XrtParams, check, fptr, iptr
+
+
+def default_params() -> XrtParams:
+    p = XrtParams()
+    _ffi.load().xrt_default_params(C.byref(p))
+    return p
+
+
+def make_params(**kw) -> XrtParams:
+    """default_params() with fields overridden by keyword (gravity as a 3-sequence)."""
+    p = default_params()
+    for k, v in kw.items():
+        if k == "gravity":
+            for i in range(3):
+                p.gravity[i] = v[i]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def build_solver(specs, params: XrtParams, seed: int = 0, device: int = 0):
+    """A started ClothSolver with one instance per scenes.ClothSpec (pins, LRA slots and
+    the jittered initial state applied).  Returns (solver, instance ids)."""
+    s = ClothSolver(device=device, params=params)
+    insts = []
+    for spec in specs:
+        i = s.AddCloth(spec.pos, spec.tris, spec.stretch_compliance, spec.bend_compliance)
+        if len(spec.pinned):
+            s.Pin(i, spec.pinned)
+        if len(spec.lra_slots):
+            s.AddLraSlots(i, spec.lra_slots)
+        s.SetState(i, spec.x0)
+        insts.append(i)
+    s.Start(seed)
+    return s, insts
+
+
+@dataclass
+class Counts:
+    nv: int
+    nt: int
+    n_edges: int
+    n_bends: int
+    stretch_colors: int
+    bend_colors: int
+    lra_slots: int
+    lra_constraints: int
+
+
+class _Coloring:
+    def __init__(self, n_colors: int):
+        self.n_colors = n_colors
+
+
+class StretchSolver:
+    """BasicConstraint::StretchSolver over one instance of a started ClothSolver."""
+
+    def __init__(self, solver: "ClothSolver", inst: int):
+        self._s, self._i = solver, inst
+
+    def Coloring(self) -> _Coloring:
+        return _Coloring(self._s.counts(self._i).stretch_colors)
+
+    def Solve(self, color: int, dt: float, iteration: int) -> None:
+        check(self._s.lib.xrt_stretch_solve(self._s.ctx, self._i, color, dt, iteration))
+
+
+class BendSolver:
+    """BasicConstraint::BendSolver over one instance of a started ClothSolver."""
+
+    def __init__(self, solver: "ClothSolver", inst: int):
+        self._s, self._i = solver, inst
+
+    def Coloring(self) -> _Coloring:
+        return _Coloring(self._s.counts(self._i).bend_colors)
+
+    def Solve(self, color: int, dt: float, iteration: int = 0) -> None:
+        check(self._s.lib.xrt_bend_solve(self._s.ctx, self._i, color, dt, iteration))
+
+
+class ClothSolver:
+    """Owns one xrt_ctx (device state of one or more cloth instances)."""
+
+    def __init__(self, device: int = 0, params: XrtParams | None = None):
+        self.lib = _ffi.load()
+        ctx = C.c_void_p()
+        check(self.lib.xrt_create(C.byref(ctx), device))
+        self.ctx = ctx
+        self.params = params if params is not None else default_params()
+        check(self.lib.xrt_set_params(self.ctx, C.byref(self.params)))
+
+    # -------------------------------------------------------------- lifetime
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.xrt_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ----------------------------------------------------------------- setup
+    def set_params(self, params: XrtParams) -> None:
+        self.params = params
+        check(self.lib.xrt_set_params(self.ctx, C.byref(params)))
+
+    def AddCloth(self, pos: np.ndarray, tris: np.ndarray, stretch_compliance: float,
+                 bend_compliance: float) -> int:
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+        tris = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1)
+        out = C.c_int32(-1)
+        check(self.lib.xrt_add_cloth(self.ctx, fptr(pos), pos.size // 3, iptr(tris), tris.size // 3,
+                                     stretch_compliance, bend_compliance, C.byref(out)))
+        return out.value
+
+    def Pin(self, inst: int, idx) -> None:
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        check(self.lib.xrt_pin_vertices(self.ctx, inst, iptr(idx), idx.size))
+
+    def SetStretchCompliance(self, inst: int, comp: np.ndarray) -> None:
+        comp = np.ascontiguousarray(comp, dtype=np.float32)
+        check(self.lib.xrt_set_stretch_compliance(self.ctx, inst, fptr(comp)))
+
+    def AddLraSlots(self, inst: int, slots) -> None:
+        slots = np.ascontiguousarray(slots, dtype=np.int32)
+        check(self.lib.xrt_add_lra_slots(self.ctx, inst, iptr(slots), slots.size))
+
+    def Start(self, seed: int = 0) -> None:
+        check(self.lib.xrt_start(self.ctx, C.c_uint64(seed)))
+
+    # ----------------------------------------------------------------- query
+    def num_instances(self) -> int:
+        n = C.c_int32()
+        check(self.lib.xrt_num_instances(self.ctx, C.byref(n)))
+        return n.value
+
+    def device_info(self) -> dict:
+        info = (C.c_int64 * 4)()
+        check(self.lib.xrt_device_info(self.ctx, info))
+        return {"cu_count": info[0], "max_lds_bytes": info[1], "wave_size": info[2],
+                "l2_bytes": info[3]}
+
+    def counts(self, inst: int) -> Counts:
+        c = np.zeros(8, np.int32)
+        check(self.lib.xrt_get_counts(self.ctx, inst, iptr(c)))
+        return Counts(*[int(v) for v in c])
+
+    def edges(self, inst: int, colors: bool = True):
+        n = self.counts(inst)
+        e = np.zeros(2 * n.n_edges, np.int32)
+        r = np.zeros(n.n_edges, np.float32)
+        col = np.zeros(n.n_edges, np.int32) if colors else None
+        check(self.lib.xrt_get_edges(self.ctx, inst, iptr(e), fptr(r), iptr(col)))
+        return e.reshape(-1, 2), r, col
+
+    def bends(self, inst: int, colors: bool = True):
+        n = self.counts(inst)
+        b = np.zeros(4 * n.n_bends, np.int32)
+        t = np.zeros(n.n_bends, np.float32)
+        col = np.zeros(n.n_bends, np.int32) if colors else None
+        check(self.lib.xrt_get_bends(self.ctx, inst, iptr(b), fptr(t), iptr(col)))
+        return b.reshape(-1, 4), t, col
+
+    def lra(self, inst: int):
+        n = self.counts(inst)
+        sv = np.zeros(max(n.lra_slots, 1), np.int32)
+        ptr = np.zeros(n.nv + 1, np.int32)
+        sl = np.zeros(max(n.lra_constraints, 1), np.int32)
+        rest = np.zeros(max(n.lra_constraints, 1), np.float32)
+        check(self.lib.xrt_get_lra(self.ctx, inst, iptr(sv), iptr(ptr), iptr(sl), fptr(rest)))
+        return sv[:n.lra_slots], ptr, sl[:n.lra_constraints], rest[:n.lra_constraints]
+
+    def inv_mass(self, inst: int) -> np.ndarray:
+        w = np.zeros(self.counts(inst).nv, np.float32)
+        check(self.lib.xrt_get_inv_mass(self.ctx, inst, fptr(w)))
+        return w
+
+    # ----------------------------------------------------------------- state
+    def SetState(self, inst: int, x0=None, v=None) -> None:
+        x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float32).reshape(-1)
+        v = None if v is None else np.ascontiguousarray(v, dtype=np.float32).reshape(-1)
+        check(self.lib.xrt_set_state(self.ctx, inst, fptr(x0), fptr(v)))
+
+    def GetState(self, inst: int):
+        nv = self.counts(inst).nv
+        x0 = np.zeros(3 * nv, np.float32)
+        v = np.zeros(3 * nv, np.float32)
+        check(self.lib.xrt_get_state(self.ctx, inst, fptr(x0), fptr(v)))
+        return x0.reshape(-1, 3), v.reshape(-1, 3)
+
+    def GetPredicted(self, inst: int) -> np.ndarray:
+        x = np.zeros(3 * self.counts(inst).nv, np.float32)
+        check(self.lib.xrt_get_predicted(self.ctx, inst, fptr(x)))
+        return x.reshape(-1, 3)
+
+    # ------------------------------------------------------ reference-shaped
+    def StretchSolver(self, inst: int = 0) -> StretchSolver:
+        return StretchSolver(self, inst)
+
+    def BendSolver(self, inst: int = 0) -> BendSolver:
+        return BendSolver(self, inst)
+
+    def PredictPositions(self, inst: int, dt: float) -> None:
+        check(self.lib.xrt_predict_positions(self.ctx, inst, dt))
+
+    def SolveLongRangeAttachments(self, inst: int) -> None:
+        check(self.lib.xrt_solve_lra(self.ctx, inst))
+
+    def SolveGround(self, inst: int) -> None:
+        check(self.lib.xrt_solve_ground(self.ctx, inst))
+
+    def ApplyDeltas(self, inst: int) -> None:
+        check(self.lib.xrt_apply_deltas(self.ctx, inst))
+
+    def Finalize(self, inst: int, dt: float) -> None:
+        check(self.lib.xrt_finalize(self.ctx, inst, dt))
+
+    # --------------------------------------------------------------- drivers
+    def SubstepPlain(self, inst: int) -> None:
+        check(self.lib.xrt_substep_plain(self.ctx, inst))
+
+    def FramesPlain(self, n: int) -> None:
+        check(self.lib.xrt_frames_plain(self.ctx, n))
+
+    def SubstepBatched(self) -> None:
+        check(self.lib.xrt_substep_batched(self.ctx))
+
+    def FramesBatched(self, n: int) -> None:
+        check(self.lib.xrt_frames_batched(self.ctx, n))
+
+    def sync(self) -> None:
+        check(self.lib.xrt_sync(self.ctx))
+
+    def time_frames(self, batched: bool, n: int) -> float:
+        ms = C.c_float()
+        check(self.lib.xrt_time_frames(self.ctx, int(batched), n, C.byref(ms)))
+        return ms.value
+
+    def time_batched_substep(self, n: int) -> float:
+        ms = C.c_float()
+        check(self.lib.xrt_time_batched_substep(self.ctx, n, C.byref(ms)))
+        return ms.value
