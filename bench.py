@@ -88,6 +88,7 @@ def main() -> int:
     ap.add_argument("--plain-frames", type=int, default=1)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain path (profiling runs)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build measures one GPU only (instances are independent; "
@@ -124,11 +125,14 @@ def main() -> int:
     achieved = bytes_sub / (kern_ms * 1e-3) / 1e9
 
     # plain path on the same workload
-    solver.FramesPlain(1)
-    solver.sync()
-    pms = solver.time_frames(False, a.plain_frames)
-    plain = n_verts * params.num_substeps * a.plain_frames / (pms * 1e-3)
-    print(f"[bench] plain {pms:.1f} ms / {a.plain_frames} frame(s)", file=sys.stderr, flush=True)
+    plain = None
+    if not a.no_plain:
+        solver.FramesPlain(1)
+        solver.sync()
+        pms = solver.time_frames(False, a.plain_frames)
+        plain = n_verts * params.num_substeps * a.plain_frames / (pms * 1e-3)
+        print(f"[bench] plain {pms:.1f} ms / {a.plain_frames} frame(s)", file=sys.stderr,
+              flush=True)
     solver.close()
 
     cpu, parity, parity_n = None, None, 0
@@ -154,7 +158,7 @@ def main() -> int:
                    "substeps": params.num_substeps, "iterations": params.num_iterations,
                    "bend_model": "consistent", "step": "one frame of all instances"},
         "plain_value": plain,
-        "batched_over_plain": value / plain,
+        "batched_over_plain": (value / plain) if plain else None,
         "parity_rel_err": parity,
         "parity_instances": parity_n,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -63,9 +63,14 @@ typedef struct xrt_params {
    * XRT_BEND_CONSISTENT: rest angle pi - theta0 (the solver's own normals), d clamped to
    * [-1,1], PBD step along -q (Mueller 2007).  Same cost; stable for small compliance. */
   int32_t bend_model;
+  /* constraint coloring at xrt_start: XRT_COLOR_GREEDY (first fit in constraint order,
+   * fewest colors on structured meshes) or XRT_COLOR_PALETTE (GraphColoring::Paint,
+   * xrtailor/src/physics/graph_coloring/Base.cu:163-317, made deterministic from the seed) */
+  int32_t coloring;
 } xrt_params;
 
 enum { XRT_BEND_REFERENCE = 0, XRT_BEND_CONSISTENT = 1 };
+enum { XRT_COLOR_GREEDY = 0, XRT_COLOR_PALETTE = 1 };
 
 typedef struct xrt_ctx xrt_ctx;
 
@@ -95,7 +100,8 @@ int xrt_build_bends(const int32_t* tris, int32_t nt, const int32_t* edges, int32
                     int32_t* bends_out, int32_t max_bends, int32_t* nb_out);
 /* constraint coloring (GraphColoring::Paint, xrtailor/src/physics/graph_coloring/Base.cu:163-317) */
 int xrt_color_constraints(const int32_t* cons, int32_t arity, int32_t n, int32_t nv,
-                          uint64_t seed, int32_t* colors_out, int32_t* ncolors_out);
+                          uint64_t seed, int32_t mode, int32_t* colors_out,
+                          int32_t* ncolors_out);
 
 /* -------------------------------------------------------------- scene setup */
 /* ClothSolverUniversal::AddCloth (xrtailor/src/pipeline/impl/universal/ClothSolver.cpp:247-350):

@@ -59,7 +59,8 @@ def load() -> C.CDLL:
         lib.orc_rest_lengths.restype = None
         lib.orc_rest_angles.argtypes = [_D, _I, C.c_int, _D]
         lib.orc_rest_angles.restype = None
-        lib.orc_color_constraints.argtypes = [_I, C.c_int, C.c_int, C.c_int, C.c_uint64, _I]
+        lib.orc_color_constraints.argtypes = [_I, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int,
+                                              _I]
         for fn in ("orc_predict", "orc_lra", "orc_ground", "orc_finalize", "orc_substep"):
             getattr(lib, fn).restype = None
         lib.orc_predict.argtypes = [C.POINTER(OrcCloth), C.POINTER(OrcParams), C.c_double]
@@ -124,12 +125,13 @@ def rest_angles(x: np.ndarray, bends: np.ndarray) -> np.ndarray:
     return out
 
 
-def color(cons: np.ndarray, nv: int, seed: int):
+def color(cons: np.ndarray, nv: int, seed: int, mode: int = 0):
+    """mode 0: first fit in index order; 1: seeded palette coloring."""
     c = np.ascontiguousarray(cons, np.int32)
     arity = c.shape[1]
     out = np.zeros(c.shape[0], np.int32)
     n = load().orc_color_constraints(_i(c.reshape(-1)), arity, c.shape[0], nv, C.c_uint64(seed),
-                                     _i(out))
+                                     mode, _i(out))
     return out, n
 
 

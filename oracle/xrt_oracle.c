@@ -166,7 +166,7 @@ static int cmp_i32(const void* a, const void* b) {
 }
 
 int orc_color_constraints(const int32_t* cons, int arity, int n, int nv, uint64_t seed,
-                          int32_t* colors_out) {
+                          int mode, int32_t* colors_out) {
   if (n <= 0) return 0;
   /* vertex -> constraints CSR */
   int32_t* vptr = (int32_t*)calloc((size_t)nv + 1, sizeof(int32_t));
@@ -215,17 +215,38 @@ int orc_color_constraints(const int32_t* cons, int arity, int n, int nv, uint64_
   uint64_t* rs = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)n);
   uint64_t s0 = seed;
   uint64_t base = orc_splitmix64(&s0);
+  /* conflict ranking: seeded hash priority (ties by index) instead of the bare index, so
+   * conflicts resolve in O(log n) rounds rather than a wave along the index order */
+  uint64_t* prio = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)n);
+  int32_t* act = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
   for (int c = 0; c < n; c++) {
     int d = aptr[c + 1] - aptr[c];
     vsize[c] = d / min_deg + 1; /* palette size floor(deg/shrinking)+1 (Base.cu:19-20) */
     color[c] = 0;
     rs[c] = base + (uint64_t)c * 0xD1B54A32D192ED03ULL;
+    uint64_t ps = rs[c] ^ 0x5851F42D4C957F2DULL;
+    prio[c] = orc_splitmix64(&ps);
+    act[c] = c;
   }
-  int remaining = n;
-  while (remaining > 0) {
-    /* tentative coloring (Base.cu:35-69) */
+  int nact = n;
+  if (mode == 0) {
+    /* first fit in index order: smallest color no lower-indexed neighbour holds */
+    int32_t* stamp = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 2));
+    for (int k = 0; k < n + 2; k++) stamp[k] = -1;
     for (int c = 0; c < n; c++) {
-      if (colored[c]) continue;
+      for (int k = aptr[c]; k < aptr[c + 1]; k++)
+        if (adj[k] < c) stamp[color[adj[k]]] = c;
+      int col = 0;
+      while (stamp[col] == c) col++;
+      color[c] = col;
+    }
+    free(stamp);
+    nact = 0;
+  }
+  while (nact > 0) {
+    /* tentative coloring (Base.cu:35-69) */
+    for (int a = 0; a < nact; a++) {
+      int c = act[a];
       uint64_t r = orc_splitmix64(&rs[c]);
       double u = ((double)(r >> 11) + 1.0) * (1.0 / 9007199254740992.0); /* (0,1] */
       int vs = vsize[c];
@@ -238,36 +259,42 @@ int orc_color_constraints(const int32_t* cons, int arity, int n, int nv, uint64_
       color[c] = lc;
     }
     /* conflict resolution on the start-of-round state (Base.cu:71-118) */
-    for (int c = 0; c < n; c++) {
+    for (int a = 0; a < nact; a++) {
+      int c = act[a];
       accept[c] = 0;
-      if (colored[c]) continue;
       int conflict_uncolored = 0, conflict_colored = 0, highest = 1;
       for (int k = aptr[c]; k < aptr[c + 1]; k++) {
         int j = adj[k];
         if (color[j] == color[c]) { if (colored[j]) conflict_colored = 1; else conflict_uncolored = 1; }
-        if (!colored[j] && c < j) highest = 0;
+        if (!colored[j] && (prio[c] < prio[j] || (prio[c] == prio[j] && c < j))) highest = 0;
       }
       if (conflict_colored) continue;
       if (conflict_uncolored && !highest) continue;
       accept[c] = 1;
     }
-    for (int c = 0; c < n; c++) {
+    for (int a = 0; a < nact; a++) {
+      int c = act[a];
       if (!accept[c]) continue;
-      colored[c] = 1; remaining--;
+      colored[c] = 1;
       for (int k = aptr[c]; k < aptr[c + 1]; k++) {
         int j = adj[k];
         int cap_j = pal_ptr[j + 1] - pal_ptr[j];
         if (color[c] < cap_j) forbid[pal_ptr[j] + color[c]] = 1;
       }
     }
-    /* feed the hungry (Base.cu:120-139) */
-    for (int c = 0; c < n; c++) {
+    /* feed the hungry (Base.cu:120-139), then drop the newly colored from the active list */
+    int na = 0;
+    for (int a = 0; a < nact; a++) {
+      int c = act[a];
       if (colored[c]) continue;
       int avail = 0;
       for (int t = 0; t < vsize[c]; t++) if (!forbid[pal_ptr[c] + t]) { avail = 1; break; }
       if (!avail && vsize[c] < pal_ptr[c + 1] - pal_ptr[c]) vsize[c]++;
+      act[na++] = c;
     }
+    nact = na;
   }
+  free(prio); free(act);
   /* compact used colors to 0..k-1 (ascending original color) */
   int maxc = 0;
   for (int c = 0; c < n; c++) if (color[c] > maxc) maxc = color[c];

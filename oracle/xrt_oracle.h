@@ -40,14 +40,16 @@ void orc_rest_lengths(const double* x, const int32_t* edges, int ne, double* out
 void orc_rest_angles(const double* x, const int32_t* bends, int nb, double* out);
 
 /* ---------------------------------------------------------------- coloring */
-/* Palette graph coloring of constraints (conflict = shared vertex), restating
- * GraphColoring::Paint (xrtailor/src/physics/graph_coloring/Base.cu:12-146,163-317) and
- * the adjacency builders (impl/StretchColoring.cu:6-58, GraphColoringHelper.cu:6-37),
- * made deterministic: SplitMix64 per-constraint streams replace curand, and each
- * round's conflict resolution reads the start-of-round state.
- * cons: n x arity vertex ids.  Returns the number of colors used. */
+/* Coloring of constraints (conflict = shared vertex; adjacency as in
+ * impl/StretchColoring.cu:6-58, GraphColoringHelper.cu:6-37).
+ * mode 0: first fit in constraint index order (seed unused).
+ * mode 1: palette coloring restating GraphColoring::Paint
+ *   (xrtailor/src/physics/graph_coloring/Base.cu:12-146,163-317) made deterministic:
+ *   SplitMix64 per-constraint streams replace curand, each round's conflict resolution
+ *   reads the start-of-round state, conflicts are ranked by a hash priority.
+ * cons: n x arity vertex ids.  Colors are compacted to 0..k-1; returns k. */
 int orc_color_constraints(const int32_t* cons, int arity, int n, int nv, uint64_t seed,
-                          int32_t* colors_out);
+                          int mode, int32_t* colors_out);
 
 /* --------------------------------------------------------------- dynamics */
 typedef struct orc_params {

@@ -35,13 +35,13 @@ def _lib_bends(tris, edges):
     return out[: 4 * nb.value].reshape(-1, 4)
 
 
-def _lib_color(cons, nv, seed):
+def _lib_color(cons, nv, seed, mode=0):
     lib = _ffi.load()
     c = np.ascontiguousarray(cons, np.int32)
     out = np.zeros(c.shape[0], np.int32)
     n = C.c_int32()
     _ffi.check(lib.xrt_color_constraints(_ffi.iptr(c.reshape(-1)), c.shape[1], c.shape[0], nv,
-                                         C.c_uint64(seed), _ffi.iptr(out), C.byref(n)))
+                                         C.c_uint64(seed), mode, _ffi.iptr(out), C.byref(n)))
     return out, n.value
 
 
@@ -103,14 +103,15 @@ def test_nonmanifold_edge_yields_face_pairs():
     np.testing.assert_array_equal(b, orc.build_bends(tris, orc.build_edges(tris)))
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("arity", [2, 4])
 @pytest.mark.parametrize("seed", [0, 7])
-def test_coloring_valid_deterministic_and_equal_to_oracle(arity, seed):
+def test_coloring_valid_deterministic_and_equal_to_oracle(mode, arity, seed):
     pos, tris = scenes.grid(24, 1.0)
     e = orc.build_edges(tris)
     cons = e if arity == 2 else orc.build_bends(tris, e)
-    col, n = _lib_color(cons, len(pos), seed)
-    col_o, n_o = orc.color(cons, len(pos), seed)
+    col, n = _lib_color(cons, len(pos), seed, mode)
+    col_o, n_o = orc.color(cons, len(pos), seed, mode)
     np.testing.assert_array_equal(col, col_o)
     assert n == n_o
     assert set(np.unique(col)) == set(range(n))  # compact
@@ -118,16 +119,40 @@ def test_coloring_valid_deterministic_and_equal_to_oracle(arity, seed):
     for c in range(n):
         verts = cons[col == c].reshape(-1)
         assert len(np.unique(verts)) == len(verts)
-    col2, _ = _lib_color(cons, len(pos), seed)
+    col2, _ = _lib_color(cons, len(pos), seed, mode)
     np.testing.assert_array_equal(col, col2)
-    assert n <= (12 if arity == 2 else 40)
+    assert n <= (12 if arity == 2 else 24)
 
 
-def test_coloring_depends_on_seed():
+def test_greedy_coloring_near_lower_bound_on_grids():
+    """Edges at a grid vertex (degree 6) need 6 colors; the greedy pass reaches that, and
+    the bend set (>= 12 elements per interior vertex) needs at most 14."""
+    pos, tris = scenes.grid(48, 1.0)
+    e = orc.build_edges(tris)
+    _, n = _lib_color(e, len(pos), 0, 0)
+    assert n == 6
+    _, nb = _lib_color(orc.build_bends(tris, e), len(pos), 0, 0)
+    assert 12 <= nb <= 14
+
+
+def test_coloring_is_greedy_first_fit_on_a_path():
+    # chain of edges 0-1-2-3-4: first fit alternates 0,1,0,1
+    cons = np.array([[0, 1], [1, 2], [2, 3], [3, 4]], np.int32)
+    col, n = _lib_color(cons, 5, 0, 0)
+    np.testing.assert_array_equal(col, [0, 1, 0, 1])
+    # a star (all edges share vertex 0) needs one color per edge
+    star = np.array([[0, k] for k in range(1, 6)], np.int32)
+    col, n = _lib_color(star, 6, 0, 0)
+    assert n == 5 and len(set(col)) == 5
+    col, n = _lib_color(star, 6, 3, 1)
+    assert n == 5 and len(set(col)) == 5
+
+
+def test_palette_coloring_depends_on_seed():
     pos, tris = scenes.grid(16, 1.0)
     e = orc.build_edges(tris)
-    a, _ = _lib_color(e, len(pos), 1)
-    b, _ = _lib_color(e, len(pos), 2)
+    a, _ = _lib_color(e, len(pos), 1, 1)
+    b, _ = _lib_color(e, len(pos), 2, 1)
     assert (a != b).any()
 
 
@@ -136,7 +161,11 @@ def test_host_helpers_reject_bad_input():
     cons = np.array([[0, 5]], np.int32)
     out = np.zeros(1, np.int32)
     n = C.c_int32()
-    rc = lib.xrt_color_constraints(_ffi.iptr(cons.reshape(-1)), 2, 1, 3, C.c_uint64(0),
+    rc = lib.xrt_color_constraints(_ffi.iptr(cons.reshape(-1)), 2, 1, 3, C.c_uint64(0), 0,
+                                   _ffi.iptr(out), C.byref(n))
+    assert rc == -1
+    ok = np.array([[0, 1]], np.int32)
+    rc = lib.xrt_color_constraints(_ffi.iptr(ok.reshape(-1)), 2, 1, 3, C.c_uint64(0), 9,
                                    _ffi.iptr(out), C.byref(n))
     assert rc == -1
     pos, tris = scenes.grid(4, 1.0)

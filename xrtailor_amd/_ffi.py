@@ -50,6 +50,7 @@ class XrtParams(C.Structure):
         ("enable_self_contact", C.c_int32),
         ("pair_update_interval", C.c_int32),
         ("bend_model", C.c_int32),
+        ("coloring", C.c_int32),
     ]
 
 
@@ -72,7 +73,7 @@ SIGNATURES = {
     "xrt_device_info": [_P, C.POINTER(C.c_int64)],
     "xrt_build_edges": [_IP, _I, _IP, _I, _IP],
     "xrt_build_bends": [_IP, _I, _IP, _I, _IP, _I, _IP],
-    "xrt_color_constraints": [_IP, _I, _I, _I, C.c_uint64, _IP, _IP],
+    "xrt_color_constraints": [_IP, _I, _I, _I, C.c_uint64, _I, _IP, _IP],
     "xrt_add_cloth": [_P, _FP, _I, _IP, _I, _F, _F, _IP],
     "xrt_pin_vertices": [_P, _I, _IP, _I],
     "xrt_set_stretch_compliance": [_P, _I, _FP],

@@ -225,6 +225,7 @@ void xrt_default_params(xrt_params* p) {
   p->enable_self_contact = 0;
   p->pair_update_interval = 10;
   p->bend_model = XRT_BEND_REFERENCE;
+  p->coloring = XRT_COLOR_GREEDY;
 }
 
 int xrt_create(xrt_ctx** out, int device) {
@@ -277,6 +278,8 @@ int xrt_set_params(xrt_ctx* ctx, const xrt_params* p) {
   XRT_REQUIRE(p->num_collision_passes > 0, XRT_ERR_INVALID_ARG, "num_collision_passes must be > 0");
   XRT_REQUIRE(p->bend_model == XRT_BEND_REFERENCE || p->bend_model == XRT_BEND_CONSISTENT,
               XRT_ERR_INVALID_ARG, "unknown bend_model");
+  XRT_REQUIRE(p->coloring == XRT_COLOR_GREEDY || p->coloring == XRT_COLOR_PALETTE,
+              XRT_ERR_INVALID_ARG, "unknown coloring mode");
   XRT_REQUIRE(std::isfinite(p->gravity[0]) && std::isfinite(p->gravity[1]) &&
                   std::isfinite(p->gravity[2]) && std::isfinite(p->damping),
               XRT_ERR_INVALID_ARG, "non-finite parameter");
@@ -328,13 +331,16 @@ int xrt_build_bends(const int32_t* tris, int32_t nt, const int32_t* edges, int32
 }
 
 int xrt_color_constraints(const int32_t* cons, int32_t arity, int32_t n, int32_t nv,
-                          uint64_t seed, int32_t* colors_out, int32_t* ncolors_out) {
+                          uint64_t seed, int32_t mode, int32_t* colors_out,
+                          int32_t* ncolors_out) {
   XRT_REQUIRE(arity > 0 && n >= 0 && nv >= 0 && ncolors_out && (n == 0 || (cons && colors_out)),
               XRT_ERR_INVALID_ARG, "bad coloring arguments");
+  XRT_REQUIRE(mode == XRT_COLOR_GREEDY || mode == XRT_COLOR_PALETTE, XRT_ERR_INVALID_ARG,
+              "unknown coloring mode");
   for (int i = 0; i < n * arity; ++i)
     XRT_REQUIRE(cons[i] >= 0 && cons[i] < nv, XRT_ERR_INVALID_ARG, "vertex id out of range");
   int ncol = 0;
-  std::vector<int32_t> col = ColorConstraints(cons, arity, n, nv, seed, &ncol);
+  std::vector<int32_t> col = ColorConstraints(cons, arity, n, nv, seed, mode, &ncol);
   std::copy(col.begin(), col.end(), colors_out);
   *ncolors_out = ncol;
   return XRT_OK;
@@ -431,8 +437,9 @@ int xrt_start(xrt_ctx* ctx, uint64_t seed) {
   for (size_t ii = 0; ii < ctx->inst.size(); ++ii) {
     HostInstance& h = ctx->inst[ii];
     const uint64_t iseed = seed + ii;
-    h.s_color = ColorConstraints(h.edges.data(), 2, h.ne(), h.nv, iseed, &h.s_ncol);
-    h.b_color = ColorConstraints(h.bends.data(), 4, h.nb(), h.nv, iseed, &h.b_ncol);
+    const int mode = ctx->params.coloring;
+    h.s_color = ColorConstraints(h.edges.data(), 2, h.ne(), h.nv, iseed, mode, &h.s_ncol);
+    h.b_color = ColorConstraints(h.bends.data(), 4, h.nb(), h.nv, iseed, mode, &h.b_ncol);
     ColorSort(h.s_color, h.s_ncol, &h.s_order, &h.s_coloff);
     ColorSort(h.b_color, h.b_ncol, &h.b_order, &h.b_coloff);
     // per-target LRA CSR, slots in registration order; unreachable targets are dropped

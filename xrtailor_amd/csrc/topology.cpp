@@ -118,7 +118,7 @@ uint64_t SplitMix64(uint64_t* state) {
 }
 
 std::vector<int32_t> ColorConstraints(const int32_t* cons, int arity, int n, int nv,
-                                      uint64_t seed, int* n_colors) {
+                                      uint64_t seed, int mode, int* n_colors) {
   *n_colors = 0;
   if (n <= 0) return {};
   // vertex -> constraints
@@ -143,61 +143,82 @@ std::vector<int32_t> ColorConstraints(const int32_t* cons, int arity, int n, int
     adj.insert(adj.end(), tmp.begin(), tmp.end());
     aptr[c + 1] = static_cast<int32_t>(adj.size());
   }
-  int min_deg = 1 << 30;
-  for (int c = 0; c < n; ++c) min_deg = std::min(min_deg, aptr[c + 1] - aptr[c]);
-  min_deg = std::max(min_deg, 1);
-
-  std::vector<int32_t> pal_ptr(n + 1, 0);
-  for (int c = 0; c < n; ++c) pal_ptr[c + 1] = pal_ptr[c] + (aptr[c + 1] - aptr[c]) + 2;
-  std::vector<uint8_t> forbid(pal_ptr[n], 0), colored(n, 0), accept(n, 0);
-  std::vector<int32_t> vsize(n), color(n, 0);
-  std::vector<uint64_t> rs(n);
-  uint64_t s0 = seed;
-  const uint64_t base = SplitMix64(&s0);
-  for (int c = 0; c < n; ++c) {
-    vsize[c] = (aptr[c + 1] - aptr[c]) / min_deg + 1;  // Initialization_Kernel, Base.cu:19-20
-    rs[c] = base + static_cast<uint64_t>(c) * 0xD1B54A32D192ED03ULL;
-  }
-  int remaining = n;
-  while (remaining > 0) {
-    // tentative coloring (TentativeColoring_Kernel, Base.cu:35-69)
+  std::vector<int32_t> color(n, 0);
+  if (mode == kColorGreedy) {
+    // first fit in constraint index order: each constraint takes the smallest color no
+    // lower-indexed neighbour holds (the index order follows the mesh's face order)
+    std::vector<int32_t> stamp(static_cast<size_t>(n) + 2, -1);
     for (int c = 0; c < n; ++c) {
-      if (colored[c]) continue;
-      const uint64_t r = SplitMix64(&rs[c]);
-      const double u = (static_cast<double>(r >> 11) + 1.0) * (1.0 / 9007199254740992.0);
-      const int vs = vsize[c];
-      int lc = static_cast<int>(u * (static_cast<double>(vs) - 1.0 + 0.999999));
-      for (int t = 0; t < vs && forbid[pal_ptr[c] + lc]; ++t) lc = (lc + 1) % vs;
-      color[c] = lc;
+      for (int k = aptr[c]; k < aptr[c + 1]; ++k)
+        if (adj[k] < c) stamp[color[adj[k]]] = c;
+      int col = 0;
+      while (stamp[col] == c) ++col;
+      color[c] = col;
     }
-    // conflict resolution against the start-of-round state (ConflictResolution_Kernel, Base.cu:71-118)
+  } else {
+    int min_deg = 1 << 30;
+    for (int c = 0; c < n; ++c) min_deg = std::min(min_deg, aptr[c + 1] - aptr[c]);
+    min_deg = std::max(min_deg, 1);
+    std::vector<int32_t> pal_ptr(n + 1, 0);
+    for (int c = 0; c < n; ++c) pal_ptr[c + 1] = pal_ptr[c] + (aptr[c + 1] - aptr[c]) + 2;
+    std::vector<uint8_t> forbid(pal_ptr[n], 0), colored(n, 0), accept(n, 0);
+    std::vector<int32_t> vsize(n), act(n);
+    std::vector<uint64_t> rs(n), prio(n);
+    uint64_t s0 = seed;
+    const uint64_t base = SplitMix64(&s0);
     for (int c = 0; c < n; ++c) {
-      accept[c] = 0;
-      if (colored[c]) continue;
-      bool conflict_uncolored = false, conflict_colored = false, highest = true;
-      for (int k = aptr[c]; k < aptr[c + 1]; ++k) {
-        const int j = adj[k];
-        if (color[j] == color[c]) (colored[j] ? conflict_colored : conflict_uncolored) = true;
-        if (!colored[j] && c < j) highest = false;
+      vsize[c] = (aptr[c + 1] - aptr[c]) / min_deg + 1;  // Initialization_Kernel, Base.cu:19-20
+      rs[c] = base + static_cast<uint64_t>(c) * 0xD1B54A32D192ED03ULL;
+      uint64_t ps = rs[c] ^ 0x5851F42D4C957F2DULL;
+      prio[c] = SplitMix64(&ps);  // conflict ranking: hash priority, ties by index
+      act[c] = c;
+    }
+    auto outranks = [&](int a, int b) { return prio[a] > prio[b] || (prio[a] == prio[b] && a > b); };
+    int nact = n;
+    while (nact > 0) {
+      // tentative coloring (TentativeColoring_Kernel, Base.cu:35-69)
+      for (int q = 0; q < nact; ++q) {
+        const int c = act[q];
+        const uint64_t r = SplitMix64(&rs[c]);
+        const double u = (static_cast<double>(r >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+        const int vs = vsize[c];
+        int lc = static_cast<int>(u * (static_cast<double>(vs) - 1.0 + 0.999999));
+        for (int t = 0; t < vs && forbid[pal_ptr[c] + lc]; ++t) lc = (lc + 1) % vs;
+        color[c] = lc;
       }
-      if (conflict_colored || (conflict_uncolored && !highest)) continue;
-      accept[c] = 1;
-    }
-    for (int c = 0; c < n; ++c) {
-      if (!accept[c]) continue;
-      colored[c] = 1;
-      --remaining;
-      for (int k = aptr[c]; k < aptr[c + 1]; ++k) {
-        const int j = adj[k];
-        if (color[c] < pal_ptr[j + 1] - pal_ptr[j]) forbid[pal_ptr[j] + color[c]] = 1;
+      // conflict resolution on the start-of-round state (ConflictResolution_Kernel, Base.cu:71-118)
+      for (int q = 0; q < nact; ++q) {
+        const int c = act[q];
+        accept[c] = 0;
+        bool conflict_uncolored = false, conflict_colored = false, highest = true;
+        for (int k = aptr[c]; k < aptr[c + 1]; ++k) {
+          const int j = adj[k];
+          if (color[j] == color[c]) (colored[j] ? conflict_colored : conflict_uncolored) = true;
+          if (!colored[j] && outranks(j, c)) highest = false;
+        }
+        if (conflict_colored || (conflict_uncolored && !highest)) continue;
+        accept[c] = 1;
       }
-    }
-    // feed the hungry (FeedTheHungry_Kernel, Base.cu:120-139)
-    for (int c = 0; c < n; ++c) {
-      if (colored[c]) continue;
-      bool avail = false;
-      for (int t = 0; t < vsize[c] && !avail; ++t) avail = !forbid[pal_ptr[c] + t];
-      if (!avail && vsize[c] < pal_ptr[c + 1] - pal_ptr[c]) vsize[c]++;
+      for (int q = 0; q < nact; ++q) {
+        const int c = act[q];
+        if (!accept[c]) continue;
+        colored[c] = 1;
+        for (int k = aptr[c]; k < aptr[c + 1]; ++k) {
+          const int j = adj[k];
+          if (color[c] < pal_ptr[j + 1] - pal_ptr[j]) forbid[pal_ptr[j] + color[c]] = 1;
+        }
+      }
+      // feed the hungry (FeedTheHungry_Kernel, Base.cu:120-139); keep the uncolored active
+      int na = 0;
+      for (int q = 0; q < nact; ++q) {
+        const int c = act[q];
+        if (colored[c]) continue;
+        bool avail = false;
+        for (int t = 0; t < vsize[c] && !avail; ++t) avail = !forbid[pal_ptr[c] + t];
+        if (!avail && vsize[c] < pal_ptr[c + 1] - pal_ptr[c]) vsize[c]++;
+        act[na++] = c;
+      }
+      nact = na;
     }
   }
   // compact to 0..k-1 so every used color is visited by the color loop

@@ -20,10 +20,15 @@ std::vector<int32_t> BuildBends(const int32_t* tris, int nt, const std::vector<i
 std::vector<float> RestLengths(const float* pos, const std::vector<int32_t>& edges);
 std::vector<float> RestAngles(const float* pos, const std::vector<int32_t>& bends);
 
-// Deterministic palette coloring (GraphColoring::Paint,
-// xrtailor/src/physics/graph_coloring/Base.cu:12-317): returns compact colors 0..k-1.
+// Constraint coloring, conflict = shared vertex; returns compact colors 0..k-1.
+//  kColorGreedy : first fit in constraint index order (seed unused)
+//  kColorPalette: GraphColoring::Paint (xrtailor/src/physics/graph_coloring/Base.cu:12-317)
+//                 made deterministic (SplitMix64 streams, start-of-round resolution, hash
+//                 priority instead of the bare index for ranking conflicts)
+constexpr int kColorGreedy = 0;
+constexpr int kColorPalette = 1;
 std::vector<int32_t> ColorConstraints(const int32_t* cons, int arity, int n, int nv,
-                                      uint64_t seed, int* n_colors);
+                                      uint64_t seed, int mode, int* n_colors);
 
 // Dijkstra over the edge graph (Geodesic::ComputeGeodesicDistance,
 // xrtailor/src/utils/Geodesic.cpp:6-79).
