@@ -175,6 +175,46 @@ int xrt_time_batched_substep(xrt_ctx* ctx, int32_t n, float* avg_ms);
 /* wall time (ms, HIP events on the context stream) of `nframes` plain or batched frames */
 int xrt_time_frames(xrt_ctx* ctx, int32_t batched, int32_t nframes, float* ms);
 
+/* ------------------------------------------------------------------- LBVH */
+/* Standalone linear BVH over triangle faces (Karras 2012), the reference's BVH class
+ * (xrtailor/include/xrtailor/physics/broad_phase/lbvh/BVH.cuh:77-125).  Leaves are the
+ * faces; node arrays follow the reference layout: internal nodes [0, F-1), leaves
+ * [F-1, 2F-1) in Morton order; node = (parent, left, right, object), -1 for none
+ * (0xFFFFFFFF in the reference); boxes = (lower xyz, upper xyz). */
+typedef struct xrt_bvh xrt_bvh;
+int xrt_bvh_create(xrt_bvh** out, int device);
+int xrt_bvh_destroy(xrt_bvh* bvh);
+/* BVH::UpdateData(faces, tol, ccd, n) + BVH::Construct(scr) (BVH.cu:565-573,618-722): face
+ * corners x0_k + tol n_k (and x_k + tol n_k when ccd); x and n may be NULL (n = 0). */
+int xrt_bvh_build(xrt_bvh* bvh, const float* x0, const float* x, const float* n, int32_t nv,
+                  const int32_t* tris, int32_t nt, float tol, int32_t ccd, float scr);
+/* BVH::RefitMidstep(faces, tol) (BVH.cu:795-818): corners x1 + tol n1, leaf boxes grow,
+ * parents are re-merged */
+int xrt_bvh_refit_midstep(xrt_bvh* bvh, const float* x1, const float* n1, float tol);
+/* BVH::Update(faces, ccd) (BVH.cu:839-852): leaf boxes grow by the faces' bounds of x (and
+ * x0 with ccd), ancestors grow */
+int xrt_bvh_update(xrt_bvh* bvh, const float* x, const float* x0, int32_t ccd);
+/* BVH::NumObjects */
+int xrt_bvh_num_objects(xrt_bvh* bvh, int32_t* n);
+/* sorted Morton codes (F), sorted object order (F), NodesHost (4*(2F-1)), AABBsHost
+ * (6*(2F-1)), InternalEscapeIndexTableHost (F), ExternalEscapeIndexTableHost (F),
+ * GetRootAABB (6); any pointer may be NULL */
+int xrt_bvh_get(xrt_bvh* bvh, uint32_t* morton, int32_t* order, int32_t* nodes, float* aabbs,
+                int32_t* i_escape, int32_t* e_escape, float* root);
+/* QueryDeviceStackless (BVH.cu:957-1021): strict point containment (p1 == NULL) or segment
+ * p0->p1 overlap; hits is n*256 (LBVH_MAX_BUFFER_SIZE), counts is n */
+int xrt_bvh_query(xrt_bvh* bvh, const float* p0, const float* p1, int32_t n, int32_t* hits,
+                  int32_t* counts);
+/* Traverse(bvh_lhs, bvh_rhs, faces_lhs, faces_rhs, thickness) (BVH.cuh:139-152): pairs of
+ * (lhs face, rhs face) whose leaf boxes overlap after dilating the lhs box by thickness;
+ * *n_out receives the count; XRT_ERR_CAPACITY if it exceeds cap (first cap written) */
+int xrt_bvh_traverse(xrt_bvh* lhs, xrt_bvh* rhs, float thickness, int32_t* pairs, int64_t cap,
+                     int64_t* n_out);
+/* PhysicsMesh::UpdateNormals vertex normals (PhysicsMeshHelper.cu:196-210,282-295), summed
+ * per vertex in ascending face order (deterministic) */
+int xrt_vertex_normals(xrt_bvh* bvh, const float* x, int32_t nv, const int32_t* tris, int32_t nt,
+                       float* n_out);
+
 #ifdef __cplusplus
 }
 #endif

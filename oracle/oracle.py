@@ -219,3 +219,139 @@ def params_from(xp) -> tuple[OrcParams, float]:
     p.bend_model = xp.bend_model
     dt = float(np.float32(xp.frame_time) / np.float32(xp.num_substeps))
     return p, dt
+
+
+# ---------------------------------------------------------------------- LBVH
+_F = C.POINTER(C.c_float)
+_U = C.POINTER(C.c_uint32)
+_bvh_bound = False
+
+
+def _bind_bvh():
+    global _bvh_bound
+    lib = load()
+    if _bvh_bound:
+        return lib
+    lib.orc_vertex_normals.argtypes = [_F, C.c_int, _I, C.c_int, _F]
+    lib.orc_vertex_normals.restype = None
+    lib.orc_expand_bits.argtypes = [C.c_uint32]
+    lib.orc_expand_bits.restype = C.c_uint32
+    lib.orc_morton_code.argtypes = [C.c_float, C.c_float, C.c_float]
+    lib.orc_morton_code.restype = C.c_uint32
+    lib.orc_bvh_create.restype = C.c_void_p
+    lib.orc_bvh_destroy.argtypes = [C.c_void_p]
+    lib.orc_bvh_destroy.restype = None
+    lib.orc_bvh_build.argtypes = [C.c_void_p, _F, _F, _F, C.c_int, _I, C.c_int, C.c_float, C.c_int,
+                                  C.c_float]
+    lib.orc_bvh_num_objects.argtypes = [C.c_void_p]
+    lib.orc_bvh_get.argtypes = [C.c_void_p, _U, _I, _I, _F, _I, _F]
+    lib.orc_bvh_get.restype = None
+    lib.orc_bvh_refit_midstep.argtypes = [C.c_void_p, _F, _F, C.c_float]
+    lib.orc_bvh_refit_midstep.restype = None
+    lib.orc_bvh_update.argtypes = [C.c_void_p, _F, _F, C.c_int]
+    lib.orc_bvh_update.restype = None
+    lib.orc_bvh_query.argtypes = [C.c_void_p, _F, _F, _I, C.c_int]
+    lib.orc_overlap_segment.argtypes = [_F, _F, _F]
+    lib.orc_bvh_traverse.argtypes = [C.c_void_p, C.c_void_p, C.c_float, _I, C.c_long]
+    lib.orc_bvh_traverse.restype = C.c_long
+    _bvh_bound = True
+    return lib
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, np.float32).reshape(-1)
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(_F)
+
+
+def vertex_normals(x, tris) -> np.ndarray:
+    lib = _bind_bvh()
+    x = _f32(x)
+    t = np.ascontiguousarray(tris, np.int32).reshape(-1)
+    out = np.zeros_like(x)
+    lib.orc_vertex_normals(_fp(x), x.size // 3, _i(t), t.size // 3, _fp(out))
+    return out.reshape(-1, 3)
+
+
+def expand_bits(v: int) -> int:
+    return int(_bind_bvh().orc_expand_bits(v))
+
+
+def morton_code(x: float, y: float, z: float) -> int:
+    return int(_bind_bvh().orc_morton_code(x, y, z))
+
+
+def overlap_segment(box6, p0, p1) -> bool:
+    b, a, c = _f32(box6), _f32(p0), _f32(p1)
+    return bool(_bind_bvh().orc_overlap_segment(_fp(b), _fp(a), _fp(c)))
+
+
+class Bvh:
+    """FP32-strict oracle LBVH with the same surface as xrtailor_amd.bvh.BVH."""
+
+    def __init__(self):
+        self.lib = _bind_bvh()
+        self.h = C.c_void_p(self.lib.orc_bvh_create())
+
+    def __del__(self):
+        try:
+            self.lib.orc_bvh_destroy(self.h)
+        except Exception:
+            pass
+
+    def Build(self, x0, tris, tol=0.0, n=None, x=None, ccd=False, scr=0.0):
+        x0, x, n = _f32(x0), _f32(x), _f32(n)
+        t = np.ascontiguousarray(tris, np.int32).reshape(-1)
+        self._keep = (x0, x, n, t)
+        self.lib.orc_bvh_build(self.h, _fp(x0), _fp(x), _fp(n), x0.size // 3, _i(t), t.size // 3,
+                               tol, int(ccd), scr)
+
+    def RefitMidstep(self, x1, n1, tol):
+        x1, n1 = _f32(x1), _f32(n1)
+        self.lib.orc_bvh_refit_midstep(self.h, _fp(x1), _fp(n1), tol)
+
+    def Update(self, x, x0=None, ccd=False):
+        x, x0 = _f32(x), _f32(x0)
+        self.lib.orc_bvh_update(self.h, _fp(x), _fp(x0), int(ccd))
+
+    def NumObjects(self) -> int:
+        return int(self.lib.orc_bvh_num_objects(self.h))
+
+    def host(self) -> dict:
+        f = self.NumObjects()
+        nn = max(2 * f - 1, 0)
+        m = np.zeros(f, np.uint32)
+        o = np.zeros(f, np.int32)
+        nodes = np.zeros(4 * nn, np.int32)
+        box = np.zeros(6 * nn, np.float32)
+        esc = np.zeros(max(nn, 1), np.int32)
+        root = np.zeros(6, np.float32)
+        self.lib.orc_bvh_get(self.h, m.ctypes.data_as(_U), _i(o), _i(nodes), _fp(box), _i(esc),
+                             _fp(root))
+        ni = f - 1
+        ie = np.full(f, -1, np.int32)
+        ie[:ni] = esc[:ni]
+        return dict(morton=m, order=o, nodes=nodes.reshape(-1, 4), aabbs=box.reshape(-1, 6),
+                    i_escape=ie, e_escape=esc[ni:nn].copy(), root=root, escape=esc[:nn])
+
+    def Query(self, p0, p1=None, cap=256):
+        p0, p1 = _f32(p0), _f32(p1)
+        n = p0.size // 3
+        out = []
+        buf = np.zeros(cap, np.int32)
+        for q in range(n):
+            a = p0[3 * q:3 * q + 3].copy()
+            b = None if p1 is None else p1[3 * q:3 * q + 3].copy()
+            k = self.lib.orc_bvh_query(self.h, _fp(a), _fp(b), _i(buf), cap)
+            out.append(buf[:k].copy())
+        return out
+
+
+def traverse(lhs: Bvh, rhs: Bvh, thickness: float) -> np.ndarray:
+    lib = _bind_bvh()
+    n = lib.orc_bvh_traverse(lhs.h, rhs.h, thickness, None, 0)
+    pairs = np.zeros(2 * max(n, 1), np.int32)
+    lib.orc_bvh_traverse(lhs.h, rhs.h, thickness, _i(pairs), n)
+    return pairs[: 2 * n].reshape(-1, 2)

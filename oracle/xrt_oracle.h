@@ -124,6 +124,32 @@ void orc_geodesic(const double* x, int nv, const int32_t* edges, int ne, int src
 /* 64-bit SplitMix64 mixer shared with the library (test of the coloring streams). */
 uint64_t orc_splitmix64(uint64_t* state);
 
+/* ------------------------------------------------- LBVH (FP32 strict, bvh_oracle.c) */
+/* vertex normals, summed per vertex in ascending face order
+ * (UpdateNodeNormals, xrtailor/src/physics/PhysicsMeshHelper.cu:196-210,282-295) */
+void orc_vertex_normals(const float* x, int nv, const int32_t* tris, int nt, float* n_out);
+uint32_t orc_expand_bits(uint32_t v);
+uint32_t orc_morton_code(float x, float y, float z);
+typedef struct orc_bvh orc_bvh;
+orc_bvh* orc_bvh_create(void);
+void orc_bvh_destroy(orc_bvh* b);
+/* UpdateData + Construct (xrtailor/src/physics/broad_phase/lbvh/BVH.cu:382-722); x, n may be NULL */
+int orc_bvh_build(orc_bvh* b, const float* x0, const float* x, const float* n, int nv,
+                  const int32_t* tris, int nt, float tol, int ccd, float scr);
+int orc_bvh_num_objects(const orc_bvh* b);
+/* sorted Morton codes, sorted object order, nodes (parent,left,right,object; -1 = none),
+ * boxes (lower xyz, upper xyz), node-indexed escape table, root box */
+void orc_bvh_get(const orc_bvh* b, uint32_t* morton, int32_t* order, int32_t* nodes, float* box,
+                 int32_t* escape, float* root);
+const float* orc_bvh_pred(const orc_bvh* b);
+void orc_bvh_refit_midstep(orc_bvh* b, const float* x1, const float* n1, float tol);
+void orc_bvh_update(orc_bvh* b, const float* x, const float* x0, int ccd);
+/* stackless point (p1 == NULL) or segment query; returns hits (<= cap) */
+int orc_bvh_query(const orc_bvh* b, const float* p0, const float* p1, int32_t* hits, int cap);
+int orc_overlap_segment(const float* box6, const float* p0, const float* p1);
+long orc_bvh_traverse(const orc_bvh* lhs, const orc_bvh* rhs, float thickness, int32_t* pairs,
+                      long cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -101,7 +101,7 @@ def _mixed_specs(bend_model=1):
                                     bend=_bend_compliance(bend_model, 10 ** (-4 + k / 2)))
         out.append(spec)
     # one instance with LRA slots on its pinned row
-    lra = scenes.hanging_cloth(seed=30, n=24)
+    lra = scenes.hanging_cloth(seed=30, n=24, bend=_bend_compliance(bend_model, 1e-4))
     lra.lra_slots = np.array([0, 23], np.int32)
     out.append(lra)
     return out

@@ -101,6 +101,16 @@ SIGNATURES = {
     "xrt_frames_batched": [_P, _I],
     "xrt_time_batched_substep": [_P, _I, _FP],
     "xrt_time_frames": [_P, _I, _I, _FP],
+    "xrt_bvh_create": [C.POINTER(_P), C.c_int],
+    "xrt_bvh_destroy": [_P],
+    "xrt_bvh_build": [_P, _FP, _FP, _FP, _I, _IP, _I, _F, _I, _F],
+    "xrt_bvh_refit_midstep": [_P, _FP, _FP, _F],
+    "xrt_bvh_update": [_P, _FP, _FP, _I],
+    "xrt_bvh_num_objects": [_P, _IP],
+    "xrt_bvh_get": [_P, C.POINTER(C.c_uint32), _IP, _IP, _FP, _IP, _IP, _FP],
+    "xrt_bvh_query": [_P, _FP, _FP, _I, _IP, _IP],
+    "xrt_bvh_traverse": [_P, _P, _F, _IP, C.c_int64, C.POINTER(C.c_int64)],
+    "xrt_vertex_normals": [_P, _FP, _I, _IP, _I, _FP],
 }
 _RESTYPES = {"xrt_last_error": C.c_char_p, "xrt_default_params": None}
 

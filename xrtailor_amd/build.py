@@ -50,7 +50,8 @@ def build_lib(jobs: int = 8) -> Path:
     OBJ.mkdir(parents=True, exist_ok=True)
     steps = []
     for src in sorted(CSRC.glob("*.hip")):
-        flags = HIP_FLAGS + (["-ffp-contract=off"] if src.name in STRICT_FP else [])
+        strict = ["-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt"]
+        flags = HIP_FLAGS + (strict if src.name in STRICT_FP else [])
         steps.append([HIPCC, *flags, "-c", str(src), "-o", str(OBJ / (src.stem + ".o"))])
     for src in sorted(CSRC.glob("*.cpp")):
         steps.append(["g++", *CXX_FLAGS, "-c", str(src), "-o", str(OBJ / (src.stem + ".cpp.o"))])
