@@ -220,8 +220,10 @@ __global__ __launch_bounds__(kBlock) void RadixScatterKernel(
     __syncthreads();
     if (valid) {
       const uint32_t pos = wc[w][d] + rank;
-      kout[pos] = key;
-      vout[pos] = vin[i];
+      if (pos < static_cast<uint32_t>(n)) {  // always true when histogram and scatter agree
+        kout[pos] = key;
+        vout[pos] = vin[i];
+      }
     }
     __syncthreads();
   }
@@ -322,7 +324,8 @@ __global__ __launch_bounds__(kBlock) void BottomUpKernel(const int4* __restrict_
   if (i >= nf) return;
   int node = nf - 1 + i;
   int p = nodes[node].x;
-  while (p != -1) {
+  // a root-ward walk is at most nf - 1 levels; the bound also guards against a malformed tree
+  for (int level = 0; p >= 0 && p < nf - 1 && level < nf; ++level) {
     __threadfence();
     const int old = atomicAdd(&flags[p], 1);
     if (old == 0) return;
@@ -347,9 +350,9 @@ __global__ __launch_bounds__(kBlock) void EscapeKernel(const int4* __restrict__ 
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nn) return;
   int cur = k, e = -1;
-  for (;;) {
+  for (int level = 0; level < nn; ++level) {
     const int p = nodes[cur].x;
-    if (p < 0) break;
+    if (p < 0 || p >= nn) break;
     const int4 pn = nodes[p];
     if (pn.y == cur) {
       e = pn.z;
@@ -432,7 +435,8 @@ __global__ __launch_bounds__(kBlock) void CountPairsKernel(BvhView l, BvhView r,
   const Box lb = l.box[l.nf - 1 + id];
   int node = 0;
   uint32_t c = 0;
-  while (node != -1) {
+  const int max_steps = 2 * r.nf;  // each node is visited at most once
+  for (int step = 0; node != -1 && step < max_steps; ++step) {
     const int4 nd = r.nodes[node];
     if (OverlapDilated(r.box[node], lb, th)) {
       if (nd.w >= 0) {
@@ -457,13 +461,15 @@ __global__ __launch_bounds__(kBlock) void FindPairsKernel(BvhView l, BvhView r, 
   if (id >= l.nf) return;
   const Box lb = l.box[l.nf - 1 + id];
   const int lobj = l.nodes[l.nf - 1 + id].w;
-  int64_t index = static_cast<int64_t>(off[id]) + cnt[id] - 1;
+  const int64_t lo = off[id];
+  int64_t index = lo + cnt[id] - 1;
   int node = 0;
-  while (node != -1) {
+  const int max_steps = 2 * r.nf;
+  for (int step = 0; node != -1 && step < max_steps; ++step) {
     const int4 nd = r.nodes[node];
     if (OverlapDilated(r.box[node], lb, th)) {
       if (nd.w >= 0) {
-        if (index < cap) pairs[index] = make_int2(lobj, nd.w);
+        if (index >= lo && index < cap) pairs[index] = make_int2(lobj, nd.w);
         --index;
         node = r.escape[node];
       } else {

@@ -77,7 +77,8 @@ __device__ __forceinline__ bool OverlapDilated(const Box& rhs, const Box& lhs, f
 template <bool kSegment, typename F>
 __device__ __forceinline__ int StacklessQuery(const BvhView& t, float4 p, float4 q, F&& f) {
   int node = 0, found = 0;
-  while (node != -1) {
+  const int max_steps = 2 * t.nf;  // a stackless pass visits each node at most once
+  for (int step = 0; node != -1 && step < max_steps; ++step) {
     if (found >= kMaxHits) break;
     const Box bx = t.box[node];
     const bool hit = kSegment ? OverlapSegment(bx, p, q) : OverlapPoint(bx, p);
