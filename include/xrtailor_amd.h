@@ -67,6 +67,9 @@ typedef struct xrt_params {
    * fewest colors on structured meshes) or XRT_COLOR_PALETTE (GraphColoring::Paint,
    * xrtailor/src/physics/graph_coloring/Base.cu:163-317, made deterministic from the seed) */
   int32_t coloring;
+  /* capacity of the repulsion face-pair buffer per cloth face (fixed at xrt_start); a
+   * larger pair count is reported by xrt_get_contacts and the excess pairs are ignored */
+  int32_t pair_capacity_per_face;
 } xrt_params;
 
 enum { XRT_BEND_REFERENCE = 0, XRT_BEND_CONSISTENT = 1 };
@@ -119,9 +122,23 @@ int xrt_set_stretch_compliance(xrt_ctx* ctx, int32_t inst, const float* complian
  * ClothObjectBase::GenerateGeodesicLongRangeConstraints
  * (xrtailor/src/utils/Geodesic.cpp:6-79; xrtailor/src/pipeline/base/ClothObjectBase.cpp:80-94) */
 int xrt_add_lra_slots(xrt_ctx* ctx, int32_t inst, const int32_t* slots, int32_t n);
-/* ClothSolverBase::Start / SetupInternalDynamics (ClothSolverBase.cpp:453-487): colors the
- * stretch and bend constraints of every instance from `seed` and uploads the scene. */
+/* ClothSolverUniversal::AddObstacle (xrtailor/src/pipeline/impl/universal/ClothSolver.cpp:189-245):
+ * the instance's obstacle mesh (nodes not free, inverse mass 1e-3) */
+int xrt_set_obstacle(xrt_ctx* ctx, int32_t inst, const float* pos, int32_t nv, const int32_t* tris,
+                     int32_t nt);
+/* ClothSolverBase::Start / SetupInternalDynamics / SetupRTriangle (ClothSolverBase.cpp:453-525):
+ * colors the stretch and bend constraints of every instance from `seed`, builds the contact
+ * topology (R-triangles, edge-face adjacency) and uploads the scene. */
 int xrt_start(xrt_ctx* ctx, uint64_t seed);
+/* UpdateObstacleAnimationFrame: the obstacle's target positions for the next frame (x); the
+ * substeps interpolate from the frame's start positions (PhysicsMesh::Interpolate) */
+int xrt_update_obstacle(xrt_ctx* ctx, int32_t inst, const float* pos);
+/* contact records of the last collision pass: VF-DCD (face, nearest point) and VF-CCD (face,
+ * hit point) per cloth vertex, EE-DCD per cloth edge as 16 floats (dist, s, t, -, p2, -, q2,
+ * -, fn, -); stats = obstacle pairs, total pairs, pair capacity, overflow flag.  Any pointer
+ * may be NULL. */
+int xrt_get_contacts(xrt_ctx* ctx, int32_t inst, int32_t* vf_face, float* vf_q, int32_t* rt_face,
+                     float* rt_q, float* ee16, int32_t stats[4]);
 
 /* ---------------------------------------------------------------- state I/O */
 int xrt_num_instances(xrt_ctx* ctx, int32_t* n);

@@ -1,0 +1,224 @@
+"""Oracle orchestration of a substep with contacts (test infrastructure only).
+
+Restates the substep order over the oracle's C primitives: the union of
+ClothSolverBase::SimulateSwift's body-cloth PBD collision
+(xrtailor/src/pipeline/base/ClothSolverBase.cpp:190-240) and SimulateQuality's PBD repulsion
+(:277-313):
+
+  frame start: obstacle LBVH from its x0 (tol 0); cloth LBVH from x0
+  predict
+  per iteration:
+    collision pass when (substep*iters + it) % (substeps*iters/passes) == 0:
+      x1 = lerp(x0, x, (cnt+1)/passes) of the obstacle, midstep normals,
+      RefitMidstep(bvh_tolerance), generate VF-DCD, VF-CCD, EE-DCD, cnt++
+    pair refresh when it % interval == 0: cloth LBVH Update(x, x0, ccd) and
+      Traverse(cloth, obstacle) ++ Traverse(cloth, cloth) at thickness 1e-6
+    repulsion over the pairs, applied x relaxation rate
+    stretch colors, bend colors, LRA, ground
+    VF-CCD, VF-DCD solves; EE-DCD deltas applied x relaxation factor
+  finalize; frame end: obstacle x0 = x
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import oracle as orc
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int32)
+_U = C.POINTER(C.c_uint32)
+_B = C.POINTER(C.c_uint8)
+
+
+class RMeshIn(C.Structure):
+    _fields_ = [("x", _D), ("x0", _D), ("w", _D), ("free_", _B), ("tris", _I), ("edges", _I),
+                ("fe", _I), ("ef_ptr", _I), ("ef", _I), ("face_n", _D), ("rtri", _U)]
+
+
+_bound = False
+
+
+def _lib():
+    global _bound
+    lib = orc._bind_bvh()
+    if not _bound:
+        lib.orc_face_edges.argtypes = [_I, C.c_int, _I, C.c_int, _I]
+        lib.orc_face_edges.restype = None
+        lib.orc_edge_faces.argtypes = [_I, C.c_int, C.c_int, _I, _I]
+        lib.orc_rtri.argtypes = [_I, C.c_int, C.c_int, _U]
+        lib.orc_rtri.restype = None
+        lib.orc_nearest_point.argtypes = [_D, _D, _D]
+        lib.orc_nearest_point.restype = C.c_double
+        lib.orc_ray_intersect.argtypes = [_D, _D, C.c_double, _D]
+        lib.orc_ray_intersect.restype = C.c_double
+        lib.orc_closest_seg_seg.argtypes = [_D, _D, _D, _D, _D, _D]
+        lib.orc_closest_seg_seg.restype = C.c_double
+        lib.orc_signed_vf.argtypes = [_D, _D, _D]
+        lib.orc_signed_vf.restype = C.c_double
+        lib.orc_signed_ee.argtypes = [_D, _D, _D]
+        lib.orc_signed_ee.restype = C.c_double
+        lib.orc_vf_dcd_generate.argtypes = [C.c_void_p, _I, _D, C.c_int, _I, _D]
+        lib.orc_vf_dcd_generate.restype = None
+        lib.orc_vf_ccd_generate.argtypes = [C.c_void_p, _I, _D, _D, C.c_int, _I, _D]
+        lib.orc_vf_ccd_generate.restype = None
+        lib.orc_ee_dcd_generate.argtypes = [C.c_void_p, _I, _U, _D, _I, C.c_int, _D]
+        lib.orc_ee_dcd_generate.restype = None
+        lib.orc_bvh_pred.argtypes = [C.c_void_p]
+        lib.orc_bvh_pred.restype = C.POINTER(C.c_float)
+        lib.orc_vf_solve.argtypes = [C.POINTER(C.c_float), _I, _I, _D, _D, C.c_int, C.c_double]
+        lib.orc_vf_solve.restype = None
+        lib.orc_ee_solve.argtypes = [_D, _D, _I, C.c_int, _D, _I]
+        lib.orc_ee_solve.restype = None
+        lib.orc_apply_deltas.argtypes = [_D, _D, _I, C.c_int, C.c_double]
+        lib.orc_apply_deltas.restype = None
+        lib.orc_repulsion.argtypes = [C.POINTER(RMeshIn), C.POINTER(RMeshIn), _I, _I, C.c_long,
+                                      C.c_double, _D, _I]
+        lib.orc_repulsion.restype = None
+        _bound = True
+    return lib
+
+
+def _i(a):
+    return a.ctypes.data_as(_I)
+
+
+def _d(a):
+    return a.ctypes.data_as(_D)
+
+
+def contact_topology(pos, tris, edges):
+    """(face_edges, ef_ptr, ef, rtri, rest face normals) of a mesh."""
+    lib = _lib()
+    t = np.ascontiguousarray(tris, np.int32).reshape(-1)
+    e = np.ascontiguousarray(edges, np.int32).reshape(-1)
+    nt, ne, nv = t.size // 3, e.size // 2, len(pos)
+    fe = np.zeros(3 * nt, np.int32)
+    lib.orc_face_edges(_i(t), nt, _i(e), ne, _i(fe))
+    ptr = np.zeros(ne + 1, np.int32)
+    ef = np.zeros(3 * nt, np.int32)
+    lib.orc_edge_faces(_i(fe), nt, ne, _i(ptr), _i(ef))
+    rt = np.zeros(nt, np.uint32)
+    lib.orc_rtri(_i(t), nt, nv, rt.ctypes.data_as(_U))
+    p = np.asarray(pos, np.float64)[np.asarray(tris)]
+    n = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    n = n / np.linalg.norm(n, axis=1, keepdims=True)
+    return fe, ptr, ef, rt, np.ascontiguousarray(n.reshape(-1))
+
+
+class ContactScene:
+    """Oracle twin of one solver instance with its obstacle and self contact."""
+
+    def __init__(self, cloth: orc.Cloth, rest_pos, tris, edges, params: orc.OrcParams, xp,
+                 obstacle=None):
+        self.lib = _lib()
+        self.c = cloth
+        self.p = params
+        self.xp = xp
+        self.tris = np.ascontiguousarray(tris, np.int32)
+        self.edges = np.ascontiguousarray(edges, np.int32)
+        self.nv, self.nf, self.ne = cloth.nv, len(self.tris), len(self.edges)
+        self.fe, self.efp, self.ef, self.rtri, self.fn = contact_topology(rest_pos, self.tris, self.edges)
+        self.free = np.ascontiguousarray((cloth.w > 0).astype(np.uint8))
+        self.has_obs = obstacle is not None
+        if self.has_obs:
+            opos, otris = obstacle
+            self.o_tris = np.ascontiguousarray(otris, np.int32)
+            self.o_edges = orc.build_edges(self.o_tris)
+            self.o_x0 = np.ascontiguousarray(opos, np.float32).copy()
+            self.o_x = self.o_x0.copy()
+            (self.o_fe, self.o_efp, self.o_ef, self.o_rtri, self.o_fn) = contact_topology(
+                opos, self.o_tris, self.o_edges)
+            self.o_w = np.full(len(opos), 1e-3)
+            self.o_free = np.zeros(len(opos), np.uint8)
+            self.obs_bvh = orc.Bvh()
+        self.cloth_bvh = orc.Bvh()
+        self.vf_face = np.full(self.nv, -1, np.int32)
+        self.vf_q = np.zeros(3 * self.nv)
+        self.rt_face = np.full(self.nv, -1, np.int32)
+        self.rt_q = np.zeros(3 * self.nv)
+        self.ee = np.zeros(12 * max(self.ne, 1))
+        self.delta = np.zeros(3 * self.nv)
+        self.count = np.zeros(self.nv, np.int32)
+        self.pairs = np.zeros((0, 2), np.int32)
+        self.kind = np.zeros(0, np.int32)
+        self.n_obs_pairs = 0
+        self.frame_substep = 0
+        self.pass_cnt = 0
+        self.dt = float(np.float32(xp.frame_time) / np.float32(xp.num_substeps))
+
+    def _rmesh(self, obstacle: bool) -> RMeshIn:
+        if not obstacle:
+            return RMeshIn(_d(self.c.x), _d(self.c.x0), _d(self.c.w), self.free.ctypes.data_as(_B),
+                           _i(self.tris), _i(self.edges), _i(self.fe), _i(self.efp), _i(self.ef),
+                           _d(self.fn), self.rtri.ctypes.data_as(_U))
+        self._ox = np.ascontiguousarray(self.o_x, np.float64).reshape(-1)
+        self._ox0 = np.ascontiguousarray(self.o_x0, np.float64).reshape(-1)
+        return RMeshIn(_d(self._ox), _d(self._ox0), _d(self.o_w), self.o_free.ctypes.data_as(_B),
+                       _i(self.o_tris), _i(self.o_edges), _i(self.o_fe), _i(self.o_efp),
+                       _i(self.o_ef), _d(self.o_fn), self.o_rtri.ctypes.data_as(_U))
+
+    def substep(self):
+        lib, c, p, xp, dt = self.lib, self.c, self.p, self.xp, self.dt
+        has_obs = self.has_obs and xp.enable_obstacle_contact
+        self_c = bool(xp.enable_self_contact)
+        if self.frame_substep == 0:
+            if self.has_obs:
+                self.obs_bvh.Build(self.o_x0, self.o_tris)
+            if self_c:
+                self.cloth_bvh.Build(c.x0.reshape(-1, 3).astype(np.float32), self.tris)
+            self.pass_cnt = 0
+        freq = max(1, xp.num_substeps * xp.num_iterations // xp.num_collision_passes)
+        interval = max(1, xp.pair_update_interval)
+        lib.orc_predict(C.byref(c.s), C.byref(p), dt)
+        for it in range(p.num_iterations):
+            step = self.frame_substep * p.num_iterations + it
+            if has_obs and step % freq == 0:
+                alpha = np.float32(np.float32(self.pass_cnt) + np.float32(1.0)) / np.float32(xp.num_collision_passes)
+                x1 = (np.float32(1.0) - alpha) * self.o_x0 + alpha * self.o_x
+                x1 = x1.astype(np.float32)
+                n1 = orc.vertex_normals(x1, self.o_tris)
+                self.obs_bvh.RefitMidstep(x1, n1, xp.bvh_tolerance)
+                ot = np.ascontiguousarray(self.o_tris).reshape(-1)
+                lib.orc_vf_dcd_generate(self.obs_bvh.h, _i(ot), _d(c.x), self.nv, _i(self.vf_face), _d(self.vf_q))
+                lib.orc_vf_ccd_generate(self.obs_bvh.h, _i(ot), _d(c.x0), _d(c.x), self.nv,
+                                        _i(self.rt_face), _d(self.rt_q))
+                lib.orc_ee_dcd_generate(self.obs_bvh.h, _i(ot), self.o_rtri.ctypes.data_as(_U), _d(c.x),
+                                        _i(self.edges), self.ne, _d(self.ee))
+                self.pass_cnt += 1
+            if self_c and it % interval == 0:
+                self.cloth_bvh.Update(c.x.reshape(-1, 3).astype(np.float32),
+                                      c.x0.reshape(-1, 3).astype(np.float32), ccd=True)
+                po = orc.traverse(self.cloth_bvh, self.obs_bvh, 1e-6) if self.has_obs else np.zeros((0, 2), np.int32)
+                ps = orc.traverse(self.cloth_bvh, self.cloth_bvh, 1e-6)
+                self.n_obs_pairs = len(po)
+                self.pairs = np.ascontiguousarray(np.concatenate([po, ps]), np.int32)
+                self.kind = np.ascontiguousarray(np.r_[np.zeros(len(po)), np.ones(len(ps))], np.int32)
+            if self_c:
+                cm = self._rmesh(False)
+                om = self._rmesh(True) if self.has_obs else None
+                lib.orc_repulsion(C.byref(cm), C.byref(om) if om is not None else None,
+                                  _i(self.pairs.reshape(-1)), _i(self.kind), len(self.pairs),
+                                  xp.pbd_thickness, _d(self.delta), _i(self.count))
+                lib.orc_apply_deltas(_d(c.x), _d(self.delta), _i(self.count), self.nv, xp.repulsion_relaxation)
+            for k in range(c.s_ncol):
+                lib.orc_stretch_color(C.byref(c.s), k, dt, it)
+            if p.solve_bending:
+                for k in range(c.b_ncol):
+                    lib.orc_bend_color(C.byref(c.s), C.byref(p), k, dt)
+            lib.orc_lra(C.byref(c.s), C.byref(p))
+            lib.orc_ground(C.byref(c.s), C.byref(p))
+            if has_obs:
+                pred = lib.orc_bvh_pred(self.obs_bvh.h)
+                ot = np.ascontiguousarray(self.o_tris).reshape(-1)
+                lib.orc_vf_solve(pred, _i(ot), _i(self.rt_face), _d(self.rt_q), _d(c.x), self.nv, 0.0)
+                lib.orc_vf_solve(pred, _i(ot), _i(self.vf_face), _d(self.vf_q), _d(c.x), self.nv, 1e-3)
+                lib.orc_ee_solve(_d(self.ee), _d(c.x), _i(self.edges), self.ne, _d(self.delta), _i(self.count))
+                lib.orc_apply_deltas(_d(c.x), _d(self.delta), _i(self.count), self.nv, xp.relaxation_factor)
+        lib.orc_finalize(C.byref(c.s), C.byref(p), dt)
+        self.frame_substep += 1
+        if self.frame_substep >= xp.num_substeps:
+            self.frame_substep = 0
+            if self.has_obs:
+                self.o_x0 = self.o_x.copy()

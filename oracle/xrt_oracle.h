@@ -150,6 +150,40 @@ int orc_overlap_segment(const float* box6, const float* p0, const float* p1);
 long orc_bvh_traverse(const orc_bvh* lhs, const orc_bvh* rhs, float thickness, int32_t* pairs,
                       long cap);
 
+/* ------------------------------------------- contacts (FP64, contact_oracle.c) */
+/* face edge ids (edge of nodes[i], nodes[i+1]), edge -> faces CSR (ascending), R-triangle
+ * ownership bits (xrtailor/src/physics/representative_triangle/RTriangle.cu:19-85) */
+void orc_face_edges(const int32_t* tris, int nt, const int32_t* edges, int ne, int32_t* fe);
+int orc_edge_faces(const int32_t* fe, int nt, int ne, int32_t* ptr, int32_t* ef);
+void orc_rtri(const int32_t* tris, int nt, int nv, uint32_t* out);
+/* narrow phase (xrtailor/src/math/BasicPrimitiveTests.cu) */
+double orc_nearest_point(const double* p3, const double* tri9, double* q3);
+double orc_ray_intersect(const double* o3, const double* d3, double length, const double* tri9);
+double orc_closest_seg_seg(const double* p1, const double* q1, const double* p2, const double* q2,
+                           double* s, double* t);
+double orc_signed_vf(const double* x12, double* n3, double* w4);
+double orc_signed_ee(const double* x12, double* n3, double* w4);
+/* body-cloth PBD collision (xrtailor/src/physics/pbd_collision/PBDCollision.cu) */
+void orc_vf_dcd_generate(const orc_bvh* obs, const int32_t* obs_tris, const double* x, int nv,
+                         int32_t* face_out, double* q_out);
+void orc_vf_ccd_generate(const orc_bvh* obs, const int32_t* obs_tris, const double* x0,
+                         const double* x, int nv, int32_t* face_out, double* q_out);
+void orc_ee_dcd_generate(const orc_bvh* obs, const int32_t* obs_tris, const uint32_t* obs_rtri,
+                         const double* x, const int32_t* edges, int ne, double* rec12);
+void orc_vf_solve(const float* obs_pred, const int32_t* obs_tris, const int32_t* face,
+                  const double* q, double* x, int nv, double offset);
+void orc_ee_solve(const double* rec12, const double* x, const int32_t* edges, int ne,
+                  double* delta, int32_t* count);
+void orc_apply_deltas(double* x, double* delta, int32_t* count, int nv, double relax);
+/* PBD repulsion over face pairs (xrtailor/src/physics/repulsion/PBDRepulsion.cu:264-516) */
+typedef struct orc_rmesh_in {
+  const double* x; const double* x0; const double* w; const uint8_t* free_;
+  const int32_t* tris; const int32_t* edges; const int32_t* fe;
+  const int32_t* ef_ptr; const int32_t* ef; const double* face_n; const uint32_t* rtri;
+} orc_rmesh_in;
+void orc_repulsion(const orc_rmesh_in* cloth, const orc_rmesh_in* obstacle, const int32_t* pairs,
+                   const int32_t* kind, long np, double h, double* delta, int32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,114 @@
+"""GPU body-cloth collision (VF-DCD, VF-CCD, EE-DCD) and PBD-repulsion self contact
+against the oracle twin, through the C ABI (plain path)."""
+import numpy as np
+import pytest
+
+from contact_helpers import oracle_scene, sphere_drop, two_layers
+from gpu_helpers import build, rel_err
+from xrtailor_amd import XrtError
+from xrtailor_amd.solver import make_params
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _setup(spec, xp, v0=None, seed=1):
+    s, (i,) = build([spec], xp, seed=seed)
+    if v0 is not None:
+        s.SetState(i, None, v0)
+    twin = oracle_scene(spec, xp, solver=s, inst=i)
+    return s, i, twin
+
+
+def _ee_oracle(twin):
+    return twin.ee.reshape(-1, 12)
+
+
+@pytest.mark.parametrize("self_contact", [0, 1])
+def test_sphere_drop_single_substep(self_contact):
+    print(f"case sphere drop self={self_contact}", flush=True)
+    spec, v0 = sphere_drop()
+    xp = make_params(bend_model=1, enable_obstacle_contact=1, enable_self_contact=self_contact,
+                     enable_ground=0)
+    s, i, twin = _setup(spec, xp, v0)
+    s.SubstepPlain(i)
+    twin.substep()
+    g = s.Contacts(i)
+    np.testing.assert_array_equal(g["vf_face"], twin.vf_face)
+    np.testing.assert_array_equal(g["rt_face"], twin.rt_face)
+    hit = g["vf_face"] >= 0
+    print(f"  VF-DCD {hit.sum()} VF-CCD {(g['rt_face'] >= 0).sum()} "
+          f"EE {(g['ee'][:, 0] < 1e6).sum()} pairs {g['pairs']}", flush=True)
+    assert hit.sum() > 0
+    ee_o = _ee_oracle(twin)
+    valid = ee_o[:, 0] < 1e6
+    np.testing.assert_array_equal(g["ee"][:, 0] < 1e6, valid)
+    if valid.any():
+        np.testing.assert_allclose(g["ee"][valid, 0], ee_o[valid, 0], rtol=1e-4, atol=1e-6)
+    if self_contact:
+        assert g["pairs"] == len(twin.pairs) and g["obs_pairs"] == twin.n_obs_pairs
+        assert not g["overflow"]
+    x0, _ = s.GetState(i)
+    err = rel_err(x0, twin.c.state()[0])
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err <= TOL
+
+
+def test_sphere_drop_rollout_report():
+    spec, v0 = sphere_drop()
+    xp = make_params(bend_model=1, enable_obstacle_contact=1, enable_self_contact=1, enable_ground=0)
+    s, i, twin = _setup(spec, xp, v0)
+    worst = 0.0
+    for f in range(3):
+        for _ in range(xp.num_substeps):
+            s.SubstepPlain(i)
+            twin.substep()
+        x0, _ = s.GetState(i)
+        d = np.linalg.norm(x0 - twin.c.state()[0], axis=1)
+        print(f"  frame {f}: max {d.max():.3e} m rms {np.sqrt((d ** 2).mean()):.3e} m", flush=True)
+        worst = max(worst, d.max())
+    r = np.linalg.norm(s.GetState(i)[0], axis=1)
+    assert r.min() > 0.3 - 2e-3
+    assert worst < 1e-3
+
+
+def test_two_layers_self_contact_single_substep():
+    spec = two_layers()
+    xp = make_params(bend_model=1, enable_self_contact=1, enable_ground=0, gravity=(0, 0, 0),
+                     pbd_thickness=0.002)
+    s, i, twin = _setup(spec, xp)
+    s.SubstepPlain(i)
+    twin.substep()
+    g = s.Contacts(i)
+    print(f"  pairs gpu {g['pairs']} oracle {len(twin.pairs)}", flush=True)
+    assert g["pairs"] == len(twin.pairs) and g["obs_pairs"] == 0
+    err = rel_err(s.GetState(i)[0], twin.c.state()[0])
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err <= TOL
+
+
+def test_animated_obstacle_frame():
+    """The obstacle rises 1 cm over a frame (UpdateObstacle); its substeps interpolate."""
+    spec, v0 = sphere_drop(height=0.31, v_down=-0.3)
+    xp = make_params(bend_model=1, enable_obstacle_contact=1, enable_ground=0)
+    s, i, twin = _setup(spec, xp, v0)
+    target = (spec.obstacle[0] + np.array([0.0, 0.01, 0.0])).astype(np.float32)
+    s.UpdateObstacle(i, target)
+    twin.o_x = target.copy()
+    for _ in range(xp.num_substeps):
+        s.SubstepPlain(i)
+        twin.substep()
+    d = np.linalg.norm(s.GetState(i)[0] - twin.c.state()[0], axis=1)
+    print(f"  frame max {d.max():.3e} m", flush=True)
+    assert d.max() < 1e-4
+
+
+def test_contact_error_paths():
+    spec, v0 = sphere_drop(n=8)
+    xp = make_params(bend_model=1, enable_obstacle_contact=1)
+    s, (i,) = build([spec], xp, seed=1)
+    with pytest.raises(XrtError):
+        s.SubstepBatched()                      # contacts run on the plain path
+    with pytest.raises(XrtError):
+        s.UpdateObstacle(i, np.full((len(spec.obstacle[0]), 3), np.nan, np.float32))

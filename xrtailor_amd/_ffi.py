@@ -51,6 +51,7 @@ class XrtParams(C.Structure):
         ("pair_update_interval", C.c_int32),
         ("bend_model", C.c_int32),
         ("coloring", C.c_int32),
+        ("pair_capacity_per_face", C.c_int32),
     ]
 
 
@@ -78,7 +79,10 @@ SIGNATURES = {
     "xrt_pin_vertices": [_P, _I, _IP, _I],
     "xrt_set_stretch_compliance": [_P, _I, _FP],
     "xrt_add_lra_slots": [_P, _I, _IP, _I],
+    "xrt_set_obstacle": [_P, _I, _FP, _I, _IP, _I],
     "xrt_start": [_P, C.c_uint64],
+    "xrt_update_obstacle": [_P, _I, _FP],
+    "xrt_get_contacts": [_P, _I, _IP, _FP, _IP, _FP, _FP, _IP],
     "xrt_num_instances": [_P, _IP],
     "xrt_get_counts": [_P, _I, _IP],
     "xrt_get_edges": [_P, _I, _IP, _FP, _IP],

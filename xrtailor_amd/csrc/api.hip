@@ -134,6 +134,7 @@ int SubstepPlain(xrt_ctx* c, int32_t inst) {
   const HostInstance& h = c->inst[inst];
   const InstDesc& d = h.desc;
   const StepConsts k = MakeConsts(c->params);
+  if (c->contact[inst]) return SubstepPlainContact(c, inst, k, ModeOf(k));
   hipStream_t s = c->stream;
   float4* x0 = c->x0.ptr + d.vbeg;
   float4* v = c->v.ptr + d.vbeg;
@@ -163,6 +164,10 @@ int SubstepPlain(xrt_ctx* c, int32_t inst) {
 }
 
 int SubstepBatched(xrt_ctx* c) {
+  for (const auto& cd : c->contact)
+    XRT_REQUIRE(!cd || (!(cd->obs_nf > 0 && c->params.enable_obstacle_contact) &&
+                        !c->params.enable_self_contact),
+                XRT_ERR_UNSUPPORTED, "the batched substep runs without contacts; use the plain path");
   const StepConsts k = MakeConsts(c->params);
   XRT_CHECK_HIP(LaunchBatchedSubstep(c->stream, MakeBatchArgs(c), k,
                                      static_cast<int>(c->inst.size()), c->batched_block,
@@ -226,6 +231,7 @@ void xrt_default_params(xrt_params* p) {
   p->pair_update_interval = 10;
   p->bend_model = XRT_BEND_REFERENCE;
   p->coloring = XRT_COLOR_GREEDY;
+  p->pair_capacity_per_face = 64;
 }
 
 int xrt_create(xrt_ctx** out, int device) {
@@ -553,6 +559,8 @@ int xrt_start(xrt_ctx* ctx, uint64_t seed) {
   ctx->lds_verts = lds_verts;
   ctx->batched_lds = sizeof(float4) * (static_cast<size_t>(lds_verts) + max_slots);
   if (ctx->batched_lds > 64 * 1024) XRT_CHECK_HIP(ConfigureBatchedKernel(ctx->batched_lds));
+  int rc = ContactStart(ctx);
+  if (rc != XRT_OK) return rc;
   ctx->started = true;
   return XRT_OK;
 }

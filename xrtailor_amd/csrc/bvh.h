@@ -12,7 +12,7 @@
 
 #include <cstdint>
 
-#include "context.h"
+#include "common.h"
 
 namespace xrt {
 

@@ -1,0 +1,546 @@
+// Contact kernels (built with -ffp-contract=off like the LBVH, so that the queries'
+// float predicates round as in the FP32 oracle).
+#include <hip/hip_runtime.h>
+
+#include "bvh_device.h"
+#include "contact.h"
+
+namespace xrt {
+namespace {
+constexpr int kBlock = 256;
+constexpr float kEpsC = 1e-6f;            // EPSILON
+constexpr float kScalarMax = 3.402823466e38f;  // SCALAR_MAX
+constexpr float kFix = 4294967296.0f;     // 2^32: 32.32 fixed point
+inline int Blocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+struct F3 {
+  float x, y, z;
+};
+__device__ __forceinline__ F3 L3(float4 a) { return F3{a.x, a.y, a.z}; }
+__device__ __forceinline__ F3 A(F3 a, F3 b) { return F3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 S(F3 a, F3 b) { return F3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 M(F3 a, float s) { return F3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float D(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 C(F3 a, F3 b) {
+  return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float Len(F3 a) { return sqrtf(D(a, a)); }
+__device__ __forceinline__ F3 Nz(F3 a) { return M(a, 1.0f / Len(a)); }
+__device__ __forceinline__ float Mixed(F3 a, F3 b, F3 c) { return D(a, C(b, c)); }
+__device__ __forceinline__ float Cl(float x, float a, float b) { return x < a ? a : (x > b ? b : x); }
+__device__ __forceinline__ F3 FaceNormal(F3 a, F3 b, F3 c) { return Nz(C(S(b, a), S(c, a))); }
+
+// Ericson ClosestPtPointTriangle (BasicPrimitiveTests.cu:6-100)
+__device__ float NearestPoint(F3 p, F3 a, F3 b, F3 c, F3& q) {
+  const F3 e1 = S(b, a), e2 = S(c, a), e3 = S(c, b);
+  const F3 v1 = S(p, a);
+  const float d1 = D(e1, v1), d2 = D(e2, v1);
+  if (d1 <= 0 && d2 <= 0) { q = a; return Len(S(p, q)); }
+  const F3 v2 = S(p, b);
+  const float d3 = D(e1, v2), d4 = D(e2, v2);
+  if (d3 >= 0 && d4 <= d3) { q = b; return Len(S(p, q)); }
+  const float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0 && d1 >= 0 && d3 <= 0) { q = A(a, M(e1, d1 / (d1 - d3))); return Len(S(p, q)); }
+  const F3 v3 = S(p, c);
+  const float d5 = D(e1, v3), d6 = D(e2, v3);
+  if (d6 >= 0 && d5 <= d6) { q = c; return Len(S(p, q)); }
+  const float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0 && d2 >= 0 && d6 <= 0) { q = A(a, M(e2, d2 / (d2 - d6))); return Len(S(p, q)); }
+  const float va = d3 * d6 - d5 * d4;
+  if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+    q = A(b, M(e3, (d4 - d3) / ((d4 - d3) + (d5 - d6))));
+    return Len(S(p, q));
+  }
+  const float dd = 1.0f / (va + vb + vc);
+  q = A(a, A(M(e1, vb * dd), M(e2, vc * dd)));
+  return Len(S(p, q));
+}
+
+// Moller-Trumbore, hit if 0 < s < length (BasicPrimitiveTests.cu:242-284)
+__device__ float RayIntersect(F3 o, F3 d, float length, F3 a, F3 b, F3 c) {
+  const F3 e1 = S(b, a), e2 = S(c, a);
+  const F3 n = C(d, e2);
+  const float det = D(e1, n);
+  if (fabsf(det) < kEpsC) return INFINITY;
+  const float inv = 1.0f / det;
+  const F3 t = S(o, a);
+  const float u = D(t, n) * inv;
+  if (u < 0 || u > 1) return INFINITY;
+  const F3 q = C(t, e1);
+  const float v = D(d, q) * inv;
+  if (v < 0 || v + u > 1) return INFINITY;
+  const float s = D(e2, q) * inv;
+  if (s > 0 && s < length) return s;
+  return INFINITY;
+}
+
+// Ericson ClosestPtSegmentSegment with EPSILON, t < EPS -> 0 (BasicPrimitiveTests.cu:303-361)
+__device__ float ClosestSegSeg(F3 p1, F3 q1, F3 p2, F3 q2, float& s, float& t) {
+  const F3 d1 = S(q1, p1), d2 = S(q2, p2), r = S(p1, p2);
+  const float a = D(d1, d1), e = D(d2, d2), f = D(d2, r);
+  if (a <= kEpsC && e <= kEpsC) {
+    s = t = 0;
+  } else if (a <= kEpsC) {
+    s = 0;
+    t = Cl(f / e, 0.0f, 1.0f);
+  } else {
+    const float c = D(d1, r);
+    if (e <= kEpsC) {
+      t = 0;
+      s = Cl(-c / a, 0.0f, 1.0f);
+    } else {
+      const float b = D(d1, d2), den = a * e - b * b;
+      s = den > kEpsC ? Cl((b * f - c * e) / den, 0.0f, 1.0f) : 0.0f;
+      t = (b * s + f) / e;
+      if (t < kEpsC) { t = 0; s = Cl(-c / a, 0.0f, 1.0f); }
+      else if (t > 1) { t = 1; s = Cl((b - c) / a, 0.0f, 1.0f); }
+    }
+  }
+  return Len(S(A(p1, M(d1, s)), A(p2, M(d2, t))));
+}
+
+// SignedVertexFaceDistance (BasicPrimitiveTests.cu:369-386); w zeroed when degenerate
+__device__ float SignedVF(F3 x, F3 y0, F3 y1, F3 y2, F3& n, float* w) {
+  n = C(Nz(S(y1, y0)), Nz(S(y2, y0)));
+  if (D(n, n) < 1e-6f) { w[0] = w[1] = w[2] = w[3] = 0; return INFINITY; }
+  n = Nz(n);
+  const float h = D(S(x, y0), n);
+  const float b0 = Mixed(S(y1, x), S(y2, x), n);
+  const float b1 = Mixed(S(y2, x), S(y0, x), n);
+  const float b2 = Mixed(S(y0, x), S(y1, x), n);
+  w[0] = 1.0f;
+  w[1] = -b0 / (b0 + b1 + b2);
+  w[2] = -b1 / (b0 + b1 + b2);
+  w[3] = -b2 / (b0 + b1 + b2);
+  return h;
+}
+
+// SignedEdgeEdgeDistance (BasicPrimitiveTests.cu:420-436): n is left unnormalized (the
+// reference discards glm::normalize(n))
+__device__ float SignedEE(F3 x0, F3 x1, F3 y0, F3 y1, F3& n, float* w) {
+  n = C(Nz(S(x1, x0)), Nz(S(y1, y0)));
+  if (D(n, n) < 1e-6f) { w[0] = w[1] = w[2] = w[3] = 0; return INFINITY; }
+  const float h = D(S(x0, y0), n);
+  const float a0 = Mixed(S(y1, x1), S(y0, x1), n), a1 = Mixed(S(y0, x0), S(y1, x0), n);
+  const float b0 = Mixed(S(x0, y1), S(x1, y1), n), b1 = Mixed(S(x1, y0), S(x0, y0), n);
+  w[0] = a0 / (a0 + a1);
+  w[1] = a1 / (a0 + a1);
+  w[2] = -b0 / (b0 + b1);
+  w[3] = -b1 / (b0 + b1);
+  return h;
+}
+
+__device__ __forceinline__ void AddFixed(long long* dq, int32_t* cnt, int i, F3 v) {
+  atomicAdd(reinterpret_cast<unsigned long long*>(dq + 3 * i),
+            static_cast<unsigned long long>(__float2ll_rn(v.x * kFix)));
+  atomicAdd(reinterpret_cast<unsigned long long*>(dq + 3 * i + 1),
+            static_cast<unsigned long long>(__float2ll_rn(v.y * kFix)));
+  atomicAdd(reinterpret_cast<unsigned long long*>(dq + 3 * i + 2),
+            static_cast<unsigned long long>(__float2ll_rn(v.z * kFix)));
+  atomicAdd(cnt + i, 1);
+}
+
+// ------------------------------------------------------------------ kernels
+// PhysicsMesh::Interpolate (xrtailor/src/physics/PhysicsMesh.cu:610-620)
+__global__ __launch_bounds__(kBlock) void InterpolateKernel(const float4* __restrict__ x0,
+                                                            const float4* __restrict__ x,
+                                                            float4* __restrict__ x1, int n,
+                                                            float alpha) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 a = x0[i], b = x[i];
+  const float r = 1.0f - alpha;
+  x1[i] = make_float4(r * a.x + alpha * b.x, r * a.y + alpha * b.y, r * a.z + alpha * b.z, a.w);
+}
+
+// GenerateVF_DCD_Kernel (PBDCollision.cu:17-43)
+__global__ __launch_bounds__(kBlock) void VfDcdGenerateKernel(const float4* __restrict__ x, int n,
+                                                              BvhView t, int32_t* __restrict__ face,
+                                                              float4* __restrict__ qout) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p4 = x[i];
+  const F3 p = L3(p4);
+  float best = kScalarMax;
+  int bi = -1;
+  F3 bq{0, 0, 0};
+  StacklessQuery<false>(t, p4, p4, [&](int f) {
+    const int4 tr = t.tris[f];
+    F3 q;
+    const float d = NearestPoint(p, L3(t.pred[tr.x]), L3(t.pred[tr.y]), L3(t.pred[tr.z]), q);
+    if (d < best) { best = d; bi = f; bq = q; }
+  });
+  face[i] = bi;
+  qout[i] = make_float4(bq.x, bq.y, bq.z, 0.0f);
+}
+
+// GenerateVFCCD_Kernel (PBDCollision.cu:236-270)
+__global__ __launch_bounds__(kBlock) void VfCcdGenerateKernel(const float4* __restrict__ x0,
+                                                              const float4* __restrict__ x, int n,
+                                                              BvhView t, int32_t* __restrict__ face,
+                                                              float4* __restrict__ qout) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  face[i] = -1;
+  qout[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float4 a4 = x0[i], b4 = x[i];
+  const F3 p1 = L3(a4);
+  F3 d = S(L3(b4), p1);
+  const float len = Len(d);
+  if (len < kEpsC) return;
+  d = M(d, 1.0f / len);
+  float best = kScalarMax;
+  StacklessQuery<true>(t, a4, b4, [&](int f) {
+    const int4 tr = t.tris[f];
+    const float s = RayIntersect(p1, d, len, L3(t.pred[tr.x]), L3(t.pred[tr.y]), L3(t.pred[tr.z]));
+    if (s < best) {
+      best = s;
+      face[i] = f;
+      const F3 q = A(p1, M(d, s));
+      qout[i] = make_float4(q.x, q.y, q.z, 0.0f);
+    }
+  });
+}
+
+// GenerateEE_DCD_Kernel (PBDCollision.cu:100-150)
+__global__ __launch_bounds__(kBlock) void EeDcdGenerateKernel(const float4* __restrict__ x,
+                                                              const int2* __restrict__ edges, int ne,
+                                                              BvhView t,
+                                                              const uint32_t* __restrict__ rtri,
+                                                              EeRec* __restrict__ rec) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int2 ed = edges[e];
+  const float4 pc4 = x[ed.x], qc4 = x[ed.y];
+  const F3 pc = L3(pc4), qc = L3(qc4);
+  EeRec r;
+  r.a = make_float4(kScalarMax, 0.0f, 0.0f, 0.0f);
+  r.p2 = r.q2 = r.fn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  StacklessQuery<true>(t, pc4, qc4, [&](int f) {
+    const int4 tr = t.tris[f];
+    const uint32_t info = rtri[f];
+    const F3 ov[3] = {L3(t.pred[tr.x]), L3(t.pred[tr.y]), L3(t.pred[tr.z])};
+    for (int k = 0; k < 3; ++k) {
+      if (!((info >> (3 + k)) & 1u)) continue;
+      const int j = (k + 1) % 3;
+      float s, tt;
+      const float dist = ClosestSegSeg(pc, qc, ov[k], ov[j], s, tt);
+      if (dist == 0.0f) continue;  // the reference uses the distance as the success flag
+      if (dist < r.a.x) {
+        r.a = make_float4(dist, s, tt, 0.0f);
+        r.p2 = make_float4(ov[k].x, ov[k].y, ov[k].z, 0.0f);
+        r.q2 = make_float4(ov[j].x, ov[j].y, ov[j].z, 0.0f);
+        const F3 fn = FaceNormal(ov[0], ov[1], ov[2]);
+        r.fn = make_float4(fn.x, fn.y, fn.z, 0.0f);
+      }
+    }
+  });
+  rec[e] = r;
+}
+
+// SolveVFDCD_Kernel / SolveVFCCD_Kernel (PBDCollision.cu:54-76,280-300)
+__global__ __launch_bounds__(kBlock) void VfSolveKernel(float4* __restrict__ x, int n, BvhView t,
+                                                        const int32_t* __restrict__ face,
+                                                        const float4* __restrict__ q, float offset) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int f = face[i];
+  if (f < 0) return;
+  const int4 tr = t.tris[f];
+  const F3 nrm = FaceNormal(L3(t.pred[tr.x]), L3(t.pred[tr.y]), L3(t.pred[tr.z]));
+  float4 xi = x[i];
+  const float lam = D(nrm, S(L3(xi), L3(q[i]))) - offset;
+  if (lam < 0) {
+    xi.x += -lam * nrm.x;
+    xi.y += -lam * nrm.y;
+    xi.z += -lam * nrm.z;
+    x[i] = xi;
+  }
+}
+
+// SolveEEDCD_Kernel (PBDCollision.cu:155-214)
+__global__ __launch_bounds__(kBlock) void EeSolveKernel(const float4* __restrict__ x,
+                                                        const int2* __restrict__ edges, int ne,
+                                                        const EeRec* __restrict__ rec,
+                                                        long long* __restrict__ dq,
+                                                        int32_t* __restrict__ cnt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const EeRec r = rec[e];
+  if (r.a.x > 1e6f) return;
+  const int2 ed = edges[e];
+  const float s = r.a.y, t = r.a.z;
+  const F3 fn = L3(r.fn);
+  const F3 c1 = A(M(L3(x[ed.x]), 1 - s), M(L3(x[ed.y]), s));
+  const F3 c2 = A(M(L3(r.p2), 1 - t), M(L3(r.q2), t));
+  const F3 d = S(c1, c2);
+  if (Len(d) < kEpsC) return;
+  const float lam = D(d, fn) - 1e-3f;
+  if (lam > 0) return;
+  AddFixed(dq, cnt, ed.x, M(fn, -(1 - s) * lam));
+  AddFixed(dq, cnt, ed.y, M(fn, -s * lam));
+}
+
+// ApplyDeltas_Kernel / ApplyPBDImpulse_Kernel: x += delta / count * relax, then reset
+__global__ __launch_bounds__(kBlock) void ApplyFixedKernel(float4* __restrict__ x, int n,
+                                                           long long* __restrict__ dq,
+                                                           int32_t* __restrict__ cnt, float relax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cnt[i];
+  if (c <= 0) return;
+  const double sc = static_cast<double>(relax) / (static_cast<double>(c) * 4294967296.0);
+  float4 xi = x[i];
+  xi.x += static_cast<float>(static_cast<double>(dq[3 * i]) * sc);
+  xi.y += static_cast<float>(static_cast<double>(dq[3 * i + 1]) * sc);
+  xi.z += static_cast<float>(static_cast<double>(dq[3 * i + 2]) * sc);
+  x[i] = xi;
+  dq[3 * i] = dq[3 * i + 1] = dq[3 * i + 2] = 0;
+  cnt[i] = 0;
+}
+
+// ------------------------------------------------------------- repulsion
+struct RMesh {
+  MeshTopo t;
+  MeshState s;
+  __device__ __forceinline__ F3 X(int i) const { return L3(s.x[i]); }
+  __device__ __forceinline__ F3 X0(int i) const { return L3(s.x0[i]); }
+  __device__ __forceinline__ float W(int i) const { return s.fixed_w >= 0.0f ? s.fixed_w : s.x0[i].w; }
+  __device__ __forceinline__ bool Free(int i) const { return s.fixed_w < 0.0f && s.x0[i].w > 0.0f; }
+};
+
+__device__ F3 EdgeNormal(const RMesh& m, int e) {  // Edge::ComputeNormal (Edge.cu:36-50)
+  F3 sum{0, 0, 0};
+  int cnt = 0;
+  for (int k = m.t.ef_ptr[e]; k < m.t.ef_ptr[e + 1] && cnt < 4; ++k, ++cnt) {
+    const int4 tr = m.t.tris[m.t.ef[k]];
+    sum = A(sum, FaceNormal(m.X(tr.x), m.X(tr.y), m.X(tr.z)));
+  }
+  return cnt ? M(sum, 1.0f / static_cast<float>(cnt)) : sum;
+}
+
+__device__ bool InEdge(float w, const RMesh& m0, int e0, const RMesh& m1, int e1) {
+  const int2 a = m0.t.edges[e0];
+  const F3 x = A(M(m0.X(a.x), 1 - w), M(m0.X(a.y), w));  // BasicPrimitiveTests.cu:286-301
+  bool in = true;
+  const int2 b = m1.t.edges[e1];
+  const F3 n0 = m1.X(b.x), n1 = m1.X(b.y);
+  int cnt = 0;
+  for (int k = m1.t.ef_ptr[e1]; k < m1.t.ef_ptr[e1 + 1] && cnt < 4; ++k, ++cnt) {
+    const F3 fn = L3(m1.t.face_n[m1.t.ef[k]]);
+    in &= Mixed(S(n1, n0), fn, S(x, n0)) >= 0;
+  }
+  return in;
+}
+
+__device__ __forceinline__ void Accumulate(const RMesh* const* ms, const int* ids, const F3* corr,
+                                           long long* dq, int32_t* cnt) {
+  for (int i = 0; i < 4; ++i) {  // AccumulateRepulsions (PBDRepulsion.cu:264-277)
+    if (Len(corr[i]) < kEpsC) continue;
+    if (!ms[i]->Free(ids[i])) continue;
+    AddFixed(dq, cnt, ids[i], corr[i]);
+  }
+}
+
+// EvaluateVFProximity (PBDRepulsion.cu:329-409)
+__device__ void EvalVF(const RMesh& c, int node, const RMesh& o, int face, bool self, float h,
+                       long long* dq, int32_t* cnt) {
+  const int4 tr = o.t.tris[face];
+  if (self && (node == tr.x || node == tr.y || node == tr.z)) return;
+  float w[4];
+  F3 pn;
+  const float d0 = SignedVF(c.X0(node), o.X0(tr.x), o.X0(tr.y), o.X0(tr.z), pn, w);
+  const bool face_free = o.Free(tr.x) && o.Free(tr.y) && o.Free(tr.z);
+  if (d0 < 0.0f && face_free) pn = M(pn, -1.0f);
+  F3 fn;
+  SignedVF(c.X(node), o.X(tr.x), o.X(tr.y), o.X(tr.z), fn, w);
+  if (!(fminf(-w[1], fminf(-w[2], -w[3])) >= 1e-6f)) return;
+  const F3 p1 = o.X(tr.x);
+  F3 n = FaceNormal(p1, o.X(tr.y), o.X(tr.z));
+  const float side = D(n, pn) > 0.0f ? 1.0f : -1.0f;
+  const F3 q = S(c.X(node), p1), p2 = S(o.X(tr.y), p1), p3 = S(o.X(tr.z), p1);
+  n = M(C(p2, p3), side);
+  const float c23 = Len(n);
+  n = Nz(n);
+  const float Cv = D(q, n) - h;
+  if (Cv > 0) return;
+  const F3 dcq = n;
+  const F3 dcp2 = M(A(C(p3, q), M(C(n, p3), D(n, q))), 1.0f / c23);
+  const F3 dcp3 = M(A(C(p2, q), M(C(n, p2), D(n, q))), -1.0f / c23);
+  const F3 dcp1 = S(S(M(dcq, -1.0f), dcp2), dcp3);
+  const float wq = c.W(node), w1 = o.W(tr.x), w2 = o.W(tr.y), w3 = o.W(tr.z);
+  const float den = wq * D(dcq, dcq) + w1 * D(dcp1, dcp1) + w2 * D(dcp2, dcp2) + w3 * D(dcp3, dcp3);
+  if (den < kEpsC) return;
+  const float s = Cv / den;
+  const F3 corr[4] = {M(dcq, -wq * s), M(dcp1, -w1 * s), M(dcp2, -w2 * s), M(dcp3, -w3 * s)};
+  const RMesh* ms[4] = {&c, &o, &o, &o};
+  const int ids[4] = {node, tr.x, tr.y, tr.z};
+  Accumulate(ms, ids, corr, dq, cnt);
+}
+
+// EvaluateEEProximity (PBDRepulsion.cu:411-468)
+__device__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self, float h,
+                       long long* dq, int32_t* cnt) {
+  const int2 a = c.t.edges[e0], b = o.t.edges[e1];
+  if (self && (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y)) return;
+  float w[4];
+  F3 pn;
+  const float d0 = SignedEE(c.X0(a.x), c.X0(a.y), o.X0(b.x), o.X0(b.y), pn, w);
+  const bool e1_free = o.Free(b.x) && o.Free(b.y);
+  if (!e1_free) pn = EdgeNormal(o, e1);
+  else if (d0 < 0.0f) pn = M(pn, -1.0f);
+  float bw[4];
+  F3 ncur;
+  SignedEE(c.X(a.x), c.X(a.y), o.X(b.x), o.X(b.y), ncur, bw);
+  const float mn = fminf(fminf(bw[0], bw[1]), fminf(-bw[2], -bw[3]));
+  if (!(mn >= 1e-6f && InEdge(bw[1], c, e0, o, e1) && InEdge(-bw[3], o, e1, c, e0))) return;
+  const F3 dd = A(A(M(c.X(a.x), bw[0]), M(c.X(a.y), bw[1])), A(M(o.X(b.x), bw[2]), M(o.X(b.y), bw[3])));
+  const float Cv = D(dd, pn) - h;
+  if (Cv > 0) return;
+  bw[2] *= -1.0f;
+  bw[3] *= -1.0f;
+  const F3 g[4] = {M(pn, bw[0]), M(pn, bw[1]), M(pn, -bw[2]), M(pn, -bw[3])};
+  const float wq[4] = {c.W(a.x), c.W(a.y), o.W(b.x), o.W(b.y)};
+  float den = 0.0f;
+  for (int i = 0; i < 4; ++i) den += wq[i] * bw[i] * bw[i];
+  if (den < kEpsC) return;
+  const float s = Cv / den;
+  F3 corr[4];
+  for (int i = 0; i < 4; ++i) corr[i] = M(g[i], -s * wq[i]);
+  const RMesh* ms[4] = {&c, &c, &o, &o};
+  const int ids[4] = {a.x, a.y, b.x, b.y};
+  Accumulate(ms, ids, corr, dq, cnt);
+}
+
+// EvaluateRep_Kernel (PBDRepulsion.cu:470-501) as a grid-stride loop over the device-side
+// pair count
+__global__ __launch_bounds__(kBlock) void RepulsionKernel(RMesh c, RMesh o,
+                                                          const int2* __restrict__ pairs,
+                                                          const int32_t* __restrict__ n_obs,
+                                                          const int32_t* __restrict__ n_total,
+                                                          int cap, float h,
+                                                          long long* __restrict__ dq,
+                                                          int32_t* __restrict__ cnt) {
+  const int total = min(*n_total, cap);
+  const int nobs = min(*n_obs, total);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+    const int2 pr = pairs[p];
+    const bool self = p >= nobs;
+    const RMesh& other = self ? c : o;
+    const uint32_t r0 = c.t.rtri[pr.x], r1 = other.t.rtri[pr.y];
+    const int4 f0 = c.t.tris[pr.x];
+    const int v0[3] = {f0.x, f0.y, f0.z};
+    for (int i = 0; i < 3; ++i)
+      if ((r0 >> i) & 1u) EvalVF(c, v0[i], other, pr.y, self, h, dq, cnt);
+    const int4 fe0 = c.t.face_edges[pr.x], fe1 = other.t.face_edges[pr.y];
+    const int e0[3] = {fe0.x, fe0.y, fe0.z}, e1[3] = {fe1.x, fe1.y, fe1.z};
+    for (int i = 0; i < 3; ++i) {
+      if (!((r0 >> (3 + i)) & 1u)) continue;
+      for (int j = 0; j < 3; ++j) {
+        if (!((r1 >> (3 + j)) & 1u)) continue;
+        EvalEE(c, e0[i], other, e1[j], self, h, dq, cnt);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void FindPairsAtKernel(BvhView l, BvhView r, float th,
+                                                            const uint32_t* __restrict__ cnt,
+                                                            const uint32_t* __restrict__ off,
+                                                            const int32_t* __restrict__ base,
+                                                            int2* __restrict__ pairs, int cap) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= l.nf) return;
+  const Box lb = l.box[l.nf - 1 + id];
+  const int lobj = l.nodes[l.nf - 1 + id].w;
+  const int64_t lo = static_cast<int64_t>(*base) + off[id];
+  int64_t index = lo + cnt[id] - 1;
+  int node = 0;
+  const int max_steps = 2 * r.nf;
+  for (int step = 0; node != -1 && step < max_steps; ++step) {
+    const int4 nd = r.nodes[node];
+    if (OverlapDilated(r.box[node], lb, th)) {
+      if (nd.w >= 0) {
+        if (index >= lo && index < cap) pairs[index] = make_int2(lobj, nd.w);
+        --index;
+        node = r.escape[node];
+      } else {
+        node = nd.y;
+      }
+    } else {
+      node = r.escape[node];
+    }
+  }
+}
+
+__global__ void AddTotalsKernel(const int32_t* a, const int32_t* b, int32_t* out) {
+  if (threadIdx.x == 0) *out = *a + *b;
+}
+}  // namespace
+
+hipError_t LaunchInterpolate(hipStream_t s, const float4* x0, const float4* x, float4* x1, int n,
+                             float alpha) {
+  if (n <= 0) return hipSuccess;
+  InterpolateKernel<<<Blocks(n), kBlock, 0, s>>>(x0, x, x1, n, alpha);
+  return hipGetLastError();
+}
+hipError_t LaunchVfDcdGenerate(hipStream_t s, const float4* x, int n, const Lbvh& obs,
+                               int32_t* face, float4* q) {
+  if (n <= 0 || !obs.active) return hipSuccess;
+  VfDcdGenerateKernel<<<Blocks(n), kBlock, 0, s>>>(x, n, obs.view(), face, q);
+  return hipGetLastError();
+}
+hipError_t LaunchVfCcdGenerate(hipStream_t s, const float4* x0, const float4* x, int n,
+                               const Lbvh& obs, int32_t* face, float4* q) {
+  if (n <= 0 || !obs.active) return hipSuccess;
+  VfCcdGenerateKernel<<<Blocks(n), kBlock, 0, s>>>(x0, x, n, obs.view(), face, q);
+  return hipGetLastError();
+}
+hipError_t LaunchEeDcdGenerate(hipStream_t s, const float4* x, const int2* edges, int ne,
+                               const Lbvh& obs, const uint32_t* obs_rtri, EeRec* rec) {
+  if (ne <= 0 || !obs.active) return hipSuccess;
+  EeDcdGenerateKernel<<<Blocks(ne), kBlock, 0, s>>>(x, edges, ne, obs.view(), obs_rtri, rec);
+  return hipGetLastError();
+}
+hipError_t LaunchVfSolve(hipStream_t s, float4* x, int n, const Lbvh& obs, const int32_t* face,
+                         const float4* q, float offset) {
+  if (n <= 0 || !obs.active) return hipSuccess;
+  VfSolveKernel<<<Blocks(n), kBlock, 0, s>>>(x, n, obs.view(), face, q, offset);
+  return hipGetLastError();
+}
+hipError_t LaunchEeSolve(hipStream_t s, const float4* x, const int2* edges, int ne,
+                         const EeRec* rec, long long* dq, int32_t* cnt) {
+  if (ne <= 0) return hipSuccess;
+  EeSolveKernel<<<Blocks(ne), kBlock, 0, s>>>(x, edges, ne, rec, dq, cnt);
+  return hipGetLastError();
+}
+hipError_t LaunchApplyFixedDeltas(hipStream_t s, float4* x, int n, long long* dq, int32_t* cnt,
+                                  float relax) {
+  if (n <= 0) return hipSuccess;
+  ApplyFixedKernel<<<Blocks(n), kBlock, 0, s>>>(x, n, dq, cnt, relax);
+  return hipGetLastError();
+}
+hipError_t LaunchRepulsion(hipStream_t s, const MeshTopo& cloth, const MeshState& cs,
+                           const MeshTopo& obs, const MeshState& os, const int2* pairs,
+                           const int32_t* n_obs_pairs, const int32_t* n_total, int cap, float h,
+                           long long* dq, int32_t* cnt) {
+  if (cap <= 0) return hipSuccess;
+  RMesh c{cloth, cs}, o{obs, os};
+  const int grid = std::min(Blocks(cap), 4096);
+  RepulsionKernel<<<grid, kBlock, 0, s>>>(c, o, pairs, n_obs_pairs, n_total, cap, h, dq, cnt);
+  return hipGetLastError();
+}
+hipError_t LbvhFindPairsAt(Lbvh& lhs, const Lbvh& rhs, float thickness, int2* pairs,
+                           const int32_t* base, int cap, hipStream_t s) {
+  if (!lhs.active || !rhs.active) return hipSuccess;
+  FindPairsAtKernel<<<Blocks(lhs.nf), kBlock, 0, s>>>(
+      lhs.view(), rhs.view(), thickness, reinterpret_cast<const uint32_t*>(lhs.counts.ptr),
+      reinterpret_cast<const uint32_t*>(lhs.offsets.ptr), base, pairs, cap);
+  return hipGetLastError();
+}
+hipError_t LaunchAddTotals(hipStream_t s, const int32_t* a, const int32_t* b, int32_t* out) {
+  AddTotalsKernel<<<1, 64, 0, s>>>(a, b, out);
+  return hipGetLastError();
+}
+
+}  // namespace xrt

@@ -1,0 +1,321 @@
+// Contact set-up and the contact-aware plain substep.
+//
+// Substep order (a union of ClothSolverBase::SimulateSwift's body-cloth collision,
+// xrtailor/src/pipeline/base/ClothSolverBase.cpp:190-240, and SimulateQuality's PBD
+// repulsion, :300-313):
+//   predict
+//   per iteration:
+//     collision pass (step % frequency == 0): obstacle x1 = lerp(x0, x, (cnt+1)/passes),
+//       midstep normals, RefitMidstep(bvh_tolerance); generate VF-DCD, VF-CCD, EE-DCD
+//     pair refresh (iteration % pair_update_interval == 0): cloth LBVH Update(ccd),
+//       Traverse(cloth, obstacle) ++ Traverse(cloth, cloth) at thickness 1e-6
+//     repulsion over the pairs -> impulses x relaxation rate
+//     stretch colors, bend colors, LRA, ground
+//     solve VF-CCD, VF-DCD (direct), EE-DCD (deltas) -> ApplyDeltas x relaxation factor
+//   finalize
+// Frame start: obstacle LBVH rebuilt from its x0 (tol 0), cloth LBVH rebuilt from x0.
+// Frame end: obstacle x0 = x (PhysicsMesh::PredictedToPositions).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "contact.h"
+#include "context.h"
+#include "dynamics.h"
+
+namespace xrt {
+
+namespace {
+constexpr float kPairThickness = 1e-6f;  // UpdatePairs thickness (ClothSolverBase.cpp:305-308)
+
+int UploadTopo(const ContactTopo& t, const std::vector<int32_t>& tris,
+               const std::vector<int32_t>& edges, DevBuf<int4>& dt, DevBuf<int4>& dfe,
+               DevBuf<int2>& de, DevBuf<int32_t>& dptr, DevBuf<int32_t>& def,
+               DevBuf<float4>& dn, DevBuf<uint32_t>& dr, hipStream_t s) {
+  const int nt = static_cast<int>(tris.size() / 3), ne = static_cast<int>(edges.size() / 2);
+  std::vector<int4> t4(std::max(nt, 1)), fe4(std::max(nt, 1));
+  std::vector<float4> n4(std::max(nt, 1));
+  for (int f = 0; f < nt; ++f) {
+    t4[f] = make_int4(tris[3 * f], tris[3 * f + 1], tris[3 * f + 2], 0);
+    fe4[f] = make_int4(t.face_edges[3 * f], t.face_edges[3 * f + 1], t.face_edges[3 * f + 2], 0);
+    n4[f] = make_float4(t.face_n[3 * f], t.face_n[3 * f + 1], t.face_n[3 * f + 2], 0.0f);
+  }
+  std::vector<int2> e2(std::max(ne, 1));
+  for (int e = 0; e < ne; ++e) e2[e] = make_int2(edges[2 * e], edges[2 * e + 1]);
+  XRT_CHECK_HIP(dt.Upload(t4, s));
+  XRT_CHECK_HIP(dfe.Upload(fe4, s));
+  XRT_CHECK_HIP(dn.Upload(n4, s));
+  XRT_CHECK_HIP(de.Upload(e2, s));
+  XRT_CHECK_HIP(dptr.Upload(t.ef_ptr, s));
+  XRT_CHECK_HIP(def.Upload(t.ef, s));
+  XRT_CHECK_HIP(dr.Upload(t.rtri, s));
+  return XRT_OK;
+}
+
+MeshTopo ClothTopo(const ContactData& c, const HostInstance& h) {
+  MeshTopo t;
+  t.nv = h.nv;
+  t.nf = h.nt;
+  t.ne = h.ne();
+  t.tris = c.c_tris.ptr;
+  t.edges = c.c_edges.ptr;
+  t.face_edges = c.c_fe.ptr;
+  t.ef_ptr = c.c_ef_ptr.ptr;
+  t.ef = c.c_ef.ptr;
+  t.face_n = c.c_face_n.ptr;
+  t.rtri = c.c_rtri.ptr;
+  return t;
+}
+
+MeshTopo ObsTopo(const ContactData& c) {
+  MeshTopo t;
+  t.nv = c.obs_nv;
+  t.nf = c.obs_nf;
+  t.ne = c.obs_ne;
+  t.tris = c.o_tris.ptr;
+  t.edges = c.o_edges.ptr;
+  t.face_edges = c.o_fe.ptr;
+  t.ef_ptr = c.o_ef_ptr.ptr;
+  t.ef = c.o_ef.ptr;
+  t.face_n = c.o_face_n.ptr;
+  t.rtri = c.o_rtri.ptr;
+  return t;
+}
+
+std::vector<float4> ToF4(const std::vector<float>& p) {
+  std::vector<float4> out(std::max<size_t>(p.size() / 3, 1));
+  for (size_t i = 0; i < p.size() / 3; ++i) out[i] = make_float4(p[3 * i], p[3 * i + 1], p[3 * i + 2], 0.0f);
+  return out;
+}
+}  // namespace
+
+bool ContactNeeded(const xrt_ctx* ctx, const HostInstance& h) {
+  return h.obs_nt() > 0 || ctx->params.enable_self_contact;
+}
+
+int ContactStart(xrt_ctx* ctx) {
+  hipStream_t s = ctx->stream;
+  ctx->contact.clear();
+  ctx->contact.resize(ctx->inst.size());
+  for (size_t ii = 0; ii < ctx->inst.size(); ++ii) {
+    HostInstance& h = ctx->inst[ii];
+    if (!ContactNeeded(ctx, h)) continue;
+    auto c = std::make_unique<ContactData>();
+    h.cloth_topo = BuildContactTopo(h.rest_pos.data(), h.nv, h.tris.data(), h.nt, h.edges);
+    int rc = UploadTopo(h.cloth_topo, h.tris, h.edges, c->c_tris, c->c_fe, c->c_edges, c->c_ef_ptr,
+                        c->c_ef, c->c_face_n, c->c_rtri, s);
+    if (rc != XRT_OK) return rc;
+    XRT_CHECK_HIP(c->cloth_bvh.Setup(h.tris.data(), h.nt, h.nv, s));
+    if (h.obs_nt() > 0) {
+      c->obs_nv = h.obs_nv();
+      c->obs_nf = h.obs_nt();
+      c->obs_ne = static_cast<int>(h.obs_edges.size() / 2);
+      h.obs_topo = BuildContactTopo(h.obs_pos.data(), c->obs_nv, h.obs_tris.data(), c->obs_nf,
+                                    h.obs_edges);
+      rc = UploadTopo(h.obs_topo, h.obs_tris, h.obs_edges, c->o_tris, c->o_fe, c->o_edges,
+                      c->o_ef_ptr, c->o_ef, c->o_face_n, c->o_rtri, s);
+      if (rc != XRT_OK) return rc;
+      const std::vector<float4> p = ToF4(h.obs_pos);
+      XRT_CHECK_HIP(c->o_x0.Upload(p, s));
+      XRT_CHECK_HIP(c->o_x.Upload(p, s));
+      XRT_CHECK_HIP(c->o_x1.Upload(p, s));
+      XRT_CHECK_HIP(c->o_n1.Alloc(p.size()));
+      // vertex -> corner CSR (ascending face order) for the midstep normals
+      std::vector<int32_t> vptr(c->obs_nv + 1, 0), corner(3 * c->obs_nf);
+      for (int i = 0; i < 3 * c->obs_nf; ++i) vptr[h.obs_tris[i] + 1]++;
+      for (int v = 0; v < c->obs_nv; ++v) vptr[v + 1] += vptr[v];
+      std::vector<int32_t> fill(vptr.begin(), vptr.end() - 1);
+      for (int i = 0; i < 3 * c->obs_nf; ++i) corner[fill[h.obs_tris[i]]++] = i;
+      XRT_CHECK_HIP(c->o_vptr.Upload(vptr, s));
+      XRT_CHECK_HIP(c->o_corner.Upload(corner, s));
+      XRT_CHECK_HIP(c->obs_bvh.Setup(h.obs_tris.data(), c->obs_nf, c->obs_nv, s));
+    }
+    XRT_CHECK_HIP(c->vf_face.Alloc(h.nv));
+    XRT_CHECK_HIP(c->rt_face.Alloc(h.nv));
+    XRT_CHECK_HIP(c->vf_q.Alloc(h.nv));
+    XRT_CHECK_HIP(c->rt_q.Alloc(h.nv));
+    XRT_CHECK_HIP(c->ee.Alloc(sizeof(EeRec) * std::max(h.ne(), 1)));
+    XRT_CHECK_HIP(c->dq.Alloc(3 * static_cast<size_t>(h.nv)));
+    XRT_CHECK_HIP(c->cnt.Alloc(h.nv));
+    XRT_CHECK_HIP(hipMemsetAsync(c->dq.ptr, 0, sizeof(long long) * 3 * h.nv, s));
+    XRT_CHECK_HIP(hipMemsetAsync(c->cnt.ptr, 0, sizeof(int32_t) * h.nv, s));
+    XRT_CHECK_HIP(hipMemsetAsync(c->vf_face.ptr, 0xff, sizeof(int32_t) * h.nv, s));
+    XRT_CHECK_HIP(hipMemsetAsync(c->rt_face.ptr, 0xff, sizeof(int32_t) * h.nv, s));
+    const int per_face = std::max(1, ctx->params.pair_capacity_per_face);
+    c->pair_cap = std::max(1, per_face * h.nt);
+    XRT_CHECK_HIP(c->pairs.Alloc(c->pair_cap));
+    XRT_CHECK_HIP(c->scal.Alloc(4));
+    XRT_CHECK_HIP(hipMemsetAsync(c->scal.ptr, 0, sizeof(int32_t) * 4, s));
+    ctx->contact[ii] = std::move(c);
+  }
+  XRT_CHECK_HIP(hipStreamSynchronize(s));
+  return XRT_OK;
+}
+
+// One plain substep of an instance with contacts (see the order at the top of the file).
+int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendMode& bm) {
+  HostInstance& h = ctx->inst[inst];
+  ContactData& c = *ctx->contact[inst];
+  const xrt_params& p = ctx->params;
+  const InstDesc& d = h.desc;
+  hipStream_t s = ctx->stream;
+  float4* x0 = ctx->x0.ptr + d.vbeg;
+  float4* v = ctx->v.ptr + d.vbeg;
+  float4* x = ctx->x.ptr + d.vbeg;
+  const bool has_obs = c.obs_nf > 0 && p.enable_obstacle_contact;
+  const bool self = p.enable_self_contact != 0;
+  EeRec* ee = reinterpret_cast<EeRec*>(c.ee.ptr);
+
+  if (c.frame_substep == 0) {  // frame start
+    if (c.obs_nf > 0)
+      XRT_CHECK_HIP(LbvhBuild(c.obs_bvh, c.o_x0.ptr, nullptr, nullptr, 0.0f, 0, 0.0f, s));
+    if (self) XRT_CHECK_HIP(LbvhBuild(c.cloth_bvh, x0, nullptr, nullptr, 0.0f, 0, 0.0f, s));
+    c.pass_cnt = 0;
+  }
+  const int frequency = std::max(1, p.num_substeps * p.num_iterations / p.num_collision_passes);
+  const int interval = std::max(1, p.pair_update_interval);
+
+  XRT_CHECK_HIP(LaunchPredict(s, x0, v, x, d.nv, k));
+  for (int it = 0; it < k.iterations; ++it) {
+    const int step = c.frame_substep * p.num_iterations + it;
+    if (has_obs && step % frequency == 0) {
+      const float alpha = (static_cast<float>(c.pass_cnt) + 1.0f) / static_cast<float>(p.num_collision_passes);
+      XRT_CHECK_HIP(LaunchInterpolate(s, c.o_x0.ptr, c.o_x.ptr, c.o_x1.ptr, c.obs_nv, alpha));
+      XRT_CHECK_HIP(VertexNormals(c.o_x1.ptr, c.o_tris.ptr, c.o_vptr.ptr, c.o_corner.ptr, c.obs_nv,
+                                  c.o_n1.ptr, s));
+      XRT_CHECK_HIP(LbvhRefitMidstep(c.obs_bvh, c.o_x1.ptr, c.o_n1.ptr, p.bvh_tolerance, s));
+      XRT_CHECK_HIP(LaunchVfDcdGenerate(s, x, d.nv, c.obs_bvh, c.vf_face.ptr, c.vf_q.ptr));
+      XRT_CHECK_HIP(LaunchVfCcdGenerate(s, x0, x, d.nv, c.obs_bvh, c.rt_face.ptr, c.rt_q.ptr));
+      XRT_CHECK_HIP(LaunchEeDcdGenerate(s, x, c.c_edges.ptr, h.ne(), c.obs_bvh, c.o_rtri.ptr, ee));
+      c.pass_cnt++;
+    }
+    if (self && it % interval == 0) {
+      XRT_CHECK_HIP(LbvhUpdate(c.cloth_bvh, x, x0, 1, s));
+      // cloth vs obstacle pairs at [0, n_obs), cloth vs cloth after them
+      if (c.obs_nf > 0) {
+        XRT_CHECK_HIP(LbvhCountPairs(c.cloth_bvh, c.obs_bvh, kPairThickness, s));
+        XRT_CHECK_HIP(hipMemcpyAsync(c.scal.ptr + 1, c.cloth_bvh.total.ptr, 4, hipMemcpyDeviceToDevice, s));
+        XRT_CHECK_HIP(LbvhFindPairsAt(c.cloth_bvh, c.obs_bvh, kPairThickness, c.pairs.ptr, c.scal.ptr,
+                                      c.pair_cap, s));
+      } else {
+        XRT_CHECK_HIP(hipMemsetAsync(c.scal.ptr + 1, 0, 4, s));
+      }
+      XRT_CHECK_HIP(LbvhCountPairs(c.cloth_bvh, c.cloth_bvh, kPairThickness, s));
+      XRT_CHECK_HIP(LbvhFindPairsAt(c.cloth_bvh, c.cloth_bvh, kPairThickness, c.pairs.ptr,
+                                    c.scal.ptr + 1, c.pair_cap, s));
+      XRT_CHECK_HIP(LaunchAddTotals(s, c.scal.ptr + 1, c.cloth_bvh.total.ptr, c.scal.ptr + 2));
+    }
+    if (self) {
+      MeshState cs{x, x0, -1.0f};
+      MeshState os{c.o_x.ptr, c.o_x0.ptr, 1e-3f};  // obstacle nodes: inv mass 1e-3, not free
+      XRT_CHECK_HIP(LaunchRepulsion(s, ClothTopo(c, h), cs, ObsTopo(c), os, c.pairs.ptr, c.scal.ptr + 1,
+                                    c.scal.ptr + 2, c.pair_cap, p.pbd_thickness, c.dq.ptr, c.cnt.ptr));
+      XRT_CHECK_HIP(LaunchApplyFixedDeltas(s, x, d.nv, c.dq.ptr, c.cnt.ptr, p.repulsion_relaxation));
+    }
+    for (int col = 0; col < h.s_ncol; ++col) {
+      const int b = h.sbeg + h.s_coloff[col];
+      XRT_CHECK_HIP(LaunchStretch(s, ctx->sc.ptr + b, ctx->slam.ptr + b, x,
+                                  h.s_coloff[col + 1] - h.s_coloff[col], k.inv_dt2, it));
+    }
+    if (k.solve_bending)
+      for (int col = 0; col < h.b_ncol; ++col) {
+        const int b = h.bbeg + h.b_coloff[col];
+        XRT_CHECK_HIP(LaunchBend(s, ctx->bidx.ptr + b, ctx->bpar.ptr + b, x,
+                                 h.b_coloff[col + 1] - h.b_coloff[col], k.inv_dt2, bm));
+      }
+    if (d.nslots > 0)
+      XRT_CHECK_HIP(LaunchLra(s, x, ctx->slot_vert.ptr + d.slot_beg, ctx->slot_pos.ptr, d.nslots,
+                              ctx->lra_ptr.ptr + d.lra_vbeg, ctx->lra_slot.ptr, ctx->lra_rest.ptr,
+                              d.nv, k.stretchiness));
+    if (k.enable_ground) XRT_CHECK_HIP(LaunchGround(s, x, x0, d.nv, k.ground_y, k.friction));
+    if (has_obs) {
+      XRT_CHECK_HIP(LaunchVfSolve(s, x, d.nv, c.obs_bvh, c.rt_face.ptr, c.rt_q.ptr, 0.0f));
+      XRT_CHECK_HIP(LaunchVfSolve(s, x, d.nv, c.obs_bvh, c.vf_face.ptr, c.vf_q.ptr, 1e-3f));
+      XRT_CHECK_HIP(LaunchEeSolve(s, x, c.c_edges.ptr, h.ne(), ee, c.dq.ptr, c.cnt.ptr));
+      XRT_CHECK_HIP(LaunchApplyFixedDeltas(s, x, d.nv, c.dq.ptr, c.cnt.ptr, p.relaxation_factor));
+    }
+  }
+  XRT_CHECK_HIP(LaunchFinalize(s, x0, v, x, d.nv, k));
+  if (++c.frame_substep >= p.num_substeps) {  // frame end
+    c.frame_substep = 0;
+    if (c.obs_nf > 0)
+      XRT_CHECK_HIP(hipMemcpyAsync(c.o_x0.ptr, c.o_x.ptr, sizeof(float4) * c.obs_nv,
+                                   hipMemcpyDeviceToDevice, s));
+  }
+  return XRT_OK;
+}
+
+}  // namespace xrt
+
+using namespace xrt;
+
+extern "C" {
+
+int xrt_set_obstacle(xrt_ctx* ctx, int32_t inst, const float* pos, int32_t nv, const int32_t* tris,
+                     int32_t nt) {
+  XRT_REQUIRE(ctx && inst >= 0 && inst < static_cast<int32_t>(ctx->inst.size()), XRT_ERR_INVALID_ARG,
+              "bad context or instance");
+  XRT_REQUIRE(!ctx->started, XRT_ERR_STATE, "set the obstacle before xrt_start");
+  XRT_REQUIRE(pos && tris && nv > 0 && nt > 0, XRT_ERR_INVALID_ARG, "empty or null obstacle mesh");
+  for (int i = 0; i < 3 * nt; ++i)
+    XRT_REQUIRE(tris[i] >= 0 && tris[i] < nv, XRT_ERR_INVALID_ARG, "obstacle face index out of range");
+  for (int i = 0; i < 3 * nv; ++i)
+    XRT_REQUIRE(std::isfinite(pos[i]), XRT_ERR_NOT_FINITE, "non-finite obstacle position");
+  HostInstance& h = ctx->inst[inst];
+  h.obs_pos.assign(pos, pos + 3 * nv);
+  h.obs_tris.assign(tris, tris + 3 * nt);
+  h.obs_edges = BuildEdges(tris, nt);
+  return XRT_OK;
+}
+
+int xrt_update_obstacle(xrt_ctx* ctx, int32_t inst, const float* pos) {
+  XRT_REQUIRE(ctx && inst >= 0 && inst < static_cast<int32_t>(ctx->inst.size()) && pos,
+              XRT_ERR_INVALID_ARG, "bad argument");
+  XRT_REQUIRE(ctx->started, XRT_ERR_STATE, "xrt_start has not been called");
+  ContactData* c = ctx->contact[inst].get();
+  XRT_REQUIRE(c && c->obs_nf > 0, XRT_ERR_STATE, "instance has no obstacle");
+  std::vector<float> p(pos, pos + 3 * c->obs_nv);
+  for (float f : p) XRT_REQUIRE(std::isfinite(f), XRT_ERR_NOT_FINITE, "non-finite obstacle position");
+  const std::vector<float4> q = ToF4(p);
+  XRT_CHECK_HIP(c->o_x.Upload(q.data(), q.size(), ctx->stream));
+  XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return XRT_OK;
+}
+
+int xrt_get_contacts(xrt_ctx* ctx, int32_t inst, int32_t* vf_face, float* vf_q, int32_t* rt_face,
+                     float* rt_q, float* ee16, int32_t stats[4]) {
+  XRT_REQUIRE(ctx && inst >= 0 && inst < static_cast<int32_t>(ctx->inst.size()), XRT_ERR_INVALID_ARG,
+              "bad context or instance");
+  XRT_REQUIRE(ctx->started, XRT_ERR_STATE, "xrt_start has not been called");
+  ContactData* c = ctx->contact[inst].get();
+  XRT_REQUIRE(c, XRT_ERR_STATE, "instance has no contact state");
+  const HostInstance& h = ctx->inst[inst];
+  hipStream_t s = ctx->stream;
+  std::vector<float4> q(h.nv);
+  if (vf_face) XRT_CHECK_HIP(hipMemcpyAsync(vf_face, c->vf_face.ptr, 4 * h.nv, hipMemcpyDeviceToHost, s));
+  if (rt_face) XRT_CHECK_HIP(hipMemcpyAsync(rt_face, c->rt_face.ptr, 4 * h.nv, hipMemcpyDeviceToHost, s));
+  if (ee16) XRT_CHECK_HIP(hipMemcpyAsync(ee16, c->ee.ptr, sizeof(EeRec) * h.ne(), hipMemcpyDeviceToHost, s));
+  if (stats) {
+    int32_t sc[4];
+    XRT_CHECK_HIP(hipMemcpyAsync(sc, c->scal.ptr, 16, hipMemcpyDeviceToHost, s));
+    XRT_CHECK_HIP(hipStreamSynchronize(s));
+    stats[0] = sc[1];
+    stats[1] = sc[2];
+    stats[2] = c->pair_cap;
+    stats[3] = sc[2] > c->pair_cap ? 1 : 0;
+  }
+  for (int which = 0; which < 2; ++which) {
+    float* out = which == 0 ? vf_q : rt_q;
+    if (!out) continue;
+    XRT_CHECK_HIP(hipMemcpyAsync(q.data(), which == 0 ? c->vf_q.ptr : c->rt_q.ptr, sizeof(float4) * h.nv,
+                                 hipMemcpyDeviceToHost, s));
+    XRT_CHECK_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < h.nv; ++i) {
+      out[3 * i] = q[i].x; out[3 * i + 1] = q[i].y; out[3 * i + 2] = q[i].z;
+    }
+  }
+  XRT_CHECK_HIP(hipStreamSynchronize(s));
+  return XRT_OK;
+}
+
+}  // extern "C"

@@ -3,62 +3,17 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "../../include/xrtailor_amd.h"
+#include "bvh.h"
+#include "common.h"
 #include "device.h"
+#include "topology.h"
 
 namespace xrt {
-
-void SetError(const std::string& msg);
-
-#define XRT_CHECK_HIP(expr)                                                       \
-  do {                                                                            \
-    hipError_t _e = (expr);                                                       \
-    if (_e != hipSuccess) {                                                       \
-      ::xrt::SetError(std::string(#expr) + ": " + hipGetErrorString(_e));        \
-      return XRT_ERR_HIP;                                                         \
-    }                                                                             \
-  } while (0)
-
-#define XRT_REQUIRE(cond, code, msg) \
-  do {                               \
-    if (!(cond)) {                   \
-      ::xrt::SetError(msg);          \
-      return (code);                 \
-    }                                \
-  } while (0)
-
-template <typename T>
-struct DevBuf {
-  T* ptr = nullptr;
-  size_t n = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() { Free(); }
-  void Free() {
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    n = 0;
-  }
-  hipError_t Alloc(size_t count) {
-    Free();
-    n = count;
-    if (count == 0) return hipSuccess;
-    return hipMalloc(reinterpret_cast<void**>(&ptr), count * sizeof(T));
-  }
-  hipError_t Upload(const T* host, size_t count, hipStream_t s) {
-    if (count == 0) return hipSuccess;
-    return hipMemcpyAsync(ptr, host, count * sizeof(T), hipMemcpyHostToDevice, s);
-  }
-  hipError_t Upload(const std::vector<T>& host, hipStream_t s) {
-    hipError_t e = Alloc(host.size());
-    if (e != hipSuccess) return e;
-    return Upload(host.data(), host.size(), s);
-  }
-};
 
 struct HostInstance {
   int nv = 0, nt = 0;
@@ -83,8 +38,48 @@ struct HostInstance {
   int sbeg = 0, bbeg = 0, kbeg = 0;
   int ne() const { return static_cast<int>(edges.size() / 2); }
   int nb() const { return static_cast<int>(bends.size() / 4); }
+  // obstacle (optional): rest/initial positions, faces, edges
+  std::vector<float> obs_pos;
+  std::vector<int32_t> obs_tris, obs_edges;
+  int obs_nv() const { return static_cast<int>(obs_pos.size() / 3); }
+  int obs_nt() const { return static_cast<int>(obs_tris.size() / 3); }
+  ContactTopo cloth_topo, obs_topo;  // built by xrt_start
 };
 
+}  // namespace xrt
+
+// Device-side contact state of one instance (plain path).
+namespace xrt {
+struct ContactData {
+  // cloth topology
+  DevBuf<int4> c_tris, c_fe;
+  DevBuf<int2> c_edges;
+  DevBuf<int32_t> c_ef_ptr, c_ef;
+  DevBuf<float4> c_face_n;
+  DevBuf<uint32_t> c_rtri;
+  // obstacle topology and state
+  int obs_nv = 0, obs_nf = 0, obs_ne = 0;
+  DevBuf<int4> o_tris, o_fe;
+  DevBuf<int2> o_edges;
+  DevBuf<int32_t> o_ef_ptr, o_ef;
+  DevBuf<float4> o_face_n;
+  DevBuf<uint32_t> o_rtri;
+  DevBuf<float4> o_x0, o_x, o_x1, o_n1;
+  DevBuf<int32_t> o_vptr, o_corner;  // vertex -> corner CSR for normals
+  // trees
+  Lbvh obs_bvh, cloth_bvh;
+  // contact records and Jacobi accumulators
+  DevBuf<int32_t> vf_face, rt_face;
+  DevBuf<float4> vf_q, rt_q;
+  DevBuf<char> ee;  // EeRec[E]
+  DevBuf<long long> dq;
+  DevBuf<int32_t> cnt;
+  // face pairs of the repulsion (cloth face, other face)
+  int pair_cap = 0;
+  DevBuf<int2> pairs;
+  DevBuf<int32_t> scal;  // [0] zero, [1] obstacle-pair count, [2] total pair count
+  int frame_substep = 0, pass_cnt = 0;
+};
 }  // namespace xrt
 
 struct xrt_ctx {
@@ -115,9 +110,18 @@ struct xrt_ctx {
   size_t batched_lds = 0;
   int lds_verts = 0;
   int max_slots = 0;
+  // per-instance contact state (allocated by xrt_start when contacts can occur)
+  std::vector<std::unique_ptr<xrt::ContactData>> contact;
   // graph of one batched frame
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   // timing events
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
+
+namespace xrt {
+// contact_api.hip
+bool ContactNeeded(const xrt_ctx* ctx, const HostInstance& h);
+int ContactStart(xrt_ctx* ctx);
+int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendMode& bm);
+}  // namespace xrt

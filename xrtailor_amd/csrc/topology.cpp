@@ -233,6 +233,66 @@ std::vector<int32_t> ColorConstraints(const int32_t* cons, int arity, int n, int
   return color;
 }
 
+ContactTopo BuildContactTopo(const float* pos, int nv, const int32_t* tris, int nt,
+                             const std::vector<int32_t>& edges) {
+  ContactTopo t;
+  const int ne = static_cast<int>(edges.size() / 2);
+  std::unordered_map<uint64_t, int> index;
+  index.reserve(static_cast<size_t>(ne) * 2);
+  for (int e = 0; e < ne; ++e) index.emplace(EdgeKey(edges[2 * e], edges[2 * e + 1]), e);
+  t.face_edges.resize(static_cast<size_t>(3) * nt);
+  for (int f = 0; f < nt; ++f)
+    for (int l = 0; l < 3; ++l)
+      t.face_edges[3 * f + l] = index[EdgeKey(tris[3 * f + l], tris[3 * f + (l + 1) % 3])];
+  t.ef_ptr.assign(ne + 1, 0);
+  for (int i = 0; i < 3 * nt; ++i) t.ef_ptr[t.face_edges[i] + 1]++;
+  for (int e = 0; e < ne; ++e) t.ef_ptr[e + 1] += t.ef_ptr[e];
+  t.ef.resize(static_cast<size_t>(3) * nt);
+  {
+    std::vector<int32_t> fill(t.ef_ptr.begin(), t.ef_ptr.end() - 1);
+    for (int f = 0; f < nt; ++f)
+      for (int l = 0; l < 3; ++l) t.ef[fill[t.face_edges[3 * f + l]]++] = f;
+  }
+  // vertex -> faces, ascending
+  std::vector<int32_t> vptr(nv + 1, 0), vf(static_cast<size_t>(3) * nt);
+  for (int i = 0; i < 3 * nt; ++i) vptr[tris[i] + 1]++;
+  for (int v = 0; v < nv; ++v) vptr[v + 1] += vptr[v];
+  {
+    std::vector<int32_t> fill(vptr.begin(), vptr.end() - 1);
+    for (int f = 0; f < nt; ++f)
+      for (int l = 0; l < 3; ++l) vf[fill[tris[3 * f + l]]++] = f;
+  }
+  t.rtri.assign(nt, 0u);
+  for (int fid = 0; fid < nt; ++fid) {
+    uint32_t info = 0;
+    for (int i = 0; i < 3; ++i) {
+      const int vi = tris[3 * fid + i], vj = tris[3 * fid + (i + 1) % 3];
+      if (vf[vptr[vi]] == fid) info |= 1u << i;  // first face of the vertex owns it
+      int64_t other = 0xffffffffLL;  // smallest other face on edge (vi, vj)
+      for (int a = vptr[vi]; a < vptr[vi + 1] && other == 0xffffffffLL; ++a) {
+        const int fa = vf[a];
+        if (fa == fid) continue;
+        for (int b = vptr[vj]; b < vptr[vj + 1]; ++b)
+          if (vf[b] == fa) {
+            other = fa;
+            break;
+          }
+      }
+      if (static_cast<int64_t>(fid) < other) info |= 1u << (3 + i);  // lower face owns the edge
+    }
+    t.rtri[fid] = info;
+  }
+  t.face_n.resize(static_cast<size_t>(3) * nt);
+  for (int f = 0; f < nt; ++f) {
+    const V3 a = Load(pos, tris[3 * f]), b = Load(pos, tris[3 * f + 1]), c = Load(pos, tris[3 * f + 2]);
+    const V3 n = Normalize(Cross(Sub(b, a), Sub(c, a)));
+    t.face_n[3 * f] = static_cast<float>(n.x);
+    t.face_n[3 * f + 1] = static_cast<float>(n.y);
+    t.face_n[3 * f + 2] = static_cast<float>(n.z);
+  }
+  return t;
+}
+
 std::vector<double> Geodesic(const float* pos, int nv, const std::vector<int32_t>& edges, int src) {
   const int ne = static_cast<int>(edges.size() / 2);
   std::vector<int32_t> ptr(nv + 1, 0);

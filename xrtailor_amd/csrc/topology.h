@@ -36,4 +36,17 @@ std::vector<double> Geodesic(const float* pos, int nv, const std::vector<int32_t
 
 uint64_t SplitMix64(uint64_t* state);
 
+// Contact topology of a mesh: face edge ids (edge of nodes i, i+1; Face::SetEdge,
+// xrtailor/src/memory/Face.cu:13-25), faces per edge ascending (BuildEFAdjacency,
+// xrtailor/src/physics/ClothSolverHelper.cpp:111-125), representative-triangle bits
+// (RTriBuild_Kernel, xrtailor/src/physics/representative_triangle/RTriangle.cu:19-85) and
+// rest face normals (Face::Update, xrtailor/src/memory/Face.cu:55-59).
+struct ContactTopo {
+  std::vector<int32_t> face_edges, ef_ptr, ef;
+  std::vector<float> face_n;
+  std::vector<uint32_t> rtri;
+};
+ContactTopo BuildContactTopo(const float* pos, int nv, const int32_t* tris, int nt,
+                             const std::vector<int32_t>& edges);
+
 }  // namespace xrt

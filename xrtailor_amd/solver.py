@@ -34,9 +34,11 @@ def make_params(**kw) -> XrtParams:
     return p
 
 
-def build_solver(specs, params: XrtParams, seed: int = 0, device: int = 0):
-    """A started ClothSolver with one instance per scenes.ClothSpec (pins, LRA slots and
-    the jittered initial state applied).  Returns (solver, instance ids)."""
+def build_solver(specs, params: XrtParams, seed: int = 0, device: int = 0,
+                 obstacles: bool = True):
+    """A started ClothSolver with one instance per scenes.ClothSpec (pins, LRA slots, the
+    obstacle when `obstacles` and the jittered initial state applied).
+    Returns (solver, instance ids)."""
     s = ClothSolver(device=device, params=params)
     insts = []
     for spec in specs:
@@ -45,6 +47,8 @@ def build_solver(specs, params: XrtParams, seed: int = 0, device: int = 0):
             s.Pin(i, spec.pinned)
         if len(spec.lra_slots):
             s.AddLraSlots(i, spec.lra_slots)
+        if obstacles and spec.obstacle is not None:
+            s.SetObstacle(i, *spec.obstacle)
         s.SetState(i, spec.x0)
         insts.append(i)
     s.Start(seed)
@@ -148,6 +152,31 @@ class ClothSolver:
     def AddLraSlots(self, inst: int, slots) -> None:
         slots = np.ascontiguousarray(slots, dtype=np.int32)
         check(self.lib.xrt_add_lra_slots(self.ctx, inst, iptr(slots), slots.size))
+
+    def SetObstacle(self, inst: int, pos: np.ndarray, tris: np.ndarray) -> None:
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+        tris = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1)
+        check(self.lib.xrt_set_obstacle(self.ctx, inst, fptr(pos), pos.size // 3, iptr(tris),
+                                        tris.size // 3))
+
+    def UpdateObstacle(self, inst: int, pos: np.ndarray) -> None:
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+        check(self.lib.xrt_update_obstacle(self.ctx, inst, fptr(pos)))
+
+    def Contacts(self, inst: int) -> dict:
+        """Contact records of the last collision pass and the pair statistics."""
+        n = self.counts(inst)
+        vf = np.zeros(n.nv, np.int32)
+        vq = np.zeros(3 * n.nv, np.float32)
+        rt = np.zeros(n.nv, np.int32)
+        rq = np.zeros(3 * n.nv, np.float32)
+        ee = np.zeros(16 * max(n.n_edges, 1), np.float32)
+        st = np.zeros(4, np.int32)
+        check(self.lib.xrt_get_contacts(self.ctx, inst, iptr(vf), fptr(vq), iptr(rt), fptr(rq),
+                                        fptr(ee), iptr(st)))
+        return dict(vf_face=vf, vf_q=vq.reshape(-1, 3), rt_face=rt, rt_q=rq.reshape(-1, 3),
+                    ee=ee[:16 * n.n_edges].reshape(-1, 16), obs_pairs=int(st[0]),
+                    pairs=int(st[1]), pair_cap=int(st[2]), overflow=bool(st[3]))
 
     def Start(self, seed: int = 0) -> None:
         check(self.lib.xrt_start(self.ctx, C.c_uint64(seed)))
