@@ -427,6 +427,7 @@ __global__ __launch_bounds__(kBlock) void RepulsionKernel(RMesh c, RMesh o,
     const int2 pr = pairs[p];
     const bool self = p >= nobs;
     const RMesh& other = self ? c : o;
+    if (pr.x < 0 || pr.x >= c.t.nf || pr.y < 0 || pr.y >= other.t.nf) continue;
     const uint32_t r0 = c.t.rtri[pr.x], r1 = other.t.rtri[pr.y];
     const int4 f0 = c.t.tris[pr.x];
     const int v0[3] = {f0.x, f0.y, f0.z};
