@@ -19,12 +19,16 @@ def sphere_drop(n=24, side=1.0, r=0.3, height=0.305, v_down=-1.0, seed=3):
     return spec, v0
 
 
-def two_layers(n=16, side=0.5, gap=0.0015, wrinkle=8e-4, seed=5):
+def two_layers(n=16, side=0.5, gap=0.0015, wrinkle=8e-4, angle=0.45, seed=5):
     """Two cloth sheets in one mesh, gap below the repulsion thickness; the sheets are
     wrinkled (vertical jitter) so that their face boxes overlap (pairs are searched at
-    thickness 1e-6)."""
+    thickness 1e-6), and the upper one is turned by `angle` about y so that edge pairs
+    cross instead of running parallel (the edge-edge weights of near-parallel edges are
+    ill-conditioned in FP32)."""
     pos, tris = scenes.grid(n, side, 0.0)
     pos2 = pos.copy()
+    c, s = np.cos(angle), np.sin(angle)
+    pos2[:, 0], pos2[:, 2] = c * pos[:, 0] - s * pos[:, 2], s * pos[:, 0] + c * pos[:, 2]
     pos2[:, 1] += gap
     P = np.concatenate([pos, pos2])
     T = np.concatenate([tris, tris + len(pos)]).astype(np.int32)
