@@ -147,6 +147,11 @@ class ContactScene:
         self.frame_substep = 0
         self.pass_cnt = 0
         self.dt = float(np.float32(xp.frame_time) / np.float32(xp.num_substeps))
+        self.tape = None  # a list: substep() appends each operator's input state (adjoint.py)
+
+    def _rec(self, *entry):
+        if self.tape is not None:
+            self.tape.append(entry)
 
     def _rmesh(self, obstacle: bool) -> RMeshIn:
         if not obstacle:
@@ -172,6 +177,8 @@ class ContactScene:
         freq = max(1, xp.num_substeps * xp.num_iterations // xp.num_collision_passes)
         interval = max(1, xp.pair_update_interval)
         lib.orc_predict(C.byref(c.s), C.byref(p), dt)
+        self._rec("predict")
+        x0s = c.x0.copy() if self.tape is not None else None
         for it in range(p.num_iterations):
             step = self.frame_substep * p.num_iterations + it
             if has_obs and step % freq == 0:
@@ -196,6 +203,7 @@ class ContactScene:
                 self.pairs = np.ascontiguousarray(np.concatenate([po, ps]), np.int32)
                 self.kind = np.ascontiguousarray(np.r_[np.zeros(len(po)), np.ones(len(ps))], np.int32)
             if self_c:
+                self._rec("repulsion")
                 cm = self._rmesh(False)
                 om = self._rmesh(True) if self.has_obs else None
                 lib.orc_repulsion(C.byref(cm), C.byref(om) if om is not None else None,
@@ -203,19 +211,32 @@ class ContactScene:
                                   xp.pbd_thickness, _d(self.delta), _i(self.count))
                 lib.orc_apply_deltas(_d(c.x), _d(self.delta), _i(self.count), self.nv, xp.repulsion_relaxation)
             for k in range(c.s_ncol):
+                self._rec("stretch", k, it, c.x.copy(), c.s_lam.copy())
                 lib.orc_stretch_color(C.byref(c.s), k, dt, it)
             if p.solve_bending:
                 for k in range(c.b_ncol):
+                    self._rec("bend", k, c.x.copy())
                     lib.orc_bend_color(C.byref(c.s), C.byref(p), k, dt)
+            if len(c.slot_vert) > 0:
+                self._rec("lra")
             lib.orc_lra(C.byref(c.s), C.byref(p))
+            if p.enable_ground:
+                self._rec("ground", c.x.copy(), x0s)
             lib.orc_ground(C.byref(c.s), C.byref(p))
             if has_obs:
                 pred = lib.orc_bvh_pred(self.obs_bvh.h)
                 ot = np.ascontiguousarray(self.o_tris).reshape(-1)
+                if self.tape is not None:
+                    predc = np.ctypeslib.as_array(pred, (3 * len(self.o_x0),)).copy()
+                    self._rec("vf", predc, self.rt_face.copy(), self.rt_q.copy(), c.x.copy(), 0.0)
                 lib.orc_vf_solve(pred, _i(ot), _i(self.rt_face), _d(self.rt_q), _d(c.x), self.nv, 0.0)
+                if self.tape is not None:
+                    self._rec("vf", predc, self.vf_face.copy(), self.vf_q.copy(), c.x.copy(), 1e-3)
                 lib.orc_vf_solve(pred, _i(ot), _i(self.vf_face), _d(self.vf_q), _d(c.x), self.nv, 1e-3)
+                self._rec("ee", self.ee.copy(), c.x.copy(), xp.relaxation_factor)
                 lib.orc_ee_solve(_d(self.ee), _d(c.x), _i(self.edges), self.ne, _d(self.delta), _i(self.count))
                 lib.orc_apply_deltas(_d(c.x), _d(self.delta), _i(self.count), self.nv, xp.relaxation_factor)
+        self._rec("finalize")
         lib.orc_finalize(C.byref(c.s), C.byref(p), dt)
         self.frame_substep += 1
         if self.frame_substep >= xp.num_substeps:

@@ -184,6 +184,25 @@ typedef struct orc_rmesh_in {
 void orc_repulsion(const orc_rmesh_in* cloth, const orc_rmesh_in* obstacle, const int32_t* pairs,
                    const int32_t* kind, long np, double h, double* delta, int32_t* count);
 
+/* ------------------------------------ adjoint products (FP64, adjoint_oracle.c) */
+/* Each reads the operator's input state (c->x before the operator, c->x0 = start of the
+ * substep) and turns the adjoint of its output into the adjoint of its input. */
+/* color k of the stretch set; lam_pre = multipliers before the color (ignored at iter 0);
+ * glam (ns) multiplier adjoints; gs (ns) += d/d compliance */
+void orc_adj_stretch_color(const orc_cloth* c, int k, double dt, int iter, const double* lam_pre,
+                           double* gx, double* glam, double* gs);
+/* color k of the bend set; gb (nb) += d/d bend compliance */
+void orc_adj_bend_color(const orc_cloth* c, const orc_params* p, int k, double dt, double* gx,
+                        double* gb);
+/* ground plane with friction; gx0 += adjoint through the friction's x0 */
+void orc_adj_ground(const orc_cloth* c, const orc_params* p, double* gx, double* gx0);
+/* VF-DCD / VF-CCD solve with the records (face, q) held fixed */
+void orc_adj_vf(const float* obs_pred, const int32_t* obs_tris, const int32_t* face, const double* q,
+                const double* x_pre, int nv, double offset, double* gx);
+/* EE-DCD solve + ApplyDeltas(relax) with the records held fixed */
+void orc_adj_ee(const double* rec12, const double* x_pre, const int32_t* edges, int ne, int nv,
+                double relax, double* gx);
+
 #ifdef __cplusplus
 }
 #endif
