@@ -192,6 +192,40 @@ int xrt_time_batched_substep(xrt_ctx* ctx, int32_t n, float* avg_ms);
 /* wall time (ms, HIP events on the context stream) of `nframes` plain or batched frames */
 int xrt_time_frames(xrt_ctx* ctx, int32_t batched, int32_t nframes, float* ms);
 
+/* ---------------------------------------------------------------- adjoint */
+/* Differentiable rollout.  The reference has no gradients; these entry points are the
+ * differentiation interface of its substep (ClothSolverBase::SimulateSwift,
+ * xrtailor/src/pipeline/base/ClothSolverBase.cpp:190-240): reverse-mode adjoints of
+ * predict, the colored stretch / bend projections, the ground plane, finalize and the
+ * body-cloth VF / EE solves, the latter replaying the forward's contact records.
+ * Unsupported (XRT_ERR_UNSUPPORTED): self contact, long-range attachments, an obstacle
+ * moved by xrt_update_obstacle since its last frame end.
+ *
+ * xrt_adjoint_begin: runs `nframes` frames of every instance from the current state (a
+ * frame boundary), keeping the state every `checkpoint_every` frames; the context is left
+ * at the final state.  Tape memory is one frame (recomputed from the checkpoints). */
+int xrt_adjoint_begin(xrt_ctx* ctx, int32_t nframes, int32_t checkpoint_every);
+/* dL/dx and dL/dv of the final state of one instance (3*nv each, NULL = zero); seeds not
+ * given stay zero */
+int xrt_adjoint_seed(xrt_ctx* ctx, int32_t inst, const float* grad_x, const float* grad_v);
+/* reverse pass over the recorded frames; afterwards the gradients below are available and
+ * the context is at the final state again */
+int xrt_adjoint_backward(xrt_ctx* ctx);
+/* gradients of one instance: per-edge stretch compliance (ne, the instance's edge order)
+ * and per-element bend compliance (nb, bend order); either may be NULL */
+int xrt_adjoint_get_grad_compliance(xrt_ctx* ctx, int32_t inst, float* grad_stretch,
+                                    float* grad_bend);
+/* gradients w.r.t. the initial positions and velocities (3*nv each; either may be NULL) */
+int xrt_adjoint_get_grad_state(xrt_ctx* ctx, int32_t inst, float* grad_x0, float* grad_v0);
+/* reverse of ONE substep from the current state (taken as a frame start) with the current
+ * adjoint seeds as the adjoints of its end state; the seeds become the start state's
+ * adjoints, compliance gradients accumulate; the state is not advanced */
+int xrt_adjoint_substep(xrt_ctx* ctx);
+/* timing helper (HIP events, ms per substep over `n` substeps of every instance from the
+ * current state, which is restored): ms[0] forward substep (wide launches), ms[1] taped
+ * forward substep, ms[2] reverse substep */
+int xrt_time_adjoint(xrt_ctx* ctx, int32_t n, float ms[3]);
+
 /* ------------------------------------------------------------------- LBVH */
 /* Standalone linear BVH over triangle faces (Karras 2012), the reference's BVH class
  * (xrtailor/include/xrtailor/physics/broad_phase/lbvh/BVH.cuh:77-125).  Leaves are the

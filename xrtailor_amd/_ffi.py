@@ -105,6 +105,13 @@ SIGNATURES = {
     "xrt_frames_batched": [_P, _I],
     "xrt_time_batched_substep": [_P, _I, _FP],
     "xrt_time_frames": [_P, _I, _I, _FP],
+    "xrt_adjoint_begin": [_P, _I, _I],
+    "xrt_adjoint_seed": [_P, _I, _FP, _FP],
+    "xrt_adjoint_backward": [_P],
+    "xrt_adjoint_get_grad_compliance": [_P, _I, _FP, _FP],
+    "xrt_adjoint_get_grad_state": [_P, _I, _FP, _FP],
+    "xrt_adjoint_substep": [_P],
+    "xrt_time_adjoint": [_P, _I, _FP],
     "xrt_bvh_create": [C.POINTER(_P), C.c_int],
     "xrt_bvh_destroy": [_P],
     "xrt_bvh_build": [_P, _FP, _FP, _FP, _I, _IP, _I, _F, _I, _F],

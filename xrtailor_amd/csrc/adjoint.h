@@ -1,0 +1,74 @@
+// Wide (all instances per launch) forward kernels with an optional tape, and the
+// reverse-mode (adjoint) kernels of the substep.
+//
+// A wide launch processes one color of every instance: grid (ceil(max_cnt / 256), n_inst),
+// blockIdx.y = instance, so a whole substep is a fixed sequence of launches whatever the
+// number of instances.  The forward replays bit for bit: the tape-writing and the plain
+// wide forward are the same kernel.
+//
+// Tape of one substep (float4 units, all instances concatenated):
+//   x0s                 N        start-of-substep x0
+//   per iteration:
+//     stretch           2 E      (x_i.xyz, lambda before), (x_j.xyz, -) per constraint
+//     bend              3 B      x1..x4 packed (12 floats) per element
+//     ground            N        x before the ground plane (when enabled)
+//     contact           N        x before the VF solves (when obstacles are in contact)
+//     ee                N        x before the EE solve
+// Adjoint state: gx0, gv, gx float4[N] (w unused); glam float[E]; gcomp float[E],
+// gbcomp float[B] accumulate d/d compliance.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "bvh.h"
+#include "contact.h"
+#include "device.h"
+
+namespace xrt {
+
+struct WideArgs {
+  const InstDesc* inst;
+  const int32_t* s_coloff;
+  const int32_t* b_coloff;
+  const StretchC* sc;
+  float* slam;
+  const int4* bidx;
+  const float2* bpar;
+  float4* x;         // iterate
+  const float4* xw;  // a state array whose .w is the inverse mass (reverse kernels)
+};
+
+// ------------------------------------------------------------------ forward
+hipError_t LaunchWideStretch(hipStream_t s, const WideArgs& a, int n_inst, int color, int max_cnt,
+                             float inv_dt2, int iter, float4* tape);
+hipError_t LaunchWideBend(hipStream_t s, const WideArgs& a, int n_inst, int color, int max_cnt,
+                          float inv_dt2, const BendMode& m, float4* tape);
+hipError_t LaunchCopy4(hipStream_t s, float4* dst, const float4* src, int n);
+hipError_t LaunchZero4(hipStream_t s, float4* p, int n);
+hipError_t LaunchZero1(hipStream_t s, float* p, int n);
+
+// ------------------------------------------------------------------ reverse
+// finalize: gx = gx0 + gv * f ; gx0 = -gv * f      (f = damp_factor / dt)
+hipError_t LaunchFinalizeAdj(hipStream_t s, float4* gx0, const float4* gv, float4* gx, int n,
+                             float f);
+// predict: gx0 += gx ; gv = gx * dt
+hipError_t LaunchPredictAdj(hipStream_t s, float4* gx0, float4* gv, const float4* gx, int n,
+                            float dt);
+hipError_t LaunchGroundAdj(hipStream_t s, const float4* xpre, const float4* x0s, float4* gx,
+                           float4* gx0, int n, float ground_y, float mu);
+hipError_t LaunchWideStretchAdj(hipStream_t s, const WideArgs& a, int n_inst, int color,
+                                int max_cnt, float inv_dt2, int iter, const float4* tape,
+                                float4* gx, float* glam, float* gcomp);
+hipError_t LaunchWideBendAdj(hipStream_t s, const WideArgs& a, int n_inst, int color, int max_cnt,
+                             float inv_dt2, const BendMode& m, const float4* tape, float4* gx,
+                             float* gbcomp);
+// contacts (adjoint_contact.hip, built FP-strict like the forward contact kernels so that
+// the replayed activity tests decide as the forward did); one instance per call
+// obs_pred: the obstacle's contact positions (x1 + tol n1) of the pass that made the records
+hipError_t LaunchVfAdj(hipStream_t s, const float4* x_pc, int n, const int4* obs_tris,
+                       const float4* obs_pred, const int32_t* rt_face, const float4* rt_q,
+                       const int32_t* vf_face, const float4* vf_q, float4* gx);
+// cnt (int[nv]) and gadd (double[3 nv]) are zero on entry and on return
+hipError_t LaunchEeAdj(hipStream_t s, const float4* x_pe, const int2* edges, int ne, int nv,
+                       const EeRec* rec, float relax, float4* gx, int32_t* cnt, double* gadd);
+
+}  // namespace xrt

@@ -1,0 +1,155 @@
+// Reverse-mode kernels of the body-cloth contact solves, replaying the forward's contact
+// records (built with -ffp-contract=off like contact.hip: the activity tests below are the
+// forward's expressions and decide identically on the taped input).
+//
+// VF (CCD offset 0, then DCD offset 1e-3): x' = x - min(n.(x - q) - off, 0) n with the
+//   record (face, q) fixed; n = FaceNormal(pred face), so gx' -> gx - n (n.gx) when active.
+//   Exact under replay: q lies in the face plane, n.q does not move with x.
+// EE: delta_i += -(1 - s) lam fn, delta_j += -s lam fn with lam = (c1 - c2).fn - 1e-3,
+//   c1 = (1 - s) x_i + s x_j, then x += relax delta / count.  The record (s, t, fn and the
+//   obstacle edge) is held fixed; counts are recounted from the taped state.
+#include <hip/hip_runtime.h>
+
+#include "adjoint.h"
+#include "bvh_device.h"
+#include "contact_math.h"
+
+namespace xrt {
+namespace {
+using namespace cm;
+constexpr int kBs = 256;
+inline int Blocks(int n) { return (n + kBs - 1) / kBs; }
+
+__device__ __forceinline__ bool VfActive(BvhView t, int f, const float4 q, float offset, F3 x,
+                                         F3& nrm, float& lam) {
+  const int4 tr = t.tris[f];
+  nrm = FaceNormal(L3(t.pred[tr.x]), L3(t.pred[tr.y]), L3(t.pred[tr.z]));
+  lam = D(nrm, S(x, L3(q))) - offset;
+  return lam < 0;
+}
+
+// the EE record of edge e is active at this state (SolveEEDCD's tests)
+__device__ __forceinline__ bool EeActive(const EeRec& r, F3 xi, F3 xj) {
+  if (r.a.x > 1e6f) return false;
+  const float s = r.a.y, t = r.a.z;
+  const F3 c1 = A(M(xi, 1 - s), M(xj, s));
+  const F3 c2 = A(M(L3(r.p2), 1 - t), M(L3(r.q2), t));
+  const F3 d = S(c1, c2);
+  if (Len(d) < kEpsC) return false;
+  return !(D(d, L3(r.fn)) - 1e-3f > 0);
+}
+}  // namespace
+
+__global__ __launch_bounds__(kBs) void VfAdjKernel(const float4* __restrict__ x_pc, int n, BvhView t,
+                                                   const int32_t* __restrict__ rt_face,
+                                                   const float4* __restrict__ rt_q,
+                                                   const int32_t* __restrict__ vf_face,
+                                                   const float4* __restrict__ vf_q,
+                                                   float4* __restrict__ gx) {
+  const int i = blockIdx.x * kBs + threadIdx.x;
+  if (i >= n) return;
+  // forward replay: CCD solve, then DCD solve on its result
+  float4 xi = x_pc[i];
+  F3 nc{0, 0, 0}, nd{0, 0, 0};
+  bool ac = false, ad = false;
+  float lam;
+  const int fc = rt_face[i];
+  if (fc >= 0 && VfActive(t, fc, rt_q[i], 0.0f, L3(xi), nc, lam)) {
+    ac = true;
+    xi.x += -lam * nc.x;
+    xi.y += -lam * nc.y;
+    xi.z += -lam * nc.z;
+  }
+  const int fd = vf_face[i];
+  if (fd >= 0) ad = VfActive(t, fd, vf_q[i], 1e-3f, L3(xi), nd, lam);
+  if (!ac && !ad) return;
+  float4 g = gx[i];
+  if (ad) {
+    const float p = nd.x * g.x + nd.y * g.y + nd.z * g.z;
+    g.x -= p * nd.x; g.y -= p * nd.y; g.z -= p * nd.z;
+  }
+  if (ac) {
+    const float p = nc.x * g.x + nc.y * g.y + nc.z * g.z;
+    g.x -= p * nc.x; g.y -= p * nc.y; g.z -= p * nc.z;
+  }
+  gx[i] = g;
+}
+
+__global__ __launch_bounds__(kBs) void EeAdjCountKernel(const float4* __restrict__ x,
+                                                        const int2* __restrict__ edges, int ne,
+                                                        const EeRec* __restrict__ rec,
+                                                        int32_t* __restrict__ cnt) {
+  const int e = blockIdx.x * kBs + threadIdx.x;
+  if (e >= ne) return;
+  const int2 ed = edges[e];
+  if (!EeActive(rec[e], L3(x[ed.x]), L3(x[ed.y]))) return;
+  atomicAdd(cnt + ed.x, 1);
+  atomicAdd(cnt + ed.y, 1);
+}
+
+__global__ __launch_bounds__(kBs) void EeAdjKernel(const float4* __restrict__ x,
+                                                   const int2* __restrict__ edges, int ne,
+                                                   const EeRec* __restrict__ rec, float relax,
+                                                   const float4* __restrict__ gx,
+                                                   const int32_t* __restrict__ cnt,
+                                                   double* __restrict__ gadd) {
+  const int e = blockIdx.x * kBs + threadIdx.x;
+  if (e >= ne) return;
+  const int2 ed = edges[e];
+  const EeRec r = rec[e];
+  if (!EeActive(r, L3(x[ed.x]), L3(x[ed.y]))) return;
+  const float s = r.a.y;
+  const F3 fn = L3(r.fn);
+  const float4 gi = gx[ed.x], gj = gx[ed.y];
+  const float ci = relax / static_cast<float>(cnt[ed.x]);
+  const float cj = relax / static_cast<float>(cnt[ed.y]);
+  // adjoint of lam, then of c1 = (1 - s) x_i + s x_j
+  const float gl = -(1 - s) * ci * (fn.x * gi.x + fn.y * gi.y + fn.z * gi.z) -
+                   s * cj * (fn.x * gj.x + fn.y * gj.y + fn.z * gj.z);
+  const F3 gd = M(fn, gl);
+  const F3 ai = M(gd, 1 - s), aj = M(gd, s);
+  atomicAdd(gadd + 3 * ed.x, static_cast<double>(ai.x));
+  atomicAdd(gadd + 3 * ed.x + 1, static_cast<double>(ai.y));
+  atomicAdd(gadd + 3 * ed.x + 2, static_cast<double>(ai.z));
+  atomicAdd(gadd + 3 * ed.y, static_cast<double>(aj.x));
+  atomicAdd(gadd + 3 * ed.y + 1, static_cast<double>(aj.y));
+  atomicAdd(gadd + 3 * ed.y + 2, static_cast<double>(aj.z));
+}
+
+__global__ __launch_bounds__(kBs) void EeAdjApplyKernel(float4* __restrict__ gx, int n,
+                                                        int32_t* __restrict__ cnt,
+                                                        double* __restrict__ gadd) {
+  const int i = blockIdx.x * kBs + threadIdx.x;
+  if (i >= n) return;
+  if (cnt[i] > 0) {
+    float4 g = gx[i];
+    g.x += static_cast<float>(gadd[3 * i]);
+    g.y += static_cast<float>(gadd[3 * i + 1]);
+    g.z += static_cast<float>(gadd[3 * i + 2]);
+    gx[i] = g;
+    gadd[3 * i] = gadd[3 * i + 1] = gadd[3 * i + 2] = 0.0;
+    cnt[i] = 0;
+  }
+}
+
+hipError_t LaunchVfAdj(hipStream_t s, const float4* x_pc, int n, const int4* obs_tris,
+                       const float4* obs_pred, const int32_t* rt_face, const float4* rt_q,
+                       const int32_t* vf_face, const float4* vf_q, float4* gx) {
+  if (n <= 0) return hipSuccess;
+  BvhView t;
+  t.tris = obs_tris;
+  t.pred = const_cast<float4*>(obs_pred);
+  VfAdjKernel<<<Blocks(n), kBs, 0, s>>>(x_pc, n, t, rt_face, rt_q, vf_face, vf_q, gx);
+  return hipGetLastError();
+}
+
+hipError_t LaunchEeAdj(hipStream_t s, const float4* x_pe, const int2* edges, int ne, int nv,
+                       const EeRec* rec, float relax, float4* gx, int32_t* cnt, double* gadd) {
+  if (ne <= 0 || nv <= 0) return hipSuccess;
+  EeAdjCountKernel<<<Blocks(ne), kBs, 0, s>>>(x_pe, edges, ne, rec, cnt);
+  EeAdjKernel<<<Blocks(ne), kBs, 0, s>>>(x_pe, edges, ne, rec, relax, gx, cnt, gadd);
+  EeAdjApplyKernel<<<Blocks(nv), kBs, 0, s>>>(gx, nv, cnt, gadd);
+  return hipGetLastError();
+}
+
+}  // namespace xrt

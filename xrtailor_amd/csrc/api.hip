@@ -13,6 +13,7 @@
 namespace xrt {
 namespace {
 thread_local std::string g_last_error;
+}  // namespace
 
 StepConsts MakeConsts(const xrt_params& p) {
   StepConsts k{};
@@ -40,6 +41,7 @@ StepConsts MakeConsts(const xrt_params& p) {
 
 BendMode ModeOf(const StepConsts& k) { return BendMode{k.bend_dlo, k.bend_th0, k.bend_th1, k.bend_sgn}; }
 
+namespace {
 void DestroyGraph(xrt_ctx* c) {
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);

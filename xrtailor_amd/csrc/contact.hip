@@ -4,31 +4,14 @@
 
 #include "bvh_device.h"
 #include "contact.h"
+#include "contact_math.h"
 
 namespace xrt {
 namespace {
+using namespace cm;
 constexpr int kBlock = 256;
-constexpr float kEpsC = 1e-6f;            // EPSILON
-constexpr float kScalarMax = 3.402823466e38f;  // SCALAR_MAX
 constexpr float kFix = 4294967296.0f;     // 2^32: 32.32 fixed point
 inline int Blocks(int n) { return (n + kBlock - 1) / kBlock; }
-
-struct F3 {
-  float x, y, z;
-};
-__device__ __forceinline__ F3 L3(float4 a) { return F3{a.x, a.y, a.z}; }
-__device__ __forceinline__ F3 A(F3 a, F3 b) { return F3{a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ F3 S(F3 a, F3 b) { return F3{a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ F3 M(F3 a, float s) { return F3{a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ float D(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ F3 C(F3 a, F3 b) {
-  return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-__device__ __forceinline__ float Len(F3 a) { return sqrtf(D(a, a)); }
-__device__ __forceinline__ F3 Nz(F3 a) { return M(a, 1.0f / Len(a)); }
-__device__ __forceinline__ float Mixed(F3 a, F3 b, F3 c) { return D(a, C(b, c)); }
-__device__ __forceinline__ float Cl(float x, float a, float b) { return x < a ? a : (x > b ? b : x); }
-__device__ __forceinline__ F3 FaceNormal(F3 a, F3 b, F3 c) { return Nz(C(S(b, a), S(c, a))); }
 
 // Ericson ClosestPtPointTriangle (BasicPrimitiveTests.cu:6-100)
 __device__ float NearestPoint(F3 p, F3 a, F3 b, F3 c, F3& q) {

@@ -152,31 +152,38 @@ int ContactStart(xrt_ctx* ctx) {
   return XRT_OK;
 }
 
-// One plain substep of an instance with contacts (see the order at the top of the file).
-int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendMode& bm) {
+// Contact phases of one instance's substep, shared by the plain driver below and the
+// wide (all instances per launch) driver of the adjoint.
+int ContactSubstepBegin(xrt_ctx* ctx, int inst) {
+  ContactData& c = *ctx->contact[inst];
+  hipStream_t s = ctx->stream;
+  if (c.frame_substep == 0) {  // frame start
+    if (c.obs_nf > 0)
+      XRT_CHECK_HIP(LbvhBuild(c.obs_bvh, c.o_x0.ptr, nullptr, nullptr, 0.0f, 0, 0.0f, s));
+    if (ctx->params.enable_self_contact)
+      XRT_CHECK_HIP(LbvhBuild(c.cloth_bvh, ctx->x0.ptr + ctx->inst[inst].desc.vbeg, nullptr,
+                              nullptr, 0.0f, 0, 0.0f, s));
+    c.pass_cnt = 0;
+  }
+  return XRT_OK;
+}
+
+// collision pass when due (*pass_ran says whether it ran), pair refresh and repulsion
+int ContactIterPre(xrt_ctx* ctx, int inst, int it, bool* pass_ran) {
   HostInstance& h = ctx->inst[inst];
   ContactData& c = *ctx->contact[inst];
   const xrt_params& p = ctx->params;
   const InstDesc& d = h.desc;
   hipStream_t s = ctx->stream;
   float4* x0 = ctx->x0.ptr + d.vbeg;
-  float4* v = ctx->v.ptr + d.vbeg;
   float4* x = ctx->x.ptr + d.vbeg;
   const bool has_obs = c.obs_nf > 0 && p.enable_obstacle_contact;
   const bool self = p.enable_self_contact != 0;
   EeRec* ee = reinterpret_cast<EeRec*>(c.ee.ptr);
-
-  if (c.frame_substep == 0) {  // frame start
-    if (c.obs_nf > 0)
-      XRT_CHECK_HIP(LbvhBuild(c.obs_bvh, c.o_x0.ptr, nullptr, nullptr, 0.0f, 0, 0.0f, s));
-    if (self) XRT_CHECK_HIP(LbvhBuild(c.cloth_bvh, x0, nullptr, nullptr, 0.0f, 0, 0.0f, s));
-    c.pass_cnt = 0;
-  }
   const int frequency = std::max(1, p.num_substeps * p.num_iterations / p.num_collision_passes);
   const int interval = std::max(1, p.pair_update_interval);
-
-  XRT_CHECK_HIP(LaunchPredict(s, x0, v, x, d.nv, k));
-  for (int it = 0; it < k.iterations; ++it) {
+  if (pass_ran) *pass_ran = false;
+  {
     const int step = c.frame_substep * p.num_iterations + it;
     if (has_obs && step % frequency == 0) {
       const float alpha = (static_cast<float>(c.pass_cnt) + 1.0f) / static_cast<float>(p.num_collision_passes);
@@ -188,6 +195,7 @@ int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendM
       XRT_CHECK_HIP(LaunchVfCcdGenerate(s, x0, x, d.nv, c.obs_bvh, c.rt_face.ptr, c.rt_q.ptr));
       XRT_CHECK_HIP(LaunchEeDcdGenerate(s, x, c.c_edges.ptr, h.ne(), c.obs_bvh, c.o_rtri.ptr, ee));
       c.pass_cnt++;
+      if (pass_ran) *pass_ran = true;
     }
     if (self && it % interval == 0) {
       XRT_CHECK_HIP(LbvhUpdate(c.cloth_bvh, x, x0, 1, s));
@@ -212,6 +220,60 @@ int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendM
                                     c.scal.ptr + 2, c.pair_cap, p.pbd_thickness, c.dq.ptr, c.cnt.ptr));
       XRT_CHECK_HIP(LaunchApplyFixedDeltas(s, x, d.nv, c.dq.ptr, c.cnt.ptr, p.repulsion_relaxation));
     }
+  }
+  return XRT_OK;
+}
+
+// VF-CCD solve (offset 0), then VF-DCD solve (offset 1e-3)
+int ContactVfSolve(xrt_ctx* ctx, int inst) {
+  ContactData& c = *ctx->contact[inst];
+  if (!(c.obs_nf > 0 && ctx->params.enable_obstacle_contact)) return XRT_OK;
+  const InstDesc& d = ctx->inst[inst].desc;
+  float4* x = ctx->x.ptr + d.vbeg;
+  XRT_CHECK_HIP(LaunchVfSolve(ctx->stream, x, d.nv, c.obs_bvh, c.rt_face.ptr, c.rt_q.ptr, 0.0f));
+  XRT_CHECK_HIP(LaunchVfSolve(ctx->stream, x, d.nv, c.obs_bvh, c.vf_face.ptr, c.vf_q.ptr, 1e-3f));
+  return XRT_OK;
+}
+
+// EE-DCD deltas -> ApplyDeltas x relaxation factor
+int ContactEeSolve(xrt_ctx* ctx, int inst) {
+  ContactData& c = *ctx->contact[inst];
+  if (!(c.obs_nf > 0 && ctx->params.enable_obstacle_contact)) return XRT_OK;
+  const HostInstance& h = ctx->inst[inst];
+  float4* x = ctx->x.ptr + h.desc.vbeg;
+  EeRec* ee = reinterpret_cast<EeRec*>(c.ee.ptr);
+  XRT_CHECK_HIP(LaunchEeSolve(ctx->stream, x, c.c_edges.ptr, h.ne(), ee, c.dq.ptr, c.cnt.ptr));
+  XRT_CHECK_HIP(LaunchApplyFixedDeltas(ctx->stream, x, h.desc.nv, c.dq.ptr, c.cnt.ptr,
+                                       ctx->params.relaxation_factor));
+  return XRT_OK;
+}
+
+// frame bookkeeping after finalize; frame end: obstacle x0 = x
+int ContactSubstepEnd(xrt_ctx* ctx, int inst) {
+  ContactData& c = *ctx->contact[inst];
+  if (++c.frame_substep >= ctx->params.num_substeps) {
+    c.frame_substep = 0;
+    c.obstacle_moving = false;
+    if (c.obs_nf > 0)
+      XRT_CHECK_HIP(hipMemcpyAsync(c.o_x0.ptr, c.o_x.ptr, sizeof(float4) * c.obs_nv,
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return XRT_OK;
+}
+
+// One plain substep of an instance with contacts (see the order at the top of the file).
+int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendMode& bm) {
+  HostInstance& h = ctx->inst[inst];
+  const InstDesc& d = h.desc;
+  hipStream_t s = ctx->stream;
+  float4* x0 = ctx->x0.ptr + d.vbeg;
+  float4* v = ctx->v.ptr + d.vbeg;
+  float4* x = ctx->x.ptr + d.vbeg;
+  int rc = ContactSubstepBegin(ctx, inst);
+  if (rc != XRT_OK) return rc;
+  XRT_CHECK_HIP(LaunchPredict(s, x0, v, x, d.nv, k));
+  for (int it = 0; it < k.iterations; ++it) {
+    if ((rc = ContactIterPre(ctx, inst, it, nullptr)) != XRT_OK) return rc;
     for (int col = 0; col < h.s_ncol; ++col) {
       const int b = h.sbeg + h.s_coloff[col];
       XRT_CHECK_HIP(LaunchStretch(s, ctx->sc.ptr + b, ctx->slam.ptr + b, x,
@@ -228,21 +290,11 @@ int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendM
                               ctx->lra_ptr.ptr + d.lra_vbeg, ctx->lra_slot.ptr, ctx->lra_rest.ptr,
                               d.nv, k.stretchiness));
     if (k.enable_ground) XRT_CHECK_HIP(LaunchGround(s, x, x0, d.nv, k.ground_y, k.friction));
-    if (has_obs) {
-      XRT_CHECK_HIP(LaunchVfSolve(s, x, d.nv, c.obs_bvh, c.rt_face.ptr, c.rt_q.ptr, 0.0f));
-      XRT_CHECK_HIP(LaunchVfSolve(s, x, d.nv, c.obs_bvh, c.vf_face.ptr, c.vf_q.ptr, 1e-3f));
-      XRT_CHECK_HIP(LaunchEeSolve(s, x, c.c_edges.ptr, h.ne(), ee, c.dq.ptr, c.cnt.ptr));
-      XRT_CHECK_HIP(LaunchApplyFixedDeltas(s, x, d.nv, c.dq.ptr, c.cnt.ptr, p.relaxation_factor));
-    }
+    if ((rc = ContactVfSolve(ctx, inst)) != XRT_OK) return rc;
+    if ((rc = ContactEeSolve(ctx, inst)) != XRT_OK) return rc;
   }
   XRT_CHECK_HIP(LaunchFinalize(s, x0, v, x, d.nv, k));
-  if (++c.frame_substep >= p.num_substeps) {  // frame end
-    c.frame_substep = 0;
-    if (c.obs_nf > 0)
-      XRT_CHECK_HIP(hipMemcpyAsync(c.o_x0.ptr, c.o_x.ptr, sizeof(float4) * c.obs_nv,
-                                   hipMemcpyDeviceToDevice, s));
-  }
-  return XRT_OK;
+  return ContactSubstepEnd(ctx, inst);
 }
 
 }  // namespace xrt
@@ -278,6 +330,7 @@ int xrt_update_obstacle(xrt_ctx* ctx, int32_t inst, const float* pos) {
   for (float f : p) XRT_REQUIRE(std::isfinite(f), XRT_ERR_NOT_FINITE, "non-finite obstacle position");
   const std::vector<float4> q = ToF4(p);
   XRT_CHECK_HIP(c->o_x.Upload(q.data(), q.size(), ctx->stream));
+  c->obstacle_moving = true;
   XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return XRT_OK;
 }

@@ -79,6 +79,32 @@ struct ContactData {
   DevBuf<int2> pairs;
   DevBuf<int32_t> scal;  // [0] zero, [1] obstacle-pair count, [2] total pair count
   int frame_substep = 0, pass_cnt = 0;
+  bool obstacle_moving = false;  // xrt_update_obstacle since the last frame end
+};
+
+// Adjoint (differentiable rollout) state: checkpoints, one frame of tape, adjoints.
+struct AdjointData {
+  int n_frames = 0, ckpt_every = 1, n_passes = 0, frequency = 1;
+  bool recorded = false, has_grad = false, contacts = false;
+  int n_total = 0, e_total = 0, b_total = 0;  // vertices / stretch / bend over instances
+  std::vector<int> s_max, b_max;              // largest color c over instances
+  // tape of one frame: substep s at s * sub_stride; iteration it at n_total + it * it_stride
+  size_t it_stride = 0, sub_stride = 0, off_s = 0, off_b = 0, off_g = 0, off_c = 0, off_e = 0;
+  DevBuf<float4> tape;
+  // contact records of each collision pass of a frame, per instance
+  struct Records {
+    DevBuf<int32_t> vf_face, rt_face;
+    DevBuf<float4> vf_q, rt_q, pred;
+    DevBuf<char> ee;
+  };
+  std::vector<std::unique_ptr<Records>> rec;
+  DevBuf<float4> ckpt;    // (x0, v) every ckpt_every frames: [n_ckpt][2][n_total]
+  DevBuf<float4> fstart;  // (x0, v) at each frame start of a segment: [ckpt_every][2][n_total]
+  DevBuf<float4> final_state;  // (x0, v) after the recorded frames
+  DevBuf<float4> gx0, gv, gx;
+  DevBuf<float> glam, gcomp, gbcomp;
+  DevBuf<int32_t> ee_cnt;
+  DevBuf<double> ee_gadd;
 };
 }  // namespace xrt
 
@@ -112,6 +138,8 @@ struct xrt_ctx {
   int max_slots = 0;
   // per-instance contact state (allocated by xrt_start when contacts can occur)
   std::vector<std::unique_ptr<xrt::ContactData>> contact;
+  // differentiable rollout (allocated by xrt_adjoint_begin)
+  std::unique_ptr<xrt::AdjointData> adj;
   // graph of one batched frame
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -120,8 +148,17 @@ struct xrt_ctx {
 };
 
 namespace xrt {
+// api.hip
+StepConsts MakeConsts(const xrt_params& p);
+BendMode ModeOf(const StepConsts& k);
 // contact_api.hip
 bool ContactNeeded(const xrt_ctx* ctx, const HostInstance& h);
 int ContactStart(xrt_ctx* ctx);
 int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendMode& bm);
+// contact phases of one instance's substep (shared by the plain and the wide drivers)
+int ContactSubstepBegin(xrt_ctx* ctx, int inst);
+int ContactIterPre(xrt_ctx* ctx, int inst, int it, bool* pass_ran);
+int ContactVfSolve(xrt_ctx* ctx, int inst);
+int ContactEeSolve(xrt_ctx* ctx, int inst);
+int ContactSubstepEnd(xrt_ctx* ctx, int inst);
 }  // namespace xrt

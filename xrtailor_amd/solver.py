@@ -293,3 +293,41 @@ class ClothSolver:
         ms = C.c_float()
         check(self.lib.xrt_time_batched_substep(self.ctx, n, C.byref(ms)))
         return ms.value
+
+    # --------------------------------------------------------------- adjoint
+    def AdjointBegin(self, n_frames: int, checkpoint_every: int = 10) -> None:
+        """Run n_frames frames of every instance, checkpointing for the reverse pass."""
+        check(self.lib.xrt_adjoint_begin(self.ctx, n_frames, checkpoint_every))
+
+    def AdjointSeed(self, inst: int, grad_x=None, grad_v=None) -> None:
+        gx = None if grad_x is None else np.ascontiguousarray(grad_x, np.float32).reshape(-1)
+        gv = None if grad_v is None else np.ascontiguousarray(grad_v, np.float32).reshape(-1)
+        check(self.lib.xrt_adjoint_seed(self.ctx, inst, fptr(gx), fptr(gv)))
+
+    def AdjointBackward(self) -> None:
+        check(self.lib.xrt_adjoint_backward(self.ctx))
+
+    def AdjointSubstep(self) -> None:
+        check(self.lib.xrt_adjoint_substep(self.ctx))
+
+    def GradCompliance(self, inst: int):
+        """(d/d stretch compliance per edge, d/d bend compliance per element)."""
+        n = self.counts(inst)
+        gs = np.zeros(max(n.n_edges, 1), np.float32)
+        gb = np.zeros(max(n.n_bends, 1), np.float32)
+        check(self.lib.xrt_adjoint_get_grad_compliance(self.ctx, inst, fptr(gs), fptr(gb)))
+        return gs[:n.n_edges], gb[:n.n_bends]
+
+    def GradState(self, inst: int):
+        """(d/d initial positions, d/d initial velocities), each (nv, 3)."""
+        nv = self.counts(inst).nv
+        gx = np.zeros(3 * nv, np.float32)
+        gv = np.zeros(3 * nv, np.float32)
+        check(self.lib.xrt_adjoint_get_grad_state(self.ctx, inst, fptr(gx), fptr(gv)))
+        return gx.reshape(-1, 3), gv.reshape(-1, 3)
+
+    def time_adjoint(self, n: int) -> tuple[float, float, float]:
+        """ms per substep: (forward, taped forward, reverse) over every instance."""
+        ms = (C.c_float * 3)()
+        check(self.lib.xrt_time_adjoint(self.ctx, n, ms))
+        return float(ms[0]), float(ms[1]), float(ms[2])
