@@ -53,7 +53,7 @@ def cpu_baseline_and_parity(specs, params, seed, budget_s=12.0):
     from oracle import oracle as orc
     from xrtailor_amd.solver import build_solver
 
-    solver, insts = build_solver(specs, params, seed=seed)
+    solver, insts = build_solver(specs, params, seed=seed, obstacles=False)
     solver.SubstepBatched()
     solver.sync()
     op, dt = orc.params_from(params)
@@ -77,6 +77,52 @@ def cpu_baseline_and_parity(specs, params, seed, budget_s=12.0):
     return base, worst, n_inst
 
 
+def adjoint_leg(frames: int = 60, checkpoint_every: int = 10, n: int = 128, seed: int = 5) -> dict:
+    """Differentiable rollout: n x n cloth dropped on a sphere (body contact on), `frames`
+    frames with a checkpoint every `checkpoint_every`.  Loss = mean squared distance of the
+    final vertices to a target made by a forward run with per-edge compliances perturbed
+    log-normally (sigma 0.5).  Reports the per-substep cost of the adjoint (taped forward +
+    reverse) against the forward substep on the same wide launches, and the whole
+    rollout's backward (checkpoint recompute included) against its forward."""
+    import numpy as np
+
+    from xrtailor_amd import scenes
+    from xrtailor_amd.solver import build_solver, make_params
+
+    spec = scenes.differentiable_drop(seed=seed, n=n)
+    params = make_params(bend_model=1, enable_obstacle_contact=1, ground_height=-1.0)
+    solver, (i,) = build_solver([spec], params, seed=seed)
+    ne = solver.counts(i).n_edges
+    z = scenes.Rng(seed + 1).normal(ne)
+    base = np.full(ne, spec.stretch_compliance)
+    solver.SetStretchCompliance(i, (base * np.exp(0.5 * z)).astype(np.float32))
+    solver.AdjointBegin(frames, frames)
+    target, _ = solver.GetState(i)
+    solver.SetStretchCompliance(i, base.astype(np.float32))
+    solver.SetState(i, spec.x0, np.zeros_like(spec.x0))
+    ms_fwd, ms_tape, ms_rev = solver.time_adjoint(20)
+    t0 = time.perf_counter()
+    solver.AdjointBegin(frames, checkpoint_every)
+    t_fwd = time.perf_counter() - t0
+    x, _ = solver.GetState(i)
+    nv = len(x)
+    loss = float(((x.astype(np.float64) - target) ** 2).sum(axis=1).mean())
+    solver.AdjointSeed(i, (2.0 / nv) * (x - target), None)
+    t0 = time.perf_counter()
+    solver.AdjointBackward()
+    t_bwd = time.perf_counter() - t0
+    gs, _ = solver.GradCompliance(i)
+    _, gv = solver.GradState(i)
+    solver.close()
+    return {"workload": f"{n}x{n} cloth over a sphere (icosphere s4 r 0.3), body contact, "
+                        f"{frames} frames, checkpoint every {checkpoint_every}",
+            "substep_ms": {"forward": ms_fwd, "taped_forward": ms_tape, "reverse": ms_rev},
+            "substep_ratio": (ms_tape + ms_rev) / ms_fwd,
+            "rollout_forward_s": t_fwd, "rollout_backward_s": t_bwd,
+            "rollout_ratio": t_bwd / t_fwd, "loss": loss,
+            "grad_finite": bool(np.isfinite(gs).all() and np.isfinite(gv).all())}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +135,7 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain path (profiling runs)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable rollout")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build measures one GPU only (instances are independent; "
@@ -106,7 +153,7 @@ def main() -> int:
         wname = "hanging cloth 64x64, 1 m, two corners pinned"
     params = make_params(bend_model=1, ground_height=-1.0)
     print(f"[bench] building {len(specs)} instances", file=sys.stderr, flush=True)
-    solver, insts = build_solver(specs, params, seed=a.seed)
+    solver, insts = build_solver(specs, params, seed=a.seed, obstacles=False)
     bytes_sub, n_verts = algorithmic_bytes(solver, insts)
     info = solver.device_info()
 
@@ -141,6 +188,13 @@ def main() -> int:
         print(f"[bench] cpu {cpu['value']:.3e} vs/s, parity {parity:.3e}", file=sys.stderr,
               flush=True)
 
+    adj = None
+    if not a.no_adjoint:
+        print("[bench] differentiable rollout", file=sys.stderr, flush=True)
+        adj = adjoint_leg()
+        print(f"[bench] adjoint substep ratio {adj['substep_ratio']:.2f}, rollout ratio "
+              f"{adj['rollout_ratio']:.2f}", file=sys.stderr, flush=True)
+
     out = {
         "metric": "vertex-substeps/s batched vs plain on 1 GPU at fixed err; adjoint:forward ratio",
         "value": value,
@@ -166,6 +220,8 @@ def main() -> int:
                      "kernel": "BatchedSubstepKernel", "kernel_ms": kern_ms,
                      "bytes_per_launch": bytes_sub},
         "cpu_baseline": cpu,
+        "adjoint_forward_ratio": adj["substep_ratio"] if adj else None,
+        "adjoint": adj,
         "device": info,
     }
     print(json.dumps(out), flush=True)

@@ -136,6 +136,17 @@ def drop_cloth(seed: int = 2, n: int = 256, side: float = 2.0, jitter: float = 1
     return ClothSpec(pos, tris, np.zeros(0, np.int32), 1e-8, 1e-4, x0, obstacle)
 
 
+def differentiable_drop(seed: int = 5, n: int = 128, side: float = 1.0, height: float = 0.35,
+                        radius: float = 0.3, jitter: float = 1e-3):
+    """Differentiable-rollout workload: an n x n cloth released above a sphere (icosphere
+    s4, radius 0.3 m, at the origin), stretch compliance 1e-8 per edge, bend 1e-4."""
+    pos, tris = grid(n, side, height)
+    rng = Rng(seed)
+    x0 = (pos + jitter * rng.normal(pos.size).reshape(-1, 3)).astype(np.float32)
+    spec = ClothSpec(pos, tris, np.zeros(0, np.int32), 1e-8, 1e-4, x0, icosphere(4, radius))
+    return spec
+
+
 def batched(seed: int = 4, n_instances: int = 256, side_lo: int = 32, side_hi: int = 192,
             cloth_side: float = 1.0, jitter: float = 1e-3) -> list[ClothSpec]:
     """Independent instances: grid side round(lo*(hi/lo)^u) (log-uniform), stretch
