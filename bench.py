@@ -42,18 +42,20 @@ def algorithmic_bytes(solver, insts) -> tuple[int, int]:
     return total, nv
 
 
-def cpu_baseline_and_parity(specs, params, seed, budget_s=12.0):
+def cpu_baseline_and_parity(specs, params, seed, contacts, budget_s=12.0):
     """CPU baseline leg: the serial FP64 oracle (1 core) advances each instance by one
-    substep from the initial state until the time budget is spent; the GPU's batched
-    substep from the same state is compared with it (relative error)."""
+    substep from the initial state (with its obstacle and self contact when `contacts`)
+    until the time budget is spent; the GPU's batched substep from the same state is
+    compared with it (relative error)."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from gpu_helpers import oracle_of, rel_err
     from oracle import oracle as orc
+    from oracle import scene as osc
     from xrtailor_amd.solver import build_solver
 
-    solver, insts = build_solver(specs, params, seed=seed, obstacles=False)
+    solver, insts = build_solver(specs, params, seed=seed, obstacles=contacts)
     solver.SubstepBatched()
     solver.sync()
     op, dt = orc.params_from(params)
@@ -61,8 +63,16 @@ def cpu_baseline_and_parity(specs, params, seed, budget_s=12.0):
     for i, sp in zip(insts, specs):
         oc = oracle_of(solver, i, sp.stretch_compliance, sp.bend_compliance,
                        x0=sp.x0, v=np.zeros_like(sp.x0))
-        t0 = time.perf_counter()
-        oc.substep(op, dt)
+        if contacts:
+            edges = solver.edges(i, colors=False)[0]
+            obs = None if sp.obstacle is None else (np.asarray(sp.obstacle[0], np.float32),
+                                                    sp.obstacle[1])
+            twin = osc.ContactScene(oc, sp.pos, sp.tris, edges, op, params, obstacle=obs)
+            t0 = time.perf_counter()
+            twin.substep()
+        else:
+            t0 = time.perf_counter()
+            oc.substep(op, dt)
         t_used += time.perf_counter() - t0
         worst = max(worst, rel_err(solver.GetState(i)[0], oc.state()[0]))
         done_vs += oc.nv
@@ -70,8 +80,9 @@ def cpu_baseline_and_parity(specs, params, seed, budget_s=12.0):
         if t_used >= budget_s:
             break
     solver.close()
+    what = "with body and self contact" if contacts else "without contacts"
     base = {"value": done_vs / t_used, "unit": "vertex-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"one substep of the first {n_inst} of {len(specs)} instances "
+            "sample": f"one substep ({what}) of the first {n_inst} of {len(specs)} instances "
                       f"({done_vs} vertex-substeps, {t_used:.1f} s), serial FP64 oracle; "
                       f"{os.cpu_count()} host cores present"}
     return base, worst, n_inst
@@ -136,6 +147,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain path (profiling runs)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable rollout")
+    ap.add_argument("--no-contacts", action="store_true",
+                    help="batched workload without its obstacles and self contact")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build measures one GPU only (instances are independent; "
@@ -151,9 +164,13 @@ def main() -> int:
     else:
         specs = [scenes.hanging_cloth(seed=a.seed)]
         wname = "hanging cloth 64x64, 1 m, two corners pinned"
-    params = make_params(bend_model=1, ground_height=-1.0)
+    contacts = a.workload == "batched" and not a.no_contacts
+    params = make_params(bend_model=1, ground_height=-1.0, enable_obstacle_contact=int(contacts),
+                         enable_self_contact=int(contacts))
+    if contacts:
+        wname += ", sphere / capsule obstacle and self contact"
     print(f"[bench] building {len(specs)} instances", file=sys.stderr, flush=True)
-    solver, insts = build_solver(specs, params, seed=a.seed, obstacles=False)
+    solver, insts = build_solver(specs, params, seed=a.seed, obstacles=contacts)
     bytes_sub, n_verts = algorithmic_bytes(solver, insts)
     info = solver.device_info()
 
@@ -184,7 +201,8 @@ def main() -> int:
 
     cpu, parity, parity_n = None, None, 0
     if not a.no_cpu:
-        cpu, parity, parity_n = cpu_baseline_and_parity(specs, params, a.seed, a.cpu_budget)
+        cpu, parity, parity_n = cpu_baseline_and_parity(specs, params, a.seed, contacts,
+                                                        a.cpu_budget)
         print(f"[bench] cpu {cpu['value']:.3e} vs/s, parity {parity:.3e}", file=sys.stderr,
               flush=True)
 
@@ -210,7 +228,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": wname, "instances": len(specs), "vertices": n_verts,
                    "substeps": params.num_substeps, "iterations": params.num_iterations,
-                   "bend_model": "consistent", "step": "one frame of all instances"},
+                   "bend_model": "consistent", "obstacle_contact": contacts,
+                   "self_contact": contacts, "step": "one frame of all instances"},
         "plain_value": plain,
         "batched_over_plain": (value / plain) if plain else None,
         "parity_rel_err": parity,

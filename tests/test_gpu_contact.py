@@ -109,6 +109,14 @@ def test_contact_error_paths():
     xp = make_params(bend_model=1, enable_obstacle_contact=1)
     s, (i,) = build([spec], xp, seed=1)
     with pytest.raises(XrtError):
-        s.SubstepBatched()                      # contacts run on the plain path
-    with pytest.raises(XrtError):
         s.UpdateObstacle(i, np.full((len(spec.obstacle[0]), 3), np.nan, np.float32))
+    # batched contacts refuse long-range attachments
+    pos, tris = spec.pos, spec.tris
+    from xrtailor_amd.solver import ClothSolver
+    t = ClothSolver(params=xp)
+    j = t.AddCloth(pos, tris, 1e-7, 1e-3)
+    t.AddLraSlots(j, np.array([0], np.int32))
+    t.SetObstacle(j, *spec.obstacle)
+    t.Start(1)
+    with pytest.raises(XrtError):
+        t.SubstepBatched()

@@ -50,25 +50,17 @@ int AdjSetup(xrt_ctx* c) {
   XRT_REQUIRE(n_inst > 0 && n_inst <= 65535, XRT_ERR_INVALID_ARG, "1..65535 instances");
   if (!c->adj) c->adj = std::make_unique<AdjointData>();
   AdjointData& A = *c->adj;
-  A.n_total = A.e_total = A.b_total = 0;
-  A.s_max.clear();
-  A.b_max.clear();
+  const WideConfig& W = c->wide;
+  A.n_total = W.n_total;
+  A.e_total = W.e_total;
+  A.b_total = W.b_total;
+  A.s_max = W.s_max;
+  A.b_max = W.b_max;
   A.contacts = false;
   for (int i = 0; i < n_inst; ++i) {
     const HostInstance& h = c->inst[i];
     XRT_REQUIRE(h.desc.nslots == 0, XRT_ERR_UNSUPPORTED,
                 "the adjoint does not support long-range attachments");
-    A.n_total = std::max(A.n_total, h.desc.vbeg + h.nv);
-    A.e_total += h.ne();
-    A.b_total += h.nb();
-    for (int col = 0; col < h.s_ncol; ++col) {
-      if (col >= static_cast<int>(A.s_max.size())) A.s_max.push_back(0);
-      A.s_max[col] = std::max(A.s_max[col], h.s_coloff[col + 1] - h.s_coloff[col]);
-    }
-    for (int col = 0; col < h.b_ncol; ++col) {
-      if (col >= static_cast<int>(A.b_max.size())) A.b_max.push_back(0);
-      A.b_max[col] = std::max(A.b_max[col], h.b_coloff[col + 1] - h.b_coloff[col]);
-    }
     if (c->contact[i]) {
       XRT_REQUIRE(c->contact[i]->frame_substep == 0, XRT_ERR_STATE,
                   "the adjoint starts at a frame boundary");
@@ -177,58 +169,59 @@ int SnapshotRecords(xrt_ctx* c, int i, int pass) {
   return XRT_OK;
 }
 
-// One substep of every instance as wide launches; with sub >= 0 the operators' input
-// states go to tape slot `sub`.
-int WideSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, int sub) {
-  AdjointData& A = *c->adj;
+}  // namespace
+
+// One substep of every instance as wide launches, the contact phases included; with a
+// tape owner and sub >= 0 the operators' input states go to tape slot `sub`.
+int WideSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, AdjointData* A, int sub) {
+  const WideConfig& W = c->wide;
   hipStream_t s = c->stream;
-  const int N = A.n_total, n_inst = static_cast<int>(c->inst.size());
+  const int N = W.n_total, n_inst = static_cast<int>(c->inst.size());
   const WideArgs wa = MakeWideArgs(c);
-  float4* T = sub >= 0 ? A.tape.ptr + static_cast<size_t>(sub) * A.sub_stride : nullptr;
+  float4* T = (A && sub >= 0) ? A->tape.ptr + static_cast<size_t>(sub) * A->sub_stride : nullptr;
+  bool obs_contact = false;
+  for (int i = 0; i < n_inst; ++i) obs_contact = obs_contact || ObsContact(c, i);
   int rc;
-  for (int i = 0; i < n_inst; ++i)
-    if (c->contact[i] && (rc = ContactSubstepBegin(c, i)) != XRT_OK) return rc;
+  if ((rc = MultiContactSubstepBegin(c)) != XRT_OK) return rc;
   if (T) XRT_CHECK_HIP(LaunchCopy4(s, T, c->x0.ptr, N));
   XRT_CHECK_HIP(LaunchPredict(s, c->x0.ptr, c->v.ptr, c->x.ptr, N, k));
   for (int it = 0; it < k.iterations; ++it) {
-    float4* TI = T ? T + N + static_cast<size_t>(it) * A.it_stride : nullptr;
-    for (int i = 0; i < n_inst; ++i) {
-      if (!c->contact[i]) continue;
-      bool ran = false;
-      if ((rc = ContactIterPre(c, i, it, &ran)) != XRT_OK) return rc;
-      if (TI && ran && (rc = SnapshotRecords(c, i, c->contact[i]->pass_cnt - 1)) != XRT_OK) return rc;
-    }
-    for (int col = 0; col < static_cast<int>(A.s_max.size()); ++col)
-      XRT_CHECK_HIP(LaunchWideStretch(s, wa, n_inst, col, A.s_max[col], k.inv_dt2, it,
-                                      TI ? TI + A.off_s : nullptr));
+    float4* TI = T ? T + N + static_cast<size_t>(it) * A->it_stride : nullptr;
+    bool ran = false;
+    if ((rc = MultiContactIterPre(c, it, &ran)) != XRT_OK) return rc;
+    if (TI && ran)
+      for (int i = 0; i < n_inst; ++i)
+        if (ObsContact(c, i) && (rc = SnapshotRecords(c, i, c->contact[i]->pass_cnt - 1)) != XRT_OK)
+          return rc;
+    for (int col = 0; col < static_cast<int>(W.s_max.size()); ++col)
+      XRT_CHECK_HIP(LaunchWideStretch(s, wa, n_inst, col, W.s_max[col], k.inv_dt2, it,
+                                      TI ? TI + A->off_s : nullptr));
     if (k.solve_bending)
-      for (int col = 0; col < static_cast<int>(A.b_max.size()); ++col)
-        XRT_CHECK_HIP(LaunchWideBend(s, wa, n_inst, col, A.b_max[col], k.inv_dt2, bm,
-                                     TI ? TI + A.off_b : nullptr));
+      for (int col = 0; col < static_cast<int>(W.b_max.size()); ++col)
+        XRT_CHECK_HIP(LaunchWideBend(s, wa, n_inst, col, W.b_max[col], k.inv_dt2, bm,
+                                     TI ? TI + A->off_b : nullptr));
     if (k.enable_ground) {
-      if (TI) XRT_CHECK_HIP(LaunchCopy4(s, TI + A.off_g, c->x.ptr, N));
+      if (TI) XRT_CHECK_HIP(LaunchCopy4(s, TI + A->off_g, c->x.ptr, N));
       XRT_CHECK_HIP(LaunchGround(s, c->x.ptr, c->x0.ptr, N, k.ground_y, k.friction));
     }
-    if (A.contacts) {
-      if (TI) XRT_CHECK_HIP(LaunchCopy4(s, TI + A.off_c, c->x.ptr, N));
-      for (int i = 0; i < n_inst; ++i)
-        if (c->contact[i] && (rc = ContactVfSolve(c, i)) != XRT_OK) return rc;
-      if (TI) XRT_CHECK_HIP(LaunchCopy4(s, TI + A.off_e, c->x.ptr, N));
-      for (int i = 0; i < n_inst; ++i)
-        if (c->contact[i] && (rc = ContactEeSolve(c, i)) != XRT_OK) return rc;
+    if (obs_contact) {
+      if (TI) XRT_CHECK_HIP(LaunchCopy4(s, TI + A->off_c, c->x.ptr, N));
+      if ((rc = MultiContactVfSolve(c)) != XRT_OK) return rc;
+      if (TI) XRT_CHECK_HIP(LaunchCopy4(s, TI + A->off_e, c->x.ptr, N));
+      if ((rc = MultiContactEeSolve(c)) != XRT_OK) return rc;
     }
   }
   XRT_CHECK_HIP(LaunchFinalize(s, c->x0.ptr, c->v.ptr, c->x.ptr, N, k));
-  for (int i = 0; i < n_inst; ++i)
-    if (c->contact[i] && (rc = ContactSubstepEnd(c, i)) != XRT_OK) return rc;
-  return XRT_OK;
+  return MultiContactSubstepEnd(c);
 }
+
+namespace {
 
 int WideFrame(xrt_ctx* c, bool tape) {
   const StepConsts k = MakeConsts(c->params);
   const BendMode bm = ModeOf(k);
   for (int sub = 0; sub < c->params.num_substeps; ++sub) {
-    const int rc = WideSubstep(c, k, bm, tape ? sub : -1);
+    const int rc = WideSubstep(c, k, bm, c->adj.get(), tape ? sub : -1);
     if (rc != XRT_OK) return rc;
   }
   return XRT_OK;
@@ -447,7 +440,7 @@ int xrt_adjoint_substep(xrt_ctx* ctx) {
   if (A.final_state.n != 2 * static_cast<size_t>(A.n_total))
     XRT_CHECK_HIP(A.final_state.Alloc(2 * static_cast<size_t>(A.n_total)));
   if ((rc = SaveState(ctx, A.final_state.ptr)) != XRT_OK) return rc;
-  if ((rc = WideSubstep(ctx, k, bm, 0)) != XRT_OK) return rc;
+  if ((rc = WideSubstep(ctx, k, bm, &A, 0)) != XRT_OK) return rc;
   RewindContacts(ctx);
   if ((rc = LoadState(ctx, A.final_state.ptr)) != XRT_OK) return rc;
   if ((rc = ReverseSubstep(ctx, k, bm, 0)) != XRT_OK) return rc;
@@ -476,8 +469,8 @@ int xrt_time_adjoint(xrt_ctx* ctx, int32_t n, float ms[3]) {
     XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     XRT_CHECK_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     for (int j = 0; j < n; ++j) {
-      if (phase == 0) rc = WideSubstep(ctx, k, bm, -1);
-      else if (phase == 1) rc = WideSubstep(ctx, k, bm, j % S);
+      if (phase == 0) rc = WideSubstep(ctx, k, bm, nullptr, -1);
+      else if (phase == 1) rc = WideSubstep(ctx, k, bm, &A, j % S);
       else rc = ReverseSubstep(ctx, k, bm, (n - 1 - j) % S);
       if (rc != XRT_OK) return rc;
     }

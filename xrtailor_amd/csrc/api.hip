@@ -165,12 +165,16 @@ int SubstepPlain(xrt_ctx* c, int32_t inst) {
   return XRT_OK;
 }
 
+// Without contacts: one persistent launch.  With body or self contact: the wide substep
+// (each color, contact phase and LBVH operation of every instance in one launch).
 int SubstepBatched(xrt_ctx* c) {
-  for (const auto& cd : c->contact)
-    XRT_REQUIRE(!cd || (!(cd->obs_nf > 0 && c->params.enable_obstacle_contact) &&
-                        !c->params.enable_self_contact),
-                XRT_ERR_UNSUPPORTED, "the batched substep runs without contacts; use the plain path");
   const StepConsts k = MakeConsts(c->params);
+  if (ContactsActive(c)) {
+    for (const auto& h : c->inst)
+      XRT_REQUIRE(h.desc.nslots == 0, XRT_ERR_UNSUPPORTED,
+                  "batched contacts with long-range attachments: use the plain path");
+    return WideSubstep(c, k, ModeOf(k), nullptr, -1);
+  }
   XRT_CHECK_HIP(LaunchBatchedSubstep(c->stream, MakeBatchArgs(c), k,
                                      static_cast<int>(c->inst.size()), c->batched_block,
                                      c->batched_lds));
@@ -561,6 +565,22 @@ int xrt_start(xrt_ctx* ctx, uint64_t seed) {
   ctx->lds_verts = lds_verts;
   ctx->batched_lds = sizeof(float4) * (static_cast<size_t>(lds_verts) + max_slots);
   if (ctx->batched_lds > 64 * 1024) XRT_CHECK_HIP(ConfigureBatchedKernel(ctx->batched_lds));
+  // wide launches (one color of every instance): grid sized by the largest color
+  WideConfig& w = ctx->wide;
+  w = WideConfig{};
+  w.n_total = vtot;
+  w.e_total = stot;
+  w.b_total = btot;
+  for (const auto& h : ctx->inst) {
+    for (int col = 0; col < h.s_ncol; ++col) {
+      if (col >= static_cast<int>(w.s_max.size())) w.s_max.push_back(0);
+      w.s_max[col] = std::max(w.s_max[col], h.s_coloff[col + 1] - h.s_coloff[col]);
+    }
+    for (int col = 0; col < h.b_ncol; ++col) {
+      if (col >= static_cast<int>(w.b_max.size())) w.b_max.push_back(0);
+      w.b_max[col] = std::max(w.b_max[col], h.b_coloff[col + 1] - h.b_coloff[col]);
+    }
+  }
   int rc = ContactStart(ctx);
   if (rc != XRT_OK) return rc;
   ctx->started = true;

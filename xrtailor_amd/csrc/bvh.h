@@ -31,6 +31,10 @@ struct BvhView {
   Box* box = nullptr;
   int32_t* escape = nullptr;
   uint32_t* root = nullptr;  // 6 order-preserving encoded floats (lower xyz, upper xyz)
+  int32_t* flags = nullptr;     // bottom-up arrival counters (internal nodes)
+  uint32_t* counts = nullptr;   // pair traversal: pairs per lhs leaf
+  uint32_t* offsets = nullptr;  //                 their exclusive prefix
+  int32_t* total = nullptr;     //                 and their sum (1 int)
 };
 
 // Owning storage of one LBVH plus its build scratch.

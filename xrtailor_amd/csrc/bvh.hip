@@ -161,27 +161,7 @@ __global__ __launch_bounds__(kBlock) void RadixHistKernel(const uint32_t* __rest
 // single-workgroup exclusive scan of n uint32 (in place); *total receives the sum
 __global__ __launch_bounds__(1024) void ExclusiveScanKernel(uint32_t* __restrict__ a, int n,
                                                             int32_t* __restrict__ total) {
-  __shared__ uint32_t part[1024];
-  const int t = threadIdx.x;
-  const int chunk = (n + 1023) / 1024;
-  const int b = t * chunk, e = min(n, b + chunk);
-  uint32_t s = 0;
-  for (int i = b; i < e; ++i) s += a[i];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const uint32_t v = t >= off ? part[t - off] : 0u;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  uint32_t run = t > 0 ? part[t - 1] : 0u;
-  for (int i = b; i < e; ++i) {
-    const uint32_t x = a[i];
-    a[i] = run;
-    run += x;
-  }
-  if (t == 1023 && total) *total = static_cast<int32_t>(part[1023]);
+  BlockExclusiveScan(a, n, total);
 }
 
 __global__ __launch_bounds__(kBlock) void RadixScatterKernel(
@@ -322,21 +302,7 @@ __global__ __launch_bounds__(kBlock) void BottomUpKernel(const int4* __restrict_
                                                          int grow) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nf) return;
-  int node = nf - 1 + i;
-  int p = nodes[node].x;
-  // a root-ward walk is at most nf - 1 levels; the bound also guards against a malformed tree
-  for (int level = 0; p >= 0 && p < nf - 1 && level < nf; ++level) {
-    __threadfence();
-    const int old = atomicAdd(&flags[p], 1);
-    if (old == 0) return;
-    __threadfence();
-    const int4 pn = nodes[p];
-    Box m = MergeBox(box[pn.y], box[pn.z]);
-    if (grow) m = MergeBox(m, box[p]);
-    box[p] = m;
-    node = p;
-    p = pn.x;
-  }
+  BottomUpBody(nodes, box, flags, nf, grow, i);
 }
 
 __global__ __launch_bounds__(kBlock) void ZeroFlagsKernel(int32_t* flags, int n) {
@@ -370,13 +336,7 @@ __global__ __launch_bounds__(kBlock) void RefitPredKernel(const float4* __restri
                                                           int nv, float4* __restrict__ pred) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
-  const float4 p = x1[i];
-  if (n1) {
-    const float4 n = n1[i];
-    pred[i] = make_float4(p.x + tol * n.x, p.y + tol * n.y, p.z + tol * n.z, 0.0f);
-  } else {
-    pred[i] = make_float4(p.x, p.y, p.z, 0.0f);
-  }
+  RefitPredBody(x1, n1, tol, i, pred);
 }
 
 // leaf grows by its 3 pred corners (midstep) or by Face::ComputeBounds(ccd) of x [and x0]
@@ -388,24 +348,7 @@ __global__ __launch_bounds__(kBlock) void GrowLeavesKernel(const int4* __restric
                                                            int32_t* __restrict__ flags) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nf) return;
-  const int ni = nf - 1;
-  const int4 t = tris[nodes[ni + i].w];
-  Box b = box[ni + i];
-  float4 fb_lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
-  float4 fb_hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
-  const int vid[3] = {t.x, t.y, t.z};
-  for (int c = 0; c < 3; ++c) {
-    const float4 p = pa[vid[c]];
-    fb_lo.x = Mn(fb_lo.x, p.x); fb_lo.y = Mn(fb_lo.y, p.y); fb_lo.z = Mn(fb_lo.z, p.z);
-    fb_hi.x = Mx(fb_hi.x, p.x); fb_hi.y = Mx(fb_hi.y, p.y); fb_hi.z = Mx(fb_hi.z, p.z);
-    if (pb) {
-      const float4 q = pb[vid[c]];
-      fb_lo.x = Mn(fb_lo.x, q.x); fb_lo.y = Mn(fb_lo.y, q.y); fb_lo.z = Mn(fb_lo.z, q.z);
-      fb_hi.x = Mx(fb_hi.x, q.x); fb_hi.y = Mx(fb_hi.y, q.y); fb_hi.z = Mx(fb_hi.z, q.z);
-    }
-  }
-  box[ni + i] = MergeBox(b, Box{fb_lo, fb_hi});
-  if (i < ni) flags[i] = 0;
+  GrowLeafBody(nodes, tris, pa, pb, nf, i, box, flags);
 }
 
 }  // namespace
@@ -432,24 +375,7 @@ __global__ __launch_bounds__(kBlock) void CountPairsKernel(BvhView l, BvhView r,
                                                            uint32_t* __restrict__ cnt) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= l.nf) return;
-  const Box lb = l.box[l.nf - 1 + id];
-  int node = 0;
-  uint32_t c = 0;
-  const int max_steps = 2 * r.nf;  // each node is visited at most once
-  for (int step = 0; node != -1 && step < max_steps; ++step) {
-    const int4 nd = r.nodes[node];
-    if (OverlapDilated(r.box[node], lb, th)) {
-      if (nd.w >= 0) {
-        ++c;
-        node = r.escape[node];
-      } else {
-        node = nd.y;
-      }
-    } else {
-      node = r.escape[node];
-    }
-  }
-  cnt[id] = c;
+  cnt[id] = CountPairsBody(l, r, th, id);
 }
 
 // FindPairs_Kernel (BVH.cu:1189-1227): each leaf's pairs fill its range backwards
@@ -459,26 +385,7 @@ __global__ __launch_bounds__(kBlock) void FindPairsKernel(BvhView l, BvhView r, 
                                                           int2* __restrict__ pairs, int cap) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= l.nf) return;
-  const Box lb = l.box[l.nf - 1 + id];
-  const int lobj = l.nodes[l.nf - 1 + id].w;
-  const int64_t lo = off[id];
-  int64_t index = lo + cnt[id] - 1;
-  int node = 0;
-  const int max_steps = 2 * r.nf;
-  for (int step = 0; node != -1 && step < max_steps; ++step) {
-    const int4 nd = r.nodes[node];
-    if (OverlapDilated(r.box[node], lb, th)) {
-      if (nd.w >= 0) {
-        if (index >= lo && index < cap) pairs[index] = make_int2(lobj, nd.w);
-        --index;
-        node = r.escape[node];
-      } else {
-        node = nd.y;
-      }
-    } else {
-      node = r.escape[node];
-    }
-  }
+  FindPairsBody(l, r, th, id, static_cast<int64_t>(off[id]), cnt[id], pairs, cap);
 }
 
 // UpdateNodeNormals (xrtailor/src/physics/PhysicsMeshHelper.cu:196-210,282-295) as a
@@ -490,25 +397,7 @@ __global__ __launch_bounds__(kBlock) void VertexNormalsKernel(const float4* __re
                                                               int nv, float4* __restrict__ n) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nv) return;
-  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-  for (int k = vptr[v]; k < vptr[v + 1]; ++k) {
-    const int c = corner[k];
-    const int4 t = tris[c / 3];
-    const int j = c % 3;
-    const int a = j == 0 ? t.x : (j == 1 ? t.y : t.z);
-    const int b = j == 0 ? t.y : (j == 1 ? t.z : t.x);
-    const int d = j == 0 ? t.z : (j == 1 ? t.x : t.y);
-    const float4 p = x[a], pb = x[b], pd = x[d];
-    const float e0x = pb.x - p.x, e0y = pb.y - p.y, e0z = pb.z - p.z;
-    const float e1x = pd.x - p.x, e1y = pd.y - p.y, e1z = pd.z - p.z;
-    const float den = (e0x * e0x + e0y * e0y + e0z * e0z) * (e1x * e1x + e1y * e1y + e1z * e1z);
-    const float cx = e0y * e1z - e0z * e1y, cy = e0z * e1x - e0x * e1z, cz = e0x * e1y - e0y * e1x;
-    sx = sx + cx / den;
-    sy = sy + cy / den;
-    sz = sz + cz / den;
-  }
-  const float l = sqrtf(sx * sx + sy * sy + sz * sz);
-  n[v] = make_float4(sx / l, sy / l, sz / l, 0.0f);
+  n[v] = VertexNormalBody(x, tris, vptr, corner, v);
 }
 }  // namespace
 
@@ -527,6 +416,10 @@ BvhView Lbvh::view() const {
   t.box = box.ptr;
   t.escape = escape.ptr;
   t.root = root.ptr;
+  t.flags = flags.ptr;
+  t.counts = reinterpret_cast<uint32_t*>(counts.ptr);
+  t.offsets = reinterpret_cast<uint32_t*>(offsets.ptr);
+  t.total = total.ptr;
   return t;
 }
 

@@ -71,6 +71,176 @@ __device__ __forceinline__ bool OverlapDilated(const Box& rhs, const Box& lhs, f
   return true;
 }
 
+// ------------------------------------------------------------------------------------
+// Per-element bodies of the refit / traversal kernels, shared by the single-tree kernels
+// (bvh.hip) and the multi-instance ones (contact.hip) so both run the same arithmetic.
+
+// RefitExternalNodeMidstep (BVH.cu:770-818) / Update: pred = x1 + tol n1 (or x)
+__device__ __forceinline__ void RefitPredBody(const float4* __restrict__ x1,
+                                              const float4* __restrict__ n1, float tol, int i,
+                                              float4* __restrict__ pred) {
+  const float4 p = x1[i];
+  if (n1) {
+    const float4 n = n1[i];
+    pred[i] = make_float4(p.x + tol * n.x, p.y + tol * n.y, p.z + tol * n.z, 0.0f);
+  } else {
+    pred[i] = make_float4(p.x, p.y, p.z, 0.0f);
+  }
+}
+
+// leaf i grows by its 3 corners of pa (midstep) or by Face::ComputeBounds(ccd) of pa [and
+// pb]; resets the arrival flag of internal node i for the bottom-up pass that follows
+__device__ __forceinline__ void GrowLeafBody(const int4* __restrict__ nodes,
+                                             const int4* __restrict__ tris,
+                                             const float4* __restrict__ pa,
+                                             const float4* __restrict__ pb, int nf, int i,
+                                             Box* __restrict__ box, int32_t* __restrict__ flags) {
+  const int ni = nf - 1;
+  const int4 t = tris[nodes[ni + i].w];
+  Box b = box[ni + i];
+  float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+  float4 hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+  const int vid[3] = {t.x, t.y, t.z};
+  for (int c = 0; c < 3; ++c) {
+    const float4 p = pa[vid[c]];
+    lo.x = Mn(lo.x, p.x); lo.y = Mn(lo.y, p.y); lo.z = Mn(lo.z, p.z);
+    hi.x = Mx(hi.x, p.x); hi.y = Mx(hi.y, p.y); hi.z = Mx(hi.z, p.z);
+    if (pb) {
+      const float4 q = pb[vid[c]];
+      lo.x = Mn(lo.x, q.x); lo.y = Mn(lo.y, q.y); lo.z = Mn(lo.z, q.z);
+      hi.x = Mx(hi.x, q.x); hi.y = Mx(hi.y, q.y); hi.z = Mx(hi.z, q.z);
+    }
+  }
+  box[ni + i] = MergeBox(b, Box{lo, hi});
+  if (i < ni) flags[i] = 0;
+}
+
+// bottom-up boxes from leaf i: the second thread to reach a node merges its two children
+// (the union of the descendant leaves, as AssignAABBAtomic_Kernel's atomic min / max,
+// BVH.cu:540-557, and AssignAABBCAS_Kernel, BVH.cu:515-538).  grow keeps the old box too
+// (Update's grow-only semantics).
+__device__ __forceinline__ void BottomUpBody(const int4* __restrict__ nodes, Box* __restrict__ box,
+                                             int32_t* __restrict__ flags, int nf, int grow, int i) {
+  int node = nf - 1 + i;
+  int p = nodes[node].x;
+  // a root-ward walk is at most nf - 1 levels; the bound also guards against a malformed tree
+  for (int level = 0; p >= 0 && p < nf - 1 && level < nf; ++level) {
+    __threadfence();
+    const int old = atomicAdd(&flags[p], 1);
+    if (old == 0) return;
+    __threadfence();
+    const int4 pn = nodes[p];
+    Box m = MergeBox(box[pn.y], box[pn.z]);
+    if (grow) m = MergeBox(m, box[p]);
+    box[p] = m;
+    node = p;
+    p = pn.x;
+  }
+}
+
+// CountPairs_Kernel (BVH.cu:1149-1179): rhs leaves overlapping lhs leaf id dilated by th
+__device__ __forceinline__ uint32_t CountPairsBody(const BvhView& l, const BvhView& r, float th,
+                                                   int id) {
+  const Box lb = l.box[l.nf - 1 + id];
+  int node = 0;
+  uint32_t c = 0;
+  const int max_steps = 2 * r.nf;  // each node is visited at most once
+  for (int step = 0; node != -1 && step < max_steps; ++step) {
+    const int4 nd = r.nodes[node];
+    if (OverlapDilated(r.box[node], lb, th)) {
+      if (nd.w >= 0) {
+        ++c;
+        node = r.escape[node];
+      } else {
+        node = nd.y;
+      }
+    } else {
+      node = r.escape[node];
+    }
+  }
+  return c;
+}
+
+// FindPairs_Kernel (BVH.cu:1189-1227): leaf id's pairs fill [lo, lo + n) backwards; only
+// slots below cap are written
+__device__ __forceinline__ void FindPairsBody(const BvhView& l, const BvhView& r, float th, int id,
+                                              int64_t lo, uint32_t n, int2* __restrict__ pairs,
+                                              int64_t cap) {
+  const Box lb = l.box[l.nf - 1 + id];
+  const int lobj = l.nodes[l.nf - 1 + id].w;
+  int64_t index = lo + n - 1;
+  int node = 0;
+  const int max_steps = 2 * r.nf;
+  for (int step = 0; node != -1 && step < max_steps; ++step) {
+    const int4 nd = r.nodes[node];
+    if (OverlapDilated(r.box[node], lb, th)) {
+      if (nd.w >= 0) {
+        if (index >= lo && index < cap) pairs[index] = make_int2(lobj, nd.w);
+        --index;
+        node = r.escape[node];
+      } else {
+        node = nd.y;
+      }
+    } else {
+      node = r.escape[node];
+    }
+  }
+}
+
+// UpdateNodeNormals (xrtailor/src/physics/PhysicsMeshHelper.cu:196-210,282-295) for vertex
+// v as a gather over a corner CSR in ascending face order (deterministic)
+__device__ __forceinline__ float4 VertexNormalBody(const float4* __restrict__ x,
+                                                   const int4* __restrict__ tris,
+                                                   const int32_t* __restrict__ vptr,
+                                                   const int32_t* __restrict__ corner, int v) {
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  for (int k = vptr[v]; k < vptr[v + 1]; ++k) {
+    const int c = corner[k];
+    const int4 t = tris[c / 3];
+    const int j = c % 3;
+    const int a = j == 0 ? t.x : (j == 1 ? t.y : t.z);
+    const int b = j == 0 ? t.y : (j == 1 ? t.z : t.x);
+    const int d = j == 0 ? t.z : (j == 1 ? t.x : t.y);
+    const float4 p = x[a], pb = x[b], pd = x[d];
+    const float e0x = pb.x - p.x, e0y = pb.y - p.y, e0z = pb.z - p.z;
+    const float e1x = pd.x - p.x, e1y = pd.y - p.y, e1z = pd.z - p.z;
+    const float den = (e0x * e0x + e0y * e0y + e0z * e0z) * (e1x * e1x + e1y * e1y + e1z * e1z);
+    const float cx = e0y * e1z - e0z * e1y, cy = e0z * e1x - e0x * e1z, cz = e0x * e1y - e0y * e1x;
+    sx = sx + cx / den;
+    sy = sy + cy / den;
+    sz = sz + cz / den;
+  }
+  const float l = sqrtf(sx * sx + sy * sy + sz * sz);
+  return make_float4(sx / l, sy / l, sz / l, 0.0f);
+}
+
+// exclusive scan of n uint32 in place by one 1024-thread workgroup; *total (if given)
+// receives the sum
+__device__ __forceinline__ void BlockExclusiveScan(uint32_t* __restrict__ a, int n,
+                                                   int32_t* __restrict__ total) {
+  __shared__ uint32_t part[1024];
+  const int t = threadIdx.x;
+  const int chunk = (n + 1023) / 1024;
+  const int b = t * chunk, e = min(n, b + chunk);
+  uint32_t s = 0;
+  for (int i = b; i < e; ++i) s += a[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = t > 0 ? part[t - 1] : 0u;
+  for (int i = b; i < e; ++i) {
+    const uint32_t x = a[i];
+    a[i] = run;
+    run += x;
+  }
+  if (t == 1023 && total) *total = static_cast<int32_t>(part[1023]);
+}
+
 // QueryDeviceStackless (xrtailor/src/physics/broad_phase/lbvh/BVH.cu:957-1021): visits the
 // leaves whose boxes contain p (segment == false) or meet segment p->q, in traversal order,
 // calling f(object) for at most kMaxHits of them.

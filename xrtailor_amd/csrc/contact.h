@@ -64,4 +64,63 @@ hipError_t LbvhFindPairsAt(Lbvh& lhs, const Lbvh& rhs, float thickness, int2* pa
                            const int32_t* base, int cap, hipStream_t s);
 hipError_t LaunchAddTotals(hipStream_t s, const int32_t* a, const int32_t* b, int32_t* out);
 
+// ----------------------------------------------------------------- all instances at once
+// Device view of one instance's contact state for the multi-instance kernels.  Every
+// pointer refers to that instance's own buffers; cloth ids are instance-local and the
+// cloth state arrays passed to the launchers are offset by vbeg.
+struct InstContactDev {
+  int vbeg = 0, nv = 0, nf = 0, ne = 0;
+  int has_obs = 0;  // obstacle mesh present
+  int obs_nv = 0, obs_nf = 0;
+  int pair_cap = 0;
+  MeshTopo cloth, obs;
+  BvhView cbvh, obvh;
+  float4* o_x0 = nullptr;
+  float4* o_x = nullptr;
+  float4* o_x1 = nullptr;
+  float4* o_n1 = nullptr;
+  const int32_t* o_vptr = nullptr;
+  const int32_t* o_corner = nullptr;
+  int32_t* vf_face = nullptr;
+  int32_t* rt_face = nullptr;
+  float4* vf_q = nullptr;
+  float4* rt_q = nullptr;
+  EeRec* ee = nullptr;
+  long long* dq = nullptr;
+  int32_t* cnt = nullptr;
+  int2* pairs = nullptr;
+  int32_t* scal = nullptr;
+};
+
+// block -> instance maps: per kind, the prefix sums (n + 1 entries, device) of the
+// 256-thread blocks each instance needs, and their totals (host)
+struct MultiMaps {
+  enum Kind { kVert, kEdge, kFace, kObsVert, kObsFace, kPair, kKinds };
+  const int32_t* off[kKinds] = {};
+  int blocks[kKinds] = {};
+  int n = 0;
+};
+
+// collision pass: obstacle x1 = lerp(x0, x, alpha), midstep normals, RefitMidstep(tol)
+hipError_t MultiObstacleRefit(hipStream_t s, const InstContactDev* ic, const MultiMaps& m,
+                              float alpha, float tol);
+// collision pass: VF-DCD, VF-CCD and EE-DCD records of every instance with an obstacle
+hipError_t MultiGenerate(hipStream_t s, const InstContactDev* ic, const MultiMaps& m,
+                         const float4* x0, const float4* x);
+// pair refresh: cloth LBVH Update(x, x0, ccd), Traverse(cloth, obstacle) ++
+// Traverse(cloth, cloth) into each instance's pair list (scal[1] obstacle pairs, scal[2]
+// total)
+hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& m,
+                      const float4* x, const float4* x0, float thickness);
+// PBD repulsion over every instance's pairs, applied x relax
+hipError_t MultiRepulsion(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
+                          const float4* x0, float h, float relax);
+// VF-CCD (offset 0) then VF-DCD (offset 1e-3) solves
+hipError_t MultiVfSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x);
+// EE-DCD deltas applied x relax
+hipError_t MultiEeSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
+                        float relax);
+// frame end: obstacle x0 = x
+hipError_t MultiObstacleFrameEnd(hipStream_t s, const InstContactDev* ic, const MultiMaps& m);
+
 }  // namespace xrt

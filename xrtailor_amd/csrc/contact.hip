@@ -123,25 +123,23 @@ __device__ __forceinline__ void AddFixed(long long* dq, int32_t* cnt, int i, F3 
   atomicAdd(cnt + i, 1);
 }
 
-// ------------------------------------------------------------------ kernels
+// ------------------------------------------------------------------ bodies
+// Each body handles one vertex / edge / pair; the single-instance kernels below and the
+// multi-instance kernels (one launch for every instance) call the same body.
+
 // PhysicsMesh::Interpolate (xrtailor/src/physics/PhysicsMesh.cu:610-620)
-__global__ __launch_bounds__(kBlock) void InterpolateKernel(const float4* __restrict__ x0,
-                                                            const float4* __restrict__ x,
-                                                            float4* __restrict__ x1, int n,
-                                                            float alpha) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void InterpolateBody(const float4* __restrict__ x0,
+                                                const float4* __restrict__ x,
+                                                float4* __restrict__ x1, int i, float alpha) {
   const float4 a = x0[i], b = x[i];
   const float r = 1.0f - alpha;
   x1[i] = make_float4(r * a.x + alpha * b.x, r * a.y + alpha * b.y, r * a.z + alpha * b.z, a.w);
 }
 
 // GenerateVF_DCD_Kernel (PBDCollision.cu:17-43)
-__global__ __launch_bounds__(kBlock) void VfDcdGenerateKernel(const float4* __restrict__ x, int n,
-                                                              BvhView t, int32_t* __restrict__ face,
-                                                              float4* __restrict__ qout) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void VfDcdGenerateBody(const float4* __restrict__ x, int i,
+                                                  const BvhView& t, int32_t* __restrict__ face,
+                                                  float4* __restrict__ qout) {
   const float4 p4 = x[i];
   const F3 p = L3(p4);
   float best = kScalarMax;
@@ -158,12 +156,10 @@ __global__ __launch_bounds__(kBlock) void VfDcdGenerateKernel(const float4* __re
 }
 
 // GenerateVFCCD_Kernel (PBDCollision.cu:236-270)
-__global__ __launch_bounds__(kBlock) void VfCcdGenerateKernel(const float4* __restrict__ x0,
-                                                              const float4* __restrict__ x, int n,
-                                                              BvhView t, int32_t* __restrict__ face,
-                                                              float4* __restrict__ qout) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void VfCcdGenerateBody(const float4* __restrict__ x0,
+                                                  const float4* __restrict__ x, int i,
+                                                  const BvhView& t, int32_t* __restrict__ face,
+                                                  float4* __restrict__ qout) {
   face[i] = -1;
   qout[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float4 a4 = x0[i], b4 = x[i];
@@ -186,13 +182,11 @@ __global__ __launch_bounds__(kBlock) void VfCcdGenerateKernel(const float4* __re
 }
 
 // GenerateEE_DCD_Kernel (PBDCollision.cu:100-150)
-__global__ __launch_bounds__(kBlock) void EeDcdGenerateKernel(const float4* __restrict__ x,
-                                                              const int2* __restrict__ edges, int ne,
-                                                              BvhView t,
-                                                              const uint32_t* __restrict__ rtri,
-                                                              EeRec* __restrict__ rec) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
+__device__ __forceinline__ void EeDcdGenerateBody(const float4* __restrict__ x,
+                                                  const int2* __restrict__ edges, int e,
+                                                  const BvhView& t,
+                                                  const uint32_t* __restrict__ rtri,
+                                                  EeRec* __restrict__ rec) {
   const int2 ed = edges[e];
   const float4 pc4 = x[ed.x], qc4 = x[ed.y];
   const F3 pc = L3(pc4), qc = L3(qc4);
@@ -222,11 +216,9 @@ __global__ __launch_bounds__(kBlock) void EeDcdGenerateKernel(const float4* __re
 }
 
 // SolveVFDCD_Kernel / SolveVFCCD_Kernel (PBDCollision.cu:54-76,280-300)
-__global__ __launch_bounds__(kBlock) void VfSolveKernel(float4* __restrict__ x, int n, BvhView t,
-                                                        const int32_t* __restrict__ face,
-                                                        const float4* __restrict__ q, float offset) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void VfSolveBody(float4* __restrict__ x, int i, const BvhView& t,
+                                            const int32_t* __restrict__ face,
+                                            const float4* __restrict__ q, float offset) {
   const int f = face[i];
   if (f < 0) return;
   const int4 tr = t.tris[f];
@@ -242,13 +234,10 @@ __global__ __launch_bounds__(kBlock) void VfSolveKernel(float4* __restrict__ x, 
 }
 
 // SolveEEDCD_Kernel (PBDCollision.cu:155-214)
-__global__ __launch_bounds__(kBlock) void EeSolveKernel(const float4* __restrict__ x,
-                                                        const int2* __restrict__ edges, int ne,
-                                                        const EeRec* __restrict__ rec,
-                                                        long long* __restrict__ dq,
-                                                        int32_t* __restrict__ cnt) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
+__device__ __forceinline__ void EeSolveBody(const float4* __restrict__ x,
+                                            const int2* __restrict__ edges, int e,
+                                            const EeRec* __restrict__ rec,
+                                            long long* __restrict__ dq, int32_t* __restrict__ cnt) {
   const EeRec r = rec[e];
   if (r.a.x > 1e6f) return;
   const int2 ed = edges[e];
@@ -265,11 +254,9 @@ __global__ __launch_bounds__(kBlock) void EeSolveKernel(const float4* __restrict
 }
 
 // ApplyDeltas_Kernel / ApplyPBDImpulse_Kernel: x += delta / count * relax, then reset
-__global__ __launch_bounds__(kBlock) void ApplyFixedKernel(float4* __restrict__ x, int n,
-                                                           long long* __restrict__ dq,
-                                                           int32_t* __restrict__ cnt, float relax) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void ApplyFixedBody(float4* __restrict__ x, int i,
+                                               long long* __restrict__ dq,
+                                               int32_t* __restrict__ cnt, float relax) {
   const int c = cnt[i];
   if (c <= 0) return;
   const double sc = static_cast<double>(relax) / (static_cast<double>(c) * 4294967296.0);
@@ -395,8 +382,86 @@ __device__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self
   Accumulate(ms, ids, corr, dq, cnt);
 }
 
-// EvaluateRep_Kernel (PBDRepulsion.cu:470-501) as a grid-stride loop over the device-side
-// pair count
+// EvaluateRep_Kernel body (PBDRepulsion.cu:470-501) for pair p: cloth face pr.x against
+// obstacle face pr.y (p < nobs) or cloth face pr.y
+__device__ __forceinline__ void RepulsionPairBody(const RMesh& c, const RMesh& o, int2 pr,
+                                                  bool self, float h, long long* __restrict__ dq,
+                                                  int32_t* __restrict__ cnt) {
+  const RMesh& other = self ? c : o;
+  if (pr.x < 0 || pr.x >= c.t.nf || pr.y < 0 || pr.y >= other.t.nf) return;
+  const uint32_t r0 = c.t.rtri[pr.x], r1 = other.t.rtri[pr.y];
+  const int4 f0 = c.t.tris[pr.x];
+  const int v0[3] = {f0.x, f0.y, f0.z};
+  for (int i = 0; i < 3; ++i)
+    if ((r0 >> i) & 1u) EvalVF(c, v0[i], other, pr.y, self, h, dq, cnt);
+  const int4 fe0 = c.t.face_edges[pr.x], fe1 = other.t.face_edges[pr.y];
+  const int e0[3] = {fe0.x, fe0.y, fe0.z}, e1[3] = {fe1.x, fe1.y, fe1.z};
+  for (int i = 0; i < 3; ++i) {
+    if (!((r0 >> (3 + i)) & 1u)) continue;
+    for (int j = 0; j < 3; ++j) {
+      if (!((r1 >> (3 + j)) & 1u)) continue;
+      EvalEE(c, e0[i], other, e1[j], self, h, dq, cnt);
+    }
+  }
+}
+
+// ------------------------------------------------------- single-instance kernels
+__global__ __launch_bounds__(kBlock) void InterpolateKernel(const float4* __restrict__ x0,
+                                                            const float4* __restrict__ x,
+                                                            float4* __restrict__ x1, int n,
+                                                            float alpha) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) InterpolateBody(x0, x, x1, i, alpha);
+}
+
+__global__ __launch_bounds__(kBlock) void VfDcdGenerateKernel(const float4* __restrict__ x, int n,
+                                                              BvhView t, int32_t* __restrict__ face,
+                                                              float4* __restrict__ qout) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) VfDcdGenerateBody(x, i, t, face, qout);
+}
+
+__global__ __launch_bounds__(kBlock) void VfCcdGenerateKernel(const float4* __restrict__ x0,
+                                                              const float4* __restrict__ x, int n,
+                                                              BvhView t, int32_t* __restrict__ face,
+                                                              float4* __restrict__ qout) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) VfCcdGenerateBody(x0, x, i, t, face, qout);
+}
+
+__global__ __launch_bounds__(kBlock) void EeDcdGenerateKernel(const float4* __restrict__ x,
+                                                              const int2* __restrict__ edges, int ne,
+                                                              BvhView t,
+                                                              const uint32_t* __restrict__ rtri,
+                                                              EeRec* __restrict__ rec) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < ne) EeDcdGenerateBody(x, edges, e, t, rtri, rec);
+}
+
+__global__ __launch_bounds__(kBlock) void VfSolveKernel(float4* __restrict__ x, int n, BvhView t,
+                                                        const int32_t* __restrict__ face,
+                                                        const float4* __restrict__ q, float offset) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) VfSolveBody(x, i, t, face, q, offset);
+}
+
+__global__ __launch_bounds__(kBlock) void EeSolveKernel(const float4* __restrict__ x,
+                                                        const int2* __restrict__ edges, int ne,
+                                                        const EeRec* __restrict__ rec,
+                                                        long long* __restrict__ dq,
+                                                        int32_t* __restrict__ cnt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < ne) EeSolveBody(x, edges, e, rec, dq, cnt);
+}
+
+__global__ __launch_bounds__(kBlock) void ApplyFixedKernel(float4* __restrict__ x, int n,
+                                                           long long* __restrict__ dq,
+                                                           int32_t* __restrict__ cnt, float relax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ApplyFixedBody(x, i, dq, cnt, relax);
+}
+
+// a grid-stride loop over the device-side pair count
 __global__ __launch_bounds__(kBlock) void RepulsionKernel(RMesh c, RMesh o,
                                                           const int2* __restrict__ pairs,
                                                           const int32_t* __restrict__ n_obs,
@@ -406,26 +471,8 @@ __global__ __launch_bounds__(kBlock) void RepulsionKernel(RMesh c, RMesh o,
                                                           int32_t* __restrict__ cnt) {
   const int total = min(*n_total, cap);
   const int nobs = min(*n_obs, total);
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
-    const int2 pr = pairs[p];
-    const bool self = p >= nobs;
-    const RMesh& other = self ? c : o;
-    if (pr.x < 0 || pr.x >= c.t.nf || pr.y < 0 || pr.y >= other.t.nf) continue;
-    const uint32_t r0 = c.t.rtri[pr.x], r1 = other.t.rtri[pr.y];
-    const int4 f0 = c.t.tris[pr.x];
-    const int v0[3] = {f0.x, f0.y, f0.z};
-    for (int i = 0; i < 3; ++i)
-      if ((r0 >> i) & 1u) EvalVF(c, v0[i], other, pr.y, self, h, dq, cnt);
-    const int4 fe0 = c.t.face_edges[pr.x], fe1 = other.t.face_edges[pr.y];
-    const int e0[3] = {fe0.x, fe0.y, fe0.z}, e1[3] = {fe1.x, fe1.y, fe1.z};
-    for (int i = 0; i < 3; ++i) {
-      if (!((r0 >> (3 + i)) & 1u)) continue;
-      for (int j = 0; j < 3; ++j) {
-        if (!((r1 >> (3 + j)) & 1u)) continue;
-        EvalEE(c, e0[i], other, e1[j], self, h, dq, cnt);
-      }
-    }
-  }
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x)
+    RepulsionPairBody(c, o, pairs[p], p >= nobs, h, dq, cnt);
 }
 
 __global__ __launch_bounds__(kBlock) void FindPairsAtKernel(BvhView l, BvhView r, float th,
@@ -435,31 +482,183 @@ __global__ __launch_bounds__(kBlock) void FindPairsAtKernel(BvhView l, BvhView r
                                                             int2* __restrict__ pairs, int cap) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= l.nf) return;
-  const Box lb = l.box[l.nf - 1 + id];
-  const int lobj = l.nodes[l.nf - 1 + id].w;
-  const int64_t lo = static_cast<int64_t>(*base) + off[id];
-  int64_t index = lo + cnt[id] - 1;
-  int node = 0;
-  const int max_steps = 2 * r.nf;
-  for (int step = 0; node != -1 && step < max_steps; ++step) {
-    const int4 nd = r.nodes[node];
-    if (OverlapDilated(r.box[node], lb, th)) {
-      if (nd.w >= 0) {
-        if (index >= lo && index < cap) pairs[index] = make_int2(lobj, nd.w);
-        --index;
-        node = r.escape[node];
-      } else {
-        node = nd.y;
-      }
-    } else {
-      node = r.escape[node];
-    }
-  }
+  FindPairsBody(l, r, th, id, static_cast<int64_t>(*base) + off[id], cnt[id], pairs, cap);
 }
 
 __global__ void AddTotalsKernel(const int32_t* a, const int32_t* b, int32_t* out) {
   if (threadIdx.x == 0) *out = *a + *b;
 }
+
+// -------------------------------------------------------- multi-instance kernels
+// blockIdx.x -> (instance, block within the instance) through a prefix table of block
+// counts (instances that need no block are skipped by the search)
+__device__ __forceinline__ int InstOfBlock(const int32_t* __restrict__ off, int n, int b) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+#define XRT_MULTI_INDEX(kind, count_expr)                                  \
+  const int k = InstOfBlock(off, n, blockIdx.x);                           \
+  const InstContactDev& c = ic[k];                                         \
+  const int i = (blockIdx.x - off[k]) * kBlock + threadIdx.x;              \
+  if (i >= (count_expr)) return;
+
+__global__ __launch_bounds__(kBlock) void MultiInterpolateKernel(const InstContactDev* ic,
+                                                                 const int32_t* off, int n,
+                                                                 float alpha) {
+  XRT_MULTI_INDEX(kObsVert, c.obs_nv)
+  InterpolateBody(c.o_x0, c.o_x, c.o_x1, i, alpha);
+}
+
+// midstep normals and the contact points pred = x1 + tol n1
+__global__ __launch_bounds__(kBlock) void MultiNormalPredKernel(const InstContactDev* ic,
+                                                                const int32_t* off, int n,
+                                                                float tol) {
+  XRT_MULTI_INDEX(kObsVert, c.obs_nv)
+  c.o_n1[i] = VertexNormalBody(c.o_x1, c.obs.tris, c.o_vptr, c.o_corner, i);
+  RefitPredBody(c.o_x1, c.o_n1, tol, i, c.obvh.pred);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiObsGrowKernel(const InstContactDev* ic,
+                                                             const int32_t* off, int n) {
+  XRT_MULTI_INDEX(kObsFace, c.obs_nf)
+  GrowLeafBody(c.obvh.nodes, c.obvh.tris, c.obvh.pred, nullptr, c.obs_nf, i, c.obvh.box,
+               c.obvh.flags);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiObsBottomUpKernel(const InstContactDev* ic,
+                                                                 const int32_t* off, int n) {
+  XRT_MULTI_INDEX(kObsFace, c.obs_nf)
+  if (c.obs_nf > 1) BottomUpBody(c.obvh.nodes, c.obvh.box, c.obvh.flags, c.obs_nf, 0, i);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiVfGenerateKernel(const InstContactDev* ic,
+                                                                const int32_t* off, int n,
+                                                                const float4* __restrict__ x0,
+                                                                const float4* __restrict__ x) {
+  XRT_MULTI_INDEX(kVert, c.nv)
+  if (!c.has_obs) return;
+  VfDcdGenerateBody(x + c.vbeg, i, c.obvh, c.vf_face, c.vf_q);
+  VfCcdGenerateBody(x0 + c.vbeg, x + c.vbeg, i, c.obvh, c.rt_face, c.rt_q);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiEeGenerateKernel(const InstContactDev* ic,
+                                                                const int32_t* off, int n,
+                                                                const float4* __restrict__ x) {
+  XRT_MULTI_INDEX(kEdge, c.ne)
+  if (!c.has_obs) return;
+  EeDcdGenerateBody(x + c.vbeg, c.cloth.edges, i, c.obvh, c.obs.rtri, c.ee);
+}
+
+// cloth LBVH Update(ccd): pred = x, then leaves grow by bounds(x, x0)
+__global__ __launch_bounds__(kBlock) void MultiClothPredKernel(const InstContactDev* ic,
+                                                               const int32_t* off, int n,
+                                                               const float4* __restrict__ x) {
+  XRT_MULTI_INDEX(kVert, c.nv)
+  RefitPredBody(x + c.vbeg, nullptr, 0.0f, i, c.cbvh.pred);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiClothGrowKernel(const InstContactDev* ic,
+                                                               const int32_t* off, int n,
+                                                               const float4* __restrict__ x,
+                                                               const float4* __restrict__ x0) {
+  XRT_MULTI_INDEX(kFace, c.nf)
+  GrowLeafBody(c.cbvh.nodes, c.cbvh.tris, x + c.vbeg, x0 + c.vbeg, c.nf, i, c.cbvh.box,
+               c.cbvh.flags);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiClothBottomUpKernel(const InstContactDev* ic,
+                                                                   const int32_t* off, int n) {
+  XRT_MULTI_INDEX(kFace, c.nf)
+  if (c.nf > 1) BottomUpBody(c.cbvh.nodes, c.cbvh.box, c.cbvh.flags, c.nf, 1, i);
+}
+
+// pairs per cloth leaf against the obstacle (self = 0) or the cloth itself (self = 1)
+__global__ __launch_bounds__(kBlock) void MultiCountPairsKernel(const InstContactDev* ic,
+                                                                const int32_t* off, int n,
+                                                                float th, int self) {
+  XRT_MULTI_INDEX(kFace, c.nf)
+  uint32_t v = 0;
+  if (self) v = CountPairsBody(c.cbvh, c.cbvh, th, i);
+  else if (c.has_obs) v = CountPairsBody(c.cbvh, c.obvh, th, i);
+  c.cbvh.counts[i] = v;
+  c.cbvh.offsets[i] = v;
+}
+
+// one workgroup per instance: exclusive scan of the per-leaf counts; the sum goes to
+// scal[1] (obstacle pass) or to the tree's total (self pass)
+__global__ __launch_bounds__(1024) void MultiScanKernel(const InstContactDev* ic, int n, int self) {
+  const InstContactDev& c = ic[blockIdx.x];
+  if (c.nf <= 0 || !c.scal) return;
+  BlockExclusiveScan(c.cbvh.offsets, c.nf, self ? c.cbvh.total : c.scal + 1);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiFindPairsKernel(const InstContactDev* ic,
+                                                               const int32_t* off, int n, float th,
+                                                               int self) {
+  XRT_MULTI_INDEX(kFace, c.nf)
+  if (!self && !c.has_obs) return;
+  const int64_t lo = static_cast<int64_t>(c.scal[self ? 1 : 0]) + c.cbvh.offsets[i];
+  FindPairsBody(c.cbvh, self ? c.cbvh : c.obvh, th, i, lo, c.cbvh.counts[i], c.pairs, c.pair_cap);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiAddTotalsKernel(const InstContactDev* ic, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || !ic[k].scal) return;
+  ic[k].scal[2] = ic[k].scal[1] + *ic[k].cbvh.total;
+}
+
+// each instance's blocks stride over its own device-side pair count
+__global__ __launch_bounds__(kBlock) void MultiRepulsionKernel(const InstContactDev* ic,
+                                                               const int32_t* off, int n,
+                                                               const float4* __restrict__ x,
+                                                               const float4* __restrict__ x0,
+                                                               float h) {
+  const int k = InstOfBlock(off, n, blockIdx.x);
+  const InstContactDev& c = ic[k];
+  const int nblk = off[k + 1] - off[k];
+  const int total = min(c.scal[2], c.pair_cap);
+  const int nobs = min(c.scal[1], total);
+  const RMesh cm{c.cloth, MeshState{x + c.vbeg, x0 + c.vbeg, -1.0f}};
+  const RMesh om{c.obs, MeshState{c.o_x, c.o_x0, 1e-3f}};  // obstacle nodes: inv mass 1e-3
+  for (int p = (blockIdx.x - off[k]) * kBlock + threadIdx.x; p < total; p += nblk * kBlock)
+    RepulsionPairBody(cm, om, c.pairs[p], p >= nobs, h, c.dq, c.cnt);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiApplyKernel(const InstContactDev* ic,
+                                                           const int32_t* off, int n,
+                                                           float4* __restrict__ x, float relax) {
+  XRT_MULTI_INDEX(kVert, c.nv)
+  ApplyFixedBody(x + c.vbeg, i, c.dq, c.cnt, relax);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiVfSolveKernel(const InstContactDev* ic,
+                                                             const int32_t* off, int n,
+                                                             float4* __restrict__ x) {
+  XRT_MULTI_INDEX(kVert, c.nv)
+  if (!c.has_obs) return;
+  VfSolveBody(x + c.vbeg, i, c.obvh, c.rt_face, c.rt_q, 0.0f);
+  VfSolveBody(x + c.vbeg, i, c.obvh, c.vf_face, c.vf_q, 1e-3f);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiEeSolveKernel(const InstContactDev* ic,
+                                                             const int32_t* off, int n,
+                                                             const float4* __restrict__ x) {
+  XRT_MULTI_INDEX(kEdge, c.ne)
+  if (!c.has_obs) return;
+  EeSolveBody(x + c.vbeg, c.cloth.edges, i, c.ee, c.dq, c.cnt);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiObsCopyKernel(const InstContactDev* ic,
+                                                             const int32_t* off, int n) {
+  XRT_MULTI_INDEX(kObsVert, c.obs_nv)
+  c.o_x0[i] = c.o_x[i];
+}
+#undef XRT_MULTI_INDEX
 }  // namespace
 
 hipError_t LaunchInterpolate(hipStream_t s, const float4* x0, const float4* x, float4* x1, int n,
@@ -524,6 +723,76 @@ hipError_t LbvhFindPairsAt(Lbvh& lhs, const Lbvh& rhs, float thickness, int2* pa
 }
 hipError_t LaunchAddTotals(hipStream_t s, const int32_t* a, const int32_t* b, int32_t* out) {
   AddTotalsKernel<<<1, 64, 0, s>>>(a, b, out);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ multi-instance launchers
+namespace {
+using MK = MultiMaps::Kind;
+inline bool Empty(const MultiMaps& m, MK k) { return m.n <= 0 || m.blocks[k] <= 0; }
+}  // namespace
+
+hipError_t MultiObstacleRefit(hipStream_t s, const InstContactDev* ic, const MultiMaps& m,
+                              float alpha, float tol) {
+  if (Empty(m, MK::kObsVert)) return hipSuccess;
+  MultiInterpolateKernel<<<m.blocks[MK::kObsVert], kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n, alpha);
+  MultiNormalPredKernel<<<m.blocks[MK::kObsVert], kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n, tol);
+  MultiObsGrowKernel<<<m.blocks[MK::kObsFace], kBlock, 0, s>>>(ic, m.off[MK::kObsFace], m.n);
+  MultiObsBottomUpKernel<<<m.blocks[MK::kObsFace], kBlock, 0, s>>>(ic, m.off[MK::kObsFace], m.n);
+  return hipGetLastError();
+}
+
+hipError_t MultiGenerate(hipStream_t s, const InstContactDev* ic, const MultiMaps& m,
+                         const float4* x0, const float4* x) {
+  if (Empty(m, MK::kVert)) return hipSuccess;
+  MultiVfGenerateKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x0, x);
+  if (!Empty(m, MK::kEdge))
+    MultiEeGenerateKernel<<<m.blocks[MK::kEdge], kBlock, 0, s>>>(ic, m.off[MK::kEdge], m.n, x);
+  return hipGetLastError();
+}
+
+hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& m,
+                      const float4* x, const float4* x0, float thickness) {
+  if (Empty(m, MK::kFace)) return hipSuccess;
+  const int fb = m.blocks[MK::kFace];
+  const int32_t* fo = m.off[MK::kFace];
+  MultiClothPredKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x);
+  MultiClothGrowKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, x, x0);
+  MultiClothBottomUpKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n);
+  for (int self = 0; self < 2; ++self) {
+    MultiCountPairsKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
+    MultiScanKernel<<<m.n, 1024, 0, s>>>(ic, m.n, self);
+    MultiFindPairsKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
+  }
+  MultiAddTotalsKernel<<<Blocks(m.n), kBlock, 0, s>>>(ic, m.n);
+  return hipGetLastError();
+}
+
+hipError_t MultiRepulsion(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
+                          const float4* x0, float h, float relax) {
+  if (Empty(m, MK::kPair)) return hipSuccess;
+  MultiRepulsionKernel<<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
+  MultiApplyKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, relax);
+  return hipGetLastError();
+}
+
+hipError_t MultiVfSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x) {
+  if (Empty(m, MK::kVert)) return hipSuccess;
+  MultiVfSolveKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x);
+  return hipGetLastError();
+}
+
+hipError_t MultiEeSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
+                        float relax) {
+  if (Empty(m, MK::kEdge)) return hipSuccess;
+  MultiEeSolveKernel<<<m.blocks[MK::kEdge], kBlock, 0, s>>>(ic, m.off[MK::kEdge], m.n, x);
+  MultiApplyKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, relax);
+  return hipGetLastError();
+}
+
+hipError_t MultiObstacleFrameEnd(hipStream_t s, const InstContactDev* ic, const MultiMaps& m) {
+  if (Empty(m, MK::kObsVert)) return hipSuccess;
+  MultiObsCopyKernel<<<m.blocks[MK::kObsVert], kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n);
   return hipGetLastError();
 }
 

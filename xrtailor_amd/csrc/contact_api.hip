@@ -149,7 +149,7 @@ int ContactStart(xrt_ctx* ctx) {
     ctx->contact[ii] = std::move(c);
   }
   XRT_CHECK_HIP(hipStreamSynchronize(s));
-  return XRT_OK;
+  return MultiContactSetup(ctx);
 }
 
 // Contact phases of one instance's substep, shared by the plain driver below and the
@@ -258,6 +258,154 @@ int ContactSubstepEnd(xrt_ctx* ctx, int inst) {
       XRT_CHECK_HIP(hipMemcpyAsync(c.o_x0.ptr, c.o_x.ptr, sizeof(float4) * c.obs_nv,
                                    hipMemcpyDeviceToDevice, ctx->stream));
   }
+  return XRT_OK;
+}
+
+// ------------------------------------------------------------ every instance at once
+bool ContactsActive(const xrt_ctx* ctx) {
+  if (ctx->params.enable_self_contact) return true;
+  if (!ctx->params.enable_obstacle_contact) return false;
+  for (const auto& c : ctx->contact)
+    if (c && c->obs_nf > 0) return true;
+  return false;
+}
+
+// device views of every instance's contact state and the block -> instance maps
+int MultiContactSetup(xrt_ctx* ctx) {
+  const int n = static_cast<int>(ctx->inst.size());
+  std::vector<InstContactDev> ic(std::max(n, 1));
+  std::vector<int32_t> off(MultiMaps::kKinds * static_cast<size_t>(n + 1), 0);
+  auto blocks = [](int count) { return (count + 255) / 256; };
+  for (int k = 0; k < n; ++k) {
+    int cnt[MultiMaps::kKinds] = {0, 0, 0, 0, 0, 0};
+    if (ctx->contact[k]) {
+      HostInstance& h = ctx->inst[k];
+      ContactData& c = *ctx->contact[k];
+      InstContactDev& d = ic[k];
+      d.vbeg = h.desc.vbeg;
+      d.nv = h.nv;
+      d.nf = h.nt;
+      d.ne = h.ne();
+      d.has_obs = c.obs_nf > 0;
+      d.obs_nv = c.obs_nv;
+      d.obs_nf = c.obs_nf;
+      d.pair_cap = c.pair_cap;
+      d.cloth = ClothTopo(c, h);
+      d.obs = ObsTopo(c);
+      d.cbvh = c.cloth_bvh.view();
+      d.obvh = c.obs_bvh.view();
+      d.o_x0 = c.o_x0.ptr;
+      d.o_x = c.o_x.ptr;
+      d.o_x1 = c.o_x1.ptr;
+      d.o_n1 = c.o_n1.ptr;
+      d.o_vptr = c.o_vptr.ptr;
+      d.o_corner = c.o_corner.ptr;
+      d.vf_face = c.vf_face.ptr;
+      d.rt_face = c.rt_face.ptr;
+      d.vf_q = c.vf_q.ptr;
+      d.rt_q = c.rt_q.ptr;
+      d.ee = reinterpret_cast<EeRec*>(c.ee.ptr);
+      d.dq = c.dq.ptr;
+      d.cnt = c.cnt.ptr;
+      d.pairs = c.pairs.ptr;
+      d.scal = c.scal.ptr;
+      cnt[MultiMaps::kVert] = blocks(h.nv);
+      cnt[MultiMaps::kEdge] = blocks(h.ne());
+      cnt[MultiMaps::kFace] = blocks(h.nt);
+      cnt[MultiMaps::kObsVert] = blocks(c.obs_nv);
+      cnt[MultiMaps::kObsFace] = blocks(c.obs_nf);
+      // repulsion: about 4 pairs per thread at the usual ~16 pairs per face
+      cnt[MultiMaps::kPair] = std::max(1, (16 * h.nt + 1023) / 1024);
+    }
+    for (int kind = 0; kind < MultiMaps::kKinds; ++kind) {
+      int32_t* o = off.data() + static_cast<size_t>(kind) * (n + 1);
+      o[k + 1] = o[k] + cnt[kind];
+    }
+  }
+  XRT_CHECK_HIP(ctx->ic.Upload(ic, ctx->stream));
+  XRT_CHECK_HIP(ctx->ic_off.Upload(off, ctx->stream));
+  ctx->maps.n = n;
+  for (int kind = 0; kind < MultiMaps::kKinds; ++kind) {
+    ctx->maps.off[kind] = ctx->ic_off.ptr + static_cast<size_t>(kind) * (n + 1);
+    ctx->maps.blocks[kind] = off[static_cast<size_t>(kind) * (n + 1) + n];
+  }
+  XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return XRT_OK;
+}
+
+namespace {
+ContactData* FirstContact(xrt_ctx* ctx) {
+  for (auto& c : ctx->contact)
+    if (c) return c.get();
+  return nullptr;
+}
+}  // namespace
+
+int MultiContactSubstepBegin(xrt_ctx* ctx) {
+  for (size_t i = 0; i < ctx->contact.size(); ++i)
+    if (ctx->contact[i]) {
+      const int rc = ContactSubstepBegin(ctx, static_cast<int>(i));
+      if (rc != XRT_OK) return rc;
+    }
+  return XRT_OK;
+}
+
+int MultiContactIterPre(xrt_ctx* ctx, int it, bool* pass_ran) {
+  if (pass_ran) *pass_ran = false;
+  ContactData* first = FirstContact(ctx);
+  if (!first) return XRT_OK;
+  const xrt_params& p = ctx->params;
+  hipStream_t s = ctx->stream;
+  const int frequency = std::max(1, p.num_substeps * p.num_iterations / p.num_collision_passes);
+  const int interval = std::max(1, p.pair_update_interval);
+  const bool obs = ContactsActive(ctx) && p.enable_obstacle_contact &&
+                   std::any_of(ctx->contact.begin(), ctx->contact.end(),
+                               [](const auto& c) { return c && c->obs_nf > 0; });
+  const int step = first->frame_substep * p.num_iterations + it;
+  if (obs && step % frequency == 0) {
+    int pass = -1;
+    for (auto& c : ctx->contact)
+      if (c && c->obs_nf > 0) pass = c->pass_cnt;
+    const float alpha = (static_cast<float>(pass) + 1.0f) / static_cast<float>(p.num_collision_passes);
+    XRT_CHECK_HIP(MultiObstacleRefit(s, ctx->ic.ptr, ctx->maps, alpha, p.bvh_tolerance));
+    XRT_CHECK_HIP(MultiGenerate(s, ctx->ic.ptr, ctx->maps, ctx->x0.ptr, ctx->x.ptr));
+    for (auto& c : ctx->contact)
+      if (c && c->obs_nf > 0) c->pass_cnt++;
+    if (pass_ran) *pass_ran = true;
+  }
+  if (p.enable_self_contact) {
+    if (it % interval == 0)
+      XRT_CHECK_HIP(MultiPairs(s, ctx->ic.ptr, ctx->maps, ctx->x.ptr, ctx->x0.ptr, kPairThickness));
+    XRT_CHECK_HIP(MultiRepulsion(s, ctx->ic.ptr, ctx->maps, ctx->x.ptr, ctx->x0.ptr, p.pbd_thickness,
+                                 p.repulsion_relaxation));
+  }
+  return XRT_OK;
+}
+
+int MultiContactVfSolve(xrt_ctx* ctx) {
+  if (!ctx->params.enable_obstacle_contact) return XRT_OK;
+  XRT_CHECK_HIP(MultiVfSolve(ctx->stream, ctx->ic.ptr, ctx->maps, ctx->x.ptr));
+  return XRT_OK;
+}
+
+int MultiContactEeSolve(xrt_ctx* ctx) {
+  if (!ctx->params.enable_obstacle_contact) return XRT_OK;
+  XRT_CHECK_HIP(MultiEeSolve(ctx->stream, ctx->ic.ptr, ctx->maps, ctx->x.ptr,
+                             ctx->params.relaxation_factor));
+  return XRT_OK;
+}
+
+int MultiContactSubstepEnd(xrt_ctx* ctx) {
+  bool frame_end = false;
+  for (auto& c : ctx->contact) {
+    if (!c) continue;
+    if (++c->frame_substep >= ctx->params.num_substeps) {
+      c->frame_substep = 0;
+      c->obstacle_moving = false;
+      frame_end = true;
+    }
+  }
+  if (frame_end) XRT_CHECK_HIP(MultiObstacleFrameEnd(ctx->stream, ctx->ic.ptr, ctx->maps));
   return XRT_OK;
 }
 

@@ -10,6 +10,7 @@
 #include "../../include/xrtailor_amd.h"
 #include "bvh.h"
 #include "common.h"
+#include "contact.h"
 #include "device.h"
 #include "topology.h"
 
@@ -82,12 +83,18 @@ struct ContactData {
   bool obstacle_moving = false;  // xrt_update_obstacle since the last frame end
 };
 
+// Sizes of the wide launches (one color of every instance per launch).
+struct WideConfig {
+  int n_total = 0, e_total = 0, b_total = 0;  // vertices / stretch / bend over instances
+  std::vector<int> s_max, b_max;              // largest color c over instances
+};
+
 // Adjoint (differentiable rollout) state: checkpoints, one frame of tape, adjoints.
 struct AdjointData {
   int n_frames = 0, ckpt_every = 1, n_passes = 0, frequency = 1;
   bool recorded = false, has_grad = false, contacts = false;
-  int n_total = 0, e_total = 0, b_total = 0;  // vertices / stretch / bend over instances
-  std::vector<int> s_max, b_max;              // largest color c over instances
+  int n_total = 0, e_total = 0, b_total = 0;  // copies of the context's WideConfig
+  std::vector<int> s_max, b_max;
   // tape of one frame: substep s at s * sub_stride; iteration it at n_total + it * it_stride
   size_t it_stride = 0, sub_stride = 0, off_s = 0, off_b = 0, off_g = 0, off_c = 0, off_e = 0;
   DevBuf<float4> tape;
@@ -138,6 +145,13 @@ struct xrt_ctx {
   int max_slots = 0;
   // per-instance contact state (allocated by xrt_start when contacts can occur)
   std::vector<std::unique_ptr<xrt::ContactData>> contact;
+  // device views of the contact state and the block -> instance maps of the
+  // multi-instance contact kernels
+  xrt::DevBuf<xrt::InstContactDev> ic;
+  xrt::DevBuf<int32_t> ic_off;
+  xrt::MultiMaps maps;
+  // wide (all instances per launch) launch sizes
+  xrt::WideConfig wide;
   // differentiable rollout (allocated by xrt_adjoint_begin)
   std::unique_ptr<xrt::AdjointData> adj;
   // graph of one batched frame
@@ -161,4 +175,15 @@ int ContactIterPre(xrt_ctx* ctx, int inst, int it, bool* pass_ran);
 int ContactVfSolve(xrt_ctx* ctx, int inst);
 int ContactEeSolve(xrt_ctx* ctx, int inst);
 int ContactSubstepEnd(xrt_ctx* ctx, int inst);
+// the same phases for every instance at once (multi-instance kernels)
+bool ContactsActive(const xrt_ctx* ctx);
+int MultiContactSetup(xrt_ctx* ctx);
+int MultiContactSubstepBegin(xrt_ctx* ctx);
+int MultiContactIterPre(xrt_ctx* ctx, int it, bool* pass_ran);
+int MultiContactVfSolve(xrt_ctx* ctx);
+int MultiContactEeSolve(xrt_ctx* ctx);
+int MultiContactSubstepEnd(xrt_ctx* ctx);
+// adjoint_api.hip: one substep of every instance as wide launches with the contact phases
+// above; with a tape owner and sub >= 0 the operators' inputs go to tape slot `sub`
+int WideSubstep(xrt_ctx* ctx, const StepConsts& k, const BendMode& bm, AdjointData* tape, int sub);
 }  // namespace xrt
