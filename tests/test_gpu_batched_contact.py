@@ -39,12 +39,14 @@ def _pair(specs, xp, seed=3):
     return a, ia, b, ib
 
 
-def _compare(a, ia, b, ib, what):
+def _compare(a, ia, b, ib, what, specs=None, self_contact=True):
     worst = 0.0
-    for i, j in zip(ia, ib):
+    for k, (i, j) in enumerate(zip(ia, ib)):
         xa, va = a.GetState(i)
         xb, vb = b.GetState(j)
         worst = max(worst, float(np.abs(xa - xb).max()), float(np.abs(va - vb).max()))
+        if not self_contact and specs[k].obstacle is None:
+            continue                          # no contact state without obstacle or self contact
         ca, cb = a.Contacts(i), b.Contacts(j)
         np.testing.assert_array_equal(ca["vf_face"], cb["vf_face"])
         np.testing.assert_array_equal(ca["rt_face"], cb["rt_face"])
@@ -65,9 +67,10 @@ def test_batched_substeps_match_plain(self_contact):
         for j in ib:
             b.SubstepPlain(j)
     a.sync()
-    assert _compare(a, ia, b, ib, "3 substeps") == 0.0
-    hits = sum(int((a.Contacts(i)["vf_face"] >= 0).sum()) for i in ia)
-    pairs = sum(a.Contacts(i)["pairs"] for i in ia)
+    assert _compare(a, ia, b, ib, "3 substeps", specs, bool(self_contact)) == 0.0
+    with_state = [i for i, sp in zip(ia, specs) if self_contact or sp.obstacle is not None]
+    hits = sum(int((a.Contacts(i)["vf_face"] >= 0).sum()) for i in with_state)
+    pairs = sum(a.Contacts(i)["pairs"] for i in with_state)
     print(f"  VF-DCD records {hits}, pairs {pairs}", flush=True)
     assert hits > 0
     assert (pairs > 0) == bool(self_contact)

@@ -109,17 +109,17 @@ __device__ __forceinline__ void SubstepBody(const BatchArgs& a, const InstDesc& 
   const int32_t* sco = a.s_coloff + d.s_cbeg;
   const int32_t* bco = a.b_coloff + d.b_cbeg;
   const int32_t* lptr = a.lra_ptr + d.lra_vbeg;
+  // constraints per thread per round: more gathers in flight where x lives in global memory
+  constexpr int kUs = kLds ? 2 : 4, kUb = kLds ? 1 : 2;
   for (int it = 0; it < k.iterations; ++it) {
     for (int c = 0; c < d.s_ncol; ++c) {
-      const int end = sco[c + 1];
-      for (int e = sco[c] + tid; e < end; e += bs) StretchProject(X, a.sc[e], a.slam + e, k.inv_dt2, it);
+      StretchColor<kUs>(X, a.sc, a.slam, sco[c], sco[c + 1], k.inv_dt2, it, tid, bs);
       __syncthreads();
     }
     if (k.solve_bending) {
       const BendMode m{k.bend_dlo, k.bend_th0, k.bend_th1, k.bend_sgn};
       for (int c = 0; c < d.b_ncol; ++c) {
-        const int end = bco[c + 1];
-        for (int e = bco[c] + tid; e < end; e += bs) BendProject(X, a.bidx[e], a.bpar[e], k.inv_dt2, m);
+        BendColor<kUb>(X, a.bidx, a.bpar, bco[c], bco[c + 1], k.inv_dt2, m, tid, bs);
         __syncthreads();
       }
     }
