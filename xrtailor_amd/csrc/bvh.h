@@ -58,9 +58,11 @@ struct Lbvh {
   hipError_t Setup(const int32_t* tris_host, int nf, int nv, hipStream_t s);
 };
 
-// BVH::UpdateData + BVH::Construct; x0/x/n are float4 per-vertex arrays (n, x may be null)
+// BVH::UpdateData + BVH::Construct; x0/x/n are float4 per-vertex arrays (n, x may be null).
+// merge = false leaves the internal boxes to a later multi-tree merge (the batched frame
+// start merges every instance's trees in one launch).
 hipError_t LbvhBuild(Lbvh& b, const float4* x0, const float4* x, const float4* n, float tol,
-                     int ccd, float scr, hipStream_t s);
+                     int ccd, float scr, hipStream_t s, bool merge = true);
 // BVH::RefitMidstep: pred = x1 + tol n1, leaves grow, parents re-merged
 hipError_t LbvhRefitMidstep(Lbvh& b, const float4* x1, const float4* n1, float tol, hipStream_t s);
 // BVH::Update(ccd): pred = x, leaves grow by bounds(x[, x0]), ancestors grow

@@ -305,6 +305,7 @@ __global__ __launch_bounds__(kBlock) void BottomUpKernel(const int4* __restrict_
   BottomUpBody(nodes, box, flags, nf, grow, i);
 }
 
+
 __global__ __launch_bounds__(kBlock) void ZeroFlagsKernel(int32_t* flags, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) flags[i] = 0;
@@ -455,7 +456,7 @@ hipError_t Lbvh::Setup(const int32_t* tris_host, int nfaces, int nverts, hipStre
 }
 
 hipError_t LbvhBuild(Lbvh& b, const float4* x0, const float4* x, const float4* n, float tol,
-                     int ccd, float scr, hipStream_t s) {
+                     int ccd, float scr, hipStream_t s, bool merge) {
   if (b.nf <= 0) {
     b.active = false;
     return hipSuccess;
@@ -485,7 +486,7 @@ hipError_t LbvhBuild(Lbvh& b, const float4* x0, const float4* x, const float4* n
                                                 b.nodes.ptr, b.box.ptr, b.code.ptr, b.flags.ptr);
   if (nf > 1) {
     KarrasKernel<<<Blocks(nf - 1), kBlock, 0, s>>>(b.code.ptr, nf, b.nodes.ptr);
-    BottomUpKernel<<<Blocks(nf), kBlock, 0, s>>>(b.nodes.ptr, b.box.ptr, b.flags.ptr, nf, 0);
+    if (merge) BottomUpKernel<<<Blocks(nf), kBlock, 0, s>>>(b.nodes.ptr, b.box.ptr, b.flags.ptr, nf, 0);
   }
   EscapeKernel<<<Blocks(2 * nf - 1), kBlock, 0, s>>>(b.nodes.ptr, 2 * nf - 1, b.escape.ptr);
   b.active = true;

@@ -138,6 +138,31 @@ __device__ __forceinline__ void BottomUpBody(const int4* __restrict__ nodes, Box
   }
 }
 
+// The same merge for a whole tree by ONE workgroup: its threads walk leaves tid, tid + bs,
+// ...; the second arriver at a node merges it.  All walkers share the CU, so workgroup-scope
+// ordering suffices (no device-wide fence per level).  A walker never waits, so the
+// sequential leaf loop cannot stall.  Node boxes are unions (min / max), independent of
+// the arrival order, hence identical to BottomUpBody's.
+__device__ __forceinline__ void BottomUpTreeWG(const int4* __restrict__ nodes,
+                                               Box* __restrict__ box,
+                                               int32_t* __restrict__ flags, int nf, int grow) {
+  for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+    int p = nodes[nf - 1 + i].x;
+    for (int level = 0; p >= 0 && p < nf - 1 && level < nf; ++level) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      const int old = __hip_atomic_fetch_add(&flags[p], 1, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0) break;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      const int4 pn = nodes[p];
+      Box m = MergeBox(box[pn.y], box[pn.z]);
+      if (grow) m = MergeBox(m, box[p]);
+      box[p] = m;
+      p = pn.x;
+    }
+  }
+}
+
 // CountPairs_Kernel (BVH.cu:1149-1179): rhs leaves overlapping lhs leaf id dilated by th
 __device__ __forceinline__ uint32_t CountPairsBody(const BvhView& l, const BvhView& r, float th,
                                                    int id) {

@@ -120,6 +120,9 @@ hipError_t MultiVfSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps
 // EE-DCD deltas applied x relax
 hipError_t MultiEeSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
                         float relax);
+// frame start: merge the internal boxes of trees built with merge = false (cloth trees
+// when `cloth`, obstacle trees always)
+hipError_t MultiBuildMerge(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, bool cloth);
 // frame end: obstacle x0 = x
 hipError_t MultiObstacleFrameEnd(hipStream_t s, const InstContactDev* ic, const MultiMaps& m);
 

@@ -100,6 +100,20 @@ __device__ float SignedVF(F3 x, F3 y0, F3 y1, F3 y2, F3& n, float* w) {
 
 // SignedEdgeEdgeDistance (BasicPrimitiveTests.cu:420-436): n is left unnormalized (the
 // reference discards glm::normalize(n))
+// with the unit edge directions ux = Nz(x1 - x0), uy = Nz(y1 - y0) given
+__device__ float SignedEEu(F3 x0, F3 x1, F3 y0, F3 y1, F3 ux, F3 uy, F3& n, float* w) {
+  n = C(ux, uy);
+  if (D(n, n) < 1e-6f) { w[0] = w[1] = w[2] = w[3] = 0; return INFINITY; }
+  const float h = D(S(x0, y0), n);
+  const float a0 = Mixed(S(y1, x1), S(y0, x1), n), a1 = Mixed(S(y0, x0), S(y1, x0), n);
+  const float b0 = Mixed(S(x0, y1), S(x1, y1), n), b1 = Mixed(S(x1, y0), S(x0, y0), n);
+  w[0] = a0 / (a0 + a1);
+  w[1] = a1 / (a0 + a1);
+  w[2] = -b0 / (b0 + b1);
+  w[3] = -b1 / (b0 + b1);
+  return h;
+}
+
 __device__ float SignedEE(F3 x0, F3 x1, F3 y0, F3 y1, F3& n, float* w) {
   n = C(Nz(S(x1, x0)), Nz(S(y1, y0)));
   if (D(n, n) < 1e-6f) { w[0] = w[1] = w[2] = w[3] = 0; return INFINITY; }
@@ -317,14 +331,17 @@ __device__ void EvalVF(const RMesh& c, int node, const RMesh& o, int face, bool 
                        long long* dq, int32_t* cnt) {
   const int4 tr = o.t.tris[face];
   if (self && (node == tr.x || node == tr.y || node == tr.z)) return;
+  // the inside test on the current state rejects most pairs; it is evaluated before the
+  // start-of-substep side (both are pure, so the order does not change any result)
   float w[4];
-  F3 pn;
-  const float d0 = SignedVF(c.X0(node), o.X0(tr.x), o.X0(tr.y), o.X0(tr.z), pn, w);
-  const bool face_free = o.Free(tr.x) && o.Free(tr.y) && o.Free(tr.z);
-  if (d0 < 0.0f && face_free) pn = M(pn, -1.0f);
   F3 fn;
   SignedVF(c.X(node), o.X(tr.x), o.X(tr.y), o.X(tr.z), fn, w);
   if (!(fminf(-w[1], fminf(-w[2], -w[3])) >= 1e-6f)) return;
+  float w0[4];
+  F3 pn;
+  const float d0 = SignedVF(c.X0(node), o.X0(tr.x), o.X0(tr.y), o.X0(tr.z), pn, w0);
+  const bool face_free = o.Free(tr.x) && o.Free(tr.y) && o.Free(tr.z);
+  if (d0 < 0.0f && face_free) pn = M(pn, -1.0f);
   const F3 p1 = o.X(tr.x);
   F3 n = FaceNormal(p1, o.X(tr.y), o.X(tr.z));
   const float side = D(n, pn) > 0.0f ? 1.0f : -1.0f;
@@ -349,21 +366,23 @@ __device__ void EvalVF(const RMesh& c, int node, const RMesh& o, int face, bool 
 }
 
 // EvaluateEEProximity (PBDRepulsion.cu:411-468)
-__device__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self, float h,
-                       long long* dq, int32_t* cnt) {
+// ua, ub: unit directions of the two edges in the current state, Nz(x_a.y - x_a.x)
+__device__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self, float h, F3 ua,
+                       F3 ub, long long* dq, int32_t* cnt) {
   const int2 a = c.t.edges[e0], b = o.t.edges[e1];
   if (self && (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y)) return;
+  // current-state tests first, then the start-of-substep side (pure; same results)
+  float bw[4];
+  F3 ncur;
+  SignedEEu(c.X(a.x), c.X(a.y), o.X(b.x), o.X(b.y), ua, ub, ncur, bw);
+  const float mn = fminf(fminf(bw[0], bw[1]), fminf(-bw[2], -bw[3]));
+  if (!(mn >= 1e-6f && InEdge(bw[1], c, e0, o, e1) && InEdge(-bw[3], o, e1, c, e0))) return;
   float w[4];
   F3 pn;
   const float d0 = SignedEE(c.X0(a.x), c.X0(a.y), o.X0(b.x), o.X0(b.y), pn, w);
   const bool e1_free = o.Free(b.x) && o.Free(b.y);
   if (!e1_free) pn = EdgeNormal(o, e1);
   else if (d0 < 0.0f) pn = M(pn, -1.0f);
-  float bw[4];
-  F3 ncur;
-  SignedEE(c.X(a.x), c.X(a.y), o.X(b.x), o.X(b.y), ncur, bw);
-  const float mn = fminf(fminf(bw[0], bw[1]), fminf(-bw[2], -bw[3]));
-  if (!(mn >= 1e-6f && InEdge(bw[1], c, e0, o, e1) && InEdge(-bw[3], o, e1, c, e0))) return;
   const F3 dd = A(A(M(c.X(a.x), bw[0]), M(c.X(a.y), bw[1])), A(M(o.X(b.x), bw[2]), M(o.X(b.y), bw[3])));
   const float Cv = D(dd, pn) - h;
   if (Cv > 0) return;
@@ -394,13 +413,25 @@ __device__ __forceinline__ void RepulsionPairBody(const RMesh& c, const RMesh& o
   const int v0[3] = {f0.x, f0.y, f0.z};
   for (int i = 0; i < 3; ++i)
     if ((r0 >> i) & 1u) EvalVF(c, v0[i], other, pr.y, self, h, dq, cnt);
+  if (!((r0 >> 3) & 7u) || !((r1 >> 3) & 7u)) return;  // no owned edge pair
   const int4 fe0 = c.t.face_edges[pr.x], fe1 = other.t.face_edges[pr.y];
   const int e0[3] = {fe0.x, fe0.y, fe0.z}, e1[3] = {fe1.x, fe1.y, fe1.z};
+  // unit directions of the owned edges, computed at most once per pair and only for edge
+  // pairs that share no vertex (the same values EvalEE would compute itself)
+  F3 u0[3], u1[3];
+  bool h0[3] = {false, false, false}, h1[3] = {false, false, false};
+#pragma unroll
   for (int i = 0; i < 3; ++i) {
     if (!((r0 >> (3 + i)) & 1u)) continue;
+    const int2 ea = c.t.edges[e0[i]];
+#pragma unroll
     for (int j = 0; j < 3; ++j) {
       if (!((r1 >> (3 + j)) & 1u)) continue;
-      EvalEE(c, e0[i], other, e1[j], self, h, dq, cnt);
+      const int2 eb = other.t.edges[e1[j]];
+      if (self && (ea.x == eb.x || ea.x == eb.y || ea.y == eb.x || ea.y == eb.y)) continue;
+      if (!h0[i]) { u0[i] = Nz(S(c.X(ea.y), c.X(ea.x))); h0[i] = true; }
+      if (!h1[j]) { u1[j] = Nz(S(other.X(eb.y), other.X(eb.x))); h1[j] = true; }
+      EvalEE(c, e0[i], other, e1[j], self, h, u0[i], u1[j], dq, cnt);
     }
   }
 }
@@ -530,10 +561,11 @@ __global__ __launch_bounds__(kBlock) void MultiObsGrowKernel(const InstContactDe
                c.obvh.flags);
 }
 
-__global__ __launch_bounds__(kBlock) void MultiObsBottomUpKernel(const InstContactDev* ic,
-                                                                 const int32_t* off, int n) {
-  XRT_MULTI_INDEX(kObsFace, c.obs_nf)
-  if (c.obs_nf > 1) BottomUpBody(c.obvh.nodes, c.obvh.box, c.obvh.flags, c.obs_nf, 0, i);
+// one workgroup per instance merges its obstacle tree
+__global__ __launch_bounds__(1024) void MultiObsBottomUpKernel(const InstContactDev* ic, int n) {
+  const InstContactDev& c = ic[blockIdx.x];
+  if (c.has_obs && c.obs_nf > 1)
+    BottomUpTreeWG(c.obvh.nodes, c.obvh.box, c.obvh.flags, c.obs_nf, 0);
 }
 
 __global__ __launch_bounds__(kBlock) void MultiVfGenerateKernel(const InstContactDev* ic,
@@ -571,10 +603,25 @@ __global__ __launch_bounds__(kBlock) void MultiClothGrowKernel(const InstContact
                c.cbvh.flags);
 }
 
-__global__ __launch_bounds__(kBlock) void MultiClothBottomUpKernel(const InstContactDev* ic,
-                                                                   const int32_t* off, int n) {
-  XRT_MULTI_INDEX(kFace, c.nf)
-  if (c.nf > 1) BottomUpBody(c.cbvh.nodes, c.cbvh.box, c.cbvh.flags, c.nf, 1, i);
+// frame start: one workgroup per tree merges the freshly built trees of every instance
+// (workgroups [0, n): cloth trees when `cloth`, [n, 2n): obstacle trees)
+__global__ __launch_bounds__(1024) void MultiBuildMergeKernel(const InstContactDev* ic, int n,
+                                                              int cloth) {
+  const bool obs = blockIdx.x >= static_cast<unsigned>(n);
+  const InstContactDev& c = ic[obs ? blockIdx.x - n : blockIdx.x];
+  if (!c.scal) return;
+  if (obs) {
+    if (c.has_obs && c.obs_nf > 1)
+      BottomUpTreeWG(c.obvh.nodes, c.obvh.box, c.obvh.flags, c.obs_nf, 0);
+  } else if (cloth && c.nf > 1) {
+    BottomUpTreeWG(c.cbvh.nodes, c.cbvh.box, c.cbvh.flags, c.nf, 0);
+  }
+}
+
+// one workgroup per instance merges its cloth tree (grow-only)
+__global__ __launch_bounds__(1024) void MultiClothBottomUpKernel(const InstContactDev* ic, int n) {
+  const InstContactDev& c = ic[blockIdx.x];
+  if (c.scal && c.nf > 1) BottomUpTreeWG(c.cbvh.nodes, c.cbvh.box, c.cbvh.flags, c.nf, 1);
 }
 
 // pairs per cloth leaf against the obstacle (self = 0) or the cloth itself (self = 1)
@@ -738,7 +785,7 @@ hipError_t MultiObstacleRefit(hipStream_t s, const InstContactDev* ic, const Mul
   MultiInterpolateKernel<<<m.blocks[MK::kObsVert], kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n, alpha);
   MultiNormalPredKernel<<<m.blocks[MK::kObsVert], kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n, tol);
   MultiObsGrowKernel<<<m.blocks[MK::kObsFace], kBlock, 0, s>>>(ic, m.off[MK::kObsFace], m.n);
-  MultiObsBottomUpKernel<<<m.blocks[MK::kObsFace], kBlock, 0, s>>>(ic, m.off[MK::kObsFace], m.n);
+  MultiObsBottomUpKernel<<<m.n, 1024, 0, s>>>(ic, m.n);
   return hipGetLastError();
 }
 
@@ -758,7 +805,7 @@ hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& 
   const int32_t* fo = m.off[MK::kFace];
   MultiClothPredKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x);
   MultiClothGrowKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, x, x0);
-  MultiClothBottomUpKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n);
+  MultiClothBottomUpKernel<<<m.n, 1024, 0, s>>>(ic, m.n);
   for (int self = 0; self < 2; ++self) {
     MultiCountPairsKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
     MultiScanKernel<<<m.n, 1024, 0, s>>>(ic, m.n, self);
@@ -787,6 +834,12 @@ hipError_t MultiEeSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps
   if (Empty(m, MK::kEdge)) return hipSuccess;
   MultiEeSolveKernel<<<m.blocks[MK::kEdge], kBlock, 0, s>>>(ic, m.off[MK::kEdge], m.n, x);
   MultiApplyKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, relax);
+  return hipGetLastError();
+}
+
+hipError_t MultiBuildMerge(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, bool cloth) {
+  if (m.n <= 0) return hipSuccess;
+  MultiBuildMergeKernel<<<2 * m.n, 1024, 0, s>>>(ic, m.n, cloth ? 1 : 0);
   return hipGetLastError();
 }
 

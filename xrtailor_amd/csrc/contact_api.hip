@@ -341,12 +341,24 @@ ContactData* FirstContact(xrt_ctx* ctx) {
 }
 }  // namespace
 
+// frame start: every instance's trees are built without their bottom-up merge, then all
+// of them are merged in one launch
 int MultiContactSubstepBegin(xrt_ctx* ctx) {
-  for (size_t i = 0; i < ctx->contact.size(); ++i)
-    if (ctx->contact[i]) {
-      const int rc = ContactSubstepBegin(ctx, static_cast<int>(i));
-      if (rc != XRT_OK) return rc;
-    }
+  ContactData* first = FirstContact(ctx);
+  if (!first || first->frame_substep != 0) return XRT_OK;
+  hipStream_t s = ctx->stream;
+  const bool self = ctx->params.enable_self_contact != 0;
+  for (size_t i = 0; i < ctx->contact.size(); ++i) {
+    ContactData* c = ctx->contact[i].get();
+    if (!c) continue;
+    if (c->obs_nf > 0)
+      XRT_CHECK_HIP(LbvhBuild(c->obs_bvh, c->o_x0.ptr, nullptr, nullptr, 0.0f, 0, 0.0f, s, false));
+    if (self)
+      XRT_CHECK_HIP(LbvhBuild(c->cloth_bvh, ctx->x0.ptr + ctx->inst[i].desc.vbeg, nullptr, nullptr,
+                              0.0f, 0, 0.0f, s, false));
+    c->pass_cnt = 0;
+  }
+  XRT_CHECK_HIP(MultiBuildMerge(s, ctx->ic.ptr, ctx->maps, self));
   return XRT_OK;
 }
 
