@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define XRT_ABI_VERSION 1
+#define XRT_ABI_VERSION 2
 
 enum {
   XRT_OK = 0,
@@ -70,6 +70,12 @@ typedef struct xrt_params {
   /* capacity of the repulsion face-pair buffer per cloth face (fixed at xrt_start); a
    * larger pair count is reported by xrt_get_contacts and the excess pairs are ignored */
   int32_t pair_capacity_per_face;
+  /* capacities of the batched path's repulsion test lists (VF and EE items per cloth face,
+   * defaults 64 and 128), rebuilt from the pairs at each refresh; an instance whose items
+   * exceed a list evaluates that half per pair instead, with the same results.  0 turns
+   * the list off.  No counterpart in the reference, which evaluates per pair. */
+  int32_t repulsion_vf_items_per_face;
+  int32_t repulsion_ee_items_per_face;
 } xrt_params;
 
 enum { XRT_BEND_REFERENCE = 0, XRT_BEND_CONSISTENT = 1 };

@@ -93,3 +93,31 @@ def test_batched_frames_graph_match_plain():
     a.sync()
     b.FramesPlain(1)
     assert _compare(a, ia, b, ib, "3 frames") == 0.0
+
+
+@pytest.mark.parametrize("vf_cap,ee_cap", [(1, 1), (64, 1), (0, 0)])
+def test_repulsion_item_list_fallback(vf_cap, ee_cap):
+    """The batched repulsion runs one VF or EE test item per thread from lists rebuilt at
+    each pair refresh; an instance whose items overflow a list (1 item per face here) or
+    whose lists are off (0) evaluates that half per pair.  Every variant matches the plain
+    per-pair path bit for bit, and the repulsion is active in the scene (turning its
+    relaxation to 0 changes the result)."""
+    print(f"case repulsion item lists vf_cap={vf_cap} ee_cap={ee_cap}", flush=True)
+    kw = dict(bend_model=1, enable_obstacle_contact=1, enable_self_contact=1, ground_height=-1.0,
+              num_substeps=4, num_iterations=6, num_collision_passes=6, pair_update_interval=3)
+    xp = make_params(repulsion_vf_items_per_face=vf_cap, repulsion_ee_items_per_face=ee_cap, **kw)
+    specs = _specs(seed=13)
+    a, ia, b, ib = _pair(specs, xp)
+    for step in range(4):
+        a.SubstepBatched()
+        for j in ib:
+            b.SubstepPlain(j)
+    a.sync()
+    assert _compare(a, ia, b, ib, "4 substeps") == 0.0
+    c, ic, _, _ = _pair(specs, make_params(repulsion_relaxation=0.0, **kw))
+    for step in range(4):
+        c.SubstepBatched()
+    c.sync()
+    moved = max(float(np.abs(a.GetState(i)[0] - c.GetState(j)[0]).max()) for i, j in zip(ia, ic))
+    print(f"  repulsion moved the cloth by up to {moved:.3e} m", flush=True)
+    assert moved > 0.0

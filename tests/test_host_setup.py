@@ -58,12 +58,14 @@ def test_abi_exports_every_declared_symbol():
 
 def test_abi_version_and_defaults():
     lib = _ffi.load()
-    assert lib.xrt_abi_version() == 1
+    assert lib.xrt_abi_version() == 2
     p = _ffi.XrtParams()
     lib.xrt_default_params(C.byref(p))
     assert p.num_substeps == 10 and p.num_iterations == 10
     assert p.damping == pytest.approx(0.98) and p.relaxation_factor == pytest.approx(0.25)
     assert p.bend_model == 0
+    assert p.pair_capacity_per_face == 64
+    assert p.repulsion_vf_items_per_face == 64 and p.repulsion_ee_items_per_face == 128
 
 
 @pytest.mark.parametrize("n", [2, 5, 64])

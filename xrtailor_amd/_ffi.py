@@ -52,6 +52,8 @@ class XrtParams(C.Structure):
         ("bend_model", C.c_int32),
         ("coloring", C.c_int32),
         ("pair_capacity_per_face", C.c_int32),
+        ("repulsion_vf_items_per_face", C.c_int32),
+        ("repulsion_ee_items_per_face", C.c_int32),
     ]
 
 

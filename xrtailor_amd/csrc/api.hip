@@ -238,6 +238,8 @@ void xrt_default_params(xrt_params* p) {
   p->bend_model = XRT_BEND_REFERENCE;
   p->coloring = XRT_COLOR_GREEDY;
   p->pair_capacity_per_face = 64;
+  p->repulsion_vf_items_per_face = 64;
+  p->repulsion_ee_items_per_face = 128;
 }
 
 int xrt_create(xrt_ctx** out, int device) {
@@ -292,6 +294,8 @@ int xrt_set_params(xrt_ctx* ctx, const xrt_params* p) {
               XRT_ERR_INVALID_ARG, "unknown bend_model");
   XRT_REQUIRE(p->coloring == XRT_COLOR_GREEDY || p->coloring == XRT_COLOR_PALETTE,
               XRT_ERR_INVALID_ARG, "unknown coloring mode");
+  XRT_REQUIRE(p->repulsion_vf_items_per_face >= 0 && p->repulsion_ee_items_per_face >= 0,
+              XRT_ERR_INVALID_ARG, "repulsion item capacities must be >= 0");
   XRT_REQUIRE(std::isfinite(p->gravity[0]) && std::isfinite(p->gravity[1]) &&
                   std::isfinite(p->gravity[2]) && std::isfinite(p->damping),
               XRT_ERR_INVALID_ARG, "non-finite parameter");

@@ -89,7 +89,11 @@ struct InstContactDev {
   long long* dq = nullptr;
   int32_t* cnt = nullptr;
   int2* pairs = nullptr;
-  int32_t* scal = nullptr;
+  int32_t* scal = nullptr;  // [1] obstacle pairs, [2] all pairs, [3] VF items, [4] EE items
+  // repulsion test items rebuilt at each pair refresh (capacities 0: lists off)
+  int vf_cap = 0, ee_cap = 0;
+  int2* rep_vf = nullptr;
+  int2* rep_ee = nullptr;
 };
 
 // block -> instance maps: per kind, the prefix sums (n + 1 entries, device) of the
@@ -109,10 +113,11 @@ hipError_t MultiGenerate(hipStream_t s, const InstContactDev* ic, const MultiMap
                          const float4* x0, const float4* x);
 // pair refresh: cloth LBVH Update(x, x0, ccd), Traverse(cloth, obstacle) ++
 // Traverse(cloth, cloth) into each instance's pair list (scal[1] obstacle pairs, scal[2]
-// total)
+// total), then the pairs' repulsion test items
 hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& m,
                       const float4* x, const float4* x0, float thickness);
-// PBD repulsion over every instance's pairs, applied x relax
+// PBD repulsion over every instance's pairs (one VF or EE test item per thread), applied
+// x relax
 hipError_t MultiRepulsion(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
                           const float4* x0, float h, float relax);
 // VF-CCD (offset 0) then VF-DCD (offset 1e-3) solves

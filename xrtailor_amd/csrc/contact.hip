@@ -402,17 +402,32 @@ __device__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self
 }
 
 // EvaluateRep_Kernel body (PBDRepulsion.cu:470-501) for pair p: cloth face pr.x against
-// obstacle face pr.y (p < nobs) or cloth face pr.y
-__device__ __forceinline__ void RepulsionPairBody(const RMesh& c, const RMesh& o, int2 pr,
-                                                  bool self, float h, long long* __restrict__ dq,
-                                                  int32_t* __restrict__ cnt) {
+// obstacle face pr.y (p < nobs) or cloth face pr.y.  The VF and EE halves are separate
+// bodies so that the multi-instance path can run them as separate launches.
+__device__ __forceinline__ bool PairValid(const RMesh& c, const RMesh& other, int2 pr) {
+  return pr.x >= 0 && pr.x < c.t.nf && pr.y >= 0 && pr.y < other.t.nf;
+}
+
+__device__ __forceinline__ void RepulsionPairVfBody(const RMesh& c, const RMesh& o, int2 pr,
+                                                    bool self, float h,
+                                                    long long* __restrict__ dq,
+                                                    int32_t* __restrict__ cnt) {
   const RMesh& other = self ? c : o;
-  if (pr.x < 0 || pr.x >= c.t.nf || pr.y < 0 || pr.y >= other.t.nf) return;
-  const uint32_t r0 = c.t.rtri[pr.x], r1 = other.t.rtri[pr.y];
+  if (!PairValid(c, other, pr)) return;
+  const uint32_t r0 = c.t.rtri[pr.x];
   const int4 f0 = c.t.tris[pr.x];
   const int v0[3] = {f0.x, f0.y, f0.z};
   for (int i = 0; i < 3; ++i)
     if ((r0 >> i) & 1u) EvalVF(c, v0[i], other, pr.y, self, h, dq, cnt);
+}
+
+__device__ __forceinline__ void RepulsionPairEeBody(const RMesh& c, const RMesh& o, int2 pr,
+                                                    bool self, float h,
+                                                    long long* __restrict__ dq,
+                                                    int32_t* __restrict__ cnt) {
+  const RMesh& other = self ? c : o;
+  if (!PairValid(c, other, pr)) return;
+  const uint32_t r0 = c.t.rtri[pr.x], r1 = other.t.rtri[pr.y];
   if (!((r0 >> 3) & 7u) || !((r1 >> 3) & 7u)) return;  // no owned edge pair
   const int4 fe0 = c.t.face_edges[pr.x], fe1 = other.t.face_edges[pr.y];
   const int e0[3] = {fe0.x, fe0.y, fe0.z}, e1[3] = {fe1.x, fe1.y, fe1.z};
@@ -434,6 +449,75 @@ __device__ __forceinline__ void RepulsionPairBody(const RMesh& c, const RMesh& o
       EvalEE(c, e0[i], other, e1[j], self, h, u0[i], u1[j], dq, cnt);
     }
   }
+}
+
+__device__ __forceinline__ void RepulsionPairBody(const RMesh& c, const RMesh& o, int2 pr,
+                                                  bool self, float h, long long* __restrict__ dq,
+                                                  int32_t* __restrict__ cnt) {
+  RepulsionPairVfBody(c, o, pr, self, h, dq, cnt);
+  RepulsionPairEeBody(c, o, pr, self, h, dq, cnt);
+}
+
+// Test items of one pair: the owned VF tests (cloth vertex, other face) whose vertex is not
+// a corner of the other face, and the owned EE tests (cloth edge, other edge) that share no
+// vertex -- exactly the tests RepulsionPairVfBody / RepulsionPairEeBody evaluate past their
+// adjacency checks.  emit(is_ee, item) is called once per item; bit 31 of item.y marks a
+// cloth-vs-cloth item.
+constexpr int kSelfBit = static_cast<int>(0x80000000u);
+template <class Emit>
+__device__ __forceinline__ void PairItems(const RMesh& c, const RMesh& o, int2 pr, bool self,
+                                          Emit&& emit) {
+  const RMesh& other = self ? c : o;
+  if (!PairValid(c, other, pr)) return;
+  const int tag = self ? kSelfBit : 0;
+  const uint32_t r0 = c.t.rtri[pr.x], r1 = other.t.rtri[pr.y];
+  const int4 f0 = c.t.tris[pr.x], f1 = other.t.tris[pr.y];
+  const int v0[3] = {f0.x, f0.y, f0.z};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (!((r0 >> i) & 1u)) continue;
+    if (self && (v0[i] == f1.x || v0[i] == f1.y || v0[i] == f1.z)) continue;
+    emit(false, make_int2(v0[i], pr.y | tag));
+  }
+  if (!((r0 >> 3) & 7u) || !((r1 >> 3) & 7u)) return;
+  const int4 fe0 = c.t.face_edges[pr.x], fe1 = other.t.face_edges[pr.y];
+  const int e0[3] = {fe0.x, fe0.y, fe0.z}, e1[3] = {fe1.x, fe1.y, fe1.z};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (!((r0 >> (3 + i)) & 1u)) continue;
+    const int2 ea = c.t.edges[e0[i]];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      if (!((r1 >> (3 + j)) & 1u)) continue;
+      const int2 eb = other.t.edges[e1[j]];
+      if (self && (ea.x == eb.x || ea.x == eb.y || ea.y == eb.x || ea.y == eb.y)) continue;
+      emit(true, make_int2(e0[i], e1[j] | tag));
+    }
+  }
+}
+
+// one VF item: cloth vertex it.x against face it.y of the obstacle or the cloth
+__device__ __forceinline__ void RepulsionVfItemBody(const RMesh& c, const RMesh& o, int2 it,
+                                                    float h, long long* __restrict__ dq,
+                                                    int32_t* __restrict__ cnt) {
+  const bool self = it.y < 0;
+  const RMesh& other = self ? c : o;
+  const int f = it.y & 0x7fffffff;
+  if (it.x < 0 || it.x >= c.t.nv || f >= other.t.nf) return;
+  EvalVF(c, it.x, other, f, self, h, dq, cnt);
+}
+
+// one EE item: cloth edge it.x against edge it.y of the obstacle or the cloth
+__device__ __forceinline__ void RepulsionEeItemBody(const RMesh& c, const RMesh& o, int2 it,
+                                                    float h, long long* __restrict__ dq,
+                                                    int32_t* __restrict__ cnt) {
+  const bool self = it.y < 0;
+  const RMesh& other = self ? c : o;
+  const int e1 = it.y & 0x7fffffff;
+  if (it.x < 0 || it.x >= c.t.ne || e1 >= other.t.ne) return;
+  const int2 ea = c.t.edges[it.x], eb = other.t.edges[e1];
+  const F3 ua = Nz(S(c.X(ea.y), c.X(ea.x))), ub = Nz(S(other.X(eb.y), other.X(eb.x)));
+  EvalEE(c, it.x, other, e1, self, h, ua, ub, dq, cnt);
 }
 
 // ------------------------------------------------------- single-instance kernels
@@ -657,23 +741,110 @@ __global__ __launch_bounds__(kBlock) void MultiAddTotalsKernel(const InstContact
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n || !ic[k].scal) return;
   ic[k].scal[2] = ic[k].scal[1] + *ic[k].cbvh.total;
+  ic[k].scal[3] = 0;  // item counts, refilled by MultiRepItemsKernel
+  ic[k].scal[4] = 0;
 }
 
-// each instance's blocks stride over its own device-side pair count
-__global__ __launch_bounds__(kBlock) void MultiRepulsionKernel(const InstContactDev* ic,
-                                                               const int32_t* off, int n,
-                                                               const float4* __restrict__ x,
-                                                               const float4* __restrict__ x0,
-                                                               float h) {
+// inclusive prefix sum over the 64 lanes of a wave
+__device__ __forceinline__ int WaveInclusiveScan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// pair refresh: every instance's pairs -> its VF and EE item lists (scal[3], scal[4] hold
+// the item counts, which may exceed the capacities; the lists then stay unused).  Slots are
+// reserved with one atomic per wave; the item order is free because the repulsion sums in
+// fixed point.
+__global__ __launch_bounds__(kBlock) void MultiRepItemsKernel(const InstContactDev* ic,
+                                                              const int32_t* off, int n) {
   const int k = InstOfBlock(off, n, blockIdx.x);
   const InstContactDev& c = ic[k];
+  if (c.ee_cap <= 0) return;  // item lists off for this instance
   const int nblk = off[k + 1] - off[k];
   const int total = min(c.scal[2], c.pair_cap);
   const int nobs = min(c.scal[1], total);
+  const RMesh cm{c.cloth, MeshState{}};
+  const RMesh om{c.obs, MeshState{}};
+  const int lane = threadIdx.x & 63;
+  // the loop bound is uniform over the block, so every lane reaches the shuffles
+  for (int b = (blockIdx.x - off[k]) * kBlock; b < total; b += nblk * kBlock) {
+    const int p = b + threadIdx.x;
+    int2 pr = make_int2(-1, -1);
+    if (p < total) pr = c.pairs[p];
+    const bool self = p >= nobs;
+    int nvf = 0, nee = 0;
+    PairItems(cm, om, pr, self, [&](bool is_ee, int2) { if (is_ee) ++nee; else ++nvf; });
+    const int vf_incl = WaveInclusiveScan(nvf), ee_incl = WaveInclusiveScan(nee);
+    const int vf_tot = __shfl(vf_incl, 63, 64), ee_tot = __shfl(ee_incl, 63, 64);
+    int vf_base = 0, ee_base = 0;
+    if (lane == 63) {
+      if (vf_tot) vf_base = atomicAdd(c.scal + 3, vf_tot);
+      if (ee_tot) ee_base = atomicAdd(c.scal + 4, ee_tot);
+    }
+    int vo = __shfl(vf_base, 63, 64) + vf_incl - nvf;
+    int eo = __shfl(ee_base, 63, 64) + ee_incl - nee;
+    PairItems(cm, om, pr, self, [&](bool is_ee, int2 it) {
+      if (is_ee) {
+        if (eo < c.ee_cap) c.rep_ee[eo] = it;
+        ++eo;
+      } else {
+        if (vo < c.vf_cap) c.rep_vf[vo] = it;
+        ++vo;
+      }
+    });
+  }
+}
+
+// repulsion VF tests of every instance: one item per thread, or the pairs when the
+// instance's VF list overflowed (or is off)
+__global__ __launch_bounds__(kBlock) void MultiRepVfKernel(const InstContactDev* ic,
+                                                           const int32_t* off, int n,
+                                                           const float4* __restrict__ x,
+                                                           const float4* __restrict__ x0,
+                                                           float h) {
+  const int k = InstOfBlock(off, n, blockIdx.x);
+  const InstContactDev& c = ic[k];
+  const int nblk = off[k + 1] - off[k];
+  const int t0 = (blockIdx.x - off[k]) * kBlock + threadIdx.x, stride = nblk * kBlock;
   const RMesh cm{c.cloth, MeshState{x + c.vbeg, x0 + c.vbeg, -1.0f}};
   const RMesh om{c.obs, MeshState{c.o_x, c.o_x0, 1e-3f}};  // obstacle nodes: inv mass 1e-3
-  for (int p = (blockIdx.x - off[k]) * kBlock + threadIdx.x; p < total; p += nblk * kBlock)
-    RepulsionPairBody(cm, om, c.pairs[p], p >= nobs, h, c.dq, c.cnt);
+  const int nvf = c.scal[3];
+  if (c.ee_cap > 0 && nvf <= c.vf_cap) {
+    for (int q = t0; q < nvf; q += stride) RepulsionVfItemBody(cm, om, c.rep_vf[q], h, c.dq, c.cnt);
+  } else {
+    const int total = min(c.scal[2], c.pair_cap);
+    const int nobs = min(c.scal[1], total);
+    for (int p = t0; p < total; p += stride)
+      RepulsionPairVfBody(cm, om, c.pairs[p], p >= nobs, h, c.dq, c.cnt);
+  }
+}
+
+// repulsion EE tests of every instance (as MultiRepVfKernel)
+__global__ __launch_bounds__(kBlock) void MultiRepEeKernel(const InstContactDev* ic,
+                                                           const int32_t* off, int n,
+                                                           const float4* __restrict__ x,
+                                                           const float4* __restrict__ x0,
+                                                           float h) {
+  const int k = InstOfBlock(off, n, blockIdx.x);
+  const InstContactDev& c = ic[k];
+  const int nblk = off[k + 1] - off[k];
+  const int t0 = (blockIdx.x - off[k]) * kBlock + threadIdx.x, stride = nblk * kBlock;
+  const RMesh cm{c.cloth, MeshState{x + c.vbeg, x0 + c.vbeg, -1.0f}};
+  const RMesh om{c.obs, MeshState{c.o_x, c.o_x0, 1e-3f}};
+  const int nee = c.scal[4];
+  if (c.ee_cap > 0 && nee <= c.ee_cap) {
+    for (int q = t0; q < nee; q += stride) RepulsionEeItemBody(cm, om, c.rep_ee[q], h, c.dq, c.cnt);
+  } else {
+    const int total = min(c.scal[2], c.pair_cap);
+    const int nobs = min(c.scal[1], total);
+    for (int p = t0; p < total; p += stride)
+      RepulsionPairEeBody(cm, om, c.pairs[p], p >= nobs, h, c.dq, c.cnt);
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void MultiApplyKernel(const InstContactDev* ic,
@@ -812,13 +983,16 @@ hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& 
     MultiFindPairsKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
   }
   MultiAddTotalsKernel<<<Blocks(m.n), kBlock, 0, s>>>(ic, m.n);
+  if (!Empty(m, MK::kPair))
+    MultiRepItemsKernel<<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n);
   return hipGetLastError();
 }
 
 hipError_t MultiRepulsion(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
                           const float4* x0, float h, float relax) {
   if (Empty(m, MK::kPair)) return hipSuccess;
-  MultiRepulsionKernel<<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
+  MultiRepVfKernel<<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
+  MultiRepEeKernel<<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
   MultiApplyKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, relax);
   return hipGetLastError();
 }

@@ -144,8 +144,20 @@ int ContactStart(xrt_ctx* ctx) {
     const int per_face = std::max(1, ctx->params.pair_capacity_per_face);
     c->pair_cap = std::max(1, per_face * h.nt);
     XRT_CHECK_HIP(c->pairs.Alloc(c->pair_cap));
-    XRT_CHECK_HIP(c->scal.Alloc(4));
-    XRT_CHECK_HIP(hipMemsetAsync(c->scal.ptr, 0, sizeof(int32_t) * 4, s));
+    XRT_CHECK_HIP(c->scal.Alloc(8));
+    XRT_CHECK_HIP(hipMemsetAsync(c->scal.ptr, 0, sizeof(int32_t) * 8, s));
+    // repulsion item lists (at most 3 VF and 9 EE items per pair; an instance whose items
+    // overflow a list falls back to the per-pair tests for that half); off when either
+    // capacity is 0 or the counts could overflow an int
+    const int64_t vf_cap = static_cast<int64_t>(std::max(0, ctx->params.repulsion_vf_items_per_face)) * h.nt;
+    const int64_t ee_cap = static_cast<int64_t>(std::max(0, ctx->params.repulsion_ee_items_per_face)) * h.nt;
+    if (ctx->params.enable_self_contact && vf_cap > 0 && ee_cap > 0 &&
+        9 * static_cast<int64_t>(c->pair_cap) <= static_cast<int64_t>(INT32_MAX)) {
+      c->vf_cap = static_cast<int>(std::min<int64_t>(vf_cap, 3 * static_cast<int64_t>(c->pair_cap)));
+      c->ee_cap = static_cast<int>(std::min<int64_t>(ee_cap, 9 * static_cast<int64_t>(c->pair_cap)));
+      XRT_CHECK_HIP(c->rep_vf.Alloc(c->vf_cap));
+      XRT_CHECK_HIP(c->rep_ee.Alloc(c->ee_cap));
+    }
     ctx->contact[ii] = std::move(c);
   }
   XRT_CHECK_HIP(hipStreamSynchronize(s));
@@ -309,6 +321,10 @@ int MultiContactSetup(xrt_ctx* ctx) {
       d.cnt = c.cnt.ptr;
       d.pairs = c.pairs.ptr;
       d.scal = c.scal.ptr;
+      d.vf_cap = c.vf_cap;
+      d.ee_cap = c.ee_cap;
+      d.rep_vf = c.rep_vf.ptr;
+      d.rep_ee = c.rep_ee.ptr;
       cnt[MultiMaps::kVert] = blocks(h.nv);
       cnt[MultiMaps::kEdge] = blocks(h.ne());
       cnt[MultiMaps::kFace] = blocks(h.nt);

@@ -78,7 +78,11 @@ struct ContactData {
   // face pairs of the repulsion (cloth face, other face)
   int pair_cap = 0;
   DevBuf<int2> pairs;
-  DevBuf<int32_t> scal;  // [0] zero, [1] obstacle-pair count, [2] total pair count
+  DevBuf<int32_t> scal;  // [0] zero, [1] obstacle-pair count, [2] total pair count,
+                         // [3] / [4] repulsion VF / EE item counts (batched path)
+  // repulsion test items of the batched path (capacities 0: lists off)
+  int vf_cap = 0, ee_cap = 0;
+  DevBuf<int2> rep_vf, rep_ee;
   int frame_substep = 0, pass_cnt = 0;
   bool obstacle_moving = false;  // xrt_update_obstacle since the last frame end
 };
