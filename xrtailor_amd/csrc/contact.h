@@ -94,7 +94,12 @@ struct InstContactDev {
   int vf_cap = 0, ee_cap = 0;
   int2* rep_vf = nullptr;
   int2* rep_ee = nullptr;
+  // pair refresh: the first kPairStage hits of each cloth leaf, hit j of leaf i at
+  // stage[j * nf + i] (null: the pairs are found by a second traversal)
+  int2* stage = nullptr;
 };
+
+constexpr int kPairStage = 32;
 
 // block -> instance maps: per kind, the prefix sums (n + 1 entries, device) of the
 // 256-thread blocks each instance needs, and their totals (host)

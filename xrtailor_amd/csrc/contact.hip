@@ -100,20 +100,6 @@ __device__ float SignedVF(F3 x, F3 y0, F3 y1, F3 y2, F3& n, float* w) {
 
 // SignedEdgeEdgeDistance (BasicPrimitiveTests.cu:420-436): n is left unnormalized (the
 // reference discards glm::normalize(n))
-// with the unit edge directions ux = Nz(x1 - x0), uy = Nz(y1 - y0) given
-__device__ float SignedEEu(F3 x0, F3 x1, F3 y0, F3 y1, F3 ux, F3 uy, F3& n, float* w) {
-  n = C(ux, uy);
-  if (D(n, n) < 1e-6f) { w[0] = w[1] = w[2] = w[3] = 0; return INFINITY; }
-  const float h = D(S(x0, y0), n);
-  const float a0 = Mixed(S(y1, x1), S(y0, x1), n), a1 = Mixed(S(y0, x0), S(y1, x0), n);
-  const float b0 = Mixed(S(x0, y1), S(x1, y1), n), b1 = Mixed(S(x1, y0), S(x0, y0), n);
-  w[0] = a0 / (a0 + a1);
-  w[1] = a1 / (a0 + a1);
-  w[2] = -b0 / (b0 + b1);
-  w[3] = -b1 / (b0 + b1);
-  return h;
-}
-
 __device__ float SignedEE(F3 x0, F3 x1, F3 y0, F3 y1, F3& n, float* w) {
   n = C(Nz(S(x1, x0)), Nz(S(y1, y0)));
   if (D(n, n) < 1e-6f) { w[0] = w[1] = w[2] = w[3] = 0; return INFINITY; }
@@ -332,11 +318,22 @@ __device__ void EvalVF(const RMesh& c, int node, const RMesh& o, int face, bool 
   const int4 tr = o.t.tris[face];
   if (self && (node == tr.x || node == tr.y || node == tr.z)) return;
   // the inside test on the current state rejects most pairs; it is evaluated before the
-  // start-of-substep side (both are pure, so the order does not change any result)
-  float w[4];
-  F3 fn;
-  SignedVF(c.X(node), o.X(tr.x), o.X(tr.y), o.X(tr.z), fn, w);
-  if (!(fminf(-w[1], fminf(-w[2], -w[3])) >= 1e-6f)) return;
+  // start-of-substep side (both are pure, so the order does not change any result).  It is
+  // SignedVF's weights with an early exit: min(-w1, -w2, -w3) >= 1e-6 needs b0, b1, b2 of
+  // one strict sign (a NaN or a zero weight cannot pass either), so that is checked before
+  // the divisions, on the same expressions
+  {
+    const F3 x = c.X(node), y0 = o.X(tr.x), y1 = o.X(tr.y), y2 = o.X(tr.z);
+    F3 n = C(Nz(S(y1, y0)), Nz(S(y2, y0)));
+    if (D(n, n) < 1e-6f) return;  // degenerate: weights 0
+    n = Nz(n);
+    const float b0 = Mixed(S(y1, x), S(y2, x), n);
+    const float b1 = Mixed(S(y2, x), S(y0, x), n);
+    const float b2 = Mixed(S(y0, x), S(y1, x), n);
+    if (!((b0 > 0 && b1 > 0 && b2 > 0) || (b0 < 0 && b1 < 0 && b2 < 0))) return;
+    const float w1 = -b0 / (b0 + b1 + b2), w2 = -b1 / (b0 + b1 + b2), w3 = -b2 / (b0 + b1 + b2);
+    if (!(fminf(-w1, fminf(-w2, -w3)) >= 1e-6f)) return;
+  }
   float w0[4];
   F3 pn;
   const float d0 = SignedVF(c.X0(node), o.X0(tr.x), o.X0(tr.y), o.X0(tr.z), pn, w0);
@@ -371,10 +368,18 @@ __device__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self
                        F3 ub, long long* dq, int32_t* cnt) {
   const int2 a = c.t.edges[e0], b = o.t.edges[e1];
   if (self && (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y)) return;
-  // current-state tests first, then the start-of-substep side (pure; same results)
-  float bw[4];
-  F3 ncur;
-  SignedEEu(c.X(a.x), c.X(a.y), o.X(b.x), o.X(b.y), ua, ub, ncur, bw);
+  // current-state tests first, then the start-of-substep side (pure; same results).  The
+  // weights are SignedEE's with early exits: an accepted pair has every weight > 0 (a NaN
+  // weight fails InEdge or leaves another weight <= 0), which needs a0, a1 and b0, b1 each
+  // of one strict sign, checked before the divisions on the same expressions
+  const F3 x0 = c.X(a.x), x1 = c.X(a.y), y0 = o.X(b.x), y1 = o.X(b.y);
+  const F3 ncur = C(ua, ub);
+  if (D(ncur, ncur) < 1e-6f) return;  // degenerate: weights 0
+  const float a0 = Mixed(S(y1, x1), S(y0, x1), ncur), a1 = Mixed(S(y0, x0), S(y1, x0), ncur);
+  if (!((a0 > 0 && a1 > 0) || (a0 < 0 && a1 < 0))) return;
+  const float b0 = Mixed(S(x0, y1), S(x1, y1), ncur), b1 = Mixed(S(x1, y0), S(x0, y0), ncur);
+  if (!((b0 > 0 && b1 > 0) || (b0 < 0 && b1 < 0))) return;
+  float bw[4] = {a0 / (a0 + a1), a1 / (a0 + a1), -b0 / (b0 + b1), -b1 / (b0 + b1)};
   const float mn = fminf(fminf(bw[0], bw[1]), fminf(-bw[2], -bw[3]));
   if (!(mn >= 1e-6f && InEdge(bw[1], c, e0, o, e1) && InEdge(-bw[3], o, e1, c, e0))) return;
   float w[4];
@@ -714,8 +719,11 @@ __global__ __launch_bounds__(kBlock) void MultiCountPairsKernel(const InstContac
                                                                 float th, int self) {
   XRT_MULTI_INDEX(kFace, c.nf)
   uint32_t v = 0;
-  if (self) v = CountPairsBody(c.cbvh, c.cbvh, th, i);
-  else if (c.has_obs) v = CountPairsBody(c.cbvh, c.obvh, th, i);
+  if (self || c.has_obs) {
+    const BvhView& r = self ? c.cbvh : c.obvh;
+    v = c.stage ? CountStageBody(c.cbvh, r, th, i, c.stage + i, c.nf, kPairStage)
+                : CountPairsBody(c.cbvh, r, th, i);
+  }
   c.cbvh.counts[i] = v;
   c.cbvh.offsets[i] = v;
 }
@@ -728,13 +736,25 @@ __global__ __launch_bounds__(1024) void MultiScanKernel(const InstContactDev* ic
   BlockExclusiveScan(c.cbvh.offsets, c.nf, self ? c.cbvh.total : c.scal + 1);
 }
 
-__global__ __launch_bounds__(kBlock) void MultiFindPairsKernel(const InstContactDev* ic,
+// FindPairs of every instance's leaves (pairs of leaf i fill [lo, lo + n) backwards, slots
+// below the capacity only): copied from the count pass's stage when the leaf had at most
+// kPairStage hits, else found by a second traversal
+__global__ __launch_bounds__(kBlock) void MultiEmitPairsKernel(const InstContactDev* ic,
                                                                const int32_t* off, int n, float th,
                                                                int self) {
   XRT_MULTI_INDEX(kFace, c.nf)
   if (!self && !c.has_obs) return;
+  const int64_t cap = c.pair_cap;
+  const uint32_t cnt = c.cbvh.counts[i];
   const int64_t lo = static_cast<int64_t>(c.scal[self ? 1 : 0]) + c.cbvh.offsets[i];
-  FindPairsBody(c.cbvh, self ? c.cbvh : c.obvh, th, i, lo, c.cbvh.counts[i], c.pairs, c.pair_cap);
+  if (!c.stage || cnt > static_cast<uint32_t>(kPairStage)) {
+    FindPairsBody(c.cbvh, self ? c.cbvh : c.obvh, th, i, lo, cnt, c.pairs, cap);
+    return;
+  }
+  for (uint32_t j = 0; j < cnt; ++j) {
+    const int64_t idx = lo + cnt - 1 - j;
+    if (idx < cap) c.pairs[idx] = c.stage[static_cast<int64_t>(j) * c.nf + i];
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void MultiAddTotalsKernel(const InstContactDev* ic, int n) {
@@ -980,7 +1000,7 @@ hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& 
   for (int self = 0; self < 2; ++self) {
     MultiCountPairsKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
     MultiScanKernel<<<m.n, 1024, 0, s>>>(ic, m.n, self);
-    MultiFindPairsKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
+    MultiEmitPairsKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
   }
   MultiAddTotalsKernel<<<Blocks(m.n), kBlock, 0, s>>>(ic, m.n);
   if (!Empty(m, MK::kPair))

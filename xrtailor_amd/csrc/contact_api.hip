@@ -158,6 +158,8 @@ int ContactStart(xrt_ctx* ctx) {
       XRT_CHECK_HIP(c->rep_vf.Alloc(c->vf_cap));
       XRT_CHECK_HIP(c->rep_ee.Alloc(c->ee_cap));
     }
+    if (ctx->params.enable_self_contact)
+      XRT_CHECK_HIP(c->stage.Alloc(static_cast<size_t>(kPairStage) * std::max(h.nt, 1)));
     ctx->contact[ii] = std::move(c);
   }
   XRT_CHECK_HIP(hipStreamSynchronize(s));
@@ -325,6 +327,7 @@ int MultiContactSetup(xrt_ctx* ctx) {
       d.ee_cap = c.ee_cap;
       d.rep_vf = c.rep_vf.ptr;
       d.rep_ee = c.rep_ee.ptr;
+      d.stage = c.stage.ptr;
       cnt[MultiMaps::kVert] = blocks(h.nv);
       cnt[MultiMaps::kEdge] = blocks(h.ne());
       cnt[MultiMaps::kFace] = blocks(h.nt);

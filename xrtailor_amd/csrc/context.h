@@ -83,6 +83,7 @@ struct ContactData {
   // repulsion test items of the batched path (capacities 0: lists off)
   int vf_cap = 0, ee_cap = 0;
   DevBuf<int2> rep_vf, rep_ee;
+  DevBuf<int2> stage;  // kPairStage hits per cloth leaf of the batched pair refresh
   int frame_substep = 0, pass_cnt = 0;
   bool obstacle_moving = false;  // xrt_update_obstacle since the last frame end
 };
