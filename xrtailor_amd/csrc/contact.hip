@@ -83,7 +83,7 @@ __device__ float ClosestSegSeg(F3 p1, F3 q1, F3 p2, F3 q2, float& s, float& t) {
 }
 
 // SignedVertexFaceDistance (BasicPrimitiveTests.cu:369-386); w zeroed when degenerate
-__device__ float SignedVF(F3 x, F3 y0, F3 y1, F3 y2, F3& n, float* w) {
+__device__ __forceinline__ float SignedVF(F3 x, F3 y0, F3 y1, F3 y2, F3& n, float* w) {
   n = C(Nz(S(y1, y0)), Nz(S(y2, y0)));
   if (D(n, n) < 1e-6f) { w[0] = w[1] = w[2] = w[3] = 0; return INFINITY; }
   n = Nz(n);
@@ -100,7 +100,7 @@ __device__ float SignedVF(F3 x, F3 y0, F3 y1, F3 y2, F3& n, float* w) {
 
 // SignedEdgeEdgeDistance (BasicPrimitiveTests.cu:420-436): n is left unnormalized (the
 // reference discards glm::normalize(n))
-__device__ float SignedEE(F3 x0, F3 x1, F3 y0, F3 y1, F3& n, float* w) {
+__device__ __forceinline__ float SignedEE(F3 x0, F3 x1, F3 y0, F3 y1, F3& n, float* w) {
   n = C(Nz(S(x1, x0)), Nz(S(y1, y0)));
   if (D(n, n) < 1e-6f) { w[0] = w[1] = w[2] = w[3] = 0; return INFINITY; }
   const float h = D(S(x0, y0), n);
@@ -303,11 +303,16 @@ __device__ bool InEdge(float w, const RMesh& m0, int e0, const RMesh& m1, int e1
   return in;
 }
 
-__device__ __forceinline__ void Accumulate(const RMesh* const* ms, const int* ids, const F3* corr,
-                                           long long* dq, int32_t* cnt) {
-  for (int i = 0; i < 4; ++i) {  // AccumulateRepulsions (PBDRepulsion.cu:264-277)
+// AccumulateRepulsions (PBDRepulsion.cu:264-277): nodes [0, kSplit) belong to m0, the
+// rest to m1
+template <int kSplit>
+__device__ __forceinline__ void Accumulate(const RMesh& m0, const RMesh& m1, const int* ids,
+                                           const F3* corr, long long* dq, int32_t* cnt) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const RMesh& m = i < kSplit ? m0 : m1;
     if (Len(corr[i]) < kEpsC) continue;
-    if (!ms[i]->Free(ids[i])) continue;
+    if (!m.Free(ids[i])) continue;
     AddFixed(dq, cnt, ids[i], corr[i]);
   }
 }
@@ -357,9 +362,8 @@ __device__ void EvalVF(const RMesh& c, int node, const RMesh& o, int face, bool 
   if (den < kEpsC) return;
   const float s = Cv / den;
   const F3 corr[4] = {M(dcq, -wq * s), M(dcp1, -w1 * s), M(dcp2, -w2 * s), M(dcp3, -w3 * s)};
-  const RMesh* ms[4] = {&c, &o, &o, &o};
   const int ids[4] = {node, tr.x, tr.y, tr.z};
-  Accumulate(ms, ids, corr, dq, cnt);
+  Accumulate<1>(c, o, ids, corr, dq, cnt);
 }
 
 // EvaluateEEProximity (PBDRepulsion.cu:411-468)
@@ -401,9 +405,8 @@ __device__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self
   const float s = Cv / den;
   F3 corr[4];
   for (int i = 0; i < 4; ++i) corr[i] = M(g[i], -s * wq[i]);
-  const RMesh* ms[4] = {&c, &c, &o, &o};
   const int ids[4] = {a.x, a.y, b.x, b.y};
-  Accumulate(ms, ids, corr, dq, cnt);
+  Accumulate<2>(c, o, ids, corr, dq, cnt);
 }
 
 // EvaluateRep_Kernel body (PBDRepulsion.cu:470-501) for pair p: cloth face pr.x against
@@ -502,27 +505,38 @@ __device__ __forceinline__ void PairItems(const RMesh& c, const RMesh& o, int2 p
 }
 
 // one VF item: cloth vertex it.x against face it.y of the obstacle or the cloth
+// (the two meshes are separate inlined calls: selecting one of two structs by reference
+// would keep both in scratch memory)
 __device__ __forceinline__ void RepulsionVfItemBody(const RMesh& c, const RMesh& o, int2 it,
                                                     float h, long long* __restrict__ dq,
                                                     int32_t* __restrict__ cnt) {
-  const bool self = it.y < 0;
-  const RMesh& other = self ? c : o;
   const int f = it.y & 0x7fffffff;
-  if (it.x < 0 || it.x >= c.t.nv || f >= other.t.nf) return;
-  EvalVF(c, it.x, other, f, self, h, dq, cnt);
+  if (it.x < 0 || it.x >= c.t.nv) return;
+  if (it.y < 0) {
+    if (f < c.t.nf) EvalVF(c, it.x, c, f, true, h, dq, cnt);
+  } else {
+    if (f < o.t.nf) EvalVF(c, it.x, o, f, false, h, dq, cnt);
+  }
 }
 
 // one EE item: cloth edge it.x against edge it.y of the obstacle or the cloth
+template <bool kSelf>
+__device__ __forceinline__ void EeItem(const RMesh& c, const RMesh& other, int e0, int e1,
+                                       float h, long long* __restrict__ dq,
+                                       int32_t* __restrict__ cnt) {
+  if (e1 >= other.t.ne) return;
+  const int2 ea = c.t.edges[e0], eb = other.t.edges[e1];
+  const F3 ua = Nz(S(c.X(ea.y), c.X(ea.x))), ub = Nz(S(other.X(eb.y), other.X(eb.x)));
+  EvalEE(c, e0, other, e1, kSelf, h, ua, ub, dq, cnt);
+}
+
 __device__ __forceinline__ void RepulsionEeItemBody(const RMesh& c, const RMesh& o, int2 it,
                                                     float h, long long* __restrict__ dq,
                                                     int32_t* __restrict__ cnt) {
-  const bool self = it.y < 0;
-  const RMesh& other = self ? c : o;
+  if (it.x < 0 || it.x >= c.t.ne) return;
   const int e1 = it.y & 0x7fffffff;
-  if (it.x < 0 || it.x >= c.t.ne || e1 >= other.t.ne) return;
-  const int2 ea = c.t.edges[it.x], eb = other.t.edges[e1];
-  const F3 ua = Nz(S(c.X(ea.y), c.X(ea.x))), ub = Nz(S(other.X(eb.y), other.X(eb.x)));
-  EvalEE(c, it.x, other, e1, self, h, ua, ub, dq, cnt);
+  if (it.y < 0) EeItem<true>(c, c, it.x, e1, h, dq, cnt);
+  else EeItem<false>(c, o, it.x, e1, h, dq, cnt);
 }
 
 // ------------------------------------------------------- single-instance kernels
@@ -820,50 +834,47 @@ __global__ __launch_bounds__(kBlock) void MultiRepItemsKernel(const InstContactD
   }
 }
 
-// repulsion VF tests of every instance: one item per thread, or the pairs when the
-// instance's VF list overflowed (or is off)
-__global__ __launch_bounds__(kBlock) void MultiRepVfKernel(const InstContactDev* ic,
-                                                           const int32_t* off, int n,
-                                                           const float4* __restrict__ x,
-                                                           const float4* __restrict__ x0,
-                                                           float h) {
+// repulsion tests of every instance, one VF (is_ee = 0) or EE item per thread; instances
+// whose list overflowed (or is off) are left to MultiRepPairsKernel
+template <int is_ee>
+__global__ __launch_bounds__(kBlock) void MultiRepItemKernel(const InstContactDev* ic,
+                                                             const int32_t* off, int n,
+                                                             const float4* __restrict__ x,
+                                                             const float4* __restrict__ x0,
+                                                             float h) {
   const int k = InstOfBlock(off, n, blockIdx.x);
   const InstContactDev& c = ic[k];
+  const int cnt = c.scal[is_ee ? 4 : 3];
+  if (c.ee_cap <= 0 || cnt > (is_ee ? c.ee_cap : c.vf_cap)) return;
   const int nblk = off[k + 1] - off[k];
-  const int t0 = (blockIdx.x - off[k]) * kBlock + threadIdx.x, stride = nblk * kBlock;
   const RMesh cm{c.cloth, MeshState{x + c.vbeg, x0 + c.vbeg, -1.0f}};
   const RMesh om{c.obs, MeshState{c.o_x, c.o_x0, 1e-3f}};  // obstacle nodes: inv mass 1e-3
-  const int nvf = c.scal[3];
-  if (c.ee_cap > 0 && nvf <= c.vf_cap) {
-    for (int q = t0; q < nvf; q += stride) RepulsionVfItemBody(cm, om, c.rep_vf[q], h, c.dq, c.cnt);
-  } else {
-    const int total = min(c.scal[2], c.pair_cap);
-    const int nobs = min(c.scal[1], total);
-    for (int p = t0; p < total; p += stride)
-      RepulsionPairVfBody(cm, om, c.pairs[p], p >= nobs, h, c.dq, c.cnt);
+  const int2* items = is_ee ? c.rep_ee : c.rep_vf;
+  for (int q = (blockIdx.x - off[k]) * kBlock + threadIdx.x; q < cnt; q += nblk * kBlock) {
+    if (is_ee) RepulsionEeItemBody(cm, om, items[q], h, c.dq, c.cnt);
+    else RepulsionVfItemBody(cm, om, items[q], h, c.dq, c.cnt);
   }
 }
 
-// repulsion EE tests of every instance (as MultiRepVfKernel)
-__global__ __launch_bounds__(kBlock) void MultiRepEeKernel(const InstContactDev* ic,
-                                                           const int32_t* off, int n,
-                                                           const float4* __restrict__ x,
-                                                           const float4* __restrict__ x0,
-                                                           float h) {
+// the per-pair VF and / or EE halves of the instances whose item lists overflowed or are off
+__global__ __launch_bounds__(kBlock) void MultiRepPairsKernel(const InstContactDev* ic,
+                                                              const int32_t* off, int n,
+                                                              const float4* __restrict__ x,
+                                                              const float4* __restrict__ x0,
+                                                              float h) {
   const int k = InstOfBlock(off, n, blockIdx.x);
   const InstContactDev& c = ic[k];
+  const bool vf = c.ee_cap <= 0 || c.scal[3] > c.vf_cap;
+  const bool ee = c.ee_cap <= 0 || c.scal[4] > c.ee_cap;
+  if (!vf && !ee) return;
   const int nblk = off[k + 1] - off[k];
-  const int t0 = (blockIdx.x - off[k]) * kBlock + threadIdx.x, stride = nblk * kBlock;
   const RMesh cm{c.cloth, MeshState{x + c.vbeg, x0 + c.vbeg, -1.0f}};
   const RMesh om{c.obs, MeshState{c.o_x, c.o_x0, 1e-3f}};
-  const int nee = c.scal[4];
-  if (c.ee_cap > 0 && nee <= c.ee_cap) {
-    for (int q = t0; q < nee; q += stride) RepulsionEeItemBody(cm, om, c.rep_ee[q], h, c.dq, c.cnt);
-  } else {
-    const int total = min(c.scal[2], c.pair_cap);
-    const int nobs = min(c.scal[1], total);
-    for (int p = t0; p < total; p += stride)
-      RepulsionPairEeBody(cm, om, c.pairs[p], p >= nobs, h, c.dq, c.cnt);
+  const int total = min(c.scal[2], c.pair_cap);
+  const int nobs = min(c.scal[1], total);
+  for (int p = (blockIdx.x - off[k]) * kBlock + threadIdx.x; p < total; p += nblk * kBlock) {
+    if (vf) RepulsionPairVfBody(cm, om, c.pairs[p], p >= nobs, h, c.dq, c.cnt);
+    if (ee) RepulsionPairEeBody(cm, om, c.pairs[p], p >= nobs, h, c.dq, c.cnt);
   }
 }
 
@@ -1011,8 +1022,9 @@ hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& 
 hipError_t MultiRepulsion(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
                           const float4* x0, float h, float relax) {
   if (Empty(m, MK::kPair)) return hipSuccess;
-  MultiRepVfKernel<<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
-  MultiRepEeKernel<<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
+  MultiRepItemKernel<0><<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
+  MultiRepItemKernel<1><<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
+  MultiRepPairsKernel<<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
   MultiApplyKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, relax);
   return hipGetLastError();
 }
