@@ -35,6 +35,7 @@ struct BvhView {
   uint32_t* counts = nullptr;   // pair traversal: pairs per lhs leaf
   uint32_t* offsets = nullptr;  //                 their exclusive prefix
   int32_t* total = nullptr;     //                 and their sum (1 int)
+  Box* packed = nullptr;        // traversal records (PackNodeBody), batched pair refresh
 };
 
 // Owning storage of one LBVH plus its build scratch.
@@ -53,6 +54,7 @@ struct Lbvh {
   DevBuf<uint32_t> root;
   DevBuf<uint32_t> hist;  // radix histogram / offsets
   DevBuf<int32_t> counts, offsets, total;  // pair traversal scratch (per lhs leaf)
+  DevBuf<Box> packed;                      // traversal records of the nodes
   BvhView view() const;
   // (re)allocate for a mesh; uploads faces (host int32[3*nf])
   hipError_t Setup(const int32_t* tris_host, int nf, int nv, hipStream_t s);

@@ -421,6 +421,7 @@ BvhView Lbvh::view() const {
   t.counts = reinterpret_cast<uint32_t*>(counts.ptr);
   t.offsets = reinterpret_cast<uint32_t*>(offsets.ptr);
   t.total = total.ptr;
+  t.packed = packed.ptr;
   return t;
 }
 
@@ -446,6 +447,7 @@ hipError_t Lbvh::Setup(const int32_t* tris_host, int nfaces, int nverts, hipStre
   if ((e = nodes.Alloc(nn)) != hipSuccess) return e;
   if ((e = box.Alloc(nn)) != hipSuccess) return e;
   if ((e = escape.Alloc(nn)) != hipSuccess) return e;
+  if ((e = packed.Alloc(nn)) != hipSuccess) return e;
   if ((e = flags.Alloc(std::max(nf, 1))) != hipSuccess) return e;
   if ((e = root.Alloc(6)) != hipSuccess) return e;
   if ((e = hist.Alloc(static_cast<size_t>(256) * nblocks)) != hipSuccess) return e;

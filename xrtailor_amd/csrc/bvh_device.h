@@ -213,6 +213,40 @@ __device__ __forceinline__ uint32_t CountStageBody(const BvhView& l, const BvhVi
   return c;
 }
 
+// Traversal record of node i for the pair traversal: its box with lo.w = left child
+// (internal node) or -(object + 1) (leaf) and hi.w = escape, so that one 32-byte load serves
+// a visit.  Written from nodes / box / escape after the boxes change.
+__device__ __forceinline__ void PackNodeBody(const BvhView& t, int i) {
+  const int4 nd = t.nodes[i];
+  Box b = t.box[i];
+  b.lo.w = __int_as_float(nd.w >= 0 ? -(nd.w + 1) : nd.y);
+  b.hi.w = __int_as_float(t.escape[i]);
+  t.packed[i] = b;
+}
+
+// the stackless traversal of CountPairsBody / FindPairsBody over packed records: on_leaf
+// (rhs object) for every rhs leaf overlapping lb dilated by th, in the same order
+template <class OnLeaf>
+__device__ __forceinline__ void TraversePacked(const Box* __restrict__ pk, int nf, const Box& lb,
+                                               float th, OnLeaf&& on_leaf) {
+  int node = 0;
+  const int max_steps = 2 * nf;  // each node is visited at most once
+  for (int step = 0; node != -1 && step < max_steps; ++step) {
+    const Box b = pk[node];
+    const int code = __float_as_int(b.lo.w), esc = __float_as_int(b.hi.w);
+    if (OverlapDilated(b, lb, th)) {
+      if (code < 0) {
+        on_leaf(-code - 1);
+        node = esc;
+      } else {
+        node = code;
+      }
+    } else {
+      node = esc;
+    }
+  }
+}
+
 // FindPairs_Kernel (BVH.cu:1189-1227): leaf id's pairs fill [lo, lo + n) backwards; only
 // slots below cap are written
 __device__ __forceinline__ void FindPairsBody(const BvhView& l, const BvhView& r, float th, int id,

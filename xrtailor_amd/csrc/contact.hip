@@ -466,43 +466,36 @@ __device__ __forceinline__ void RepulsionPairBody(const RMesh& c, const RMesh& o
   RepulsionPairEeBody(c, o, pr, self, h, dq, cnt);
 }
 
-// Test items of one pair: the owned VF tests (cloth vertex, other face) whose vertex is not
-// a corner of the other face, and the owned EE tests (cloth edge, other edge) that share no
-// vertex -- exactly the tests RepulsionPairVfBody / RepulsionPairEeBody evaluate past their
-// adjacency checks.  emit(is_ee, item) is called once per item; bit 31 of item.y marks a
-// cloth-vs-cloth item.
-constexpr int kSelfBit = static_cast<int>(0x80000000u);
-template <class Emit>
-__device__ __forceinline__ void PairItems(const RMesh& c, const RMesh& o, int2 pr, bool self,
-                                          Emit&& emit) {
-  const RMesh& other = self ? c : o;
-  if (!PairValid(c, other, pr)) return;
-  const int tag = self ? kSelfBit : 0;
-  const uint32_t r0 = c.t.rtri[pr.x], r1 = other.t.rtri[pr.y];
-  const int4 f0 = c.t.tris[pr.x], f1 = other.t.tris[pr.y];
-  const int v0[3] = {f0.x, f0.y, f0.z};
+// Test items of one pair as a mask: bit i (0..2) for the VF test of vertex i of face pr.x,
+// bit 3 + 3i + j for the EE test of edge i of face pr.x against edge j of face pr.y.  These
+// are the owned tests that RepulsionPairVfBody / RepulsionPairEeBody evaluate past their
+// adjacency checks: edge l of a face joins its vertices l and l + 1 (face_edges), so the
+// shared-vertex tests need only the two faces' vertices.
+__device__ __forceinline__ uint32_t PairItemMask(int4 f0, uint32_t r0, int4 f1, uint32_t r1,
+                                                 bool self) {
+  const int v0[3] = {f0.x, f0.y, f0.z}, v1[3] = {f1.x, f1.y, f1.z};
+  uint32_t m = 0;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    if (!((r0 >> i) & 1u)) continue;
-    if (self && (v0[i] == f1.x || v0[i] == f1.y || v0[i] == f1.z)) continue;
-    emit(false, make_int2(v0[i], pr.y | tag));
-  }
-  if (!((r0 >> 3) & 7u) || !((r1 >> 3) & 7u)) return;
-  const int4 fe0 = c.t.face_edges[pr.x], fe1 = other.t.face_edges[pr.y];
-  const int e0[3] = {fe0.x, fe0.y, fe0.z}, e1[3] = {fe1.x, fe1.y, fe1.z};
+  for (int i = 0; i < 3; ++i)
+    if (((r0 >> i) & 1u) && !(self && (v0[i] == v1[0] || v0[i] == v1[1] || v0[i] == v1[2])))
+      m |= 1u << i;
+  if (!((r0 >> 3) & 7u) || !((r1 >> 3) & 7u)) return m;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     if (!((r0 >> (3 + i)) & 1u)) continue;
-    const int2 ea = c.t.edges[e0[i]];
+    const int a0 = v0[i], a1 = v0[(i + 1) % 3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       if (!((r1 >> (3 + j)) & 1u)) continue;
-      const int2 eb = other.t.edges[e1[j]];
-      if (self && (ea.x == eb.x || ea.x == eb.y || ea.y == eb.x || ea.y == eb.y)) continue;
-      emit(true, make_int2(e0[i], e1[j] | tag));
+      const int b0 = v1[j], b1 = v1[(j + 1) % 3];
+      if (self && (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1)) continue;
+      m |= 1u << (3 + 3 * i + j);
     }
   }
+  return m;
 }
+
+constexpr int kSelfBit = static_cast<int>(0x80000000u);  // item.y: cloth-vs-cloth item
 
 // one VF item: cloth vertex it.x against face it.y of the obstacle or the cloth
 // (the two meshes are separate inlined calls: selecting one of two structs by reference
@@ -727,32 +720,85 @@ __global__ __launch_bounds__(1024) void MultiClothBottomUpKernel(const InstConta
   if (c.scal && c.nf > 1) BottomUpTreeWG(c.cbvh.nodes, c.cbvh.box, c.cbvh.flags, c.nf, 1);
 }
 
-// pairs per cloth leaf against the obstacle (self = 0) or the cloth itself (self = 1)
+// traversal records of every cloth tree (thread i: leaf nf - 1 + i and internal node i)
+__global__ __launch_bounds__(kBlock) void MultiPackClothKernel(const InstContactDev* ic,
+                                                               const int32_t* off, int n) {
+  XRT_MULTI_INDEX(kFace, c.nf)
+  PackNodeBody(c.cbvh, c.nf - 1 + i);
+  if (i < c.nf - 1) PackNodeBody(c.cbvh, i);
+}
+
+// traversal records of every obstacle tree
+__global__ __launch_bounds__(kBlock) void MultiPackObsKernel(const InstContactDev* ic,
+                                                             const int32_t* off, int n) {
+  XRT_MULTI_INDEX(kObsFace, c.obs_nf)
+  if (!c.has_obs) return;
+  PackNodeBody(c.obvh, c.obs_nf - 1 + i);
+  if (i < c.obs_nf - 1) PackNodeBody(c.obvh, i);
+}
+
+// pairs per cloth leaf against the obstacle (self = 0) or the cloth itself (self = 1),
+// traversing the packed records, with the repulsion item counts of those pairs when the
+// item lists are on; the first kPairStage hits of leaf i are kept at stage[j * nf + i] as
+// (rhs object, item mask) (when the instance has a stage)
 __global__ __launch_bounds__(kBlock) void MultiCountPairsKernel(const InstContactDev* ic,
                                                                 const int32_t* off, int n,
                                                                 float th, int self) {
   XRT_MULTI_INDEX(kFace, c.nf)
-  uint32_t v = 0;
+  uint32_t v = 0, nvf = 0, nee = 0;
   if (self || c.has_obs) {
-    const BvhView& r = self ? c.cbvh : c.obvh;
-    v = c.stage ? CountStageBody(c.cbvh, r, th, i, c.stage + i, c.nf, kPairStage)
-                : CountPairsBody(c.cbvh, r, th, i);
+    const Box lb = c.cbvh.packed[c.nf - 1 + i];
+    const int lobj = -__float_as_int(lb.lo.w) - 1;
+    const bool items = c.ee_cap > 0;
+    const MeshTopo& ot = self ? c.cloth : c.obs;
+    int4 f0 = make_int4(0, 0, 0, 0);
+    uint32_t r0 = 0;
+    if (items) {
+      f0 = c.cloth.tris[lobj];
+      r0 = c.cloth.rtri[lobj];
+    }
+    int2* st = c.stage + i;
+    const int stride = c.nf;
+    TraversePacked(self ? c.cbvh.packed : c.obvh.packed, self ? c.nf : c.obs_nf, lb, th,
+                   [&](int robj) {
+                     uint32_t m = 0;
+                     if (items) {
+                       m = PairItemMask(f0, r0, ot.tris[robj], ot.rtri[robj], self != 0);
+                       nvf += __popc(m & 7u);
+                       nee += __popc(m >> 3);
+                     }
+                     if (c.stage && v < static_cast<uint32_t>(kPairStage))
+                       st[static_cast<int64_t>(v) * stride] = make_int2(robj, static_cast<int>(m));
+                     ++v;
+                   });
   }
   c.cbvh.counts[i] = v;
   c.cbvh.offsets[i] = v;
+  if (c.ee_cap > 0) {
+    c.ivf[i] = nvf;
+    c.iee[i] = nee;
+  }
 }
 
 // one workgroup per instance: exclusive scan of the per-leaf counts; the sum goes to
-// scal[1] (obstacle pass) or to the tree's total (self pass)
+// scal[1] (obstacle pass) or to the tree's total (self pass); likewise the item counts,
+// whose sums go to scal[5], scal[6] (obstacle pass) or scal[7], scal[8]
 __global__ __launch_bounds__(1024) void MultiScanKernel(const InstContactDev* ic, int n, int self) {
   const InstContactDev& c = ic[blockIdx.x];
   if (c.nf <= 0 || !c.scal) return;
   BlockExclusiveScan(c.cbvh.offsets, c.nf, self ? c.cbvh.total : c.scal + 1);
+  if (c.ee_cap <= 0) return;
+  __syncthreads();  // the scans share their workgroup buffer
+  BlockExclusiveScan(c.ivf, c.nf, c.scal + (self ? 7 : 5));
+  __syncthreads();
+  BlockExclusiveScan(c.iee, c.nf, c.scal + (self ? 8 : 6));
 }
 
 // FindPairs of every instance's leaves (pairs of leaf i fill [lo, lo + n) backwards, slots
 // below the capacity only): copied from the count pass's stage when the leaf had at most
-// kPairStage hits, else found by a second traversal
+// kPairStage hits, else found by a second traversal.  With the item lists on, the items of
+// leaf i's pairs (hit order) go to the lists from its prefix offsets, the cloth-pair items
+// after all obstacle-pair items; only slots below the list capacities are written.
 __global__ __launch_bounds__(kBlock) void MultiEmitPairsKernel(const InstContactDev* ic,
                                                                const int32_t* off, int n, float th,
                                                                int self) {
@@ -761,13 +807,57 @@ __global__ __launch_bounds__(kBlock) void MultiEmitPairsKernel(const InstContact
   const int64_t cap = c.pair_cap;
   const uint32_t cnt = c.cbvh.counts[i];
   const int64_t lo = static_cast<int64_t>(c.scal[self ? 1 : 0]) + c.cbvh.offsets[i];
-  if (!c.stage || cnt > static_cast<uint32_t>(kPairStage)) {
-    FindPairsBody(c.cbvh, self ? c.cbvh : c.obvh, th, i, lo, cnt, c.pairs, cap);
+  const Box lb = c.cbvh.packed[c.nf - 1 + i];
+  const int lobj = -__float_as_int(lb.lo.w) - 1;
+  const bool items = c.ee_cap > 0;
+  const MeshTopo& ot = self ? c.cloth : c.obs;
+  const int tag = self ? kSelfBit : 0;
+  int4 f0 = make_int4(0, 0, 0, 0), fe0 = make_int4(0, 0, 0, 0);
+  uint32_t r0 = 0;
+  int64_t vo = 0, eo = 0;
+  if (items) {
+    f0 = c.cloth.tris[lobj];
+    fe0 = c.cloth.face_edges[lobj];
+    r0 = c.cloth.rtri[lobj];
+    vo = (self ? static_cast<int64_t>(c.scal[5]) : 0) + c.ivf[i];
+    eo = (self ? static_cast<int64_t>(c.scal[6]) : 0) + c.iee[i];
+  }
+  auto emit_items = [&](int robj, uint32_t m) {
+    if (!m) return;
+    const int v0[3] = {f0.x, f0.y, f0.z};
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      if ((m >> q) & 1u) {
+        if (vo < c.vf_cap) c.rep_vf[vo] = make_int2(v0[q], robj | tag);
+        ++vo;
+      }
+    if (!(m >> 3)) return;
+    const int4 fe1 = ot.face_edges[robj];
+    const int e0[3] = {fe0.x, fe0.y, fe0.z}, e1[3] = {fe1.x, fe1.y, fe1.z};
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+      if ((m >> (3 + q)) & 1u) {
+        if (eo < c.ee_cap) c.rep_ee[eo] = make_int2(e0[q / 3], e1[q % 3] | tag);
+        ++eo;
+      }
+  };
+  if (!c.stage || cnt > static_cast<uint32_t>(kPairStage)) {  // no stage when self contact
+    int64_t index = lo + cnt - 1;                              // came on after start
+    TraversePacked(self ? c.cbvh.packed : c.obvh.packed, self ? c.nf : c.obs_nf, lb, th,
+                   [&](int robj) {
+                     if (index >= lo && index < cap) c.pairs[index] = make_int2(lobj, robj);
+                     --index;
+                     if (items)
+                       emit_items(robj, PairItemMask(f0, r0, ot.tris[robj], ot.rtri[robj],
+                                                     self != 0));
+                   });
     return;
   }
   for (uint32_t j = 0; j < cnt; ++j) {
+    const int2 e = c.stage[static_cast<int64_t>(j) * c.nf + i];
     const int64_t idx = lo + cnt - 1 - j;
-    if (idx < cap) c.pairs[idx] = c.stage[static_cast<int64_t>(j) * c.nf + i];
+    if (idx < cap) c.pairs[idx] = make_int2(lobj, e.x);
+    if (items) emit_items(e.x, static_cast<uint32_t>(e.y));
   }
 }
 
@@ -775,67 +865,22 @@ __global__ __launch_bounds__(kBlock) void MultiAddTotalsKernel(const InstContact
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n || !ic[k].scal) return;
   ic[k].scal[2] = ic[k].scal[1] + *ic[k].cbvh.total;
-  ic[k].scal[3] = 0;  // item counts, refilled by MultiRepItemsKernel
-  ic[k].scal[4] = 0;
+  if (ic[k].ee_cap > 0) {
+    ic[k].scal[3] = ic[k].scal[5] + ic[k].scal[7];
+    ic[k].scal[4] = ic[k].scal[6] + ic[k].scal[8];
+  }
 }
 
-// inclusive prefix sum over the 64 lanes of a wave
-__device__ __forceinline__ int WaveInclusiveScan(int v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int t = __shfl_up(v, d, 64);
-    if (lane >= d) v += t;
-  }
-  return v;
-}
-
-// pair refresh: every instance's pairs -> its VF and EE item lists (scal[3], scal[4] hold
-// the item counts, which may exceed the capacities; the lists then stay unused).  Slots are
-// reserved with one atomic per wave; the item order is free because the repulsion sums in
-// fixed point.
-__global__ __launch_bounds__(kBlock) void MultiRepItemsKernel(const InstContactDev* ic,
-                                                              const int32_t* off, int n) {
-  const int k = InstOfBlock(off, n, blockIdx.x);
-  const InstContactDev& c = ic[k];
-  if (c.ee_cap <= 0) return;  // item lists off for this instance
-  const int nblk = off[k + 1] - off[k];
-  const int total = min(c.scal[2], c.pair_cap);
-  const int nobs = min(c.scal[1], total);
-  const RMesh cm{c.cloth, MeshState{}};
-  const RMesh om{c.obs, MeshState{}};
-  const int lane = threadIdx.x & 63;
-  // the loop bound is uniform over the block, so every lane reaches the shuffles
-  for (int b = (blockIdx.x - off[k]) * kBlock; b < total; b += nblk * kBlock) {
-    const int p = b + threadIdx.x;
-    int2 pr = make_int2(-1, -1);
-    if (p < total) pr = c.pairs[p];
-    const bool self = p >= nobs;
-    int nvf = 0, nee = 0;
-    PairItems(cm, om, pr, self, [&](bool is_ee, int2) { if (is_ee) ++nee; else ++nvf; });
-    const int vf_incl = WaveInclusiveScan(nvf), ee_incl = WaveInclusiveScan(nee);
-    const int vf_tot = __shfl(vf_incl, 63, 64), ee_tot = __shfl(ee_incl, 63, 64);
-    int vf_base = 0, ee_base = 0;
-    if (lane == 63) {
-      if (vf_tot) vf_base = atomicAdd(c.scal + 3, vf_tot);
-      if (ee_tot) ee_base = atomicAdd(c.scal + 4, ee_tot);
-    }
-    int vo = __shfl(vf_base, 63, 64) + vf_incl - nvf;
-    int eo = __shfl(ee_base, 63, 64) + ee_incl - nee;
-    PairItems(cm, om, pr, self, [&](bool is_ee, int2 it) {
-      if (is_ee) {
-        if (eo < c.ee_cap) c.rep_ee[eo] = it;
-        ++eo;
-      } else {
-        if (vo < c.vf_cap) c.rep_vf[vo] = it;
-        ++vo;
-      }
-    });
-  }
+// the item lists of instance c are complete and used for VF (is_ee = 0) or EE tests: lists
+// on, every pair stored and every item within the list
+__device__ __forceinline__ bool ItemsUsable(const InstContactDev& c, int is_ee) {
+  return c.ee_cap > 0 && c.scal[2] <= c.pair_cap &&
+         c.scal[is_ee ? 4 : 3] <= (is_ee ? c.ee_cap : c.vf_cap);
 }
 
 // repulsion tests of every instance, one VF (is_ee = 0) or EE item per thread; instances
-// whose list overflowed (or is off) are left to MultiRepPairsKernel
+// whose list is not usable (off, overflowed, or pairs beyond the pair capacity) are left to
+// MultiRepPairsKernel
 template <int is_ee>
 __global__ __launch_bounds__(kBlock) void MultiRepItemKernel(const InstContactDev* ic,
                                                              const int32_t* off, int n,
@@ -844,8 +889,8 @@ __global__ __launch_bounds__(kBlock) void MultiRepItemKernel(const InstContactDe
                                                              float h) {
   const int k = InstOfBlock(off, n, blockIdx.x);
   const InstContactDev& c = ic[k];
+  if (!ItemsUsable(c, is_ee)) return;
   const int cnt = c.scal[is_ee ? 4 : 3];
-  if (c.ee_cap <= 0 || cnt > (is_ee ? c.ee_cap : c.vf_cap)) return;
   const int nblk = off[k + 1] - off[k];
   const RMesh cm{c.cloth, MeshState{x + c.vbeg, x0 + c.vbeg, -1.0f}};
   const RMesh om{c.obs, MeshState{c.o_x, c.o_x0, 1e-3f}};  // obstacle nodes: inv mass 1e-3
@@ -856,7 +901,7 @@ __global__ __launch_bounds__(kBlock) void MultiRepItemKernel(const InstContactDe
   }
 }
 
-// the per-pair VF and / or EE halves of the instances whose item lists overflowed or are off
+// the per-pair VF and / or EE halves of the instances whose item lists are not usable
 __global__ __launch_bounds__(kBlock) void MultiRepPairsKernel(const InstContactDev* ic,
                                                               const int32_t* off, int n,
                                                               const float4* __restrict__ x,
@@ -864,8 +909,8 @@ __global__ __launch_bounds__(kBlock) void MultiRepPairsKernel(const InstContactD
                                                               float h) {
   const int k = InstOfBlock(off, n, blockIdx.x);
   const InstContactDev& c = ic[k];
-  const bool vf = c.ee_cap <= 0 || c.scal[3] > c.vf_cap;
-  const bool ee = c.ee_cap <= 0 || c.scal[4] > c.ee_cap;
+  const bool vf = !ItemsUsable(c, 0);
+  const bool ee = !ItemsUsable(c, 1);
   if (!vf && !ee) return;
   const int nblk = off[k + 1] - off[k];
   const RMesh cm{c.cloth, MeshState{x + c.vbeg, x0 + c.vbeg, -1.0f}};
@@ -1008,14 +1053,15 @@ hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& 
   MultiClothPredKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x);
   MultiClothGrowKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, x, x0);
   MultiClothBottomUpKernel<<<m.n, 1024, 0, s>>>(ic, m.n);
+  MultiPackClothKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n);
+  if (!Empty(m, MK::kObsFace))
+    MultiPackObsKernel<<<m.blocks[MK::kObsFace], kBlock, 0, s>>>(ic, m.off[MK::kObsFace], m.n);
   for (int self = 0; self < 2; ++self) {
     MultiCountPairsKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
     MultiScanKernel<<<m.n, 1024, 0, s>>>(ic, m.n, self);
     MultiEmitPairsKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
   }
   MultiAddTotalsKernel<<<Blocks(m.n), kBlock, 0, s>>>(ic, m.n);
-  if (!Empty(m, MK::kPair))
-    MultiRepItemsKernel<<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n);
   return hipGetLastError();
 }
 

@@ -144,8 +144,8 @@ int ContactStart(xrt_ctx* ctx) {
     const int per_face = std::max(1, ctx->params.pair_capacity_per_face);
     c->pair_cap = std::max(1, per_face * h.nt);
     XRT_CHECK_HIP(c->pairs.Alloc(c->pair_cap));
-    XRT_CHECK_HIP(c->scal.Alloc(8));
-    XRT_CHECK_HIP(hipMemsetAsync(c->scal.ptr, 0, sizeof(int32_t) * 8, s));
+    XRT_CHECK_HIP(c->scal.Alloc(16));
+    XRT_CHECK_HIP(hipMemsetAsync(c->scal.ptr, 0, sizeof(int32_t) * 16, s));
     // repulsion item lists (at most 3 VF and 9 EE items per pair; an instance whose items
     // overflow a list falls back to the per-pair tests for that half); off when either
     // capacity is 0 or the counts could overflow an int
@@ -157,6 +157,8 @@ int ContactStart(xrt_ctx* ctx) {
       c->ee_cap = static_cast<int>(std::min<int64_t>(ee_cap, 9 * static_cast<int64_t>(c->pair_cap)));
       XRT_CHECK_HIP(c->rep_vf.Alloc(c->vf_cap));
       XRT_CHECK_HIP(c->rep_ee.Alloc(c->ee_cap));
+      XRT_CHECK_HIP(c->ivf.Alloc(std::max(h.nt, 1)));
+      XRT_CHECK_HIP(c->iee.Alloc(std::max(h.nt, 1)));
     }
     if (ctx->params.enable_self_contact)
       XRT_CHECK_HIP(c->stage.Alloc(static_cast<size_t>(kPairStage) * std::max(h.nt, 1)));
@@ -328,6 +330,8 @@ int MultiContactSetup(xrt_ctx* ctx) {
       d.rep_vf = c.rep_vf.ptr;
       d.rep_ee = c.rep_ee.ptr;
       d.stage = c.stage.ptr;
+      d.ivf = c.ivf.ptr;
+      d.iee = c.iee.ptr;
       cnt[MultiMaps::kVert] = blocks(h.nv);
       cnt[MultiMaps::kEdge] = blocks(h.ne());
       cnt[MultiMaps::kFace] = blocks(h.nt);

@@ -79,10 +79,11 @@ struct ContactData {
   int pair_cap = 0;
   DevBuf<int2> pairs;
   DevBuf<int32_t> scal;  // [0] zero, [1] obstacle-pair count, [2] total pair count,
-                         // [3] / [4] repulsion VF / EE item counts (batched path)
+                         // [3..8] repulsion item counts (batched path, InstContactDev)
   // repulsion test items of the batched path (capacities 0: lists off)
   int vf_cap = 0, ee_cap = 0;
   DevBuf<int2> rep_vf, rep_ee;
+  DevBuf<uint32_t> ivf, iee;  // per cloth leaf item counts / offsets
   DevBuf<int2> stage;  // kPairStage hits per cloth leaf of the batched pair refresh
   int frame_substep = 0, pass_cnt = 0;
   bool obstacle_moving = false;  // xrt_update_obstacle since the last frame end
