@@ -109,7 +109,8 @@ struct InstContactDev {
 constexpr int kPairStage = 32;
 
 // block -> instance maps: per kind, the prefix sums (n + 1 entries, device) of the
-// 256-thread blocks each instance needs, and their totals (host)
+// 256-thread blocks each instance needs, followed by the instance of each block (device),
+// and the block totals (host)
 struct MultiMaps {
   enum Kind { kVert, kEdge, kFace, kObsVert, kObsFace, kPair, kKinds };
   const int32_t* off[kKinds] = {};

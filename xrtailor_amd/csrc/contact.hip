@@ -619,13 +619,9 @@ __global__ void AddTotalsKernel(const int32_t* a, const int32_t* b, int32_t* out
 // -------------------------------------------------------- multi-instance kernels
 // blockIdx.x -> (instance, block within the instance) through a prefix table of block
 // counts (instances that need no block are skipped by the search)
+// the instance of block b: the map's block table follows its n + 1 prefix sums
 __device__ __forceinline__ int InstOfBlock(const int32_t* __restrict__ off, int n, int b) {
-  int lo = 0, hi = n;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (off[mid] <= b) lo = mid; else hi = mid;
-  }
-  return lo;
+  return off[n + 1 + b];
 }
 
 #define XRT_MULTI_INDEX(kind, count_expr)                                  \
@@ -1070,7 +1066,8 @@ hipError_t MultiRepulsion(hipStream_t s, const InstContactDev* ic, const MultiMa
   if (Empty(m, MK::kPair)) return hipSuccess;
   MultiRepItemKernel<0><<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
   MultiRepItemKernel<1><<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
-  MultiRepPairsKernel<<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
+  // the fallback runs on the smaller per-vertex grid: most of its blocks exit at once
+  MultiRepPairsKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, x0, h);
   MultiApplyKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, relax);
   return hipGetLastError();
 }

@@ -345,11 +345,20 @@ int MultiContactSetup(xrt_ctx* ctx) {
       o[k + 1] = o[k] + cnt[kind];
     }
   }
+  // device layout per kind: the n + 1 prefix sums, then the instance of every block
+  std::vector<int32_t> maps;
+  size_t seg[MultiMaps::kKinds];
+  for (int kind = 0; kind < MultiMaps::kKinds; ++kind) {
+    const int32_t* o = off.data() + static_cast<size_t>(kind) * (n + 1);
+    seg[kind] = maps.size();
+    maps.insert(maps.end(), o, o + n + 1);
+    for (int k = 0; k < n; ++k) maps.insert(maps.end(), static_cast<size_t>(o[k + 1] - o[k]), k);
+  }
   XRT_CHECK_HIP(ctx->ic.Upload(ic, ctx->stream));
-  XRT_CHECK_HIP(ctx->ic_off.Upload(off, ctx->stream));
+  XRT_CHECK_HIP(ctx->ic_off.Upload(maps, ctx->stream));
   ctx->maps.n = n;
   for (int kind = 0; kind < MultiMaps::kKinds; ++kind) {
-    ctx->maps.off[kind] = ctx->ic_off.ptr + static_cast<size_t>(kind) * (n + 1);
+    ctx->maps.off[kind] = ctx->ic_off.ptr + seg[kind];
     ctx->maps.blocks[kind] = off[static_cast<size_t>(kind) * (n + 1) + n];
   }
   XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
