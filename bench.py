@@ -6,16 +6,23 @@ in [32, 192]), per-instance stretch / bend compliance log-uniform in [1e-9, 1e-6
 [1e-5, 1e-3], 10 substeps x 10 iterations per frame at 60 fps, ground plane, FP32,
 synthetic from seed.  A step is one frame (10 substeps) of all instances.
 
-value        batched vertex-substeps/s (one persistent launch per substep, one hipGraph
-             per frame), inputs resident in HBM
-plain_value  the same workload one instance at a time through per-color launches
-roofline     dominant kernel = the batched substep kernel; algorithmic bytes per launch
-             = sum_i (64 N_i + 16 E_i + 24 B_i + 12 K_i); HBM peak 8 TB/s
+value        batched vertex-substeps/s (one hipGraph per frame; every contact phase and
+             constraint color of all instances per launch; without contacts one persistent
+             launch per substep), inputs resident in HBM
+plain_value  the same workload one instance at a time through per-instance launches
+roofline     dominant kernel, per-launch device time from HIP events on the solver stream:
+             with contacts the EE repulsion item kernel, algorithmic bytes per launch
+             = 8 (EE items) + 16 N + 8 E (items, positions, edges read once); without
+             contacts the persistent substep kernel, sum_i (64 N_i + 16 E_i + 24 B_i +
+             12 K_i); HBM peak 8 TB/s.  traffic: HBM bytes per launch from the committed
+             rocprofv3 FETCH_SIZE / WRITE_SIZE summary of that kernel (profiles/traffic.json),
+             null when absent
 cpu_baseline the serial FP64 oracle on one core, one substep per instance over a bounded
              sample; the same leg reports the single-substep relative error of the GPU's
              batched substep against it (parity at the measured configuration)
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload batched|hanging]
+                    [--no-graph]   (un-captured substeps, for profiler runs)
 """
 from __future__ import annotations
 
@@ -29,6 +36,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, already scaled by the copy calibration)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for r in rec.get("kernels", []):
+        if r.get("kernel") == kernel and r.get("workload") == workload:
+            return r.get("bytes_per_launch")
+    return None
 
 
 def algorithmic_bytes(solver, insts) -> tuple[int, int]:
@@ -149,6 +171,8 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable rollout")
     ap.add_argument("--no-contacts", action="store_true",
                     help="batched workload without its obstacles and self contact")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time un-captured batched substeps instead of graph replays")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build measures one GPU only (instances are independent; "
@@ -175,18 +199,37 @@ def main() -> int:
     info = solver.device_info()
 
     # batched: warmup, then K timed frames (graph replays), HIP events on the solver stream
-    solver.FramesBatched(a.warmup)
-    solver.sync()
     t0 = time.perf_counter()
-    ms = solver.time_frames(True, a.steps)
+    if a.no_graph:
+        for _ in range(a.warmup * params.num_substeps):
+            solver.SubstepBatched()
+        solver.sync()
+        t0 = time.perf_counter()
+        nsub = a.steps * params.num_substeps
+        ms = solver.time_batched_substep(nsub) * nsub
+    else:
+        solver.FramesBatched(a.warmup)
+        solver.sync()
+        t0 = time.perf_counter()
+        ms = solver.time_frames(True, a.steps)
     wall = time.perf_counter() - t0
     value = n_verts * params.num_substeps * a.steps / (ms * 1e-3)
     print(f"[bench] batched {ms:.2f} ms / {a.steps} frames (wall {wall:.3f} s)",
           file=sys.stderr, flush=True)
 
     # dominant kernel: per-launch device time from HIP events around each launch
-    kern_ms = solver.time_batched_substep(max(10, a.steps))
-    achieved = bytes_sub / (kern_ms * 1e-3) / 1e9
+    rep_counts = None
+    if contacts:
+        kernel = "MultiRepItemKernel<1>"
+        kern_ms, rep_counts = solver.time_repulsion(2)
+        n_edges = sum(solver.counts(i).n_edges for i in insts)
+        bytes_kern = 8 * rep_counts["ee_items"] + 16 * n_verts + 8 * n_edges
+    else:
+        kernel = "BatchedSubstepKernel"
+        kern_ms = solver.time_batched_substep(max(10, a.steps))
+        bytes_kern = bytes_sub
+    achieved = bytes_kern / (kern_ms * 1e-3) / 1e9
+    traffic = measured_traffic(kernel, wname)
 
     # plain path on the same workload
     plain = None
@@ -229,15 +272,16 @@ def main() -> int:
         "config": {"workload": wname, "instances": len(specs), "vertices": n_verts,
                    "substeps": params.num_substeps, "iterations": params.num_iterations,
                    "bend_model": "consistent", "obstacle_contact": contacts,
-                   "self_contact": contacts, "step": "one frame of all instances"},
+                   "self_contact": contacts, "step": "one frame of all instances",
+                   "graph": not a.no_graph},
         "plain_value": plain,
         "batched_over_plain": (value / plain) if plain else None,
         "parity_rel_err": parity,
         "parity_instances": parity_n,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "BatchedSubstepKernel", "kernel_ms": kern_ms,
-                     "bytes_per_launch": bytes_sub},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": kernel, "kernel_ms": kern_ms, "bytes_per_launch": bytes_kern,
+                     "repulsion_counts": rep_counts},
         "cpu_baseline": cpu,
         "adjoint_forward_ratio": adj["substep_ratio"] if adj else None,
         "adjoint": adj,

@@ -231,6 +231,13 @@ int xrt_adjoint_substep(xrt_ctx* ctx);
  * current state, which is restored): ms[0] forward substep (wide launches), ms[1] taped
  * forward substep, ms[2] reverse substep */
 int xrt_time_adjoint(xrt_ctx* ctx, int32_t n, float ms[3]);
+/* Runs n batched substeps (not captured) with HIP events around every launch of the EE
+ * repulsion kernel (one edge-edge test item per thread over every instance's item list)
+ * and returns its average device time per launch; counts = EE items, VF items, stored
+ * pairs and instances on the item path, averaged over the n substeps' pair refreshes.  The
+ * state advances by n substeps.  No counterpart in the reference (measurement hook; the EE
+ * tests are EvaluateEEProximity, xrtailor/src/physics/repulsion/PBDRepulsion.cu:411-468). */
+int xrt_time_repulsion(xrt_ctx* ctx, int32_t n, float* avg_ms, int64_t counts[4]);
 
 /* ------------------------------------------------------------------- LBVH */
 /* Standalone linear BVH over triangle faces (Karras 2012), the reference's BVH class

@@ -121,3 +121,22 @@ def test_repulsion_item_list_fallback(vf_cap, ee_cap):
     moved = max(float(np.abs(a.GetState(i)[0] - c.GetState(j)[0]).max()) for i, j in zip(ia, ic))
     print(f"  repulsion moved the cloth by up to {moved:.3e} m", flush=True)
     assert moved > 0.0
+
+
+def test_time_repulsion_counts():
+    """The repulsion timing hook times the EE kernel inside real substeps and reports the
+    item counts of the pair refreshes; without self contact it refuses."""
+    print("case repulsion timing hook", flush=True)
+    xp = make_params(bend_model=1, enable_obstacle_contact=1, enable_self_contact=1,
+                     ground_height=-1.0)
+    specs = _specs(seed=14)
+    a, ia, _, _ = _pair(specs, xp)
+    ms, cnt = a.time_repulsion(2)
+    print(f"  {ms:.4f} ms per EE launch, {cnt}", flush=True)
+    assert ms > 0.0
+    assert cnt["ee_items"] > 0 and cnt["vf_items"] > 0 and cnt["pairs"] > 0
+    assert cnt["item_path_instances"] == len(specs)
+    b, ib = build_solver(specs, make_params(bend_model=1, enable_obstacle_contact=1,
+                                            ground_height=-1.0), seed=3)
+    with pytest.raises(RuntimeError):
+        b.time_repulsion(1)

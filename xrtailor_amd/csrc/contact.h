@@ -130,9 +130,10 @@ hipError_t MultiGenerate(hipStream_t s, const InstContactDev* ic, const MultiMap
 hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& m,
                       const float4* x, const float4* x0, float thickness);
 // PBD repulsion over every instance's pairs (one VF or EE test item per thread), applied
-// x relax
+// x relax; events (if not null) are recorded around the EE kernel
 hipError_t MultiRepulsion(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
-                          const float4* x0, float h, float relax);
+                          const float4* x0, float h, float relax, hipEvent_t ee_begin = nullptr,
+                          hipEvent_t ee_end = nullptr);
 // VF-CCD (offset 0) then VF-DCD (offset 1e-3) solves
 hipError_t MultiVfSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x);
 // EE-DCD deltas applied x relax

@@ -1062,10 +1062,19 @@ hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& 
 }
 
 hipError_t MultiRepulsion(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
-                          const float4* x0, float h, float relax) {
+                          const float4* x0, float h, float relax, hipEvent_t ee_begin,
+                          hipEvent_t ee_end) {
   if (Empty(m, MK::kPair)) return hipSuccess;
   MultiRepItemKernel<0><<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
+  if (ee_begin) {
+    const hipError_t e = hipEventRecord(ee_begin, s);
+    if (e != hipSuccess) return e;
+  }
   MultiRepItemKernel<1><<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
+  if (ee_end) {
+    const hipError_t e = hipEventRecord(ee_end, s);
+    if (e != hipSuccess) return e;
+  }
   // the fallback runs on the smaller per-vertex grid: most of its blocks exit at once
   MultiRepPairsKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, x0, h);
   MultiApplyKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, relax);

@@ -420,8 +420,14 @@ int MultiContactIterPre(xrt_ctx* ctx, int it, bool* pass_ran) {
   if (p.enable_self_contact) {
     if (it % interval == 0)
       XRT_CHECK_HIP(MultiPairs(s, ctx->ic.ptr, ctx->maps, ctx->x.ptr, ctx->x0.ptr, kPairThickness));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->rep_next + 1 < ctx->rep_events.size()) {
+      e0 = ctx->rep_events[ctx->rep_next];
+      e1 = ctx->rep_events[ctx->rep_next + 1];
+      ctx->rep_next += 2;
+    }
     XRT_CHECK_HIP(MultiRepulsion(s, ctx->ic.ptr, ctx->maps, ctx->x.ptr, ctx->x0.ptr, p.pbd_thickness,
-                                 p.repulsion_relaxation));
+                                 p.repulsion_relaxation, e0, e1));
   }
   return XRT_OK;
 }
@@ -494,6 +500,61 @@ int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendM
 using namespace xrt;
 
 extern "C" {
+
+int xrt_time_repulsion(xrt_ctx* ctx, int32_t n, float* avg_ms, int64_t counts[4]) {
+  XRT_REQUIRE(ctx && ctx->started, XRT_ERR_STATE, "xrt_start has not been called");
+  XRT_REQUIRE(n > 0 && avg_ms && counts, XRT_ERR_INVALID_ARG, "n must be > 0, outputs non-null");
+  XRT_REQUIRE(ctx->params.enable_self_contact && ContactsActive(ctx), XRT_ERR_STATE,
+              "the repulsion runs with self contact only");
+  XRT_REQUIRE(ctx->rep_events.empty(), XRT_ERR_STATE, "repulsion timing already running");
+  hipStream_t s = ctx->stream;
+  const size_t pairs = static_cast<size_t>(n) * std::max(1, ctx->params.num_iterations);
+  ctx->rep_events.assign(2 * pairs, nullptr);
+  ctx->rep_next = 0;
+  int rc = XRT_OK;
+  for (auto& e : ctx->rep_events)
+    if (hipEventCreate(&e) != hipSuccess) rc = XRT_ERR_HIP;
+  int64_t sums[4] = {0, 0, 0, 0};
+  for (int sub = 0; sub < n && rc == XRT_OK; ++sub) {
+    rc = xrt_substep_batched(ctx);
+    if (rc != XRT_OK) break;
+    // item counts of this substep's last pair refresh
+    for (auto& c : ctx->contact) {
+      if (!c) continue;
+      int32_t sc[9];
+      if (hipMemcpyAsync(sc, c->scal.ptr, sizeof(sc), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        rc = XRT_ERR_HIP;
+        break;
+      }
+      sums[0] += sc[4];
+      sums[1] += sc[3];
+      sums[2] += std::min(sc[2], c->pair_cap);
+      sums[3] += (c->ee_cap > 0 && sc[2] <= c->pair_cap && sc[4] <= c->ee_cap) ? 1 : 0;
+    }
+  }
+  float total = 0.0f;
+  const size_t used = ctx->rep_next / 2;
+  if (rc == XRT_OK && hipStreamSynchronize(s) != hipSuccess) rc = XRT_ERR_HIP;
+  for (size_t q = 0; q < used && rc == XRT_OK; ++q) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->rep_events[2 * q], ctx->rep_events[2 * q + 1]) != hipSuccess)
+      rc = XRT_ERR_HIP;
+    total += ms;
+  }
+  for (auto& e : ctx->rep_events)
+    if (e) (void)hipEventDestroy(e);
+  ctx->rep_events.clear();
+  ctx->rep_next = 0;
+  if (rc != XRT_OK) {
+    if (rc == XRT_ERR_HIP) SetError("HIP error while timing the repulsion");
+    return rc;
+  }
+  XRT_REQUIRE(used > 0, XRT_ERR_STATE, "no repulsion launch was timed");
+  for (int q = 0; q < 4; ++q) counts[q] = sums[q] / n;
+  *avg_ms = total / static_cast<float>(used);
+  return XRT_OK;
+}
 
 int xrt_set_obstacle(xrt_ctx* ctx, int32_t inst, const float* pos, int32_t nv, const int32_t* tris,
                      int32_t nt) {

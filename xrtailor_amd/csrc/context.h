@@ -165,6 +165,9 @@ struct xrt_ctx {
   hipGraphExec_t graph_exec = nullptr;
   // timing events
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // xrt_time_repulsion: event pairs recorded around the EE repulsion launches (next free pair)
+  std::vector<hipEvent_t> rep_events;
+  size_t rep_next = 0;
 };
 
 namespace xrt {

@@ -326,6 +326,15 @@ class ClothSolver:
         check(self.lib.xrt_adjoint_get_grad_state(self.ctx, inst, fptr(gx), fptr(gv)))
         return gx.reshape(-1, 3), gv.reshape(-1, 3)
 
+    def time_repulsion(self, n: int) -> tuple[float, dict]:
+        """ms per launch of the batched EE repulsion kernel over n un-captured batched
+        substeps (the state advances), and the item / pair counts averaged over them."""
+        ms = C.c_float()
+        cnt = (C.c_int64 * 4)()
+        check(self.lib.xrt_time_repulsion(self.ctx, n, C.byref(ms), cnt))
+        return float(ms.value), dict(ee_items=int(cnt[0]), vf_items=int(cnt[1]),
+                                     pairs=int(cnt[2]), item_path_instances=int(cnt[3]))
+
     def time_adjoint(self, n: int) -> tuple[float, float, float]:
         """ms per substep: (forward, taped forward, reverse) over every instance."""
         ms = (C.c_float * 3)()
