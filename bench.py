@@ -23,6 +23,8 @@ cpu_baseline the serial FP64 oracle on one core, one substep per instance over a
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload batched|hanging]
                     [--no-graph]   (un-captured substeps, for profiler runs)
+    python bench.py --copy-calibration   (5 device copies of 256 MiB: the known-size kernel
+                                          that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)
 """
 from __future__ import annotations
 
@@ -173,7 +175,19 @@ def main() -> int:
                     help="batched workload without its obstacles and self contact")
     ap.add_argument("--no-graph", action="store_true",
                     help="time un-captured batched substeps instead of graph replays")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only 5 device copies of 256 MiB (PMC byte-counter calibration)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        n = 64 << 20
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_copy": 4 * n, "copies": 5}), flush=True)
+        return 0
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build measures one GPU only (instances are independent; "
                                    "N GPUs would each run their own share): use --gpus 1"}))
