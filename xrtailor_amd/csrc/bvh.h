@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "common.h"
 
@@ -59,6 +60,47 @@ struct Lbvh {
   // (re)allocate for a mesh; uploads faces (host int32[3*nf])
   hipError_t Setup(const int32_t* tris_host, int nf, int nv, hipStream_t s);
 };
+
+// One tree of a multi-tree build: its buffers and the points it is built from.
+struct TreeBuildDev {
+  const float4* x0 = nullptr;
+  int nv = 0, nf = 0, nblocks = 0;
+  const int4* tris = nullptr;
+  float4* v = nullptr;
+  float4* pred = nullptr;
+  Box* leaf = nullptr;
+  uint32_t* morton = nullptr;
+  uint32_t* morton_tmp = nullptr;
+  int32_t* order = nullptr;
+  int32_t* order_tmp = nullptr;
+  uint64_t* code = nullptr;
+  int4* nodes = nullptr;
+  Box* box = nullptr;
+  int32_t* escape = nullptr;
+  uint32_t* root = nullptr;
+  uint32_t* hist = nullptr;
+  int32_t* flags = nullptr;
+};
+
+// A set of trees rebuilt together, every build step of every tree in one launch (the
+// frame-start build: BVH::UpdateData + BVH::Construct from x0, tolerance 0, no CCD).
+// Per kind of work the device maps hold the n + 1 prefix sums of the trees' block counts,
+// then the tree of every block.
+struct TreeSet {
+  enum Kind { kVert, kFace, kTile, kNode, kKinds };
+  int n = 0;
+  std::vector<Lbvh*> trees;
+  DevBuf<TreeBuildDev> dev;
+  DevBuf<int32_t> maps;
+  const int32_t* off[kKinds] = {};
+  int blocks[kKinds] = {};
+  // trees with faces only; x0[i] is tree i's point array (nv float4)
+  hipError_t Setup(const std::vector<Lbvh*>& trees, const std::vector<const float4*>& x0,
+                   hipStream_t s);
+};
+
+// every tree of the set as LbvhBuild(tree, x0, null, null, 0, 0, 0, s, merge = false)
+hipError_t MultiLbvhBuild(TreeSet& set, hipStream_t s);
 
 // BVH::UpdateData + BVH::Construct; x0/x/n are float4 per-vertex arrays (n, x may be null).
 // merge = false leaves the internal boxes to a later multi-tree merge (the batched frame

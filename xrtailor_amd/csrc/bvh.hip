@@ -45,14 +45,13 @@ __device__ __forceinline__ uint32_t MortonCode(float x, float y, float z) {  // 
 __device__ __forceinline__ int Clz64(uint64_t v) { return v == 0 ? 64 : __clzll(v); }
 
 // ------------------------------------------------------------------ build
-__global__ __launch_bounds__(kBlock) void CornerPointsKernel(const float4* __restrict__ x0,
-                                                             const float4* __restrict__ x,
-                                                             const float4* __restrict__ n,
-                                                             float tol, int nv,
-                                                             float4* __restrict__ v,
-                                                             float4* __restrict__ pred) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
+// Each build step is a body over (block, thread) so that the single-tree kernels and the
+// multi-tree kernels (every tree of a set per launch) run the same code.
+__device__ __forceinline__ void CornerPointsBody(const float4* __restrict__ x0,
+                                                 const float4* __restrict__ x,
+                                                 const float4* __restrict__ n, float tol, int i,
+                                                 float4* __restrict__ v,
+                                                 float4* __restrict__ pred) {
   float4 p = x0[i];
   float4 q = x ? x[i] : p;
   if (n) {
@@ -66,20 +65,32 @@ __global__ __launch_bounds__(kBlock) void CornerPointsKernel(const float4* __res
   pred[i] = q;
 }
 
-__global__ void InitRootKernel(uint32_t* root) {
-  const int k = threadIdx.x;
+__global__ __launch_bounds__(kBlock) void CornerPointsKernel(const float4* __restrict__ x0,
+                                                             const float4* __restrict__ x,
+                                                             const float4* __restrict__ n,
+                                                             float tol, int nv,
+                                                             float4* __restrict__ v,
+                                                             float4* __restrict__ pred) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nv) CornerPointsBody(x0, x, n, tol, i, v, pred);
+}
+
+__device__ __forceinline__ void InitRootBody(uint32_t* root, int k) {
   if (k < 3) root[k] = EncodeOrdered(INFINITY);
   else if (k < 6) root[k] = EncodeOrdered(-INFINITY);
 }
 
-// InitAABBs_Kernel (BVH.cu:423-462) + the root reduction (BVH.cu:642-647)
-__global__ __launch_bounds__(kBlock) void LeafBoxKernel(const int4* __restrict__ tris,
-                                                        const float4* __restrict__ v,
-                                                        const float4* __restrict__ pred, int nf,
-                                                        float scr, Box* __restrict__ leaf,
-                                                        uint32_t* __restrict__ root) {
+__global__ void InitRootKernel(uint32_t* root) { InitRootBody(root, threadIdx.x); }
+
+// InitAABBs_Kernel (BVH.cu:423-462) + the root reduction (BVH.cu:642-647) for faces of
+// block `blk`; every thread of the block takes part (block reduction)
+__device__ __forceinline__ void LeafBoxBody(const int4* __restrict__ tris,
+                                            const float4* __restrict__ v,
+                                            const float4* __restrict__ pred, int nf, float scr,
+                                            Box* __restrict__ leaf, uint32_t* __restrict__ root,
+                                            int blk) {
   __shared__ uint32_t red[6][kBlock / 64];
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blk * kBlock + threadIdx.x;
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   if (f < nf) {
     const int4 t = tris[f];
@@ -122,13 +133,19 @@ __global__ __launch_bounds__(kBlock) void LeafBoxKernel(const int4* __restrict__
   }
 }
 
+__global__ __launch_bounds__(kBlock) void LeafBoxKernel(const int4* __restrict__ tris,
+                                                        const float4* __restrict__ v,
+                                                        const float4* __restrict__ pred, int nf,
+                                                        float scr, Box* __restrict__ leaf,
+                                                        uint32_t* __restrict__ root) {
+  LeafBoxBody(tris, v, pred, nf, scr, leaf, root, blockIdx.x);
+}
+
 // ComputeMortons_Kernel (BVH.cu:464-481)
-__global__ __launch_bounds__(kBlock) void MortonKernel(const Box* __restrict__ leaf,
-                                                       const uint32_t* __restrict__ root, int nf,
-                                                       uint32_t* __restrict__ morton,
-                                                       int32_t* __restrict__ order) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
+__device__ __forceinline__ void MortonBody(const Box* __restrict__ leaf,
+                                           const uint32_t* __restrict__ root, int f,
+                                           uint32_t* __restrict__ morton,
+                                           int32_t* __restrict__ order) {
   const float rlx = DecodeOrderedDev(root[0]), rly = DecodeOrderedDev(root[1]), rlz = DecodeOrderedDev(root[2]);
   const float rhx = DecodeOrderedDev(root[3]), rhy = DecodeOrderedDev(root[4]), rhz = DecodeOrderedDev(root[5]);
   const Box b = leaf[f];
@@ -139,23 +156,36 @@ __global__ __launch_bounds__(kBlock) void MortonKernel(const Box* __restrict__ l
   order[f] = f;
 }
 
+__global__ __launch_bounds__(kBlock) void MortonKernel(const Box* __restrict__ leaf,
+                                                       const uint32_t* __restrict__ root, int nf,
+                                                       uint32_t* __restrict__ morton,
+                                                       int32_t* __restrict__ order) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < nf) MortonBody(leaf, root, f, morton, order);
+}
+
 // -------------------------------------------------------------- radix sort
 // Stable LSD radix sort of (key, value), 8-bit digits: per-tile histograms, one
 // exclusive scan over the digit-major table, then a scatter whose in-tile ranks come
 // from wave64 ballots (lanes with equal digits), so equal keys keep their input order.
-__global__ __launch_bounds__(kBlock) void RadixHistKernel(const uint32_t* __restrict__ keys, int n,
-                                                          int shift, int nblocks,
-                                                          uint32_t* __restrict__ hist) {
+__device__ __forceinline__ void RadixHistBody(const uint32_t* __restrict__ keys, int n, int shift,
+                                              int nblocks, uint32_t* __restrict__ hist, int blk) {
   __shared__ uint32_t cnt[256];
   cnt[threadIdx.x] = 0;
   __syncthreads();
-  const int base = blockIdx.x * kSortTile;
+  const int base = blk * kSortTile;
   for (int r = 0; r < kSortItems; ++r) {
     const int i = base + r * kBlock + threadIdx.x;
     if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
   }
   __syncthreads();
-  hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+  hist[threadIdx.x * nblocks + blk] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void RadixHistKernel(const uint32_t* __restrict__ keys, int n,
+                                                          int shift, int nblocks,
+                                                          uint32_t* __restrict__ hist) {
+  RadixHistBody(keys, n, shift, nblocks, hist, blockIdx.x);
 }
 
 // single-workgroup exclusive scan of n uint32 (in place); *total receives the sum
@@ -164,15 +194,16 @@ __global__ __launch_bounds__(1024) void ExclusiveScanKernel(uint32_t* __restrict
   BlockExclusiveScan(a, n, total);
 }
 
-__global__ __launch_bounds__(kBlock) void RadixScatterKernel(
+__device__ __forceinline__ void RadixScatterBody(
     const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin, uint32_t* __restrict__ kout,
-    int32_t* __restrict__ vout, int n, int shift, int nblocks, const uint32_t* __restrict__ offs) {
+    int32_t* __restrict__ vout, int n, int shift, int nblocks, const uint32_t* __restrict__ offs,
+    int blk) {
   __shared__ uint32_t run[256];
   __shared__ uint32_t wc[kBlock / 64][256];
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  run[t] = offs[t * nblocks + blockIdx.x];
+  run[t] = offs[t * nblocks + blk];
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int base = blockIdx.x * kSortTile;
+  const int base = blk * kSortTile;
   for (int r = 0; r < kSortItems; ++r) {
     for (int k = 0; k < kBlock / 64; ++k) wc[k][t] = 0u;
     __syncthreads();
@@ -209,18 +240,21 @@ __global__ __launch_bounds__(kBlock) void RadixScatterKernel(
   }
 }
 
+__global__ __launch_bounds__(kBlock) void RadixScatterKernel(
+    const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+    int32_t* __restrict__ vout, int n, int shift, int nblocks, const uint32_t* __restrict__ offs) {
+  RadixScatterBody(kin, vin, kout, vout, n, shift, nblocks, offs, blockIdx.x);
+}
+
 // ConstructLeafNodes_Kernel (BVH.cu:489-502) + internal-node reset + 64-bit keys
 // (m << 32 | object): unique keys reproduce the 32-bit tree when codes are unique and the
 // reference's 64-bit fallback when they are not (BVH.cu:671-683)
-__global__ __launch_bounds__(kBlock) void LeafNodesKernel(const uint32_t* __restrict__ morton,
-                                                          const int32_t* __restrict__ order,
-                                                          const Box* __restrict__ leaf, int nf,
-                                                          int4* __restrict__ nodes,
-                                                          Box* __restrict__ box,
-                                                          uint64_t* __restrict__ code,
-                                                          int32_t* __restrict__ flags) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nf) return;
+__device__ __forceinline__ void LeafNodesBody(const uint32_t* __restrict__ morton,
+                                              const int32_t* __restrict__ order,
+                                              const Box* __restrict__ leaf, int nf, int i,
+                                              int4* __restrict__ nodes, Box* __restrict__ box,
+                                              uint64_t* __restrict__ code,
+                                              int32_t* __restrict__ flags) {
   const int ni = nf - 1;
   const int obj = order[i];
   nodes[ni + i] = make_int4(-1, -1, -1, obj);
@@ -232,12 +266,21 @@ __global__ __launch_bounds__(kBlock) void LeafNodesKernel(const uint32_t* __rest
   }
 }
 
+__global__ __launch_bounds__(kBlock) void LeafNodesKernel(const uint32_t* __restrict__ morton,
+                                                          const int32_t* __restrict__ order,
+                                                          const Box* __restrict__ leaf, int nf,
+                                                          int4* __restrict__ nodes,
+                                                          Box* __restrict__ box,
+                                                          uint64_t* __restrict__ code,
+                                                          int32_t* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nf) LeafNodesBody(morton, order, leaf, nf, i, nodes, box, code, flags);
+}
+
 // determine_range / find_split / ConstructInternalNodes64Bits_Kernel (BVH.cu:102-210,297-317)
-__global__ __launch_bounds__(kBlock) void KarrasKernel(const uint64_t* __restrict__ code, int nf,
-                                                       int4* __restrict__ nodes) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void KarrasBody(const uint64_t* __restrict__ code, int nf, int idx,
+                                           int4* __restrict__ nodes) {
   const int ni = nf - 1;
-  if (idx >= ni) return;
   int first, last;
   if (idx == 0) {
     first = 0;
@@ -291,6 +334,12 @@ __global__ __launch_bounds__(kBlock) void KarrasKernel(const uint64_t* __restric
   nodes[right].x = idx;
 }
 
+__global__ __launch_bounds__(kBlock) void KarrasKernel(const uint64_t* __restrict__ code, int nf,
+                                                       int4* __restrict__ nodes) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < nf - 1) KarrasBody(code, nf, idx, nodes);
+}
+
 
 // bottom-up boxes: the second thread to reach a node merges its two children (the
 // result is the union of the descendant leaves, as AssignAABBAtomic_Kernel's atomic
@@ -312,10 +361,8 @@ __global__ __launch_bounds__(kBlock) void ZeroFlagsKernel(int32_t* flags, int n)
 }
 
 // Build{Internal,External}EscapeIndexTable_Kernel (BVH.cu:212-273), node-indexed
-__global__ __launch_bounds__(kBlock) void EscapeKernel(const int4* __restrict__ nodes, int nn,
-                                                       int32_t* __restrict__ escape) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nn) return;
+__device__ __forceinline__ void EscapeBody(const int4* __restrict__ nodes, int nn, int k,
+                                           int32_t* __restrict__ escape) {
   int cur = k, e = -1;
   for (int level = 0; level < nn; ++level) {
     const int p = nodes[cur].x;
@@ -328,6 +375,12 @@ __global__ __launch_bounds__(kBlock) void EscapeKernel(const int4* __restrict__ 
     cur = p;
   }
   escape[k] = e;
+}
+
+__global__ __launch_bounds__(kBlock) void EscapeKernel(const int4* __restrict__ nodes, int nn,
+                                                       int32_t* __restrict__ escape) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nn) EscapeBody(nodes, nn, k, escape);
 }
 
 // ---------------------------------------------------------------- refit
@@ -401,6 +454,163 @@ __global__ __launch_bounds__(kBlock) void VertexNormalsKernel(const float4* __re
   n[v] = VertexNormalBody(x, tris, vptr, corner, v);
 }
 }  // namespace
+
+namespace {
+// ------------------------------------------------------- multi-tree build
+__device__ __forceinline__ int TreeOfBlock(const int32_t* __restrict__ off, int n, int b) {
+  return off[n + 1 + b];
+}
+
+#define XRT_TREE_INDEX()                                          \
+  const int k = TreeOfBlock(off, n, blockIdx.x);                  \
+  const TreeBuildDev& t = set[k];                                 \
+  const int lb = blockIdx.x - off[k];                             \
+  const int i = lb * kBlock + threadIdx.x;                        \
+  (void)i;
+
+__global__ __launch_bounds__(kBlock) void MultiCornerKernel(const TreeBuildDev* set,
+                                                            const int32_t* off, int n) {
+  XRT_TREE_INDEX()
+  if (i < t.nv) CornerPointsBody(t.x0, nullptr, nullptr, 0.0f, i, t.v, t.pred);
+}
+
+__global__ void MultiInitRootKernel(const TreeBuildDev* set) {
+  InitRootBody(set[blockIdx.x].root, threadIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiLeafBoxKernel(const TreeBuildDev* set,
+                                                             const int32_t* off, int n) {
+  XRT_TREE_INDEX()
+  LeafBoxBody(t.tris, t.v, t.pred, t.nf, 0.0f, t.leaf, t.root, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiMortonKernel(const TreeBuildDev* set,
+                                                            const int32_t* off, int n) {
+  XRT_TREE_INDEX()
+  if (i < t.nf) MortonBody(t.leaf, t.root, i, t.morton, t.order);
+}
+
+// pass p sorts digit p; even passes read (morton, order), odd ones the tmp buffers
+__global__ __launch_bounds__(kBlock) void MultiRadixHistKernel(const TreeBuildDev* set,
+                                                               const int32_t* off, int n,
+                                                               int pass) {
+  XRT_TREE_INDEX()
+  RadixHistBody(pass & 1 ? t.morton_tmp : t.morton, t.nf, 8 * pass, t.nblocks, t.hist, lb);
+}
+
+__global__ __launch_bounds__(1024) void MultiRadixScanKernel(const TreeBuildDev* set) {
+  const TreeBuildDev& t = set[blockIdx.x];
+  BlockExclusiveScan(t.hist, 256 * t.nblocks, nullptr);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiRadixScatterKernel(const TreeBuildDev* set,
+                                                                  const int32_t* off, int n,
+                                                                  int pass) {
+  XRT_TREE_INDEX()
+  const bool odd = pass & 1;
+  RadixScatterBody(odd ? t.morton_tmp : t.morton, odd ? t.order_tmp : t.order,
+                   odd ? t.morton : t.morton_tmp, odd ? t.order : t.order_tmp, t.nf, 8 * pass,
+                   t.nblocks, t.hist, lb);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiLeafNodesKernel(const TreeBuildDev* set,
+                                                               const int32_t* off, int n) {
+  XRT_TREE_INDEX()
+  if (i < t.nf) LeafNodesBody(t.morton, t.order, t.leaf, t.nf, i, t.nodes, t.box, t.code, t.flags);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiKarrasKernel(const TreeBuildDev* set,
+                                                            const int32_t* off, int n) {
+  XRT_TREE_INDEX()
+  if (i < t.nf - 1) KarrasBody(t.code, t.nf, i, t.nodes);
+}
+
+__global__ __launch_bounds__(kBlock) void MultiEscapeKernel(const TreeBuildDev* set,
+                                                            const int32_t* off, int n) {
+  XRT_TREE_INDEX()
+  if (i < 2 * t.nf - 1) EscapeBody(t.nodes, 2 * t.nf - 1, i, t.escape);
+}
+#undef XRT_TREE_INDEX
+}  // namespace
+
+hipError_t TreeSet::Setup(const std::vector<Lbvh*>& in, const std::vector<const float4*>& x0,
+                          hipStream_t s) {
+  trees.clear();
+  std::vector<TreeBuildDev> d;
+  for (size_t q = 0; q < in.size(); ++q) {
+    Lbvh& b = *in[q];
+    if (b.nf <= 0) continue;
+    TreeBuildDev t;
+    t.x0 = x0[q];
+    t.nv = b.nv;
+    t.nf = b.nf;
+    t.nblocks = b.nblocks;
+    t.tris = b.tris.ptr;
+    t.v = b.v.ptr;
+    t.pred = b.pred.ptr;
+    t.leaf = b.leaf.ptr;
+    t.morton = b.morton.ptr;
+    t.morton_tmp = b.morton_tmp.ptr;
+    t.order = b.order.ptr;
+    t.order_tmp = b.order_tmp.ptr;
+    t.code = b.code.ptr;
+    t.nodes = b.nodes.ptr;
+    t.box = b.box.ptr;
+    t.escape = b.escape.ptr;
+    t.root = b.root.ptr;
+    t.hist = b.hist.ptr;
+    t.flags = b.flags.ptr;
+    d.push_back(t);
+    trees.push_back(&b);
+  }
+  n = static_cast<int>(d.size());
+  std::vector<int32_t> m;
+  size_t seg[kKinds];
+  for (int kind = 0; kind < kKinds; ++kind) {
+    std::vector<int32_t> o(n + 1, 0);
+    for (int k = 0; k < n; ++k) {
+      const TreeBuildDev& t = d[k];
+      const int c = kind == kVert ? Blocks(t.nv)
+                  : kind == kFace ? Blocks(t.nf)
+                  : kind == kTile ? t.nblocks
+                                  : Blocks(2 * t.nf - 1);
+      o[k + 1] = o[k] + c;
+    }
+    blocks[kind] = o[n];
+    seg[kind] = m.size();
+    m.insert(m.end(), o.begin(), o.end());
+    for (int k = 0; k < n; ++k) m.insert(m.end(), static_cast<size_t>(o[k + 1] - o[k]), k);
+  }
+  if (n == 0) return hipSuccess;
+  hipError_t e;
+  if ((e = dev.Upload(d, s)) != hipSuccess) return e;
+  if ((e = maps.Upload(m, s)) != hipSuccess) return e;
+  for (int kind = 0; kind < kKinds; ++kind) off[kind] = maps.ptr + seg[kind];
+  return hipStreamSynchronize(s);
+}
+
+hipError_t MultiLbvhBuild(TreeSet& set, hipStream_t s) {
+  if (set.n == 0) return hipSuccess;
+  using K = TreeSet;
+  const TreeBuildDev* d = set.dev.ptr;
+  const int n = set.n;
+  MultiCornerKernel<<<set.blocks[K::kVert], kBlock, 0, s>>>(d, set.off[K::kVert], n);
+  MultiInitRootKernel<<<n, 64, 0, s>>>(d);
+  MultiLeafBoxKernel<<<set.blocks[K::kFace], kBlock, 0, s>>>(d, set.off[K::kFace], n);
+  MultiMortonKernel<<<set.blocks[K::kFace], kBlock, 0, s>>>(d, set.off[K::kFace], n);
+  for (int pass = 0; pass < 4; ++pass) {  // the result lands back in (morton, order)
+    MultiRadixHistKernel<<<set.blocks[K::kTile], kBlock, 0, s>>>(d, set.off[K::kTile], n, pass);
+    MultiRadixScanKernel<<<n, 1024, 0, s>>>(d);
+    MultiRadixScatterKernel<<<set.blocks[K::kTile], kBlock, 0, s>>>(d, set.off[K::kTile], n, pass);
+  }
+  MultiLeafNodesKernel<<<set.blocks[K::kFace], kBlock, 0, s>>>(d, set.off[K::kFace], n);
+  MultiKarrasKernel<<<set.blocks[K::kFace], kBlock, 0, s>>>(d, set.off[K::kFace], n);
+  MultiEscapeKernel<<<set.blocks[K::kNode], kBlock, 0, s>>>(d, set.off[K::kNode], n);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  for (Lbvh* b : set.trees) b->active = true;
+  return hipSuccess;
+}
 
 float DecodeOrdered(uint32_t u) { return DecodeOrderedDev(u); }
 

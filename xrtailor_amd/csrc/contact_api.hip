@@ -356,6 +356,22 @@ int MultiContactSetup(xrt_ctx* ctx) {
   }
   XRT_CHECK_HIP(ctx->ic.Upload(ic, ctx->stream));
   XRT_CHECK_HIP(ctx->ic_off.Upload(maps, ctx->stream));
+  {  // the trees rebuilt at every frame start of the batched path
+    std::vector<Lbvh*> ct, ot;
+    std::vector<const float4*> cx, ox;
+    for (int k = 0; k < n; ++k) {
+      ContactData* c = ctx->contact[k].get();
+      if (!c) continue;
+      ct.push_back(&c->cloth_bvh);
+      cx.push_back(ctx->x0.ptr + ctx->inst[k].desc.vbeg);
+      if (c->obs_nf > 0) {
+        ot.push_back(&c->obs_bvh);
+        ox.push_back(c->o_x0.ptr);
+      }
+    }
+    XRT_CHECK_HIP(ctx->cloth_trees.Setup(ct, cx, ctx->stream));
+    XRT_CHECK_HIP(ctx->obs_trees.Setup(ot, ox, ctx->stream));
+  }
   ctx->maps.n = n;
   for (int kind = 0; kind < MultiMaps::kKinds; ++kind) {
     ctx->maps.off[kind] = ctx->ic_off.ptr + seg[kind];
@@ -373,23 +389,18 @@ ContactData* FirstContact(xrt_ctx* ctx) {
 }
 }  // namespace
 
-// frame start: every instance's trees are built without their bottom-up merge, then all
-// of them are merged in one launch
+// frame start: the obstacle trees (and the cloth trees with self contact) of every
+// instance are rebuilt together, each build step one launch over all trees, then merged
+// in one launch (the same steps as LbvhBuild per tree)
 int MultiContactSubstepBegin(xrt_ctx* ctx) {
   ContactData* first = FirstContact(ctx);
   if (!first || first->frame_substep != 0) return XRT_OK;
   hipStream_t s = ctx->stream;
   const bool self = ctx->params.enable_self_contact != 0;
-  for (size_t i = 0; i < ctx->contact.size(); ++i) {
-    ContactData* c = ctx->contact[i].get();
-    if (!c) continue;
-    if (c->obs_nf > 0)
-      XRT_CHECK_HIP(LbvhBuild(c->obs_bvh, c->o_x0.ptr, nullptr, nullptr, 0.0f, 0, 0.0f, s, false));
-    if (self)
-      XRT_CHECK_HIP(LbvhBuild(c->cloth_bvh, ctx->x0.ptr + ctx->inst[i].desc.vbeg, nullptr, nullptr,
-                              0.0f, 0, 0.0f, s, false));
-    c->pass_cnt = 0;
-  }
+  XRT_CHECK_HIP(MultiLbvhBuild(ctx->obs_trees, s));
+  if (self) XRT_CHECK_HIP(MultiLbvhBuild(ctx->cloth_trees, s));
+  for (auto& c : ctx->contact)
+    if (c) c->pass_cnt = 0;
   XRT_CHECK_HIP(MultiBuildMerge(s, ctx->ic.ptr, ctx->maps, self));
   return XRT_OK;
 }

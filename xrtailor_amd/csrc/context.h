@@ -155,6 +155,7 @@ struct xrt_ctx {
   // multi-instance contact kernels
   xrt::DevBuf<xrt::InstContactDev> ic;
   xrt::DevBuf<int32_t> ic_off;
+  xrt::TreeSet cloth_trees, obs_trees;  // frame-start builds of the batched path
   xrt::MultiMaps maps;
   // wide (all instances per launch) launch sizes
   xrt::WideConfig wide;
