@@ -457,14 +457,21 @@ __global__ __launch_bounds__(kBlock) void VertexNormalsKernel(const float4* __re
 
 namespace {
 // ------------------------------------------------------- multi-tree build
-__device__ __forceinline__ int TreeOfBlock(const int32_t* __restrict__ off, int n, int b) {
-  return off[n + 1 + b];
+// A map of B logical blocks (off[n] = B) runs on a grid rounded up to a multiple of 8:
+// physical block b takes logical block (b % 8) * ceil(B / 8) + b / 8, so each XCD (the
+// hardware deals workgroups to the 8 XCDs in turn) walks a contiguous range of trees.
+__device__ __forceinline__ int XcdTreeBlock(const int32_t* __restrict__ off, int n) {
+  const int total = off[n];
+  const int l = static_cast<int>(blockIdx.x & 7u) * ((total + 7) >> 3) + static_cast<int>(blockIdx.x >> 3);
+  return l < total ? l : -1;
 }
 
 #define XRT_TREE_INDEX()                                          \
-  const int k = TreeOfBlock(off, n, blockIdx.x);                  \
+  const int lblk = XcdTreeBlock(off, n);                          \
+  if (lblk < 0) return;                                           \
+  const int k = off[n + 1 + lblk];                                \
   const TreeBuildDev& t = set[k];                                 \
-  const int lb = blockIdx.x - off[k];                             \
+  const int lb = lblk - off[k];                                   \
   const int i = lb * kBlock + threadIdx.x;                        \
   (void)i;
 
@@ -594,18 +601,18 @@ hipError_t MultiLbvhBuild(TreeSet& set, hipStream_t s) {
   using K = TreeSet;
   const TreeBuildDev* d = set.dev.ptr;
   const int n = set.n;
-  MultiCornerKernel<<<set.blocks[K::kVert], kBlock, 0, s>>>(d, set.off[K::kVert], n);
+  MultiCornerKernel<<<(set.blocks[K::kVert] + 7) / 8 * 8, kBlock, 0, s>>>(d, set.off[K::kVert], n);
   MultiInitRootKernel<<<n, 64, 0, s>>>(d);
-  MultiLeafBoxKernel<<<set.blocks[K::kFace], kBlock, 0, s>>>(d, set.off[K::kFace], n);
-  MultiMortonKernel<<<set.blocks[K::kFace], kBlock, 0, s>>>(d, set.off[K::kFace], n);
+  MultiLeafBoxKernel<<<(set.blocks[K::kFace] + 7) / 8 * 8, kBlock, 0, s>>>(d, set.off[K::kFace], n);
+  MultiMortonKernel<<<(set.blocks[K::kFace] + 7) / 8 * 8, kBlock, 0, s>>>(d, set.off[K::kFace], n);
   for (int pass = 0; pass < 4; ++pass) {  // the result lands back in (morton, order)
-    MultiRadixHistKernel<<<set.blocks[K::kTile], kBlock, 0, s>>>(d, set.off[K::kTile], n, pass);
+    MultiRadixHistKernel<<<(set.blocks[K::kTile] + 7) / 8 * 8, kBlock, 0, s>>>(d, set.off[K::kTile], n, pass);
     MultiRadixScanKernel<<<n, 1024, 0, s>>>(d);
-    MultiRadixScatterKernel<<<set.blocks[K::kTile], kBlock, 0, s>>>(d, set.off[K::kTile], n, pass);
+    MultiRadixScatterKernel<<<(set.blocks[K::kTile] + 7) / 8 * 8, kBlock, 0, s>>>(d, set.off[K::kTile], n, pass);
   }
-  MultiLeafNodesKernel<<<set.blocks[K::kFace], kBlock, 0, s>>>(d, set.off[K::kFace], n);
-  MultiKarrasKernel<<<set.blocks[K::kFace], kBlock, 0, s>>>(d, set.off[K::kFace], n);
-  MultiEscapeKernel<<<set.blocks[K::kNode], kBlock, 0, s>>>(d, set.off[K::kNode], n);
+  MultiLeafNodesKernel<<<(set.blocks[K::kFace] + 7) / 8 * 8, kBlock, 0, s>>>(d, set.off[K::kFace], n);
+  MultiKarrasKernel<<<(set.blocks[K::kFace] + 7) / 8 * 8, kBlock, 0, s>>>(d, set.off[K::kFace], n);
+  MultiEscapeKernel<<<(set.blocks[K::kNode] + 7) / 8 * 8, kBlock, 0, s>>>(d, set.off[K::kNode], n);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   for (Lbvh* b : set.trees) b->active = true;

@@ -12,6 +12,7 @@ using namespace cm;
 constexpr int kBlock = 256;
 constexpr float kFix = 4294967296.0f;     // 2^32: 32.32 fixed point
 inline int Blocks(int n) { return (n + kBlock - 1) / kBlock; }
+inline int XcdGrid(int blocks) { return (blocks + 7) / 8 * 8; }  // see XcdBlock
 
 // Ericson ClosestPtPointTriangle (BasicPrimitiveTests.cu:6-100)
 __device__ float NearestPoint(F3 p, F3 a, F3 b, F3 c, F3& q) {
@@ -317,10 +318,10 @@ __device__ __forceinline__ void Accumulate(const RMesh& m0, const RMesh& m1, con
   }
 }
 
-// EvaluateVFProximity (PBDRepulsion.cu:329-409)
-__device__ void EvalVF(const RMesh& c, int node, const RMesh& o, int face, bool self, float h,
-                       long long* dq, int32_t* cnt) {
-  const int4 tr = o.t.tris[face];
+// EvaluateVFProximity (PBDRepulsion.cu:329-409) of vertex `node` against the face with
+// vertices tr (the face id itself is not needed)
+__device__ void EvalVFt(const RMesh& c, int node, const RMesh& o, int4 tr, bool self, float h,
+                        long long* dq, int32_t* cnt) {
   if (self && (node == tr.x || node == tr.y || node == tr.z)) return;
   // the inside test on the current state rejects most pairs; it is evaluated before the
   // start-of-substep side (both are pure, so the order does not change any result).  It is
@@ -366,11 +367,18 @@ __device__ void EvalVF(const RMesh& c, int node, const RMesh& o, int face, bool 
   Accumulate<1>(c, o, ids, corr, dq, cnt);
 }
 
-// EvaluateEEProximity (PBDRepulsion.cu:411-468)
-// ua, ub: unit directions of the two edges in the current state, Nz(x_a.y - x_a.x)
-__device__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self, float h, F3 ua,
-                       F3 ub, long long* dq, int32_t* cnt) {
-  const int2 a = c.t.edges[e0], b = o.t.edges[e1];
+__device__ __forceinline__ void EvalVF(const RMesh& c, int node, const RMesh& o, int face,
+                                       bool self, float h, long long* dq, int32_t* cnt) {
+  EvalVFt(c, node, o, o.t.tris[face], self, h, dq, cnt);
+}
+
+// EvaluateEEProximity (PBDRepulsion.cu:411-468) of the edges with vertices a (cloth) and
+// b (obstacle or cloth); ua, ub: their unit directions in the current state,
+// Nz(x_a.y - x_a.x).  The edge ids, ids() = (e0, e1), are needed only by the pairs that
+// pass the inside test.
+template <class Ids>
+__device__ void EvalEEv(const RMesh& c, int2 a, const RMesh& o, int2 b, bool self, float h, F3 ua,
+                        F3 ub, Ids&& ids, long long* dq, int32_t* cnt) {
   if (self && (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y)) return;
   // current-state tests first, then the start-of-substep side (pure; same results).  The
   // weights are SignedEE's with early exits: an accepted pair has every weight > 0 (a NaN
@@ -385,7 +393,10 @@ __device__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self
   if (!((b0 > 0 && b1 > 0) || (b0 < 0 && b1 < 0))) return;
   float bw[4] = {a0 / (a0 + a1), a1 / (a0 + a1), -b0 / (b0 + b1), -b1 / (b0 + b1)};
   const float mn = fminf(fminf(bw[0], bw[1]), fminf(-bw[2], -bw[3]));
-  if (!(mn >= 1e-6f && InEdge(bw[1], c, e0, o, e1) && InEdge(-bw[3], o, e1, c, e0))) return;
+  if (!(mn >= 1e-6f)) return;
+  const int2 e = ids();
+  const int e0 = e.x, e1 = e.y;
+  if (!(InEdge(bw[1], c, e0, o, e1) && InEdge(-bw[3], o, e1, c, e0))) return;
   float w[4];
   F3 pn;
   const float d0 = SignedEE(c.X0(a.x), c.X0(a.y), o.X0(b.x), o.X0(b.y), pn, w);
@@ -405,8 +416,14 @@ __device__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self
   const float s = Cv / den;
   F3 corr[4];
   for (int i = 0; i < 4; ++i) corr[i] = M(g[i], -s * wq[i]);
-  const int ids[4] = {a.x, a.y, b.x, b.y};
-  Accumulate<2>(c, o, ids, corr, dq, cnt);
+  const int ids4[4] = {a.x, a.y, b.x, b.y};
+  Accumulate<2>(c, o, ids4, corr, dq, cnt);
+}
+
+__device__ __forceinline__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self,
+                                       float h, F3 ua, F3 ub, long long* dq, int32_t* cnt) {
+  EvalEEv(c, c.t.edges[e0], o, o.t.edges[e1], self, h, ua, ub,
+          [&] { return make_int2(e0, e1); }, dq, cnt);
 }
 
 // EvaluateRep_Kernel body (PBDRepulsion.cu:470-501) for pair p: cloth face pr.x against
@@ -512,7 +529,6 @@ __device__ __forceinline__ void RepulsionVfItemBody(const RMesh& c, const RMesh&
   }
 }
 
-// one EE item: cloth edge it.x against edge it.y of the obstacle or the cloth
 template <bool kSelf>
 __device__ __forceinline__ void EeItem(const RMesh& c, const RMesh& other, int e0, int e1,
                                        float h, long long* __restrict__ dq,
@@ -523,6 +539,7 @@ __device__ __forceinline__ void EeItem(const RMesh& c, const RMesh& other, int e
   EvalEE(c, e0, other, e1, kSelf, h, ua, ub, dq, cnt);
 }
 
+// one EE item: cloth edge it.x against edge it.y of the obstacle or the cloth
 __device__ __forceinline__ void RepulsionEeItemBody(const RMesh& c, const RMesh& o, int2 it,
                                                     float h, long long* __restrict__ dq,
                                                     int32_t* __restrict__ cnt) {
@@ -617,17 +634,27 @@ __global__ void AddTotalsKernel(const int32_t* a, const int32_t* b, int32_t* out
 }
 
 // -------------------------------------------------------- multi-instance kernels
-// blockIdx.x -> (instance, block within the instance) through a prefix table of block
-// counts (instances that need no block are skipped by the search)
-// the instance of block b: the map's block table follows its n + 1 prefix sums
+// A map of B logical blocks (off[n] = B) is launched on XcdGrid(B) blocks.  The hardware
+// deals consecutive workgroups to the 8 XCDs in turn, so physical block b runs logical
+// block (b % 8) * ceil(B / 8) + b / 8: each XCD walks a contiguous range of logical blocks
+// -- a few instances at a time, whose data then stay in that XCD's L2.  -1: idle block.
+__device__ __forceinline__ int XcdBlock(const int32_t* __restrict__ off, int n) {
+  const int total = off[n];
+  const int l = static_cast<int>(blockIdx.x & 7u) * ((total + 7) >> 3) + static_cast<int>(blockIdx.x >> 3);
+  return l < total ? l : -1;
+}
+
+// the instance of logical block b: the map's block table follows its n + 1 prefix sums
 __device__ __forceinline__ int InstOfBlock(const int32_t* __restrict__ off, int n, int b) {
   return off[n + 1 + b];
 }
 
 #define XRT_MULTI_INDEX(kind, count_expr)                                  \
-  const int k = InstOfBlock(off, n, blockIdx.x);                           \
+  const int lblk = XcdBlock(off, n);                                       \
+  if (lblk < 0) return;                                                    \
+  const int k = InstOfBlock(off, n, lblk);                                 \
   const InstContactDev& c = ic[k];                                         \
-  const int i = (blockIdx.x - off[k]) * kBlock + threadIdx.x;              \
+  const int i = (lblk - off[k]) * kBlock + threadIdx.x;                    \
   if (i >= (count_expr)) return;
 
 __global__ __launch_bounds__(kBlock) void MultiInterpolateKernel(const InstContactDev* ic,
@@ -883,7 +910,9 @@ __global__ __launch_bounds__(kBlock) void MultiRepItemKernel(const InstContactDe
                                                              const float4* __restrict__ x,
                                                              const float4* __restrict__ x0,
                                                              float h) {
-  const int k = InstOfBlock(off, n, blockIdx.x);
+  const int lblk = XcdBlock(off, n);
+  if (lblk < 0) return;
+  const int k = InstOfBlock(off, n, lblk);
   const InstContactDev& c = ic[k];
   if (!ItemsUsable(c, is_ee)) return;
   const int cnt = c.scal[is_ee ? 4 : 3];
@@ -891,7 +920,7 @@ __global__ __launch_bounds__(kBlock) void MultiRepItemKernel(const InstContactDe
   const RMesh cm{c.cloth, MeshState{x + c.vbeg, x0 + c.vbeg, -1.0f}};
   const RMesh om{c.obs, MeshState{c.o_x, c.o_x0, 1e-3f}};  // obstacle nodes: inv mass 1e-3
   const int2* items = is_ee ? c.rep_ee : c.rep_vf;
-  for (int q = (blockIdx.x - off[k]) * kBlock + threadIdx.x; q < cnt; q += nblk * kBlock) {
+  for (int q = (lblk - off[k]) * kBlock + threadIdx.x; q < cnt; q += nblk * kBlock) {
     if (is_ee) RepulsionEeItemBody(cm, om, items[q], h, c.dq, c.cnt);
     else RepulsionVfItemBody(cm, om, items[q], h, c.dq, c.cnt);
   }
@@ -903,7 +932,9 @@ __global__ __launch_bounds__(kBlock) void MultiRepPairsKernel(const InstContactD
                                                               const float4* __restrict__ x,
                                                               const float4* __restrict__ x0,
                                                               float h) {
-  const int k = InstOfBlock(off, n, blockIdx.x);
+  const int lblk = XcdBlock(off, n);
+  if (lblk < 0) return;
+  const int k = InstOfBlock(off, n, lblk);
   const InstContactDev& c = ic[k];
   const bool vf = !ItemsUsable(c, 0);
   const bool ee = !ItemsUsable(c, 1);
@@ -913,7 +944,7 @@ __global__ __launch_bounds__(kBlock) void MultiRepPairsKernel(const InstContactD
   const RMesh om{c.obs, MeshState{c.o_x, c.o_x0, 1e-3f}};
   const int total = min(c.scal[2], c.pair_cap);
   const int nobs = min(c.scal[1], total);
-  for (int p = (blockIdx.x - off[k]) * kBlock + threadIdx.x; p < total; p += nblk * kBlock) {
+  for (int p = (lblk - off[k]) * kBlock + threadIdx.x; p < total; p += nblk * kBlock) {
     if (vf) RepulsionPairVfBody(cm, om, c.pairs[p], p >= nobs, h, c.dq, c.cnt);
     if (ee) RepulsionPairEeBody(cm, om, c.pairs[p], p >= nobs, h, c.dq, c.cnt);
   }
@@ -1025,9 +1056,9 @@ inline bool Empty(const MultiMaps& m, MK k) { return m.n <= 0 || m.blocks[k] <= 
 hipError_t MultiObstacleRefit(hipStream_t s, const InstContactDev* ic, const MultiMaps& m,
                               float alpha, float tol) {
   if (Empty(m, MK::kObsVert)) return hipSuccess;
-  MultiInterpolateKernel<<<m.blocks[MK::kObsVert], kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n, alpha);
-  MultiNormalPredKernel<<<m.blocks[MK::kObsVert], kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n, tol);
-  MultiObsGrowKernel<<<m.blocks[MK::kObsFace], kBlock, 0, s>>>(ic, m.off[MK::kObsFace], m.n);
+  MultiInterpolateKernel<<<XcdGrid(m.blocks[MK::kObsVert]), kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n, alpha);
+  MultiNormalPredKernel<<<XcdGrid(m.blocks[MK::kObsVert]), kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n, tol);
+  MultiObsGrowKernel<<<XcdGrid(m.blocks[MK::kObsFace]), kBlock, 0, s>>>(ic, m.off[MK::kObsFace], m.n);
   MultiObsBottomUpKernel<<<m.n, 1024, 0, s>>>(ic, m.n);
   return hipGetLastError();
 }
@@ -1035,9 +1066,9 @@ hipError_t MultiObstacleRefit(hipStream_t s, const InstContactDev* ic, const Mul
 hipError_t MultiGenerate(hipStream_t s, const InstContactDev* ic, const MultiMaps& m,
                          const float4* x0, const float4* x) {
   if (Empty(m, MK::kVert)) return hipSuccess;
-  MultiVfGenerateKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x0, x);
+  MultiVfGenerateKernel<<<XcdGrid(m.blocks[MK::kVert]), kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x0, x);
   if (!Empty(m, MK::kEdge))
-    MultiEeGenerateKernel<<<m.blocks[MK::kEdge], kBlock, 0, s>>>(ic, m.off[MK::kEdge], m.n, x);
+    MultiEeGenerateKernel<<<XcdGrid(m.blocks[MK::kEdge]), kBlock, 0, s>>>(ic, m.off[MK::kEdge], m.n, x);
   return hipGetLastError();
 }
 
@@ -1046,16 +1077,16 @@ hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& 
   if (Empty(m, MK::kFace)) return hipSuccess;
   const int fb = m.blocks[MK::kFace];
   const int32_t* fo = m.off[MK::kFace];
-  MultiClothPredKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x);
-  MultiClothGrowKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, x, x0);
+  MultiClothPredKernel<<<XcdGrid(m.blocks[MK::kVert]), kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x);
+  MultiClothGrowKernel<<<XcdGrid(fb), kBlock, 0, s>>>(ic, fo, m.n, x, x0);
   MultiClothBottomUpKernel<<<m.n, 1024, 0, s>>>(ic, m.n);
-  MultiPackClothKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n);
+  MultiPackClothKernel<<<XcdGrid(fb), kBlock, 0, s>>>(ic, fo, m.n);
   if (!Empty(m, MK::kObsFace))
-    MultiPackObsKernel<<<m.blocks[MK::kObsFace], kBlock, 0, s>>>(ic, m.off[MK::kObsFace], m.n);
+    MultiPackObsKernel<<<XcdGrid(m.blocks[MK::kObsFace]), kBlock, 0, s>>>(ic, m.off[MK::kObsFace], m.n);
   for (int self = 0; self < 2; ++self) {
-    MultiCountPairsKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
+    MultiCountPairsKernel<<<XcdGrid(fb), kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
     MultiScanKernel<<<m.n, 1024, 0, s>>>(ic, m.n, self);
-    MultiEmitPairsKernel<<<fb, kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
+    MultiEmitPairsKernel<<<XcdGrid(fb), kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
   }
   MultiAddTotalsKernel<<<Blocks(m.n), kBlock, 0, s>>>(ic, m.n);
   return hipGetLastError();
@@ -1065,33 +1096,33 @@ hipError_t MultiRepulsion(hipStream_t s, const InstContactDev* ic, const MultiMa
                           const float4* x0, float h, float relax, hipEvent_t ee_begin,
                           hipEvent_t ee_end) {
   if (Empty(m, MK::kPair)) return hipSuccess;
-  MultiRepItemKernel<0><<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
+  MultiRepItemKernel<0><<<XcdGrid(m.blocks[MK::kPair]), kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
   if (ee_begin) {
     const hipError_t e = hipEventRecord(ee_begin, s);
     if (e != hipSuccess) return e;
   }
-  MultiRepItemKernel<1><<<m.blocks[MK::kPair], kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
+  MultiRepItemKernel<1><<<XcdGrid(m.blocks[MK::kPair]), kBlock, 0, s>>>(ic, m.off[MK::kPair], m.n, x, x0, h);
   if (ee_end) {
     const hipError_t e = hipEventRecord(ee_end, s);
     if (e != hipSuccess) return e;
   }
   // the fallback runs on the smaller per-vertex grid: most of its blocks exit at once
-  MultiRepPairsKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, x0, h);
-  MultiApplyKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, relax);
+  MultiRepPairsKernel<<<XcdGrid(m.blocks[MK::kVert]), kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, x0, h);
+  MultiApplyKernel<<<XcdGrid(m.blocks[MK::kVert]), kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, relax);
   return hipGetLastError();
 }
 
 hipError_t MultiVfSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x) {
   if (Empty(m, MK::kVert)) return hipSuccess;
-  MultiVfSolveKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x);
+  MultiVfSolveKernel<<<XcdGrid(m.blocks[MK::kVert]), kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x);
   return hipGetLastError();
 }
 
 hipError_t MultiEeSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
                         float relax) {
   if (Empty(m, MK::kEdge)) return hipSuccess;
-  MultiEeSolveKernel<<<m.blocks[MK::kEdge], kBlock, 0, s>>>(ic, m.off[MK::kEdge], m.n, x);
-  MultiApplyKernel<<<m.blocks[MK::kVert], kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, relax);
+  MultiEeSolveKernel<<<XcdGrid(m.blocks[MK::kEdge]), kBlock, 0, s>>>(ic, m.off[MK::kEdge], m.n, x);
+  MultiApplyKernel<<<XcdGrid(m.blocks[MK::kVert]), kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x, relax);
   return hipGetLastError();
 }
 
@@ -1103,7 +1134,7 @@ hipError_t MultiBuildMerge(hipStream_t s, const InstContactDev* ic, const MultiM
 
 hipError_t MultiObstacleFrameEnd(hipStream_t s, const InstContactDev* ic, const MultiMaps& m) {
   if (Empty(m, MK::kObsVert)) return hipSuccess;
-  MultiObsCopyKernel<<<m.blocks[MK::kObsVert], kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n);
+  MultiObsCopyKernel<<<XcdGrid(m.blocks[MK::kObsVert]), kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n);
   return hipGetLastError();
 }
 
