@@ -376,15 +376,16 @@ __device__ __forceinline__ void EvalVF(const RMesh& c, int node, const RMesh& o,
 // b (obstacle or cloth); ua, ub: their unit directions in the current state,
 // Nz(x_a.y - x_a.x).  The edge ids, ids() = (e0, e1), are needed only by the pairs that
 // pass the inside test.
+// x0, x1, y0, y1: the current positions of a.x, a.y, b.x, b.y (loaded by the caller)
 template <class Ids>
-__device__ void EvalEEv(const RMesh& c, int2 a, const RMesh& o, int2 b, bool self, float h, F3 ua,
-                        F3 ub, Ids&& ids, long long* dq, int32_t* cnt) {
+__device__ void EvalEEp(const RMesh& c, int2 a, const RMesh& o, int2 b, F3 x0, F3 x1, F3 y0,
+                        F3 y1, bool self, float h, F3 ua, F3 ub, Ids&& ids, long long* dq,
+                        int32_t* cnt) {
   if (self && (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y)) return;
   // current-state tests first, then the start-of-substep side (pure; same results).  The
   // weights are SignedEE's with early exits: an accepted pair has every weight > 0 (a NaN
   // weight fails InEdge or leaves another weight <= 0), which needs a0, a1 and b0, b1 each
   // of one strict sign, checked before the divisions on the same expressions
-  const F3 x0 = c.X(a.x), x1 = c.X(a.y), y0 = o.X(b.x), y1 = o.X(b.y);
   const F3 ncur = C(ua, ub);
   if (D(ncur, ncur) < 1e-6f) return;  // degenerate: weights 0
   const float a0 = Mixed(S(y1, x1), S(y0, x1), ncur), a1 = Mixed(S(y0, x0), S(y1, x0), ncur);
@@ -403,7 +404,7 @@ __device__ void EvalEEv(const RMesh& c, int2 a, const RMesh& o, int2 b, bool sel
   const bool e1_free = o.Free(b.x) && o.Free(b.y);
   if (!e1_free) pn = EdgeNormal(o, e1);
   else if (d0 < 0.0f) pn = M(pn, -1.0f);
-  const F3 dd = A(A(M(c.X(a.x), bw[0]), M(c.X(a.y), bw[1])), A(M(o.X(b.x), bw[2]), M(o.X(b.y), bw[3])));
+  const F3 dd = A(A(M(x0, bw[0]), M(x1, bw[1])), A(M(y0, bw[2]), M(y1, bw[3])));
   const float Cv = D(dd, pn) - h;
   if (Cv > 0) return;
   bw[2] *= -1.0f;
@@ -418,6 +419,13 @@ __device__ void EvalEEv(const RMesh& c, int2 a, const RMesh& o, int2 b, bool sel
   for (int i = 0; i < 4; ++i) corr[i] = M(g[i], -s * wq[i]);
   const int ids4[4] = {a.x, a.y, b.x, b.y};
   Accumulate<2>(c, o, ids4, corr, dq, cnt);
+}
+
+template <class Ids>
+__device__ __forceinline__ void EvalEEv(const RMesh& c, int2 a, const RMesh& o, int2 b, bool self,
+                                        float h, F3 ua, F3 ub, Ids&& ids, long long* dq,
+                                        int32_t* cnt) {
+  EvalEEp(c, a, o, b, c.X(a.x), c.X(a.y), o.X(b.x), o.X(b.y), self, h, ua, ub, ids, dq, cnt);
 }
 
 __device__ __forceinline__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self,
