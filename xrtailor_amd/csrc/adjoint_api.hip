@@ -20,7 +20,7 @@
 namespace xrt {
 namespace {
 
-constexpr size_t kMaxTapeBytes = size_t(96) << 30;  // one frame of tape
+constexpr size_t kMaxTapeBytes = size_t(96) << 30;  // tape and records of every frame slot
 
 WideArgs MakeWideArgs(xrt_ctx* c) {
   WideArgs a{};
@@ -40,8 +40,10 @@ bool ObsContact(const xrt_ctx* c, size_t i) {
   return c->contact[i] && c->contact[i]->obs_nf > 0 && c->params.enable_obstacle_contact;
 }
 
-// validates the scene for the adjoint and sizes every buffer except the checkpoints
-int AdjSetup(xrt_ctx* c) {
+// validates the scene for the adjoint and sizes every buffer except the checkpoints; the
+// tape and the contact records get `want_slots` frame slots when they fit in 96 GiB and in
+// 60 % of the free device memory, else one
+int AdjSetup(xrt_ctx* c, int want_slots = 1) {
   XRT_REQUIRE(c->started, XRT_ERR_STATE, "xrt_start has not been called");
   XRT_REQUIRE(!c->params.enable_self_contact, XRT_ERR_UNSUPPORTED,
               "the adjoint does not support self contact (PBD repulsion)");
@@ -78,12 +80,28 @@ int AdjSetup(xrt_ctx* c) {
   A.off_e = A.off_c + (A.contacts ? N : 0);
   A.it_stride = A.off_e + (A.contacts ? N : 0);
   A.sub_stride = N + static_cast<size_t>(p.num_iterations) * A.it_stride;
-  const size_t tape_n = static_cast<size_t>(p.num_substeps) * A.sub_stride;
-  XRT_REQUIRE(tape_n * sizeof(float4) <= kMaxTapeBytes, XRT_ERR_CAPACITY,
-              "one frame of adjoint tape exceeds 96 GiB");
-  if (A.tape.n != tape_n) XRT_CHECK_HIP(A.tape.Alloc(tape_n));
+  const size_t frame_n = static_cast<size_t>(p.num_substeps) * A.sub_stride;
   A.frequency = std::max(1, p.num_substeps * p.num_iterations / p.num_collision_passes);
   A.n_passes = (p.num_substeps * p.num_iterations + A.frequency - 1) / A.frequency;
+  size_t rec_bytes = 0;  // contact records of one frame, every instance
+  for (int i = 0; i < n_inst; ++i) {
+    if (!ObsContact(c, i)) continue;
+    const HostInstance& h = c->inst[i];
+    rec_bytes += static_cast<size_t>(A.n_passes) *
+                 (static_cast<size_t>(h.nv) * 40 + 16 * static_cast<size_t>(c->contact[i]->obs_nv) +
+                  sizeof(EeRec) * std::max(h.ne(), 1));
+  }
+  const size_t frame_bytes = frame_n * sizeof(float4) + rec_bytes;
+  XRT_REQUIRE(frame_bytes <= kMaxTapeBytes, XRT_ERR_CAPACITY,
+              "one frame of adjoint tape exceeds 96 GiB");
+  size_t free_b = 0, total_b = 0;
+  XRT_CHECK_HIP(hipMemGetInfo(&free_b, &total_b));
+  const size_t held = A.tape.n * sizeof(float4);  // reusable
+  const size_t budget = std::min(kMaxTapeBytes, (free_b + held) / 10 * 6);
+  A.slots = (want_slots > 1 && static_cast<size_t>(want_slots) * frame_bytes <= budget) ? want_slots : 1;
+  A.slot = 0;
+  const size_t tape_n = static_cast<size_t>(A.slots) * frame_n;
+  if (A.tape.n != tape_n) XRT_CHECK_HIP(A.tape.Alloc(tape_n));
   A.rec.clear();
   A.rec.resize(n_inst);
   for (int i = 0; i < n_inst; ++i) {
@@ -91,7 +109,7 @@ int AdjSetup(xrt_ctx* c) {
     const HostInstance& h = c->inst[i];
     const ContactData& cd = *c->contact[i];
     auto r = std::make_unique<AdjointData::Records>();
-    const size_t np = static_cast<size_t>(A.n_passes);
+    const size_t np = static_cast<size_t>(A.n_passes) * A.slots;
     XRT_CHECK_HIP(r->vf_face.Alloc(np * h.nv));
     XRT_CHECK_HIP(r->rt_face.Alloc(np * h.nv));
     XRT_CHECK_HIP(r->vf_q.Alloc(np * h.nv));
@@ -149,14 +167,16 @@ int LoadState(xrt_ctx* c, const float4* src) {
   return XRT_OK;
 }
 
-// contact records of pass p of the current frame -> the tape
-int SnapshotRecords(xrt_ctx* c, int i, int pass) {
+// contact records of pass p of the current frame -> the records of the current frame slot
+int SnapshotRecords(xrt_ctx* c, int i, int frame_pass) {
   AdjointData& A = *c->adj;
   const HostInstance& h = c->inst[i];
   const ContactData& cd = *c->contact[i];
   AdjointData::Records& r = *A.rec[i];
   hipStream_t s = c->stream;
-  XRT_REQUIRE(pass >= 0 && pass < A.n_passes, XRT_ERR_STATE, "collision pass index out of range");
+  XRT_REQUIRE(frame_pass >= 0 && frame_pass < A.n_passes, XRT_ERR_STATE,
+              "collision pass index out of range");
+  const size_t pass = static_cast<size_t>(A.slot) * A.n_passes + frame_pass;
   const size_t nv = h.nv, ne = static_cast<size_t>(std::max(h.ne(), 1));
   XRT_CHECK_HIP(hipMemcpyAsync(r.vf_face.ptr + pass * nv, cd.vf_face.ptr, 4 * nv, hipMemcpyDeviceToDevice, s));
   XRT_CHECK_HIP(hipMemcpyAsync(r.rt_face.ptr + pass * nv, cd.rt_face.ptr, 4 * nv, hipMemcpyDeviceToDevice, s));
@@ -178,7 +198,10 @@ int WideSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, AdjointData
   hipStream_t s = c->stream;
   const int N = W.n_total, n_inst = static_cast<int>(c->inst.size());
   const WideArgs wa = MakeWideArgs(c);
-  float4* T = (A && sub >= 0) ? A->tape.ptr + static_cast<size_t>(sub) * A->sub_stride : nullptr;
+  float4* T = (A && sub >= 0)
+                  ? A->tape.ptr + (static_cast<size_t>(A->slot) * c->params.num_substeps + sub) *
+                                      A->sub_stride
+                  : nullptr;
   bool obs_contact = false;
   for (int i = 0; i < n_inst; ++i) obs_contact = obs_contact || ObsContact(c, i);
   int rc;
@@ -234,13 +257,14 @@ int ReverseSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, int sub)
   hipStream_t s = c->stream;
   const int N = A.n_total, n_inst = static_cast<int>(c->inst.size());
   const WideArgs wa = MakeWideArgs(c);
-  const float4* T = A.tape.ptr + static_cast<size_t>(sub) * A.sub_stride;
+  const float4* T =
+      A.tape.ptr + (static_cast<size_t>(A.slot) * c->params.num_substeps + sub) * A.sub_stride;
   XRT_CHECK_HIP(LaunchFinalizeAdj(s, A.gx0.ptr, A.gv.ptr, A.gx.ptr, N, k.damp_factor / k.dt));
   XRT_CHECK_HIP(LaunchZero1(s, A.glam.ptr, static_cast<int>(A.glam.n)));
   for (int it = k.iterations - 1; it >= 0; --it) {
     const float4* TI = T + N + static_cast<size_t>(it) * A.it_stride;
     if (A.contacts) {
-      const int pass = (sub * k.iterations + it) / A.frequency;
+      const int pass = A.slot * A.n_passes + (sub * k.iterations + it) / A.frequency;
       for (int i = 0; i < n_inst; ++i) {
         if (!ObsContact(c, i)) continue;
         const HostInstance& h = c->inst[i];
@@ -301,7 +325,7 @@ int xrt_adjoint_begin(xrt_ctx* ctx, int32_t nframes, int32_t checkpoint_every) {
   XRT_REQUIRE(ctx != nullptr, XRT_ERR_INVALID_ARG, "null context");
   XRT_REQUIRE(nframes > 0 && checkpoint_every > 0, XRT_ERR_INVALID_ARG,
               "nframes and checkpoint_every must be > 0");
-  int rc = AdjSetup(ctx);
+  int rc = AdjSetup(ctx, std::min(checkpoint_every, nframes));
   if (rc != XRT_OK) return rc;
   AdjointData& A = *ctx->adj;
   A.recorded = false;
@@ -367,6 +391,22 @@ int xrt_adjoint_backward(xrt_ctx* ctx) {
   for (int seg = n_ckpt - 1; seg >= 0; --seg) {
     const int f0 = seg * K, f1 = std::min(f0 + K, A.n_frames);
     if ((rc = LoadState(ctx, A.ckpt.ptr + static_cast<size_t>(seg) * 2 * N)) != XRT_OK) return rc;
+    if (A.slots >= f1 - f0) {
+      // one taped forward over the segment (frame f in slot f - f0), then its reverse
+      RewindContacts(ctx);
+      for (int f = f0; f < f1; ++f) {
+        A.slot = f - f0;
+        if ((rc = WideFrame(ctx, true)) != XRT_OK) return rc;
+      }
+      for (int f = f1 - 1; f >= f0; --f) {
+        A.slot = f - f0;
+        for (int sub = ctx->params.num_substeps - 1; sub >= 0; --sub)
+          if ((rc = ReverseSubstep(ctx, k, bm, sub)) != XRT_OK) return rc;
+      }
+      A.slot = 0;
+      continue;
+    }
+    // one frame slot: frame starts from an untaped pass, then each frame taped and reversed
     for (int f = f0; f < f1; ++f) {
       if ((rc = SaveState(ctx, A.fstart.ptr + static_cast<size_t>(f - f0) * 2 * N)) != XRT_OK) return rc;
       if (f + 1 < f1 && (rc = WideFrame(ctx, false)) != XRT_OK) return rc;

@@ -101,10 +101,14 @@ struct AdjointData {
   bool recorded = false, has_grad = false, contacts = false;
   int n_total = 0, e_total = 0, b_total = 0;  // copies of the context's WideConfig
   std::vector<int> s_max, b_max;
-  // tape of one frame: substep s at s * sub_stride; iteration it at n_total + it * it_stride
+  // tape of `slots` frames: frame slot f, substep s at (f * num_substeps + s) * sub_stride;
+  // iteration it at n_total + it * it_stride within the substep.  `slot` is the frame slot
+  // the taped forward writes and the reverse reads.
   size_t it_stride = 0, sub_stride = 0, off_s = 0, off_b = 0, off_g = 0, off_c = 0, off_e = 0;
+  int slots = 1, slot = 0;
   DevBuf<float4> tape;
-  // contact records of each collision pass of a frame, per instance
+  // contact records of each collision pass of each frame slot (slot * n_passes + pass), per
+  // instance
   struct Records {
     DevBuf<int32_t> vf_face, rt_face;
     DevBuf<float4> vf_q, rt_q, pred;
