@@ -209,7 +209,9 @@ int xrt_time_frames(xrt_ctx* ctx, int32_t batched, int32_t nframes, float* ms);
  *
  * xrt_adjoint_begin: runs `nframes` frames of every instance from the current state (a
  * frame boundary), keeping the state every `checkpoint_every` frames; the context is left
- * at the final state.  Tape memory is one frame (recomputed from the checkpoints). */
+ * at the final state.  The reverse pass recomputes each segment between checkpoints once
+ * with the tape on: `checkpoint_every` frames of tape when they fit (96 GiB and 60 % of the
+ * free device memory), else one frame at a time. */
 int xrt_adjoint_begin(xrt_ctx* ctx, int32_t nframes, int32_t checkpoint_every);
 /* dL/dx and dL/dv of the final state of one instance (3*nv each, NULL = zero); seeds not
  * given stay zero */
