@@ -140,3 +140,33 @@ def test_time_repulsion_counts():
                                             ground_height=-1.0), seed=3)
     with pytest.raises(RuntimeError):
         b.time_repulsion(1)
+
+
+def test_bench_workload_subset_frame_matches_plain():
+    """Instances drawn from the benchmark's own size distribution (grid sides up to 192,
+    sphere / capsule obstacles, self contact, default 10 x 10 substeps): one whole frame
+    through the captured graph equals the plain per-instance path bit for bit, with the
+    repulsion item lists in use on every instance."""
+    print("case bench workload subset: 24 instances, 1 frame, graph vs plain", flush=True)
+    specs = scenes.batched(seed=4, n_instances=256)[::11][:24]
+    xp = make_params(bend_model=1, ground_height=-1.0, enable_obstacle_contact=1,
+                     enable_self_contact=1)
+    a, ia = build_solver(specs, xp, seed=4)
+    b, ib = build_solver(specs, xp, seed=4)
+    a.FramesBatched(1)
+    a.sync()
+    b.FramesPlain(1)
+    worst = 0.0
+    for i, j in zip(ia, ib):
+        xa, va = a.GetState(i)
+        xb, vb = b.GetState(j)
+        worst = max(worst, float(np.abs(xa - xb).max()), float(np.abs(va - vb).max()))
+        ca, cb = a.Contacts(i), b.Contacts(j)
+        assert ca["pairs"] == cb["pairs"] and ca["obs_pairs"] == cb["obs_pairs"]
+        np.testing.assert_array_equal(ca["vf_face"], cb["vf_face"])
+    nv = sum(len(sp.pos) for sp in specs)
+    print(f"  {nv} vertices: max |batched - plain| = {worst:.3e}", flush=True)
+    assert worst == 0.0
+    _, cnt = a.time_repulsion(1)
+    print(f"  {cnt}", flush=True)
+    assert cnt["item_path_instances"] == len(specs)

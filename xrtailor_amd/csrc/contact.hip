@@ -730,6 +730,12 @@ __global__ __launch_bounds__(kBlock) void MultiClothGrowKernel(const InstContact
                c.cbvh.flags);
 }
 
+// one workgroup per instance merges its cloth tree (grow-only)
+__global__ __launch_bounds__(1024) void MultiClothBottomUpKernel(const InstContactDev* ic, int n) {
+  const InstContactDev& c = ic[blockIdx.x];
+  if (c.scal && c.nf > 1) BottomUpTreeWG(c.cbvh.nodes, c.cbvh.box, c.cbvh.flags, c.nf, 1);
+}
+
 // frame start: one workgroup per tree merges the freshly built trees of every instance
 // (workgroups [0, n): cloth trees when `cloth`, [n, 2n): obstacle trees)
 __global__ __launch_bounds__(1024) void MultiBuildMergeKernel(const InstContactDev* ic, int n,
@@ -745,11 +751,6 @@ __global__ __launch_bounds__(1024) void MultiBuildMergeKernel(const InstContactD
   }
 }
 
-// one workgroup per instance merges its cloth tree (grow-only)
-__global__ __launch_bounds__(1024) void MultiClothBottomUpKernel(const InstContactDev* ic, int n) {
-  const InstContactDev& c = ic[blockIdx.x];
-  if (c.scal && c.nf > 1) BottomUpTreeWG(c.cbvh.nodes, c.cbvh.box, c.cbvh.flags, c.nf, 1);
-}
 
 // traversal records of every cloth tree (thread i: leaf nf - 1 + i and internal node i)
 __global__ __launch_bounds__(kBlock) void MultiPackClothKernel(const InstContactDev* ic,
@@ -825,17 +826,28 @@ __global__ __launch_bounds__(1024) void MultiScanKernel(const InstContactDev* ic
   BlockExclusiveScan(c.iee, c.nf, c.scal + (self ? 8 : 6));
 }
 
+// the item lists of instance c are complete and used for VF (is_ee = 0) or EE tests: lists
+// on, every pair stored and every item within the list (valid once the totals of both
+// traversal passes are in scal)
+__device__ __forceinline__ bool ItemsUsable(const InstContactDev& c, int is_ee) {
+  return c.ee_cap > 0 && c.scal[2] <= c.pair_cap &&
+         c.scal[is_ee ? 4 : 3] <= (is_ee ? c.ee_cap : c.vf_cap);
+}
+
 // FindPairs of every instance's leaves (pairs of leaf i fill [lo, lo + n) backwards, slots
 // below the capacity only): copied from the count pass's stage when the leaf had at most
 // kPairStage hits, else found by a second traversal.  With the item lists on, the items of
 // leaf i's pairs (hit order) go to the lists from its prefix offsets, the cloth-pair items
-// after all obstacle-pair items; only slots below the list capacities are written.
+// after all obstacle-pair items; only slots below the list capacities are written.  The
+// cloth-pair pass runs after the totals: when both lists will be used, nothing reads the
+// cloth pairs and they are not stored.
 __global__ __launch_bounds__(kBlock) void MultiEmitPairsKernel(const InstContactDev* ic,
                                                                const int32_t* off, int n, float th,
                                                                int self) {
   XRT_MULTI_INDEX(kFace, c.nf)
   if (!self && !c.has_obs) return;
-  const int64_t cap = c.pair_cap;
+  const bool store = !self || !(ItemsUsable(c, 0) && ItemsUsable(c, 1));
+  const int64_t cap = store ? c.pair_cap : 0;
   const uint32_t cnt = c.cbvh.counts[i];
   const int64_t lo = static_cast<int64_t>(c.scal[self ? 1 : 0]) + c.cbvh.offsets[i];
   const Box lb = c.cbvh.packed[c.nf - 1 + i];
@@ -902,12 +914,6 @@ __global__ __launch_bounds__(kBlock) void MultiAddTotalsKernel(const InstContact
   }
 }
 
-// the item lists of instance c are complete and used for VF (is_ee = 0) or EE tests: lists
-// on, every pair stored and every item within the list
-__device__ __forceinline__ bool ItemsUsable(const InstContactDev& c, int is_ee) {
-  return c.ee_cap > 0 && c.scal[2] <= c.pair_cap &&
-         c.scal[is_ee ? 4 : 3] <= (is_ee ? c.ee_cap : c.vf_cap);
-}
 
 // repulsion tests of every instance, one VF (is_ee = 0) or EE item per thread; instances
 // whose list is not usable (off, overflowed, or pairs beyond the pair capacity) are left to
@@ -1091,12 +1097,14 @@ hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& 
   MultiPackClothKernel<<<XcdGrid(fb), kBlock, 0, s>>>(ic, fo, m.n);
   if (!Empty(m, MK::kObsFace))
     MultiPackObsKernel<<<XcdGrid(m.blocks[MK::kObsFace]), kBlock, 0, s>>>(ic, m.off[MK::kObsFace], m.n);
+  // obstacle pass, then the cloth pass, whose emit follows the totals (it knows whether the
+  // item lists will be used)
   for (int self = 0; self < 2; ++self) {
     MultiCountPairsKernel<<<XcdGrid(fb), kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
     MultiScanKernel<<<m.n, 1024, 0, s>>>(ic, m.n, self);
+    if (self) MultiAddTotalsKernel<<<Blocks(m.n), kBlock, 0, s>>>(ic, m.n);
     MultiEmitPairsKernel<<<XcdGrid(fb), kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
   }
-  MultiAddTotalsKernel<<<Blocks(m.n), kBlock, 0, s>>>(ic, m.n);
   return hipGetLastError();
 }
 
