@@ -3,7 +3,8 @@
 // Per-mesh contact topology (instance-local ids):
 //   tris int4[F], edges int2[E], face_edges int4[F] (edge of nodes i,i+1), ef_ptr/ef CSR
 //   (faces per edge, ascending, <= 4 used), face_n float4[F] rest face normals,
-//   rtri u32[F] representative-triangle bits (vertex i: bit i, edge i: bit 3+i)
+//   rtri u32[F] representative-triangle bits (vertex i: bit i, edge i: bit 3+i),
+//   trir int4[F] = (tris.xyz, rtri)
 // Contact records (per cloth vertex / edge):
 //   vf_face int, vf_q float4 (VF DCD); rt_face int, rt_q float4 (VF CCD);
 //   ee: float4 a = (dist, s, t, -), p2, q2, fn (EE DCD)
@@ -27,6 +28,7 @@ struct MeshTopo {
   const int32_t* ef = nullptr;
   const float4* face_n = nullptr;
   const uint32_t* rtri = nullptr;
+  const int4* trir = nullptr;  // (vertices, rtri bits) per face: one load for the item masks
 };
 
 struct MeshState {      // positions of one mesh (w of x / x0 = inverse mass)
@@ -104,6 +106,9 @@ struct InstContactDev {
   // stage[j * nf + i] as (rhs object, item mask) (null: the pairs are found by a second
   // traversal)
   int2* stage = nullptr;
+  // pair refresh: per 256-leaf block of the count pass, its pair / VF / EE item sums
+  // (3 x Blocks(nf)), then their exclusive prefix
+  uint32_t* bsum = nullptr;
 };
 
 constexpr int kPairStage = 32;

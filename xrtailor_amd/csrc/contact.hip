@@ -773,29 +773,59 @@ __global__ __launch_bounds__(kBlock) void MultiPackObsKernel(const InstContactDe
 // traversing the packed records, with the repulsion item counts of those pairs when the
 // item lists are on; the first kPairStage hits of leaf i are kept at stage[j * nf + i] as
 // (rhs object, item mask) (when the instance has a stage)
+// exclusive scan of three counters over the 256 threads of a block; tot = the block sums
+__device__ __forceinline__ void BlockScan3(uint32_t v[3], uint32_t tot[3]) {
+  __shared__ uint32_t wsum[kBlock / 64][3];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t inc[3] = {v[0], v[1], v[2]};
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const uint32_t t = __shfl_up(inc[q], d, 64);
+      if (lane >= d) inc[q] += t;
+    }
+  if (lane == 63)
+    for (int q = 0; q < 3; ++q) wsum[w][q] = inc[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    uint32_t before = 0, all = 0;
+    for (int j = 0; j < kBlock / 64; ++j) {
+      if (j < w) before += wsum[j][q];
+      all += wsum[j][q];
+    }
+    v[q] = before + inc[q] - v[q];
+    tot[q] = all;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void MultiCountPairsKernel(const InstContactDev* ic,
                                                                 const int32_t* off, int n,
                                                                 float th, int self) {
-  XRT_MULTI_INDEX(kFace, c.nf)
+  const int lblk = XcdBlock(off, n);
+  if (lblk < 0) return;
+  const int k = InstOfBlock(off, n, lblk);
+  const InstContactDev& c = ic[k];
+  const int lb = lblk - off[k];
+  const int i = lb * kBlock + threadIdx.x;  // every thread reaches the block scan
   uint32_t v = 0, nvf = 0, nee = 0;
-  if (self || c.has_obs) {
+  if (i < c.nf && (self || c.has_obs)) {
     const Box lb = c.cbvh.packed[c.nf - 1 + i];
     const int lobj = -__float_as_int(lb.lo.w) - 1;
     const bool items = c.ee_cap > 0;
     const MeshTopo& ot = self ? c.cloth : c.obs;
     int4 f0 = make_int4(0, 0, 0, 0);
-    uint32_t r0 = 0;
-    if (items) {
-      f0 = c.cloth.tris[lobj];
-      r0 = c.cloth.rtri[lobj];
-    }
+    if (items) f0 = c.cloth.trir[lobj];
+    const uint32_t r0 = static_cast<uint32_t>(f0.w);
     int2* st = c.stage + i;
     const int stride = c.nf;
     TraversePacked(self ? c.cbvh.packed : c.obvh.packed, self ? c.nf : c.obs_nf, lb, th,
                    [&](int robj) {
                      uint32_t m = 0;
                      if (items) {
-                       m = PairItemMask(f0, r0, ot.tris[robj], ot.rtri[robj], self != 0);
+                       const int4 f1 = ot.trir[robj];
+                       m = PairItemMask(f0, r0, f1, static_cast<uint32_t>(f1.w), self != 0);
                        nvf += __popc(m & 7u);
                        nee += __popc(m >> 3);
                      }
@@ -804,26 +834,38 @@ __global__ __launch_bounds__(kBlock) void MultiCountPairsKernel(const InstContac
                      ++v;
                    });
   }
-  c.cbvh.counts[i] = v;
-  c.cbvh.offsets[i] = v;
-  if (c.ee_cap > 0) {
-    c.ivf[i] = nvf;
-    c.iee[i] = nee;
+  // block-local exclusive prefixes per leaf, block sums for MultiScanKernel
+  uint32_t e[3] = {v, nvf, nee}, tot[3];
+  BlockScan3(e, tot);
+  if (i < c.nf) {
+    c.cbvh.counts[i] = v;
+    c.cbvh.offsets[i] = e[0];
+    if (c.ee_cap > 0) {
+      c.ivf[i] = e[1];
+      c.iee[i] = e[2];
+    }
+  }
+  if (threadIdx.x == 0) {
+    const int nb = (c.nf + kBlock - 1) / kBlock;
+    c.bsum[lb] = tot[0];
+    c.bsum[nb + lb] = tot[1];
+    c.bsum[2 * nb + lb] = tot[2];
   }
 }
 
-// one workgroup per instance: exclusive scan of the per-leaf counts; the sum goes to
-// scal[1] (obstacle pass) or to the tree's total (self pass); likewise the item counts,
-// whose sums go to scal[5], scal[6] (obstacle pass) or scal[7], scal[8]
+// one workgroup per instance: exclusive scan of the count pass's block sums (the per-leaf
+// offsets are these plus the block-local prefixes); the pair sum goes to scal[1] (obstacle
+// pass) or to the tree's total (cloth pass), the item sums to scal[5], scal[6] (obstacle
+// pass) or scal[7], scal[8]
 __global__ __launch_bounds__(1024) void MultiScanKernel(const InstContactDev* ic, int n, int self) {
   const InstContactDev& c = ic[blockIdx.x];
   if (c.nf <= 0 || !c.scal) return;
-  BlockExclusiveScan(c.cbvh.offsets, c.nf, self ? c.cbvh.total : c.scal + 1);
-  if (c.ee_cap <= 0) return;
+  const int nb = (c.nf + kBlock - 1) / kBlock;
+  BlockExclusiveScan(c.bsum, nb, self ? c.cbvh.total : c.scal + 1);
   __syncthreads();  // the scans share their workgroup buffer
-  BlockExclusiveScan(c.ivf, c.nf, c.scal + (self ? 7 : 5));
+  BlockExclusiveScan(c.bsum + nb, nb, c.scal + (self ? 7 : 5));
   __syncthreads();
-  BlockExclusiveScan(c.iee, c.nf, c.scal + (self ? 8 : 6));
+  BlockExclusiveScan(c.bsum + 2 * nb, nb, c.scal + (self ? 8 : 6));
 }
 
 // the item lists of instance c are complete and used for VF (is_ee = 0) or EE tests: lists
@@ -849,7 +891,8 @@ __global__ __launch_bounds__(kBlock) void MultiEmitPairsKernel(const InstContact
   const bool store = !self || !(ItemsUsable(c, 0) && ItemsUsable(c, 1));
   const int64_t cap = store ? c.pair_cap : 0;
   const uint32_t cnt = c.cbvh.counts[i];
-  const int64_t lo = static_cast<int64_t>(c.scal[self ? 1 : 0]) + c.cbvh.offsets[i];
+  const int nb = (c.nf + kBlock - 1) / kBlock, blk = lblk - off[k];
+  const int64_t lo = static_cast<int64_t>(c.scal[self ? 1 : 0]) + c.bsum[blk] + c.cbvh.offsets[i];
   const Box lb = c.cbvh.packed[c.nf - 1 + i];
   const int lobj = -__float_as_int(lb.lo.w) - 1;
   const bool items = c.ee_cap > 0;
@@ -862,8 +905,8 @@ __global__ __launch_bounds__(kBlock) void MultiEmitPairsKernel(const InstContact
     f0 = c.cloth.tris[lobj];
     fe0 = c.cloth.face_edges[lobj];
     r0 = c.cloth.rtri[lobj];
-    vo = (self ? static_cast<int64_t>(c.scal[5]) : 0) + c.ivf[i];
-    eo = (self ? static_cast<int64_t>(c.scal[6]) : 0) + c.iee[i];
+    vo = (self ? static_cast<int64_t>(c.scal[5]) : 0) + c.bsum[nb + blk] + c.ivf[i];
+    eo = (self ? static_cast<int64_t>(c.scal[6]) : 0) + c.bsum[2 * nb + blk] + c.iee[i];
   }
   auto emit_items = [&](int robj, uint32_t m) {
     if (!m) return;
@@ -890,9 +933,11 @@ __global__ __launch_bounds__(kBlock) void MultiEmitPairsKernel(const InstContact
                    [&](int robj) {
                      if (index >= lo && index < cap) c.pairs[index] = make_int2(lobj, robj);
                      --index;
-                     if (items)
-                       emit_items(robj, PairItemMask(f0, r0, ot.tris[robj], ot.rtri[robj],
+                     if (items) {
+                       const int4 f1 = ot.trir[robj];
+                       emit_items(robj, PairItemMask(f0, r0, f1, static_cast<uint32_t>(f1.w),
                                                      self != 0));
+                     }
                    });
     return;
   }

@@ -31,12 +31,13 @@ constexpr float kPairThickness = 1e-6f;  // UpdatePairs thickness (ClothSolverBa
 int UploadTopo(const ContactTopo& t, const std::vector<int32_t>& tris,
                const std::vector<int32_t>& edges, DevBuf<int4>& dt, DevBuf<int4>& dfe,
                DevBuf<int2>& de, DevBuf<int32_t>& dptr, DevBuf<int32_t>& def,
-               DevBuf<float4>& dn, DevBuf<uint32_t>& dr, hipStream_t s) {
+               DevBuf<float4>& dn, DevBuf<uint32_t>& dr, DevBuf<int4>& dtr, hipStream_t s) {
   const int nt = static_cast<int>(tris.size() / 3), ne = static_cast<int>(edges.size() / 2);
-  std::vector<int4> t4(std::max(nt, 1)), fe4(std::max(nt, 1));
+  std::vector<int4> t4(std::max(nt, 1)), fe4(std::max(nt, 1)), tr4(std::max(nt, 1));
   std::vector<float4> n4(std::max(nt, 1));
   for (int f = 0; f < nt; ++f) {
     t4[f] = make_int4(tris[3 * f], tris[3 * f + 1], tris[3 * f + 2], 0);
+    tr4[f] = make_int4(tris[3 * f], tris[3 * f + 1], tris[3 * f + 2], static_cast<int>(t.rtri[f]));
     fe4[f] = make_int4(t.face_edges[3 * f], t.face_edges[3 * f + 1], t.face_edges[3 * f + 2], 0);
     n4[f] = make_float4(t.face_n[3 * f], t.face_n[3 * f + 1], t.face_n[3 * f + 2], 0.0f);
   }
@@ -49,6 +50,7 @@ int UploadTopo(const ContactTopo& t, const std::vector<int32_t>& tris,
   XRT_CHECK_HIP(dptr.Upload(t.ef_ptr, s));
   XRT_CHECK_HIP(def.Upload(t.ef, s));
   XRT_CHECK_HIP(dr.Upload(t.rtri, s));
+  XRT_CHECK_HIP(dtr.Upload(tr4, s));
   return XRT_OK;
 }
 
@@ -64,6 +66,7 @@ MeshTopo ClothTopo(const ContactData& c, const HostInstance& h) {
   t.ef = c.c_ef.ptr;
   t.face_n = c.c_face_n.ptr;
   t.rtri = c.c_rtri.ptr;
+  t.trir = c.c_trir.ptr;
   return t;
 }
 
@@ -79,6 +82,7 @@ MeshTopo ObsTopo(const ContactData& c) {
   t.ef = c.o_ef.ptr;
   t.face_n = c.o_face_n.ptr;
   t.rtri = c.o_rtri.ptr;
+  t.trir = c.o_trir.ptr;
   return t;
 }
 
@@ -103,7 +107,7 @@ int ContactStart(xrt_ctx* ctx) {
     auto c = std::make_unique<ContactData>();
     h.cloth_topo = BuildContactTopo(h.rest_pos.data(), h.nv, h.tris.data(), h.nt, h.edges);
     int rc = UploadTopo(h.cloth_topo, h.tris, h.edges, c->c_tris, c->c_fe, c->c_edges, c->c_ef_ptr,
-                        c->c_ef, c->c_face_n, c->c_rtri, s);
+                        c->c_ef, c->c_face_n, c->c_rtri, c->c_trir, s);
     if (rc != XRT_OK) return rc;
     XRT_CHECK_HIP(c->cloth_bvh.Setup(h.tris.data(), h.nt, h.nv, s));
     if (h.obs_nt() > 0) {
@@ -113,7 +117,7 @@ int ContactStart(xrt_ctx* ctx) {
       h.obs_topo = BuildContactTopo(h.obs_pos.data(), c->obs_nv, h.obs_tris.data(), c->obs_nf,
                                     h.obs_edges);
       rc = UploadTopo(h.obs_topo, h.obs_tris, h.obs_edges, c->o_tris, c->o_fe, c->o_edges,
-                      c->o_ef_ptr, c->o_ef, c->o_face_n, c->o_rtri, s);
+                      c->o_ef_ptr, c->o_ef, c->o_face_n, c->o_rtri, c->o_trir, s);
       if (rc != XRT_OK) return rc;
       const std::vector<float4> p = ToF4(h.obs_pos);
       XRT_CHECK_HIP(c->o_x0.Upload(p, s));
@@ -162,6 +166,7 @@ int ContactStart(xrt_ctx* ctx) {
     }
     if (ctx->params.enable_self_contact)
       XRT_CHECK_HIP(c->stage.Alloc(static_cast<size_t>(kPairStage) * std::max(h.nt, 1)));
+    XRT_CHECK_HIP(c->bsum.Alloc(3 * static_cast<size_t>(std::max(1, (h.nt + 255) / 256))));
     ctx->contact[ii] = std::move(c);
   }
   XRT_CHECK_HIP(hipStreamSynchronize(s));
@@ -330,6 +335,7 @@ int MultiContactSetup(xrt_ctx* ctx) {
       d.rep_vf = c.rep_vf.ptr;
       d.rep_ee = c.rep_ee.ptr;
       d.stage = c.stage.ptr;
+      d.bsum = c.bsum.ptr;
       d.ivf = c.ivf.ptr;
       d.iee = c.iee.ptr;
       cnt[MultiMaps::kVert] = blocks(h.nv);

@@ -58,6 +58,7 @@ struct ContactData {
   DevBuf<int32_t> c_ef_ptr, c_ef;
   DevBuf<float4> c_face_n;
   DevBuf<uint32_t> c_rtri;
+  DevBuf<int4> c_trir;
   // obstacle topology and state
   int obs_nv = 0, obs_nf = 0, obs_ne = 0;
   DevBuf<int4> o_tris, o_fe;
@@ -65,6 +66,7 @@ struct ContactData {
   DevBuf<int32_t> o_ef_ptr, o_ef;
   DevBuf<float4> o_face_n;
   DevBuf<uint32_t> o_rtri;
+  DevBuf<int4> o_trir;
   DevBuf<float4> o_x0, o_x, o_x1, o_n1;
   DevBuf<int32_t> o_vptr, o_corner;  // vertex -> corner CSR for normals
   // trees
@@ -85,6 +87,7 @@ struct ContactData {
   DevBuf<int2> rep_vf, rep_ee;
   DevBuf<uint32_t> ivf, iee;  // per cloth leaf item counts / offsets
   DevBuf<int2> stage;  // kPairStage hits per cloth leaf of the batched pair refresh
+  DevBuf<uint32_t> bsum;  // per-block sums of the batched pair refresh
   int frame_substep = 0, pass_cnt = 0;
   bool obstacle_moving = false;  // xrt_update_obstacle since the last frame end
 };
