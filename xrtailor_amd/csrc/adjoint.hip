@@ -33,12 +33,34 @@ __device__ __forceinline__ void Acc(float3& a, float3 b) { a.x += b.x; a.y += b.
 }  // namespace
 
 // ------------------------------------------------------------------ forward
+// (instance, block within its color range) of this block: from the color's block map
+// (walked in XCD-contiguous order: the hardware deals workgroups to the 8 XCDs in turn),
+// or grid.y / grid.x.  false: idle block.
+__device__ __forceinline__ bool WideBlock(const int32_t* __restrict__ map, int n_inst, int& inst,
+                                          int& blk) {
+  if (!map) {
+    inst = blockIdx.y;
+    blk = blockIdx.x;
+    return true;
+  }
+  const int total = map[n_inst];
+  const int l = static_cast<int>(blockIdx.x & 7u) * ((total + 7) >> 3) + static_cast<int>(blockIdx.x >> 3);
+  if (l >= total) return false;
+  inst = map[n_inst + 1 + l];
+  blk = l - map[inst];
+  return true;
+}
+
 __global__ __launch_bounds__(kBs) void WideStretchKernel(WideArgs a, int color, float inv_dt2,
-                                                         int iter, float4* __restrict__ tape) {
-  const InstDesc d = a.inst[blockIdx.y];
+                                                         int iter, float4* __restrict__ tape,
+                                                         const int32_t* __restrict__ map,
+                                                         int n_inst) {
+  int inst, blk;
+  if (!WideBlock(map, n_inst, inst, blk)) return;
+  const InstDesc d = a.inst[inst];
   if (color >= d.s_ncol) return;
   const int beg = a.s_coloff[d.s_cbeg + color], end = a.s_coloff[d.s_cbeg + color + 1];
-  const int k = beg + blockIdx.x * kBs + threadIdx.x;
+  const int k = beg + blk * kBs + threadIdx.x;
   if (k >= end) return;
   float4* X = a.x + d.vbeg;
   const StretchC c = a.sc[k];
@@ -51,11 +73,15 @@ __global__ __launch_bounds__(kBs) void WideStretchKernel(WideArgs a, int color, 
 }
 
 __global__ __launch_bounds__(kBs) void WideBendKernel(WideArgs a, int color, float inv_dt2,
-                                                      BendMode m, float4* __restrict__ tape) {
-  const InstDesc d = a.inst[blockIdx.y];
+                                                      BendMode m, float4* __restrict__ tape,
+                                                      const int32_t* __restrict__ map,
+                                                      int n_inst) {
+  int inst, blk;
+  if (!WideBlock(map, n_inst, inst, blk)) return;
+  const InstDesc d = a.inst[inst];
   if (color >= d.b_ncol) return;
   const int beg = a.b_coloff[d.b_cbeg + color], end = a.b_coloff[d.b_cbeg + color + 1];
-  const int k = beg + blockIdx.x * kBs + threadIdx.x;
+  const int k = beg + blk * kBs + threadIdx.x;
   if (k >= end) return;
   float4* X = a.x + d.vbeg;
   const int4 id = a.bidx[k];
@@ -284,15 +310,29 @@ __global__ __launch_bounds__(kBs) void WideBendAdjKernel(WideArgs a, int color, 
 
 // ------------------------------------------------------------------ launchers
 hipError_t LaunchWideStretch(hipStream_t s, const WideArgs& a, int n_inst, int color, int max_cnt,
-                             float inv_dt2, int iter, float4* tape) {
+                             float inv_dt2, int iter, float4* tape, const int32_t* map,
+                             int blocks) {
   if (max_cnt <= 0 || n_inst <= 0) return hipSuccess;
-  WideStretchKernel<<<dim3(Blocks(max_cnt), n_inst), kBs, 0, s>>>(a, color, inv_dt2, iter, tape);
+  if (map) {
+    if (blocks > 0)
+      WideStretchKernel<<<(blocks + 7) / 8 * 8, kBs, 0, s>>>(a, color, inv_dt2, iter, tape, map, n_inst);
+  } else {
+    WideStretchKernel<<<dim3(Blocks(max_cnt), n_inst), kBs, 0, s>>>(a, color, inv_dt2, iter, tape,
+                                                                    nullptr, n_inst);
+  }
   return hipGetLastError();
 }
 hipError_t LaunchWideBend(hipStream_t s, const WideArgs& a, int n_inst, int color, int max_cnt,
-                          float inv_dt2, const BendMode& m, float4* tape) {
+                          float inv_dt2, const BendMode& m, float4* tape, const int32_t* map,
+                          int blocks) {
   if (max_cnt <= 0 || n_inst <= 0) return hipSuccess;
-  WideBendKernel<<<dim3(Blocks(max_cnt), n_inst), kBs, 0, s>>>(a, color, inv_dt2, m, tape);
+  if (map) {
+    if (blocks > 0)
+      WideBendKernel<<<(blocks + 7) / 8 * 8, kBs, 0, s>>>(a, color, inv_dt2, m, tape, map, n_inst);
+  } else {
+    WideBendKernel<<<dim3(Blocks(max_cnt), n_inst), kBs, 0, s>>>(a, color, inv_dt2, m, tape,
+                                                                 nullptr, n_inst);
+  }
   return hipGetLastError();
 }
 hipError_t LaunchCopy4(hipStream_t s, float4* dst, const float4* src, int n) {

@@ -218,11 +218,13 @@ int WideSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, AdjointData
           return rc;
     for (int col = 0; col < static_cast<int>(W.s_max.size()); ++col)
       XRT_CHECK_HIP(LaunchWideStretch(s, wa, n_inst, col, W.s_max[col], k.inv_dt2, it,
-                                      TI ? TI + A->off_s : nullptr));
+                                      TI ? TI + A->off_s : nullptr, W.maps.ptr + W.s_map[col],
+                                      W.s_blocks[col]));
     if (k.solve_bending)
       for (int col = 0; col < static_cast<int>(W.b_max.size()); ++col)
         XRT_CHECK_HIP(LaunchWideBend(s, wa, n_inst, col, W.b_max[col], k.inv_dt2, bm,
-                                     TI ? TI + A->off_b : nullptr));
+                                     TI ? TI + A->off_b : nullptr, W.maps.ptr + W.b_map[col],
+                                     W.b_blocks[col]));
     if (k.enable_ground) {
       if (TI) XRT_CHECK_HIP(LaunchCopy4(s, TI + A->off_g, c->x.ptr, N));
       XRT_CHECK_HIP(LaunchGround(s, c->x.ptr, c->x0.ptr, N, k.ground_y, k.friction));

@@ -571,7 +571,8 @@ int xrt_start(xrt_ctx* ctx, uint64_t seed) {
   if (ctx->batched_lds > 64 * 1024) XRT_CHECK_HIP(ConfigureBatchedKernel(ctx->batched_lds));
   // wide launches (one color of every instance): grid sized by the largest color
   WideConfig& w = ctx->wide;
-  w = WideConfig{};
+  w.s_max.clear();
+  w.b_max.clear();
   w.n_total = vtot;
   w.e_total = stot;
   w.b_total = btot;
@@ -584,6 +585,33 @@ int xrt_start(xrt_ctx* ctx, uint64_t seed) {
       if (col >= static_cast<int>(w.b_max.size())) w.b_max.push_back(0);
       w.b_max[col] = std::max(w.b_max[col], h.b_coloff[col + 1] - h.b_coloff[col]);
     }
+  }
+  {  // per-color block maps of the wide launches (only blocks with elements are launched)
+    const int ni = static_cast<int>(ctx->inst.size());
+    std::vector<int32_t> maps;
+    auto build = [&](bool stretch, std::vector<size_t>& seg, std::vector<int>& tot) {
+      const size_t ncol = stretch ? w.s_max.size() : w.b_max.size();
+      seg.assign(ncol, 0);
+      tot.assign(ncol, 0);
+      for (size_t col = 0; col < ncol; ++col) {
+        std::vector<int32_t> o(ni + 1, 0);
+        for (int k = 0; k < ni; ++k) {
+          const HostInstance& h = ctx->inst[k];
+          const int nc = stretch ? h.s_ncol : h.b_ncol;
+          const std::vector<int32_t>& co = stretch ? h.s_coloff : h.b_coloff;
+          const int cnt = static_cast<int>(col) < nc ? co[col + 1] - co[col] : 0;
+          o[k + 1] = o[k] + (cnt + 255) / 256;
+        }
+        seg[col] = maps.size();
+        tot[col] = o[ni];
+        maps.insert(maps.end(), o.begin(), o.end());
+        for (int k = 0; k < ni; ++k) maps.insert(maps.end(), static_cast<size_t>(o[k + 1] - o[k]), k);
+      }
+    };
+    build(true, w.s_map, w.s_blocks);
+    build(false, w.b_map, w.b_blocks);
+    XRT_CHECK_HIP(w.maps.Upload(maps, ctx->stream));
+    XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   }
   int rc = ContactStart(ctx);
   if (rc != XRT_OK) return rc;

@@ -96,6 +96,11 @@ struct ContactData {
 struct WideConfig {
   int n_total = 0, e_total = 0, b_total = 0;  // vertices / stretch / bend over instances
   std::vector<int> s_max, b_max;              // largest color c over instances
+  // per color: offset of its block map in `maps` (n + 1 prefix sums of the instances'
+  // 256-thread block counts, then the instance of every block) and its block total
+  std::vector<size_t> s_map, b_map;
+  std::vector<int> s_blocks, b_blocks;
+  DevBuf<int32_t> maps;
 };
 
 // Adjoint (differentiable rollout) state: checkpoints, one frame of tape, adjoints.
