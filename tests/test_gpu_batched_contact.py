@@ -170,3 +170,27 @@ def test_bench_workload_subset_frame_matches_plain():
     _, cnt = a.time_repulsion(1)
     print(f"  {cnt}", flush=True)
     assert cnt["item_path_instances"] == len(specs)
+
+
+def test_garment_lra_contacts_batched_matches_plain():
+    """Tube garments with long-range attachments (top ring) around moving capsule-chain
+    bodies, body and self contact: whole frames through the captured graph (wide LRA
+    launches included) equal the plain path bit for bit, the body updated every frame."""
+    print("case garments: LRA + contacts + animated bodies, graph vs plain", flush=True)
+    pairs = [scenes.garment(seed=6 + k, ring=24 + 8 * k, rows=16 + 4 * k, joints=4, gap=0.002,
+                            body_subdiv=2) for k in range(3)]
+    specs = [p for p, _ in pairs]
+    xp = make_params(bend_model=1, enable_obstacle_contact=1, enable_self_contact=1,
+                     ground_height=-1.0, num_substeps=4, num_iterations=6,
+                     num_collision_passes=6, pair_update_interval=3)
+    a, ia = build_solver(specs, xp, seed=3)
+    b, ib = build_solver(specs, xp, seed=3)
+    for f in range(3):
+        for (spec, body), i, j in zip(pairs, ia, ib):
+            target = body.at((f + 1) / 60.0)
+            a.UpdateObstacle(i, target)
+            b.UpdateObstacle(j, target)
+        a.FramesBatched(1)
+        a.sync()
+        b.FramesPlain(1)
+        assert _compare(a, ia, b, ib, f"frame {f}") == 0.0

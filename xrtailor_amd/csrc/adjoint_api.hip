@@ -225,6 +225,10 @@ int WideSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, AdjointData
         XRT_CHECK_HIP(LaunchWideBend(s, wa, n_inst, col, W.b_max[col], k.inv_dt2, bm,
                                      TI ? TI + A->off_b : nullptr, W.maps.ptr + W.b_map[col],
                                      W.b_blocks[col]));
+    if (c->total_slots > 0)  // never taped: the adjoint refuses long-range attachments
+      XRT_CHECK_HIP(LaunchWideLra(s, c->desc.ptr, n_inst, c->max_slots, c->max_nv, c->x.ptr,
+                                  c->slot_vert.ptr, c->slot_pos_all.ptr, c->lra_ptr.ptr,
+                                  c->lra_slot.ptr, c->lra_rest.ptr, k.stretchiness));
     if (k.enable_ground) {
       if (TI) XRT_CHECK_HIP(LaunchCopy4(s, TI + A->off_g, c->x.ptr, N));
       XRT_CHECK_HIP(LaunchGround(s, c->x.ptr, c->x0.ptr, N, k.ground_y, k.friction));

@@ -169,12 +169,7 @@ int SubstepPlain(xrt_ctx* c, int32_t inst) {
 // (each color, contact phase and LBVH operation of every instance in one launch).
 int SubstepBatched(xrt_ctx* c) {
   const StepConsts k = MakeConsts(c->params);
-  if (ContactsActive(c)) {
-    for (const auto& h : c->inst)
-      XRT_REQUIRE(h.desc.nslots == 0, XRT_ERR_UNSUPPORTED,
-                  "batched contacts with long-range attachments: use the plain path");
-    return WideSubstep(c, k, ModeOf(k), nullptr, -1);
-  }
+  if (ContactsActive(c)) return WideSubstep(c, k, ModeOf(k), nullptr, -1);
   XRT_CHECK_HIP(LaunchBatchedSubstep(c->stream, MakeBatchArgs(c), k,
                                      static_cast<int>(c->inst.size()), c->batched_block,
                                      c->batched_lds));
@@ -553,6 +548,8 @@ int xrt_start(xrt_ctx* ctx, uint64_t seed) {
   XRT_CHECK_HIP(ctx->lra_slot.Upload(hlslot, s));
   XRT_CHECK_HIP(ctx->lra_rest.Upload(hlrest, s));
   XRT_CHECK_HIP(ctx->slot_pos.Alloc(std::max(max_slots, 1)));
+  ctx->total_slots = slot_tot;
+  XRT_CHECK_HIP(ctx->slot_pos_all.Alloc(std::max(slot_tot, 1)));
   XRT_CHECK_HIP(ctx->desc.Upload(hdesc, s));
   XRT_CHECK_HIP(ctx->order.Upload(order, s));
   XRT_CHECK_HIP(hipMemsetAsync(ctx->slam.ptr, 0, sizeof(float) * std::max(stot, 1), s));
@@ -565,6 +562,7 @@ int xrt_start(xrt_ctx* ctx, uint64_t seed) {
   if (lds_cap > slot_bytes) lds_verts = static_cast<int>((lds_cap - slot_bytes) / sizeof(float4));
   int max_nv = 0;
   for (const auto& h : ctx->inst) max_nv = std::max(max_nv, h.nv);
+  ctx->max_nv = max_nv;
   lds_verts = std::min(lds_verts, max_nv);
   ctx->lds_verts = lds_verts;
   ctx->batched_lds = sizeof(float4) * (static_cast<size_t>(lds_verts) + max_slots);

@@ -152,6 +152,8 @@ struct xrt_ctx {
   xrt::DevBuf<float2> bpar;
   xrt::DevBuf<int32_t> s_coloff, b_coloff;
   xrt::DevBuf<int32_t> slot_vert, lra_ptr, lra_slot;
+  xrt::DevBuf<float4> slot_pos_all;  // slot snapshot of every instance (wide LRA)
+  int total_slots = 0, max_nv = 0;
   xrt::DevBuf<float> lra_rest;
   xrt::DevBuf<float4> slot_pos;  // plain-path LRA snapshot
   xrt::DevBuf<xrt::InstDesc> desc;

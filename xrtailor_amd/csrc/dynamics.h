@@ -10,6 +10,11 @@ hipError_t LaunchStretch(hipStream_t s, const StretchC* sc, float* lam, float4* 
                          float inv_dt2, int iter);
 hipError_t LaunchBend(hipStream_t s, const int4* bidx, const float2* bpar, float4* x, int cnt,
                       float inv_dt2, const BendMode& m);
+// LRA of every instance (slot_pos_all: one slot snapshot per slot of every instance)
+hipError_t LaunchWideLra(hipStream_t s, const InstDesc* inst, int n_inst, int max_slots, int max_nv,
+                         float4* x, const int32_t* slot_vert, float4* slot_pos_all,
+                         const int32_t* ptr, const int32_t* slot, const float* rest,
+                         float stretchiness);
 hipError_t LaunchLra(hipStream_t s, float4* x, const int32_t* slot_vert, float4* slot_pos, int ns,
                      const int32_t* ptr, const int32_t* slot, const float* rest, int n,
                      float stretchiness);

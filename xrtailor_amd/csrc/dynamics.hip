@@ -192,6 +192,44 @@ hipError_t LaunchLra(hipStream_t s, float4* x, const int32_t* slot_vert, float4*
   LraKernel<<<Blocks(n, 256), 256, 0, s>>>(x, slot_pos, ptr, slot, rest, n, stretchiness);
   return hipGetLastError();
 }
+// every instance's LRA pass in two launches: the slot snapshot of all instances, then every
+// target vertex (the same bodies and slot ordinals as LaunchLra per instance)
+__global__ __launch_bounds__(256) void WideGatherSlotsKernel(const InstDesc* __restrict__ inst,
+                                                             const float4* __restrict__ x,
+                                                             const int32_t* __restrict__ slot_vert,
+                                                             float4* __restrict__ slot_pos) {
+  const InstDesc d = inst[blockIdx.y];
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j < d.nslots) slot_pos[d.slot_beg + j] = x[d.vbeg + slot_vert[d.slot_beg + j]];
+}
+
+__global__ __launch_bounds__(256) void WideLraKernel(const InstDesc* __restrict__ inst,
+                                                     float4* __restrict__ x,
+                                                     const float4* __restrict__ slot_pos,
+                                                     const int32_t* __restrict__ ptr,
+                                                     const int32_t* __restrict__ slot,
+                                                     const float* __restrict__ rest,
+                                                     float stretchiness) {
+  const InstDesc d = inst[blockIdx.y];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (d.nslots == 0 || i >= d.nv) return;
+  const int kb = ptr[d.lra_vbeg + i], ke = ptr[d.lra_vbeg + i + 1];
+  if (kb == ke) return;
+  x[d.vbeg + i] = LraVertex(x[d.vbeg + i], slot_pos + d.slot_beg, slot, rest, kb, ke, stretchiness);
+}
+
+hipError_t LaunchWideLra(hipStream_t s, const InstDesc* inst, int n_inst, int max_slots, int max_nv,
+                         float4* x, const int32_t* slot_vert, float4* slot_pos_all,
+                         const int32_t* ptr, const int32_t* slot, const float* rest,
+                         float stretchiness) {
+  if (n_inst <= 0 || max_slots <= 0 || max_nv <= 0) return hipSuccess;
+  WideGatherSlotsKernel<<<dim3(Blocks(max_slots, 256), n_inst), 256, 0, s>>>(inst, x, slot_vert,
+                                                                            slot_pos_all);
+  WideLraKernel<<<dim3(Blocks(max_nv, 256), n_inst), 256, 0, s>>>(inst, x, slot_pos_all, ptr, slot,
+                                                                  rest, stretchiness);
+  return hipGetLastError();
+}
+
 hipError_t LaunchGround(hipStream_t s, float4* x, const float4* x0, int n, float ground_y,
                         float mu) {
   if (n <= 0) return hipSuccess;

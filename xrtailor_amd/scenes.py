@@ -7,6 +7,7 @@ Workloads (sizes follow the solver's benchmark configurations):
   hanging_cloth   64x64 grid, 1 m, two top corners pinned, stretch 1e-8 / bend 1e-4
   drop_cloth      256x256 grid, 2 m, 0.5 m above an icosphere (s5, r 0.5)
   batched         256 instances, side round(32*6^u), log-uniform compliances
+  garment         tube garment (top ring = LRA slots) around a skinned capsule-chain body
   diff_rollout    128x128 cloth over a sphere
 """
 from __future__ import annotations
@@ -108,6 +109,100 @@ def capsule(radius: float, half_length: float, subdiv: int = 3, center=(0.0, 0.0
     return pos + np.asarray(center, float), tris
 
 
+def tube(ring: int, rows: int, radius: float, height: float, y0: float = 0.0):
+    """Open cylinder around the y axis: `rows` rings of `ring` vertices, ring j at height
+    y0 + height * j / (rows - 1); vertex (i, j) has index j*ring + i.  Quads are split on
+    the (i,j)-(i+1,j+1) diagonal, wound counter-clockwise seen from outside."""
+    th = 2.0 * np.pi * np.arange(ring) / ring
+    ys = y0 + height * np.arange(rows) / (rows - 1)
+    pos = np.empty((rows * ring, 3))
+    for j in range(rows):
+        pos[j * ring:(j + 1) * ring] = np.stack(
+            [radius * np.cos(th), np.full(ring, ys[j]), radius * np.sin(th)], 1)
+    tris = []
+    for j in range(rows - 1):
+        for i in range(ring):
+            a, b = j * ring + i, j * ring + (i + 1) % ring
+            c, d = (j + 1) * ring + (i + 1) % ring, (j + 1) * ring + i
+            tris += [(a, c, b), (a, d, c)]
+    return pos, np.array(tris, np.int32)
+
+
+def capsule_y(radius: float, half_length: float, subdiv: int = 3, center=(0.0, 0.0, 0.0)):
+    """Capsule along y: an icosphere whose +y / -y halves are moved apart."""
+    pos, tris = icosphere(subdiv, radius)
+    pos[:, 1] += np.sign(pos[:, 1]) * half_length * (np.abs(pos[:, 1]) > 1e-9)
+    return pos + np.asarray(center, float), tris
+
+
+@dataclass
+class ChainBody:
+    """A capsule-chain body: one capsule along y over `joints` segments of length `seg`
+    starting at y0, skinned to the chain by linear blend skinning.  Joint j bends the
+    chain above it about the x and z axes by theta_j(t) = sum_k a_jk (sin(2 pi f_jk t +
+    phi_jk) - sin(phi_jk)) (three terms per axis, f in [0.2, 2] Hz, sum |a| <= 0.6 rad per
+    joint; the offset starts the motion from the rest pose)."""
+    pos: np.ndarray    # rest vertices (nb, 3)
+    tris: np.ndarray
+    y0: float
+    seg: float
+    amp: np.ndarray    # (joints, 2, 3) amplitudes, frequencies, phases per axis and term
+    freq: np.ndarray
+    phase: np.ndarray
+
+    def angles(self, t: float) -> np.ndarray:
+        """(joints, 2) bend angles about x and z, zero at t = 0 (the rest pose)."""
+        a = self.amp * (np.sin(2.0 * np.pi * self.freq * t + self.phase) - np.sin(self.phase))
+        return a.sum(-1)
+
+    def at(self, t: float) -> np.ndarray:
+        """Skinned vertex positions at time t (float32)."""
+        nj = self.amp.shape[0]
+        ang = self.angles(t)
+        rot = np.eye(3)
+        rots, origins = [], []
+        p_rest = np.array([0.0, self.y0, 0.0])
+        p_now = p_rest.copy()
+        for j in range(nj):  # forward kinematics: joint j at y0 + j seg
+            ax, az = ang[j]
+            cx, sx, cz, sz = np.cos(ax), np.sin(ax), np.cos(az), np.sin(az)
+            rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+            rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+            rot = rot @ rx @ rz
+            rots.append(rot.copy())
+            origins.append((p_rest.copy(), p_now.copy()))
+            step = np.array([0.0, self.seg, 0.0])
+            p_rest = p_rest + step
+            p_now = p_now + rot @ step
+        # segment s covers [y0 + s seg, y0 + (s+1) seg); blend linearly with the segment
+        # below over the lower half of each segment
+        u = np.clip((self.pos[:, 1] - self.y0) / self.seg, 0.0, nj - 1e-9)
+        s = np.floor(u).astype(int)
+        f = u - s
+        w_lo = np.where((f < 0.5) & (s > 0), 0.5 - f, 0.0)
+        out = np.zeros_like(self.pos)
+        for k in range(nj):
+            wk = np.where(s == k, 1.0 - w_lo, 0.0) + np.where(s == k + 1, w_lo, 0.0)
+            if not wk.any():
+                continue
+            r0, r1 = origins[k]
+            moved = (self.pos - r0) @ rots[k].T + r1
+            out += wk[:, None] * moved
+        return out.astype(np.float32)
+
+
+def chain_body(seed: int, joints: int, radius: float, seg: float, y0: float,
+               subdiv: int = 3) -> ChainBody:
+    rng = Rng(seed)
+    half = 0.5 * joints * seg - radius
+    pos, tris = capsule_y(radius, max(half, 0.0), subdiv, (0.0, y0 + 0.5 * joints * seg, 0.0))
+    amp = rng.uniform(joints * 2 * 3).reshape(joints, 2, 3)
+    amp = 0.6 * amp / amp.sum(-1, keepdims=True).sum(-2, keepdims=True)  # sum |a| = 0.6 / joint
+    freq = 0.2 + 1.8 * rng.uniform(joints * 2 * 3).reshape(joints, 2, 3)
+    phase = 2.0 * np.pi * rng.uniform(joints * 2 * 3).reshape(joints, 2, 3)
+    return ChainBody(pos, tris, y0, seg, amp, freq, phase)
+
+
 # ------------------------------------------------------------------ workloads
 @dataclass
 class ClothSpec:
@@ -145,6 +240,23 @@ def differentiable_drop(seed: int = 5, n: int = 128, side: float = 1.0, height: 
     x0 = (pos + jitter * rng.normal(pos.size).reshape(-1, 3)).astype(np.float32)
     spec = ClothSpec(pos, tris, np.zeros(0, np.int32), 1e-8, 1e-4, x0, icosphere(4, radius))
     return spec
+
+
+def garment(seed: int = 6, ring: int = 160, rows: int = 125, joints: int = 24,
+            radius: float = 0.2, height: float = 0.6, gap: float = 0.01,
+            jitter: float = 1e-3, body_subdiv: int = 5) -> tuple[ClothSpec, ChainBody]:
+    """Tube garment (ring x rows vertices, radius `radius`, top ring at y = height as LRA
+    slots) around a capsule-chain body whose surface lies `gap` inside it; stretch 1e-8,
+    bend 1e-4.  The body moves by ChainBody.at(t)."""
+    pos, tris = tube(ring, rows, radius, height, 0.0)
+    rng = Rng(seed)
+    x0 = (pos + jitter * rng.normal(pos.size).reshape(-1, 3)).astype(np.float32)
+    seg = (height + 0.2) / joints
+    body = chain_body(seed + 1, joints, radius - gap, seg, -0.1, body_subdiv)
+    slots = np.arange((rows - 1) * ring, rows * ring, dtype=np.int32)
+    spec = ClothSpec(pos, tris, np.zeros(0, np.int32), 1e-8, 1e-4, x0, (body.pos, body.tris),
+                     slots)
+    return spec, body
 
 
 def batched(seed: int = 4, n_instances: int = 256, side_lo: int = 32, side_hi: int = 192,
