@@ -178,3 +178,43 @@ def test_garment_tight_contact_within_oracle_sensitivity():
     print(f"  GPU deviation {devs[0]:.3e} m; oracle self-sensitivity {sens:.3e} m", flush=True)
     assert sens > 1e-5
     assert devs[0] <= 2.0 * sens
+
+
+def test_drop_cloth_full_size_single_substep():
+    """The drop workload at full size (256 x 256 grid, 2 m, icosphere s5 r 0.5, ground
+    y = -0.5, body and self contact), started 4 mm above the sphere top at 0.5 m/s down so
+    that the substep has contacts: contact records and the 1.96 M pairs equal the oracle
+    twin's.  Body-cloth contact at the apex makes this state ill-conditioned at the
+    1e-5 level -- the FP64 oracle's own one-substep result moves by ~2e-5 m when its start
+    state is perturbed at FP32 resolution (6e-8 relative) -- so the positions are held to
+    twice that sensitivity (asserted to exceed 1e-5 here)."""
+    print("case drop cloth 256x256 full size, one substep", flush=True)
+    from xrtailor_amd import scenes
+    spec = scenes.drop_cloth(height=0.504)
+    xp = make_params(bend_model=1, enable_obstacle_contact=1, enable_self_contact=1,
+                     ground_height=-0.5)
+    v0 = np.zeros_like(spec.x0)
+    v0[:, 1] = -0.5
+    s, i, twin = _setup(spec, xp, v0)
+    s.SubstepPlain(i)
+    twin.substep()
+    g = s.Contacts(i)
+    np.testing.assert_array_equal(g["vf_face"], twin.vf_face)
+    np.testing.assert_array_equal(g["rt_face"], twin.rt_face)
+    assert g["pairs"] == len(twin.pairs) and g["obs_pairs"] == twin.n_obs_pairs
+    x_ref = twin.c.state()[0].copy()
+    err = rel_err(s.GetState(i)[0], x_ref)
+    dev = float(np.abs(s.GetState(i)[0] - x_ref).max())
+    # the oracle from a start state perturbed at FP32 resolution (CPU only)
+    x0p = spec.x0.astype(np.float64) * (1.0 + 6e-8 * np.random.default_rng(2).standard_normal(spec.x0.shape))
+    sp = scenes.ClothSpec(spec.pos, spec.tris, spec.pinned, spec.stretch_compliance,
+                          spec.bend_compliance, x0p, spec.obstacle)
+    tw = oracle_scene(sp, xp, v0=v0, seed=1)
+    tw.substep()
+    sens = float(np.abs(tw.c.state()[0] - x_ref).max())
+    print(f"  VF-DCD {(g['vf_face'] >= 0).sum()}, pairs {g['pairs']} (obstacle "
+          f"{g['obs_pairs']}), rel err {err:.3e}; deviation {dev:.3e} m, oracle "
+          f"self-sensitivity {sens:.3e} m", flush=True)
+    assert (g["vf_face"] >= 0).sum() > 0
+    assert sens > 1e-5
+    assert dev <= 2.0 * sens

@@ -224,8 +224,9 @@ def hanging_cloth(seed: int = 1, n: int = 64, side: float = 1.0, jitter: float =
     return ClothSpec(pos, tris, pinned, stretch, bend, x0)
 
 
-def drop_cloth(seed: int = 2, n: int = 256, side: float = 2.0, jitter: float = 1e-3) -> ClothSpec:
-    pos, tris = grid(n, side, 1.0)
+def drop_cloth(seed: int = 2, n: int = 256, side: float = 2.0, jitter: float = 1e-3,
+               height: float = 1.0) -> ClothSpec:
+    pos, tris = grid(n, side, height)
     x0 = (pos + jitter * Rng(seed).normal(pos.size).reshape(-1, 3)).astype(np.float32)
     obstacle = icosphere(5, 0.5)
     return ClothSpec(pos, tris, np.zeros(0, np.int32), 1e-8, 1e-4, x0, obstacle)
