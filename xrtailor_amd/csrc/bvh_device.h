@@ -186,33 +186,6 @@ __device__ __forceinline__ uint32_t CountPairsBody(const BvhView& l, const BvhVi
   return c;
 }
 
-// CountPairsBody that also keeps the first k hits in traversal order: hit j of this leaf
-// at stage[j * stride] as (lhs object, rhs object)
-__device__ __forceinline__ uint32_t CountStageBody(const BvhView& l, const BvhView& r, float th,
-                                                   int id, int2* __restrict__ stage, int stride,
-                                                   uint32_t k) {
-  const Box lb = l.box[l.nf - 1 + id];
-  const int lobj = l.nodes[l.nf - 1 + id].w;
-  int node = 0;
-  uint32_t c = 0;
-  const int max_steps = 2 * r.nf;
-  for (int step = 0; node != -1 && step < max_steps; ++step) {
-    const int4 nd = r.nodes[node];
-    if (OverlapDilated(r.box[node], lb, th)) {
-      if (nd.w >= 0) {
-        if (c < k) stage[static_cast<int64_t>(c) * stride] = make_int2(lobj, nd.w);
-        ++c;
-        node = r.escape[node];
-      } else {
-        node = nd.y;
-      }
-    } else {
-      node = r.escape[node];
-    }
-  }
-  return c;
-}
-
 // Traversal record of node i for the pair traversal: its box with lo.w = left child
 // (internal node) or -(object + 1) (leaf) and hi.w = escape, so that one 32-byte load serves
 // a visit.  Written from nodes / box / escape after the boxes change.

@@ -421,16 +421,10 @@ __device__ void EvalEEp(const RMesh& c, int2 a, const RMesh& o, int2 b, F3 x0, F
   Accumulate<2>(c, o, ids4, corr, dq, cnt);
 }
 
-template <class Ids>
-__device__ __forceinline__ void EvalEEv(const RMesh& c, int2 a, const RMesh& o, int2 b, bool self,
-                                        float h, F3 ua, F3 ub, Ids&& ids, long long* dq,
-                                        int32_t* cnt) {
-  EvalEEp(c, a, o, b, c.X(a.x), c.X(a.y), o.X(b.x), o.X(b.y), self, h, ua, ub, ids, dq, cnt);
-}
-
 __device__ __forceinline__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self,
                                        float h, F3 ua, F3 ub, long long* dq, int32_t* cnt) {
-  EvalEEv(c, c.t.edges[e0], o, o.t.edges[e1], self, h, ua, ub,
+  const int2 a = c.t.edges[e0], b = o.t.edges[e1];
+  EvalEEp(c, a, o, b, c.X(a.x), c.X(a.y), o.X(b.x), o.X(b.y), self, h, ua, ub,
           [&] { return make_int2(e0, e1); }, dq, cnt);
 }
 
@@ -689,7 +683,7 @@ __global__ __launch_bounds__(kBlock) void MultiObsGrowKernel(const InstContactDe
 }
 
 // one workgroup per instance merges its obstacle tree
-__global__ __launch_bounds__(1024) void MultiObsBottomUpKernel(const InstContactDev* ic, int n) {
+__global__ __launch_bounds__(1024) void MultiObsBottomUpKernel(const InstContactDev* ic) {
   const InstContactDev& c = ic[blockIdx.x];
   if (c.has_obs && c.obs_nf > 1)
     BottomUpTreeWG(c.obvh.nodes, c.obvh.box, c.obvh.flags, c.obs_nf, 0);
@@ -731,7 +725,7 @@ __global__ __launch_bounds__(kBlock) void MultiClothGrowKernel(const InstContact
 }
 
 // one workgroup per instance merges its cloth tree (grow-only)
-__global__ __launch_bounds__(1024) void MultiClothBottomUpKernel(const InstContactDev* ic, int n) {
+__global__ __launch_bounds__(1024) void MultiClothBottomUpKernel(const InstContactDev* ic) {
   const InstContactDev& c = ic[blockIdx.x];
   if (c.scal && c.nf > 1) BottomUpTreeWG(c.cbvh.nodes, c.cbvh.box, c.cbvh.flags, c.nf, 1);
 }
@@ -857,7 +851,7 @@ __global__ __launch_bounds__(kBlock) void MultiCountPairsKernel(const InstContac
 // offsets are these plus the block-local prefixes); the pair sum goes to scal[1] (obstacle
 // pass) or to the tree's total (cloth pass), the item sums to scal[5], scal[6] (obstacle
 // pass) or scal[7], scal[8]
-__global__ __launch_bounds__(1024) void MultiScanKernel(const InstContactDev* ic, int n, int self) {
+__global__ __launch_bounds__(1024) void MultiScanKernel(const InstContactDev* ic, int self) {
   const InstContactDev& c = ic[blockIdx.x];
   if (c.nf <= 0 || !c.scal) return;
   const int nb = (c.nf + kBlock - 1) / kBlock;
@@ -1118,7 +1112,7 @@ hipError_t MultiObstacleRefit(hipStream_t s, const InstContactDev* ic, const Mul
   MultiInterpolateKernel<<<XcdGrid(m.blocks[MK::kObsVert]), kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n, alpha);
   MultiNormalPredKernel<<<XcdGrid(m.blocks[MK::kObsVert]), kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n, tol);
   MultiObsGrowKernel<<<XcdGrid(m.blocks[MK::kObsFace]), kBlock, 0, s>>>(ic, m.off[MK::kObsFace], m.n);
-  MultiObsBottomUpKernel<<<m.n, 1024, 0, s>>>(ic, m.n);
+  MultiObsBottomUpKernel<<<m.n, 1024, 0, s>>>(ic);
   return hipGetLastError();
 }
 
@@ -1138,7 +1132,7 @@ hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& 
   const int32_t* fo = m.off[MK::kFace];
   MultiClothPredKernel<<<XcdGrid(m.blocks[MK::kVert]), kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x);
   MultiClothGrowKernel<<<XcdGrid(fb), kBlock, 0, s>>>(ic, fo, m.n, x, x0);
-  MultiClothBottomUpKernel<<<m.n, 1024, 0, s>>>(ic, m.n);
+  MultiClothBottomUpKernel<<<m.n, 1024, 0, s>>>(ic);
   MultiPackClothKernel<<<XcdGrid(fb), kBlock, 0, s>>>(ic, fo, m.n);
   if (!Empty(m, MK::kObsFace))
     MultiPackObsKernel<<<XcdGrid(m.blocks[MK::kObsFace]), kBlock, 0, s>>>(ic, m.off[MK::kObsFace], m.n);
@@ -1146,7 +1140,7 @@ hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& 
   // item lists will be used)
   for (int self = 0; self < 2; ++self) {
     MultiCountPairsKernel<<<XcdGrid(fb), kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
-    MultiScanKernel<<<m.n, 1024, 0, s>>>(ic, m.n, self);
+    MultiScanKernel<<<m.n, 1024, 0, s>>>(ic, self);
     if (self) MultiAddTotalsKernel<<<Blocks(m.n), kBlock, 0, s>>>(ic, m.n);
     MultiEmitPairsKernel<<<XcdGrid(fb), kBlock, 0, s>>>(ic, fo, m.n, thickness, self);
   }
