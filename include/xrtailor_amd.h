@@ -187,13 +187,16 @@ int xrt_substep_plain(xrt_ctx* ctx, int32_t inst);
 /* `nframes` frames of every instance, one instance after another (one cloth per launch
  * sequence) -- the plain path of the batched workload */
 int xrt_frames_plain(xrt_ctx* ctx, int32_t nframes);
-/* one substep of all instances in one persistent launch (CSR instance table) */
+/* one substep of all instances: without contacts one persistent launch (CSR instance
+ * table); with body or self contact every constraint color, LRA pass, contact phase and
+ * LBVH operation of all instances in one launch each.  Results equal xrt_substep_plain's
+ * bit for bit. */
 int xrt_substep_batched(xrt_ctx* ctx);
 /* `nframes` frames of all instances; the frame's substep launches are captured once in
  * a hipGraph and replayed */
 int xrt_frames_batched(xrt_ctx* ctx, int32_t nframes);
-/* timing helper: average device time (ms) of the batched substep kernel over `n`
- * launches, bracketed by HIP events on the context stream */
+/* timing helper: average device time (ms) of one batched substep over `n` substeps,
+ * bracketed by HIP events on the context stream */
 int xrt_time_batched_substep(xrt_ctx* ctx, int32_t n, float* avg_ms);
 /* wall time (ms, HIP events on the context stream) of `nframes` plain or batched frames */
 int xrt_time_frames(xrt_ctx* ctx, int32_t batched, int32_t nframes, float* ms);
