@@ -376,27 +376,33 @@ __device__ __forceinline__ void EvalVF(const RMesh& c, int node, const RMesh& o,
 // b (obstacle or cloth); ua, ub: their unit directions in the current state,
 // Nz(x_a.y - x_a.x).  The edge ids, ids() = (e0, e1), are needed only by the pairs that
 // pass the inside test.
-// x0, x1, y0, y1: the current positions of a.x, a.y, b.x, b.y (loaded by the caller)
-template <class Ids>
-__device__ void EvalEEp(const RMesh& c, int2 a, const RMesh& o, int2 b, F3 x0, F3 x1, F3 y0,
-                        F3 y1, bool self, float h, F3 ua, F3 ub, Ids&& ids, long long* dq,
-                        int32_t* cnt) {
-  if (self && (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y)) return;
-  // current-state tests first, then the start-of-substep side (pure; same results).  The
-  // weights are SignedEE's with early exits: an accepted pair has every weight > 0 (a NaN
-  // weight fails InEdge or leaves another weight <= 0), which needs a0, a1 and b0, b1 each
-  // of one strict sign, checked before the divisions on the same expressions
+// The inside test of EvaluateEEProximity (PBDRepulsion.cu:411-468) on the current state of
+// the edges with vertices a (cloth) and b (obstacle or cloth): x0, x1, y0, y1 are their
+// positions, ua, ub their unit directions Nz(x_a.y - x_a.x).  True, with SignedEE's weights
+// in bw, when the barycentric part passes; InEdge and the response follow in EEResponse.
+// The weights take early exits: an accepted pair has every weight > 0 (a NaN weight fails
+// InEdge or leaves another weight <= 0), which needs a0, a1 and b0, b1 each of one strict
+// sign, checked before the divisions on the same expressions.
+__device__ __forceinline__ bool EEInside(int2 a, int2 b, F3 x0, F3 x1, F3 y0, F3 y1, bool self,
+                                         F3 ua, F3 ub, float4& bw) {
+  if (self && (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y)) return false;
   const F3 ncur = C(ua, ub);
-  if (D(ncur, ncur) < 1e-6f) return;  // degenerate: weights 0
+  if (D(ncur, ncur) < 1e-6f) return false;  // degenerate: weights 0
   const float a0 = Mixed(S(y1, x1), S(y0, x1), ncur), a1 = Mixed(S(y0, x0), S(y1, x0), ncur);
-  if (!((a0 > 0 && a1 > 0) || (a0 < 0 && a1 < 0))) return;
+  if (!((a0 > 0 && a1 > 0) || (a0 < 0 && a1 < 0))) return false;
   const float b0 = Mixed(S(x0, y1), S(x1, y1), ncur), b1 = Mixed(S(x1, y0), S(x0, y0), ncur);
-  if (!((b0 > 0 && b1 > 0) || (b0 < 0 && b1 < 0))) return;
-  float bw[4] = {a0 / (a0 + a1), a1 / (a0 + a1), -b0 / (b0 + b1), -b1 / (b0 + b1)};
-  const float mn = fminf(fminf(bw[0], bw[1]), fminf(-bw[2], -bw[3]));
-  if (!(mn >= 1e-6f)) return;
-  const int2 e = ids();
-  const int e0 = e.x, e1 = e.y;
+  if (!((b0 > 0 && b1 > 0) || (b0 < 0 && b1 < 0))) return false;
+  bw = make_float4(a0 / (a0 + a1), a1 / (a0 + a1), -b0 / (b0 + b1), -b1 / (b0 + b1));
+  const float mn = fminf(fminf(bw.x, bw.y), fminf(-bw.z, -bw.w));
+  return mn >= 1e-6f;
+}
+
+// The rest of EvaluateEEProximity for a pair that passed EEInside with weights w: the
+// InEdge tests, the start-of-substep side and the constraint, accumulated into dq / cnt
+__device__ void EEResponse(const RMesh& c, int2 a, const RMesh& o, int2 b, F3 x0, F3 x1, F3 y0,
+                           F3 y1, int e0, int e1, float h, float4 w4, long long* dq,
+                           int32_t* cnt) {
+  float bw[4] = {w4.x, w4.y, w4.z, w4.w};
   if (!(InEdge(bw[1], c, e0, o, e1) && InEdge(-bw[3], o, e1, c, e0))) return;
   float w[4];
   F3 pn;
@@ -424,8 +430,10 @@ __device__ void EvalEEp(const RMesh& c, int2 a, const RMesh& o, int2 b, F3 x0, F
 __device__ __forceinline__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self,
                                        float h, F3 ua, F3 ub, long long* dq, int32_t* cnt) {
   const int2 a = c.t.edges[e0], b = o.t.edges[e1];
-  EvalEEp(c, a, o, b, c.X(a.x), c.X(a.y), o.X(b.x), o.X(b.y), self, h, ua, ub,
-          [&] { return make_int2(e0, e1); }, dq, cnt);
+  const F3 x0 = c.X(a.x), x1 = c.X(a.y), y0 = o.X(b.x), y1 = o.X(b.y);
+  float4 bw;
+  if (EEInside(a, b, x0, x1, y0, y1, self, ua, ub, bw))
+    EEResponse(c, a, o, b, x0, x1, y0, y1, e0, e1, h, bw, dq, cnt);
 }
 
 // EvaluateRep_Kernel body (PBDRepulsion.cu:470-501) for pair p: cloth face pr.x against
@@ -633,6 +641,46 @@ __global__ __launch_bounds__(kBlock) void FindPairsAtKernel(BvhView l, BvhView r
 
 __global__ void AddTotalsKernel(const int32_t* a, const int32_t* b, int32_t* out) {
   if (threadIdx.x == 0) *out = *a + *b;
+}
+
+// An EE item that passed the inside test, queued for its response: the cloth edge, the
+// other edge (bit 31: a cloth edge) and the weights
+struct EeQueued {
+  int e0, e1;
+  float4 bw;
+};
+
+// the inside test of EE item `it` (RepulsionEeItemBody up to its response); ent receives the
+// queued form when it passes
+__device__ __forceinline__ bool EeItemInside(const RMesh& c, const RMesh& o, int2 it,
+                                             EeQueued& ent) {
+  if (it.x < 0 || it.x >= c.t.ne) return false;
+  const bool self = it.y < 0;
+  const int e1 = it.y & 0x7fffffff;
+  if (e1 >= (self ? c.t.ne : o.t.ne)) return false;
+  const int2* oe = self ? c.t.edges : o.t.edges;  // pointers, not meshes, are selected
+  const float4* ox = self ? c.s.x : o.s.x;
+  const int2 a = c.t.edges[it.x], b = oe[e1];
+  const F3 x0 = L3(c.s.x[a.x]), x1 = L3(c.s.x[a.y]), y0 = L3(ox[b.x]), y1 = L3(ox[b.y]);
+  const F3 ua = Nz(S(x1, x0)), ub = Nz(S(y1, y0));
+  ent.e0 = it.x;
+  ent.e1 = it.y;
+  return EEInside(a, b, x0, x1, y0, y1, self, ua, ub, ent.bw);
+}
+
+// the response of a queued EE item (positions reloaded: the pass does not move them)
+__device__ __forceinline__ void EeItemRespond(const RMesh& c, const RMesh& o, const EeQueued& e,
+                                              float h, long long* __restrict__ dq,
+                                              int32_t* __restrict__ cnt) {
+  const int e1 = e.e1 & 0x7fffffff;
+  const int2 a = c.t.edges[e.e0];
+  if (e.e1 < 0) {
+    const int2 b = c.t.edges[e1];
+    EEResponse(c, a, c, b, c.X(a.x), c.X(a.y), c.X(b.x), c.X(b.y), e.e0, e1, h, e.bw, dq, cnt);
+  } else {
+    const int2 b = o.t.edges[e1];
+    EEResponse(c, a, o, b, c.X(a.x), c.X(a.y), o.X(b.x), o.X(b.y), e.e0, e1, h, e.bw, dq, cnt);
+  }
 }
 
 // -------------------------------------------------------- multi-instance kernels
@@ -973,9 +1021,41 @@ __global__ __launch_bounds__(kBlock) void MultiRepItemKernel(const InstContactDe
   const RMesh cm{c.cloth, MeshState{x + c.vbeg, x0 + c.vbeg, -1.0f}};
   const RMesh om{c.obs, MeshState{c.o_x, c.o_x0, 1e-3f}};  // obstacle nodes: inv mass 1e-3
   const int2* items = is_ee ? c.rep_ee : c.rep_vf;
-  for (int q = (lblk - off[k]) * kBlock + threadIdx.x; q < cnt; q += nblk * kBlock) {
-    if (is_ee) RepulsionEeItemBody(cm, om, items[q], h, c.dq, c.cnt);
-    else RepulsionVfItemBody(cm, om, items[q], h, c.dq, c.cnt);
+  if (!is_ee) {
+    for (int q = (lblk - off[k]) * kBlock + threadIdx.x; q < cnt; q += nblk * kBlock)
+      RepulsionVfItemBody(cm, om, items[q], h, c.dq, c.cnt);
+    return;
+  }
+  // EE: each wave runs the inside test on 64 items per step and queues the survivors in
+  // LDS; every 64 queued (and at the end) all lanes run responses together, so the rare
+  // response path does not idle most of a wave.  Same tests on the same values; the sums
+  // are fixed point, so the order of the responses changes nothing.
+  __shared__ EeQueued queue[kBlock / 64][128];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  EeQueued* qw = queue[w];
+  int qn = 0;  // queued entries (the same in every lane)
+  int base = (lblk - off[k]) * kBlock + w * 64;
+  const int stride = nblk * kBlock;
+  while (true) {
+    const bool have = base < cnt;  // uniform over the wave
+    if (have) {
+      const int qi = base + lane;
+      EeQueued ent;
+      const bool pass = qi < cnt && EeItemInside(cm, om, items[qi], ent);
+      const uint64_t m = __ballot(pass);
+      if (pass) qw[qn + __popcll(m & below)] = ent;
+      qn += __popcll(m);
+      base += stride;
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (qn >= 64 || (!have && qn > 0)) {
+      const int take = qn < 64 ? qn : 64;
+      if (lane < take) EeItemRespond(cm, om, qw[qn - take + lane], h, c.dq, c.cnt);
+      qn -= take;
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (!have && qn == 0) break;
   }
 }
 
