@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define XRT_ABI_VERSION 2
+#define XRT_ABI_VERSION 3
 
 enum {
   XRT_OK = 0,
@@ -76,6 +76,20 @@ typedef struct xrt_params {
    * the list off.  No counterpart in the reference, which evaluates per pair. */
   int32_t repulsion_vf_items_per_face;
   int32_t repulsion_ee_items_per_face;
+  /* swift-mode particle self-collision (SimParams::enable_self_collision,
+   * ParticleCollisionSettings, xrtailor/include/xrtailor/config/Base.hpp:59-66): a spatial
+   * hash of the predicted positions every `inter_leaved_hash` substeps of a frame, then
+   * CollideParticles at the start of every iteration.  Each instance's particle diameter is
+   * particle_diameter x the length of its first triangle's first edge (ClothObject.hpp:
+   * 36-37); hash cells are diameter x hash_cell_size wide; friction is ground_friction
+   * (SimParams::sdf_friction, shared with the plane as in the reference).  Positions at
+   * xrt_start are the "initial positions": pairs closer than the diameter there never
+   * collide (SpatialHashGPU::SetInitialPositions). */
+  int32_t enable_particle_collision;
+  int32_t max_num_neighbors;     /* ParticleCollisionSettings::max_neighbor_size (64)  */
+  int32_t inter_leaved_hash;     /* ParticleCollisionSettings::inter_leaved_hash (3)   */
+  float particle_diameter;       /* ParticleCollisionSettings::particle_diameter (1.3) */
+  float hash_cell_size;          /* ParticleCollisionSettings::hash_cell_size (1.5)    */
 } xrt_params;
 
 enum { XRT_BEND_REFERENCE = 0, XRT_BEND_CONSISTENT = 1 };
@@ -179,6 +193,21 @@ int xrt_solve_ground(xrt_ctx* ctx, int32_t inst);
 int xrt_apply_deltas(xrt_ctx* ctx, int32_t inst);
 /* Finalize (ClothSolverHelper.cuh:20-41) */
 int xrt_finalize(xrt_ctx* ctx, int32_t inst, float dt);
+/* SpatialHashGPU::Hash(nodes, n) of one instance's predicted positions x
+ * (xrtailor/include/xrtailor/physics/broad_phase/spatial_hash/SpatialHashGPU.cuh:26-28):
+ * hash, sort, cell table, neighbor lists (needs particle collision on) */
+int xrt_particle_hash(xrt_ctx* ctx, int32_t inst);
+/* CollideParticles(deltas, counts, nodes, neighbors) + ApplyDeltas x relaxation_factor
+ * (xrtailor/include/xrtailor/physics/ClothSolverHelper.cuh:20-41) */
+int xrt_collide_particles(xrt_ctx* ctx, int32_t inst);
+/* the hash state of one instance (SpatialHashGPU's public buffers); any output may be
+ * null.  neighbors: nv * max_num_neighbors, entry s of particle i at s * nv + i, a list
+ * ends at 0xffffffff unless full; sorted_hash / sorted_index: nv; cell_start / cell_end:
+ * 2 nv (cell_end is meaningful where cell_start != 0xffffffff).  info: particle diameter,
+ * cell spacing. */
+int xrt_get_particle_hash(xrt_ctx* ctx, int32_t inst, uint32_t* neighbors, uint32_t* sorted_hash,
+                          int32_t* sorted_index, uint32_t* cell_start, uint32_t* cell_end,
+                          float info[2]);
 
 /* ------------------------------------------------------------------ drivers */
 /* one substep of one instance through the plain per-color launches, in the order of

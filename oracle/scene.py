@@ -6,8 +6,10 @@ ClothSolverBase::SimulateSwift's body-cloth PBD collision
 (:277-313):
 
   frame start: obstacle LBVH from its x0 (tol 0); cloth LBVH from x0
-  predict
+  predict; particle spatial hash when frame substep % inter_leaved_hash == 0
+    (SimulateSwift :193-197)
   per iteration:
+    particle collision: CollideParticles, ApplyDeltas x relaxation factor (:199-202)
     collision pass when (substep*iters + it) % (substeps*iters/passes) == 0:
       x1 = lerp(x0, x, (cnt+1)/passes) of the obstacle, midstep normals,
       RefitMidstep(bvh_tolerance), generate VF-DCD, VF-CCD, EE-DCD, cnt++
@@ -25,6 +27,7 @@ import ctypes as C
 import numpy as np
 
 from . import oracle as orc
+from . import particle as opart
 
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int32)
@@ -148,6 +151,13 @@ class ContactScene:
         self.pass_cnt = 0
         self.dt = float(np.float32(xp.frame_time) / np.float32(xp.num_substeps))
         self.tape = None  # a list: substep() appends each operator's input state (adjoint.py)
+        # swift-mode particle self-collision: diameter from the rest mesh, initial positions
+        # = the state now (FP32), neighbor lists zero until the first hash
+        self.particles = bool(getattr(xp, "enable_particle_collision", 0))
+        if self.particles:
+            self.p_diam = opart.particle_diameter(rest_pos, tris, xp.particle_diameter)
+            self.p_orig = cloth.x0.reshape(-1, 3).astype(np.float32)
+            self.p_nb = np.zeros((xp.max_num_neighbors, self.nv), np.uint32)
 
     def _rec(self, *entry):
         if self.tape is not None:
@@ -178,9 +188,17 @@ class ContactScene:
         interval = max(1, xp.pair_update_interval)
         lib.orc_predict(C.byref(c.s), C.byref(p), dt)
         self._rec("predict")
+        if self.particles and self.frame_substep % xp.inter_leaved_hash == 0:
+            self.p_nb = opart.neighbors(c.x.reshape(-1, 3).astype(np.float32), self.p_orig,
+                                        self.p_diam, xp.hash_cell_size,
+                                        xp.max_num_neighbors)["neighbors"]
         x0s = c.x0.copy() if self.tape is not None else None
         for it in range(p.num_iterations):
             step = self.frame_substep * p.num_iterations + it
+            if self.particles:
+                pd, pc = opart.collide(c.x, c.x0, c.w, self.p_nb, self.p_diam, xp.ground_friction)
+                pd = np.ascontiguousarray(pd.reshape(-1))
+                lib.orc_apply_deltas(_d(c.x), _d(pd), _i(pc), self.nv, xp.relaxation_factor)
             if has_obs and step % freq == 0:
                 alpha = np.float32(np.float32(self.pass_cnt) + np.float32(1.0)) / np.float32(xp.num_collision_passes)
                 x1 = (np.float32(1.0) - alpha) * self.o_x0 + alpha * self.o_x

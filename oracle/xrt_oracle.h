@@ -203,6 +203,18 @@ void orc_adj_vf(const float* obs_pred, const int32_t* obs_tris, const int32_t* f
 void orc_adj_ee(const double* rec12, const double* x_pre, const int32_t* edges, int ne, int nv,
                 double relax, double* gx);
 
+/* ------------------------------ particle self-collision (swift mode, particle_oracle.c) */
+int32_t orc_hash_coords(int32_t x, int32_t y, int32_t z, int32_t table);
+/* spatial hash + neighbor lists (column-major, n * max_nb) from FP32 positions; the
+ * sorted keys / indices (n) and the cell table (2n) are optional outputs */
+void orc_particle_neighbors(const float* x, const float* orig, int n, float diameter,
+                            float cell_size, int max_nb, uint32_t* nb, uint32_t* s_hash,
+                            uint32_t* s_index, uint32_t* cell_start, uint32_t* cell_end);
+/* CollideParticles: per-particle deltas (3n) and counts (n), assigned */
+void orc_collide_particles(const double* x, const double* x0, const double* w, int n,
+                           const uint32_t* nb, int max_nb, double diameter, double mu,
+                           double* delta, int32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

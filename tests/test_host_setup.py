@@ -58,7 +58,7 @@ def test_abi_exports_every_declared_symbol():
 
 def test_abi_version_and_defaults():
     lib = _ffi.load()
-    assert lib.xrt_abi_version() == 2
+    assert lib.xrt_abi_version() == 3
     p = _ffi.XrtParams()
     lib.xrt_default_params(C.byref(p))
     assert p.num_substeps == 10 and p.num_iterations == 10
@@ -66,6 +66,11 @@ def test_abi_version_and_defaults():
     assert p.bend_model == 0
     assert p.pair_capacity_per_face == 64
     assert p.repulsion_vf_items_per_face == 64 and p.repulsion_ee_items_per_face == 128
+    # ParticleCollisionSettings defaults (xrtailor/include/xrtailor/config/Base.hpp:59-66)
+    assert p.enable_particle_collision == 0 and p.max_num_neighbors == 64
+    assert p.inter_leaved_hash == 3
+    assert p.particle_diameter == pytest.approx(1.3) and p.hash_cell_size == pytest.approx(1.5)
+    assert C.sizeof(p) == 4 * 30  # 30 four-byte fields
 
 
 @pytest.mark.parametrize("n", [2, 5, 64])

@@ -54,6 +54,11 @@ class XrtParams(C.Structure):
         ("pair_capacity_per_face", C.c_int32),
         ("repulsion_vf_items_per_face", C.c_int32),
         ("repulsion_ee_items_per_face", C.c_int32),
+        ("enable_particle_collision", C.c_int32),
+        ("max_num_neighbors", C.c_int32),
+        ("inter_leaved_hash", C.c_int32),
+        ("particle_diameter", C.c_float),
+        ("hash_cell_size", C.c_float),
     ]
 
 
@@ -101,6 +106,10 @@ SIGNATURES = {
     "xrt_solve_ground": [_P, _I],
     "xrt_apply_deltas": [_P, _I],
     "xrt_finalize": [_P, _I, _F],
+    "xrt_particle_hash": [_P, _I],
+    "xrt_collide_particles": [_P, _I],
+    "xrt_get_particle_hash": [_P, _I, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _IP,
+                              C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _FP],
     "xrt_substep_plain": [_P, _I],
     "xrt_frames_plain": [_P, _I],
     "xrt_substep_batched": [_P],

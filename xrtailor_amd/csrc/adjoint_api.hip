@@ -47,6 +47,8 @@ int AdjSetup(xrt_ctx* c, int want_slots = 1) {
   XRT_REQUIRE(c->started, XRT_ERR_STATE, "xrt_start has not been called");
   XRT_REQUIRE(!c->params.enable_self_contact, XRT_ERR_UNSUPPORTED,
               "the adjoint does not support self contact (PBD repulsion)");
+  XRT_REQUIRE(!c->params.enable_particle_collision, XRT_ERR_UNSUPPORTED,
+              "the adjoint does not support particle self-collision");
   const xrt_params& p = c->params;
   const int n_inst = static_cast<int>(c->inst.size());
   XRT_REQUIRE(n_inst > 0 && n_inst <= 65535, XRT_ERR_INVALID_ARG, "1..65535 instances");
@@ -208,8 +210,10 @@ int WideSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, AdjointData
   if ((rc = MultiContactSubstepBegin(c)) != XRT_OK) return rc;
   if (T) XRT_CHECK_HIP(LaunchCopy4(s, T, c->x0.ptr, N));
   XRT_CHECK_HIP(LaunchPredict(s, c->x0.ptr, c->v.ptr, c->x.ptr, N, k));
+  if ((rc = ParticleSubstepBegin(c, -1)) != XRT_OK) return rc;
   for (int it = 0; it < k.iterations; ++it) {
     float4* TI = T ? T + N + static_cast<size_t>(it) * A->it_stride : nullptr;
+    if ((rc = ParticleIter(c, -1)) != XRT_OK) return rc;
     bool ran = false;
     if ((rc = MultiContactIterPre(c, it, &ran)) != XRT_OK) return rc;
     if (TI && ran)
@@ -241,6 +245,7 @@ int WideSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, AdjointData
     }
   }
   XRT_CHECK_HIP(LaunchFinalize(s, c->x0.ptr, c->v.ptr, c->x.ptr, N, k));
+  if ((rc = ParticleSubstepEnd(c, -1)) != XRT_OK) return rc;
   return MultiContactSubstepEnd(c);
 }
 
@@ -388,6 +393,9 @@ int xrt_adjoint_backward(xrt_ctx* ctx) {
   AdjointData& A = *ctx->adj;
   XRT_REQUIRE(!A.has_grad, XRT_ERR_STATE,
               "the seeds were consumed by the previous reverse pass: seed again first");
+  XRT_REQUIRE(!ctx->params.enable_particle_collision && !ctx->params.enable_self_contact,
+              XRT_ERR_UNSUPPORTED,
+              "particle self-collision / self contact were enabled after xrt_adjoint_begin");
   const StepConsts k = MakeConsts(ctx->params);
   const BendMode bm = ModeOf(k);
   const size_t N = static_cast<size_t>(A.n_total);

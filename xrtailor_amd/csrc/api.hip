@@ -142,7 +142,10 @@ int SubstepPlain(xrt_ctx* c, int32_t inst) {
   float4* v = c->v.ptr + d.vbeg;
   float4* x = c->x.ptr + d.vbeg;
   XRT_CHECK_HIP(LaunchPredict(s, x0, v, x, d.nv, k));
+  int rc = ParticleSubstepBegin(c, inst);
+  if (rc != XRT_OK) return rc;
   for (int it = 0; it < k.iterations; ++it) {
+    if ((rc = ParticleIter(c, inst)) != XRT_OK) return rc;
     for (int col = 0; col < h.s_ncol; ++col) {
       const int b = h.sbeg + h.s_coloff[col];
       XRT_CHECK_HIP(LaunchStretch(s, c->sc.ptr + b, c->slam.ptr + b, x,
@@ -162,18 +165,19 @@ int SubstepPlain(xrt_ctx* c, int32_t inst) {
     if (k.enable_ground) XRT_CHECK_HIP(LaunchGround(s, x, x0, d.nv, k.ground_y, k.friction));
   }
   XRT_CHECK_HIP(LaunchFinalize(s, x0, v, x, d.nv, k));
-  return XRT_OK;
+  return ParticleSubstepEnd(c, inst);
 }
 
-// Without contacts: one persistent launch.  With body or self contact: the wide substep
-// (each color, contact phase and LBVH operation of every instance in one launch).
+// Without contacts: one persistent launch.  With body, self contact or particle
+// collision: the wide substep (each color, contact phase, hash step and LBVH operation of
+// every instance in one launch).
 int SubstepBatched(xrt_ctx* c) {
   const StepConsts k = MakeConsts(c->params);
-  if (ContactsActive(c)) return WideSubstep(c, k, ModeOf(k), nullptr, -1);
+  if (ContactsActive(c) || ParticlesOn(c)) return WideSubstep(c, k, ModeOf(k), nullptr, -1);
   XRT_CHECK_HIP(LaunchBatchedSubstep(c->stream, MakeBatchArgs(c), k,
                                      static_cast<int>(c->inst.size()), c->batched_block,
                                      c->batched_lds));
-  return XRT_OK;
+  return ParticleSubstepEnd(c, -1);
 }
 
 int BuildGraph(xrt_ctx* c) {
@@ -235,6 +239,11 @@ void xrt_default_params(xrt_params* p) {
   p->pair_capacity_per_face = 64;
   p->repulsion_vf_items_per_face = 64;
   p->repulsion_ee_items_per_face = 128;
+  p->enable_particle_collision = 0;
+  p->max_num_neighbors = 64;
+  p->inter_leaved_hash = 3;
+  p->particle_diameter = 1.3f;
+  p->hash_cell_size = 1.5f;
 }
 
 int xrt_create(xrt_ctx** out, int device) {
@@ -294,8 +303,14 @@ int xrt_set_params(xrt_ctx* ctx, const xrt_params* p) {
   XRT_REQUIRE(std::isfinite(p->gravity[0]) && std::isfinite(p->gravity[1]) &&
                   std::isfinite(p->gravity[2]) && std::isfinite(p->damping),
               XRT_ERR_INVALID_ARG, "non-finite parameter");
+  XRT_REQUIRE(p->max_num_neighbors >= 1 && p->max_num_neighbors <= 1024 && p->inter_leaved_hash >= 1,
+              XRT_ERR_INVALID_ARG, "max_num_neighbors must be in [1, 1024], inter_leaved_hash >= 1");
+  XRT_REQUIRE(p->particle_diameter > 0.0f && p->hash_cell_size > 0.0f &&
+                  std::isfinite(p->particle_diameter) && std::isfinite(p->hash_cell_size),
+              XRT_ERR_INVALID_ARG, "particle_diameter and hash_cell_size must be finite and > 0");
   ctx->params = *p;
   DestroyGraph(ctx);
+  if (ctx->started) return ParticleConfigure(ctx);
   return XRT_OK;
 }
 
@@ -613,6 +628,7 @@ int xrt_start(xrt_ctx* ctx, uint64_t seed) {
   }
   int rc = ContactStart(ctx);
   if (rc != XRT_OK) return rc;
+  if ((rc = ParticleStart(ctx)) != XRT_OK) return rc;
   ctx->started = true;
   return XRT_OK;
 }

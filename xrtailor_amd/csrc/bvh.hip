@@ -11,8 +11,6 @@ namespace xrt {
 
 namespace {
 constexpr int kBlock = 256;
-constexpr int kSortItems = 8;
-constexpr int kSortTile = kBlock * kSortItems;  // keys per radix block
 
 inline int Blocks(int n, int bs = kBlock) { return (n + bs - 1) / bs; }
 
@@ -165,23 +163,6 @@ __global__ __launch_bounds__(kBlock) void MortonKernel(const Box* __restrict__ l
 }
 
 // -------------------------------------------------------------- radix sort
-// Stable LSD radix sort of (key, value), 8-bit digits: per-tile histograms, one
-// exclusive scan over the digit-major table, then a scatter whose in-tile ranks come
-// from wave64 ballots (lanes with equal digits), so equal keys keep their input order.
-__device__ __forceinline__ void RadixHistBody(const uint32_t* __restrict__ keys, int n, int shift,
-                                              int nblocks, uint32_t* __restrict__ hist, int blk) {
-  __shared__ uint32_t cnt[256];
-  cnt[threadIdx.x] = 0;
-  __syncthreads();
-  const int base = blk * kSortTile;
-  for (int r = 0; r < kSortItems; ++r) {
-    const int i = base + r * kBlock + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
-  }
-  __syncthreads();
-  hist[threadIdx.x * nblocks + blk] = cnt[threadIdx.x];
-}
-
 __global__ __launch_bounds__(kBlock) void RadixHistKernel(const uint32_t* __restrict__ keys, int n,
                                                           int shift, int nblocks,
                                                           uint32_t* __restrict__ hist) {
@@ -192,52 +173,6 @@ __global__ __launch_bounds__(kBlock) void RadixHistKernel(const uint32_t* __rest
 __global__ __launch_bounds__(1024) void ExclusiveScanKernel(uint32_t* __restrict__ a, int n,
                                                             int32_t* __restrict__ total) {
   BlockExclusiveScan(a, n, total);
-}
-
-__device__ __forceinline__ void RadixScatterBody(
-    const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin, uint32_t* __restrict__ kout,
-    int32_t* __restrict__ vout, int n, int shift, int nblocks, const uint32_t* __restrict__ offs,
-    int blk) {
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wc[kBlock / 64][256];
-  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  run[t] = offs[t * nblocks + blk];
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int base = blk * kSortTile;
-  for (int r = 0; r < kSortItems; ++r) {
-    for (int k = 0; k < kBlock / 64; ++k) wc[k][t] = 0u;
-    __syncthreads();
-    const int i = base + r * kBlock + t;
-    const bool valid = i < n;
-    const uint32_t key = valid ? kin[i] : 0u;
-    const uint32_t d = (key >> shift) & 255u;
-    uint64_t peers = __ballot(valid);
-    for (int bit = 0; bit < 8; ++bit) {
-      const uint64_t bb = __ballot(valid && ((d >> bit) & 1u));
-      peers &= ((d >> bit) & 1u) ? bb : ~bb;
-    }
-    const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt));
-    if (valid && (peers & lt) == 0) wc[w][d] = static_cast<uint32_t>(__popcll(peers));
-    __syncthreads();
-    {  // per digit: ordered prefix over the waves of this round
-      uint32_t acc = run[t];
-      for (int k = 0; k < kBlock / 64; ++k) {
-        const uint32_t c = wc[k][t];
-        wc[k][t] = acc;
-        acc += c;
-      }
-      run[t] = acc;
-    }
-    __syncthreads();
-    if (valid) {
-      const uint32_t pos = wc[w][d] + rank;
-      if (pos < static_cast<uint32_t>(n)) {  // always true when histogram and scatter agree
-        kout[pos] = key;
-        vout[pos] = vin[i];
-      }
-    }
-    __syncthreads();
-  }
 }
 
 __global__ __launch_bounds__(kBlock) void RadixScatterKernel(
@@ -457,17 +392,8 @@ __global__ __launch_bounds__(kBlock) void VertexNormalsKernel(const float4* __re
 
 namespace {
 // ------------------------------------------------------- multi-tree build
-// A map of B logical blocks (off[n] = B) runs on a grid rounded up to a multiple of 8:
-// physical block b takes logical block (b % 8) * ceil(B / 8) + b / 8, so each XCD (the
-// hardware deals workgroups to the 8 XCDs in turn) walks a contiguous range of trees.
-__device__ __forceinline__ int XcdTreeBlock(const int32_t* __restrict__ off, int n) {
-  const int total = off[n];
-  const int l = static_cast<int>(blockIdx.x & 7u) * ((total + 7) >> 3) + static_cast<int>(blockIdx.x >> 3);
-  return l < total ? l : -1;
-}
-
 #define XRT_TREE_INDEX()                                          \
-  const int lblk = XcdTreeBlock(off, n);                          \
+  const int lblk = XcdBlock(off, n);                          \
   if (lblk < 0) return;                                           \
   const int k = off[n + 1 + lblk];                                \
   const TreeBuildDev& t = set[k];                                 \
@@ -606,9 +532,9 @@ hipError_t MultiLbvhBuild(TreeSet& set, hipStream_t s) {
   MultiLeafBoxKernel<<<(set.blocks[K::kFace] + 7) / 8 * 8, kBlock, 0, s>>>(d, set.off[K::kFace], n);
   MultiMortonKernel<<<(set.blocks[K::kFace] + 7) / 8 * 8, kBlock, 0, s>>>(d, set.off[K::kFace], n);
   for (int pass = 0; pass < 4; ++pass) {  // the result lands back in (morton, order)
-    MultiRadixHistKernel<<<(set.blocks[K::kTile] + 7) / 8 * 8, kBlock, 0, s>>>(d, set.off[K::kTile], n, pass);
+    MultiRadixHistKernel<<<XcdGrid(set.blocks[K::kTile]), kBlock, 0, s>>>(d, set.off[K::kTile], n, pass);
     MultiRadixScanKernel<<<n, 1024, 0, s>>>(d);
-    MultiRadixScatterKernel<<<(set.blocks[K::kTile] + 7) / 8 * 8, kBlock, 0, s>>>(d, set.off[K::kTile], n, pass);
+    MultiRadixScatterKernel<<<XcdGrid(set.blocks[K::kTile]), kBlock, 0, s>>>(d, set.off[K::kTile], n, pass);
   }
   MultiLeafNodesKernel<<<(set.blocks[K::kFace] + 7) / 8 * 8, kBlock, 0, s>>>(d, set.off[K::kFace], n);
   MultiKarrasKernel<<<(set.blocks[K::kFace] + 7) / 8 * 8, kBlock, 0, s>>>(d, set.off[K::kFace], n);

@@ -6,6 +6,7 @@
 #include <cstdint>
 
 #include "bvh.h"
+#include "device.h"
 
 namespace xrt {
 
@@ -271,6 +272,75 @@ __device__ __forceinline__ float4 VertexNormalBody(const float4* __restrict__ x,
   }
   const float l = sqrtf(sx * sx + sy * sy + sz * sz);
   return make_float4(sx / l, sy / l, sz / l, 0.0f);
+}
+
+// ------------------------------------------------------------ radix sort bodies
+// (256-thread workgroups; a tile of kSortTile keys per workgroup)
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortBlock * kSortItems;
+
+// Stable LSD radix sort of (key, value), 8-bit digits: per-tile histograms, one
+// exclusive scan over the digit-major table, then a scatter whose in-tile ranks come
+// from wave64 ballots (lanes with equal digits), so equal keys keep their input order.
+__device__ __forceinline__ void RadixHistBody(const uint32_t* __restrict__ keys, int n, int shift,
+                                              int nblocks, uint32_t* __restrict__ hist, int blk) {
+  __shared__ uint32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = blk * kSortTile;
+  for (int r = 0; r < kSortItems; ++r) {
+    const int i = base + r * kSortBlock + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nblocks + blk] = cnt[threadIdx.x];
+}
+
+__device__ __forceinline__ void RadixScatterBody(
+    const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+    int32_t* __restrict__ vout, int n, int shift, int nblocks, const uint32_t* __restrict__ offs,
+    int blk) {
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wc[kSortBlock / 64][256];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  run[t] = offs[t * nblocks + blk];
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int base = blk * kSortTile;
+  for (int r = 0; r < kSortItems; ++r) {
+    for (int k = 0; k < kSortBlock / 64; ++k) wc[k][t] = 0u;
+    __syncthreads();
+    const int i = base + r * kSortBlock + t;
+    const bool valid = i < n;
+    const uint32_t key = valid ? kin[i] : 0u;
+    const uint32_t d = (key >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+    for (int bit = 0; bit < 8; ++bit) {
+      const uint64_t bb = __ballot(valid && ((d >> bit) & 1u));
+      peers &= ((d >> bit) & 1u) ? bb : ~bb;
+    }
+    const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt));
+    if (valid && (peers & lt) == 0) wc[w][d] = static_cast<uint32_t>(__popcll(peers));
+    __syncthreads();
+    {  // per digit: ordered prefix over the waves of this round
+      uint32_t acc = run[t];
+      for (int k = 0; k < kSortBlock / 64; ++k) {
+        const uint32_t c = wc[k][t];
+        wc[k][t] = acc;
+        acc += c;
+      }
+      run[t] = acc;
+    }
+    __syncthreads();
+    if (valid) {
+      const uint32_t pos = wc[w][d] + rank;
+      if (pos < static_cast<uint32_t>(n)) {  // always true when histogram and scatter agree
+        kout[pos] = key;
+        vout[pos] = vin[i];
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // exclusive scan of n uint32 in place by one 1024-thread workgroup; *total (if given)

@@ -12,7 +12,6 @@ using namespace cm;
 constexpr int kBlock = 256;
 constexpr float kFix = 4294967296.0f;     // 2^32: 32.32 fixed point
 inline int Blocks(int n) { return (n + kBlock - 1) / kBlock; }
-inline int XcdGrid(int blocks) { return (blocks + 7) / 8 * 8; }  // see XcdBlock
 
 // Ericson ClosestPtPointTriangle (BasicPrimitiveTests.cu:6-100)
 __device__ float NearestPoint(F3 p, F3 a, F3 b, F3 c, F3& q) {
@@ -684,20 +683,6 @@ __device__ __forceinline__ void EeItemRespond(const RMesh& c, const RMesh& o, co
 }
 
 // -------------------------------------------------------- multi-instance kernels
-// A map of B logical blocks (off[n] = B) is launched on XcdGrid(B) blocks.  The hardware
-// deals consecutive workgroups to the 8 XCDs in turn, so physical block b runs logical
-// block (b % 8) * ceil(B / 8) + b / 8: each XCD walks a contiguous range of logical blocks
-// -- a few instances at a time, whose data then stay in that XCD's L2.  -1: idle block.
-__device__ __forceinline__ int XcdBlock(const int32_t* __restrict__ off, int n) {
-  const int total = off[n];
-  const int l = static_cast<int>(blockIdx.x & 7u) * ((total + 7) >> 3) + static_cast<int>(blockIdx.x >> 3);
-  return l < total ? l : -1;
-}
-
-// the instance of logical block b: the map's block table follows its n + 1 prefix sums
-__device__ __forceinline__ int InstOfBlock(const int32_t* __restrict__ off, int n, int b) {
-  return off[n + 1 + b];
-}
 
 #define XRT_MULTI_INDEX(kind, count_expr)                                  \
   const int lblk = XcdBlock(off, n);                                       \

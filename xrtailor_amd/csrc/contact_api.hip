@@ -1,10 +1,11 @@
 // Contact set-up and the contact-aware plain substep.
 //
-// Substep order (a union of ClothSolverBase::SimulateSwift's body-cloth collision,
-// xrtailor/src/pipeline/base/ClothSolverBase.cpp:190-240, and SimulateQuality's PBD
-// repulsion, :300-313):
-//   predict
+// Substep order (a union of ClothSolverBase::SimulateSwift's body-cloth collision and
+// particle self-collision, xrtailor/src/pipeline/base/ClothSolverBase.cpp:190-240, and
+// SimulateQuality's PBD repulsion, :300-313):
+//   predict; spatial hash when the frame's substep % inter_leaved_hash == 0
 //   per iteration:
+//     particle collision (CollideParticles + ApplyDeltas x relaxation factor)
 //     collision pass (step % frequency == 0): obstacle x1 = lerp(x0, x, (cnt+1)/passes),
 //       midstep normals, RefitMidstep(bvh_tolerance); generate VF-DCD, VF-CCD, EE-DCD
 //     pair refresh (iteration % pair_update_interval == 0): cloth LBVH Update(ccd),
@@ -487,7 +488,9 @@ int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendM
   int rc = ContactSubstepBegin(ctx, inst);
   if (rc != XRT_OK) return rc;
   XRT_CHECK_HIP(LaunchPredict(s, x0, v, x, d.nv, k));
+  if ((rc = ParticleSubstepBegin(ctx, inst)) != XRT_OK) return rc;
   for (int it = 0; it < k.iterations; ++it) {
+    if ((rc = ParticleIter(ctx, inst)) != XRT_OK) return rc;
     if ((rc = ContactIterPre(ctx, inst, it, nullptr)) != XRT_OK) return rc;
     for (int col = 0; col < h.s_ncol; ++col) {
       const int b = h.sbeg + h.s_coloff[col];
@@ -509,6 +512,7 @@ int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendM
     if ((rc = ContactEeSolve(ctx, inst)) != XRT_OK) return rc;
   }
   XRT_CHECK_HIP(LaunchFinalize(s, x0, v, x, d.nv, k));
+  if ((rc = ParticleSubstepEnd(ctx, inst)) != XRT_OK) return rc;
   return ContactSubstepEnd(ctx, inst);
 }
 

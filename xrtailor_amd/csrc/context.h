@@ -12,6 +12,7 @@
 #include "common.h"
 #include "contact.h"
 #include "device.h"
+#include "particle.h"
 #include "topology.h"
 
 namespace xrt {
@@ -90,6 +91,25 @@ struct ContactData {
   DevBuf<uint32_t> bsum;  // per-block sums of the batched pair refresh
   int frame_substep = 0, pass_cnt = 0;
   bool obstacle_moving = false;  // xrt_update_obstacle since the last frame end
+};
+
+// Particle self-collision state of every instance (particle.h has the HBM layout).
+struct ParticleData {
+  struct MapRef {
+    size_t off = 0;
+    int n = 0, blocks = 0;
+  };
+  bool sized = false;      // buffers allocated for max_nb
+  int max_nb = 0, passes = 0;
+  std::vector<ParticleInstDev> inst;
+  std::vector<int> substep;  // substep of the current frame, per instance
+  DevBuf<ParticleInstDev> dev;
+  DevBuf<uint32_t> keys, keys_tmp, cell_start, cell_end, nb, hist;
+  DevBuf<int32_t> vals, vals_tmp;
+  DevBuf<float4> orig, delta;
+  DevBuf<int32_t> maps;  // block maps: every instance (vertex, tile), then each instance's
+  MapRef all_v, all_t;
+  std::vector<MapRef> one_v, one_t;
 };
 
 // Sizes of the wide launches (one color of every instance per launch).
@@ -173,6 +193,8 @@ struct xrt_ctx {
   xrt::MultiMaps maps;
   // wide (all instances per launch) launch sizes
   xrt::WideConfig wide;
+  // swift-mode particle self-collision
+  xrt::ParticleData particles;
   // differentiable rollout (allocated by xrt_adjoint_begin)
   std::unique_ptr<xrt::AdjointData> adj;
   // graph of one batched frame
@@ -207,6 +229,15 @@ int MultiContactIterPre(xrt_ctx* ctx, int it, bool* pass_ran);
 int MultiContactVfSolve(xrt_ctx* ctx);
 int MultiContactEeSolve(xrt_ctx* ctx);
 int MultiContactSubstepEnd(xrt_ctx* ctx);
+// particle_api.hip: swift-mode particle self-collision.  ParticleStart snapshots the
+// initial positions (xrt_start); ParticleConfigure sizes the buffers for the current
+// parameters.  The phases take an instance, or -1 for every instance at once.
+bool ParticlesOn(const xrt_ctx* ctx);
+int ParticleStart(xrt_ctx* ctx);
+int ParticleConfigure(xrt_ctx* ctx);
+int ParticleSubstepBegin(xrt_ctx* ctx, int inst);  // after predict: hash when due
+int ParticleIter(xrt_ctx* ctx, int inst);          // iteration start: collide + apply
+int ParticleSubstepEnd(xrt_ctx* ctx, int inst);    // after finalize
 // adjoint_api.hip: one substep of every instance as wide launches with the contact phases
 // above; with a tape owner and sub >= 0 the operators' inputs go to tape slot `sub`
 int WideSubstep(xrt_ctx* ctx, const StepConsts& k, const BendMode& bm, AdjointData* tape, int sub);

@@ -17,6 +17,25 @@ namespace xrt {
 
 constexpr float kEps = 1e-6f;  // EPSILON (xrtailor/include/xrtailor/core/Scalar.hpp:37)
 
+// Multi-instance launches map B logical blocks (off[n] = B; off holds the n + 1 prefix
+// sums of the instances' block counts, then the instance of every block) onto a grid of
+// XcdGrid(B) blocks.  The hardware deals consecutive workgroups to the 8 XCDs in turn, so
+// physical block b runs logical block (b % 8) * ceil(B / 8) + b / 8: each XCD walks a
+// contiguous range of logical blocks -- a few instances at a time, whose data then stay
+// in that XCD's L2.  -1: idle block.
+inline int XcdGrid(int blocks) { return (blocks + 7) / 8 * 8; }
+
+__device__ __forceinline__ int XcdBlock(const int32_t* __restrict__ off, int n) {
+  const int total = off[n];
+  const int l = static_cast<int>(blockIdx.x & 7u) * ((total + 7) >> 3) + static_cast<int>(blockIdx.x >> 3);
+  return l < total ? l : -1;
+}
+
+// the instance of logical block b of such a map
+__device__ __forceinline__ int InstOfBlock(const int32_t* __restrict__ off, int n, int b) {
+  return off[n + 1 + b];
+}
+
 struct StretchC {  // BasicConstraint::StretchConstraint without lambda (BasicConstraint.cuh:8-14)
   int32_t i0, i1;
   float rest, compliance;

@@ -98,6 +98,37 @@ class BendSolver:
         check(self._s.lib.xrt_bend_solve(self._s.ctx, self._i, color, dt, iteration))
 
 
+class SpatialHash:
+    """SpatialHashGPU of one instance (xrtailor/include/xrtailor/physics/broad_phase/
+    spatial_hash/SpatialHashGPU.cuh:19-35): Hash() over the predicted positions and its
+    public buffers."""
+
+    def __init__(self, solver: "ClothSolver", inst: int):
+        self._s, self._i = solver, inst
+
+    def Hash(self) -> None:
+        check(self._s.lib.xrt_particle_hash(self._s.ctx, self._i))
+
+    def buffers(self) -> dict:
+        """neighbors (max_num_neighbors, nv): row k holds every particle's k-th neighbor;
+        particle_hash / particle_index: the sorted pairs; cell_start / cell_end (2 nv)."""
+        s, i = self._s, self._i
+        nv, m = s.counts(i).nv, s.params.max_num_neighbors
+        u32 = C.POINTER(C.c_uint32)
+        nb = np.zeros(nv * m, np.uint32)
+        ph = np.zeros(nv, np.uint32)
+        pi = np.zeros(nv, np.int32)
+        cs = np.zeros(2 * nv, np.uint32)
+        ce = np.zeros(2 * nv, np.uint32)
+        info = np.zeros(2, np.float32)
+        check(s.lib.xrt_get_particle_hash(s.ctx, i, nb.ctypes.data_as(u32), ph.ctypes.data_as(u32),
+                                          iptr(pi), cs.ctypes.data_as(u32), ce.ctypes.data_as(u32),
+                                          fptr(info)))
+        return {"neighbors": nb.reshape(m, nv), "particle_hash": ph, "particle_index": pi,
+                "cell_start": cs, "cell_end": ce, "diameter": float(info[0]),
+                "cell_spacing": float(info[1])}
+
+
 class ClothSolver:
     """Owns one xrt_ctx (device state of one or more cloth instances)."""
 
@@ -129,8 +160,8 @@ class ClothSolver:
 
     # ----------------------------------------------------------------- setup
     def set_params(self, params: XrtParams) -> None:
-        self.params = params
         check(self.lib.xrt_set_params(self.ctx, C.byref(params)))
+        self.params = params
 
     def AddCloth(self, pos: np.ndarray, tris: np.ndarray, stretch_compliance: float,
                  bend_compliance: float) -> int:
@@ -267,6 +298,13 @@ class ClothSolver:
 
     def Finalize(self, inst: int, dt: float) -> None:
         check(self.lib.xrt_finalize(self.ctx, inst, dt))
+
+    def SpatialHash(self, inst: int = 0) -> SpatialHash:
+        return SpatialHash(self, inst)
+
+    def CollideParticles(self, inst: int) -> None:
+        """CollideParticles + ApplyDeltas x relaxation_factor (ClothSolverHelper.cu:171-219)."""
+        check(self.lib.xrt_collide_particles(self.ctx, inst))
 
     # --------------------------------------------------------------- drivers
     def SubstepPlain(self, inst: int) -> None:
