@@ -21,7 +21,11 @@ cpu_baseline the serial FP64 oracle on one core, one substep per instance over a
              sample; the same leg reports the single-substep relative error of the GPU's
              batched substep against it (parity at the measured configuration)
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload batched|hanging]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload batched|swift|hanging]
+                    (swift: the same 256 cloths with body contact and the particle
+                     self-collision of the reference's swift mode instead of PBD repulsion;
+                     roofline kernel ParticleCollideKernel, algorithmic bytes per launch
+                     48 N + 4 (neighbor entries + N))
                     [--no-graph]   (un-captured substeps, for profiler runs)
     python bench.py --copy-calibration   (5 device copies of 256 MiB: the known-size kernel
                                           that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)
@@ -104,7 +108,10 @@ def cpu_baseline_and_parity(specs, params, seed, contacts, budget_s=12.0):
         if t_used >= budget_s:
             break
     solver.close()
-    what = "with body and self contact" if contacts else "without contacts"
+    what = "without contacts"
+    if contacts:
+        what = ("with body contact and particle self-collision" if params.enable_particle_collision
+                else "with body and self contact")
     base = {"value": done_vs / t_used, "unit": "vertex-substeps/s", "cores": 1, "kind": "port",
             "sample": f"one substep ({what}) of the first {n_inst} of {len(specs)} instances "
                       f"({done_vs} vertex-substeps, {t_used:.1f} s), serial FP64 oracle; "
@@ -163,7 +170,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="batched", choices=["batched", "hanging"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "swift", "hanging"])
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--plain-frames", type=int, default=1)
@@ -196,16 +203,21 @@ def main() -> int:
     from xrtailor_amd import scenes
     from xrtailor_amd.solver import build_solver, make_params
 
-    if a.workload == "batched":
+    if a.workload in ("batched", "swift"):
         specs = scenes.batched(seed=a.seed, n_instances=a.instances)
         wname = f"batched {a.instances} cloths, side round(32*6^u) in [32,192], seed {a.seed}"
     else:
         specs = [scenes.hanging_cloth(seed=a.seed)]
         wname = "hanging cloth 64x64, 1 m, two corners pinned"
-    contacts = a.workload == "batched" and not a.no_contacts
+    swift = a.workload == "swift"
+    contacts = a.workload in ("batched", "swift") and not a.no_contacts
     params = make_params(bend_model=1, ground_height=-1.0, enable_obstacle_contact=int(contacts),
-                         enable_self_contact=int(contacts))
-    if contacts:
+                         enable_self_contact=int(contacts and not swift),
+                         enable_particle_collision=int(swift))
+    if swift:
+        wname += (", sphere / capsule obstacle" if contacts else "") + \
+            ", particle self-collision (swift mode)"
+    elif contacts:
         wname += ", sphere / capsule obstacle and self contact"
     print(f"[bench] building {len(specs)} instances", file=sys.stderr, flush=True)
     solver, insts = build_solver(specs, params, seed=a.seed, obstacles=contacts)
@@ -232,8 +244,16 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     # dominant kernel: per-launch device time from HIP events around each launch
-    rep_counts = None
-    if contacts:
+    rep_counts, particle = None, None
+    if swift:
+        kernel = "ParticleCollideKernel"
+        kern_ms, hash_ms, pc = solver.time_particles(2 * params.num_substeps)
+        # x, x0 read, delta written (48 B per particle), list entries and sentinels (4 B)
+        bytes_kern = 48 * n_verts + 4 * (pc["entries"] + n_verts)
+        particle = {"collide_ms": kern_ms, "hash_ms": hash_ms, **pc,
+                    "max_num_neighbors": params.max_num_neighbors,
+                    "inter_leaved_hash": params.inter_leaved_hash}
+    elif contacts:
         kernel = "MultiRepItemKernel<1>"
         kern_ms, rep_counts = solver.time_repulsion(2)
         n_edges = sum(solver.counts(i).n_edges for i in insts)
@@ -286,7 +306,8 @@ def main() -> int:
         "config": {"workload": wname, "instances": len(specs), "vertices": n_verts,
                    "substeps": params.num_substeps, "iterations": params.num_iterations,
                    "bend_model": "consistent", "obstacle_contact": contacts,
-                   "self_contact": contacts, "step": "one frame of all instances",
+                   "self_contact": contacts and not swift, "particle_collision": swift,
+                   "step": "one frame of all instances",
                    "graph": not a.no_graph},
         "plain_value": plain,
         "batched_over_plain": (value / plain) if plain else None,
@@ -295,7 +316,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel, "kernel_ms": kern_ms, "bytes_per_launch": bytes_kern,
-                     "repulsion_counts": rep_counts},
+                     "repulsion_counts": rep_counts, "particle": particle},
         "cpu_baseline": cpu,
         "adjoint_forward_ratio": adj["substep_ratio"] if adj else None,
         "adjoint": adj,

@@ -272,6 +272,11 @@ int xrt_time_adjoint(xrt_ctx* ctx, int32_t n, float ms[3]);
  * state advances by n substeps.  No counterpart in the reference (measurement hook; the EE
  * tests are EvaluateEEProximity, xrtailor/src/physics/repulsion/PBDRepulsion.cu:411-468). */
 int xrt_time_repulsion(xrt_ctx* ctx, int32_t n, float* avg_ms, int64_t counts[4]);
+/* timing helper (particle collision on): `n` batched substeps with HIP events around each
+ * particle collision kernel and each hash step; ms[0] / ms[1] = their average device time,
+ * counts[0] = neighbor-list entries after the last hash, counts[1] = particles.  The state
+ * advances by n substeps.  No counterpart in the reference (measurement hook). */
+int xrt_time_particles(xrt_ctx* ctx, int32_t n, float ms[2], int64_t counts[2]);
 
 /* ------------------------------------------------------------------- LBVH */
 /* Standalone linear BVH over triangle faces (Karras 2012), the reference's BVH class

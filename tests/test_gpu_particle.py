@@ -187,3 +187,21 @@ def test_particle_params_and_errors():
     with pytest.raises(XrtError):
         s.AdjointBegin(1, 1)  # the adjoint has no particle collision
     s.close()
+
+
+def test_time_particles_counts():
+    """The timing hook: device times of the collision kernel and the hash, the list-entry
+    count equal to the lists' lengths read back."""
+    print("case particle timing hook counts", flush=True)
+    cases = [particle_layers(n=12, seed=1), particle_layers(n=16, seed=2)]
+    xp = _params(num_substeps=3, num_iterations=2)
+    s, insts = build([c[0] for c in cases], xp, seed=1)
+    for k, (_, st, v) in zip(insts, cases):
+        s.SetState(k, st, v)
+    col_ms, hash_ms, c = s.time_particles(3)
+    print(f"  collide {col_ms * 1e3:.1f} us, hash {hash_ms * 1e3:.1f} us, {c}", flush=True)
+    assert col_ms > 0 and hash_ms > 0
+    assert c["particles"] == sum(len(cc[1]) for cc in cases)
+    total = sum(int(_listed(s.SpatialHash(k).buffers()["neighbors"]).sum()) for k in insts)
+    assert c["entries"] == total > 0
+    s.close()

@@ -124,6 +124,7 @@ SIGNATURES = {
     "xrt_adjoint_substep": [_P],
     "xrt_time_adjoint": [_P, _I, _FP],
     "xrt_time_repulsion": [_P, _I, _FP, C.POINTER(C.c_int64)],
+    "xrt_time_particles": [_P, _I, _FP, C.POINTER(C.c_int64)],
     "xrt_bvh_create": [C.POINTER(_P), C.c_int],
     "xrt_bvh_destroy": [_P],
     "xrt_bvh_build": [_P, _FP, _FP, _FP, _I, _IP, _I, _F, _I, _F],

@@ -110,6 +110,10 @@ struct ParticleData {
   DevBuf<int32_t> maps;  // block maps: every instance (vertex, tile), then each instance's
   MapRef all_v, all_t;
   std::vector<MapRef> one_v, one_t;
+  DevBuf<unsigned long long> count;  // list-entry counter of xrt_time_particles
+  // xrt_time_particles: event pairs around the collision kernels and the hash steps
+  std::vector<hipEvent_t> collide_events, hash_events;
+  size_t collide_next = 0, hash_next = 0;
 };
 
 // Sizes of the wide launches (one color of every instance per launch).

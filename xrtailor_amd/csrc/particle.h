@@ -48,8 +48,13 @@ struct ParticleMap {
 // count), cell table, neighbor lists.
 hipError_t LaunchParticleHash(hipStream_t s, const ParticleBufs& b, const ParticleMap& vmap,
                               const ParticleMap& tmap, int passes, const float4* x);
-// CollideParticles: Jacobi deltas from the neighbor lists, then ApplyDeltas x relax
+// CollideParticles: Jacobi deltas from the neighbor lists, then ApplyDeltas x relax;
+// events (optional) are recorded around the collision kernel alone
 hipError_t LaunchCollideParticles(hipStream_t s, const ParticleBufs& b, const ParticleMap& vmap,
-                                  float4* x, const float4* x0, float mu, float relax);
+                                  float4* x, const float4* x0, float mu, float relax,
+                                  hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+// adds the number of list entries (up to each sentinel) of the map's instances to *total
+hipError_t LaunchCountNeighbors(hipStream_t s, const ParticleBufs& b, const ParticleMap& vmap,
+                                unsigned long long* total);
 
 }  // namespace xrt

@@ -219,6 +219,24 @@ __global__ __launch_bounds__(kBlock) void ParticleApplyKernel(ParticleBufs b, co
     x[c.vbeg + i] = p;
   }
 }
+// list lengths (entries before the sentinel), one atomic per wave
+__global__ __launch_bounds__(kBlock) void ParticleCountKernel(ParticleBufs b, const int32_t* off,
+                                                              int n, unsigned long long* total) {
+  const int lblk = XcdBlock(off, n);
+  if (lblk < 0) return;
+  const int ent = InstOfBlock(off, n, lblk);
+  const ParticleInstDev& c = b.inst[InstOfEntry(off, n, ent)];
+  const int i = (lblk - off[ent]) * kBlock + threadIdx.x;
+  unsigned long long len = 0;
+  if (i < c.nv) {
+    const uint32_t* list = b.nb + c.nbeg;
+    while (len < static_cast<unsigned long long>(c.max_nb) &&
+           list[static_cast<int64_t>(len) * c.nv + i] < static_cast<uint32_t>(c.nv))
+      ++len;
+  }
+  for (int o = 32; o > 0; o >>= 1) len += __shfl_down(len, o);
+  if ((threadIdx.x & 63) == 0 && len) atomicAdd(total, len);
+}
 }  // namespace
 
 hipError_t LaunchParticleHash(hipStream_t s, const ParticleBufs& b, const ParticleMap& vmap,
@@ -236,10 +254,26 @@ hipError_t LaunchParticleHash(hipStream_t s, const ParticleBufs& b, const Partic
 }
 
 hipError_t LaunchCollideParticles(hipStream_t s, const ParticleBufs& b, const ParticleMap& vmap,
-                                  float4* x, const float4* x0, float mu, float relax) {
+                                  float4* x, const float4* x0, float mu, float relax,
+                                  hipEvent_t ev_begin, hipEvent_t ev_end) {
   if (vmap.blocks <= 0) return hipSuccess;
+  if (ev_begin) {
+    const hipError_t e = hipEventRecord(ev_begin, s);
+    if (e != hipSuccess) return e;
+  }
   ParticleCollideKernel<<<XcdGrid(vmap.blocks), kBlock, 0, s>>>(b, vmap.off, vmap.n, x, x0, mu);
+  if (ev_end) {
+    const hipError_t e = hipEventRecord(ev_end, s);
+    if (e != hipSuccess) return e;
+  }
   ParticleApplyKernel<<<XcdGrid(vmap.blocks), kBlock, 0, s>>>(b, vmap.off, vmap.n, x, relax);
+  return hipGetLastError();
+}
+
+hipError_t LaunchCountNeighbors(hipStream_t s, const ParticleBufs& b, const ParticleMap& vmap,
+                                unsigned long long* total) {
+  if (vmap.blocks <= 0) return hipSuccess;
+  ParticleCountKernel<<<XcdGrid(vmap.blocks), kBlock, 0, s>>>(b, vmap.off, vmap.n, total);
   return hipGetLastError();
 }
 

@@ -78,19 +78,48 @@ int CheckParticles(xrt_ctx* c, int32_t inst) {
   return XRT_OK;
 }
 
+// the next free event pair of a timing run (nullptrs when none is running)
+void NextPair(std::vector<hipEvent_t>& ev, size_t& next, hipEvent_t& a, hipEvent_t& b) {
+  a = b = nullptr;
+  if (next + 1 < ev.size()) {
+    a = ev[next];
+    b = ev[next + 1];
+    next += 2;
+  }
+}
+
 int Hash(xrt_ctx* c, int inst) {
   ParticleData& P = c->particles;
   const ParticleMap vm = Map(P, inst < 0 ? P.all_v : P.one_v[inst]);
   const ParticleMap tm = Map(P, inst < 0 ? P.all_t : P.one_t[inst]);
+  hipEvent_t e0, e1;
+  NextPair(P.hash_events, P.hash_next, e0, e1);
+  if (e0) XRT_CHECK_HIP(hipEventRecord(e0, c->stream));
   XRT_CHECK_HIP(LaunchParticleHash(c->stream, Bufs(c), vm, tm, P.passes, c->x.ptr));
+  if (e1) XRT_CHECK_HIP(hipEventRecord(e1, c->stream));
   return XRT_OK;
 }
 
 int Collide(xrt_ctx* c, int inst) {
   ParticleData& P = c->particles;
   const ParticleMap vm = Map(P, inst < 0 ? P.all_v : P.one_v[inst]);
+  hipEvent_t e0, e1;
+  NextPair(P.collide_events, P.collide_next, e0, e1);
   XRT_CHECK_HIP(LaunchCollideParticles(c->stream, Bufs(c), vm, c->x.ptr, c->x0.ptr,
-                                       c->params.ground_friction, c->params.relaxation_factor));
+                                       c->params.ground_friction, c->params.relaxation_factor,
+                                       e0, e1));
+  return XRT_OK;
+}
+
+// average elapsed ms over the recorded pairs [0, used)
+int AverageMs(const std::vector<hipEvent_t>& ev, size_t used, float* avg) {
+  float total = 0.0f;
+  for (size_t q = 0; q + 1 < used; q += 2) {
+    float ms = 0.0f;
+    XRT_CHECK_HIP(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+    total += ms;
+  }
+  *avg = used >= 2 ? total / static_cast<float>(used / 2) : 0.0f;
   return XRT_OK;
 }
 }  // namespace
@@ -228,6 +257,48 @@ int xrt_collide_particles(xrt_ctx* ctx, int32_t inst) {
   int rc = CheckParticles(ctx, inst);
   if (rc != XRT_OK) return rc;
   return Collide(ctx, inst);
+}
+
+int xrt_time_particles(xrt_ctx* ctx, int32_t n, float ms[2], int64_t counts[2]) {
+  XRT_REQUIRE(ctx && ctx->started, XRT_ERR_STATE, "xrt_start has not been called");
+  XRT_REQUIRE(n > 0 && ms && counts, XRT_ERR_INVALID_ARG, "n must be > 0, outputs non-null");
+  XRT_REQUIRE(ParticlesOn(ctx) && ctx->particles.sized, XRT_ERR_STATE,
+              "particle collision is off (enable_particle_collision)");
+  ParticleData& P = ctx->particles;
+  XRT_REQUIRE(P.collide_events.empty() && P.hash_events.empty(), XRT_ERR_STATE,
+              "particle timing already running");
+  const size_t iters = static_cast<size_t>(std::max(1, ctx->params.num_iterations));
+  P.collide_events.assign(2 * static_cast<size_t>(n) * iters, nullptr);
+  P.hash_events.assign(2 * static_cast<size_t>(n), nullptr);
+  P.collide_next = P.hash_next = 0;
+  int rc = XRT_OK;
+  for (auto* v : {&P.collide_events, &P.hash_events})
+    for (auto& e : *v)
+      if (hipEventCreate(&e) != hipSuccess) rc = XRT_ERR_HIP;
+  for (int sub = 0; sub < n && rc == XRT_OK; ++sub) rc = xrt_substep_batched(ctx);
+  if (rc == XRT_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = XRT_ERR_HIP;
+  if (rc == XRT_OK) rc = AverageMs(P.collide_events, P.collide_next, &ms[0]);
+  if (rc == XRT_OK) rc = AverageMs(P.hash_events, P.hash_next, &ms[1]);
+  const bool timed = P.collide_next >= 2;
+  for (auto* v : {&P.collide_events, &P.hash_events}) {
+    for (auto& e : *v)
+      if (e) (void)hipEventDestroy(e);
+    v->clear();
+  }
+  P.collide_next = P.hash_next = 0;
+  if (rc != XRT_OK) return rc;
+  XRT_REQUIRE(timed, XRT_ERR_STATE, "no collision launch was timed");
+  // list entries of the last hash over every instance, and the particle count
+  if (!P.count.ptr) XRT_CHECK_HIP(P.count.Alloc(1));
+  XRT_CHECK_HIP(hipMemsetAsync(P.count.ptr, 0, sizeof(unsigned long long), ctx->stream));
+  XRT_CHECK_HIP(LaunchCountNeighbors(ctx->stream, Bufs(ctx), Map(P, P.all_v), P.count.ptr));
+  unsigned long long entries = 0;
+  XRT_CHECK_HIP(hipMemcpyAsync(&entries, P.count.ptr, sizeof(entries), hipMemcpyDeviceToHost,
+                               ctx->stream));
+  XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  counts[0] = static_cast<int64_t>(entries);
+  counts[1] = ctx->wide.n_total;
+  return XRT_OK;
 }
 
 int xrt_get_particle_hash(xrt_ctx* ctx, int32_t inst, uint32_t* neighbors, uint32_t* sorted_hash,

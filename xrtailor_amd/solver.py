@@ -378,3 +378,12 @@ class ClothSolver:
         ms = (C.c_float * 3)()
         check(self.lib.xrt_time_adjoint(self.ctx, n, ms))
         return float(ms[0]), float(ms[1]), float(ms[2])
+
+    def time_particles(self, n: int) -> tuple[float, float, dict]:
+        """(collision kernel ms, hash ms, counts) averaged over n batched substeps (the
+        state advances); counts: neighbor-list entries after the last hash, particles."""
+        ms = np.zeros(2, np.float32)
+        cnt = np.zeros(2, np.int64)
+        check(self.lib.xrt_time_particles(self.ctx, n, fptr(ms),
+                                          cnt.ctypes.data_as(C.POINTER(C.c_int64))))
+        return float(ms[0]), float(ms[1]), {"entries": int(cnt[0]), "particles": int(cnt[1])}
