@@ -117,7 +117,11 @@ __global__ __launch_bounds__(kBlock) void ParticleCellKernel(ParticleBufs b, con
 
 // CacheNeighbors_Kernel (SpatialHashGPU.cu:56-97): thread t takes the particle at sorted
 // position t; the 27 cells around its cell (x-major), each cell's first max_nb particles,
-// those within the cell spacing now and farther apart than the diameter at the start
+// those within the cell spacing now and farther apart than the diameter at the start.
+// The 27 cell ranges are loaded up front and each cell's candidates kChunkN at a time
+// (independent loads in flight); the tests and the writes keep the reference's order.
+constexpr int kChunkN = 4;
+
 __global__ __launch_bounds__(kBlock) void ParticleNeighborKernel(ParticleBufs b, const int32_t* off,
                                                                  int n, const float4* __restrict__ x) {
   XRT_PARTICLE_INDEX()
@@ -131,34 +135,52 @@ __global__ __launch_bounds__(kBlock) void ParticleNeighborKernel(ParticleBufs b,
   const int id = idx[i];
   const float4 p = xp[id], o = op[id];
   const int32_t ix = IntCoord(p.x, c.spacing), iy = IntCoord(p.y, c.spacing), iz = IntCoord(p.z, c.spacing);
+  uint32_t lo[27], hi[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const int32_t h = HashCoords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, c.table);
+    lo[q] = cs[h];
+    hi[q] = ce[h];
+  }
   const int64_t cap = static_cast<int64_t>(c.nv) * c.max_nb;
   int64_t at = id;
-  for (int32_t cx = ix - 1; cx <= ix + 1; ++cx)
-    for (int32_t cy = iy - 1; cy <= iy + 1; ++cy)
-      for (int32_t cz = iz - 1; cz <= iz + 1; ++cz) {
-        const int32_t h = HashCoords(cx, cy, cz, c.table);
-        const uint32_t start = cs[h];
-        if (start == 0xffffffffu) continue;
-        const uint32_t lim = start + static_cast<uint32_t>(c.max_nb);
-        const uint32_t end = ce[h] < lim ? ce[h] : lim;
-        for (uint32_t s = start; s < end; ++s) {
-          const int j = idx[s];
-          if (j == id) continue;
-          const float4 q = xp[j];
-          if (!(Len2(p.x - q.x, p.y - q.y, p.z - q.z) < c.spacing2)) continue;
-          const float4 r = op[j];
-          if (!(Len2(o.x - r.x, o.y - r.y, o.z - r.z) > c.diameter2)) continue;
-          list[at] = static_cast<uint32_t>(j);
-          at += c.nv;
-          if (at >= cap) return;
-        }
+#pragma unroll 1
+  for (int q = 0; q < 27; ++q) {
+    const uint32_t start = lo[q];
+    if (start == 0xffffffffu) continue;
+    const uint32_t lim = start + static_cast<uint32_t>(c.max_nb);
+    const uint32_t end = hi[q] < lim ? hi[q] : lim;
+    for (uint32_t s0 = start; s0 < end; s0 += kChunkN) {
+      int jj[kChunkN];
+      float4 qq[kChunkN], rr[kChunkN];
+#pragma unroll
+      for (int u = 0; u < kChunkN; ++u) jj[u] = s0 + u < end ? idx[s0 + u] : id;
+#pragma unroll
+      for (int u = 0; u < kChunkN; ++u) {
+        qq[u] = xp[jj[u]];
+        rr[u] = op[jj[u]];
       }
+#pragma unroll
+      for (int u = 0; u < kChunkN; ++u) {
+        if (jj[u] == id) continue;  // the particle itself, or past the cell's end
+        if (!(Len2(p.x - qq[u].x, p.y - qq[u].y, p.z - qq[u].z) < c.spacing2)) continue;
+        if (!(Len2(o.x - rr[u].x, o.y - rr[u].y, o.z - rr[u].z) > c.diameter2)) continue;
+        list[at] = static_cast<uint32_t>(jj[u]);
+        at += c.nv;
+        if (at >= cap) return;
+      }
+    }
+  }
   list[at] = 0xffffffffu;  // at < cap here
 }
 
 // CollideParticles_Kernel (ClothSolverHelper.cu:171-211) with ComputeFriction (:156-169).
 // The list ends at the first entry >= nv (the 0xffffffff sentinel; the reference tests
-// j > num_particles, the same for every list the neighbor pass writes).
+// j > num_particles, the same for every list the neighbor pass writes).  Entries and the
+// neighbors' positions are fetched kChunk at a time (independent loads in flight), then
+// accumulated in list order, so the sums round exactly as in the one-by-one loop.
+constexpr int kChunk = 8;
+
 __global__ __launch_bounds__(kBlock) void ParticleCollideKernel(ParticleBufs b, const int32_t* off,
                                                                 int n, const float4* __restrict__ x,
                                                                 const float4* __restrict__ x0,
@@ -168,38 +190,54 @@ __global__ __launch_bounds__(kBlock) void ParticleCollideKernel(ParticleBufs b, 
   const float4* xp = x + c.vbeg;
   const float4* x0p = x0 + c.vbeg;
   const uint32_t* list = b.nb + c.nbeg;
+  const uint32_t nv = static_cast<uint32_t>(c.nv);
   const float4 p = xp[i], p0 = x0p[i];
   const float vx = p.x - p0.x, vy = p.y - p0.y, vz = p.z - p0.z;
   const float wi = p.w;
   float dx = 0.0f, dy = 0.0f, dz = 0.0f;
   int cnt = 0;
-  for (int s = 0; s < c.max_nb; ++s) {
-    const uint32_t j = list[static_cast<int64_t>(s) * c.nv + i];
-    if (j >= static_cast<uint32_t>(c.nv)) break;
-    const float4 q = xp[j];
-    const float denom = wi + q.w;
-    if (denom <= 0.0f) continue;
-    const float ex = p.x - q.x, ey = p.y - q.y, ez = p.z - q.z;
-    const float dist = sqrtf(Len2(ex, ey, ez));
-    if (dist >= c.diameter) continue;
-    const float de = dist + kEps;
-    const float gx = ex / de, gy = ey / de, gz = ez / de;
-    const float lambda = (dist - c.diameter) / denom;
-    const float cx = lambda * gx, cy = lambda * gy, cz = lambda * gz;
-    ++cnt;
-    dx -= wi * cx; dy -= wi * cy; dz -= wi * cz;
-    const float4 q0 = x0p[j];
-    const float rx = vx - (q.x - q0.x), ry = vy - (q.y - q0.y), rz = vz - (q.z - q0.z);
-    const float cl = sqrtf(Len2(cx, cy, cz));
-    if (mu > 0.0f && cl > 0.0f) {
-      const float nx = cx / cl, ny = cy / cl, nz = cz / cl;
-      const float vn = rx * nx + ry * ny + rz * nz;
-      const float tx = rx - nx * vn, ty = ry - ny * vn, tz = rz - nz * vn;
-      const float tl = sqrtf(Len2(tx, ty, tz));
-      const float r = (cl * mu) / tl;
-      const float sc = r < 1.0f ? r : 1.0f;  // MathFunctions::min
-      dx += wi * (-tx * sc); dy += wi * (-ty * sc); dz += wi * (-tz * sc);
-    }
+  bool done = false;
+  for (int s0 = 0; s0 < c.max_nb && !done; s0 += kChunk) {
+    uint32_t jj[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u)
+      jj[u] = s0 + u < c.max_nb ? list[static_cast<int64_t>(s0 + u) * c.nv + i] : 0xffffffffu;
+    int len = kChunk;  // entries of this chunk before the end of the list
+#pragma unroll
+    for (int u = kChunk - 1; u >= 0; --u)
+      if (jj[u] >= nv) len = u;
+    done = len < kChunk;
+    float4 q[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) q[u] = u < len ? xp[jj[u]] : p;
+    auto pair = [&](const float4 q, uint32_t j) {
+      const float denom = wi + q.w;
+      if (denom <= 0.0f) return;
+      const float ex = p.x - q.x, ey = p.y - q.y, ez = p.z - q.z;
+      const float dist = sqrtf(Len2(ex, ey, ez));
+      if (dist >= c.diameter) return;
+      const float de = dist + kEps;
+      const float gx = ex / de, gy = ey / de, gz = ez / de;
+      const float lambda = (dist - c.diameter) / denom;
+      const float cx = lambda * gx, cy = lambda * gy, cz = lambda * gz;
+      ++cnt;
+      dx -= wi * cx; dy -= wi * cy; dz -= wi * cz;
+      const float4 q0 = x0p[j];
+      const float rx = vx - (q.x - q0.x), ry = vy - (q.y - q0.y), rz = vz - (q.z - q0.z);
+      const float cl = sqrtf(Len2(cx, cy, cz));
+      if (mu > 0.0f && cl > 0.0f) {
+        const float nx = cx / cl, ny = cy / cl, nz = cz / cl;
+        const float vn = rx * nx + ry * ny + rz * nz;
+        const float tx = rx - nx * vn, ty = ry - ny * vn, tz = rz - nz * vn;
+        const float tl = sqrtf(Len2(tx, ty, tz));
+        const float r = (cl * mu) / tl;
+        const float sc = r < 1.0f ? r : 1.0f;  // MathFunctions::min
+        dx += wi * (-tx * sc); dy += wi * (-ty * sc); dz += wi * (-tz * sc);
+      }
+    };
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u)
+      if (u < len) pair(q[u], jj[u]);
   }
   b.delta[c.vbeg + i] = make_float4(dx, dy, dz, static_cast<float>(cnt));
 }
