@@ -193,6 +193,39 @@ int xrt_solve_ground(xrt_ctx* ctx, int32_t inst);
 int xrt_apply_deltas(xrt_ctx* ctx, int32_t inst);
 /* Finalize (ClothSolverHelper.cuh:20-41) */
 int xrt_finalize(xrt_ctx* ctx, int32_t inst, float dt);
+/* ------------------------------- obstacle skinning and cloth-to-body bindings */
+/* BindingConstraint::Add(indices, stiffnesses, distances, offset)
+ * (xrtailor/include/xrtailor/physics/dynamics/BindingConstraint.cuh:27-28), before
+ * xrt_start: instance-local cloth vertices bound to the instance's obstacle (set first;
+ * >= 2 faces); distance < 1e-6 takes the distance at xrt_start.  A vertex is bound at most
+ * once.  At xrt_start each bound instance gets QueryNearestBarycentricClothToObstacle over
+ * its cloth vertices and BindingConstraint::Generate; every iteration ends with
+ * BindingConstraint::Solve against the obstacle's frame positions. */
+int xrt_add_bindings(xrt_ctx* ctx, int32_t inst, const int32_t* idx, const float* stiffness,
+                     const float* distance, int32_t n);
+/* skinning weights of the obstacle's rest vertices (the body's sparse LBS weights,
+ * xrtailor/src/runtime/rag_doll/smpl/cuda/body.cu:39-58), before xrt_start: CSR over the
+ * obstacle vertices (ptr nv + 1, joint / weight ptr[nv]), 1..256 joints */
+int xrt_set_obstacle_skinning(xrt_ctx* ctx, int32_t inst, const int32_t* ptr, const int32_t* joint,
+                              const float* weight, int32_t n_joints);
+/* linear blend skinning (body.cu:39-58) of every skinned obstacle into its frame positions
+ * (as xrt_update_obstacle does from host positions): transforms of every skinned instance
+ * in instance order, n_joints x 12 floats each (3x4 row-major [R | t]) */
+int xrt_skin_obstacles(xrt_ctx* ctx, const float* transforms);
+int xrt_skin_obstacle(xrt_ctx* ctx, int32_t inst, const float* transforms);
+/* QueryNearestBarycentricClothToObstacle (xrtailor/include/xrtailor/physics/
+ * ClothSolverHelper.cuh:20-41) of the instance's cloth x0 against its obstacle's x0 tree
+ * (bvh tolerance): per cloth vertex idx[4] = (face vertices u, v, w, face) and
+ * uvw[4] = (weights u, v, w, distance) */
+int xrt_query_nearest_barycentric(xrt_ctx* ctx, int32_t inst, int32_t* idx, float* uvw);
+/* the generated bindings (BindingConstraint::bindings): idx[4] = (cloth vertex, body
+ * vertices u, v, w), par[5] = (u, v, w, L, stiffness) */
+int xrt_get_bindings(xrt_ctx* ctx, int32_t inst, int32_t* idx, float* par);
+/* BindingConstraint::Solve(cloth_nodes, obstacle_nodes) (BindingConstraint.cuh:31) */
+int xrt_solve_bindings(xrt_ctx* ctx, int32_t inst);
+/* obstacle positions: start of the frame (x0) and the frame's target (x); either may be null */
+int xrt_get_obstacle(xrt_ctx* ctx, int32_t inst, float* x0, float* x);
+
 /* SpatialHashGPU::Hash(nodes, n) of one instance's predicted positions x
  * (xrtailor/include/xrtailor/physics/broad_phase/spatial_hash/SpatialHashGPU.cuh:26-28):
  * hash, sort, cell table, neighbor lists (needs particle collision on) */

@@ -434,3 +434,130 @@ long orc_bvh_traverse(const orc_bvh* lhs, const orc_bvh* rhs, float thickness, i
   free(tmp);
   return total;
 }
+
+/* ------------------------------------------------- nearest face (setup query) */
+/* MinDist / MinMaxDist (xrtailor/src/physics/broad_phase/lbvh/BVHHelper.cu:114-151):
+ * squared distances from p to a box and the min-max metric of Roussopoulos et al. */
+static float min_dist(const float* bx, f3 p) {
+  const float dx = fminf(bx[3], fmaxf(bx[0], p.x)) - p.x;
+  const float dy = fminf(bx[4], fmaxf(bx[1], p.y)) - p.y;
+  const float dz = fminf(bx[5], fmaxf(bx[2], p.z)) - p.z;
+  return dx * dx + dy * dy + dz * dz;
+}
+
+static float min_max_dist(const float* bx, f3 p) {
+  float rm[3], rM[3];
+  const float pp[3] = {p.x, p.y, p.z};
+  for (int k = 0; k < 3; k++) {
+    rm[k] = (bx[k] - pp[k]) * (bx[k] - pp[k]);
+    rM[k] = (bx[3 + k] - pp[k]) * (bx[3 + k] - pp[k]);
+    if ((bx[3 + k] + bx[k]) * 0.5f < pp[k]) { float t = rm[k]; rm[k] = rM[k]; rM[k] = t; }
+  }
+  const float dx = rm[0] + rM[1] + rM[2];
+  const float dy = rM[0] + rm[1] + rM[2];
+  const float dz = rM[0] + rM[1] + rm[2];
+  return mn(dx, mn(dy, dz));
+}
+
+/* distance_calculator (BVHHelper.cu:9-103): Ericson's closest point on triangle abc with
+ * the barycentric weights of the region it falls in; returns |p - q| */
+float orc_tri_nearest(const float* p3, const float* a3, const float* b3, const float* c3,
+                      float* uvw) {
+  const f3 p = {p3[0], p3[1], p3[2]}, a = {a3[0], a3[1], a3[2]};
+  const f3 b = {b3[0], b3[1], b3[2]}, c = {c3[0], c3[1], c3[2]};
+  const f3 e1 = f3sub(b, a), e2 = f3sub(c, a), e3 = f3sub(c, b);
+  f3 q;
+  const f3 v1 = f3sub(p, a);
+  const float d1 = f3dot(e1, v1), d2 = f3dot(e2, v1);
+  if (d1 <= 0 && d2 <= 0) {
+    uvw[0] = 1; uvw[1] = 0; uvw[2] = 0;
+    q = a;
+    return sqrtf(f3dot(f3sub(p, q), f3sub(p, q)));
+  }
+  const f3 v2 = f3sub(p, b);
+  const float d3 = f3dot(e1, v2), d4 = f3dot(e2, v2);
+  if (d3 >= 0 && d4 <= d3) {
+    uvw[0] = 0; uvw[1] = 1; uvw[2] = 0;
+    q = b;
+    return sqrtf(f3dot(f3sub(p, q), f3sub(p, q)));
+  }
+  const float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+    const float v = d1 / (d1 - d3);
+    q = f3add(a, f3scl(e1, v));
+    uvw[0] = 1 - v; uvw[1] = v; uvw[2] = 0;
+    return sqrtf(f3dot(f3sub(p, q), f3sub(p, q)));
+  }
+  const f3 v3 = f3sub(p, c);
+  const float d5 = f3dot(e1, v3), d6 = f3dot(e2, v3);
+  if (d6 >= 0 && d5 <= d6) {
+    uvw[0] = 0; uvw[1] = 0; uvw[2] = 1;
+    q = c;
+    return sqrtf(f3dot(f3sub(p, q), f3sub(p, q)));
+  }
+  const float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+    const float w = d2 / (d2 - d6);
+    q = f3add(a, f3scl(e2, w));
+    uvw[0] = 1 - w; uvw[1] = 0; uvw[2] = w;
+    return sqrtf(f3dot(f3sub(p, q), f3sub(p, q)));
+  }
+  const float va = d3 * d6 - d5 * d4;
+  if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+    const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    q = f3add(b, f3scl(e3, w));
+    uvw[0] = 0; uvw[1] = 1 - w; uvw[2] = w;
+    return sqrtf(f3dot(f3sub(p, q), f3sub(p, q)));
+  }
+  const float d = 1 / (va + vb + vc);
+  const float v = vb * d, w = vc * d;
+  q = f3add(f3add(a, f3scl(e1, v)), f3scl(e2, w));
+  uvw[0] = 1 - v - w; uvw[1] = v; uvw[2] = w;
+  return sqrtf(f3dot(f3sub(p, q), f3sub(p, q)));
+}
+
+/* QueryDevice (BVH.cu:881-948): depth-first search with a 64-entry stack of (node, squared
+ * min distance), children tested left then right, pushed left then right; a node popped
+ * with a squared min distance above the best (unsquared) distance is skipped, a child is
+ * kept when its min distance <= the sibling's min-max distance, and a leaf replaces the best
+ * when its distance is <= it.  Triangles are the tree's corner points (x0 + tol n).
+ * Returns the face, -1 for an empty tree, -2 when the stack would overflow. */
+int orc_bvh_nearest(const orc_bvh* b, const float* p3, float* uvw, float* dist) {
+  if (b->nf < 2) return -1;
+  const f3 p = {p3[0], p3[1], p3[2]};
+  int st_node[64];
+  float st_dist[64];
+  int sp = 0;
+  st_node[sp] = 0;
+  st_dist[sp++] = min_dist(b->box, p);
+  int nearest = -1;
+  float best = INFINITY;
+  do {
+    const int node = st_node[--sp];
+    const float md = st_dist[sp];
+    if (md > best) continue;
+    const int kid[2] = {b->nodes[4 * node + 1], b->nodes[4 * node + 2]};
+    const float kmd[2] = {min_dist(b->box + 6 * kid[0], p), min_dist(b->box + 6 * kid[1], p)};
+    const float kmm[2] = {min_max_dist(b->box + 6 * kid[0], p), min_max_dist(b->box + 6 * kid[1], p)};
+    for (int s = 0; s < 2; s++) {
+      if (!(kmd[s] <= kmm[1 - s])) continue;
+      const int obj = b->nodes[4 * kid[s] + 3];
+      if (obj >= 0) {
+        float w[3];
+        const int32_t* t = b->tris + 3 * obj;
+        const float d = orc_tri_nearest(p3, b->v + 3 * t[0], b->v + 3 * t[1], b->v + 3 * t[2], w);
+        if (d <= best) {
+          best = d;
+          nearest = obj;
+          uvw[0] = w[0]; uvw[1] = w[1]; uvw[2] = w[2];
+        }
+      } else {
+        if (sp >= 64) return -2;
+        st_node[sp] = kid[s];
+        st_dist[sp++] = kmd[s];
+      }
+    }
+  } while (sp > 0);
+  *dist = best;
+  return nearest;
+}

@@ -18,6 +18,7 @@ ClothSolverBase::SimulateSwift's body-cloth PBD collision
     repulsion over the pairs, applied x relaxation rate
     stretch colors, bend colors, LRA, ground
     VF-CCD, VF-DCD solves; EE-DCD deltas applied x relaxation factor
+    bindings: BindingConstraint::Solve against the obstacle's frame positions (:237)
   finalize; frame end: obstacle x0 = x
 """
 from __future__ import annotations
@@ -28,6 +29,7 @@ import numpy as np
 
 from . import oracle as orc
 from . import particle as opart
+from . import binding as obind
 
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int32)
@@ -151,6 +153,7 @@ class ContactScene:
         self.pass_cnt = 0
         self.dt = float(np.float32(xp.frame_time) / np.float32(xp.num_substeps))
         self.tape = None  # a list: substep() appends each operator's input state (adjoint.py)
+        self.bindings = False
         # swift-mode particle self-collision: diameter from the rest mesh, initial positions
         # = the state now (FP32), neighbor lists zero until the first hash
         self.particles = bool(getattr(xp, "enable_particle_collision", 0))
@@ -158,6 +161,25 @@ class ContactScene:
             self.p_diam = opart.particle_diameter(rest_pos, tris, xp.particle_diameter)
             self.p_orig = cloth.x0.reshape(-1, 3).astype(np.float32)
             self.p_nb = np.zeros((xp.max_num_neighbors, self.nv), np.uint32)
+
+    def setup_bindings(self, vert, stiffness, distance, tol):
+        """QueryNearestBarycentricClothToObstacle over the cloth's current x0 (the obstacle
+        tree from its x0 with tolerance `tol` along the vertex normals, FP32) and
+        BindingConstraint::Generate (FP64) for the bound vertices."""
+        ox0 = np.ascontiguousarray(self.o_x0, np.float32)
+        tree = orc.Bvh()
+        tree.Build(ox0, self.o_tris, tol=tol, n=orc.vertex_normals(ox0, self.o_tris))
+        faces, uvw, dist = obind.nearest(tree, self.c.x0.reshape(-1, 3).astype(np.float32))
+        self.near = (faces, uvw, dist)
+        vert = np.ascontiguousarray(vert, np.int32)
+        self.b_vert = vert
+        self.b_tri3 = np.ascontiguousarray(self.o_tris[faces[vert]], np.int32)
+        self.b_uvw = uvw[vert].astype(np.float64)
+        self.b_stiff = np.broadcast_to(np.asarray(stiffness, np.float64), vert.shape).copy()
+        d_in = np.broadcast_to(np.asarray(distance, np.float32), vert.shape)
+        self.b_L = obind.generate(self.c.x0.reshape(-1, 3), ox0.astype(np.float64), vert,
+                                  self.b_tri3, self.b_uvw, d_in.astype(np.float64))
+        self.bindings = True
 
     def _rec(self, *entry):
         if self.tape is not None:
@@ -254,6 +276,9 @@ class ContactScene:
                 self._rec("ee", self.ee.copy(), c.x.copy(), xp.relaxation_factor)
                 lib.orc_ee_solve(_d(self.ee), _d(c.x), _i(self.edges), self.ne, _d(self.delta), _i(self.count))
                 lib.orc_apply_deltas(_d(c.x), _d(self.delta), _i(self.count), self.nv, xp.relaxation_factor)
+            if self.bindings:
+                obind.solve(c.x.reshape(-1, 3), c.w, np.asarray(self.o_x, np.float64), self.b_vert,
+                            self.b_tri3, self.b_uvw, self.b_L, self.b_stiff)
         self._rec("finalize")
         lib.orc_finalize(C.byref(c.s), C.byref(p), dt)
         self.frame_substep += 1

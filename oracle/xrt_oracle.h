@@ -203,6 +203,23 @@ void orc_adj_vf(const float* obs_pred, const int32_t* obs_tris, const int32_t* f
 void orc_adj_ee(const double* rec12, const double* x_pre, const int32_t* edges, int ne, int nv,
                 double relax, double* gx);
 
+/* ---------------------------------------- nearest face, LBS, binding (FP32 query) */
+/* closest point of triangle abc to p: distance, barycentric weights (bvh_oracle.c) */
+float orc_tri_nearest(const float* p, const float* a, const float* b, const float* c, float* uvw);
+/* QueryDevice nearest-face search of a built tree: face, -1 empty, -2 stack overflow */
+int orc_bvh_nearest(const orc_bvh* b, const float* p, float* uvw, float* dist);
+/* linear blend skinning (binding_oracle.c, FP64): T is n_joints x 12 row-major (3x4) */
+void orc_lbs(const double* rest, int nv, const int32_t* ptr, const int32_t* joint,
+             const double* weight, const double* T, double* out);
+/* BindingConstraint::Generate: L per binding (distance < EPSILON -> current distance) */
+void orc_binding_generate(const double* cloth_x0, const double* obs_x0, const int32_t* vert,
+                          const int32_t* tri3, const double* uvw, const double* dist_in, int n,
+                          double* L);
+/* BindingConstraint::Solve over n bindings: cloth x (3nv) moved towards the obstacle x */
+void orc_binding_solve(double* x, const double* w, const double* obs_x, const int32_t* vert,
+                       const int32_t* tri3, const double* uvw, const double* L,
+                       const double* stiffness, int n);
+
 /* ------------------------------ particle self-collision (swift mode, particle_oracle.c) */
 int32_t orc_hash_coords(int32_t x, int32_t y, int32_t z, int32_t table);
 /* spatial hash + neighbor lists (column-major, n * max_nb) from FP32 positions; the

@@ -1,0 +1,178 @@
+"""GPU obstacle skinning (LBS), nearest-face records (QueryNearestBarycentricClothToObstacle)
+and the binding constraint against the oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+from contact_helpers import oracle_scene
+from gpu_helpers import rel_err
+from oracle import binding as ob
+from oracle import oracle as orc
+from xrtailor_amd import XrtError, scenes
+from xrtailor_amd.solver import build_solver, make_params
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _garment(seed=6, ring=24, rows=16, joints=4):
+    spec, body = scenes.garment(seed=seed, ring=ring, rows=rows, joints=joints, gap=0.01,
+                                body_subdiv=2)
+    top = np.arange((rows - 1) * ring, rows * ring, dtype=np.int32)
+    mid = np.arange((rows // 2) * ring, (rows // 2 + 1) * ring, dtype=np.int32)
+    bind = np.concatenate([top, mid])
+    stiff = np.concatenate([np.ones(ring), np.full(ring, 0.5)]).astype(np.float32)
+    dist = np.concatenate([np.zeros(ring), np.full(ring, 0.02)]).astype(np.float32)
+    return spec, body, bind, stiff, dist
+
+
+def _build(cases, xp, bindings=True, skin=True):
+    def before(s, insts):
+        for i, (spec, body, bind, stiff, dist) in zip(insts, cases):
+            if skin:
+                s.SetObstacleSkinning(i, *body.skinning(), body.amp.shape[0])
+            if bindings:
+                s.AddBindings(i, bind, stiff, dist)
+    return build_solver([c[0] for c in cases], xp, seed=1, before_start=before)
+
+
+def _xp(**kw):
+    base = dict(bend_model=1, enable_obstacle_contact=1, ground_height=-1.0)
+    base.update(kw)
+    return make_params(**base)
+
+
+def test_lbs_matches_oracle():
+    print("case LBS of two capsule-chain bodies", flush=True)
+    cases = [_garment(seed=6), _garment(seed=9, joints=6)]
+    s, insts = _build(cases, _xp(), bindings=False)
+    t = 0.3
+    s.SkinObstacles([c[1].transforms(t) for c in cases])
+    for i, (spec, body, *_r) in zip(insts, cases):
+        nv = len(body.pos)
+        x0, x = s.GetObstacle(i, nv)
+        ptr, joint, w = body.skinning()
+        want = ob.lbs(body.pos, ptr, joint, w, body.transforms(t))
+        err = rel_err(x, want)
+        print(f"  instance {i}: {nv} body vertices, rel err vs FP64 LBS {err:.2e}, "
+              f"vs forward kinematics {np.abs(x - body.at(t)).max():.2e} m", flush=True)
+        assert err <= 1e-6
+        np.testing.assert_array_equal(x0, body.pos.astype(np.float32))  # frame start untouched
+    # one instance alone
+    s.SkinObstacle(insts[1], cases[1][1].transforms(0.6))
+    _, x = s.GetObstacle(insts[1], len(cases[1][1].pos))
+    assert rel_err(x, cases[1][1].at(0.6)) <= 1e-6
+    s.close()
+
+
+@pytest.mark.parametrize("tol", [0.003, 0.0])
+def test_nearest_barycentric_matches_oracle(tol):
+    """Face, weights and distance bit for bit from the same FP32 inputs (ties between
+    faces sharing the nearest vertex or edge resolved in the same traversal order)."""
+    print(f"case nearest barycentric, bvh tolerance {tol}", flush=True)
+    spec, body, *_ = _garment()
+    s, (i,) = build_solver([spec], _xp(bvh_tolerance=tol), seed=1)
+    idx, uvw = s.QueryNearestBarycentric(i)
+    ox0 = np.asarray(body.pos, np.float32)
+    tree = orc.Bvh()
+    tree.Build(ox0, body.tris, tol=tol, n=orc.vertex_normals(ox0, body.tris))
+    faces, wo, do = ob.nearest(tree, s.GetState(i)[0])
+    np.testing.assert_array_equal(idx[:, 3], faces)
+    np.testing.assert_array_equal(idx[:, :3], np.asarray(body.tris)[faces])
+    np.testing.assert_array_equal(uvw[:, :3], wo)
+    np.testing.assert_array_equal(uvw[:, 3], do)
+    corner = (uvw[:, :3] == 1.0).any(axis=1).sum()
+    print(f"  {len(faces)} cloth vertices, {corner} nearest at a body vertex, "
+          f"distance {do.min():.4f}..{do.max():.4f} m", flush=True)
+    s.close()
+
+
+def test_binding_generate_and_solve_match_oracle():
+    print("case binding generate + one solve vs oracle", flush=True)
+    case = _garment()
+    spec, body, bind, stiff, dist = case
+    xp = _xp()
+    s, (i,) = _build([case], xp)
+    twin = oracle_scene(spec, xp, solver=s, inst=i)
+    twin.setup_bindings(bind, stiff, dist, xp.bvh_tolerance)
+    bidx, par = s.Bindings(i)
+    np.testing.assert_array_equal(bidx[:, 0], bind)
+    np.testing.assert_array_equal(bidx[:, 1:], twin.b_tri3)
+    np.testing.assert_array_equal(par[:, :3], twin.b_uvw.astype(np.float32))
+    np.testing.assert_allclose(par[:, 3], twin.b_L, rtol=1e-6)
+    np.testing.assert_array_equal(par[:, 4], stiff)
+    # move the body and solve once from the current state
+    s.SkinObstacle(i, body.transforms(0.25))
+    _, ox = s.GetObstacle(i, len(body.pos))
+    x_before = s.GetPredicted(i).astype(np.float64)
+    s.SolveBindings(i)
+    got = s.GetPredicted(i)
+    ref = x_before.copy()
+    ob.solve(ref, s.inv_mass(i), ox.astype(np.float64), twin.b_vert, twin.b_tri3, twin.b_uvw,
+             twin.b_L, twin.b_stiff)
+    moved = np.abs(ref - x_before).max(axis=1) > 0
+    err = rel_err(got, ref)
+    print(f"  {moved.sum()} of {len(bind)} bound vertices pulled back, rel err {err:.2e}",
+          flush=True)
+    assert moved.sum() > 0 and err <= TOL
+    s.close()
+
+
+def test_garment_with_bindings_and_skinned_body_substeps():
+    """The tube bound to the body at its top ring and waist, the body skinned on the GPU
+    each frame: plain substeps against the oracle twin (LRA, body contact and bindings)."""
+    print("case garment + bindings + GPU-skinned body vs oracle twin", flush=True)
+    case = _garment()
+    spec, body, bind, stiff, dist = case
+    xp = _xp()
+    s, (i,) = _build([case], xp)
+    twin = oracle_scene(spec, xp, solver=s, inst=i)
+    twin.setup_bindings(bind, stiff, dist, xp.bvh_tolerance)
+    s.SkinObstacle(i, body.transforms(1.0 / 60.0))
+    twin.o_x = s.GetObstacle(i, len(body.pos))[1].copy()
+    errs = []
+    for _ in range(3):
+        s.SubstepPlain(i)
+        twin.substep()
+        errs.append(rel_err(s.GetState(i)[0], twin.c.state()[0]))
+    print(f"  rel err per substep {', '.join(f'{e:.2e}' for e in errs)}", flush=True)
+    assert errs[0] <= TOL
+    assert max(errs) <= 1e-4
+    s.close()
+
+
+def test_bindings_batched_equals_plain():
+    print("case bindings + skinning, batched frames vs plain", flush=True)
+    cases = [_garment(seed=6), _garment(seed=8, ring=20, rows=14, joints=5)]
+    xp = _xp(num_substeps=4, num_iterations=4)
+    runs = []
+    for batched in (True, False):
+        s, insts = _build(cases, xp)
+        for f in range(2):
+            s.SkinObstacles([c[1].transforms((f + 1) / 60.0) for c in cases])
+            if batched:
+                s.FramesBatched(1)
+            else:
+                s.FramesPlain(1)
+        runs.append([s.GetState(k)[0] for k in insts])
+        s.close()
+    for k, (a, b) in enumerate(zip(*runs)):
+        print(f"  instance {k}: max |batched - plain| = {np.abs(a - b).max():.3e}", flush=True)
+        np.testing.assert_array_equal(a, b)
+
+
+def test_binding_errors():
+    print("case binding error paths", flush=True)
+    spec, body, bind, stiff, dist = _garment()
+    with pytest.raises(XrtError):  # bound twice
+        _build([(spec, body, np.r_[bind, bind[:1]], np.r_[stiff, stiff[:1]], np.r_[dist, dist[:1]])],
+               _xp())
+    s, (i,) = _build([(spec, body, bind, stiff, dist)], _xp())
+    with pytest.raises(XrtError):
+        s.AdjointBegin(1, 1)  # the adjoint has no bindings
+    with pytest.raises(XrtError):
+        s.AddBindings(i, bind[:2], 1.0, 0.0)  # after Start
+    s.close()
+    bare = scenes.ClothSpec(spec.pos, spec.tris, spec.pinned, 1e-8, 1e-4, spec.x0)
+    with pytest.raises(XrtError):  # no obstacle
+        build_solver([bare], _xp(), before_start=lambda s, ii: s.AddBindings(ii[0], bind, 1.0, 0.0))

@@ -106,6 +106,14 @@ SIGNATURES = {
     "xrt_solve_ground": [_P, _I],
     "xrt_apply_deltas": [_P, _I],
     "xrt_finalize": [_P, _I, _F],
+    "xrt_add_bindings": [_P, _I, _IP, _FP, _FP, _I],
+    "xrt_set_obstacle_skinning": [_P, _I, _IP, _IP, _FP, _I],
+    "xrt_skin_obstacles": [_P, _FP],
+    "xrt_skin_obstacle": [_P, _I, _FP],
+    "xrt_query_nearest_barycentric": [_P, _I, _IP, _FP],
+    "xrt_get_bindings": [_P, _I, _IP, _FP],
+    "xrt_solve_bindings": [_P, _I],
+    "xrt_get_obstacle": [_P, _I, _FP, _FP],
     "xrt_particle_hash": [_P, _I],
     "xrt_collide_particles": [_P, _I],
     "xrt_get_particle_hash": [_P, _I, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _IP,

@@ -35,7 +35,8 @@ HIP_FLAGS = [
 ]
 # kernels whose float results feed integer outputs (Morton codes, AABB tests) are built
 # without FMA contraction so that they agree bit for bit with the FP32-strict oracle
-STRICT_FP = {"bvh.hip", "contact.hip", "adjoint_contact.hip", "particle.hip", "particle_api.hip"}
+STRICT_FP = {"bvh.hip", "contact.hip", "adjoint_contact.hip", "particle.hip", "particle_api.hip",
+             "binding.hip"}
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall"]
 
 

@@ -49,6 +49,7 @@ int AdjSetup(xrt_ctx* c, int want_slots = 1) {
               "the adjoint does not support self contact (PBD repulsion)");
   XRT_REQUIRE(!c->params.enable_particle_collision, XRT_ERR_UNSUPPORTED,
               "the adjoint does not support particle self-collision");
+  XRT_REQUIRE(!BindingsOn(c), XRT_ERR_UNSUPPORTED, "the adjoint does not support bindings");
   const xrt_params& p = c->params;
   const int n_inst = static_cast<int>(c->inst.size());
   XRT_REQUIRE(n_inst > 0 && n_inst <= 65535, XRT_ERR_INVALID_ARG, "1..65535 instances");
@@ -243,6 +244,7 @@ int WideSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, AdjointData
       if (TI) XRT_CHECK_HIP(LaunchCopy4(s, TI + A->off_e, c->x.ptr, N));
       if ((rc = MultiContactEeSolve(c)) != XRT_OK) return rc;
     }
+    if ((rc = BindingIter(c, -1)) != XRT_OK) return rc;  // never taped: the adjoint refuses
   }
   XRT_CHECK_HIP(LaunchFinalize(s, c->x0.ptr, c->v.ptr, c->x.ptr, N, k));
   if ((rc = ParticleSubstepEnd(c, -1)) != XRT_OK) return rc;

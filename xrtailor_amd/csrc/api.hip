@@ -168,12 +168,13 @@ int SubstepPlain(xrt_ctx* c, int32_t inst) {
   return ParticleSubstepEnd(c, inst);
 }
 
-// Without contacts: one persistent launch.  With body, self contact or particle
-// collision: the wide substep (each color, contact phase, hash step and LBVH operation of
-// every instance in one launch).
+// Without contacts: one persistent launch.  With body or self contact, particle collision
+// or bindings: the wide substep (each color, contact phase, hash step, binding pass and
+// LBVH operation of every instance in one launch).
 int SubstepBatched(xrt_ctx* c) {
   const StepConsts k = MakeConsts(c->params);
-  if (ContactsActive(c) || ParticlesOn(c)) return WideSubstep(c, k, ModeOf(k), nullptr, -1);
+  if (ContactsActive(c) || ParticlesOn(c) || BindingsOn(c))
+    return WideSubstep(c, k, ModeOf(k), nullptr, -1);
   XRT_CHECK_HIP(LaunchBatchedSubstep(c->stream, MakeBatchArgs(c), k,
                                      static_cast<int>(c->inst.size()), c->batched_block,
                                      c->batched_lds));
@@ -629,6 +630,7 @@ int xrt_start(xrt_ctx* ctx, uint64_t seed) {
   int rc = ContactStart(ctx);
   if (rc != XRT_OK) return rc;
   if ((rc = ParticleStart(ctx)) != XRT_OK) return rc;
+  if ((rc = BindingStart(ctx)) != XRT_OK) return rc;
   ctx->started = true;
   return XRT_OK;
 }

@@ -13,6 +13,7 @@
 //     repulsion over the pairs -> impulses x relaxation rate
 //     stretch colors, bend colors, LRA, ground
 //     solve VF-CCD, VF-DCD (direct), EE-DCD (deltas) -> ApplyDeltas x relaxation factor
+//     bindings (BindingConstraint::Solve against the obstacle's frame positions)
 //   finalize
 // Frame start: obstacle LBVH rebuilt from its x0 (tol 0), cloth LBVH rebuilt from x0.
 // Frame end: obstacle x0 = x (PhysicsMesh::PredictedToPositions).
@@ -510,6 +511,7 @@ int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendM
     if (k.enable_ground) XRT_CHECK_HIP(LaunchGround(s, x, x0, d.nv, k.ground_y, k.friction));
     if ((rc = ContactVfSolve(ctx, inst)) != XRT_OK) return rc;
     if ((rc = ContactEeSolve(ctx, inst)) != XRT_OK) return rc;
+    if ((rc = BindingIter(ctx, inst)) != XRT_OK) return rc;
   }
   XRT_CHECK_HIP(LaunchFinalize(s, x0, v, x, d.nv, k));
   if ((rc = ParticleSubstepEnd(ctx, inst)) != XRT_OK) return rc;

@@ -8,6 +8,8 @@
 #include <vector>
 
 #include "../../include/xrtailor_amd.h"
+#include "binding.h"
+#include "blockmap.h"
 #include "bvh.h"
 #include "common.h"
 #include "contact.h"
@@ -46,6 +48,13 @@ struct HostInstance {
   int obs_nv() const { return static_cast<int>(obs_pos.size() / 3); }
   int obs_nt() const { return static_cast<int>(obs_tris.size() / 3); }
   ContactTopo cloth_topo, obs_topo;  // built by xrt_start
+  // cloth-to-body bindings (xrt_add_bindings) and the obstacle's skinning weights
+  // (xrt_set_obstacle_skinning)
+  std::vector<int32_t> bind_vert;
+  std::vector<float> bind_stiff, bind_dist;
+  std::vector<int32_t> skin_ptr, skin_joint;
+  std::vector<float> skin_weight;
+  int n_joints = 0;
 };
 
 }  // namespace xrt
@@ -95,10 +104,6 @@ struct ContactData {
 
 // Particle self-collision state of every instance (particle.h has the HBM layout).
 struct ParticleData {
-  struct MapRef {
-    size_t off = 0;
-    int n = 0, blocks = 0;
-  };
   bool sized = false;      // buffers allocated for max_nb
   int max_nb = 0, passes = 0;
   std::vector<ParticleInstDev> inst;
@@ -114,6 +119,31 @@ struct ParticleData {
   // xrt_time_particles: event pairs around the collision kernels and the hash steps
   std::vector<hipEvent_t> collide_events, hash_events;
   size_t collide_next = 0, hash_next = 0;
+};
+
+// Bindings, nearest-face records and obstacle skinning of every instance (binding.h).
+struct BindingData {
+  int total = 0;                   // bindings over every instance
+  std::vector<BindInstDev> inst;   // per instance (nb 0 when it has none)
+  DevBuf<BindInstDev> dev;
+  DevBuf<BindItem> items;
+  DevBuf<int4> near_idx;           // nearest-face record of every cloth vertex
+  DevBuf<float4> near_uvw;
+  DevBuf<int32_t> flag;
+  DevBuf<int32_t> maps;            // binding blocks: every instance, then each alone
+  MapRef all;
+  std::vector<MapRef> one;
+  // skinning
+  int total_joints = 0;
+  std::vector<int> joint_off;      // per instance: offset of its transforms (-1: none)
+  std::vector<SkinInstDev> skin;
+  DevBuf<SkinInstDev> skin_dev;
+  DevBuf<int32_t> sk_ptr, sk_joint;
+  DevBuf<float> sk_weight, transforms;
+  DevBuf<float4> sk_rest;
+  DevBuf<int32_t> skin_maps;       // obstacle-vertex blocks: every instance, each alone
+  MapRef skin_all;
+  std::vector<MapRef> skin_one;
 };
 
 // Sizes of the wide launches (one color of every instance per launch).
@@ -199,6 +229,8 @@ struct xrt_ctx {
   xrt::WideConfig wide;
   // swift-mode particle self-collision
   xrt::ParticleData particles;
+  // obstacle skinning and cloth-to-body bindings
+  xrt::BindingData binding;
   // differentiable rollout (allocated by xrt_adjoint_begin)
   std::unique_ptr<xrt::AdjointData> adj;
   // graph of one batched frame
@@ -242,6 +274,12 @@ int ParticleConfigure(xrt_ctx* ctx);
 int ParticleSubstepBegin(xrt_ctx* ctx, int inst);  // after predict: hash when due
 int ParticleIter(xrt_ctx* ctx, int inst);          // iteration start: collide + apply
 int ParticleSubstepEnd(xrt_ctx* ctx, int inst);    // after finalize
+// binding_api.hip: nearest-face records, binding generation and skinning set-up at
+// xrt_start (after ContactStart); the binding solve at the end of every iteration (an
+// instance, or -1 for every instance)
+bool BindingsOn(const xrt_ctx* ctx);
+int BindingStart(xrt_ctx* ctx);
+int BindingIter(xrt_ctx* ctx, int inst);
 // adjoint_api.hip: one substep of every instance as wide launches with the contact phases
 // above; with a tape owner and sub >= 0 the operators' inputs go to tape slot `sub`
 int WideSubstep(xrt_ctx* ctx, const StepConsts& k, const BendMode& bm, AdjointData* tape, int sub);

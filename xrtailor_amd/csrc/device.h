@@ -36,6 +36,12 @@ __device__ __forceinline__ int InstOfBlock(const int32_t* __restrict__ off, int 
   return off[n + 1 + b];
 }
 
+// Maps built by AppendBlockMap (blockmap.h) list entries instead: InstOfBlock gives the
+// entry of a block and this the instance of an entry.
+__device__ __forceinline__ int InstOfEntry(const int32_t* __restrict__ off, int n, int j) {
+  return off[n + 1 + off[n] + j];
+}
+
 struct StretchC {  // BasicConstraint::StretchConstraint without lambda (BasicConstraint.cuh:8-14)
   int32_t i0, i1;
   float rest, compliance;

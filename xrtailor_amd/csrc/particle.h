@@ -15,6 +15,8 @@
 
 #include <cstdint>
 
+#include "blockmap.h"
+
 namespace xrt {
 
 struct ParticleInstDev {
@@ -36,25 +38,18 @@ struct ParticleBufs {
   float4* delta;
 };
 
-// a block map (off: n + 1 prefix sums over its entries, the entry of every block, the
-// instance of every entry) and its size
-struct ParticleMap {
-  const int32_t* off = nullptr;
-  int n = 0, blocks = 0;
-};
-
 // SpatialHashGPU::Hash of the instances of `vmap` / `tmap` (per-vertex and per-radix-tile
 // maps over the same instances): hash x, stable radix sort (passes of 8 bits, an even
 // count), cell table, neighbor lists.
-hipError_t LaunchParticleHash(hipStream_t s, const ParticleBufs& b, const ParticleMap& vmap,
-                              const ParticleMap& tmap, int passes, const float4* x);
+hipError_t LaunchParticleHash(hipStream_t s, const ParticleBufs& b, const DevMap& vmap,
+                              const DevMap& tmap, int passes, const float4* x);
 // CollideParticles: Jacobi deltas from the neighbor lists, then ApplyDeltas x relax;
 // events (optional) are recorded around the collision kernel alone
-hipError_t LaunchCollideParticles(hipStream_t s, const ParticleBufs& b, const ParticleMap& vmap,
+hipError_t LaunchCollideParticles(hipStream_t s, const ParticleBufs& b, const DevMap& vmap,
                                   float4* x, const float4* x0, float mu, float relax,
                                   hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 // adds the number of list entries (up to each sentinel) of the map's instances to *total
-hipError_t LaunchCountNeighbors(hipStream_t s, const ParticleBufs& b, const ParticleMap& vmap,
+hipError_t LaunchCountNeighbors(hipStream_t s, const ParticleBufs& b, const DevMap& vmap,
                                 unsigned long long* total);
 
 }  // namespace xrt

@@ -38,12 +38,6 @@ __device__ __forceinline__ float Len2(float dx, float dy, float dz) {
   return dx * dx + dy * dy + dz * dz;
 }
 
-// A particle map lists n entries (instances): off[0..n] the prefix sums of their block
-// counts, then the entry of every block, then the instance of every entry.
-__device__ __forceinline__ int InstOfEntry(const int32_t* __restrict__ off, int n, int j) {
-  return off[n + 1 + off[n] + j];
-}
-
 #define XRT_PARTICLE_INDEX()                                     \
   const int lblk = XcdBlock(off, n);                             \
   if (lblk < 0) return;                                          \
@@ -277,8 +271,8 @@ __global__ __launch_bounds__(kBlock) void ParticleCountKernel(ParticleBufs b, co
 }
 }  // namespace
 
-hipError_t LaunchParticleHash(hipStream_t s, const ParticleBufs& b, const ParticleMap& vmap,
-                              const ParticleMap& tmap, int passes, const float4* x) {
+hipError_t LaunchParticleHash(hipStream_t s, const ParticleBufs& b, const DevMap& vmap,
+                              const DevMap& tmap, int passes, const float4* x) {
   if (vmap.blocks <= 0) return hipSuccess;
   ParticleHashKernel<<<XcdGrid(vmap.blocks), kBlock, 0, s>>>(b, vmap.off, vmap.n, x);
   for (int pass = 0; pass < passes; ++pass) {
@@ -291,7 +285,7 @@ hipError_t LaunchParticleHash(hipStream_t s, const ParticleBufs& b, const Partic
   return hipGetLastError();
 }
 
-hipError_t LaunchCollideParticles(hipStream_t s, const ParticleBufs& b, const ParticleMap& vmap,
+hipError_t LaunchCollideParticles(hipStream_t s, const ParticleBufs& b, const DevMap& vmap,
                                   float4* x, const float4* x0, float mu, float relax,
                                   hipEvent_t ev_begin, hipEvent_t ev_end) {
   if (vmap.blocks <= 0) return hipSuccess;
@@ -308,7 +302,7 @@ hipError_t LaunchCollideParticles(hipStream_t s, const ParticleBufs& b, const Pa
   return hipGetLastError();
 }
 
-hipError_t LaunchCountNeighbors(hipStream_t s, const ParticleBufs& b, const ParticleMap& vmap,
+hipError_t LaunchCountNeighbors(hipStream_t s, const ParticleBufs& b, const DevMap& vmap,
                                 unsigned long long* total) {
   if (vmap.blocks <= 0) return hipSuccess;
   ParticleCountKernel<<<XcdGrid(vmap.blocks), kBlock, 0, s>>>(b, vmap.off, vmap.n, total);

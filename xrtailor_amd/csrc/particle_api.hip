@@ -35,8 +35,8 @@ ParticleBufs Bufs(xrt_ctx* c) {
   return b;
 }
 
-ParticleMap Map(const ParticleData& P, const ParticleData::MapRef& r) {
-  ParticleMap m;
+DevMap Map(const ParticleData& P, const MapRef& r) {
+  DevMap m;
   m.off = P.maps.ptr + r.off;
   m.n = r.n;
   m.blocks = r.blocks;
@@ -50,22 +50,6 @@ float Diameter(const HostInstance& h, float factor) {
   const float* b = &h.rest_pos[3 * static_cast<size_t>(h.tris[1])];
   const float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
   return std::sqrt(dx * dx + dy * dy + dz * dz) * factor;
-}
-
-// block map of the instances [k0, k1) with per-instance block counts `blocks`: the
-// n + 1 prefix sums, the entry (0..n-1) of every block, the instance of every entry
-ParticleData::MapRef AppendMap(std::vector<int32_t>& maps, int k0, int k1,
-                               const std::vector<int>& blocks) {
-  ParticleData::MapRef r;
-  r.off = maps.size();
-  r.n = k1 - k0;
-  std::vector<int32_t> o(r.n + 1, 0);
-  for (int j = 0; j < r.n; ++j) o[j + 1] = o[j] + blocks[k0 + j];
-  r.blocks = o[r.n];
-  maps.insert(maps.end(), o.begin(), o.end());
-  for (int j = 0; j < r.n; ++j) maps.insert(maps.end(), static_cast<size_t>(blocks[k0 + j]), j);
-  for (int j = 0; j < r.n; ++j) maps.push_back(k0 + j);
-  return r;
 }
 
 int CheckParticles(xrt_ctx* c, int32_t inst) {
@@ -90,8 +74,8 @@ void NextPair(std::vector<hipEvent_t>& ev, size_t& next, hipEvent_t& a, hipEvent
 
 int Hash(xrt_ctx* c, int inst) {
   ParticleData& P = c->particles;
-  const ParticleMap vm = Map(P, inst < 0 ? P.all_v : P.one_v[inst]);
-  const ParticleMap tm = Map(P, inst < 0 ? P.all_t : P.one_t[inst]);
+  const DevMap vm = Map(P, inst < 0 ? P.all_v : P.one_v[inst]);
+  const DevMap tm = Map(P, inst < 0 ? P.all_t : P.one_t[inst]);
   hipEvent_t e0, e1;
   NextPair(P.hash_events, P.hash_next, e0, e1);
   if (e0) XRT_CHECK_HIP(hipEventRecord(e0, c->stream));
@@ -102,7 +86,7 @@ int Hash(xrt_ctx* c, int inst) {
 
 int Collide(xrt_ctx* c, int inst) {
   ParticleData& P = c->particles;
-  const ParticleMap vm = Map(P, inst < 0 ? P.all_v : P.one_v[inst]);
+  const DevMap vm = Map(P, inst < 0 ? P.all_v : P.one_v[inst]);
   hipEvent_t e0, e1;
   NextPair(P.collide_events, P.collide_next, e0, e1);
   XRT_CHECK_HIP(LaunchCollideParticles(c->stream, Bufs(c), vm, c->x.ptr, c->x0.ptr,
@@ -142,13 +126,13 @@ int ParticleStart(xrt_ctx* ctx) {
     tb[k] = std::max(1, (nv + kSortTile - 1) / kSortTile);
   }
   std::vector<int32_t> maps;
-  P.all_v = AppendMap(maps, 0, n, vb);
-  P.all_t = AppendMap(maps, 0, n, tb);
+  P.all_v = AppendBlockMap(maps, 0, n, vb);
+  P.all_t = AppendBlockMap(maps, 0, n, tb);
   P.one_v.resize(n);
   P.one_t.resize(n);
   for (int k = 0; k < n; ++k) {
-    P.one_v[k] = AppendMap(maps, k, k + 1, vb);
-    P.one_t[k] = AppendMap(maps, k, k + 1, tb);
+    P.one_v[k] = AppendBlockMap(maps, k, k + 1, vb);
+    P.one_t[k] = AppendBlockMap(maps, k, k + 1, tb);
   }
   XRT_CHECK_HIP(P.maps.Upload(maps, ctx->stream));
   XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));

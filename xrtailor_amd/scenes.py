@@ -155,6 +155,56 @@ class ChainBody:
         a = self.amp * (np.sin(2.0 * np.pi * self.freq * t + self.phase) - np.sin(self.phase))
         return a.sum(-1)
 
+    def _kinematics(self, t: float):
+        """Per joint: rotation and (rest origin, current origin) at time t."""
+        nj = self.amp.shape[0]
+        ang = self.angles(t)
+        rot = np.eye(3)
+        rots, origins = [], []
+        p_rest = np.array([0.0, self.y0, 0.0])
+        p_now = p_rest.copy()
+        for j in range(nj):
+            ax, az = ang[j]
+            cx, sx, cz, sz = np.cos(ax), np.sin(ax), np.cos(az), np.sin(az)
+            rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+            rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+            rot = rot @ rx @ rz
+            rots.append(rot.copy())
+            origins.append((p_rest.copy(), p_now.copy()))
+            step = np.array([0.0, self.seg, 0.0])
+            p_rest = p_rest + step
+            p_now = p_now + rot @ step
+        return rots, origins
+
+    def skinning(self):
+        """Sparse LBS weights of the rest vertices as CSR (ptr, joint, weight): the blend
+        of at()."""
+        nj = self.amp.shape[0]
+        u = np.clip((self.pos[:, 1] - self.y0) / self.seg, 0.0, nj - 1e-9)
+        s = np.floor(u).astype(int)
+        f = u - s
+        w_lo = np.where((f < 0.5) & (s > 0), 0.5 - f, 0.0)
+        ptr, joint, weight = [0], [], []
+        for v in range(len(self.pos)):
+            if w_lo[v] > 0.0:
+                joint += [s[v] - 1, s[v]]
+                weight += [w_lo[v], 1.0 - w_lo[v]]
+            else:
+                joint.append(s[v])
+                weight.append(1.0)
+            ptr.append(len(joint))
+        return (np.asarray(ptr, np.int32), np.asarray(joint, np.int32),
+                np.asarray(weight, np.float32))
+
+    def transforms(self, t: float) -> np.ndarray:
+        """Joint transforms at time t, (joints, 12) float32 row-major 3x4 [R | r1 - R r0]:
+        the skinned position of a vertex is sum_k w_k (R_k p + t_k)."""
+        rots, origins = self._kinematics(t)
+        out = np.zeros((len(rots), 12))
+        for k, (R, (r0, r1)) in enumerate(zip(rots, origins)):
+            out[k] = np.concatenate([R, (r1 - R @ r0)[:, None]], axis=1).reshape(-1)
+        return out.astype(np.float32)
+
     def at(self, t: float) -> np.ndarray:
         """Skinned vertex positions at time t (float32)."""
         nj = self.amp.shape[0]

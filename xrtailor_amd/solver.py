@@ -35,9 +35,10 @@ def make_params(**kw) -> XrtParams:
 
 
 def build_solver(specs, params: XrtParams, seed: int = 0, device: int = 0,
-                 obstacles: bool = True):
+                 obstacles: bool = True, before_start=None):
     """A started ClothSolver with one instance per scenes.ClothSpec (pins, LRA slots, the
-    obstacle when `obstacles` and the jittered initial state applied).
+    obstacle when `obstacles` and the jittered initial state applied; `before_start(solver,
+    instance ids)` runs last before Start, e.g. to add bindings).
     Returns (solver, instance ids)."""
     s = ClothSolver(device=device, params=params)
     insts = []
@@ -51,6 +52,8 @@ def build_solver(specs, params: XrtParams, seed: int = 0, device: int = 0,
             s.SetObstacle(i, *spec.obstacle)
         s.SetState(i, spec.x0)
         insts.append(i)
+    if before_start is not None:
+        before_start(s, insts)
     s.Start(seed)
     return s, insts
 
@@ -138,6 +141,7 @@ class ClothSolver:
         check(self.lib.xrt_create(C.byref(ctx), device))
         self.ctx = ctx
         self.params = params if params is not None else default_params()
+        self._nbind = {}
         check(self.lib.xrt_set_params(self.ctx, C.byref(self.params)))
 
     # -------------------------------------------------------------- lifetime
@@ -189,6 +193,57 @@ class ClothSolver:
         tris = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1)
         check(self.lib.xrt_set_obstacle(self.ctx, inst, fptr(pos), pos.size // 3, iptr(tris),
                                         tris.size // 3))
+
+    def AddBindings(self, inst: int, idx, stiffness, distance) -> None:
+        """BindingConstraint::Add (before Start): cloth vertices bound to the obstacle."""
+        idx = np.ascontiguousarray(idx, np.int32)
+        st = np.ascontiguousarray(np.broadcast_to(stiffness, idx.shape), np.float32)
+        di = np.ascontiguousarray(np.broadcast_to(distance, idx.shape), np.float32)
+        check(self.lib.xrt_add_bindings(self.ctx, inst, iptr(idx), fptr(st), fptr(di), len(idx)))
+        self._nbind[inst] = self._nbind.get(inst, 0) + len(idx)
+
+    def SetObstacleSkinning(self, inst: int, ptr, joint, weight, n_joints: int) -> None:
+        """Sparse LBS weights of the obstacle's rest vertices (before Start)."""
+        ptr = np.ascontiguousarray(ptr, np.int32)
+        joint = np.ascontiguousarray(joint, np.int32)
+        weight = np.ascontiguousarray(weight, np.float32)
+        check(self.lib.xrt_set_obstacle_skinning(self.ctx, inst, iptr(ptr), iptr(joint),
+                                                 fptr(weight), n_joints))
+
+    def SkinObstacles(self, transforms) -> None:
+        """LBS of every skinned obstacle: transforms of all of them, instance order."""
+        t = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float32).reshape(-1)
+                                                 for a in transforms]), np.float32)
+        check(self.lib.xrt_skin_obstacles(self.ctx, fptr(t)))
+
+    def SkinObstacle(self, inst: int, transforms) -> None:
+        t = np.ascontiguousarray(transforms, np.float32).reshape(-1)
+        check(self.lib.xrt_skin_obstacle(self.ctx, inst, fptr(t)))
+
+    def QueryNearestBarycentric(self, inst: int):
+        """(idx (nv, 4): face vertices u, v, w and face; uvw (nv, 4): weights, distance)."""
+        nv = self.counts(inst).nv
+        idx = np.zeros(4 * nv, np.int32)
+        uvw = np.zeros(4 * nv, np.float32)
+        check(self.lib.xrt_query_nearest_barycentric(self.ctx, inst, iptr(idx), fptr(uvw)))
+        return idx.reshape(-1, 4), uvw.reshape(-1, 4)
+
+    def Bindings(self, inst: int):
+        """(idx (nb, 4): cloth vertex, body u, v, w; par (nb, 5): u, v, w, L, stiffness)."""
+        nb = self._nbind.get(inst, 0)
+        idx = np.zeros(4 * max(nb, 1), np.int32)
+        par = np.zeros(5 * max(nb, 1), np.float32)
+        check(self.lib.xrt_get_bindings(self.ctx, inst, iptr(idx), fptr(par)))
+        return idx[: 4 * nb].reshape(-1, 4), par[: 5 * nb].reshape(-1, 5)
+
+    def SolveBindings(self, inst: int) -> None:
+        check(self.lib.xrt_solve_bindings(self.ctx, inst))
+
+    def GetObstacle(self, inst: int, nv: int):
+        x0 = np.zeros(3 * nv, np.float32)
+        x = np.zeros(3 * nv, np.float32)
+        check(self.lib.xrt_get_obstacle(self.ctx, inst, fptr(x0), fptr(x)))
+        return x0.reshape(-1, 3), x.reshape(-1, 3)
 
     def UpdateObstacle(self, inst: int, pos: np.ndarray) -> None:
         pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
