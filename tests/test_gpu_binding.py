@@ -65,12 +65,24 @@ def test_lbs_matches_oracle():
     s.close()
 
 
-@pytest.mark.parametrize("tol", [0.003, 0.0])
-def test_nearest_barycentric_matches_oracle(tol):
+def _wrap(body, off=0.02):
+    """A cloth that is the body mesh pushed out along its vertex normals: every cloth
+    vertex's nearest body point is a body vertex, shared by all its faces (exact ties)."""
+    p = np.asarray(body.pos, np.float32)
+    n = orc.vertex_normals(p, body.tris)
+    pos = (p + np.float32(off) * n).astype(np.float64)
+    return scenes.ClothSpec(pos, np.asarray(body.tris, np.int32), np.zeros(0, np.int32), 1e-8,
+                            1e-4, pos.astype(np.float32), (body.pos, body.tris))
+
+
+@pytest.mark.parametrize("scene,tol", [("garment", 0.003), ("garment", 0.0), ("wrap", 0.003)])
+def test_nearest_barycentric_matches_oracle(scene, tol):
     """Face, weights and distance bit for bit from the same FP32 inputs (ties between
     faces sharing the nearest vertex or edge resolved in the same traversal order)."""
-    print(f"case nearest barycentric, bvh tolerance {tol}", flush=True)
+    print(f"case nearest barycentric, {scene}, bvh tolerance {tol}", flush=True)
     spec, body, *_ = _garment()
+    if scene == "wrap":
+        spec = _wrap(body)
     s, (i,) = build_solver([spec], _xp(bvh_tolerance=tol), seed=1)
     idx, uvw = s.QueryNearestBarycentric(i)
     ox0 = np.asarray(body.pos, np.float32)
@@ -84,6 +96,8 @@ def test_nearest_barycentric_matches_oracle(tol):
     corner = (uvw[:, :3] == 1.0).any(axis=1).sum()
     print(f"  {len(faces)} cloth vertices, {corner} nearest at a body vertex, "
           f"distance {do.min():.4f}..{do.max():.4f} m", flush=True)
+    if scene == "wrap":
+        assert corner == len(faces)
     s.close()
 
 
@@ -99,7 +113,8 @@ def test_binding_generate_and_solve_match_oracle():
     np.testing.assert_array_equal(bidx[:, 0], bind)
     np.testing.assert_array_equal(bidx[:, 1:], twin.b_tri3)
     np.testing.assert_array_equal(par[:, :3], twin.b_uvw.astype(np.float32))
-    np.testing.assert_allclose(par[:, 3], twin.b_L, rtol=1e-6)
+    # L = |x0 - body point| from FP32 positions of magnitude ~0.5 m: a few ulps (6e-8 m)
+    np.testing.assert_allclose(par[:, 3], twin.b_L, rtol=0, atol=2e-7)
     np.testing.assert_array_equal(par[:, 4], stiff)
     # move the body and solve once from the current state
     s.SkinObstacle(i, body.transforms(0.25))
