@@ -26,6 +26,11 @@ cpu_baseline the serial FP64 oracle on one core, one substep per instance over a
                      self-collision of the reference's swift mode instead of PBD repulsion;
                      roofline kernel ParticleCollideKernel, algorithmic bytes per launch
                      48 N + 4 (neighbor entries + N))
+                    (garment: 64 tube garments on capsule-chain bodies skinned on the GPU
+                     every frame (LBS from joint transforms), bindings at the top ring and
+                     waist, LRA, body contact; a step is the skinning plus one frame, timed
+                     by the host clock; roofline kernel LbsKernel, algorithmic bytes 36 per
+                     body vertex + 8 per weight entry + 48 per joint)
                     [--no-graph]   (un-captured substeps, for profiler runs)
     python bench.py --copy-calibration   (5 device copies of 256 MiB: the known-size kernel
                                           that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)
@@ -70,7 +75,8 @@ def algorithmic_bytes(solver, insts) -> tuple[int, int]:
     return total, nv
 
 
-def cpu_baseline_and_parity(specs, params, seed, contacts, budget_s=12.0):
+def cpu_baseline_and_parity(specs, params, seed, contacts, budget_s=12.0, before_start=None,
+                            twin_setup=None):
     """CPU baseline leg: the serial FP64 oracle (1 core) advances each instance by one
     substep from the initial state (with its obstacle and self contact when `contacts`)
     until the time budget is spent; the GPU's batched substep from the same state is
@@ -83,7 +89,8 @@ def cpu_baseline_and_parity(specs, params, seed, contacts, budget_s=12.0):
     from oracle import scene as osc
     from xrtailor_amd.solver import build_solver
 
-    solver, insts = build_solver(specs, params, seed=seed, obstacles=contacts)
+    solver, insts = build_solver(specs, params, seed=seed, obstacles=contacts,
+                                 before_start=before_start)
     solver.SubstepBatched()
     solver.sync()
     op, dt = orc.params_from(params)
@@ -96,6 +103,8 @@ def cpu_baseline_and_parity(specs, params, seed, contacts, budget_s=12.0):
             obs = None if sp.obstacle is None else (np.asarray(sp.obstacle[0], np.float32),
                                                     sp.obstacle[1])
             twin = osc.ContactScene(oc, sp.pos, sp.tris, edges, op, params, obstacle=obs)
+            if twin_setup is not None:
+                twin_setup(n_inst, twin)
             t0 = time.perf_counter()
             twin.substep()
         else:
@@ -170,7 +179,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="batched", choices=["batched", "swift", "hanging"])
+    ap.add_argument("--workload", default="batched",
+                    choices=["batched", "swift", "garment", "hanging"])
+    ap.add_argument("--garments", type=int, default=64, help="instances of --workload garment")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--plain-frames", type=int, default=1)
@@ -200,33 +211,72 @@ def main() -> int:
                                    "N GPUs would each run their own share): use --gpus 1"}))
         return 2
 
+    import numpy as np
+
     from xrtailor_amd import scenes
     from xrtailor_amd.solver import build_solver, make_params
 
-    if a.workload in ("batched", "swift"):
+    gs, before = None, None
+    if a.workload == "garment":
+        gs = scenes.garments(seed=a.seed, n_instances=a.garments)
+        specs = [g[0] for g in gs]
+        wname = (f"garment {a.garments} tubes (ring 64-128 x rows 40-80) on capsule-chain bodies "
+                 f"(10,242 vertices, 12 joints) skinned on the GPU each frame, top ring and waist "
+                 f"bound to the body, LRA, body contact, seed {a.seed}")
+
+        def before(s, insts):
+            for i, g in zip(insts, gs):
+                s.SetObstacleSkinning(i, *g[1].skinning(), g[1].amp.shape[0])
+                s.AddBindings(i, g[2], g[3], g[4])
+    elif a.workload in ("batched", "swift"):
         specs = scenes.batched(seed=a.seed, n_instances=a.instances)
         wname = f"batched {a.instances} cloths, side round(32*6^u) in [32,192], seed {a.seed}"
     else:
         specs = [scenes.hanging_cloth(seed=a.seed)]
         wname = "hanging cloth 64x64, 1 m, two corners pinned"
     swift = a.workload == "swift"
-    contacts = a.workload in ("batched", "swift") and not a.no_contacts
+    contacts = a.workload in ("batched", "swift", "garment") and not a.no_contacts
+    garment = a.workload == "garment"
     params = make_params(bend_model=1, ground_height=-1.0, enable_obstacle_contact=int(contacts),
-                         enable_self_contact=int(contacts and not swift),
+                         enable_self_contact=int(contacts and not swift and not garment),
                          enable_particle_collision=int(swift))
     if swift:
         wname += (", sphere / capsule obstacle" if contacts else "") + \
             ", particle self-collision (swift mode)"
-    elif contacts:
+    elif contacts and not garment:
         wname += ", sphere / capsule obstacle and self contact"
     print(f"[bench] building {len(specs)} instances", file=sys.stderr, flush=True)
-    solver, insts = build_solver(specs, params, seed=a.seed, obstacles=contacts)
+    solver, insts = build_solver(specs, params, seed=a.seed, obstacles=contacts,
+                                 before_start=before)
     bytes_sub, n_verts = algorithmic_bytes(solver, insts)
+    # garment: a step skins every body (joint transforms of frame f, uploaded) then runs the
+    # frame; the transforms of every frame are prepared before the timed region
+    T = None
+    if garment:
+        nf = a.warmup + a.steps + 2 + a.plain_frames
+        T = [np.concatenate([g[1].transforms((f + 1) / 60.0).reshape(-1) for g in gs])
+             for f in range(nf)]
     info = solver.device_info()
 
     # batched: warmup, then K timed frames (graph replays), HIP events on the solver stream
     t0 = time.perf_counter()
-    if a.no_graph:
+    if garment:
+        def frame(f):
+            solver.SkinObstacles([T[f]])
+            if a.no_graph:
+                for _ in range(params.num_substeps):
+                    solver.SubstepBatched()
+            else:
+                solver.FramesBatched(1)
+        for f in range(a.warmup):
+            frame(f)
+        solver.sync()
+        t0 = time.perf_counter()
+        for f in range(a.warmup, a.warmup + a.steps):
+            frame(f)
+        solver.sync()
+        ms = (time.perf_counter() - t0) * 1e3
+    elif a.no_graph:
         for _ in range(a.warmup * params.num_substeps):
             solver.SubstepBatched()
         solver.sync()
@@ -244,8 +294,22 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     # dominant kernel: per-launch device time from HIP events around each launch
-    rep_counts, particle = None, None
-    if swift:
+    rep_counts, particle, binding = None, None, None
+    if garment:
+        kernel = "LbsKernel"
+        kern_ms, bind_ms = solver.time_binding(20)
+        bodies = sum(len(g[1].pos) for g in gs)
+        entries = sum(len(g[1].skinning()[1]) for g in gs)
+        joints = sum(g[1].amp.shape[0] for g in gs)
+        # rest point read, skinned point written, CSR offset (36 B per body vertex), joint and
+        # weight (8 B per entry), the 3x4 transform (48 B per joint)
+        bytes_kern = 36 * bodies + 8 * entries + 48 * joints
+        nb = sum(len(g[2]) for g in gs)
+        binding = {"lbs_ms": kern_ms, "body_vertices": bodies, "weight_entries": entries,
+                   "joints": joints, "binding_solve_ms": bind_ms, "bindings": nb,
+                   # item (48 B), cloth vertex read + written (32 B), three body points (48 B)
+                   "binding_solve_gbs": 128 * nb / (bind_ms * 1e-3) / 1e9}
+    elif swift:
         kernel = "ParticleCollideKernel"
         kern_ms, hash_ms, pc = solver.time_particles(2 * params.num_substeps)
         # x, x0 read, delta written (48 B per particle), list entries and sentinels (4 B)
@@ -267,7 +331,21 @@ def main() -> int:
 
     # plain path on the same workload
     plain = None
-    if not a.no_plain:
+    if not a.no_plain and garment:
+        f0 = a.warmup + a.steps + 1
+        solver.SkinObstacles([T[f0]])
+        solver.FramesPlain(1)
+        solver.sync()
+        t1 = time.perf_counter()
+        for f in range(a.plain_frames):
+            solver.SkinObstacles([T[f0 + 1 + f]])
+            solver.FramesPlain(1)
+        solver.sync()
+        pms = (time.perf_counter() - t1) * 1e3
+        plain = n_verts * params.num_substeps * a.plain_frames / (pms * 1e-3)
+        print(f"[bench] plain {pms:.1f} ms / {a.plain_frames} frame(s)", file=sys.stderr,
+              flush=True)
+    elif not a.no_plain:
         solver.FramesPlain(1)
         solver.sync()
         pms = solver.time_frames(False, a.plain_frames)
@@ -278,8 +356,12 @@ def main() -> int:
 
     cpu, parity, parity_n = None, None, 0
     if not a.no_cpu:
+        tw_setup = None
+        if garment:
+            def tw_setup(k, twin):
+                twin.setup_bindings(gs[k][2], gs[k][3], gs[k][4], params.bvh_tolerance)
         cpu, parity, parity_n = cpu_baseline_and_parity(specs, params, a.seed, contacts,
-                                                        a.cpu_budget)
+                                                        a.cpu_budget, before, tw_setup)
         print(f"[bench] cpu {cpu['value']:.3e} vs/s, parity {parity:.3e}", file=sys.stderr,
               flush=True)
 
@@ -306,7 +388,9 @@ def main() -> int:
         "config": {"workload": wname, "instances": len(specs), "vertices": n_verts,
                    "substeps": params.num_substeps, "iterations": params.num_iterations,
                    "bend_model": "consistent", "obstacle_contact": contacts,
-                   "self_contact": contacts and not swift, "particle_collision": swift,
+                   "self_contact": contacts and not swift and not garment,
+                   "particle_collision": swift, "bindings": garment,
+                   "skinning": "GPU LBS every frame" if garment else None,
                    "step": "one frame of all instances",
                    "graph": not a.no_graph},
         "plain_value": plain,
@@ -316,7 +400,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel, "kernel_ms": kern_ms, "bytes_per_launch": bytes_kern,
-                     "repulsion_counts": rep_counts, "particle": particle},
+                     "repulsion_counts": rep_counts, "particle": particle,
+                     "binding": binding},
         "cpu_baseline": cpu,
         "adjoint_forward_ratio": adj["substep_ratio"] if adj else None,
         "adjoint": adj,

@@ -225,6 +225,11 @@ int xrt_get_bindings(xrt_ctx* ctx, int32_t inst, int32_t* idx, float* par);
 int xrt_solve_bindings(xrt_ctx* ctx, int32_t inst);
 /* obstacle positions: start of the frame (x0) and the frame's target (x); either may be null */
 int xrt_get_obstacle(xrt_ctx* ctx, int32_t inst, float* x0, float* x);
+/* timing helper: `n` launches of the skinning of every skinned obstacle (the last uploaded
+ * transforms) and of the binding solve of every instance, bracketed by HIP events;
+ * ms[0] / ms[1] = average device time per launch.  The cloth state is moved by the solves.
+ * No counterpart in the reference (measurement hook). */
+int xrt_time_binding(xrt_ctx* ctx, int32_t n, float ms[2]);
 
 /* SpatialHashGPU::Hash(nodes, n) of one instance's predicted positions x
  * (xrtailor/include/xrtailor/physics/broad_phase/spatial_hash/SpatialHashGPU.cuh:26-28):

@@ -62,6 +62,11 @@ def test_lbs_matches_oracle():
     s.SkinObstacle(insts[1], cases[1][1].transforms(0.6))
     _, x = s.GetObstacle(insts[1], len(cases[1][1].pos))
     assert rel_err(x, cases[1][1].at(0.6)) <= 1e-6
+    # the timing hook re-skins from the uploaded transforms: the result does not change
+    lbs_ms, bind_ms = s.time_binding(3)
+    assert lbs_ms > 0 and bind_ms >= 0
+    _, x2 = s.GetObstacle(insts[1], len(cases[1][1].pos))
+    np.testing.assert_array_equal(x2, x)
     s.close()
 
 

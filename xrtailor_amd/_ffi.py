@@ -114,6 +114,7 @@ SIGNATURES = {
     "xrt_get_bindings": [_P, _I, _IP, _FP],
     "xrt_solve_bindings": [_P, _I],
     "xrt_get_obstacle": [_P, _I, _FP, _FP],
+    "xrt_time_binding": [_P, _I, _FP],
     "xrt_particle_hash": [_P, _I],
     "xrt_collide_particles": [_P, _I],
     "xrt_get_particle_hash": [_P, _I, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _IP,

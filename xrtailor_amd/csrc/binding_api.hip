@@ -165,7 +165,11 @@ int BindingStart(xrt_ctx* ctx) {
     XRT_CHECK_HIP(B.sk_joint.Upload(joint, s));
     XRT_CHECK_HIP(B.sk_weight.Upload(weight, s));
     XRT_CHECK_HIP(B.sk_rest.Upload(rest, s));
-    XRT_CHECK_HIP(B.transforms.Alloc(12 * static_cast<size_t>(B.total_joints)));
+    {  // identity transforms until the first skinning call (the rest pose)
+      std::vector<float> eye(12 * static_cast<size_t>(B.total_joints), 0.0f);
+      for (int j = 0; j < B.total_joints; ++j) eye[12 * j] = eye[12 * j + 5] = eye[12 * j + 10] = 1.0f;
+      XRT_CHECK_HIP(B.transforms.Upload(eye, s));
+    }
     B.skin.assign(n, SkinInstDev{});
     for (int k = 0; k < n; ++k) {
       const HostInstance& h = ctx->inst[k];
@@ -312,6 +316,42 @@ int xrt_solve_bindings(xrt_ctx* ctx, int32_t inst) {
   int rc = CheckInstance(ctx, inst, true);
   if (rc != XRT_OK) return rc;
   return BindingIter(ctx, inst);
+}
+
+int xrt_time_binding(xrt_ctx* ctx, int32_t n, float ms[2]) {
+  XRT_REQUIRE(ctx && ctx->started, XRT_ERR_STATE, "xrt_start has not been called");
+  XRT_REQUIRE(n > 0 && ms, XRT_ERR_INVALID_ARG, "n must be > 0, output non-null");
+  BindingData& B = ctx->binding;
+  hipStream_t s = ctx->stream;
+  std::vector<hipEvent_t> ev(4 * static_cast<size_t>(n), nullptr);
+  int rc = XRT_OK;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) rc = XRT_ERR_HIP;
+  for (int k = 0; k < n && rc == XRT_OK; ++k) {
+    if (hipEventRecord(ev[4 * k], s) != hipSuccess ||
+        LaunchLbs(s, B.skin_dev.ptr, B.transforms.ptr, Map(B.skin_maps, B.skin_all)) != hipSuccess ||
+        hipEventRecord(ev[4 * k + 1], s) != hipSuccess || hipEventRecord(ev[4 * k + 2], s) != hipSuccess)
+      rc = XRT_ERR_HIP;
+    if (rc == XRT_OK && BindingIter(ctx, -1) != XRT_OK) rc = XRT_ERR_HIP;
+    if (rc == XRT_OK && hipEventRecord(ev[4 * k + 3], s) != hipSuccess) rc = XRT_ERR_HIP;
+  }
+  if (rc == XRT_OK && hipStreamSynchronize(s) != hipSuccess) rc = XRT_ERR_HIP;
+  float tot[2] = {0.0f, 0.0f};
+  for (int k = 0; k < n && rc == XRT_OK; ++k)
+    for (int j = 0; j < 2; ++j) {
+      float t = 0.0f;
+      if (hipEventElapsedTime(&t, ev[4 * k + 2 * j], ev[4 * k + 2 * j + 1]) != hipSuccess) rc = XRT_ERR_HIP;
+      tot[j] += t;
+    }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (rc != XRT_OK) {
+    SetError("HIP error while timing the skinning / binding kernels");
+    return rc;
+  }
+  ms[0] = tot[0] / static_cast<float>(n);
+  ms[1] = tot[1] / static_cast<float>(n);
+  return XRT_OK;
 }
 
 int xrt_get_obstacle(xrt_ctx* ctx, int32_t inst, float* x0, float* x) {

@@ -310,6 +310,30 @@ def garment(seed: int = 6, ring: int = 160, rows: int = 125, joints: int = 24,
     return spec, body
 
 
+def garments(seed: int = 4, n_instances: int = 64, ring_lo: int = 64, ring_hi: int = 128,
+             rows_lo: int = 40, rows_hi: int = 80, joints: int = 12, body_subdiv: int = 5):
+    """Garment instances for the pipeline around the substep: tube garments of uneven size
+    (ring and rows uniform in the given ranges) around capsule-chain bodies of 10,242
+    vertices (subdivision 5) skinned by `joints` joints.  Each garment's top ring (its LRA
+    slots) is bound to the body with stiffness 1 and its middle row with stiffness 0.5,
+    both at their distance at the start.  Returns (spec, body, bound vertices, stiffness,
+    distance) per instance."""
+    rng = Rng(seed)
+    rings = np.rint(ring_lo + (ring_hi - ring_lo) * rng.uniform(n_instances)).astype(int)
+    rows = np.rint(rows_lo + (rows_hi - rows_lo) * rng.uniform(n_instances)).astype(int)
+    out = []
+    for i in range(n_instances):
+        r, h = int(rings[i]), int(rows[i])
+        spec, body = garment(seed=seed * 7919 + i, ring=r, rows=h, joints=joints,
+                             body_subdiv=body_subdiv)
+        top = np.arange((h - 1) * r, h * r, dtype=np.int32)
+        mid = np.arange((h // 2) * r, (h // 2 + 1) * r, dtype=np.int32)
+        bind = np.concatenate([top, mid])
+        stiff = np.concatenate([np.ones(r), np.full(r, 0.5)]).astype(np.float32)
+        out.append((spec, body, bind, stiff, np.zeros(2 * r, np.float32)))
+    return out
+
+
 def batched(seed: int = 4, n_instances: int = 256, side_lo: int = 32, side_hi: int = 192,
             cloth_side: float = 1.0, jitter: float = 1e-3) -> list[ClothSpec]:
     """Independent instances: grid side round(lo*(hi/lo)^u) (log-uniform), stretch

@@ -442,3 +442,10 @@ class ClothSolver:
         check(self.lib.xrt_time_particles(self.ctx, n, fptr(ms),
                                           cnt.ctypes.data_as(C.POINTER(C.c_int64))))
         return float(ms[0]), float(ms[1]), {"entries": int(cnt[0]), "particles": int(cnt[1])}
+
+    def time_binding(self, n: int) -> tuple[float, float]:
+        """ms per launch of the obstacle skinning (every skinned obstacle) and of the
+        binding solve (every instance), over n launches each (the state moves)."""
+        ms = np.zeros(2, np.float32)
+        check(self.lib.xrt_time_binding(self.ctx, n, fptr(ms)))
+        return float(ms[0]), float(ms[1])
