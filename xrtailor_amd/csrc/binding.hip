@@ -97,6 +97,10 @@ __device__ float TriNearest(float4 p, float4 a, float4 b, float4 c, float& u, fl
 // right and pushed in that order; a popped node whose squared min distance exceeds the best
 // (unsquared) distance is skipped; a child is kept when its min distance <= its sibling's
 // min-max distance; a leaf replaces the best when its distance is <= it.
+// A popped node is also skipped when its squared min distance exceeds best^2 (1 + 1e-5):
+// every face under it is then strictly farther than the best, so it could neither replace
+// it nor tie with it, and the result is the reference's; below 1 m this prunes what the
+// mixed squared / unsquared test keeps.
 __global__ __launch_bounds__(kBlock) void NearestBaryKernel(BvhView t, const float4* __restrict__ q,
                                                             int n, int4* __restrict__ out_idx,
                                                             float4* __restrict__ out_uvw,
@@ -114,7 +118,7 @@ __global__ __launch_bounds__(kBlock) void NearestBaryKernel(BvhView t, const flo
   bool overflow = false;
   do {
     const int node = st_node[--sp];
-    if (st_dist[sp] > best) continue;
+    if (st_dist[sp] > best || st_dist[sp] > best * best * 1.00001f) continue;
     const int4 nd = t.nodes[node];
     const int kid[2] = {nd.y, nd.z};
     const Box kb0 = t.box[kid[0]], kb1 = t.box[kid[1]];

@@ -314,10 +314,10 @@ def garments(seed: int = 4, n_instances: int = 64, ring_lo: int = 64, ring_hi: i
              rows_lo: int = 40, rows_hi: int = 80, joints: int = 12, body_subdiv: int = 5):
     """Garment instances for the pipeline around the substep: tube garments of uneven size
     (ring and rows uniform in the given ranges) around capsule-chain bodies of 10,242
-    vertices (subdivision 5) skinned by `joints` joints.  Each garment's top ring (its LRA
-    slots) is bound to the body with stiffness 1 and its middle row with stiffness 0.5,
-    both at their distance at the start.  Returns (spec, body, bound vertices, stiffness,
-    distance) per instance."""
+    vertices (subdivision 5) skinned by `joints` joints.  Every 8th vertex of the top ring
+    is an LRA slot; the top ring is bound to the body with stiffness 1 and the middle row
+    with stiffness 0.5, both at their distance at the start.  Returns (spec, body, bound
+    vertices, stiffness, distance) per instance."""
     rng = Rng(seed)
     rings = np.rint(ring_lo + (ring_hi - ring_lo) * rng.uniform(n_instances)).astype(int)
     rows = np.rint(rows_lo + (rows_hi - rows_lo) * rng.uniform(n_instances)).astype(int)
@@ -326,6 +326,7 @@ def garments(seed: int = 4, n_instances: int = 64, ring_lo: int = 64, ring_hi: i
         r, h = int(rings[i]), int(rows[i])
         spec, body = garment(seed=seed * 7919 + i, ring=r, rows=h, joints=joints,
                              body_subdiv=body_subdiv)
+        spec.lra_slots = spec.lra_slots[::8].copy()
         top = np.arange((h - 1) * r, h * r, dtype=np.int32)
         mid = np.arange((h // 2) * r, (h // 2 + 1) * r, dtype=np.int32)
         bind = np.concatenate([top, mid])
