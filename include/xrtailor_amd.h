@@ -262,6 +262,22 @@ int xrt_substep_batched(xrt_ctx* ctx);
 /* `nframes` frames of all instances; the frame's substep launches are captured once in
  * a hipGraph and replayed */
 int xrt_frames_batched(xrt_ctx* ctx, int32_t nframes);
+/* `nframes` batched frames (as xrt_frames_batched) with the per-frame cache of
+ * ClothSolverUniversal::SaveCache (xrtailor/src/pipeline/impl/universal/ClothSolver.cpp:
+ * 107-144): after every frame the positions of every instance (instance order, float
+ * [nframes][N][3], N = all vertices) and, when `normals` is non-null, their vertex normals
+ * (UpdateNormals' formula, same layout) are copied out; the copy of a frame overlaps the
+ * next frame's computation. */
+int xrt_record_frames(xrt_ctx* ctx, int32_t nframes, float* positions, float* normals);
+/* SaveObj (xrtailor/src/runtime/export/ObjSequenceExporter.cpp:22-47): one mesh as OBJ text
+ * ("o mesh", v / vn lines with 6 decimals, "s 1", 1-based faces); normals may be null */
+int xrt_write_obj(const char* path, const float* pos, const float* normals, int32_t nv,
+                  const int32_t* tris, int32_t nt);
+/* ObjSequenceExporter::ExportObjSequence (ObjSequenceExporter.cpp:151-200): frames
+ * first_frame .. first_frame + nframes - 1 of one mesh as dir/%06d.obj; pos / normals hold
+ * nframes x nv x 3 floats */
+int xrt_write_obj_sequence(const char* dir, int32_t first_frame, int32_t nframes, const float* pos,
+                           const float* normals, int32_t nv, const int32_t* tris, int32_t nt);
 /* timing helper: average device time (ms) of one batched substep over `n` substeps,
  * bracketed by HIP events on the context stream */
 int xrt_time_batched_substep(xrt_ctx* ctx, int32_t n, float* avg_ms);

@@ -124,6 +124,9 @@ SIGNATURES = {
     "xrt_substep_batched": [_P],
     "xrt_frames_batched": [_P, _I],
     "xrt_time_batched_substep": [_P, _I, _FP],
+    "xrt_record_frames": [_P, _I, _FP, _FP],
+    "xrt_write_obj": [C.c_char_p, _FP, _FP, _I, _IP, _I],
+    "xrt_write_obj_sequence": [C.c_char_p, _I, _I, _FP, _FP, _I, _IP, _I],
     "xrt_time_frames": [_P, _I, _I, _FP],
     "xrt_adjoint_begin": [_P, _I, _I],
     "xrt_adjoint_seed": [_P, _I, _FP, _FP],

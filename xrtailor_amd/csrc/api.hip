@@ -282,6 +282,7 @@ int xrt_destroy(xrt_ctx* ctx) {
   if (!ctx) return XRT_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->record.Release();
   DestroyGraph(ctx);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);

@@ -146,6 +146,25 @@ struct BindingData {
   std::vector<MapRef> skin_one;
 };
 
+// Per-frame cache of xrt_record_frames (record_api.hip): two device slots packed after
+// each frame, copied on a second stream into two pinned host slots.
+struct RecordData {
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t packed[2] = {nullptr, nullptr}, copied[2] = {nullptr, nullptr};
+  bool fresh[2] = {true, true};  // slot never copied yet (nothing to wait for)
+  size_t slot_floats = 0;
+  DevBuf<float> dev;
+  float* host = nullptr;  // pinned, 2 slots
+  DevBuf<int4> tris;      // every instance's faces, global vertex ids (normals)
+  DevBuf<int32_t> vptr, corner;
+  DevBuf<float4> normals;
+  RecordData() = default;
+  RecordData(const RecordData&) = delete;
+  RecordData& operator=(const RecordData&) = delete;
+  void Release();
+  ~RecordData() { Release(); }
+};
+
 // Sizes of the wide launches (one color of every instance per launch).
 struct WideConfig {
   int n_total = 0, e_total = 0, b_total = 0;  // vertices / stretch / bend over instances
@@ -231,6 +250,8 @@ struct xrt_ctx {
   xrt::ParticleData particles;
   // obstacle skinning and cloth-to-body bindings
   xrt::BindingData binding;
+  // per-frame cache (xrt_record_frames)
+  xrt::RecordData record;
   // differentiable rollout (allocated by xrt_adjoint_begin)
   std::unique_ptr<xrt::AdjointData> adj;
   // graph of one batched frame

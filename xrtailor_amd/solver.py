@@ -101,6 +101,27 @@ class BendSolver:
         check(self._s.lib.xrt_bend_solve(self._s.ctx, self._i, color, dt, iteration))
 
 
+def write_obj(path: str, pos, tris, normals=None) -> None:
+    """SaveObj: one mesh as OBJ text (the C ABI's writer)."""
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1)
+    t = np.ascontiguousarray(tris, np.int32).reshape(-1)
+    nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1)
+    check(_ffi.load().xrt_write_obj(str(path).encode(), fptr(pos), fptr(nrm), pos.size // 3,
+                                    iptr(t), t.size // 3))
+
+
+def write_obj_sequence(directory: str, frames, tris, normals=None, first_frame: int = 0) -> None:
+    """ExportObjSequence: frames (n, nv, 3) of one mesh as directory/%06d.obj."""
+    fr = np.ascontiguousarray(frames, np.float32)
+    n, nv = fr.shape[0], fr.shape[1]
+    t = np.ascontiguousarray(tris, np.int32).reshape(-1)
+    nrm = None if normals is None else np.ascontiguousarray(normals, np.float32)
+    check(_ffi.load().xrt_write_obj_sequence(str(directory).encode(), first_frame, n,
+                                             fptr(fr.reshape(-1)),
+                                             None if nrm is None else fptr(nrm.reshape(-1)),
+                                             nv, iptr(t), t.size // 3))
+
+
 class SpatialHash:
     """SpatialHashGPU of one instance (xrtailor/include/xrtailor/physics/broad_phase/
     spatial_hash/SpatialHashGPU.cuh:19-35): Hash() over the predicted positions and its
@@ -449,3 +470,13 @@ class ClothSolver:
         ms = np.zeros(2, np.float32)
         check(self.lib.xrt_time_binding(self.ctx, n, fptr(ms)))
         return float(ms[0]), float(ms[1])
+
+    def RecordFrames(self, n: int, normals: bool = False):
+        """n batched frames with the per-frame cache: positions (n, N, 3) of every instance
+        in instance order (and their vertex normals when asked)."""
+        N = sum(self.counts(i).nv for i in range(self.num_instances()))
+        pos = np.zeros((n, N, 3), np.float32)
+        nrm = np.zeros((n, N, 3), np.float32) if normals else None
+        check(self.lib.xrt_record_frames(self.ctx, n, fptr(pos.reshape(-1)),
+                                         None if nrm is None else fptr(nrm.reshape(-1))))
+        return pos, nrm
