@@ -193,6 +193,9 @@ def main() -> int:
                     help="batched workload without its obstacles and self contact")
     ap.add_argument("--no-graph", action="store_true",
                     help="time un-captured batched substeps instead of graph replays")
+    ap.add_argument("--record", action="store_true",
+                    help="also time the same frames with the per-frame cache (positions and "
+                         "normals copied out every frame, copy overlapped with compute)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only 5 device copies of 256 MiB (PMC byte-counter calibration)")
     a = ap.parse_args()
@@ -292,6 +295,25 @@ def main() -> int:
     value = n_verts * params.num_substeps * a.steps / (ms * 1e-3)
     print(f"[bench] batched {ms:.2f} ms / {a.steps} frames (wall {wall:.3f} s)",
           file=sys.stderr, flush=True)
+
+    record = None
+    if a.record and not garment:
+        # the per-frame cache: positions + normals of every vertex, 24 B per vertex per frame
+        # bracketed by unrecorded frames just before and after (the cost per frame changes
+        # as the simulation advances)
+        solver.RecordFrames(1, normals=True)
+        before = solver.time_frames(True, a.steps)
+        t1 = time.perf_counter()
+        solver.RecordFrames(a.steps, normals=True)
+        rms = (time.perf_counter() - t1) * 1e3
+        after = solver.time_frames(True, a.steps)
+        base = 0.5 * (before + after)
+        record = {"ms_per_frame": rms / a.steps, "frame_ms_before": before / a.steps,
+                  "frame_ms_after": after / a.steps, "overhead": rms / base - 1.0,
+                  "bytes_per_frame": 24 * n_verts,
+                  "d2h_gbs": 24 * n_verts * a.steps / (rms * 1e-3) / 1e9}
+        print(f"[bench] recorded frames {rms / a.steps:.1f} ms/frame vs {before / a.steps:.1f} "
+              f"before and {after / a.steps:.1f} after", file=sys.stderr, flush=True)
 
     # dominant kernel: per-launch device time from HIP events around each launch
     rep_counts, particle, binding = None, None, None
@@ -402,6 +424,7 @@ def main() -> int:
                      "kernel": kernel, "kernel_ms": kern_ms, "bytes_per_launch": bytes_kern,
                      "repulsion_counts": rep_counts, "particle": particle,
                      "binding": binding},
+        "record": record,
         "cpu_baseline": cpu,
         "adjoint_forward_ratio": adj["substep_ratio"] if adj else None,
         "adjoint": adj,

@@ -6,12 +6,16 @@
 // the frame.  Here each frame's positions (and the vertex normals of every cloth, the
 // reference's UpdateNormals formula over one vertex -> corner table of all instances) are
 // packed to float3 in one of two device slots; a second stream (default priority) copies the
-// slot into pinned host memory while the next frame computes, and the host moves it into the
-// caller's array.  Events order the slot reuse: a slot is repacked only after its copy ended.
+// slot into pinned host memory while the next frame computes, and a helper thread moves it
+// into the caller's array while the calling thread keeps launching frames (a large frame
+// graph must keep being enqueued, or the device runs dry).  Events order the device slot
+// reuse (repacked only after its copy ended); a pinned slot is refilled only after its
+// helper thread has joined.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "context.h"
@@ -122,6 +126,14 @@ int xrt_record_frames(xrt_ctx* ctx, int32_t nframes, float* positions, float* no
   hipStream_t s = ctx->stream;
   const int N = ctx->wide.n_total;
   const size_t frame_floats = 3 * static_cast<size_t>(N);
+  std::thread mover[2];
+  struct Joiner {  // every helper thread has ended when this returns, also on errors
+    std::thread* t;
+    ~Joiner() {
+      for (int k = 0; k < 2; ++k)
+        if (t[k].joinable()) t[k].join();
+    }
+  } joiner{mover};
   for (int f = 0; f <= nframes; ++f) {
     const int slot = f & 1;
     if (f < nframes) {
@@ -136,6 +148,7 @@ int xrt_record_frames(xrt_ctx* ctx, int32_t nframes, float* positions, float* no
         XRT_CHECK_HIP(Pack(s, R.normals.ptr, N, dslot + frame_floats));
       }
       XRT_CHECK_HIP(hipEventRecord(R.packed[slot], s));
+      if (mover[slot].joinable()) mover[slot].join();  // pinned slot moved out
       XRT_CHECK_HIP(hipStreamWaitEvent(R.copy_stream, R.packed[slot], 0));
       XRT_CHECK_HIP(hipMemcpyAsync(R.host + slot * R.slot_floats, dslot,
                                    sizeof(float) * frame_floats * (normals ? 2 : 1),
@@ -147,9 +160,12 @@ int xrt_record_frames(xrt_ctx* ctx, int32_t nframes, float* positions, float* no
       const int ps = (f - 1) & 1;
       XRT_CHECK_HIP(hipEventSynchronize(R.copied[ps]));
       const float* h = R.host + ps * R.slot_floats;
-      std::memcpy(positions + frame_floats * (f - 1), h, sizeof(float) * frame_floats);
-      if (normals)
-        std::memcpy(normals + frame_floats * (f - 1), h + frame_floats, sizeof(float) * frame_floats);
+      float* dp = positions + frame_floats * (f - 1);
+      float* dn = normals ? normals + frame_floats * (f - 1) : nullptr;
+      mover[ps] = std::thread([h, dp, dn, frame_floats]() {
+        std::memcpy(dp, h, sizeof(float) * frame_floats);
+        if (dn) std::memcpy(dn, h + frame_floats, sizeof(float) * frame_floats);
+      });
     }
   }
   return XRT_OK;
