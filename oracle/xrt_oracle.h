@@ -220,6 +220,19 @@ void orc_binding_solve(double* x, const double* w, const double* obs_x, const in
                        const int32_t* tri3, const double* uvw, const double* L,
                        const double* stiffness, int n);
 
+/* ---------------------------------- quality-mode FEM constraints (FP64, fem_oracle.c) */
+int orc_strain_init(const double* x, const int32_t* tri, double* area, double* inv);
+double orc_strain_energy(const double* x, const int32_t* tri, double area, const double* inv,
+                         const double* E, double* grad9);
+void orc_strain_solve(double* x, const double* w, const int32_t* tris, const double* area,
+                      const double* inv, int beg, int end, const double* E);
+void orc_iso_init(const double* x, const int32_t* b4, double* K, double* coeff);
+double orc_iso_energy(const double* x, const int32_t* b4, const double* K, double coeff,
+                      double* grad12);
+void orc_iso_solve(double* x, const double* w, const int32_t* bidx, const double* K,
+                   const double* coeff, double* lambda, int beg, int end, int iter, double dt,
+                   double stiffness);
+
 /* ------------------------------ particle self-collision (swift mode, particle_oracle.c) */
 int32_t orc_hash_coords(int32_t x, int32_t y, int32_t z, int32_t table);
 /* spatial hash + neighbor lists (column-major, n * max_nb) from FP32 positions; the
