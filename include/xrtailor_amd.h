@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define XRT_ABI_VERSION 3
+#define XRT_ABI_VERSION 4
 
 enum {
   XRT_OK = 0,
@@ -90,10 +90,28 @@ typedef struct xrt_params {
   int32_t inter_leaved_hash;     /* ParticleCollisionSettings::inter_leaved_hash (3)   */
   float particle_diameter;       /* ParticleCollisionSettings::particle_diameter (1.3) */
   float hash_cell_size;          /* ParticleCollisionSettings::hash_cell_size (1.5)    */
+  /* constraint model of every iteration.  XRT_CONSTRAINTS_XPBD: the colored XPBD stretch and
+   * dihedral bend constraints (swift mode).  XRT_CONSTRAINTS_FEM: the quality mode's FEM
+   * constraints in their place (ClothSolverBase.cpp:314-323, set up at :465-485): the
+   * orthotropic Green-strain triangle constraint over faces colored at xrt_start, then, with
+   * solve_bending, the isometric bending constraint over the bend hinges (XPBD, multiplier
+   * reset each substep).  Rest data come from the registered positions.  Everything else
+   * in the iteration (contacts, LRA, ground, bindings) runs as configured. */
+  int32_t constraint_model;
+  /* FEMFabricSettings (xrtailor/include/xrtailor/config/Base.hpp:47-57): Young's moduli
+   * along the two material axes and the shear modulus, the Poisson ratios (xy * yx != 1),
+   * and the isometric bending stiffness (> 0); defaults 1, 1, 1, 0.3, 0.3, 1 */
+  float fem_xx_stiffness;
+  float fem_yy_stiffness;
+  float fem_xy_stiffness;
+  float fem_xy_poisson_ratio;
+  float fem_yx_poisson_ratio;
+  float fem_bending_stiffness;
 } xrt_params;
 
 enum { XRT_BEND_REFERENCE = 0, XRT_BEND_CONSISTENT = 1 };
 enum { XRT_COLOR_GREEDY = 0, XRT_COLOR_PALETTE = 1 };
+enum { XRT_CONSTRAINTS_XPBD = 0, XRT_CONSTRAINTS_FEM = 1 };
 
 typedef struct xrt_ctx xrt_ctx;
 
@@ -171,6 +189,14 @@ int xrt_get_bends(xrt_ctx* ctx, int32_t inst, int32_t* bends, float* theta, int3
 /* LRA: slot vertices (nslots), CSR over targets (nv+1), slot ordinals and rest lengths (K) */
 int xrt_get_lra(xrt_ctx* ctx, int32_t inst, int32_t* slot_vert, int32_t* ptr, int32_t* slot,
                 float* rest);
+/* quality-mode FEM rest data of one instance (constraint_model FEM, after xrt_start):
+ * InitStrain_Kernel / InitIsometricBending_Kernel (xrtailor/src/physics/dynamics/
+ * FEMHelper.cu:42-94, 272-301) computed on the host in FP64 and stored in FP32.  Face colors
+ * and their count, rest areas (nt), inverse rest matrices (4 per face: glm [0][0], [0][1],
+ * [1][0], [1][1]), isometric weights K (4 per bend) and coefficients 3 / (A0 + A1) (nb), in
+ * face and bend order -- any pointer may be NULL */
+int xrt_get_fem(xrt_ctx* ctx, int32_t inst, int32_t* face_colors, int32_t* n_face_colors,
+                float* area, float* inv, float* iso_k, float* iso_coeff);
 int xrt_get_inv_mass(xrt_ctx* ctx, int32_t inst, float* w);
 int xrt_set_state(xrt_ctx* ctx, int32_t inst, const float* x0, const float* v);
 int xrt_get_state(xrt_ctx* ctx, int32_t inst, float* x0, float* v);
@@ -185,6 +211,16 @@ int xrt_predict_positions(xrt_ctx* ctx, int32_t inst, float dt);
 int xrt_stretch_solve(xrt_ctx* ctx, int32_t inst, int32_t color, float dt, int32_t iter);
 /* BasicConstraint::BendSolver::Solve(nodes, color, dt, iter) (BasicSolver.cuh:15-33) */
 int xrt_bend_solve(xrt_ctx* ctx, int32_t inst, int32_t color, float dt, int32_t iter);
+/* FEM::StrainSolver::Solve(nodes, color) (xrtailor/include/xrtailor/physics/dynamics/
+ * FEMSolver.cuh:40, FEMSolver.cu:36-38): one face color of the Green-strain constraint
+ * (constraint_model FEM) */
+int xrt_strain_solve(xrt_ctx* ctx, int32_t inst, int32_t color);
+/* FEM::IsometricBendingSolver::Solve(nodes, color, iter, dt) (FEMSolver.cuh:22,
+ * FEMSolver.cu:83-86): one bend
+ * color of the isometric bending constraint, multipliers reset when iter == 0
+ * (constraint_model FEM) */
+int xrt_isometric_bending_solve(xrt_ctx* ctx, int32_t inst, int32_t color, int32_t iter,
+                                float dt);
 /* SolveGeodesicLRA (xrtailor/include/xrtailor/physics/dynamics/LongRangeAttachments.cuh:11-13) */
 int xrt_solve_lra(xrt_ctx* ctx, int32_t inst);
 /* ground plane projection (SDFCollider type 1, xrtailor/include/xrtailor/physics/sdf/SDFCollider.cuh:36-43) */

@@ -16,7 +16,8 @@ ClothSolverBase::SimulateSwift's body-cloth PBD collision
     pair refresh when it % interval == 0: cloth LBVH Update(x, x0, ccd) and
       Traverse(cloth, obstacle) ++ Traverse(cloth, cloth) at thickness 1e-6
     repulsion over the pairs, applied x relaxation rate
-    stretch colors, bend colors, LRA, ground
+    stretch colors, bend colors (or, with setup_fem, the quality mode's FEM strain colors
+      and isometric bending colors, :314-323), LRA, ground
     VF-CCD, VF-DCD solves; EE-DCD deltas applied x relaxation factor
     bindings: BindingConstraint::Solve against the obstacle's frame positions (:237)
   finalize; frame end: obstacle x0 = x
@@ -30,6 +31,7 @@ import numpy as np
 from . import oracle as orc
 from . import particle as opart
 from . import binding as obind
+from . import fem as ofem
 
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int32)
@@ -154,6 +156,7 @@ class ContactScene:
         self.dt = float(np.float32(xp.frame_time) / np.float32(xp.num_substeps))
         self.tape = None  # a list: substep() appends each operator's input state (adjoint.py)
         self.bindings = False
+        self.fem = False
         # swift-mode particle self-collision: diameter from the rest mesh, initial positions
         # = the state now (FP32), neighbor lists zero until the first hash
         self.particles = bool(getattr(xp, "enable_particle_collision", 0))
@@ -180,6 +183,26 @@ class ContactScene:
         self.b_L = obind.generate(self.c.x0.reshape(-1, 3), ox0.astype(np.float64), vert,
                                   self.b_tri3, self.b_uvw, d_in.astype(np.float64))
         self.bindings = True
+
+    def setup_fem(self, face_colors, n_colors, rest_pos, E, bend_stiffness):
+        """The quality mode's FEM constraints in place of stretch and bend: faces sorted by
+        (color, index), rest data in FP64 from the FP32 rest positions and rounded to FP32
+        as the solver stores them, the isometric data over the bend elements in their color
+        order.  E = (xx, yy, xy stiffness, xy, yx Poisson ratio) as FP32 values."""
+        order, self.f_off = orc.color_sort(np.asarray(face_colors), n_colors)
+        rp = np.asarray(rest_pos, np.float32).astype(np.float64)
+        self.f_tris = np.ascontiguousarray(self.tris[order], np.int32)
+        area, inv = ofem.strain_init(rp, self.f_tris)
+        self.f_area = area.astype(np.float32).astype(np.float64)
+        self.f_inv = np.ascontiguousarray(inv.astype(np.float32).astype(np.float64))
+        self.f_E = np.asarray(E, np.float32).astype(np.float64)
+        K, cf = ofem.iso_init(rp, self.c.b_idx.reshape(-1, 4))
+        self.iso_K = np.ascontiguousarray(K.astype(np.float32).astype(np.float64))
+        self.iso_c = cf.astype(np.float32).astype(np.float64)
+        self.iso_lam = np.zeros(len(cf))
+        self.f_bstiff = float(np.float32(bend_stiffness))
+        self.f_ncol = n_colors
+        self.fem = True
 
     def _rec(self, *entry):
         if self.tape is not None:
@@ -250,10 +273,18 @@ class ContactScene:
                                   _i(self.pairs.reshape(-1)), _i(self.kind), len(self.pairs),
                                   xp.pbd_thickness, _d(self.delta), _i(self.count))
                 lib.orc_apply_deltas(_d(c.x), _d(self.delta), _i(self.count), self.nv, xp.repulsion_relaxation)
-            for k in range(c.s_ncol):
+            if self.fem:
+                for k in range(self.f_ncol):
+                    ofem.strain_solve(c.x, c.w, self.f_tris, self.f_area, self.f_inv,
+                                      int(self.f_off[k]), int(self.f_off[k + 1]), self.f_E)
+                if p.solve_bending:
+                    for k in range(c.b_ncol):
+                        ofem.iso_solve(c.x, c.w, c.b_idx, self.iso_K, self.iso_c, self.iso_lam,
+                                       int(c.b_off[k]), int(c.b_off[k + 1]), it, dt, self.f_bstiff)
+            for k in range(0 if self.fem else c.s_ncol):
                 self._rec("stretch", k, it, c.x.copy(), c.s_lam.copy())
                 lib.orc_stretch_color(C.byref(c.s), k, dt, it)
-            if p.solve_bending:
+            if p.solve_bending and not self.fem:
                 for k in range(c.b_ncol):
                     self._rec("bend", k, c.x.copy())
                     lib.orc_bend_color(C.byref(c.s), C.byref(p), k, dt)

@@ -1,0 +1,198 @@
+"""GPU quality-mode FEM constraints (orthotropic Green strain, isometric bending) against the
+oracle, through the C ABI: the rest data against the FP64 restatement, one pass of every
+color against the oracle's projections from the same FP32 state, whole substeps against the
+FP64 twin (with and without body contact), batched == plain bit for bit."""
+import numpy as np
+import pytest
+
+from contact_helpers import oracle_scene, sphere_drop
+from gpu_helpers import build, rel_err
+from oracle import fem as ofem
+from oracle import oracle as orc
+from xrtailor_amd import XrtError, scenes
+from xrtailor_amd.solver import make_params
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+E = (2.0, 1.0, 0.5, 0.4, 0.2)  # xx, yy, xy stiffness, xy, yx Poisson ratio (reciprocal)
+
+
+def _sheet(n=16, side=0.5, jitter=0.25, strain=0.03, seed=4, pin=True):
+    """A grid whose rest shape is jittered in and out of plane (faces of many shapes) and
+    whose state is the rest shape plus a small random deformation."""
+    pos, tris = scenes.grid(n, side, 0.0)
+    rng = np.random.default_rng(seed)
+    h = side / (n - 1)
+    rest = pos + jitter * h * rng.uniform(-1.0, 1.0, pos.shape) * np.array([1.0, 0.3, 1.0])
+    state = rest + strain * h * rng.standard_normal(pos.shape)
+    pinned = np.array([0, n - 1], np.int32) if pin else np.zeros(0, np.int32)
+    return scenes.ClothSpec(rest, tris, pinned, 1e-7, 1e-3, state.astype(np.float32))
+
+
+def _params(**kw):
+    base = dict(constraint_model=1, fem_xx_stiffness=E[0], fem_yy_stiffness=E[1],
+                fem_xy_stiffness=E[2], fem_xy_poisson_ratio=E[3], fem_yx_poisson_ratio=E[4],
+                fem_bending_stiffness=50.0, bend_model=1, enable_ground=0)
+    base.update(kw)
+    return make_params(**base)
+
+
+def _twin(spec, xp, s, i):
+    twin = oracle_scene(spec, xp, solver=s, inst=i)
+    f = s.fem(i)
+    twin.setup_fem(f["colors"], f["n_colors"], spec.pos, E, xp.fem_bending_stiffness)
+    return twin
+
+
+def test_rest_data_and_colors_match_oracle():
+    print("case FEM rest data and face colors", flush=True)
+    spec = _sheet()
+    xp = _params()
+    s, (i,) = build([spec], xp, seed=3)
+    f = s.fem(i)
+    tris = spec.tris
+    col, ncol = orc.color(tris, len(spec.pos), 3 + i, xp.coloring)
+    np.testing.assert_array_equal(f["colors"], col)
+    assert f["n_colors"] == ncol
+    for c in range(ncol):  # no two faces of a color share a vertex
+        v = tris[f["colors"] == c].ravel()
+        assert len(np.unique(v)) == len(v)
+    rp = spec.pos.astype(np.float32).astype(np.float64)
+    area, inv = ofem.strain_init(rp, tris)
+    bends = s.bends(i, colors=False)[0]
+    K, cf = ofem.iso_init(rp, bends)
+    errs = [rel_err(f["area"], area), rel_err(f["inv"], inv), rel_err(f["K"], K),
+            rel_err(f["coeff"], cf)]
+    print(f"  {len(tris)} faces in {ncol} colors, {len(bends)} hinges; rel err area / inv / "
+          f"K / coeff {', '.join(f'{e:.1e}' for e in errs)}", flush=True)
+    assert max(errs) <= 1e-6
+    s.close()
+
+
+def test_one_pass_matches_oracle():
+    """Every strain color once, then every isometric bending color at iterations 0 and 1
+    (the multiplier carried from the first), against the oracle's projections."""
+    print("case FEM strain + isometric bending passes", flush=True)
+    spec = _sheet(strain=0.1)
+    xp = _params()
+    s, (i,) = build([spec], xp, seed=3)
+    x = spec.x0
+    w = s.inv_mass(i).astype(np.float64)
+    f = s.fem(i)
+    twin = _twin(spec, xp, s, i)
+    dt = np.float32(xp.frame_time) / np.float32(xp.num_substeps)
+    ss, bs = s.StrainSolver(i), s.IsometricBendingSolver(i)
+    assert ss.Coloring().n_colors == f["n_colors"]
+    for c in range(f["n_colors"]):
+        ss.Solve(c)
+    got = s.GetPredicted(i)
+    ref = x.astype(np.float64).reshape(-1).copy()
+    for c in range(f["n_colors"]):
+        ofem.strain_solve(ref, w, twin.f_tris, twin.f_area, twin.f_inv, int(twin.f_off[c]),
+                          int(twin.f_off[c + 1]), twin.f_E)
+    e1 = rel_err(got, ref.reshape(-1, 3))
+    moved = np.abs(ref.reshape(-1, 3) - x).max()
+    for it in (0, 1):
+        for c in range(bs.Coloring().n_colors):
+            bs.Solve(c, it, float(dt))
+    got2 = s.GetPredicted(i)
+    ref2 = got.astype(np.float64).reshape(-1).copy()
+    cl = twin.c
+    for it in (0, 1):
+        for c in range(cl.b_ncol):
+            ofem.iso_solve(ref2, w, cl.b_idx, twin.iso_K, twin.iso_c, twin.iso_lam,
+                           int(cl.b_off[c]), int(cl.b_off[c + 1]), it, float(dt), twin.f_bstiff)
+    e2 = rel_err(got2, ref2.reshape(-1, 3))
+    moved2 = np.abs(ref2.reshape(-1, 3) - got).max()
+    print(f"  strain: moved {moved:.2e} m, rel err {e1:.2e}; bending: moved {moved2:.2e} m, "
+          f"rel err {e2:.2e}", flush=True)
+    assert moved > 1e-4 and moved2 > 1e-7
+    assert e1 <= TOL and e2 <= TOL
+    s.close()
+
+
+@pytest.mark.parametrize("scene", ["sheet", "sphere"])
+def test_substeps_match_oracle_twin(scene):
+    """Plain substeps with the FEM constraints (pinned sheet under gravity; a sheet dropped
+    on an icosphere with body contact) against the FP64 twin."""
+    print(f"case FEM substeps vs oracle twin: {scene}", flush=True)
+    if scene == "sheet":
+        spec, v = _sheet(), None
+        xp = _params()
+    else:
+        spec, v = sphere_drop(n=16)
+        xp = _params(enable_obstacle_contact=1, ground_height=-1.0)
+    s, (i,) = build([spec], xp, seed=3)
+    if v is not None:
+        s.SetState(i, spec.x0, v)
+    twin = _twin(spec, xp, s, i)
+    errs = []
+    for _ in range(4):
+        s.SubstepPlain(i)
+        twin.substep()
+        errs.append(rel_err(s.GetState(i)[0], twin.c.state()[0]))
+    print(f"  rel err per substep {', '.join(f'{e:.2e}' for e in errs)}", flush=True)
+    assert errs[0] <= TOL
+    assert max(errs) <= 1e-4
+    s.close()
+
+
+@pytest.mark.parametrize("obstacle", [0, 1])
+def test_batched_equals_plain(obstacle):
+    """Instances of three sizes: graph-captured batched frames (wide FEM launches) equal
+    plain frames bit for bit."""
+    print(f"case FEM batched vs plain, obstacle {obstacle}", flush=True)
+    if obstacle:
+        cases = [sphere_drop(n=n, seed=k) for k, n in enumerate((12, 16, 20))]
+        specs = [c[0] for c in cases]
+        vs = [c[1] for c in cases]
+    else:
+        specs = [_sheet(n=n, seed=k) for k, n in enumerate((12, 16, 20))]
+        vs = [None] * 3
+    xp = _params(enable_obstacle_contact=obstacle, num_substeps=4, num_iterations=4)
+    runs = []
+    for batched in (True, False):
+        s, insts = build(specs, xp, seed=1)
+        for k, (spec, v) in zip(insts, zip(specs, vs)):
+            if v is not None:
+                s.SetState(k, spec.x0, v)
+        if batched:
+            s.FramesBatched(2)
+            s.SubstepBatched()
+        else:
+            s.FramesPlain(2)
+            for k in insts:
+                s.SubstepPlain(k)
+        runs.append([s.GetState(k)[0] for k in insts])
+        s.close()
+    for k, (a, b) in enumerate(zip(*runs)):
+        print(f"  instance {k}: max |batched - plain| = {np.abs(a - b).max():.3e}", flush=True)
+        np.testing.assert_array_equal(a, b)
+
+
+def test_fem_params_and_errors():
+    print("case FEM parameters and error paths", flush=True)
+    spec = _sheet()
+    s, (i,) = build([spec], _params(constraint_model=0), seed=3)
+    with pytest.raises(XrtError):
+        s.StrainSolver(i).Solve(0)  # XPBD model: no FEM data
+    with pytest.raises(XrtError):
+        s.set_params(_params(fem_xy_poisson_ratio=1.0, fem_yx_poisson_ratio=1.0))
+    with pytest.raises(XrtError):
+        s.set_params(_params(fem_bending_stiffness=0.0))
+    with pytest.raises(XrtError):
+        s.set_params(_params(constraint_model=2))
+    # switched to FEM after xrt_start: set up then, same data as at xrt_start
+    s.set_params(_params())
+    f = s.fem(i)
+    s2, (j,) = build([spec], _params(), seed=3)
+    f2 = s2.fem(j)
+    for key in ("colors", "area", "inv", "K", "coeff"):
+        np.testing.assert_array_equal(f[key], f2[key])
+    with pytest.raises(XrtError):
+        s.StrainSolver(i).Solve(f["n_colors"])  # color out of range
+    with pytest.raises(XrtError):
+        s.AdjointBegin(1, 1)  # the adjoint has no FEM constraints
+    s2.close()
+    s.close()

@@ -58,7 +58,7 @@ def test_abi_exports_every_declared_symbol():
 
 def test_abi_version_and_defaults():
     lib = _ffi.load()
-    assert lib.xrt_abi_version() == 3
+    assert lib.xrt_abi_version() == 4
     p = _ffi.XrtParams()
     lib.xrt_default_params(C.byref(p))
     assert p.num_substeps == 10 and p.num_iterations == 10
@@ -70,7 +70,12 @@ def test_abi_version_and_defaults():
     assert p.enable_particle_collision == 0 and p.max_num_neighbors == 64
     assert p.inter_leaved_hash == 3
     assert p.particle_diameter == pytest.approx(1.3) and p.hash_cell_size == pytest.approx(1.5)
-    assert C.sizeof(p) == 4 * 30  # 30 four-byte fields
+    # FEMFabricSettings defaults (xrtailor/include/xrtailor/config/Base.hpp:47-57)
+    assert p.constraint_model == 0
+    assert (p.fem_xx_stiffness, p.fem_yy_stiffness, p.fem_xy_stiffness) == (1.0, 1.0, 1.0)
+    assert p.fem_xy_poisson_ratio == pytest.approx(0.3) and p.fem_yx_poisson_ratio == pytest.approx(0.3)
+    assert p.fem_bending_stiffness == 1.0
+    assert C.sizeof(p) == 4 * 37  # 37 four-byte fields
 
 
 @pytest.mark.parametrize("n", [2, 5, 64])

@@ -59,6 +59,13 @@ class XrtParams(C.Structure):
         ("inter_leaved_hash", C.c_int32),
         ("particle_diameter", C.c_float),
         ("hash_cell_size", C.c_float),
+        ("constraint_model", C.c_int32),
+        ("fem_xx_stiffness", C.c_float),
+        ("fem_yy_stiffness", C.c_float),
+        ("fem_xy_stiffness", C.c_float),
+        ("fem_xy_poisson_ratio", C.c_float),
+        ("fem_yx_poisson_ratio", C.c_float),
+        ("fem_bending_stiffness", C.c_float),
     ]
 
 
@@ -102,6 +109,9 @@ SIGNATURES = {
     "xrt_predict_positions": [_P, _I, _F],
     "xrt_stretch_solve": [_P, _I, _I, _F, _I],
     "xrt_bend_solve": [_P, _I, _I, _F, _I],
+    "xrt_strain_solve": [_P, _I, _I],
+    "xrt_isometric_bending_solve": [_P, _I, _I, _I, _F],
+    "xrt_get_fem": [_P, _I, _IP, _IP, _FP, _FP, _FP, _FP],
     "xrt_solve_lra": [_P, _I],
     "xrt_solve_ground": [_P, _I],
     "xrt_apply_deltas": [_P, _I],

@@ -50,6 +50,7 @@ int AdjSetup(xrt_ctx* c, int want_slots = 1) {
   XRT_REQUIRE(!c->params.enable_particle_collision, XRT_ERR_UNSUPPORTED,
               "the adjoint does not support particle self-collision");
   XRT_REQUIRE(!BindingsOn(c), XRT_ERR_UNSUPPORTED, "the adjoint does not support bindings");
+  XRT_REQUIRE(!FemOn(c), XRT_ERR_UNSUPPORTED, "the adjoint does not support the FEM constraints");
   const xrt_params& p = c->params;
   const int n_inst = static_cast<int>(c->inst.size());
   XRT_REQUIRE(n_inst > 0 && n_inst <= 65535, XRT_ERR_INVALID_ARG, "1..65535 instances");
@@ -221,11 +222,15 @@ int WideSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, AdjointData
       for (int i = 0; i < n_inst; ++i)
         if (ObsContact(c, i) && (rc = SnapshotRecords(c, i, c->contact[i]->pass_cnt - 1)) != XRT_OK)
           return rc;
-    for (int col = 0; col < static_cast<int>(W.s_max.size()); ++col)
-      XRT_CHECK_HIP(LaunchWideStretch(s, wa, n_inst, col, W.s_max[col], k.inv_dt2, it,
-                                      TI ? TI + A->off_s : nullptr, W.maps.ptr + W.s_map[col],
-                                      W.s_blocks[col]));
-    if (k.solve_bending)
+    if (FemOn(c)) {  // never taped: the adjoint refuses the FEM constraints
+      if ((rc = FemIter(c, -1, it)) != XRT_OK) return rc;
+    } else {
+      for (int col = 0; col < static_cast<int>(W.s_max.size()); ++col)
+        XRT_CHECK_HIP(LaunchWideStretch(s, wa, n_inst, col, W.s_max[col], k.inv_dt2, it,
+                                        TI ? TI + A->off_s : nullptr, W.maps.ptr + W.s_map[col],
+                                        W.s_blocks[col]));
+    }
+    if (k.solve_bending && !FemOn(c))
       for (int col = 0; col < static_cast<int>(W.b_max.size()); ++col)
         XRT_CHECK_HIP(LaunchWideBend(s, wa, n_inst, col, W.b_max[col], k.inv_dt2, bm,
                                      TI ? TI + A->off_b : nullptr, W.maps.ptr + W.b_map[col],
@@ -395,9 +400,11 @@ int xrt_adjoint_backward(xrt_ctx* ctx) {
   AdjointData& A = *ctx->adj;
   XRT_REQUIRE(!A.has_grad, XRT_ERR_STATE,
               "the seeds were consumed by the previous reverse pass: seed again first");
-  XRT_REQUIRE(!ctx->params.enable_particle_collision && !ctx->params.enable_self_contact,
+  XRT_REQUIRE(!ctx->params.enable_particle_collision && !ctx->params.enable_self_contact &&
+                  !FemOn(ctx),
               XRT_ERR_UNSUPPORTED,
-              "particle self-collision / self contact were enabled after xrt_adjoint_begin");
+              "particle self-collision / self contact / FEM constraints were enabled after "
+              "xrt_adjoint_begin");
   const StepConsts k = MakeConsts(ctx->params);
   const BendMode bm = ModeOf(k);
   const size_t N = static_cast<size_t>(A.n_total);

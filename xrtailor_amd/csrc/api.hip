@@ -41,6 +41,18 @@ StepConsts MakeConsts(const xrt_params& p) {
 
 BendMode ModeOf(const StepConsts& k) { return BendMode{k.bend_dlo, k.bend_th0, k.bend_th1, k.bend_sgn}; }
 
+// Sort constraints by (color, original index); returns local color offsets.
+void ColorSort(const std::vector<int32_t>& color, int ncol, std::vector<int32_t>* order,
+               std::vector<int32_t>* coloff) {
+  const int n = static_cast<int>(color.size());
+  coloff->assign(ncol + 1, 0);
+  for (int i = 0; i < n; ++i) (*coloff)[color[i] + 1]++;
+  for (int c = 0; c < ncol; ++c) (*coloff)[c + 1] += (*coloff)[c];
+  std::vector<int32_t> fill(coloff->begin(), coloff->end() - 1);
+  order->assign(n, 0);
+  for (int i = 0; i < n; ++i) (*order)[fill[color[i]]++] = i;
+}
+
 namespace {
 void DestroyGraph(xrt_ctx* c) {
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -63,17 +75,6 @@ int CheckStarted(xrt_ctx* c, int32_t inst) {
   return XRT_OK;
 }
 
-// Sort constraints by (color, original index); returns local color offsets.
-void ColorSort(const std::vector<int32_t>& color, int ncol, std::vector<int32_t>* order,
-               std::vector<int32_t>* coloff) {
-  const int n = static_cast<int>(color.size());
-  coloff->assign(ncol + 1, 0);
-  for (int i = 0; i < n; ++i) (*coloff)[color[i] + 1]++;
-  for (int c = 0; c < ncol; ++c) (*coloff)[c + 1] += (*coloff)[c];
-  std::vector<int32_t> fill(coloff->begin(), coloff->end() - 1);
-  order->assign(n, 0);
-  for (int i = 0; i < n; ++i) (*order)[fill[color[i]]++] = i;
-}
 
 int UploadState(xrt_ctx* c, int32_t inst, const float* x0, const float* v) {
   xrt::HostInstance& h = c->inst[inst];
@@ -146,12 +147,16 @@ int SubstepPlain(xrt_ctx* c, int32_t inst) {
   if (rc != XRT_OK) return rc;
   for (int it = 0; it < k.iterations; ++it) {
     if ((rc = ParticleIter(c, inst)) != XRT_OK) return rc;
-    for (int col = 0; col < h.s_ncol; ++col) {
-      const int b = h.sbeg + h.s_coloff[col];
-      XRT_CHECK_HIP(LaunchStretch(s, c->sc.ptr + b, c->slam.ptr + b, x,
-                                  h.s_coloff[col + 1] - h.s_coloff[col], k.inv_dt2, it));
+    if (FemOn(c)) {
+      if ((rc = FemIter(c, inst, it)) != XRT_OK) return rc;
+    } else {
+      for (int col = 0; col < h.s_ncol; ++col) {
+        const int b = h.sbeg + h.s_coloff[col];
+        XRT_CHECK_HIP(LaunchStretch(s, c->sc.ptr + b, c->slam.ptr + b, x,
+                                    h.s_coloff[col + 1] - h.s_coloff[col], k.inv_dt2, it));
+      }
     }
-    if (k.solve_bending) {
+    if (k.solve_bending && !FemOn(c)) {
       for (int col = 0; col < h.b_ncol; ++col) {
         const int b = h.bbeg + h.b_coloff[col];
         XRT_CHECK_HIP(LaunchBend(s, c->bidx.ptr + b, c->bpar.ptr + b, x,
@@ -168,12 +173,12 @@ int SubstepPlain(xrt_ctx* c, int32_t inst) {
   return ParticleSubstepEnd(c, inst);
 }
 
-// Without contacts: one persistent launch.  With body or self contact, particle collision
-// or bindings: the wide substep (each color, contact phase, hash step, binding pass and
-// LBVH operation of every instance in one launch).
+// Without contacts: one persistent launch.  With body or self contact, particle collision,
+// bindings or the FEM constraints: the wide substep (each color, contact phase, hash step,
+// binding pass and LBVH operation of every instance in one launch).
 int SubstepBatched(xrt_ctx* c) {
   const StepConsts k = MakeConsts(c->params);
-  if (ContactsActive(c) || ParticlesOn(c) || BindingsOn(c))
+  if (ContactsActive(c) || ParticlesOn(c) || BindingsOn(c) || FemOn(c))
     return WideSubstep(c, k, ModeOf(k), nullptr, -1);
   XRT_CHECK_HIP(LaunchBatchedSubstep(c->stream, MakeBatchArgs(c), k,
                                      static_cast<int>(c->inst.size()), c->batched_block,
@@ -245,6 +250,13 @@ void xrt_default_params(xrt_params* p) {
   p->inter_leaved_hash = 3;
   p->particle_diameter = 1.3f;
   p->hash_cell_size = 1.5f;
+  p->constraint_model = XRT_CONSTRAINTS_XPBD;
+  p->fem_xx_stiffness = 1.0f;
+  p->fem_yy_stiffness = 1.0f;
+  p->fem_xy_stiffness = 1.0f;
+  p->fem_xy_poisson_ratio = 0.3f;
+  p->fem_yx_poisson_ratio = 0.3f;
+  p->fem_bending_stiffness = 1.0f;
 }
 
 int xrt_create(xrt_ctx** out, int device) {
@@ -310,10 +322,24 @@ int xrt_set_params(xrt_ctx* ctx, const xrt_params* p) {
   XRT_REQUIRE(p->particle_diameter > 0.0f && p->hash_cell_size > 0.0f &&
                   std::isfinite(p->particle_diameter) && std::isfinite(p->hash_cell_size),
               XRT_ERR_INVALID_ARG, "particle_diameter and hash_cell_size must be finite and > 0");
+  XRT_REQUIRE(p->constraint_model == XRT_CONSTRAINTS_XPBD || p->constraint_model == XRT_CONSTRAINTS_FEM,
+              XRT_ERR_INVALID_ARG, "unknown constraint_model");
+  if (p->constraint_model == XRT_CONSTRAINTS_FEM) {
+    const float fem[6] = {p->fem_xx_stiffness, p->fem_yy_stiffness, p->fem_xy_stiffness,
+                          p->fem_xy_poisson_ratio, p->fem_yx_poisson_ratio, p->fem_bending_stiffness};
+    for (float f : fem) XRT_REQUIRE(std::isfinite(f), XRT_ERR_INVALID_ARG, "non-finite FEM parameter");
+    XRT_REQUIRE(p->fem_xx_stiffness >= 0.0f && p->fem_yy_stiffness >= 0.0f &&
+                    p->fem_xy_stiffness >= 0.0f && p->fem_bending_stiffness > 0.0f,
+                XRT_ERR_INVALID_ARG, "FEM stiffnesses must be >= 0, bending stiffness > 0");
+    XRT_REQUIRE(p->fem_xy_poisson_ratio * p->fem_yx_poisson_ratio != 1.0f, XRT_ERR_INVALID_ARG,
+                "fem_xy_poisson_ratio * fem_yx_poisson_ratio must differ from 1");
+  }
   ctx->params = *p;
   DestroyGraph(ctx);
-  if (ctx->started) return ParticleConfigure(ctx);
-  return XRT_OK;
+  if (!ctx->started) return XRT_OK;
+  const int rc = FemPrepare(ctx);
+  if (rc != XRT_OK) return rc;
+  return ParticleConfigure(ctx);
 }
 
 int xrt_get_params(xrt_ctx* ctx, xrt_params* p) {
@@ -632,6 +658,8 @@ int xrt_start(xrt_ctx* ctx, uint64_t seed) {
   if (rc != XRT_OK) return rc;
   if ((rc = ParticleStart(ctx)) != XRT_OK) return rc;
   if ((rc = BindingStart(ctx)) != XRT_OK) return rc;
+  ctx->fem.seed = seed;
+  if ((rc = FemPrepare(ctx)) != XRT_OK) return rc;
   ctx->started = true;
   return XRT_OK;
 }

@@ -493,12 +493,16 @@ int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendM
   for (int it = 0; it < k.iterations; ++it) {
     if ((rc = ParticleIter(ctx, inst)) != XRT_OK) return rc;
     if ((rc = ContactIterPre(ctx, inst, it, nullptr)) != XRT_OK) return rc;
-    for (int col = 0; col < h.s_ncol; ++col) {
-      const int b = h.sbeg + h.s_coloff[col];
-      XRT_CHECK_HIP(LaunchStretch(s, ctx->sc.ptr + b, ctx->slam.ptr + b, x,
-                                  h.s_coloff[col + 1] - h.s_coloff[col], k.inv_dt2, it));
+    if (FemOn(ctx)) {
+      if ((rc = FemIter(ctx, inst, it)) != XRT_OK) return rc;
+    } else {
+      for (int col = 0; col < h.s_ncol; ++col) {
+        const int b = h.sbeg + h.s_coloff[col];
+        XRT_CHECK_HIP(LaunchStretch(s, ctx->sc.ptr + b, ctx->slam.ptr + b, x,
+                                    h.s_coloff[col + 1] - h.s_coloff[col], k.inv_dt2, it));
+      }
     }
-    if (k.solve_bending)
+    if (k.solve_bending && !FemOn(ctx))
       for (int col = 0; col < h.b_ncol; ++col) {
         const int b = h.bbeg + h.b_coloff[col];
         XRT_CHECK_HIP(LaunchBend(s, ctx->bidx.ptr + b, ctx->bpar.ptr + b, x,

@@ -14,6 +14,7 @@
 #include "common.h"
 #include "contact.h"
 #include "device.h"
+#include "fem.h"
 #include "particle.h"
 #include "topology.h"
 
@@ -146,6 +147,26 @@ struct BindingData {
   std::vector<MapRef> skin_one;
 };
 
+// Quality-mode FEM constraints of every instance (fem.h has the HBM layout).  Built at
+// xrt_start, or at xrt_set_params when the constraint model turns to FEM later.
+struct FemData {
+  bool ready = false;
+  uint64_t seed = 0;  // xrt_start's coloring seed
+  struct Inst {
+    std::vector<int32_t> color, order, coloff;  // face colors, sorted -> face, local offsets
+    int ncol = 0, fbeg = 0;
+  };
+  std::vector<Inst> inst;
+  DevBuf<StrainC> sc;            // faces in color order per instance
+  DevBuf<int32_t> f_coloff;      // global color offsets (ncol + 1 per instance)
+  DevBuf<int2> fcol;             // per instance: first offset, colors
+  DevBuf<float4> iso_k;          // isometric bending, in the bend elements' order
+  DevBuf<float> iso_c, iso_lam;
+  std::vector<size_t> f_map;     // wide strain launches: per-color block maps in `maps`
+  std::vector<int> f_blocks;
+  DevBuf<int32_t> maps;
+};
+
 // Per-frame cache of xrt_record_frames (record_api.hip): two device slots packed after
 // each frame, copied on a second stream into two pinned host slots.
 struct RecordData {
@@ -252,6 +273,8 @@ struct xrt_ctx {
   xrt::BindingData binding;
   // per-frame cache (xrt_record_frames)
   xrt::RecordData record;
+  // quality-mode FEM constraints
+  xrt::FemData fem;
   // differentiable rollout (allocated by xrt_adjoint_begin)
   std::unique_ptr<xrt::AdjointData> adj;
   // graph of one batched frame
@@ -268,6 +291,9 @@ namespace xrt {
 // api.hip
 StepConsts MakeConsts(const xrt_params& p);
 BendMode ModeOf(const StepConsts& k);
+// sort constraints by (color, original index): order = sorted -> original, local offsets
+void ColorSort(const std::vector<int32_t>& color, int ncol, std::vector<int32_t>* order,
+               std::vector<int32_t>* coloff);
 // contact_api.hip
 bool ContactNeeded(const xrt_ctx* ctx, const HostInstance& h);
 int ContactStart(xrt_ctx* ctx);
@@ -301,6 +327,12 @@ int ParticleSubstepEnd(xrt_ctx* ctx, int inst);    // after finalize
 bool BindingsOn(const xrt_ctx* ctx);
 int BindingStart(xrt_ctx* ctx);
 int BindingIter(xrt_ctx* ctx, int inst);
+// fem_api.hip: the quality mode's FEM constraints.  FemPrepare builds them when the
+// constraint model is FEM (xrt_start, xrt_set_params); FemIter runs one iteration's strain
+// and isometric bending colors of an instance, or of every instance (-1)
+bool FemOn(const xrt_ctx* ctx);
+int FemPrepare(xrt_ctx* ctx);
+int FemIter(xrt_ctx* ctx, int inst, int it);
 // adjoint_api.hip: one substep of every instance as wide launches with the contact phases
 // above; with a tape owner and sub >= 0 the operators' inputs go to tape slot `sub`
 int WideSubstep(xrt_ctx* ctx, const StepConsts& k, const BendMode& bm, AdjointData* tape, int sub);

@@ -101,6 +101,34 @@ class BendSolver:
         check(self._s.lib.xrt_bend_solve(self._s.ctx, self._i, color, dt, iteration))
 
 
+class StrainSolver:
+    """FEM::StrainSolver (FEMSolver.cu:9-42) over one instance of a started ClothSolver
+    whose constraint_model is FEM."""
+
+    def __init__(self, solver: "ClothSolver", inst: int):
+        self._s, self._i = solver, inst
+
+    def Coloring(self) -> _Coloring:
+        return _Coloring(self._s.fem(self._i)["n_colors"])
+
+    def Solve(self, color: int) -> None:
+        check(self._s.lib.xrt_strain_solve(self._s.ctx, self._i, color))
+
+
+class IsometricBendingSolver:
+    """FEM::IsometricBendingSolver (FEMSolver.cu:44-115) over one instance of a started
+    ClothSolver whose constraint_model is FEM; it shares the bend elements and colors."""
+
+    def __init__(self, solver: "ClothSolver", inst: int):
+        self._s, self._i = solver, inst
+
+    def Coloring(self) -> _Coloring:
+        return _Coloring(self._s.counts(self._i).bend_colors)
+
+    def Solve(self, color: int, iteration: int, dt: float) -> None:
+        check(self._s.lib.xrt_isometric_bending_solve(self._s.ctx, self._i, color, iteration, dt))
+
+
 def write_obj(path: str, pos, tris, normals=None) -> None:
     """SaveObj: one mesh as OBJ text (the C ABI's writer)."""
     pos = np.ascontiguousarray(pos, np.float32).reshape(-1)
@@ -359,6 +387,27 @@ class ClothSolver:
 
     def BendSolver(self, inst: int = 0) -> BendSolver:
         return BendSolver(self, inst)
+
+    def StrainSolver(self, inst: int = 0) -> StrainSolver:
+        return StrainSolver(self, inst)
+
+    def IsometricBendingSolver(self, inst: int = 0) -> IsometricBendingSolver:
+        return IsometricBendingSolver(self, inst)
+
+    def fem(self, inst: int) -> dict:
+        """The FEM rest data of one instance in face / bend order: face colors and their
+        count, areas, inverse rest matrices (nt, 4), isometric weights K (nb, 4), coeff."""
+        n = self.counts(inst)
+        col = np.zeros(n.nt, np.int32)
+        ncol = np.zeros(1, np.int32)
+        area = np.zeros(n.nt, np.float32)
+        inv = np.zeros(4 * n.nt, np.float32)
+        k = np.zeros(4 * n.n_bends, np.float32)
+        cf = np.zeros(n.n_bends, np.float32)
+        check(self.lib.xrt_get_fem(self.ctx, inst, iptr(col), iptr(ncol), fptr(area), fptr(inv),
+                                   fptr(k), fptr(cf)))
+        return {"colors": col, "n_colors": int(ncol[0]), "area": area, "inv": inv.reshape(-1, 4),
+                "K": k.reshape(-1, 4), "coeff": cf}
 
     def PredictPositions(self, inst: int, dt: float) -> None:
         check(self.lib.xrt_predict_positions(self.ctx, inst, dt))
