@@ -21,11 +21,20 @@ cpu_baseline the serial FP64 oracle on one core, one substep per instance over a
              sample; the same leg reports the single-substep relative error of the GPU's
              batched substep against it (parity at the measured configuration)
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload batched|swift|hanging]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+                    [--workload batched|swift|quality|garment|hanging]
                     (swift: the same 256 cloths with body contact and the particle
                      self-collision of the reference's swift mode instead of PBD repulsion;
                      roofline kernel ParticleCollideKernel, algorithmic bytes per launch
                      48 N + 4 (neighbor entries + N))
+                    (quality: the same 256 cloths with body and self contact and the
+                     quality mode's FEM constraints (Green strain over faces with moduli
+                     1e3, 1e3, 4e2 and Poisson ratios 0.3, isometric bending stiffness 1e-2
+                     over hinges) in place of the XPBD stretch and bend; roofline kernel
+                     WideStrainKernel, algorithmic bytes
+                     per launch 128 per face of the color (32 B element, three float4
+                     vertices read and written), the isometric bending launches beside it
+                     at 172 B per hinge)
                     (garment: 64 tube garments on capsule-chain bodies skinned on the GPU
                      every frame (LBS from joint transforms), bindings at the top ring and
                      waist, LRA, body contact; a step is the skinning plus one frame, timed
@@ -98,10 +107,10 @@ def cpu_baseline_and_parity(specs, params, seed, contacts, budget_s=12.0, before
     for i, sp in zip(insts, specs):
         oc = oracle_of(solver, i, sp.stretch_compliance, sp.bend_compliance,
                        x0=sp.x0, v=np.zeros_like(sp.x0))
-        if contacts:
+        if contacts or twin_setup is not None:
             edges = solver.edges(i, colors=False)[0]
-            obs = None if sp.obstacle is None else (np.asarray(sp.obstacle[0], np.float32),
-                                                    sp.obstacle[1])
+            obs = None if sp.obstacle is None or not contacts else (
+                np.asarray(sp.obstacle[0], np.float32), sp.obstacle[1])
             twin = osc.ContactScene(oc, sp.pos, sp.tris, edges, op, params, obstacle=obs)
             if twin_setup is not None:
                 twin_setup(n_inst, twin)
@@ -121,6 +130,8 @@ def cpu_baseline_and_parity(specs, params, seed, contacts, budget_s=12.0, before
     if contacts:
         what = ("with body contact and particle self-collision" if params.enable_particle_collision
                 else "with body and self contact")
+    if params.constraint_model == 1:
+        what += ", FEM constraints"
     base = {"value": done_vs / t_used, "unit": "vertex-substeps/s", "cores": 1, "kind": "port",
             "sample": f"one substep ({what}) of the first {n_inst} of {len(specs)} instances "
                       f"({done_vs} vertex-substeps, {t_used:.1f} s), serial FP64 oracle; "
@@ -180,7 +191,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "swift", "garment", "hanging"])
+                    choices=["batched", "swift", "quality", "garment", "hanging"])
     ap.add_argument("--garments", type=int, default=64, help="instances of --workload garment")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seed", type=int, default=4)
@@ -231,19 +242,29 @@ def main() -> int:
             for i, g in zip(insts, gs):
                 s.SetObstacleSkinning(i, *g[1].skinning(), g[1].amp.shape[0])
                 s.AddBindings(i, g[2], g[3], g[4])
-    elif a.workload in ("batched", "swift"):
+    elif a.workload in ("batched", "swift", "quality"):
         specs = scenes.batched(seed=a.seed, n_instances=a.instances)
         wname = f"batched {a.instances} cloths, side round(32*6^u) in [32,192], seed {a.seed}"
     else:
         specs = [scenes.hanging_cloth(seed=a.seed)]
         wname = "hanging cloth 64x64, 1 m, two corners pinned"
     swift = a.workload == "swift"
-    contacts = a.workload in ("batched", "swift", "garment") and not a.no_contacts
+    quality = a.workload == "quality"
+    contacts = a.workload in ("batched", "swift", "quality", "garment") and not a.no_contacts
     garment = a.workload == "garment"
+    # quality: moduli large enough that the strain projection acts from small strains on
+    # (it is skipped while sum_i w_i |grad_i|^2 <= 1e-6, which at the reference's default
+    # moduli of 1 holds up to strains of 4-20 % on these meshes), bending stiffness of the
+    # reference's quality example configuration
+    fem = dict(constraint_model=1, fem_xx_stiffness=1e3, fem_yy_stiffness=1e3,
+               fem_xy_stiffness=4e2, fem_bending_stiffness=1e-2) if quality else {}
     params = make_params(bend_model=1, ground_height=-1.0, enable_obstacle_contact=int(contacts),
                          enable_self_contact=int(contacts and not swift and not garment),
-                         enable_particle_collision=int(swift))
-    if swift:
+                         enable_particle_collision=int(swift), **fem)
+    if quality:
+        wname += (", sphere / capsule obstacle and self contact" if contacts else "") + \
+            ", FEM strain (moduli 1e3/1e3/4e2) and isometric bending (quality mode)"
+    elif swift:
         wname += (", sphere / capsule obstacle" if contacts else "") + \
             ", particle self-collision (swift mode)"
     elif contacts and not garment:
@@ -316,8 +337,24 @@ def main() -> int:
               f"before and {after / a.steps:.1f} after", file=sys.stderr, flush=True)
 
     # dominant kernel: per-launch device time from HIP events around each launch
-    rep_counts, particle, binding = None, None, None
-    if garment:
+    rep_counts, particle, binding, fem = None, None, None, None
+    if quality:
+        kernel = "WideStrainKernel"
+        strain_ms, iso_ms = solver.time_fem(2 * params.num_iterations)
+        faces = sum(solver.counts(i).nt for i in insts)
+        hinges = sum(solver.counts(i).n_bends for i in insts)
+        f_launch = max(solver.fem(i)["n_colors"] for i in insts)
+        b_launch = max(solver.counts(i).bend_colors for i in insts)
+        kern_ms = strain_ms
+        # one face color of every instance per launch: element (32 B) read, three vertices
+        # read and written (96 B), averaged over the colors
+        bytes_kern = 128 * faces / f_launch
+        # hinge: indices (16 B), weights K (16 B), coeff (4 B), multiplier read and written
+        # (8 B), four vertices read and written (128 B)
+        fem = {"strain_ms": strain_ms, "strain_launches_per_pass": f_launch, "faces": faces,
+               "iso_bend_ms": iso_ms, "iso_bend_launches_per_pass": b_launch, "hinges": hinges,
+               "iso_bend_gbs": 172 * hinges / b_launch / (iso_ms * 1e-3) / 1e9}
+    elif garment:
         kernel = "LbsKernel"
         kern_ms, bind_ms = solver.time_binding(20)
         bodies = sum(len(g[1].pos) for g in gs)
@@ -382,6 +419,17 @@ def main() -> int:
         if garment:
             def tw_setup(k, twin):
                 twin.setup_bindings(gs[k][2], gs[k][3], gs[k][4], params.bvh_tolerance)
+        elif quality:
+            from oracle import oracle as orc
+
+            def tw_setup(k, twin):
+                # the solver colors faces like every constraint set: seed + instance
+                col, ncol = orc.color(specs[k].tris, len(specs[k].pos), a.seed + k,
+                                      params.coloring)
+                twin.setup_fem(col, ncol, specs[k].pos,
+                               (params.fem_xx_stiffness, params.fem_yy_stiffness,
+                                params.fem_xy_stiffness, params.fem_xy_poisson_ratio,
+                                params.fem_yx_poisson_ratio), params.fem_bending_stiffness)
         cpu, parity, parity_n = cpu_baseline_and_parity(specs, params, a.seed, contacts,
                                                         a.cpu_budget, before, tw_setup)
         print(f"[bench] cpu {cpu['value']:.3e} vs/s, parity {parity:.3e}", file=sys.stderr,
@@ -412,6 +460,7 @@ def main() -> int:
                    "bend_model": "consistent", "obstacle_contact": contacts,
                    "self_contact": contacts and not swift and not garment,
                    "particle_collision": swift, "bindings": garment,
+                   "constraints": "FEM strain + isometric bending" if quality else "XPBD",
                    "skinning": "GPU LBS every frame" if garment else None,
                    "step": "one frame of all instances",
                    "graph": not a.no_graph},
@@ -423,7 +472,7 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel, "kernel_ms": kern_ms, "bytes_per_launch": bytes_kern,
                      "repulsion_counts": rep_counts, "particle": particle,
-                     "binding": binding},
+                     "binding": binding, "fem": fem},
         "record": record,
         "cpu_baseline": cpu,
         "adjoint_forward_ratio": adj["substep_ratio"] if adj else None,

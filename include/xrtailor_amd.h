@@ -266,6 +266,12 @@ int xrt_get_obstacle(xrt_ctx* ctx, int32_t inst, float* x0, float* x);
  * ms[0] / ms[1] = average device time per launch.  The cloth state is moved by the solves.
  * No counterpart in the reference (measurement hook). */
 int xrt_time_binding(xrt_ctx* ctx, int32_t n, float ms[2]);
+/* device time of the FEM kernels (constraint_model FEM): `n` passes over every face color
+ * of every instance (wide strain launches), then `n` over every bend color (wide isometric
+ * bending launches, pass r as iteration r), each launch bracketed by its own HIP event pair
+ * on the context stream; ms[0] / ms[1] = average device time per launch.  The cloth state is moved by the
+ * solves.  No counterpart in the reference (measurement hook). */
+int xrt_time_fem(xrt_ctx* ctx, int32_t n, float ms[2]);
 
 /* SpatialHashGPU::Hash(nodes, n) of one instance's predicted positions x
  * (xrtailor/include/xrtailor/physics/broad_phase/spatial_hash/SpatialHashGPU.cuh:26-28):

@@ -192,6 +192,9 @@ def test_fem_params_and_errors():
         np.testing.assert_array_equal(f[key], f2[key])
     with pytest.raises(XrtError):
         s.StrainSolver(i).Solve(f["n_colors"])  # color out of range
+    strain_ms, iso_ms = s.time_fem(3)  # the timing hook runs the wide launches
+    assert strain_ms > 0 and iso_ms > 0
+    assert np.isfinite(s.GetPredicted(i)).all()
     with pytest.raises(XrtError):
         s.AdjointBegin(1, 1)  # the adjoint has no FEM constraints
     s2.close()

@@ -112,6 +112,7 @@ SIGNATURES = {
     "xrt_strain_solve": [_P, _I, _I],
     "xrt_isometric_bending_solve": [_P, _I, _I, _I, _F],
     "xrt_get_fem": [_P, _I, _IP, _IP, _FP, _FP, _FP, _FP],
+    "xrt_time_fem": [_P, _I, _FP],
     "xrt_solve_lra": [_P, _I],
     "xrt_solve_ground": [_P, _I],
     "xrt_apply_deltas": [_P, _I],

@@ -314,4 +314,42 @@ int xrt_get_fem(xrt_ctx* ctx, int32_t inst, int32_t* face_colors, int32_t* n_fac
   return XRT_OK;
 }
 
+int xrt_time_fem(xrt_ctx* ctx, int32_t n, float ms[2]) {
+  XRT_REQUIRE(ctx && ms && n > 0, XRT_ERR_INVALID_ARG, "null argument or n <= 0");
+  XRT_REQUIRE(ctx->started && FemOn(ctx) && ctx->fem.ready, XRT_ERR_STATE,
+              "constraint_model is not FEM");
+  const FemData& F = ctx->fem;
+  const WideConfig& W = ctx->wide;
+  const FemConsts k = MakeFemConsts(ctx->params, SubstepDt(ctx->params));
+  const FemArgs a = MakeFemArgs(ctx);
+  const int ni = static_cast<int>(ctx->inst.size());
+  hipStream_t s = ctx->stream;
+  // every launch bracketed by its own event pair (the gaps between launches excluded)
+  double sum[2] = {0.0, 0.0};
+  int cnt[2] = {0, 0};
+  for (int kind = 0; kind < 2; ++kind)
+    for (int r = 0; r < n; ++r) {
+      const int ncol = kind == 0 ? static_cast<int>(F.f_blocks.size())
+                                 : static_cast<int>(W.b_blocks.size());
+      for (int col = 0; col < ncol; ++col) {
+        XRT_CHECK_HIP(hipEventRecord(ctx->ev0, s));
+        if (kind == 0)
+          XRT_CHECK_HIP(LaunchWideStrain(s, a, ni, col, F.maps.ptr + F.f_map[col],
+                                         F.f_blocks[col], k));
+        else
+          XRT_CHECK_HIP(LaunchWideIsoBend(s, a, ni, col, W.maps.ptr + W.b_map[col],
+                                          W.b_blocks[col], r, k));
+        XRT_CHECK_HIP(hipEventRecord(ctx->ev1, s));
+        XRT_CHECK_HIP(hipEventSynchronize(ctx->ev1));
+        float t = 0.0f;
+        XRT_CHECK_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+        sum[kind] += t;
+        cnt[kind]++;
+      }
+    }
+  ms[0] = static_cast<float>(sum[0] / std::max(cnt[0], 1));
+  ms[1] = static_cast<float>(sum[1] / std::max(cnt[1], 1));
+  return XRT_OK;
+}
+
 }  // extern "C"

@@ -520,6 +520,14 @@ class ClothSolver:
         check(self.lib.xrt_time_binding(self.ctx, n, fptr(ms)))
         return float(ms[0]), float(ms[1])
 
+    def time_fem(self, n: int) -> tuple[float, float]:
+        """ms per launch of the wide FEM strain launches (one face color of every instance)
+        and of the wide isometric bending launches, over n passes of every color each (the
+        state moves)."""
+        ms = np.zeros(2, np.float32)
+        check(self.lib.xrt_time_fem(self.ctx, n, fptr(ms)))
+        return float(ms[0]), float(ms[1])
+
     def RecordFrames(self, n: int, normals: bool = False):
         """n batched frames with the per-frame cache: positions (n, N, 3) of every instance
         in instance order (and their vertex normals when asked)."""
