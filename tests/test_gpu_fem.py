@@ -112,11 +112,13 @@ def test_one_pass_matches_oracle():
     s.close()
 
 
-@pytest.mark.parametrize("scene", ["sheet", "sphere"])
-def test_substeps_match_oracle_twin(scene):
-    """Plain substeps with the FEM constraints (pinned sheet under gravity; a sheet dropped
-    on an icosphere with body contact) against the FP64 twin."""
-    print(f"case FEM substeps vs oracle twin: {scene}", flush=True)
+@pytest.mark.parametrize("scene,path", [("sheet", "plain"), ("sheet", "batched"),
+                                        ("sphere", "plain"), ("sphere", "batched")])
+def test_substeps_match_oracle_twin(scene, path):
+    """Substeps with the FEM constraints (pinned sheet under gravity: the batched path is
+    the persistent kernel; a sheet dropped on an icosphere with body contact: the wide
+    substep) against the FP64 twin."""
+    print(f"case FEM substeps vs oracle twin: {scene}, {path}", flush=True)
     if scene == "sheet":
         spec, v = _sheet(), None
         xp = _params()
@@ -129,7 +131,10 @@ def test_substeps_match_oracle_twin(scene):
     twin = _twin(spec, xp, s, i)
     errs = []
     for _ in range(4):
-        s.SubstepPlain(i)
+        if path == "plain":
+            s.SubstepPlain(i)
+        else:
+            s.SubstepBatched()
         twin.substep()
         errs.append(rel_err(s.GetState(i)[0], twin.c.state()[0]))
     print(f"  rel err per substep {', '.join(f'{e:.2e}' for e in errs)}", flush=True)
@@ -140,8 +145,10 @@ def test_substeps_match_oracle_twin(scene):
 
 @pytest.mark.parametrize("obstacle", [0, 1])
 def test_batched_equals_plain(obstacle):
-    """Instances of three sizes: graph-captured batched frames (wide FEM launches) equal
-    plain frames bit for bit."""
+    """Instances of three sizes: graph-captured batched frames equal plain frames bit for
+    bit.  With body contact the batched substep is the wide one (every color of every
+    instance per launch); without contacts it is the persistent kernel (one workgroup per
+    instance, the iterate in LDS)."""
     print(f"case FEM batched vs plain, obstacle {obstacle}", flush=True)
     if obstacle:
         cases = [sphere_drop(n=n, seed=k) for k, n in enumerate((12, 16, 20))]

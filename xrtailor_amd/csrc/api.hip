@@ -130,6 +130,17 @@ BatchArgs MakeBatchArgs(xrt_ctx* c) {
   a.lra_rest = c->lra_rest.ptr;
   a.lds_verts = c->lds_verts;
   a.max_slots = c->max_slots;
+  if (FemOn(c) && c->fem.ready) {
+    const FemData& F = c->fem;
+    a.fem = 1;
+    a.fcol = F.fcol.ptr;
+    a.f_coloff = F.f_coloff.ptr;
+    a.fsc = F.sc.ptr;
+    a.iso_k = F.iso_k.ptr;
+    a.iso_c = F.iso_c.ptr;
+    a.iso_lam = F.iso_lam.ptr;
+    a.fk = FemConstsOf(c->params);
+  }
   return a;
 }
 
@@ -173,12 +184,13 @@ int SubstepPlain(xrt_ctx* c, int32_t inst) {
   return ParticleSubstepEnd(c, inst);
 }
 
-// Without contacts: one persistent launch.  With body or self contact, particle collision,
-// bindings or the FEM constraints: the wide substep (each color, contact phase, hash step,
-// binding pass and LBVH operation of every instance in one launch).
+// Without contacts: one persistent launch (XPBD or FEM constraints).  With body or self
+// contact, particle collision or bindings: the wide substep (each color, contact phase, hash
+// step, binding pass and LBVH operation of every instance in one launch).
 int SubstepBatched(xrt_ctx* c) {
   const StepConsts k = MakeConsts(c->params);
-  if (ContactsActive(c) || ParticlesOn(c) || BindingsOn(c) || FemOn(c))
+  XRT_REQUIRE(!FemOn(c) || c->fem.ready, XRT_ERR_STATE, "FEM constraints are not set up");
+  if (ContactsActive(c) || ParticlesOn(c) || BindingsOn(c))
     return WideSubstep(c, k, ModeOf(k), nullptr, -1);
   XRT_CHECK_HIP(LaunchBatchedSubstep(c->stream, MakeBatchArgs(c), k,
                                      static_cast<int>(c->inst.size()), c->batched_block,

@@ -331,6 +331,7 @@ int BindingIter(xrt_ctx* ctx, int inst);
 // constraint model is FEM (xrt_start, xrt_set_params); FemIter runs one iteration's strain
 // and isometric bending colors of an instance, or of every instance (-1)
 bool FemOn(const xrt_ctx* ctx);
+FemConsts FemConstsOf(const xrt_params& p);  // at the substep dt
 int FemPrepare(xrt_ctx* ctx);
 int FemIter(xrt_ctx* ctx, int inst, int it);
 // adjoint_api.hip: one substep of every instance as wide launches with the contact phases

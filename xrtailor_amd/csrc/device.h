@@ -79,6 +79,17 @@ struct BendMode {
   float dlo, th0, th1, sgn;
 };
 
+struct StrainC {  // FEM::StrainConstraint (FEMConstraint.cuh:8-19) without the material
+  int32_t i0, i1, i2;
+  float area;
+  float inv[4];  // glm::inverse of the rest matrix: [0][0], [0][1], [1][0], [1][1]
+};
+
+struct FemConsts {
+  float c00, c01, c10, c11, c22;  // orthotropic elasticity entries (FEMHelper.cu:103-111)
+  float iso_at;                   // 1 / (bending stiffness dt^2)
+};
+
 struct BatchArgs {
   const InstDesc* inst;
   const int32_t* order;  // workgroup -> instance (largest first)
@@ -97,6 +108,17 @@ struct BatchArgs {
   const float* lra_rest;
   int32_t lds_verts;   // instances with nv <= lds_verts keep x in LDS
   int32_t max_slots;   // LDS staging for LRA slot positions
+  // quality-mode FEM constraints in place of stretch and bend when fem != 0: per instance
+  // (first face color offset, face colors), the global face color offsets, the strain
+  // elements, the isometric bending data in the bend elements' order
+  int32_t fem;
+  const int2* fcol;
+  const int32_t* f_coloff;
+  const StrainC* fsc;
+  const float4* iso_k;
+  const float* iso_c;
+  float* iso_lam;
+  FemConsts fk;
 };
 
 }  // namespace xrt

@@ -9,8 +9,21 @@
 //               batches; the iterate x lives in LDS when the instance fits.
 #include "dynamics.h"
 #include "elements.h"
+#include "fem.h"
 
 namespace xrt {
+
+// PredictPositions (ClothSolverHelper.cu:125-131): v += g dt for free vertices, x = x0 + v dt,
+// each written as one fused multiply-add so that every kernel that inlines it rounds the
+// same way (a product hoisted out of a loop would otherwise round separately)
+__device__ __forceinline__ float4 PredictVertex(const float4 p, float4& vi, const StepConsts& k) {
+  if (p.w > kEps) {
+    vi.x = fmaf(k.gx, k.dt, vi.x);
+    vi.y = fmaf(k.gy, k.dt, vi.y);
+    vi.z = fmaf(k.gz, k.dt, vi.z);
+  }
+  return make_float4(fmaf(vi.x, k.dt, p.x), fmaf(vi.y, k.dt, p.y), fmaf(vi.z, k.dt, p.z), p.w);
+}
 
 // ------------------------------------------------------------- plain kernels
 __global__ __launch_bounds__(256) void PredictKernel(const float4* __restrict__ x0,
@@ -18,11 +31,9 @@ __global__ __launch_bounds__(256) void PredictKernel(const float4* __restrict__ 
                                                      float4* __restrict__ x, int n, StepConsts k) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float4 p = x0[i];
   float4 vi = v[i];
-  if (p.w > kEps) { vi.x += k.gx * k.dt; vi.y += k.gy * k.dt; vi.z += k.gz * k.dt; }
+  x[i] = PredictVertex(x0[i], vi, k);
   v[i] = vi;
-  x[i] = make_float4(p.x + vi.x * k.dt, p.y + vi.y * k.dt, p.z + vi.z * k.dt, p.w);
 }
 
 __global__ __launch_bounds__(256) void StretchKernel(const StretchC* __restrict__ sc,
@@ -86,7 +97,7 @@ __global__ __launch_bounds__(256) void FinalizeKernel(float4* __restrict__ x0,
 
 // ------------------------------------------------------------ batched kernel
 template <bool kLds>
-__device__ __forceinline__ void SubstepBody(const BatchArgs& a, const InstDesc& d,
+__device__ __forceinline__ void SubstepBody(const BatchArgs& a, const InstDesc& d, int inst,
                                             const StepConsts& k, float4* lds_x,
                                             float4* lds_slots) {
   const int tid = threadIdx.x, bs = blockDim.x;
@@ -98,11 +109,9 @@ __device__ __forceinline__ void SubstepBody(const BatchArgs& a, const InstDesc& 
 
   // predict
   for (int i = tid; i < nv; i += bs) {
-    const float4 p = gx0[i];
     float4 vi = gv[i];
-    if (p.w > kEps) { vi.x += k.gx * k.dt; vi.y += k.gy * k.dt; vi.z += k.gz * k.dt; }
+    X[i] = PredictVertex(gx0[i], vi, k);
     gv[i] = vi;
-    X[i] = make_float4(p.x + vi.x * k.dt, p.y + vi.y * k.dt, p.z + vi.z * k.dt, p.w);
   }
   __syncthreads();
 
@@ -111,12 +120,26 @@ __device__ __forceinline__ void SubstepBody(const BatchArgs& a, const InstDesc& 
   const int32_t* lptr = a.lra_ptr + d.lra_vbeg;
   // constraints per thread per round: more gathers in flight where x lives in global memory
   constexpr int kUs = kLds ? 2 : 4, kUb = kLds ? 1 : 2;
+  const int2 fc = a.fem ? a.fcol[inst] : make_int2(0, 0);
   for (int it = 0; it < k.iterations; ++it) {
-    for (int c = 0; c < d.s_ncol; ++c) {
+    if (a.fem) {  // quality mode: FEM strain colors, then isometric bending colors
+      for (int c = 0; c < fc.y; ++c) {
+        const int end = a.f_coloff[fc.x + c + 1];
+        for (int e = a.f_coloff[fc.x + c] + tid; e < end; e += bs) StrainProject(X, a.fsc[e], a.fk);
+        __syncthreads();
+      }
+      if (k.solve_bending)
+        for (int c = 0; c < d.b_ncol; ++c) {
+          for (int e = bco[c] + tid; e < bco[c + 1]; e += bs)
+            IsoProject(X, a.bidx[e], a.iso_k[e], a.iso_c[e], a.iso_lam + e, it, a.fk);
+          __syncthreads();
+        }
+    }
+    for (int c = 0; c < (a.fem ? 0 : d.s_ncol); ++c) {
       StretchColor<kUs>(X, a.sc, a.slam, sco[c], sco[c + 1], k.inv_dt2, it, tid, bs);
       __syncthreads();
     }
-    if (k.solve_bending) {
+    if (k.solve_bending && !a.fem) {
       const BendMode m{k.bend_dlo, k.bend_th0, k.bend_th1, k.bend_sgn};
       for (int c = 0; c < d.b_ncol; ++c) {
         BendColor<kUb>(X, a.bidx, a.bpar, bco[c], bco[c + 1], k.inv_dt2, m, tid, bs);
@@ -155,12 +178,13 @@ __device__ __forceinline__ void SubstepBody(const BatchArgs& a, const InstDesc& 
 
 __global__ __launch_bounds__(1024) void BatchedSubstepKernel(BatchArgs a, StepConsts k) {
   extern __shared__ float4 lds[];
-  const InstDesc d = a.inst[a.order[blockIdx.x]];
+  const int inst = a.order[blockIdx.x];
+  const InstDesc d = a.inst[inst];
   float4* lds_slots = lds + a.lds_verts;
   if (d.nv <= a.lds_verts)
-    SubstepBody<true>(a, d, k, lds, lds_slots);
+    SubstepBody<true>(a, d, inst, k, lds, lds_slots);
   else
-    SubstepBody<false>(a, d, k, lds, lds_slots);
+    SubstepBody<false>(a, d, inst, k, lds, lds_slots);
 }
 
 // ------------------------------------------------------------------ launchers

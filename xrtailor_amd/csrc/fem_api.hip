@@ -137,6 +137,8 @@ int CheckFem(xrt_ctx* c, int32_t inst) {
 
 bool FemOn(const xrt_ctx* ctx) { return ctx->params.constraint_model == XRT_CONSTRAINTS_FEM; }
 
+FemConsts FemConstsOf(const xrt_params& p) { return MakeFemConsts(p, SubstepDt(p)); }
+
 int FemPrepare(xrt_ctx* ctx) {
   FemData& F = ctx->fem;
   if (!FemOn(ctx) || F.ready) return XRT_OK;
