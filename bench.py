@@ -34,7 +34,8 @@ cpu_baseline the serial FP64 oracle on one core, one substep per instance over a
                      WideStrainKernel, algorithmic bytes
                      per launch 128 per face of the color (32 B element, three float4
                      vertices read and written), the isometric bending launches beside it
-                     at 172 B per hinge)
+                     at 172 B per hinge; with --no-contacts the persistent kernel, 64 N +
+                     32 F + 44 hinges + 12 K per launch)
                     (garment: 64 tube garments on capsule-chain bodies skinned on the GPU
                      every frame (LBS from joint transforms), bindings at the top ring and
                      waist, LRA, body contact; a step is the skinning plus one frame, timed
@@ -359,6 +360,13 @@ def main() -> int:
         fem = {"strain_ms": strain_ms, "strain_launches_per_pass": f_launch, "faces": faces,
                "iso_bend_ms": iso_ms, "iso_bend_launches_per_pass": b_launch, "hinges": hinges,
                "iso_bend_gbs": 172 * hinges / b_launch / (iso_ms * 1e-3) / 1e9}
+        if not contacts:
+            # the frame is the persistent kernel: state read + written (64 B per vertex),
+            # strain elements 32 B, isometric data 44 B per hinge, LRA 12 B, read once
+            kernel = "BatchedSubstepKernel"
+            kern_ms = solver.time_batched_substep(max(10, a.steps))
+            lra = sum(solver.counts(i).lra_constraints for i in insts)
+            bytes_kern = 64 * n_verts + 32 * faces + 44 * hinges + 12 * lra
     elif garment:
         kernel = "LbsKernel"
         kern_ms, bind_ms = solver.time_binding(20)

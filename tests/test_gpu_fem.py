@@ -2,6 +2,8 @@
 oracle, through the C ABI: the rest data against the FP64 restatement, one pass of every
 color against the oracle's projections from the same FP32 state, whole substeps against the
 FP64 twin (with and without body contact), batched == plain bit for bit."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -138,6 +140,35 @@ def test_substeps_match_oracle_twin(scene, path):
         twin.substep()
         errs.append(rel_err(s.GetState(i)[0], twin.c.state()[0]))
     print(f"  rel err per substep {', '.join(f'{e:.2e}' for e in errs)}", flush=True)
+    assert errs[0] <= TOL
+    assert max(errs) <= 1e-4
+    s.close()
+
+
+@pytest.mark.parametrize("path", ["plain", "batched"])
+def test_lra_and_palette_coloring_match_oracle_twin(path):
+    """FEM constraints with geodesic long-range attachments (slots along the pinned edge)
+    and the palette coloring of faces and hinges, against the FP64 twin."""
+    print(f"case FEM + LRA + palette coloring vs oracle twin: {path}", flush=True)
+    n = 16
+    spec = dataclasses.replace(_sheet(n=n), lra_slots=np.arange(0, n, 3, dtype=np.int32))
+    xp = _params(coloring=1, long_range_stretchiness=1.05)
+    s, (i,) = build([spec], xp, seed=5)
+    f = s.fem(i)
+    col, ncol = orc.color(spec.tris, len(spec.pos), 5 + i, 1)
+    np.testing.assert_array_equal(f["colors"], col)
+    assert f["n_colors"] == ncol and s.counts(i).lra_constraints > 0
+    twin = _twin(spec, xp, s, i)
+    errs = []
+    for _ in range(4):
+        if path == "plain":
+            s.SubstepPlain(i)
+        else:
+            s.SubstepBatched()
+        twin.substep()
+        errs.append(rel_err(s.GetState(i)[0], twin.c.state()[0]))
+    print(f"  {ncol} face colors; rel err per substep {', '.join(f'{e:.2e}' for e in errs)}",
+          flush=True)
     assert errs[0] <= TOL
     assert max(errs) <= 1e-4
     s.close()

@@ -119,19 +119,18 @@ __device__ __forceinline__ void SubstepBody(const BatchArgs& a, const InstDesc& 
   const int32_t* bco = a.b_coloff + d.b_cbeg;
   const int32_t* lptr = a.lra_ptr + d.lra_vbeg;
   // constraints per thread per round: more gathers in flight where x lives in global memory
-  constexpr int kUs = kLds ? 2 : 4, kUb = kLds ? 1 : 2;
+  constexpr int kUs = kLds ? 2 : 4, kUb = kLds ? 1 : 2, kUf = 4;
   const int2 fc = a.fem ? a.fcol[inst] : make_int2(0, 0);
   for (int it = 0; it < k.iterations; ++it) {
     if (a.fem) {  // quality mode: FEM strain colors, then isometric bending colors
       for (int c = 0; c < fc.y; ++c) {
-        const int end = a.f_coloff[fc.x + c + 1];
-        for (int e = a.f_coloff[fc.x + c] + tid; e < end; e += bs) StrainProject(X, a.fsc[e], a.fk);
+        StrainColor<kUf>(X, a.fsc, a.f_coloff[fc.x + c], a.f_coloff[fc.x + c + 1], a.fk, tid, bs);
         __syncthreads();
       }
       if (k.solve_bending)
         for (int c = 0; c < d.b_ncol; ++c) {
-          for (int e = bco[c] + tid; e < bco[c + 1]; e += bs)
-            IsoProject(X, a.bidx[e], a.iso_k[e], a.iso_c[e], a.iso_lam + e, it, a.fk);
+          IsoColor<kUf>(X, a.bidx, a.iso_k, a.iso_c, a.iso_lam, bco[c], bco[c + 1], it, a.fk, tid,
+                        bs);
           __syncthreads();
         }
     }

@@ -140,6 +140,50 @@ __device__ __forceinline__ void IsoProject(P X, const int4 id, const float4 K4, 
   *lam_ptr = lam + dl;
 }
 
+// One color of strain elements [beg, end) inside a workgroup (the persistent kernel): U
+// elements per thread per round, their element records loaded before any is projected so
+// that the loads are in flight together (elements of a color are independent, so the
+// results do not depend on U).
+template <int U, typename P>
+__device__ __forceinline__ void StrainColor(P X, const StrainC* __restrict__ sc, int beg, int end,
+                                            const FemConsts& k, int tid, int bs) {
+  for (int e0 = beg + tid; e0 < end; e0 += U * bs) {
+    StrainC c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (e0 + u * bs < end) c[u] = sc[e0 + u * bs];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (e0 + u * bs < end) StrainProject(X, c[u], k);
+  }
+}
+
+// One color of isometric bending elements [beg, end), U per thread per round (as above).
+template <int U, typename P>
+__device__ __forceinline__ void IsoColor(P X, const int4* __restrict__ bidx,
+                                         const float4* __restrict__ iso_k,
+                                         const float* __restrict__ iso_c, float* __restrict__ lam,
+                                         int beg, int end, int iter, const FemConsts& k, int tid,
+                                         int bs) {
+  for (int e0 = beg + tid; e0 < end; e0 += U * bs) {
+    int4 id[U];
+    float4 K[U];
+    float cf[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * bs;
+      if (e < end) {
+        id[u] = bidx[e];
+        K[u] = iso_k[e];
+        cf[u] = iso_c[e];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (e0 + u * bs < end) IsoProject(X, id[u], K[u], cf[u], lam + e0 + u * bs, iter, k);
+  }
+}
+
 // one color of one instance
 hipError_t LaunchStrain(hipStream_t s, const StrainC* sc, float4* x, int cnt, const FemConsts& k);
 hipError_t LaunchIsoBend(hipStream_t s, const int4* bidx, const float4* iso_k, const float* iso_c,
