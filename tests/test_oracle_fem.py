@@ -95,3 +95,58 @@ def test_isometric_weights_energy_gradient_and_step():
         lam = np.zeros(1)
         fem.iso_solve(x.reshape(-1, 3), np.ones(4), B[None], K, c, lam, 0, 1, 0, 1e-3, 1e12)
         assert fem.iso_energy(x.reshape(-1, 3), B, K[0], c[0])[0] < e
+
+
+def _flat_sheet(n=10, side=0.3, seed=2):
+    from xrtailor_amd import scenes
+    pos, tris = scenes.grid(n, side, 0.0)
+    rng = np.random.default_rng(seed)
+    h = side / (n - 1)
+    pos[:, [0, 2]] += 0.25 * h * rng.uniform(-1.0, 1.0, (len(pos), 2))  # in plane only
+    return pos, tris
+
+
+def _fem_twin(pos, tris, x0, E, **kw):
+    from contact_helpers import oracle_scene
+    from oracle import oracle as orc
+    from xrtailor_amd import scenes
+    from xrtailor_amd.solver import make_params
+    xp = make_params(constraint_model=1, gravity=(0.0, 0.0, 0.0), enable_ground=0, **kw)
+    spec = scenes.ClothSpec(pos, tris, np.zeros(0, np.int32), 1e-7, 1e-3, x0.astype(np.float32))
+    twin = oracle_scene(spec, xp, seed=0)
+    col, ncol = orc.color(tris, len(pos), 0, 0)
+    twin.setup_fem(col, ncol, pos, E, 1.0)
+    return twin
+
+
+def test_scene_fem_rest_under_rigid_motion_is_a_fixed_point():
+    """The twin's FEM iteration (face colors, strain and isometric bending wiring) leaves a
+    flat sheet at rest, moved rigidly, exactly in place without gravity."""
+    pos, tris = _flat_sheet()
+    R, t = _rigid(np.random.default_rng(3))
+    x0 = (pos @ R.T + t).astype(np.float32)
+    twin = _fem_twin(pos, tris, x0, (1.0, 1.0, 1.0, 0.3, 0.3))
+    start = twin.c.state()[0].copy()
+    for _ in range(3):
+        twin.substep()
+    np.testing.assert_allclose(twin.c.state()[0], start, rtol=0, atol=1e-9)
+
+
+def test_scene_fem_strain_relaxes_a_stretched_sheet():
+    """A sheet stretched by 10 % along x (moduli large enough for the projection to act)
+    contracts toward its rest lengths under the twin's FEM iterations (mean edge strain;
+    single edges overshoot while the velocities the projections create carry over)."""
+    pos, tris = _flat_sheet()
+    x0 = pos * np.array([1.1, 1.0, 1.0])
+    twin = _fem_twin(pos, tris, x0, (1e3, 1e3, 4e2, 0.3, 0.3), solve_bending=0)
+
+    def strain(x):
+        e = np.unique(np.sort(np.concatenate([tris[:, [0, 1]], tris[:, [1, 2]], tris[:, [2, 0]]]), axis=1), axis=0)
+        L0 = np.linalg.norm(pos[e[:, 0]] - pos[e[:, 1]], axis=1)
+        return np.abs(np.linalg.norm(x[e[:, 0]] - x[e[:, 1]], axis=1) / L0 - 1.0).mean()
+
+    s0 = strain(twin.c.state()[0])
+    for _ in range(2):
+        twin.substep()
+    s1 = strain(twin.c.state()[0])
+    assert s0 > 0.045 and s1 < 0.5 * s0
