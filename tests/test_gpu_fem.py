@@ -237,3 +237,66 @@ def test_fem_params_and_errors():
         s.AdjointBegin(1, 1)  # the adjoint has no FEM constraints
     s2.close()
     s.close()
+
+
+@pytest.mark.parametrize("path", ["plain", "batched"])
+def test_fem_with_particle_collision_matches_oracle_twin(path):
+    """FEM constraints with the swift mode's particle self-collision on (hash after predict,
+    collision at each iteration start, then the FEM colors), against the FP64 twin, at the
+    reference's default bending stiffness (1).  Open item: at bending stiffness 50 this
+    scene's first substep parts from the twin by up to 7e-5 m on a few vertices once it
+    runs ten iterations (eight: 5e-8), while the twin's own response to FP32-level
+    perturbations of its start state stays at 4e-9 and its skip thresholds are not near;
+    parity is not claimed for that combination."""
+    from test_gpu_particle import particle_layers
+    print(f"case FEM + particle self-collision vs oracle twin: {path}", flush=True)
+    spec, state, v = particle_layers()
+    xp = _params(enable_particle_collision=1, ground_friction=0.1, fem_xx_stiffness=1e3,
+                 fem_yy_stiffness=1e3, fem_xy_stiffness=4e2, fem_bending_stiffness=1.0)
+    s, (i,) = build([spec], xp, seed=1)
+    s.SetState(i, state, v)
+    twin = oracle_scene(spec, xp, solver=s, inst=i)
+    twin.p_orig = spec.x0.astype(np.float32)  # the state at xrt_start
+    f = s.fem(i)
+    twin.setup_fem(f["colors"], f["n_colors"], spec.pos, (1e3, 1e3, 4e2, E[3], E[4]),
+                   xp.fem_bending_stiffness)
+    errs = []
+    for _ in range(3):
+        if path == "plain":
+            s.SubstepPlain(i)
+        else:
+            s.SubstepBatched()
+        twin.substep()
+        errs.append(rel_err(s.GetState(i)[0], twin.c.state()[0]))
+    print(f"  rel err per substep {', '.join(f'{e:.2e}' for e in errs)}", flush=True)
+    assert errs[0] <= TOL
+    assert max(errs) <= 1e-4
+    s.close()
+
+
+def test_fem_with_bindings_on_a_skinned_body_matches_oracle_twin():
+    """A tube garment bound to a GPU-skinned capsule-chain body (bindings, LRA, body
+    contact) with the FEM constraints, against the FP64 twin."""
+    from test_gpu_binding import _build as build_garments, _garment
+    print("case FEM + bindings + skinned body vs oracle twin", flush=True)
+    case = _garment()
+    spec, body, bind, stiff, dist = case
+    xp = _params(enable_obstacle_contact=1, ground_height=-1.0, fem_xx_stiffness=1e3,
+                 fem_yy_stiffness=1e3, fem_xy_stiffness=4e2)
+    s, (i,) = build_garments([case], xp)
+    twin = oracle_scene(spec, xp, solver=s, inst=i)
+    twin.setup_bindings(bind, stiff, dist, xp.bvh_tolerance)
+    f = s.fem(i)
+    twin.setup_fem(f["colors"], f["n_colors"], spec.pos, (1e3, 1e3, 4e2, E[3], E[4]),
+                   xp.fem_bending_stiffness)
+    s.SkinObstacle(i, body.transforms(1.0 / 60.0))
+    twin.o_x = s.GetObstacle(i, len(body.pos))[1].copy()
+    errs = []
+    for _ in range(3):
+        s.SubstepPlain(i)
+        twin.substep()
+        errs.append(rel_err(s.GetState(i)[0], twin.c.state()[0]))
+    print(f"  rel err per substep {', '.join(f'{e:.2e}' for e in errs)}", flush=True)
+    assert errs[0] <= TOL
+    assert max(errs) <= 1e-4
+    s.close()
