@@ -14,6 +14,12 @@
 
 namespace xrt {
 
+// wide launches: 256-thread blocks; the strain launch takes two faces per thread
+// (kFemPerBlock per block), the isometric bending launch one hinge per thread and the
+// bend elements' block maps
+constexpr int kFemBlock = 256;
+constexpr int kFemPerBlock = 2 * kFemBlock;
+
 struct FemArgs {
   const InstDesc* inst;
   const int2* fcol;       // per instance: first face color offset, face colors
@@ -189,7 +195,7 @@ hipError_t LaunchStrain(hipStream_t s, const StrainC* sc, float4* x, int cnt, co
 hipError_t LaunchIsoBend(hipStream_t s, const int4* bidx, const float4* iso_k, const float* iso_c,
                          float* lam, float4* x, int cnt, int iter, const FemConsts& k);
 // one color of every instance (per-color block map: n_inst + 1 prefix sums, then the
-// instance of every block)
+// instance of every block; kFemPerBlock faces or kFemBlock hinges per block)
 hipError_t LaunchWideStrain(hipStream_t s, const FemArgs& a, int n_inst, int color,
                             const int32_t* map, int blocks, const FemConsts& k);
 hipError_t LaunchWideIsoBend(hipStream_t s, const FemArgs& a, int n_inst, int color,

@@ -7,7 +7,8 @@
 // host in FP64 and stored in FP32.  Faces are colored like every other constraint set (the
 // context's coloring mode and seed), then stored color-sorted per instance; the isometric
 // bending data share the bend elements' color order, so the bend colors and the wide bend
-// block maps serve both constraint models.
+// block maps serve both constraint models (the wide strain launch, two faces per thread, has
+// block maps of its own).
 #include <algorithm>
 #include <cmath>
 
@@ -169,7 +170,7 @@ int FemPrepare(xrt_ctx* ctx) {
     for (int p = 0; p < h.nb(); ++p)
       IsoRest(h.rest_pos.data(), &h.bends[4 * h.b_order[p]], &iso_k[h.bbeg + p], &iso_c[h.bbeg + p]);
   // per-color block maps of the wide strain launches (n + 1 prefix sums, then the instance
-  // of every block; only blocks with faces are launched)
+  // of every block; kFemPerBlock faces per block; only blocks with faces are launched)
   int max_col = 0;
   for (const auto& fi : F.inst) max_col = std::max(max_col, fi.ncol);
   std::vector<int32_t> maps;
@@ -180,7 +181,7 @@ int FemPrepare(xrt_ctx* ctx) {
     for (int i = 0; i < ni; ++i) {
       const FemData::Inst& fi = F.inst[i];
       const int cnt = col < fi.ncol ? fi.coloff[col + 1] - fi.coloff[col] : 0;
-      o[i + 1] = o[i] + (cnt + 255) / 256;
+      o[i + 1] = o[i] + (cnt + kFemPerBlock - 1) / kFemPerBlock;
     }
     F.f_map[col] = maps.size();
     F.f_blocks[col] = o[ni];
