@@ -1,5 +1,13 @@
 """Pins of the contact oracle (CPU only): narrow-phase routines against dense sampling and
-closed forms, R-triangle ownership, single-contact response closed forms."""
+closed forms, R-triangle ownership, single-contact response closed forms, the PBD repulsion
+responses against central differences of the true distance functions, and the EE-DCD solve
+on hand-computed numbers.
+
+The repulsion and EE-DCD pins were checked against mistakes planted in a scratch copy of
+the oracle: a transposed cross product in dcp2, an inverse mass dropped from a correction
+or a denominator, a missing side flip, a normalised edge-edge normal, swapped barycentric
+weights, swapped (s, 1 - s), a dropped 1e-3 offset and a dropped relaxation factor each
+fail at least one of them."""
 import ctypes as C
 
 import numpy as np
@@ -163,3 +171,285 @@ def test_ccd_catches_a_vertex_that_tunnels_through():
     lib.orc_vf_solve(lib.orc_bvh_pred(b.h), otris.ctypes.data_as(_IP), face.ctypes.data_as(_IP),
                      q.ctypes.data_as(_D), xx.ctypes.data_as(_D), 1, 0.0)
     assert xx[1] == pytest.approx(0.003, abs=1e-7)
+
+
+# ------------------------------------------------------------- PBD repulsion responses
+# EvaluateVFProximity / EvaluateEEProximity (PBDRepulsion.cu:329-468) over one face pair,
+# with the R-triangle bits chosen so that exactly one VF or one EE test runs.  The expected
+# corrections come from central differences of the true point-plane / segment-segment
+# distance functions written here in numpy (not from the oracle's gradient terms), so a
+# transposed cross product in dcp2 / dcp3, a dropped inverse mass or relaxation factor or
+# a wrong side flip moves the result by O(1) of the correction and fails.
+
+def _rmesh(x, x0, w, free, tris, rtri):
+    x = np.array(x, np.float64).reshape(-1)            # copies: the apply step writes x
+    x0 = np.array(x0, np.float64).reshape(-1)
+    tris = np.ascontiguousarray(tris, np.int32)
+    edges = orc.build_edges(tris)
+    fe, ptr, ef, _, fn = osc.contact_topology(x0.reshape(-1, 3), tris, edges)
+    keep = dict(x=x, x0=x0, w=np.ascontiguousarray(w, np.float64),
+                free=np.ascontiguousarray(free, np.uint8), tris=tris.reshape(-1),
+                edges=np.ascontiguousarray(edges.reshape(-1), np.int32), fe=fe, ptr=ptr, ef=ef,
+                fn=fn, rtri=np.ascontiguousarray(rtri, np.uint32))
+    m = osc.RMeshIn(_d(keep["x"]), _d(keep["x0"]), _d(keep["w"]),
+                    keep["free"].ctypes.data_as(C.POINTER(C.c_uint8)), osc._i(keep["tris"]),
+                    osc._i(keep["edges"]), osc._i(fe), osc._i(ptr), osc._i(ef), _d(fn),
+                    keep["rtri"].ctypes.data_as(C.POINTER(C.c_uint32)))
+    return m, keep, edges, fe
+
+
+def _repulse(m, keep, pairs, kind, h, nv, relax):
+    """One EvaluateRep pass over `pairs`, then ApplyPBDImpulse (x += delta / count * relax):
+    returns the position change of every node and the per-node counts."""
+    lib = osc._lib()
+    pairs = np.ascontiguousarray(pairs, np.int32)
+    kind = np.ascontiguousarray(kind, np.int32)
+    delta = np.zeros(3 * nv)
+    count = np.zeros(nv, np.int32)
+    lib.orc_repulsion(C.byref(m), None, osc._i(pairs), osc._i(kind), len(kind), h, _d(delta),
+                      osc._i(count))
+    cnt = count.copy()
+    before = keep["x"].copy()
+    lib.orc_apply_deltas(_d(keep["x"]), _d(delta), osc._i(count), nv, relax)
+    return (keep["x"] - before).reshape(-1, 3), cnt
+
+
+def _fd_grad(f, x, eps=1e-6):
+    g = np.zeros_like(x)
+    for k in range(x.shape[0]):
+        for a in range(3):
+            xp = x.copy(); xp[k, a] += eps
+            xm = x.copy(); xm[k, a] -= eps
+            g[k, a] = (f(xp) - f(xm)) / (2 * eps)
+    return g
+
+
+def _plane_dist(p):
+    """signed distance of p[0] from the plane through p[1], p[2], p[3] (normal (p2-p1)x(p3-p1))"""
+    n = np.cross(p[2] - p[1], p[3] - p[1])
+    return float(np.dot(p[0] - p[1], n / np.linalg.norm(n)))
+
+
+_VF_TRI = np.array([[0.1, 0.0, 0.05], [1.1, 0.02, -0.1], [0.2, -0.05, 0.9]])
+
+
+def _vf_scene(d_now, d_rest, bary=(0.4, 0.35, 0.25)):
+    """Face 0 = (0, 1, 2) carries the tested vertex 0; face 1 = (3, 4, 5) is the generic
+    triangle _VF_TRI.  Vertex 0 sits at signed distance d_now (d_rest at rest) above the
+    point with barycentric weights `bary` of face 1."""
+    t = _VF_TRI
+    n = np.cross(t[1] - t[0], t[2] - t[0])
+    n /= np.linalg.norm(n)
+    foot = bary[0] * t[0] + bary[1] * t[1] + bary[2] * t[2]
+    up = np.array([0.0, 0.6, 0.0])
+    x = np.vstack([foot + d_now * n, foot + up + [0.3, 0, 0], foot + up + [0, 0, 0.3], t])
+    x0 = x.copy()
+    x0[0] = foot + d_rest * n
+    tris = np.array([[0, 1, 2], [3, 4, 5]], np.int32)
+    return x, x0, tris, n
+
+
+@pytest.mark.parametrize("w", [(1.0, 1.0, 1.0, 1.0), (0.5, 2.0, 1.0, 0.25)])
+def test_repulsion_vf_matches_point_plane_gradient(w):
+    """Vertex 0.4 h above a free triangle (above at rest too): every node moves by
+    -relax * w_k C grad_k C / sum_j w_j |grad_j C|^2 with C = signed distance - h and the
+    gradients by central differences of the point-plane distance."""
+    h, relax = 2e-3, 0.25
+    x, x0, tris, n = _vf_scene(0.4 * h, 0.01)
+    wv = np.array([w[0], 1, 1, w[1], w[2], w[3]], np.float64)
+    m, keep, _, _ = _rmesh(x, x0, wv, np.ones(6), tris, [0b000001, 0])
+    delta, cnt = _repulse(m, keep, [[0, 1]], [1], h, 6, relax)
+    ids = [0, 3, 4, 5]
+    C_ = _plane_dist(x[ids]) - h
+    g = _fd_grad(_plane_dist, x[ids].copy())
+    wk = wv[ids]
+    expect = -relax * (wk[:, None] * g) * C_ / np.sum(wk * np.sum(g * g, axis=1))
+    np.testing.assert_array_equal(cnt, [1, 0, 0, 1, 1, 1])
+    np.testing.assert_allclose(delta[ids], expect, rtol=1e-7, atol=1e-13)
+    # the vertex is pushed away from the face (+n), the face towards -n
+    assert np.dot(delta[0], n) > 0 and all(np.dot(delta[k], n) < 0 for k in (3, 4, 5))
+    # outside the triangle (one barycentric weight negative): no test fires
+    x2, x02, _, _ = _vf_scene(0.4 * h, 0.01, bary=(-0.05, 0.6, 0.45))
+    m2, keep2, _, _ = _rmesh(x2, x02, wv, np.ones(6), tris, [0b000001, 0])
+    d2, c2 = _repulse(m2, keep2, [[0, 1]], [1], h, 6, relax)
+    assert not c2.any() and not d2.any()
+    # farther than h: C > 0, nothing moves
+    x3, x03, _, _ = _vf_scene(1.2 * h, 0.01)
+    m3, keep3, _, _ = _rmesh(x3, x03, wv, np.ones(6), tris, [0b000001, 0])
+    d3, c3 = _repulse(m3, keep3, [[0, 1]], [1], h, 6, relax)
+    assert not c3.any()
+
+
+def test_repulsion_vf_side_from_rest_state():
+    """The side comes from the start-of-substep state: a vertex that was below a free face
+    is kept below (pushed to -n), while below a face whose nodes are not free (an obstacle
+    face) the rest normal is not flipped and the vertex is pushed up through to +h.  Non-free
+    nodes receive nothing.
+
+    Reference quirk kept: with side = -1 only dcq = side * n flips.  dcp2 and dcp3
+    (PBDRepulsion.cu:381-382) carry the side twice, in cross(n, p3) and in dot(n, q), so
+    they stay the gradients of the +n distance, and dcp1 = -dcq - dcp2 - dcp3 closes the
+    sum.  With G the central-difference gradients of the +n point-plane distance:
+    dcq = -G_q, dcp2 = G_p2, dcp3 = G_p3, dcp1 = G_q - G_p2 - G_p3."""
+    h, relax = 2e-3, 1.0
+    x, x0, tris, n = _vf_scene(-0.3 * h, -0.01)
+    wv = np.ones(6)
+    m, keep, _, _ = _rmesh(x, x0, wv, np.ones(6), tris, [0b000001, 0])
+    delta, cnt = _repulse(m, keep, [[0, 1]], [1], h, 6, relax)
+    ids = [0, 3, 4, 5]
+    C_ = -_plane_dist(x[ids]) - h                       # the constraint measured along -n
+    G = _fd_grad(_plane_dist, x[ids].copy())
+    g = np.stack([-G[0], G[0] - G[2] - G[3], G[2], G[3]])
+    expect = -(g * C_) / np.sum(g * g)
+    np.testing.assert_allclose(delta[ids], expect, rtol=1e-7, atol=1e-13)
+    assert np.dot(delta[0], n) < 0
+    # the weighted corrections still sum to zero (dcp1 closes the sum)
+    np.testing.assert_allclose(delta[ids].sum(0), 0.0, atol=1e-15)
+    # the same geometry against an obstacle-like face: nodes 3..5 not free
+    free = np.array([1, 1, 1, 0, 0, 0], np.uint8)
+    m2, keep2, _, _ = _rmesh(x, x0, wv, free, tris, [0b000001, 0])
+    d2, c2 = _repulse(m2, keep2, [[0, 1]], [1], h, 6, relax)
+    C2 = _plane_dist(x[ids]) - h                        # rest normal +n kept: C = -1.3 h
+    g2 = _fd_grad(_plane_dist, x[ids].copy())
+    np.testing.assert_array_equal(c2, [1, 0, 0, 0, 0, 0])
+    np.testing.assert_allclose(d2[0], -(g2[0] * C2) / np.sum(g2 * g2), rtol=1e-7)
+    assert not d2[3:].any()
+
+
+def _seg_dist(p):
+    """distance between the infinite lines p0p1 and p2p3 (the segments cross in plan)"""
+    a, b = p[1] - p[0], p[3] - p[2]
+    n = np.cross(a, b)
+    return float(abs(np.dot(p[0] - p[2], n)) / np.linalg.norm(n))
+
+
+def _ee_scene(gap, gap_rest, angle):
+    """Face 0 = (0, 1, 2) with edge (0, 1) along x at height `gap`; face 1 = (3, 4, 5) with
+    edge (3, 4) in the y = 0 plane at `angle` from the x axis, the two edges crossing in plan
+    at parameters s = 0.4, t = 0.55.  Third vertices lie on the sides that InEdge accepts."""
+    c = np.array([0.1, 0.0, -0.05])
+    da = np.array([1.0, 0.02, 0.0])
+    da /= np.linalg.norm(da)
+    db = np.array([np.cos(angle), 0.0, np.sin(angle)])
+    a0, a1 = c - 0.4 * 0.8 * da, c + 0.6 * 0.8 * da
+    b0, b1 = c - 0.55 * 0.9 * db, c + 0.45 * 0.9 * db
+    up = np.array([0.0, gap, 0.0])
+    pb = np.array([-np.sin(angle), 0.0, np.cos(angle)])  # in-plane perpendicular of edge b
+    x = np.vstack([a0 + up, a1 + up, c + up + [0.0, 0.01, 0.5], b0, b1, c + [0.0, -0.01, 0.0] - 0.5 * pb])
+    x0 = x.copy()
+    x0[:3, 1] += gap_rest - gap
+    tris = np.array([[0, 1, 2], [3, 4, 5]], np.int32)
+    return x, x0, tris
+
+
+def _edge_bit(fe, edges, f, a, b):
+    for i in range(3):
+        if set(edges[fe[3 * f + i]]) == {a, b}:
+            return 1 << (3 + i)
+    raise AssertionError("edge not on face")
+
+
+@pytest.mark.parametrize("angle", [np.pi / 2, np.pi / 3])
+def test_repulsion_ee_matches_segment_distance_gradient(angle):
+    """Two free edges crossing 0.5 h apart (2 h at rest).  The reference's proximity normal
+    is the raw cross product of the unit edge directions (|n| = sin angle, not normalised,
+    BasicPrimitiveTests.cu:420-436) and its denominator is sum w_k b_k^2, so with G_k the
+    central-difference gradient of the line-line distance D:
+        dx_k = -relax * w_k (D |n| - h) |n| G_k / sum_j w_j |G_j|^2.
+    At 60 degrees a normalised n would change C and every correction by ~15 %."""
+    h, relax = 2e-3, 0.25
+    x, x0, tris = _ee_scene(0.5 * h, 2 * h, angle)
+    wv = np.array([0.5, 2.0, 1.0, 1.0, 0.25, 1.0])
+    edges = orc.build_edges(tris)
+    fe, *_ = osc.contact_topology(x0, tris, edges)
+    r = [_edge_bit(fe, edges, 0, 0, 1), _edge_bit(fe, edges, 1, 3, 4)]
+    m, keep, _, _ = _rmesh(x, x0, wv, np.ones(6), tris, r)
+    delta, cnt = _repulse(m, keep, [[0, 1]], [1], h, 6, relax)
+    ids = [0, 1, 3, 4]
+    D = _seg_dist(x[ids])
+    G = _fd_grad(_seg_dist, x[ids].copy())
+    ua = (x0[1] - x0[0]) / np.linalg.norm(x0[1] - x0[0])
+    ub = (x0[4] - x0[3]) / np.linalg.norm(x0[4] - x0[3])
+    sn = np.linalg.norm(np.cross(ua, ub))             # ~ sin(angle): edge a has a slight slope
+    wk = wv[ids]
+    expect = -relax * wk[:, None] * G * (D * sn - h) * sn / np.sum(wk * np.sum(G * G, axis=1))
+    np.testing.assert_array_equal(cnt, [1, 1, 0, 1, 1, 0])
+    np.testing.assert_allclose(delta[ids], expect, rtol=1e-6, atol=1e-13)
+    assert delta[0, 1] > 0 and delta[3, 1] < 0      # upper edge up, lower edge down
+
+
+def test_repulsion_ee_against_fixed_edge_uses_its_face_normal():
+    """Against an edge whose nodes are not free, the constraint direction is the edge's
+    normal (mean of its faces' current normals, Edge::ComputeNormal) and only the free edge
+    moves: dx_k = -relax w_k C b_k n_e / sum_j w_j b_j^2 over all four nodes' weights."""
+    h, relax = 2e-3, 1.0
+    x, x0, tris = _ee_scene(0.5 * h, 2 * h, np.pi / 2)
+    x[5, 1] -= 0.2           # tilt the lower face so its normal is not the common normal
+    x0[5, 1] -= 0.2
+    wv = np.array([1.0, 1.0, 1.0, 1e-3, 1e-3, 1e-3])
+    free = np.array([1, 1, 1, 0, 0, 0], np.uint8)
+    edges = orc.build_edges(tris)
+    fe, *_ = osc.contact_topology(x0, tris, edges)
+    r = [_edge_bit(fe, edges, 0, 0, 1), _edge_bit(fe, edges, 1, 3, 4)]
+    m, keep, _, _ = _rmesh(x, x0, wv, free, tris, r)
+    delta, cnt = _repulse(m, keep, [[0, 1]], [1], h, 6, relax)
+    ne = np.cross(x[4] - x[3], x[5] - x[3])
+    ne /= np.linalg.norm(ne)
+    # closest points of the two lines: s on (0,1), t on (3,4)
+    a, b, r0 = x[1] - x[0], x[4] - x[3], x[0] - x[3]
+    A = np.array([[a @ a, -(a @ b)], [a @ b, -(b @ b)]])
+    s, t = np.linalg.solve(A, [-(r0 @ a), -(r0 @ b)])
+    ca, cb = x[0] + s * a, x[3] + t * b
+    C_ = (ca - cb) @ ne - h
+    bk = np.array([1 - s, s, 1 - t, t])
+    den = np.sum(wv[[0, 1, 3, 4]] * bk * bk)
+    np.testing.assert_array_equal(cnt, [1, 1, 0, 0, 0, 0])
+    np.testing.assert_allclose(delta[0], -wv[0] * C_ * bk[0] * ne / den, rtol=1e-7, atol=1e-15)
+    np.testing.assert_allclose(delta[1], -wv[1] * C_ * bk[1] * ne / den, rtol=1e-7, atol=1e-15)
+    assert not delta[2:].any()
+
+
+# ------------------------------------------------------------------ EE-DCD solve
+def test_ee_dcd_solve_closed_form():
+    """SolveEEDCD_Kernel (PBDCollision.cu:155-214) on hand-computed numbers: cloth edge
+    x1 = (0, -0.01, 0), x2 = (1, -0.03, 0) at s = 0.25 under the obstacle edge point
+    c2 = (0.25, 0, 0) (t = 0.5), fn = +y: c1 = (0.25, -0.015, 0), lambda = -0.015 - 1e-3 =
+    -0.016, delta_1 = -(1 - s) lambda fn = (0, 0.012, 0), delta_2 = -s lambda fn =
+    (0, 0.004, 0); one count each.  A swapped (s, 1 - s), a dropped offset or a flipped sign
+    changes these numbers."""
+    lib = osc._lib()
+    x = np.array([[0, -0.01, 0], [1, -0.03, 0], [0, 0.5, 0], [1, 0.5, 0]], np.float64)
+    edges = np.array([[0, 1], [2, 3]], np.int32)
+    rec = np.zeros((2, 12))
+    rec[:, 0] = 3.4028234663852886e38
+    rec[0] = [0.015, 0.25, 0, -1, 0.25, 0, 1, 0.25, 0.5, 0, 1, 0]
+    delta = np.zeros(12)
+    count = np.zeros(4, np.int32)
+    lib.orc_ee_solve(_d(rec), _d(x), osc._i(edges), 2, _d(delta), osc._i(count))
+    np.testing.assert_allclose(delta.reshape(4, 3)[:2], [[0, 0.012, 0], [0, 0.004, 0]],
+                               rtol=1e-12, atol=1e-15)
+    np.testing.assert_array_equal(count, [1, 1, 0, 0])
+    # the edge 1 mm above the obstacle edge point is exactly at the offset: lambda = 0 is
+    # still solved (lambda <= 0) with zero deltas; 2 mm above: skipped
+    for y, n_expect in ((1e-3, 1), (2e-3, 0)):
+        xs = x.copy()
+        xs[:2, 1] = y
+        delta[:] = 0
+        count[:] = 0
+        lib.orc_ee_solve(_d(rec), _d(xs), osc._i(edges), 2, _d(delta), osc._i(count))
+        assert count[0] == n_expect and abs(delta).max() < 1e-15
+    # coincident closest points (|d| < 1e-6): skipped
+    xs = x.copy()
+    xs[:2, 1] = 0.0
+    delta[:] = 0
+    count[:] = 0
+    lib.orc_ee_solve(_d(rec), _d(xs), osc._i(edges), 2, _d(delta), osc._i(count))
+    assert not count.any()
+    # applied with relaxation 0.25: x1 += 0.012 * 0.25 / 1
+    delta[:] = 0
+    count[:] = 0
+    lib.orc_ee_solve(_d(rec), _d(x), osc._i(edges), 2, _d(delta), osc._i(count))
+    xa = np.ascontiguousarray(x.reshape(-1)).copy()
+    lib.orc_apply_deltas(_d(xa), _d(delta), osc._i(count), 4, 0.25)
+    np.testing.assert_allclose(xa.reshape(4, 3)[:2, 1], [-0.01 + 0.003, -0.03 + 0.001], rtol=1e-12)
+    assert not count.any() and not delta.any()

@@ -2,7 +2,10 @@
 
 Each test is chosen so that a plausible slip in the oracle (a dropped damping term, a
 wrong sign on a correction, a transposed cross product in the bend gradients, a missed
-epsilon) fails it, without restating the oracle's own formula.
+epsilon) fails it, without restating the oracle's own formula.  The bend-step pin was
+checked against mistakes planted in a scratch copy of the oracle (a dropped sqrt(1 - d^2),
+an inverse mass dropped from the numerator or the denominator, the consistent model's sign
+or rest angle changed): each fails it.
 """
 import ctypes as C
 
@@ -278,3 +281,73 @@ def test_hanging_cloth_rollout_is_stable():
     e = c.s_idx.reshape(-1, 2)
     strain = np.linalg.norm(x0[e[:, 0]] - x0[e[:, 1]], axis=1) / c.s_rest - 1
     assert np.abs(strain).max() < 0.2
+
+
+def _dihedral_d(x):
+    """n1 . n2 with n1 = N((x1-x0) x (x2-x0)), n2 = N((x1-x0) x (x3-x0)): the quantity the
+    bend kernel measures (BasicSolverHelper.cu:28-37), written here independently"""
+    p2, p3, p4 = x[1] - x[0], x[2] - x[0], x[3] - x[0]
+    n1 = np.cross(p2, p3)
+    n2 = np.cross(p2, p4)
+    return float(np.dot(n1 / np.linalg.norm(n1), n2 / np.linalg.norm(n2)))
+
+
+def _one_hinge_step(x, w, theta0, comp, model, dt=1 / 600):
+    """One bend projection (orc_bend_color) of a single hinge; returns the new positions."""
+    e = np.array([[0, 1]], np.int32)
+    c = orc.Cloth(x, np.zeros_like(x), w, e, np.ones(1), np.zeros(1), np.zeros(1, np.int32), 1,
+                  np.array([[0, 1, 2, 3]], np.int32), np.array([theta0]), np.array([comp]),
+                  np.zeros(1, np.int32), 1)
+    p = _params()
+    p.bend_model = model
+    orc.load().orc_bend_color(C.byref(c.s), C.byref(p), 0, dt)
+    return c.x.reshape(-1, 3).copy()
+
+
+@pytest.mark.parametrize("model", [1, 0])
+@pytest.mark.parametrize("comp", [0.0, 2e-6])
+def test_bend_step_changes_the_angle_by_the_linearised_amount(model, comp):
+    """One projection of a hinge whose angle phi = acos(d) is off its target by C: to first
+    order the angle changes by  -sgn C S / (alpha + S),  S = sum_k w_k |dd/dx_k|^2 (central
+    differences of d), alpha = compliance / dt^2.  The consistent model (Mueller 2007, target
+    pi - theta0) steps against C (sgn = +1: compliance 0 lands on the target up to O(C^2));
+    the reference formula steps along +q = -grad d with target theta0 (sgn = -1: |C| grows,
+    the energy-gain quirk).  A wrong sign, a missing sqrt(1 - d^2) or an inverse mass left
+    out of the denominator (unequal masses here) each break the first-order match.  The
+    corrections also conserve linear and angular momentum (d is invariant under rigid
+    motions), which pins how the inverse masses enter the numerator."""
+    rng = scenes.Rng(11)
+    x = _hinge(2.3) + 0.01 * rng.normal(12).reshape(4, 3)
+    w = np.array([0.7, 0.5, 2.0, 0.25])
+    dt = 1 / 600
+    d = _dihedral_d(x)
+    assert 0.05 < d < 0.95                     # both clamps inactive
+    phi = np.arccos(d)
+    g = np.zeros((4, 3))
+    for k in range(4):
+        for a in range(3):
+            xp = x.copy(); xp[k, a] += 1e-6
+            xm = x.copy(); xm[k, a] -= 1e-6
+            g[k, a] = (_dihedral_d(xp) - _dihedral_d(xm)) / 2e-6
+    S = float(np.sum(w * np.sum(g * g, axis=1)))
+    alpha = comp / dt**2
+    sgn = 1.0 if model == 1 else -1.0
+    res = []
+    for C_ in (1e-2, 1e-3):
+        target = phi - C_
+        theta0 = np.pi - target if model == 1 else target
+        xn = _one_hinge_step(x, w, theta0, comp, model, dt)
+        dphi = np.arccos(_dihedral_d(xn)) - phi
+        pred = -sgn * C_ * S / (alpha + S)
+        res.append(abs(dphi - pred))
+        assert dphi == pytest.approx(pred, rel=0.05)
+        dx = xn - x
+        m = 1.0 / w
+        scale = np.abs(m[:, None] * dx).max()
+        np.testing.assert_allclose((m[:, None] * dx).sum(0), 0.0, atol=1e-12 * scale)
+        # rotation invariance holds up to the reference's +1e-6 on |p2 x p3|, |p2 x p4|
+        np.testing.assert_allclose(np.cross(x, m[:, None] * dx).sum(0), 0.0, atol=1e-7 * scale)
+    # the residual is second order in C: a tenth of C leaves ~a hundredth of the residual
+    assert res[1] < res[0] / 30
+    if comp == 0.0 and model == 1:
+        assert res[1] < 1e-5
