@@ -67,8 +67,11 @@ typedef struct xrt_params {
    * fewest colors on structured meshes) or XRT_COLOR_PALETTE (GraphColoring::Paint,
    * xrtailor/src/physics/graph_coloring/Base.cu:163-317, made deterministic from the seed) */
   int32_t coloring;
-  /* capacity of the repulsion face-pair buffer per cloth face (fixed at xrt_start); a
-   * larger pair count is reported by xrt_get_contacts and the excess pairs are ignored */
+  /* capacity of the repulsion face-pair buffer per cloth face (>= 1; a change on a started
+   * context re-allocates the lists).  The reference sizes its pair vector from the count
+   * (BVH.cuh:139-152); here a pair count above the capacity leaves the excess pairs
+   * unevaluated and makes the next xrt_sync / xrt_time_* call return XRT_ERR_CAPACITY,
+   * naming the per-face capacity that would have held them */
   int32_t pair_capacity_per_face;
   /* capacities of the batched path's repulsion test lists (VF and EE items per cloth face,
    * defaults 64 and 128), rebuilt from the pairs at each refresh; an instance whose items
@@ -125,6 +128,8 @@ int xrt_destroy(xrt_ctx* ctx);
 /* SetSimulationParams (xrtailor/src/physics/ClothSolverHelper.cu:16-23) */
 int xrt_set_params(xrt_ctx* ctx, const xrt_params* p);
 int xrt_get_params(xrt_ctx* ctx, xrt_params* p);
+/* waits for the context's stream; XRT_ERR_CAPACITY if an instance's face pairs exceeded
+ * pair_capacity_per_face since the last check (the condition is reported once) */
 int xrt_sync(xrt_ctx* ctx);
 /* device properties: [0]=compute units, [1]=max LDS bytes per workgroup,
  * [2]=wavefront size, [3]=L2 bytes */

@@ -194,3 +194,31 @@ def test_garment_lra_contacts_batched_matches_plain():
         a.sync()
         b.FramesPlain(1)
         assert _compare(a, ia, b, ib, f"frame {f}") == 0.0
+
+
+def test_pair_overflow_reported_and_lists_regrown():
+    """A face-pair list too small for the scene (1 pair per face: every cloth face already
+    pairs with itself and its neighbours) is reported, once, as XRT_ERR_CAPACITY by the next
+    sync on both the batched and the plain path; raising pair_capacity_per_face on a started
+    context re-allocates the lists, after which the run equals one started with the larger
+    capacity bit for bit."""
+    from xrtailor_amd._ffi import XRT_ERR_CAPACITY, XrtError
+    print("case pair-list overflow: capacity 1 per face, then regrown to 64", flush=True)
+    kw = dict(bend_model=1, enable_obstacle_contact=1, enable_self_contact=1, ground_height=-1.0,
+              num_substeps=2, num_iterations=4, num_collision_passes=4, pair_update_interval=2)
+    specs = _specs(seed=15)
+    small, ia, ref, ib = _pair(specs, make_params(pair_capacity_per_face=1, **kw))
+    for run in (lambda: small.FramesBatched(1), lambda: [small.SubstepPlain(i) for i in ia]):
+        run()
+        with pytest.raises(XrtError) as e:
+            small.sync()
+        assert e.value.code == XRT_ERR_CAPACITY
+        print(f"  reported: {e.value}", flush=True)
+        small.sync()                       # reported once
+    ref.set_params(make_params(pair_capacity_per_face=64, **kw))
+    grown, ic, _, _ = _pair(specs, make_params(pair_capacity_per_face=64, **kw))
+    ref.FramesBatched(2)
+    ref.sync()
+    grown.FramesBatched(2)
+    grown.sync()
+    assert _compare(ref, ib, grown, ic, "regrown lists, 2 frames") == 0.0

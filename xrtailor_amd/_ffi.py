@@ -21,6 +21,7 @@ ERRORS = {
     -5: "XRT_ERR_NOT_FINITE",
     -6: "XRT_ERR_UNSUPPORTED",
 }
+XRT_ERR_CAPACITY = -4
 
 
 class XrtError(RuntimeError):

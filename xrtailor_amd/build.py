@@ -49,17 +49,20 @@ def _run(cmd: list[str]) -> None:
 
 def build_lib(jobs: int = 8) -> Path:
     OBJ.mkdir(parents=True, exist_ok=True)
-    steps = []
+    steps, objs = [], []
     for src in sorted(CSRC.glob("*.hip")):
         strict = ["-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt"]
         flags = HIP_FLAGS + (strict if src.name in STRICT_FP else [])
-        steps.append([HIPCC, *flags, "-c", str(src), "-o", str(OBJ / (src.stem + ".o"))])
+        objs.append(str(OBJ / (src.stem + ".o")))
+        steps.append([HIPCC, *flags, "-c", str(src), "-o", objs[-1]])
     for src in sorted(CSRC.glob("*.cpp")):
-        steps.append(["g++", *CXX_FLAGS, "-c", str(src), "-o", str(OBJ / (src.stem + ".cpp.o"))])
+        objs.append(str(OBJ / (src.stem + ".cpp.o")))
+        steps.append(["g++", *CXX_FLAGS, "-c", str(src), "-o", objs[-1]])
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, steps))
-    objs = sorted(str(p) for p in OBJ.glob("*.o"))
-    tmp = LIB.with_suffix(".so.tmp")
+    # link exactly the objects of this build: an object left behind by a renamed or
+    # deleted source must not end up in the library
+    tmp = LIB.with_suffix(f".so.tmp{os.getpid():x}")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs])
     os.replace(tmp, LIB)
     return LIB
@@ -69,7 +72,7 @@ def build_oracle() -> Path:
     out = ORACLE_LIB
     out.parent.mkdir(parents=True, exist_ok=True)
     srcs = sorted(str(p) for p in ORACLE_DIR.glob("*.c"))
-    tmp = out.with_suffix(".so.tmp")
+    tmp = out.with_suffix(f".so.tmp{os.getpid():x}")
     _run(["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-ffp-contract=off", "-fno-fast-math",
           "-Wall", "-o", str(tmp), *srcs, "-lm"])
     os.replace(tmp, out)

@@ -346,11 +346,16 @@ int xrt_set_params(xrt_ctx* ctx, const xrt_params* p) {
     XRT_REQUIRE(p->fem_xy_poisson_ratio * p->fem_yx_poisson_ratio != 1.0f, XRT_ERR_INVALID_ARG,
                 "fem_xy_poisson_ratio * fem_yx_poisson_ratio must differ from 1");
   }
+  XRT_REQUIRE(p->pair_capacity_per_face >= 1, XRT_ERR_INVALID_ARG, "pair_capacity_per_face must be >= 1");
+  const bool lists = p->pair_capacity_per_face != ctx->params.pair_capacity_per_face ||
+                     p->repulsion_vf_items_per_face != ctx->params.repulsion_vf_items_per_face ||
+                     p->repulsion_ee_items_per_face != ctx->params.repulsion_ee_items_per_face;
   ctx->params = *p;
   DestroyGraph(ctx);
   if (!ctx->started) return XRT_OK;
-  const int rc = FemPrepare(ctx);
+  int rc = FemPrepare(ctx);
   if (rc != XRT_OK) return rc;
+  if (lists && (rc = ContactResizeLists(ctx)) != XRT_OK) return rc;
   return ParticleConfigure(ctx);
 }
 
@@ -362,8 +367,7 @@ int xrt_get_params(xrt_ctx* ctx, xrt_params* p) {
 
 int xrt_sync(xrt_ctx* ctx) {
   XRT_REQUIRE(ctx, XRT_ERR_INVALID_ARG, "null context");
-  XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  return XRT_OK;
+  return CheckPairOverflow(ctx);
 }
 
 int xrt_device_info(xrt_ctx* ctx, int64_t info[4]) {
@@ -906,7 +910,7 @@ int xrt_time_batched_substep(xrt_ctx* ctx, int32_t n, float* avg_ms) {
     total += ms;
   }
   *avg_ms = total / static_cast<float>(n);
-  return XRT_OK;
+  return CheckPairOverflow(ctx);
 }
 
 int xrt_time_frames(xrt_ctx* ctx, int32_t batched, int32_t nframes, float* ms) {
@@ -922,7 +926,7 @@ int xrt_time_frames(xrt_ctx* ctx, int32_t batched, int32_t nframes, float* ms) {
   XRT_CHECK_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   XRT_CHECK_HIP(hipEventSynchronize(ctx->ev1));
   XRT_CHECK_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
-  return XRT_OK;
+  return CheckPairOverflow(ctx);
 }
 
 }  // extern "C"

@@ -48,9 +48,15 @@ struct DevBuf {
     if (count == 0) return hipSuccess;
     return hipMalloc(reinterpret_cast<void**>(&ptr), count * sizeof(T));
   }
+  // The copy has completed when this returns: callers upload from temporaries (set-up
+  // vectors that die at the end of their scope), and HIP does not promise that an
+  // asynchronous copy from pageable memory has read its source when the call returns
+  // (the runtime may queue it to a worker thread). Never called under stream capture.
   hipError_t Upload(const T* host, size_t count, hipStream_t s) {
     if (count == 0) return hipSuccess;
-    return hipMemcpyAsync(ptr, host, count * sizeof(T), hipMemcpyHostToDevice, s);
+    hipError_t e = hipMemcpyAsync(ptr, host, count * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
   }
   hipError_t Upload(const std::vector<T>& host, hipStream_t s) {
     hipError_t e = Alloc(host.size());

@@ -64,7 +64,10 @@ hipError_t LaunchRepulsion(hipStream_t s, const MeshTopo& cloth, const MeshState
 // FindPairs into pairs + *base (device offset), only slots < cap
 hipError_t LbvhFindPairsAt(Lbvh& lhs, const Lbvh& rhs, float thickness, int2* pairs,
                            const int32_t* base, int cap, hipStream_t s);
-hipError_t LaunchAddTotals(hipStream_t s, const int32_t* a, const int32_t* b, int32_t* out);
+// *out = *a + *b; a total above cap (an instance of nf faces) raises *overflow to the
+// pairs-per-face figure the instance would need
+hipError_t LaunchAddTotals(hipStream_t s, const int32_t* a, const int32_t* b, int32_t* out, int cap,
+                           int nf, int32_t* overflow);
 
 // ----------------------------------------------------------------- all instances at once
 // Device view of one instance's contact state for the multi-instance kernels.  Every
@@ -109,6 +112,9 @@ struct InstContactDev {
   // pair refresh: per 256-leaf block of the count pass, its pair / VF / EE item sums
   // (3 x Blocks(nf)), then their exclusive prefix
   uint32_t* bsum = nullptr;
+  // the context's pair-overflow word: the largest pairs-per-face figure of an instance
+  // whose pairs exceeded pair_cap since it was last read (pairs beyond pair_cap are lost)
+  int32_t* overflow = nullptr;
 };
 
 constexpr int kPairStage = 32;

@@ -638,8 +638,17 @@ __global__ __launch_bounds__(kBlock) void FindPairsAtKernel(BvhView l, BvhView r
   FindPairsBody(l, r, th, id, static_cast<int64_t>(*base) + off[id], cnt[id], pairs, cap);
 }
 
-__global__ void AddTotalsKernel(const int32_t* a, const int32_t* b, int32_t* out) {
-  if (threadIdx.x == 0) *out = *a + *b;
+// a pair total above the capacity: raise the overflow word to the pairs per face needed
+__device__ __forceinline__ void NotePairOverflow(int total, int cap, int nf, int32_t* overflow) {
+  if (total > cap && overflow) atomicMax(overflow, (total + max(nf, 1) - 1) / max(nf, 1));
+}
+
+__global__ void AddTotalsKernel(const int32_t* a, const int32_t* b, int32_t* out, int cap, int nf,
+                                int32_t* overflow) {
+  if (threadIdx.x == 0) {
+    *out = *a + *b;
+    NotePairOverflow(*out, cap, nf, overflow);
+  }
 }
 
 // An EE item that passed the inside test, queued for its response: the cloth edge, the
@@ -980,6 +989,7 @@ __global__ __launch_bounds__(kBlock) void MultiAddTotalsKernel(const InstContact
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n || !ic[k].scal) return;
   ic[k].scal[2] = ic[k].scal[1] + *ic[k].cbvh.total;
+  NotePairOverflow(ic[k].scal[2], ic[k].pair_cap, ic[k].nf, ic[k].overflow);
   if (ic[k].ee_cap > 0) {
     ic[k].scal[3] = ic[k].scal[5] + ic[k].scal[7];
     ic[k].scal[4] = ic[k].scal[6] + ic[k].scal[8];
@@ -1160,8 +1170,9 @@ hipError_t LbvhFindPairsAt(Lbvh& lhs, const Lbvh& rhs, float thickness, int2* pa
       reinterpret_cast<const uint32_t*>(lhs.offsets.ptr), base, pairs, cap);
   return hipGetLastError();
 }
-hipError_t LaunchAddTotals(hipStream_t s, const int32_t* a, const int32_t* b, int32_t* out) {
-  AddTotalsKernel<<<1, 64, 0, s>>>(a, b, out);
+hipError_t LaunchAddTotals(hipStream_t s, const int32_t* a, const int32_t* b, int32_t* out, int cap,
+                           int nf, int32_t* overflow) {
+  AddTotalsKernel<<<1, 64, 0, s>>>(a, b, out, cap, nf, overflow);
   return hipGetLastError();
 }
 

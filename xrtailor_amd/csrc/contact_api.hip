@@ -99,8 +99,37 @@ bool ContactNeeded(const xrt_ctx* ctx, const HostInstance& h) {
   return h.obs_nt() > 0 || ctx->params.enable_self_contact;
 }
 
+// pair list and repulsion item lists of one instance at the context's capacities
+int AllocPairLists(xrt_ctx* ctx, const HostInstance& h, ContactData& c) {
+  const int64_t per_face = std::max(1, ctx->params.pair_capacity_per_face);
+  XRT_REQUIRE(per_face * std::max(h.nt, 1) <= static_cast<int64_t>(INT32_MAX), XRT_ERR_INVALID_ARG,
+              "pair_capacity_per_face x faces exceeds 2^31 - 1");
+  c.pair_cap = static_cast<int>(std::max<int64_t>(1, per_face * h.nt));
+  XRT_CHECK_HIP(c.pairs.Alloc(c.pair_cap));
+  // repulsion item lists (at most 3 VF and 9 EE items per pair; an instance whose items
+  // overflow a list falls back to the per-pair tests for that half); off when either
+  // capacity is 0 or the counts could overflow an int
+  const int64_t vf_cap = static_cast<int64_t>(std::max(0, ctx->params.repulsion_vf_items_per_face)) * h.nt;
+  const int64_t ee_cap = static_cast<int64_t>(std::max(0, ctx->params.repulsion_ee_items_per_face)) * h.nt;
+  c.vf_cap = c.ee_cap = 0;
+  c.rep_vf.Free();
+  c.rep_ee.Free();
+  if (ctx->params.enable_self_contact && vf_cap > 0 && ee_cap > 0 &&
+      9 * static_cast<int64_t>(c.pair_cap) <= static_cast<int64_t>(INT32_MAX)) {
+    c.vf_cap = static_cast<int>(std::min<int64_t>(vf_cap, 3 * static_cast<int64_t>(c.pair_cap)));
+    c.ee_cap = static_cast<int>(std::min<int64_t>(ee_cap, 9 * static_cast<int64_t>(c.pair_cap)));
+    XRT_CHECK_HIP(c.rep_vf.Alloc(c.vf_cap));
+    XRT_CHECK_HIP(c.rep_ee.Alloc(c.ee_cap));
+    if (!c.ivf.ptr) XRT_CHECK_HIP(c.ivf.Alloc(std::max(h.nt, 1)));
+    if (!c.iee.ptr) XRT_CHECK_HIP(c.iee.Alloc(std::max(h.nt, 1)));
+  }
+  return XRT_OK;
+}
+
 int ContactStart(xrt_ctx* ctx) {
   hipStream_t s = ctx->stream;
+  XRT_CHECK_HIP(ctx->overflow.Alloc(1));
+  XRT_CHECK_HIP(hipMemsetAsync(ctx->overflow.ptr, 0, sizeof(int32_t), s));
   ctx->contact.clear();
   ctx->contact.resize(ctx->inst.size());
   for (size_t ii = 0; ii < ctx->inst.size(); ++ii) {
@@ -147,25 +176,10 @@ int ContactStart(xrt_ctx* ctx) {
     XRT_CHECK_HIP(hipMemsetAsync(c->cnt.ptr, 0, sizeof(int32_t) * h.nv, s));
     XRT_CHECK_HIP(hipMemsetAsync(c->vf_face.ptr, 0xff, sizeof(int32_t) * h.nv, s));
     XRT_CHECK_HIP(hipMemsetAsync(c->rt_face.ptr, 0xff, sizeof(int32_t) * h.nv, s));
-    const int per_face = std::max(1, ctx->params.pair_capacity_per_face);
-    c->pair_cap = std::max(1, per_face * h.nt);
-    XRT_CHECK_HIP(c->pairs.Alloc(c->pair_cap));
     XRT_CHECK_HIP(c->scal.Alloc(16));
     XRT_CHECK_HIP(hipMemsetAsync(c->scal.ptr, 0, sizeof(int32_t) * 16, s));
-    // repulsion item lists (at most 3 VF and 9 EE items per pair; an instance whose items
-    // overflow a list falls back to the per-pair tests for that half); off when either
-    // capacity is 0 or the counts could overflow an int
-    const int64_t vf_cap = static_cast<int64_t>(std::max(0, ctx->params.repulsion_vf_items_per_face)) * h.nt;
-    const int64_t ee_cap = static_cast<int64_t>(std::max(0, ctx->params.repulsion_ee_items_per_face)) * h.nt;
-    if (ctx->params.enable_self_contact && vf_cap > 0 && ee_cap > 0 &&
-        9 * static_cast<int64_t>(c->pair_cap) <= static_cast<int64_t>(INT32_MAX)) {
-      c->vf_cap = static_cast<int>(std::min<int64_t>(vf_cap, 3 * static_cast<int64_t>(c->pair_cap)));
-      c->ee_cap = static_cast<int>(std::min<int64_t>(ee_cap, 9 * static_cast<int64_t>(c->pair_cap)));
-      XRT_CHECK_HIP(c->rep_vf.Alloc(c->vf_cap));
-      XRT_CHECK_HIP(c->rep_ee.Alloc(c->ee_cap));
-      XRT_CHECK_HIP(c->ivf.Alloc(std::max(h.nt, 1)));
-      XRT_CHECK_HIP(c->iee.Alloc(std::max(h.nt, 1)));
-    }
+    rc = AllocPairLists(ctx, h, *c);
+    if (rc != XRT_OK) return rc;
     if (ctx->params.enable_self_contact)
       XRT_CHECK_HIP(c->stage.Alloc(static_cast<size_t>(kPairStage) * std::max(h.nt, 1)));
     XRT_CHECK_HIP(c->bsum.Alloc(3 * static_cast<size_t>(std::max(1, (h.nt + 255) / 256))));
@@ -234,7 +248,8 @@ int ContactIterPre(xrt_ctx* ctx, int inst, int it, bool* pass_ran) {
       XRT_CHECK_HIP(LbvhCountPairs(c.cloth_bvh, c.cloth_bvh, kPairThickness, s));
       XRT_CHECK_HIP(LbvhFindPairsAt(c.cloth_bvh, c.cloth_bvh, kPairThickness, c.pairs.ptr,
                                     c.scal.ptr + 1, c.pair_cap, s));
-      XRT_CHECK_HIP(LaunchAddTotals(s, c.scal.ptr + 1, c.cloth_bvh.total.ptr, c.scal.ptr + 2));
+      XRT_CHECK_HIP(LaunchAddTotals(s, c.scal.ptr + 1, c.cloth_bvh.total.ptr, c.scal.ptr + 2,
+                                    c.pair_cap, h.nt, ctx->overflow.ptr));
     }
     if (self) {
       MeshState cs{x, x0, -1.0f};
@@ -294,6 +309,32 @@ bool ContactsActive(const xrt_ctx* ctx) {
 }
 
 // device views of every instance's contact state and the block -> instance maps
+int ContactResizeLists(xrt_ctx* ctx) {
+  XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  for (size_t k = 0; k < ctx->inst.size(); ++k) {
+    if (!ctx->contact[k]) continue;
+    const int rc = AllocPairLists(ctx, ctx->inst[k], *ctx->contact[k]);
+    if (rc != XRT_OK) return rc;
+  }
+  return MultiContactSetup(ctx);
+}
+
+int CheckPairOverflow(xrt_ctx* ctx) {
+  XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  if (!ctx->overflow.ptr) return XRT_OK;
+  int32_t need = 0;
+  XRT_CHECK_HIP(hipMemcpy(&need, ctx->overflow.ptr, sizeof(need), hipMemcpyDeviceToHost));
+  if (need <= 0) return XRT_OK;
+  XRT_CHECK_HIP(hipMemsetAsync(ctx->overflow.ptr, 0, sizeof(int32_t), ctx->stream));
+  XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  SetError("face-pair list overflow: an instance needed " + std::to_string(need) +
+           " pairs per face, pair_capacity_per_face is " +
+           std::to_string(ctx->params.pair_capacity_per_face) +
+           "; the pairs beyond the capacity were not evaluated. Raise pair_capacity_per_face "
+           "(xrt_set_params re-allocates the lists) and run the frames again");
+  return XRT_ERR_CAPACITY;
+}
+
 int MultiContactSetup(xrt_ctx* ctx) {
   const int n = static_cast<int>(ctx->inst.size());
   std::vector<InstContactDev> ic(std::max(n, 1));
@@ -338,6 +379,7 @@ int MultiContactSetup(xrt_ctx* ctx) {
       d.rep_ee = c.rep_ee.ptr;
       d.stage = c.stage.ptr;
       d.bsum = c.bsum.ptr;
+      d.overflow = ctx->overflow.ptr;
       d.ivf = c.ivf.ptr;
       d.iee = c.iee.ptr;
       cnt[MultiMaps::kVert] = blocks(h.nv);

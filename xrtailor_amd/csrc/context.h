@@ -263,6 +263,8 @@ struct xrt_ctx {
   // multi-instance contact kernels
   xrt::DevBuf<xrt::InstContactDev> ic;
   xrt::DevBuf<int32_t> ic_off;
+  // pair-overflow word (InstContactDev::overflow), read and cleared by CheckPairOverflow
+  xrt::DevBuf<int32_t> overflow;
   xrt::TreeSet cloth_trees, obs_trees;  // frame-start builds of the batched path
   xrt::MultiMaps maps;
   // wide (all instances per launch) launch sizes
@@ -297,6 +299,12 @@ void ColorSort(const std::vector<int32_t>& color, int ncol, std::vector<int32_t>
 // contact_api.hip
 bool ContactNeeded(const xrt_ctx* ctx, const HostInstance& h);
 int ContactStart(xrt_ctx* ctx);
+// pair / item lists re-allocated at the current capacities (after xrt_set_params changed
+// them on a started context)
+int ContactResizeLists(xrt_ctx* ctx);
+// synchronizes the stream, then XRT_ERR_CAPACITY (and clears the word) if an instance's
+// face pairs exceeded the pair capacity since the last check
+int CheckPairOverflow(xrt_ctx* ctx);
 int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendMode& bm);
 // contact phases of one instance's substep (shared by the plain and the wide drivers)
 int ContactSubstepBegin(xrt_ctx* ctx, int inst);
