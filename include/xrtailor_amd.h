@@ -183,6 +183,54 @@ int xrt_update_obstacle(xrt_ctx* ctx, int32_t inst, const float* pos);
 int xrt_get_contacts(xrt_ctx* ctx, int32_t inst, int32_t* vf_face, float* vf_q, int32_t* rt_face,
                      float* rt_q, float* ee16, int32_t stats[4]);
 
+/* ------------------------------------ reference-shaped collider calls (one instance)
+ * For an engine that keeps its own SimulateSwift / SimulateQuality loop
+ * (xrtailor/src/pipeline/base/ClothSolverBase.cpp:171-313).  Each launches on the context's
+ * stream exactly the kernels xrt_substep_plain launches for that step, so the reference's
+ * call order through these entry points reproduces xrt_substep_plain bit for bit.  They act
+ * on the instance's contact state (created by xrt_start when the instance has an obstacle
+ * or self contact is on); XRT_ERR_STATE otherwise, or without an obstacle for the calls that
+ * need one.  The obstacle (the reference's obstacle_bvh_ / PhysicsMesh) is the instance's
+ * own, set by xrt_set_obstacle. */
+/* obstacle_bvh_->UpdateData(faces, 0, false, n) + Construct() from the obstacle's frame-start
+ * positions (xrtailor/include/xrtailor/physics/broad_phase/lbvh/BVH.cuh:90-92) */
+int xrt_obstacle_bvh_construct(xrt_ctx* ctx, int32_t inst);
+/* cloth_bvh_->UpdateData(faces, 0, false, n) + Construct() from the cloth's x0 (BVH.cuh:90-92) */
+int xrt_cloth_bvh_construct(xrt_ctx* ctx, int32_t inst);
+/* obstacle_->Interpolate(cnt, num_passes): x1 = x0 + (x - x0) (cnt + 1) / num_passes
+ * (xrtailor/include/xrtailor/physics/PhysicsMesh.cuh:128) */
+int xrt_obstacle_interpolate(xrt_ctx* ctx, int32_t inst, int32_t cnt, int32_t num_passes);
+/* obstacle_->UpdateMidstepNormals() (PhysicsMesh.cuh:84) */
+int xrt_obstacle_update_midstep_normals(xrt_ctx* ctx, int32_t inst);
+/* obstacle_bvh_->RefitMidstep(faces, tolerance) (BVH.cuh:94) */
+int xrt_obstacle_bvh_refit_midstep(xrt_ctx* ctx, int32_t inst, float tolerance);
+/* obstacle_->PredictedToPositions(): obstacle x0 = x at frame end (PhysicsMesh.cuh:126) */
+int xrt_obstacle_predicted_to_positions(xrt_ctx* ctx, int32_t inst);
+/* VFConstraint::Generate / Solve (DCD, offset 1e-3),
+ * xrtailor/include/xrtailor/physics/pbd_collision/Constraint.cuh:15-30 */
+int xrt_vf_dcd_generate(xrt_ctx* ctx, int32_t inst);
+int xrt_vf_dcd_solve(xrt_ctx* ctx, int32_t inst);
+/* RTConstraint::Generate / Solve (VF-CCD, offset 0), Constraint.cuh:54-69 */
+int xrt_vf_ccd_generate(xrt_ctx* ctx, int32_t inst);
+int xrt_vf_ccd_solve(xrt_ctx* ctx, int32_t inst);
+/* EEConstraint::Generate / Solve, Constraint.cuh:32-48: the solve accumulates into the
+ * instance's Jacobi sums (the reference's deltas / delta_counts); xrt_apply_deltas applies
+ * them */
+int xrt_ee_dcd_generate(xrt_ctx* ctx, int32_t inst);
+int xrt_ee_dcd_solve(xrt_ctx* ctx, int32_t inst);
+/* PBDRepulsionSolver::UpdatePairs(cloth, obstacle, cloth_bvh, obstacle_bvh, thickness):
+ * cloth tree Update(ccd), Traverse(cloth, obstacle) then Traverse(cloth, cloth)
+ * (xrtailor/include/xrtailor/physics/repulsion/PBDRepulsion.cuh:19-20).  Needs the cloth
+ * tree built (xrt_cloth_bvh_construct).  A pair count above the capacity is reported by
+ * the next xrt_sync (pair_capacity_per_face) */
+int xrt_repulsion_update_pairs(xrt_ctx* ctx, int32_t inst, float thickness);
+/* PBDRepulsionSolver::SolveNoProximity(cloth, thickness, relaxation_rate, ...): every pair's
+ * VF / EE proximity tests, impulses applied x relaxation_rate (PBDRepulsion.cuh:27-28).  It
+ * uses the instance's Jacobi sums too: call xrt_apply_deltas after xrt_ee_dcd_solve before
+ * the next repulsion solve, as the reference's order does */
+int xrt_repulsion_solve_no_proximity(xrt_ctx* ctx, int32_t inst, float thickness,
+                                     float relaxation_rate);
+
 /* ---------------------------------------------------------------- state I/O */
 int xrt_num_instances(xrt_ctx* ctx, int32_t* n);
 /* counts[0..7] = nv, nt, n_edges, n_bends, stretch colors, bend colors, LRA slots, LRA constraints */
@@ -230,7 +278,9 @@ int xrt_isometric_bending_solve(xrt_ctx* ctx, int32_t inst, int32_t color, int32
 int xrt_solve_lra(xrt_ctx* ctx, int32_t inst);
 /* ground plane projection (SDFCollider type 1, xrtailor/include/xrtailor/physics/sdf/SDFCollider.cuh:36-43) */
 int xrt_solve_ground(xrt_ctx* ctx, int32_t inst);
-/* ApplyDeltas (ClothSolverHelper.cuh:20-41) */
+/* ApplyDeltas (xrtailor/include/xrtailor/physics/ClothSolverHelper.cuh:37): the instance's
+ * contact Jacobi sums (xrt_ee_dcd_solve) applied x relaxation_factor and cleared; no-op
+ * for an instance without contact state */
 int xrt_apply_deltas(xrt_ctx* ctx, int32_t inst);
 /* Finalize (ClothSolverHelper.cuh:20-41) */
 int xrt_finalize(xrt_ctx* ctx, int32_t inst, float dt);

@@ -117,6 +117,20 @@ SIGNATURES = {
     "xrt_solve_lra": [_P, _I],
     "xrt_solve_ground": [_P, _I],
     "xrt_apply_deltas": [_P, _I],
+    "xrt_obstacle_bvh_construct": [_P, _I],
+    "xrt_cloth_bvh_construct": [_P, _I],
+    "xrt_obstacle_interpolate": [_P, _I, _I, _I],
+    "xrt_obstacle_update_midstep_normals": [_P, _I],
+    "xrt_obstacle_bvh_refit_midstep": [_P, _I, _F],
+    "xrt_obstacle_predicted_to_positions": [_P, _I],
+    "xrt_vf_dcd_generate": [_P, _I],
+    "xrt_vf_dcd_solve": [_P, _I],
+    "xrt_vf_ccd_generate": [_P, _I],
+    "xrt_vf_ccd_solve": [_P, _I],
+    "xrt_ee_dcd_generate": [_P, _I],
+    "xrt_ee_dcd_solve": [_P, _I],
+    "xrt_repulsion_update_pairs": [_P, _I, _F],
+    "xrt_repulsion_solve_no_proximity": [_P, _I, _F, _F],
     "xrt_finalize": [_P, _I, _F],
     "xrt_add_bindings": [_P, _I, _IP, _FP, _FP, _I],
     "xrt_set_obstacle_skinning": [_P, _I, _IP, _IP, _FP, _I],

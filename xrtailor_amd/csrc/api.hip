@@ -845,7 +845,7 @@ int xrt_solve_ground(xrt_ctx* ctx, int32_t inst) {
 int xrt_apply_deltas(xrt_ctx* ctx, int32_t inst) {
   int rc = CheckStarted(ctx, inst);
   if (rc != XRT_OK) return rc;
-  return XRT_OK;  // no contact deltas are produced by the dynamics-only substep
+  return ApplyInstanceDeltas(ctx, inst);
 }
 
 int xrt_finalize(xrt_ctx* ctx, int32_t inst, float dt) {

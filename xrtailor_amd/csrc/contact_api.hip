@@ -170,6 +170,7 @@ int ContactStart(xrt_ctx* ctx) {
     XRT_CHECK_HIP(c->vf_q.Alloc(h.nv));
     XRT_CHECK_HIP(c->rt_q.Alloc(h.nv));
     XRT_CHECK_HIP(c->ee.Alloc(sizeof(EeRec) * std::max(h.ne(), 1)));
+    XRT_CHECK_HIP(hipMemsetAsync(c->ee.ptr, 0, c->ee.n, s));  // no records before a pass
     XRT_CHECK_HIP(c->dq.Alloc(3 * static_cast<size_t>(h.nv)));
     XRT_CHECK_HIP(c->cnt.Alloc(h.nv));
     XRT_CHECK_HIP(hipMemsetAsync(c->dq.ptr, 0, sizeof(long long) * 3 * h.nv, s));
@@ -205,60 +206,112 @@ int ContactSubstepBegin(xrt_ctx* ctx, int inst) {
   return XRT_OK;
 }
 
+// The phases below are what both the plain driver (ContactIterPre) and the reference-shaped
+// entry points (xrt_vf_dcd_generate, xrt_repulsion_update_pairs, ...) launch, so a caller
+// that drives the reference's order through the entry points gets the driver's bits.
+
+// obstacle x1 = lerp(x0, x, (cnt + 1) / passes) (PhysicsMesh::Interpolate)
+int PhaseObstacleInterpolate(xrt_ctx* ctx, int inst, int cnt, int passes) {
+  ContactData& c = *ctx->contact[inst];
+  const float alpha = (static_cast<float>(cnt) + 1.0f) / static_cast<float>(passes);
+  XRT_CHECK_HIP(LaunchInterpolate(ctx->stream, c.o_x0.ptr, c.o_x.ptr, c.o_x1.ptr, c.obs_nv, alpha));
+  return XRT_OK;
+}
+
+// n1 = normals of x1 (PhysicsMesh::UpdateMidstepNormals)
+int PhaseObstacleMidstepNormals(xrt_ctx* ctx, int inst) {
+  ContactData& c = *ctx->contact[inst];
+  XRT_CHECK_HIP(VertexNormals(c.o_x1.ptr, c.o_tris.ptr, c.o_vptr.ptr, c.o_corner.ptr, c.obs_nv,
+                              c.o_n1.ptr, ctx->stream));
+  return XRT_OK;
+}
+
+// BVH::RefitMidstep(faces, tolerance) of the obstacle tree
+int PhaseObstacleRefitMidstep(xrt_ctx* ctx, int inst, float tol) {
+  ContactData& c = *ctx->contact[inst];
+  XRT_CHECK_HIP(LbvhRefitMidstep(c.obs_bvh, c.o_x1.ptr, c.o_n1.ptr, tol, ctx->stream));
+  return XRT_OK;
+}
+
+// kind 0: VF-DCD, 1: VF-CCD (RTConstraint), 2: EE-DCD records against the obstacle tree
+int PhaseGenerate(xrt_ctx* ctx, int inst, int kind) {
+  const HostInstance& h = ctx->inst[inst];
+  ContactData& c = *ctx->contact[inst];
+  hipStream_t s = ctx->stream;
+  float4* x0 = ctx->x0.ptr + h.desc.vbeg;
+  float4* x = ctx->x.ptr + h.desc.vbeg;
+  if (kind == 0) XRT_CHECK_HIP(LaunchVfDcdGenerate(s, x, h.nv, c.obs_bvh, c.vf_face.ptr, c.vf_q.ptr));
+  if (kind == 1) XRT_CHECK_HIP(LaunchVfCcdGenerate(s, x0, x, h.nv, c.obs_bvh, c.rt_face.ptr, c.rt_q.ptr));
+  if (kind == 2)
+    XRT_CHECK_HIP(LaunchEeDcdGenerate(s, x, c.c_edges.ptr, h.ne(), c.obs_bvh, c.o_rtri.ptr,
+                                      reinterpret_cast<EeRec*>(c.ee.ptr)));
+  return XRT_OK;
+}
+
+// PBDRepulsionSolver::UpdatePairs: cloth LBVH Update(x, x0, ccd), Traverse(cloth, obstacle)
+// at [0, n_obs), Traverse(cloth, cloth) after them
+int PhaseUpdatePairs(xrt_ctx* ctx, int inst, float thickness) {
+  const HostInstance& h = ctx->inst[inst];
+  ContactData& c = *ctx->contact[inst];
+  hipStream_t s = ctx->stream;
+  float4* x0 = ctx->x0.ptr + h.desc.vbeg;
+  float4* x = ctx->x.ptr + h.desc.vbeg;
+  XRT_CHECK_HIP(LbvhUpdate(c.cloth_bvh, x, x0, 1, s));
+  if (c.obs_nf > 0) {
+    XRT_CHECK_HIP(LbvhCountPairs(c.cloth_bvh, c.obs_bvh, thickness, s));
+    XRT_CHECK_HIP(hipMemcpyAsync(c.scal.ptr + 1, c.cloth_bvh.total.ptr, 4, hipMemcpyDeviceToDevice, s));
+    XRT_CHECK_HIP(LbvhFindPairsAt(c.cloth_bvh, c.obs_bvh, thickness, c.pairs.ptr, c.scal.ptr,
+                                  c.pair_cap, s));
+  } else {
+    XRT_CHECK_HIP(hipMemsetAsync(c.scal.ptr + 1, 0, 4, s));
+  }
+  XRT_CHECK_HIP(LbvhCountPairs(c.cloth_bvh, c.cloth_bvh, thickness, s));
+  XRT_CHECK_HIP(LbvhFindPairsAt(c.cloth_bvh, c.cloth_bvh, thickness, c.pairs.ptr, c.scal.ptr + 1,
+                                c.pair_cap, s));
+  XRT_CHECK_HIP(LaunchAddTotals(s, c.scal.ptr + 1, c.cloth_bvh.total.ptr, c.scal.ptr + 2,
+                                c.pair_cap, h.nt, ctx->overflow.ptr));
+  return XRT_OK;
+}
+
+// PBDRepulsionSolver::SolveNoProximity: per-pair VF / EE tests -> Jacobi sums -> x relax
+int PhaseRepulsion(xrt_ctx* ctx, int inst, float thickness, float relax) {
+  const HostInstance& h = ctx->inst[inst];
+  ContactData& c = *ctx->contact[inst];
+  hipStream_t s = ctx->stream;
+  float4* x0 = ctx->x0.ptr + h.desc.vbeg;
+  float4* x = ctx->x.ptr + h.desc.vbeg;
+  MeshState cs{x, x0, -1.0f};
+  MeshState os{c.o_x.ptr, c.o_x0.ptr, 1e-3f};  // obstacle nodes: inv mass 1e-3, not free
+  XRT_CHECK_HIP(LaunchRepulsion(s, ClothTopo(c, h), cs, ObsTopo(c), os, c.pairs.ptr, c.scal.ptr + 1,
+                                c.scal.ptr + 2, c.pair_cap, thickness, c.dq.ptr, c.cnt.ptr));
+  XRT_CHECK_HIP(LaunchApplyFixedDeltas(s, x, h.nv, c.dq.ptr, c.cnt.ptr, relax));
+  return XRT_OK;
+}
+
 // collision pass when due (*pass_ran says whether it ran), pair refresh and repulsion
 int ContactIterPre(xrt_ctx* ctx, int inst, int it, bool* pass_ran) {
-  HostInstance& h = ctx->inst[inst];
   ContactData& c = *ctx->contact[inst];
   const xrt_params& p = ctx->params;
-  const InstDesc& d = h.desc;
-  hipStream_t s = ctx->stream;
-  float4* x0 = ctx->x0.ptr + d.vbeg;
-  float4* x = ctx->x.ptr + d.vbeg;
   const bool has_obs = c.obs_nf > 0 && p.enable_obstacle_contact;
   const bool self = p.enable_self_contact != 0;
-  EeRec* ee = reinterpret_cast<EeRec*>(c.ee.ptr);
   const int frequency = std::max(1, p.num_substeps * p.num_iterations / p.num_collision_passes);
   const int interval = std::max(1, p.pair_update_interval);
+  int rc = XRT_OK;
   if (pass_ran) *pass_ran = false;
-  {
-    const int step = c.frame_substep * p.num_iterations + it;
-    if (has_obs && step % frequency == 0) {
-      const float alpha = (static_cast<float>(c.pass_cnt) + 1.0f) / static_cast<float>(p.num_collision_passes);
-      XRT_CHECK_HIP(LaunchInterpolate(s, c.o_x0.ptr, c.o_x.ptr, c.o_x1.ptr, c.obs_nv, alpha));
-      XRT_CHECK_HIP(VertexNormals(c.o_x1.ptr, c.o_tris.ptr, c.o_vptr.ptr, c.o_corner.ptr, c.obs_nv,
-                                  c.o_n1.ptr, s));
-      XRT_CHECK_HIP(LbvhRefitMidstep(c.obs_bvh, c.o_x1.ptr, c.o_n1.ptr, p.bvh_tolerance, s));
-      XRT_CHECK_HIP(LaunchVfDcdGenerate(s, x, d.nv, c.obs_bvh, c.vf_face.ptr, c.vf_q.ptr));
-      XRT_CHECK_HIP(LaunchVfCcdGenerate(s, x0, x, d.nv, c.obs_bvh, c.rt_face.ptr, c.rt_q.ptr));
-      XRT_CHECK_HIP(LaunchEeDcdGenerate(s, x, c.c_edges.ptr, h.ne(), c.obs_bvh, c.o_rtri.ptr, ee));
-      c.pass_cnt++;
-      if (pass_ran) *pass_ran = true;
-    }
-    if (self && it % interval == 0) {
-      XRT_CHECK_HIP(LbvhUpdate(c.cloth_bvh, x, x0, 1, s));
-      // cloth vs obstacle pairs at [0, n_obs), cloth vs cloth after them
-      if (c.obs_nf > 0) {
-        XRT_CHECK_HIP(LbvhCountPairs(c.cloth_bvh, c.obs_bvh, kPairThickness, s));
-        XRT_CHECK_HIP(hipMemcpyAsync(c.scal.ptr + 1, c.cloth_bvh.total.ptr, 4, hipMemcpyDeviceToDevice, s));
-        XRT_CHECK_HIP(LbvhFindPairsAt(c.cloth_bvh, c.obs_bvh, kPairThickness, c.pairs.ptr, c.scal.ptr,
-                                      c.pair_cap, s));
-      } else {
-        XRT_CHECK_HIP(hipMemsetAsync(c.scal.ptr + 1, 0, 4, s));
-      }
-      XRT_CHECK_HIP(LbvhCountPairs(c.cloth_bvh, c.cloth_bvh, kPairThickness, s));
-      XRT_CHECK_HIP(LbvhFindPairsAt(c.cloth_bvh, c.cloth_bvh, kPairThickness, c.pairs.ptr,
-                                    c.scal.ptr + 1, c.pair_cap, s));
-      XRT_CHECK_HIP(LaunchAddTotals(s, c.scal.ptr + 1, c.cloth_bvh.total.ptr, c.scal.ptr + 2,
-                                    c.pair_cap, h.nt, ctx->overflow.ptr));
-    }
-    if (self) {
-      MeshState cs{x, x0, -1.0f};
-      MeshState os{c.o_x.ptr, c.o_x0.ptr, 1e-3f};  // obstacle nodes: inv mass 1e-3, not free
-      XRT_CHECK_HIP(LaunchRepulsion(s, ClothTopo(c, h), cs, ObsTopo(c), os, c.pairs.ptr, c.scal.ptr + 1,
-                                    c.scal.ptr + 2, c.pair_cap, p.pbd_thickness, c.dq.ptr, c.cnt.ptr));
-      XRT_CHECK_HIP(LaunchApplyFixedDeltas(s, x, d.nv, c.dq.ptr, c.cnt.ptr, p.repulsion_relaxation));
-    }
+  const int step = c.frame_substep * p.num_iterations + it;
+  if (has_obs && step % frequency == 0) {
+    if ((rc = PhaseObstacleInterpolate(ctx, inst, c.pass_cnt, p.num_collision_passes)) != XRT_OK ||
+        (rc = PhaseObstacleMidstepNormals(ctx, inst)) != XRT_OK ||
+        (rc = PhaseObstacleRefitMidstep(ctx, inst, p.bvh_tolerance)) != XRT_OK)
+      return rc;
+    for (int kind = 0; kind < 3; ++kind)
+      if ((rc = PhaseGenerate(ctx, inst, kind)) != XRT_OK) return rc;
+    c.pass_cnt++;
+    if (pass_ran) *pass_ran = true;
   }
+  if (self && it % interval == 0 && (rc = PhaseUpdatePairs(ctx, inst, kPairThickness)) != XRT_OK)
+    return rc;
+  if (self) return PhaseRepulsion(ctx, inst, p.pbd_thickness, p.repulsion_relaxation);
   return XRT_OK;
 }
 
@@ -564,6 +617,16 @@ int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendM
   return ContactSubstepEnd(ctx, inst);
 }
 
+// ApplyDeltas of one instance's Jacobi sums x relaxation factor (sums cleared)
+int ApplyInstanceDeltas(xrt_ctx* ctx, int32_t inst) {
+  if (!ctx->contact[inst]) return XRT_OK;  // no contact state: nothing was accumulated
+  ContactData& c = *ctx->contact[inst];
+  const InstDesc& d = ctx->inst[inst].desc;
+  XRT_CHECK_HIP(LaunchApplyFixedDeltas(ctx->stream, ctx->x.ptr + d.vbeg, d.nv, c.dq.ptr, c.cnt.ptr,
+                                       ctx->params.relaxation_factor));
+  return XRT_OK;
+}
+
 }  // namespace xrt
 
 using namespace xrt;
@@ -691,6 +754,132 @@ int xrt_get_contacts(xrt_ctx* ctx, int32_t inst, int32_t* vf_face, float* vf_q, 
   }
   XRT_CHECK_HIP(hipStreamSynchronize(s));
   return XRT_OK;
+}
+
+
+// ------------------------------------------------ reference-shaped collider entry points
+// One call per reference method, on one instance's contact state; each launches exactly the
+// kernels the plain driver launches for that step, so a caller that keeps its own
+// SimulateSwift / SimulateQuality loop reproduces xrt_substep_plain bit for bit.
+namespace {
+int ColliderInst(xrt_ctx* ctx, int32_t inst, bool need_obstacle) {
+  XRT_REQUIRE(ctx, XRT_ERR_INVALID_ARG, "null context");
+  XRT_REQUIRE(inst >= 0 && inst < static_cast<int32_t>(ctx->inst.size()), XRT_ERR_INVALID_ARG,
+              "instance out of range");
+  XRT_REQUIRE(ctx->started, XRT_ERR_STATE, "xrt_start has not been called");
+  XRT_REQUIRE(ctx->contact[inst] != nullptr, XRT_ERR_STATE,
+              "instance has no contact state (no obstacle and self contact off at xrt_start)");
+  XRT_REQUIRE(!need_obstacle || ctx->contact[inst]->obs_nf > 0, XRT_ERR_STATE,
+              "instance has no obstacle mesh");
+  return XRT_OK;
+}
+}  // namespace
+
+int xrt_obstacle_bvh_construct(xrt_ctx* ctx, int32_t inst) {
+  int rc = ColliderInst(ctx, inst, true);
+  if (rc != XRT_OK) return rc;
+  ContactData& c = *ctx->contact[inst];
+  XRT_CHECK_HIP(LbvhBuild(c.obs_bvh, c.o_x0.ptr, nullptr, nullptr, 0.0f, 0, 0.0f, ctx->stream));
+  return XRT_OK;
+}
+
+int xrt_cloth_bvh_construct(xrt_ctx* ctx, int32_t inst) {
+  int rc = ColliderInst(ctx, inst, false);
+  if (rc != XRT_OK) return rc;
+  XRT_CHECK_HIP(LbvhBuild(ctx->contact[inst]->cloth_bvh, ctx->x0.ptr + ctx->inst[inst].desc.vbeg,
+                          nullptr, nullptr, 0.0f, 0, 0.0f, ctx->stream));
+  return XRT_OK;
+}
+
+int xrt_obstacle_interpolate(xrt_ctx* ctx, int32_t inst, int32_t cnt, int32_t num_passes) {
+  int rc = ColliderInst(ctx, inst, true);
+  if (rc != XRT_OK) return rc;
+  XRT_REQUIRE(num_passes > 0 && cnt >= 0, XRT_ERR_INVALID_ARG, "num_passes must be > 0, cnt >= 0");
+  return PhaseObstacleInterpolate(ctx, inst, cnt, num_passes);
+}
+
+int xrt_obstacle_update_midstep_normals(xrt_ctx* ctx, int32_t inst) {
+  int rc = ColliderInst(ctx, inst, true);
+  if (rc != XRT_OK) return rc;
+  return PhaseObstacleMidstepNormals(ctx, inst);
+}
+
+int xrt_obstacle_bvh_refit_midstep(xrt_ctx* ctx, int32_t inst, float tolerance) {
+  int rc = ColliderInst(ctx, inst, true);
+  if (rc != XRT_OK) return rc;
+  XRT_REQUIRE(std::isfinite(tolerance), XRT_ERR_INVALID_ARG, "non-finite tolerance");
+  return PhaseObstacleRefitMidstep(ctx, inst, tolerance);
+}
+
+int xrt_obstacle_predicted_to_positions(xrt_ctx* ctx, int32_t inst) {
+  int rc = ColliderInst(ctx, inst, true);
+  if (rc != XRT_OK) return rc;
+  ContactData& c = *ctx->contact[inst];
+  XRT_CHECK_HIP(hipMemcpyAsync(c.o_x0.ptr, c.o_x.ptr, sizeof(float4) * c.obs_nv,
+                               hipMemcpyDeviceToDevice, ctx->stream));
+  return XRT_OK;
+}
+
+int xrt_vf_dcd_generate(xrt_ctx* ctx, int32_t inst) {
+  int rc = ColliderInst(ctx, inst, true);
+  return rc != XRT_OK ? rc : PhaseGenerate(ctx, inst, 0);
+}
+
+int xrt_vf_ccd_generate(xrt_ctx* ctx, int32_t inst) {
+  int rc = ColliderInst(ctx, inst, true);
+  return rc != XRT_OK ? rc : PhaseGenerate(ctx, inst, 1);
+}
+
+int xrt_ee_dcd_generate(xrt_ctx* ctx, int32_t inst) {
+  int rc = ColliderInst(ctx, inst, true);
+  return rc != XRT_OK ? rc : PhaseGenerate(ctx, inst, 2);
+}
+
+int xrt_vf_dcd_solve(xrt_ctx* ctx, int32_t inst) {
+  int rc = ColliderInst(ctx, inst, true);
+  if (rc != XRT_OK) return rc;
+  ContactData& c = *ctx->contact[inst];
+  const InstDesc& d = ctx->inst[inst].desc;
+  XRT_CHECK_HIP(LaunchVfSolve(ctx->stream, ctx->x.ptr + d.vbeg, d.nv, c.obs_bvh, c.vf_face.ptr,
+                              c.vf_q.ptr, 1e-3f));
+  return XRT_OK;
+}
+
+int xrt_vf_ccd_solve(xrt_ctx* ctx, int32_t inst) {
+  int rc = ColliderInst(ctx, inst, true);
+  if (rc != XRT_OK) return rc;
+  ContactData& c = *ctx->contact[inst];
+  const InstDesc& d = ctx->inst[inst].desc;
+  XRT_CHECK_HIP(LaunchVfSolve(ctx->stream, ctx->x.ptr + d.vbeg, d.nv, c.obs_bvh, c.rt_face.ptr,
+                              c.rt_q.ptr, 0.0f));
+  return XRT_OK;
+}
+
+int xrt_ee_dcd_solve(xrt_ctx* ctx, int32_t inst) {
+  int rc = ColliderInst(ctx, inst, true);
+  if (rc != XRT_OK) return rc;
+  ContactData& c = *ctx->contact[inst];
+  const HostInstance& h = ctx->inst[inst];
+  XRT_CHECK_HIP(LaunchEeSolve(ctx->stream, ctx->x.ptr + h.desc.vbeg, c.c_edges.ptr, h.ne(),
+                              reinterpret_cast<EeRec*>(c.ee.ptr), c.dq.ptr, c.cnt.ptr));
+  return XRT_OK;
+}
+
+int xrt_repulsion_update_pairs(xrt_ctx* ctx, int32_t inst, float thickness) {
+  int rc = ColliderInst(ctx, inst, false);
+  if (rc != XRT_OK) return rc;
+  XRT_REQUIRE(std::isfinite(thickness) && thickness >= 0.0f, XRT_ERR_INVALID_ARG,
+              "thickness must be finite and >= 0");
+  return PhaseUpdatePairs(ctx, inst, thickness);
+}
+
+int xrt_repulsion_solve_no_proximity(xrt_ctx* ctx, int32_t inst, float thickness,
+                                     float relaxation_rate) {
+  int rc = ColliderInst(ctx, inst, false);
+  if (rc != XRT_OK) return rc;
+  XRT_REQUIRE(std::isfinite(thickness) && std::isfinite(relaxation_rate), XRT_ERR_INVALID_ARG,
+              "non-finite thickness or relaxation rate");
+  return PhaseRepulsion(ctx, inst, thickness, relaxation_rate);
 }
 
 }  // extern "C"

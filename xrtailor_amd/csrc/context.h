@@ -305,6 +305,8 @@ int ContactResizeLists(xrt_ctx* ctx);
 // synchronizes the stream, then XRT_ERR_CAPACITY (and clears the word) if an instance's
 // face pairs exceeded the pair capacity since the last check
 int CheckPairOverflow(xrt_ctx* ctx);
+// ApplyDeltas of one instance's contact Jacobi sums (no-op without contact state)
+int ApplyInstanceDeltas(xrt_ctx* ctx, int32_t inst);
 int SubstepPlainContact(xrt_ctx* ctx, int inst, const StepConsts& k, const BendMode& bm);
 // contact phases of one instance's substep (shared by the plain and the wide drivers)
 int ContactSubstepBegin(xrt_ctx* ctx, int inst);

@@ -181,6 +181,77 @@ class SpatialHash:
                 "cell_spacing": float(info[1])}
 
 
+class _InstanceCall:
+    def __init__(self, solver: "ClothSolver", inst: int):
+        self._s, self._i = solver, inst
+
+    def _call(self, name: str, *args) -> None:
+        check(getattr(self._s.lib, name)(self._s.ctx, self._i, *args))
+
+
+class VFConstraint(_InstanceCall):
+    """PBDCollision::DCD::VFConstraint of one instance against its obstacle tree
+    (xrtailor/include/xrtailor/physics/pbd_collision/Constraint.cuh:15-30)."""
+
+    def Generate(self) -> None:
+        self._call("xrt_vf_dcd_generate")
+
+    def Solve(self) -> None:
+        self._call("xrt_vf_dcd_solve")
+
+
+class RTConstraint(_InstanceCall):
+    """PBDCollision::CCD::RTConstraint (VF continuous collision), Constraint.cuh:54-69."""
+
+    def Generate(self) -> None:
+        self._call("xrt_vf_ccd_generate")
+
+    def Solve(self) -> None:
+        self._call("xrt_vf_ccd_solve")
+
+
+class EEConstraint(_InstanceCall):
+    """PBDCollision::DCD::EEConstraint, Constraint.cuh:32-48; Solve accumulates the deltas
+    that ClothSolver.ApplyDeltas applies."""
+
+    def Generate(self) -> None:
+        self._call("xrt_ee_dcd_generate")
+
+    def Solve(self) -> None:
+        self._call("xrt_ee_dcd_solve")
+
+
+class PBDRepulsionSolver(_InstanceCall):
+    """PBDRepulsionSolver of one instance (xrtailor/include/xrtailor/physics/repulsion/
+    PBDRepulsion.cuh:11-40)."""
+
+    def UpdatePairs(self, thickness: float) -> None:
+        self._call("xrt_repulsion_update_pairs", thickness)
+
+    def SolveNoProximity(self, thickness: float, relaxation_rate: float) -> None:
+        self._call("xrt_repulsion_solve_no_proximity", thickness, relaxation_rate)
+
+
+class ObstacleMesh(_InstanceCall):
+    """The obstacle PhysicsMesh of one instance and its LBVH (PhysicsMesh.cuh:84-128,
+    BVH.cuh:90-96)."""
+
+    def Interpolate(self, step: int, num_steps: int) -> None:
+        self._call("xrt_obstacle_interpolate", step, num_steps)
+
+    def UpdateMidstepNormals(self) -> None:
+        self._call("xrt_obstacle_update_midstep_normals")
+
+    def PredictedToPositions(self) -> None:
+        self._call("xrt_obstacle_predicted_to_positions")
+
+    def ConstructBVH(self) -> None:
+        self._call("xrt_obstacle_bvh_construct")
+
+    def RefitMidstep(self, tolerance: float) -> None:
+        self._call("xrt_obstacle_bvh_refit_midstep", tolerance)
+
+
 class ClothSolver:
     """Owns one xrt_ctx (device state of one or more cloth instances)."""
 
@@ -315,6 +386,25 @@ class ClothSolver:
 
     def Start(self, seed: int = 0) -> None:
         check(self.lib.xrt_start(self.ctx, C.c_uint64(seed)))
+
+    # ------------------------------------------ reference-shaped collider objects
+    def VFConstraint(self, inst: int) -> VFConstraint:
+        return VFConstraint(self, inst)
+
+    def RTConstraint(self, inst: int) -> RTConstraint:
+        return RTConstraint(self, inst)
+
+    def EEConstraint(self, inst: int) -> EEConstraint:
+        return EEConstraint(self, inst)
+
+    def PBDRepulsionSolver(self, inst: int) -> PBDRepulsionSolver:
+        return PBDRepulsionSolver(self, inst)
+
+    def Obstacle(self, inst: int) -> ObstacleMesh:
+        return ObstacleMesh(self, inst)
+
+    def ConstructClothBVH(self, inst: int) -> None:
+        check(self.lib.xrt_cloth_bvh_construct(self.ctx, inst))
 
     # ----------------------------------------------------------------- query
     def num_instances(self) -> int:
