@@ -60,9 +60,12 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
-def measured_traffic(kernel: str, workload: str):
+def measured_traffic(kernel: str, workload: str, algorithmic: float):
     """HBM bytes per launch of `kernel` on `workload` from the committed PMC summary
-    (FETCH_SIZE and WRITE_SIZE per dispatch, already scaled by the copy calibration)."""
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy calibration;
+    the counters need their own profiler runs, so they cannot come from this run).  Only a
+    record whose algorithmic bytes per launch match this run's within 2 % is used (the
+    record describes the same launches); otherwise null."""
     try:
         with open(TRAFFIC_FILE) as f:
             rec = json.load(f)
@@ -70,6 +73,9 @@ def measured_traffic(kernel: str, workload: str):
         return None
     for r in rec.get("kernels", []):
         if r.get("kernel") == kernel and r.get("workload") == workload:
+            alg = r.get("algorithmic_bytes_per_launch")
+            if not alg or abs(alg - algorithmic) > 0.02 * algorithmic:
+                return None
             return r.get("bytes_per_launch")
     return None
 
@@ -85,24 +91,114 @@ def algorithmic_bytes(solver, insts) -> tuple[int, int]:
     return total, nv
 
 
+def substep_bytes(solver, insts, specs, params, pairs: int, contacts: bool) -> dict:
+    """Compulsory HBM bytes of one substep of every instance (the whole-substep roofline):
+    state read + written once (64 B per vertex), constraint records read once (16 B per
+    stretch, 24 B per bend, 12 B per LRA constraint) and, with contacts, the obstacle
+    positions (16 B per obstacle vertex), the LBVH builds (40 B per node, 2F - 1 nodes; the
+    frame-start rebuilds amortized over the frame's substeps), the obstacle refits of the
+    collision passes and the cloth-tree updates of the pair refreshes, 8 B per face pair
+    (written and read back) and the contact records of each collision pass (16 B per VF-DCD
+    and VF-CCD record, i.e. per cloth vertex, 48 B per EE record, i.e. per cloth edge).
+    Intermediates (the iterate, multipliers, Jacobi sums, repulsion test items) are not
+    counted."""
+    n = [solver.counts(i) for i in insts]
+    N = sum(c.nv for c in n)
+    E = sum(c.n_edges for c in n)
+    B = sum(c.n_bends for c in n)
+    K = sum(c.lra_constraints for c in n)
+    F = sum(c.nt for c in n)
+    dyn = 64 * N + 16 * E + 24 * B + 12 * K
+    out = {"vertices": N, "edges": E, "bends": B, "lra": K, "faces": F, "dynamics": dyn,
+           "contact": 0}
+    if contacts:
+        obs = [sp.obstacle for sp in specs if sp.obstacle is not None]
+        n_obs = sum(len(o[0]) for o in obs)
+        f_obs = sum(len(o[1]) for o in obs)
+        obs_tree = 40 * (2 * f_obs - len(obs))              # every obstacle tree's nodes
+        cloth_tree = 40 * (2 * F - len(n))                  # every cloth tree's nodes
+        ns, ni = params.num_substeps, params.num_iterations
+        passes = params.num_collision_passes / ns          # collision passes per substep
+        self_c = bool(params.enable_self_contact)
+        refreshes = (ni + params.pair_update_interval - 1) // params.pair_update_interval
+        trees = obs_tree * (1.0 / ns + passes)              # frame build + midstep refits
+        if self_c:
+            trees += cloth_tree * (1.0 / ns + refreshes)    # frame build + pair-refresh updates
+        records = passes * (2 * 16 * N + 48 * E)
+        contact = 16 * n_obs + trees + records + (8 * pairs * refreshes if self_c else 0)
+        out.update({"obstacle_vertices": n_obs, "obstacle_faces": f_obs, "pairs": pairs,
+                    "contact": contact})
+    out["bytes_per_substep"] = out["dynamics"] + out["contact"]
+    out["bytes_per_vertex_substep"] = out["bytes_per_substep"] / N
+    return out
+
+
+def rollout_leg(solver, insts, specs, frames: int, plain_at) -> dict:
+    """The workload's whole rollout from its initial state: `frames` frames, each timed with
+    HIP events on the solver stream.  The frames in `plain_at` run the plain path instead of
+    the captured batched frame; both paths give identical states, so the trajectory is the
+    same either way and the plain samples spread over the rollout."""
+    import numpy as np
+
+    for i, sp in zip(insts, specs):
+        solver.SetState(i, sp.x0, np.zeros_like(sp.x0))
+    bms, pms = [], []
+    for f in range(frames):
+        if f in plain_at:
+            pms.append(solver.time_frames(False, 1))
+        else:
+            bms.append(solver.time_frames(True, 1))
+        if (f + 1) % 20 == 0:
+            print(f"[bench] rollout: {f + 1} / {frames} frames", file=sys.stderr, flush=True)
+    vs = sum(len(sp.pos) for sp in specs) * solver.params.num_substeps
+    b = np.array(bms)
+    return {"frames": frames, "batched_frames": len(bms), "plain_frames_sampled": sorted(plain_at),
+            "batched_ms_per_frame": {"mean": float(b.mean()), "min": float(b.min()),
+                                     "median": float(np.median(b)), "max": float(b.max())},
+            "plain_ms_per_frame": [float(x) for x in pms],
+            "batched_value": vs * len(bms) / (b.sum() * 1e-3),
+            "plain_value": vs * len(pms) / (sum(pms) * 1e-3) if pms else None,
+            "batched_over_plain": float(np.mean(pms) / b.mean()) if pms else None}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline_and_parity(specs, params, seed, contacts, budget_s=12.0, before_start=None,
-                            twin_setup=None):
-    """CPU baseline leg: the serial FP64 oracle (1 core) advances each instance by one
-    substep from the initial state (with its obstacle and self contact when `contacts`)
-    until the time budget is spent; the GPU's batched substep from the same state is
-    compared with it (relative error)."""
+                            twin_setup=None, parallel=True):
+    """CPU baseline leg and parity.  The GPU's batched substep from the initial state is
+    compared with the FP64 oracle twin of each instance (relative error).
+    * serial: the oracle (1 core) advances instances one after another until the time
+      budget is spent (the reported cpu_baseline);
+    * parallel (no twin set-up hook): every instance, instance-parallel over the host's
+      cores (one worker process per core, at most 16), timed by the busiest worker's
+      substep seconds (set-up excluded), so parity covers every instance."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from gpu_helpers import oracle_of, rel_err
     from oracle import oracle as orc
     from oracle import scene as osc
+    from oracle_pool import default_workers, instance_payload, parallel_substeps
     from xrtailor_amd.solver import build_solver
 
     solver, insts = build_solver(specs, params, seed=seed, obstacles=contacts,
                                  before_start=before_start)
     solver.SubstepBatched()
     solver.sync()
+    gpu = [solver.GetState(i)[0] for i in insts]
+    payloads = None
+    if parallel and twin_setup is None:
+        payloads = [instance_payload(solver, i, sp, params, obstacle=contacts)
+                    for i, sp in zip(insts, specs)]
     op, dt = orc.params_from(params)
     done_vs, t_used, n_inst, worst = 0, 0.0, 0, 0.0
     t_wall = time.perf_counter()
@@ -122,7 +218,7 @@ def cpu_baseline_and_parity(specs, params, seed, contacts, budget_s=12.0, before
             t0 = time.perf_counter()
             oc.substep(op, dt)
         t_used += time.perf_counter() - t0
-        worst = max(worst, rel_err(solver.GetState(i)[0], oc.state()[0]))
+        worst = max(worst, rel_err(gpu[n_inst], oc.state()[0]))
         done_vs += oc.nv
         n_inst += 1
         if n_inst % 16 == 0:
@@ -138,11 +234,26 @@ def cpu_baseline_and_parity(specs, params, seed, contacts, budget_s=12.0, before
                 else "with body and self contact")
     if params.constraint_model == 1:
         what += ", FEM constraints"
+    model, ncpu = cpu_model(), os.cpu_count()
     base = {"value": done_vs / t_used, "unit": "vertex-substeps/s", "cores": 1, "kind": "port",
             "sample": f"one substep ({what}) of the first {n_inst} of {len(specs)} instances "
                       f"({done_vs} vertex-substeps, {t_used:.1f} s), serial FP64 oracle; "
-                      f"{os.cpu_count()} host cores present"}
-    return base, worst, n_inst
+                      f"host CPU {model}, {ncpu} cores present"}
+    par = None
+    if payloads is not None:
+        workers = default_workers()
+        print(f"[bench] cpu leg: every instance on {workers} worker processes", file=sys.stderr,
+              flush=True)
+        out, busy = parallel_substeps(payloads, 1, workers)
+        errs = [rel_err(g, x) for g, (x, _, _) in zip(gpu, out)]
+        worst, n_inst = max(errs), len(errs)
+        vs = sum(len(sp.pos) for sp in specs)
+        par = {"value": vs / busy, "unit": "vertex-substeps/s", "cores": workers, "kind": "port",
+               "sample": f"one substep ({what}) of all {len(specs)} instances ({vs} vertex-"
+                         f"substeps), FP64 oracle instance-parallel on {workers} worker "
+                         f"processes, timed by the busiest worker ({busy:.1f} s of substeps, "
+                         f"set-up excluded); host CPU {model}, {ncpu} cores present"}
+    return base, par, worst, n_inst
 
 
 def adjoint_leg(frames: int = 60, checkpoint_every: int = 10, n: int = 128, seed: int = 5) -> dict:
@@ -202,8 +313,15 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--plain-frames", type=int, default=1)
+    ap.add_argument("--rollout-frames", type=int, default=120,
+                    help="batched workload: time the whole rollout (0: skip)")
+    ap.add_argument("--plain-at", default="0,40,80",
+                    help="rollout frames run by the plain path instead (comma separated)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes (no autotune)")
+    ap.add_argument("--no-cpu-parallel", action="store_true",
+                    help="skip the instance-parallel oracle leg (parity over the serial sample)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain path (profiling runs)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable rollout")
     ap.add_argument("--no-contacts", action="store_true",
@@ -279,6 +397,11 @@ def main() -> int:
     solver, insts = build_solver(specs, params, seed=a.seed, obstacles=contacts,
                                  before_start=before)
     bytes_sub, n_verts = algorithmic_bytes(solver, insts)
+    # launch shapes: the project's autotuner times its candidates on these instances (state
+    # restored afterwards) before anything is timed
+    tuning = None if a.no_tune else solver.autotune(3)
+    if tuning:
+        print(f"[bench] autotune: {tuning}", file=sys.stderr, flush=True)
     # garment: a step skins every body (joint transforms of frame f, uploaded) then runs the
     # frame; the transforms of every frame are prepared before the timed region
     T = None
@@ -399,7 +522,10 @@ def main() -> int:
         kern_ms = solver.time_batched_substep(max(10, a.steps))
         bytes_kern = bytes_sub
     achieved = bytes_kern / (kern_ms * 1e-3) / 1e9
-    traffic = measured_traffic(kernel, wname)
+    traffic = measured_traffic(kernel, wname, bytes_kern)
+    sb = substep_bytes(solver, insts, specs, params,
+                       rep_counts["pairs"] if rep_counts else 0, contacts)
+    sub_gbs = value * sb["bytes_per_vertex_substep"] / 1e9
 
     # plain path on the same workload
     plain = None
@@ -424,9 +550,17 @@ def main() -> int:
         plain = n_verts * params.num_substeps * a.plain_frames / (pms * 1e-3)
         print(f"[bench] plain {pms:.1f} ms / {a.plain_frames} frame(s)", file=sys.stderr,
               flush=True)
+    rollout = None
+    if a.workload == "batched" and a.rollout_frames > 0 and not a.no_graph:
+        plain_at = set() if a.no_plain else {int(f) for f in a.plain_at.split(",") if f.strip()}
+        print(f"[bench] rollout of {a.rollout_frames} frames (plain at {sorted(plain_at)})",
+              file=sys.stderr, flush=True)
+        rollout = rollout_leg(solver, insts, specs, a.rollout_frames, plain_at)
+        print(f"[bench] rollout: batched {rollout['batched_value']:.4e} vs/s, "
+              f"ratio {rollout['batched_over_plain']}", file=sys.stderr, flush=True)
     solver.close()
 
-    cpu, parity, parity_n = None, None, 0
+    cpu, cpu_par, parity, parity_n = None, None, None, 0
     if not a.no_cpu:
         tw_setup = None
         if garment:
@@ -443,10 +577,12 @@ def main() -> int:
                                (params.fem_xx_stiffness, params.fem_yy_stiffness,
                                 params.fem_xy_stiffness, params.fem_xy_poisson_ratio,
                                 params.fem_yx_poisson_ratio), params.fem_bending_stiffness)
-        cpu, parity, parity_n = cpu_baseline_and_parity(specs, params, a.seed, contacts,
-                                                        a.cpu_budget, before, tw_setup)
-        print(f"[bench] cpu {cpu['value']:.3e} vs/s, parity {parity:.3e}", file=sys.stderr,
-              flush=True)
+        cpu, cpu_par, parity, parity_n = cpu_baseline_and_parity(
+            specs, params, a.seed, contacts, a.cpu_budget, before, tw_setup,
+            parallel=not a.no_cpu_parallel)
+        print(f"[bench] cpu {cpu['value']:.3e} vs/s (1 core)"
+              + (f", {cpu_par['value']:.3e} vs/s ({cpu_par['cores']} cores)" if cpu_par else "")
+              + f", parity {parity:.3e} over {parity_n} instances", file=sys.stderr, flush=True)
 
     adj = None
     if not a.no_adjoint:
@@ -486,8 +622,16 @@ def main() -> int:
                      "kernel": kernel, "kernel_ms": kern_ms, "bytes_per_launch": bytes_kern,
                      "repulsion_counts": rep_counts, "particle": particle,
                      "binding": binding, "fem": fem},
+        "roofline_substep": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                             "achieved": sub_gbs, "frac": sub_gbs / HBM_PEAK_GBS,
+                             "ceiling_vertex_substeps_per_s":
+                                 HBM_PEAK_GBS * 1e9 / sb["bytes_per_vertex_substep"],
+                             **sb},
+        "rollout": rollout,
+        "tuning": tuning,
         "record": record,
         "cpu_baseline": cpu,
+        "cpu_baseline_parallel": cpu_par,
         "adjoint_forward_ratio": adj["substep_ratio"] if adj else None,
         "adjoint": adj,
         "device": info,

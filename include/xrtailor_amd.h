@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define XRT_ABI_VERSION 4
+#define XRT_ABI_VERSION 5
 
 enum {
   XRT_OK = 0,
@@ -110,7 +110,32 @@ typedef struct xrt_params {
   float fem_xy_poisson_ratio;
   float fem_yx_poisson_ratio;
   float fem_bending_stiffness;
+  /* friction of the body-cloth vertex-face contacts (VF-DCD and VF-CCD solves), >= 0: each
+   * correction removes the tangential part of the vertex's substep displacement relative to
+   * the obstacle face (its vertices' mean frame displacement / substeps), at most
+   * contact_friction x |correction| (ComputeFriction, ClothSolverHelper.cu:156-169, applied
+   * to the contact correction).  The reference applies no friction on mesh contacts: 0
+   * (default) is its behaviour.  Edge-edge contacts and self-contact repulsion have none. */
+  float contact_friction;
 } xrt_params;
+
+/* Launch shapes of the batched substep (no counterpart in the reference, whose launch
+ * shapes are fixed).  None of them changes a result: they change how many threads share
+ * an instance's colors or how many independent constraints of one color a thread loads
+ * before solving them. */
+typedef struct xrt_tuning {
+  int32_t persistent_block;  /* threads per workgroup of the persistent substep (no contacts):
+                              * a multiple of 64 in [64, 1024]; default 1024 */
+  int32_t persistent_unroll; /* constraints per thread per round in the persistent substep:
+                              * 0 default (2 stretch / 1 bend in LDS, 4 / 2 in global memory,
+                              * 4 FEM), 1 light (1 / 1, 2 / 1, 2), 2 heavy (4 / 2, 8 / 4, 8) */
+  int32_t wide_per_thread;   /* stretch / bend elements per thread of the wide launches (the
+                              * batched substep with contacts): 1 (default) or 2 */
+  int32_t path;              /* set by xrt_autotune: 0 persistent substep, 1 wide substep */
+  float best_ms;             /* set by xrt_autotune: ms per substep of the chosen shapes */
+  float default_ms;          /* set by xrt_autotune: ms per substep of the defaults */
+  int32_t candidates;        /* set by xrt_autotune: configurations timed */
+} xrt_tuning;
 
 enum { XRT_BEND_REFERENCE = 0, XRT_BEND_CONSISTENT = 1 };
 enum { XRT_COLOR_GREEDY = 0, XRT_COLOR_PALETTE = 1 };
@@ -359,6 +384,18 @@ int xrt_substep_batched(xrt_ctx* ctx);
 /* `nframes` frames of all instances; the frame's substep launches are captured once in
  * a hipGraph and replayed */
 int xrt_frames_batched(xrt_ctx* ctx, int32_t nframes);
+/* Autotuner (no counterpart in the reference): times every candidate launch shape of the
+ * path the context's batched substep takes (persistent: workgroup size 1024 / 768 / 512 /
+ * 256 x unroll 0 / 1 / 2; wide: 1 or 2 elements per thread) over `substeps` batched
+ * substeps of the context's own instances, each from the current state, and keeps the
+ * fastest.  The state (positions, velocities, frame position, obstacle frame-start
+ * positions, hash schedule) is restored afterwards, so results are those of an untuned
+ * run; the next batched frame re-captures its graph.  out (may be NULL) receives the
+ * choice and the timings. */
+int xrt_autotune(xrt_ctx* ctx, int32_t substeps, xrt_tuning* out);
+int xrt_get_tuning(xrt_ctx* ctx, xrt_tuning* out);
+/* applies persistent_block / persistent_unroll / wide_per_thread (validated) */
+int xrt_set_tuning(xrt_ctx* ctx, const xrt_tuning* t);
 /* `nframes` batched frames (as xrt_frames_batched) with the per-frame cache of
  * ClothSolverUniversal::SaveCache (xrtailor/src/pipeline/impl/universal/ClothSolver.cpp:
  * 107-144): after every frame the positions of every instance (instance order, float

@@ -312,16 +312,34 @@ void orc_ee_dcd_generate(const orc_bvh* obs, const int32_t* obs_tris, const uint
 }
 
 /* SolveVFCCD_Kernel / SolveVFDCD_Kernel (PBDCollision.cu:280-300, 54-76): per vertex push
- * along the pred face normal to the contact point (+1e-3 offset for DCD) */
+ * along the pred face normal to the contact point (+1e-3 offset for DCD).  With mu > 0 the
+ * build's contact friction: ComputeFriction (ClothSolverHelper.cu:156-169) of that
+ * correction on the vertex's substep displacement x - x0 relative to the obstacle face's
+ * (the mean of its vertices' (o_x - o_x0) times inv_ns), tangential part only. */
 void orc_vf_solve(const float* obs_pred, const int32_t* obs_tris, const int32_t* face,
-                  const double* q, double* x, int nv, double offset) {
+                  const double* q, double* x, int nv, double offset, const double* x0,
+                  const float* o_x0, const float* o_x, double inv_ns, double mu) {
   for (int i = 0; i < nv; i++) {
     if (face[i] < 0) continue;
     const int32_t* t = obs_tris + 3 * face[i];
     v3 n = face_normal(ldf(obs_pred, t[0]), ldf(obs_pred, t[1]), ldf(obs_pred, t[2]));
     v3 xi = ldd(x, i);
     double lam = dt3(n, sb(xi, ldd(q, i))) - offset;
-    if (lam < 0) std3(x, i, ad(xi, sc(n, -lam)));
+    if (lam >= 0) continue;
+    v3 f = {0.0, 0.0, 0.0};
+    if (mu > 0) {
+      v3 od = {0.0, 0.0, 0.0};
+      for (int k = 0; k < 3; k++) od = ad(od, sb(ldf(o_x, t[k]), ldf(o_x0, t[k])));
+      od = sc(od, inv_ns / 3.0);
+      v3 rel = sb(sb(xi, ldd(x0, i)), od);
+      v3 tan = sb(rel, sc(n, dt3(rel, n)));
+      double tl = sqrt(dt3(tan, tan));
+      if (tl > 0) {
+        double s = -lam * mu / tl;
+        f = sc(tan, -(s < 1.0 ? s : 1.0));
+      }
+    }
+    std3(x, i, ad(ad(xi, sc(n, -lam)), f));
   }
 }
 

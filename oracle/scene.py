@@ -74,7 +74,9 @@ def _lib():
         lib.orc_ee_dcd_generate.restype = None
         lib.orc_bvh_pred.argtypes = [C.c_void_p]
         lib.orc_bvh_pred.restype = C.POINTER(C.c_float)
-        lib.orc_vf_solve.argtypes = [C.POINTER(C.c_float), _I, _I, _D, _D, C.c_int, C.c_double]
+        _Fp = C.POINTER(C.c_float)
+        lib.orc_vf_solve.argtypes = [_Fp, _I, _I, _D, _D, C.c_int, C.c_double, _D, _Fp, _Fp,
+                                     C.c_double, C.c_double]
         lib.orc_vf_solve.restype = None
         lib.orc_ee_solve.argtypes = [_D, _D, _I, C.c_int, _D, _I]
         lib.orc_ee_solve.restype = None
@@ -204,6 +206,15 @@ class ContactScene:
         self.f_ncol = n_colors
         self.fem = True
 
+    def _vf_friction(self):
+        """Friction arguments of orc_vf_solve: the cloth's start-of-substep positions, the
+        obstacle's frame-start / frame-end positions, 1 / substeps, contact_friction."""
+        F = C.POINTER(C.c_float)
+        self._ox0 = np.ascontiguousarray(self.o_x0, np.float32)
+        self._ox = np.ascontiguousarray(self.o_x, np.float32)
+        return (_d(self.c.x0), self._ox0.ctypes.data_as(F), self._ox.ctypes.data_as(F),
+                1.0 / self.xp.num_substeps, float(getattr(self.xp, "contact_friction", 0.0)))
+
     def _rec(self, *entry):
         if self.tape is not None:
             self.tape.append(entry)
@@ -300,10 +311,11 @@ class ContactScene:
                 if self.tape is not None:
                     predc = np.ctypeslib.as_array(pred, (3 * len(self.o_x0),)).copy()
                     self._rec("vf", predc, self.rt_face.copy(), self.rt_q.copy(), c.x.copy(), 0.0)
-                lib.orc_vf_solve(pred, _i(ot), _i(self.rt_face), _d(self.rt_q), _d(c.x), self.nv, 0.0)
+                fr = self._vf_friction()
+                lib.orc_vf_solve(pred, _i(ot), _i(self.rt_face), _d(self.rt_q), _d(c.x), self.nv, 0.0, *fr)
                 if self.tape is not None:
                     self._rec("vf", predc, self.vf_face.copy(), self.vf_q.copy(), c.x.copy(), 1e-3)
-                lib.orc_vf_solve(pred, _i(ot), _i(self.vf_face), _d(self.vf_q), _d(c.x), self.nv, 1e-3)
+                lib.orc_vf_solve(pred, _i(ot), _i(self.vf_face), _d(self.vf_q), _d(c.x), self.nv, 1e-3, *fr)
                 self._rec("ee", self.ee.copy(), c.x.copy(), xp.relaxation_factor)
                 lib.orc_ee_solve(_d(self.ee), _d(c.x), _i(self.edges), self.ne, _d(self.delta), _i(self.count))
                 lib.orc_apply_deltas(_d(c.x), _d(self.delta), _i(self.count), self.nv, xp.relaxation_factor)

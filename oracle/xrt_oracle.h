@@ -171,7 +171,8 @@ void orc_vf_ccd_generate(const orc_bvh* obs, const int32_t* obs_tris, const doub
 void orc_ee_dcd_generate(const orc_bvh* obs, const int32_t* obs_tris, const uint32_t* obs_rtri,
                          const double* x, const int32_t* edges, int ne, double* rec12);
 void orc_vf_solve(const float* obs_pred, const int32_t* obs_tris, const int32_t* face,
-                  const double* q, double* x, int nv, double offset);
+                  const double* q, double* x, int nv, double offset, const double* x0,
+                  const float* o_x0, const float* o_x, double inv_ns, double mu);
 void orc_ee_solve(const double* rec12, const double* x, const int32_t* edges, int ne,
                   double* delta, int32_t* count);
 void orc_apply_deltas(double* x, double* delta, int32_t* count, int nv, double relax);

@@ -1,0 +1,288 @@
+"""The workloads at their stated shapes, on the GPU through the C ABI:
+
+* garment on an animated body: a 20,000-vertex tube (160 x 125) with its shoulder ring as
+  160 long-range attachment slots, around a 6,890-vertex capsule-chain body skinned by 24
+  joints, body and self contact;
+* the 256-instance batched workload (grid sides log-uniform in [32, 192], sphere / capsule
+  obstacles, self contact): every instance against the FP64 oracle after a graph-captured
+  frame of one substep, and graph-captured frames equal to the plain path on every
+  instance;
+* the differentiable rollout: a 128 x 128 cloth over a sphere, gradients against the FP64
+  reverse and central differences of the FP64 forward, checkpoint schedules, and a batch
+  of 32 rollouts against 32 single rollouts."""
+import numpy as np
+import pytest
+
+from gpu_helpers import rel_err
+from oracle import adjoint as oadj
+from oracle_pool import _twin, instance_payload, parallel_substeps
+from xrtailor_amd import scenes
+from xrtailor_amd.solver import ClothSolver, build_solver, make_params
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+# ------------------------------------------------------------ garment on an animated body
+def _garment_solver(xp):
+    spec, body = scenes.garment_on_body()
+
+    def before(s, ids):
+        s.SetObstacleSkinning(ids[0], *body.skinning(), body.amp.shape[0])
+    s, (i,) = build_solver([spec], xp, seed=1, before_start=before)
+    return spec, body, s, i
+
+
+def test_garment_on_body_stated_shape_substep_vs_oracle():
+    print("case garment on body: 20,000-vertex tube, 6,890-vertex 24-joint body, LRA shoulder "
+          "ring, body + self contact, one substep vs the oracle twin", flush=True)
+    xp = make_params(bend_model=1, enable_obstacle_contact=1, enable_self_contact=1,
+                     ground_height=-1.0)
+    spec, body, s, i = _garment_solver(xp)
+    n = s.counts(i)
+    assert (n.nv, n.lra_slots, n.lra_constraints) == (20000, 160, 160 * 20000)
+    assert len(body.pos) == 6890 and body.amp.shape[0] == 24
+    s.SkinObstacle(i, body.transforms(1.0 / 60.0))
+    o_x0, o_x = s.GetObstacle(i, len(body.pos))
+    skin_err = float(np.abs(o_x - body.at(1.0 / 60.0)).max())
+    print(f"  GPU skinning vs FP64 forward kinematics: {skin_err:.3e} m", flush=True)
+    assert skin_err <= 1e-6
+    cl, twin, _, _ = _twin(instance_payload(s, i, spec, xp, obstacle_x=o_x))
+    s.SubstepPlain(i)
+    twin.substep()
+    g = s.Contacts(i)
+    err = rel_err(s.GetState(i)[0], cl.state()[0])
+    print(f"  rel err {err:.3e}; pairs {g['pairs']} (twin {len(twin.pairs)}), VF-DCD "
+          f"{int((g['vf_face'] >= 0).sum())}", flush=True)
+    assert g["pairs"] == len(twin.pairs) and g["obs_pairs"] == twin.n_obs_pairs
+    np.testing.assert_array_equal(g["vf_face"], twin.vf_face)
+    assert g["pairs"] > 0
+    assert err <= TOL
+    s.close()
+
+
+def test_garment_on_body_stated_shape_batched_frames_equal_plain():
+    print("case garment on body: graph-captured frames vs plain, body skinned each frame",
+          flush=True)
+    xp = make_params(bend_model=1, enable_obstacle_contact=1, enable_self_contact=1,
+                     ground_height=-1.0)
+    states = []
+    for batched in (True, False):
+        spec, body, s, i = _garment_solver(xp)
+        for f in range(2):
+            s.SkinObstacle(i, body.transforms((f + 1) / 60.0))
+            s.FramesBatched(1) if batched else s.FramesPlain(1)
+        s.sync()
+        states.append(s.GetState(i))
+        s.close()
+    for a, b in zip(*states):
+        print(f"  max |batched - plain| = {np.abs(a - b).max():.3e}", flush=True)
+        np.testing.assert_array_equal(a, b)
+
+
+# ------------------------------------------------------------------ batched 256 instances
+def _batched_params(**kw):
+    base = dict(bend_model=1, ground_height=-1.0, enable_obstacle_contact=1, enable_self_contact=1)
+    base.update(kw)
+    return make_params(**base)
+
+
+def test_batched_workload_every_instance_vs_oracle():
+    """A graph-captured frame of one substep (frame time 1/600 s: the workload's substep;
+    one collision pass) of all 256 instances against the FP64 oracle twin of every
+    instance, run instance-parallel on the host."""
+    print("case batched 256 instances: graph frame of one substep vs the oracle, every "
+          "instance", flush=True)
+    specs = scenes.batched(seed=4, n_instances=256)
+    xp = _batched_params(num_substeps=1, frame_time=1.0 / 600.0, num_collision_passes=1)
+    s, ids = build_solver(specs, xp, seed=4)
+    payloads = [instance_payload(s, i, sp, xp) for i, sp in zip(ids, specs)]
+    s.FramesBatched(1)
+    s.sync()
+    gpu = [s.GetState(i)[0] for i in ids]
+    nv = [len(sp.pos) for sp in specs]
+    s.close()
+    out, busy = parallel_substeps(payloads)
+    errs = np.array([rel_err(g, x) for g, (x, _, _) in zip(gpu, out)])
+    k = int(np.argmax(errs))
+    print(f"  {sum(nv)} vertices; rel err max {errs.max():.3e} (instance {k}, {nv[k]} vertices),"
+          f" median {np.median(errs):.3e}; largest instance {max(nv)} vertices: "
+          f"{errs[int(np.argmax(nv))]:.3e}; oracle {busy:.1f} s on the busiest worker",
+          flush=True)
+    assert max(nv) == 34969
+    assert errs.max() <= TOL
+
+
+def test_batched_workload_every_instance_graph_equals_plain():
+    print("case batched 256 instances: one graph-captured frame (10 x 10) vs plain, every "
+          "instance bit for bit", flush=True)
+    specs = scenes.batched(seed=4, n_instances=256)
+    xp = _batched_params()
+    runs = []
+    for batched in (True, False):
+        s, ids = build_solver(specs, xp, seed=4)
+        s.FramesBatched(1) if batched else s.FramesPlain(1)
+        s.sync()
+        runs.append([s.GetState(i) for i in ids])
+        s.close()
+    worst = 0.0
+    for (xa, va), (xb, vb) in zip(*runs):
+        worst = max(worst, float(np.abs(xa - xb).max()), float(np.abs(va - vb).max()))
+    print(f"  max |batched - plain| over 256 instances = {worst:.3e}", flush=True)
+    assert worst == 0.0
+
+
+# ------------------------------------------------------------- differentiable rollout 128^2
+class Rollout:
+    """The 128 x 128 cloth over a sphere (icosphere s4, r 0.3), body contact, per-edge
+    compliances perturbed log-normally (sigma 0.5) from 1e-8."""
+
+    def __init__(self, xp, n_instances=1, seed=5):
+        self.xp = xp
+        self.spec = scenes.differentiable_drop(seed=seed, n=128)
+        s = ClothSolver(params=xp)
+        self.ids = []
+        for _ in range(n_instances):
+            i = s.AddCloth(self.spec.pos, self.spec.tris, self.spec.stretch_compliance,
+                           self.spec.bend_compliance)
+            s.SetObstacle(i, *self.spec.obstacle)
+            self.ids.append(i)
+        s.Start(seed)
+        self.s = s
+        self.ne = s.counts(self.ids[0]).n_edges
+        self.nv = len(self.spec.pos)
+
+    def compliance(self, k):
+        z = scenes.Rng(100 + k).normal(self.ne)
+        return (self.spec.stretch_compliance * np.exp(0.5 * z)).astype(np.float32)
+
+    def start(self, k):
+        x0 = (self.spec.pos + 1e-3 * scenes.Rng(200 + k).normal(3 * self.nv).reshape(-1, 3))
+        return x0.astype(np.float32), np.zeros((self.nv, 3), np.float32)
+
+    def seed_vectors(self, k):
+        r = scenes.Rng(300 + k)
+        return (r.normal(3 * self.nv).reshape(-1, 3).astype(np.float32),
+                (0.01 * r.normal(3 * self.nv)).reshape(-1, 3).astype(np.float32))
+
+    def grads(self, frames, ckpt, ks, starts=None):
+        s = self.s
+        for j, (i, k) in enumerate(zip(self.ids, ks)):
+            s.SetStretchCompliance(i, self.compliance(k))
+            x0, v0 = self.start(k) if starts is None else starts[j]
+            s.SetState(i, x0, v0)
+        s.AdjointBegin(frames, ckpt)
+        for i, k in zip(self.ids, ks):
+            s.AdjointSeed(i, *self.seed_vectors(k))
+        s.AdjointBackward()
+        return [dict(gs=s.GradCompliance(i)[0], gx0=s.GradState(i)[0], gv0=s.GradState(i)[1])
+                for i in self.ids]
+
+
+def _oracle_scene_of(r, i, k, x0, v0):
+    payload = instance_payload(r.s, i, r.spec, r.xp, x0=x0, v=v0)
+    payload["s_comp"] = r.compliance(k).astype(np.float64)
+    cl, sc, _, _ = _twin(payload)
+    return cl, sc
+
+
+def _relerr(g, ref):
+    g, ref = np.asarray(g, np.float64), np.asarray(ref, np.float64)
+    return float(np.abs(g - ref).max() / max(np.abs(ref).max(), 1e-30))
+
+
+def test_rollout_full_size_gradient_vs_fp64_reverse_with_contact():
+    """From a contact-active state (the cloth draped on the sphere after 10 GPU frames): one
+    frame's gradients w.r.t. the per-edge compliances, x0 and v0 against the FP64 reverse."""
+    print("case differentiable rollout 128x128: one contact frame, GPU adjoint vs FP64 reverse",
+          flush=True)
+    xp = make_params(bend_model=1, enable_obstacle_contact=1, ground_height=-1.0)
+    r = Rollout(xp)
+    i = r.ids[0]
+    x0, v0 = r.start(0)
+    r.s.SetStretchCompliance(i, r.compliance(0))
+    r.s.SetState(i, x0, v0)
+    r.s.FramesPlain(10)
+    xs, vs = r.s.GetState(i)
+    g = r.grads(1, 10, [0], starts=[(xs, vs)])[0]
+    cl, sc = _oracle_scene_of(r, i, 0, xs, vs)
+    tape = oadj.record(sc, xp.num_substeps)
+    n_vf = max(int((e[2] >= 0).sum()) for e in tape if e[0] == "vf")
+    print(f"  VF contacts in the frame: up to {n_vf} vertices", flush=True)
+    assert n_vf > 0, "no contact in the frame"
+    a, b = r.seed_vectors(0)
+    ref = oadj.backward(sc, tape, a, b)
+    worst = 0.0
+    for key, rk in (("gs", "gs_edge"), ("gx0", "gx0"), ("gv0", "gv0")):
+        e = _relerr(g[key], ref[rk])
+        print(f"  {key}: rel err {e:.3e}", flush=True)
+        worst = max(worst, e)
+    assert worst <= 1e-3
+
+
+def test_rollout_full_size_gradient_vs_central_differences():
+    """One frame from the release state: the GPU's directional derivatives along random
+    directions in log compliance and in v0 against central differences (eps 1e-3) of the
+    FP64 forward's loss L = a . x(T) + b . v(T)."""
+    print("case differentiable rollout 128x128: GPU adjoint vs central FD of the FP64 forward",
+          flush=True)
+    xp = make_params(bend_model=1, enable_obstacle_contact=1, ground_height=-1.0)
+    r = Rollout(xp)
+    i = r.ids[0]
+    g = r.grads(1, 10, [0])[0]
+    a, b = (v.astype(np.float64) for v in r.seed_vectors(0))
+    x0, v0 = r.start(0)
+    c0 = r.compliance(0).astype(np.float64)
+
+    def loss(comp, vel):
+        payload = instance_payload(r.s, i, r.spec, xp, x0=x0, v=vel)
+        payload["s_comp"] = comp
+        cl, sc, _, _ = _twin(payload)
+        for _ in range(xp.num_substeps):
+            sc.substep()
+        x, v = cl.state()
+        return float((a * x).sum() + (b * v).sum())
+
+    eps = 1e-3
+    rng = np.random.default_rng(9)
+    d = rng.standard_normal(r.ne)
+    fd = (loss(c0 * np.exp(eps * d), v0) - loss(c0 * np.exp(-eps * d), v0)) / (2 * eps)
+    ad = float((g["gs"].astype(np.float64) * c0 * d).sum())
+    e1 = abs(ad - fd) / max(abs(fd), 1e-30)
+    print(f"  log-compliance direction: adjoint {ad:.6e} fd {fd:.6e} rel {e1:.2e}", flush=True)
+    dv = rng.standard_normal((r.nv, 3))
+    fd = (loss(c0, v0 + eps * dv) - loss(c0, v0 - eps * dv)) / (2 * eps)
+    ad = float((g["gv0"].astype(np.float64) * dv).sum())
+    e2 = abs(ad - fd) / max(abs(fd), 1e-30)
+    print(f"  v0 direction: adjoint {ad:.6e} fd {fd:.6e} rel {e2:.2e}", flush=True)
+    assert max(e1, e2) <= 1e-3
+
+
+def test_rollout_full_size_60_frames_checkpoints():
+    """The full 60-frame rollout with a checkpoint every 10 frames gives the same gradients,
+    bit for bit, as one 60-frame segment, and they are finite and non-zero."""
+    print("case differentiable rollout 128x128: 60 frames, checkpoint 10 vs 60", flush=True)
+    xp = make_params(bend_model=1, enable_obstacle_contact=1, ground_height=-1.0)
+    r = Rollout(xp)
+    g10 = r.grads(60, 10, [0])[0]
+    g60 = r.grads(60, 60, [0])[0]
+    for key in g10:
+        assert np.isfinite(g10[key]).all() and np.abs(g10[key]).max() > 0
+        np.testing.assert_array_equal(g10[key], g60[key])
+
+
+def test_rollout_batch_of_32_equals_single_rollouts():
+    """32 rollouts of the 128 x 128 cloth (each its own compliances, start jitter and loss
+    seed) in one context give every rollout the gradients it gets alone."""
+    print("case differentiable rollout 128x128: batch of 32 vs 32 single rollouts, 6 frames, "
+          "checkpoint 3", flush=True)
+    xp = make_params(bend_model=1, enable_obstacle_contact=1, ground_height=-1.0)
+    batch = Rollout(xp, n_instances=32)
+    together = batch.grads(6, 3, list(range(32)))
+    batch.s.close()
+    single = Rollout(xp, n_instances=1)
+    for k in range(32):
+        alone = single.grads(6, 3, [k])[0]
+        for key in alone:
+            np.testing.assert_array_equal(together[k][key], alone[key])
+    print("  32 / 32 rollouts identical", flush=True)

@@ -218,3 +218,35 @@ def test_drop_cloth_full_size_single_substep():
     assert (g["vf_face"] >= 0).sum() > 0
     assert sens > 1e-5
     assert dev <= 2.0 * sens
+
+
+def test_contact_friction_substeps_vs_oracle_and_batched():
+    """Contact friction 0.3 on the VF corrections (the drop workload's friction): a cloth
+    sliding sideways onto a sphere, plain substeps against the oracle twin; the friction
+    changes the result (against friction 0) and the batched path equals the plain one."""
+    print("case contact friction 0.3: sliding drop on a sphere, plain vs oracle, batched vs "
+          "plain", flush=True)
+    spec, v0 = sphere_drop(height=0.302)
+    v0[:, 0] = 0.8                              # sliding along +x while landing
+    kw = dict(bend_model=1, enable_obstacle_contact=1, enable_self_contact=1, enable_ground=0,
+              num_substeps=4, num_iterations=6, num_collision_passes=4)
+    xp = make_params(contact_friction=0.3, **kw)
+    s, i, twin = _setup(spec, xp, v0)
+    errs = []
+    for _ in range(3):
+        s.SubstepPlain(i)
+        twin.substep()
+        errs.append(rel_err(s.GetState(i)[0], twin.c.state()[0]))
+    print(f"  rel err per substep {', '.join(f'{e:.2e}' for e in errs)}", flush=True)
+    assert errs[0] <= TOL and max(errs) <= 1e-4
+    s0, i0, _ = _setup(spec, make_params(contact_friction=0.0, **kw), v0)
+    for _ in range(3):
+        s0.SubstepPlain(i0)
+    moved = float(np.abs(s.GetState(i)[0] - s0.GetState(i0)[0]).max())
+    print(f"  friction moved the cloth by up to {moved:.3e} m", flush=True)
+    assert moved > 1e-6
+    b, j, _ = _setup(spec, xp, v0)
+    for _ in range(3):
+        b.SubstepBatched()
+    b.sync()
+    np.testing.assert_array_equal(b.GetState(j)[0], s.GetState(i)[0])

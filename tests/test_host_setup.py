@@ -58,7 +58,7 @@ def test_abi_exports_every_declared_symbol():
 
 def test_abi_version_and_defaults():
     lib = _ffi.load()
-    assert lib.xrt_abi_version() == 4
+    assert lib.xrt_abi_version() == 5
     p = _ffi.XrtParams()
     lib.xrt_default_params(C.byref(p))
     assert p.num_substeps == 10 and p.num_iterations == 10
@@ -75,7 +75,9 @@ def test_abi_version_and_defaults():
     assert (p.fem_xx_stiffness, p.fem_yy_stiffness, p.fem_xy_stiffness) == (1.0, 1.0, 1.0)
     assert p.fem_xy_poisson_ratio == pytest.approx(0.3) and p.fem_yx_poisson_ratio == pytest.approx(0.3)
     assert p.fem_bending_stiffness == 1.0
-    assert C.sizeof(p) == 4 * 37  # 37 four-byte fields
+    assert p.contact_friction == 0.0             # the reference applies none on mesh contacts
+    assert C.sizeof(p) == 4 * 38  # 38 four-byte fields
+    assert C.sizeof(_ffi.XrtTuning) == 4 * 7
 
 
 @pytest.mark.parametrize("n", [2, 5, 64])

@@ -146,7 +146,7 @@ def test_vf_reverse_matches_fd():
             out = np.ascontiguousarray(xx.reshape(-1)).copy()
             lib.orc_vf_solve(pred.ctypes.data_as(oadj._F), ot.ctypes.data_as(oadj._I),
                              face.ctypes.data_as(oadj._I), oadj._d(np.ascontiguousarray(q.reshape(-1))),
-                             oadj._d(out), nv, off)
+                             oadj._d(out), nv, off, None, None, None, 0.1, 0.0)  # no friction
             return out.reshape(-1, 3)
 
         def rev(xx, gb):

@@ -133,6 +133,20 @@ def _plane_obstacle(tol=0.003):
 
 
 _IP = C.POINTER(C.c_int32)
+_FPt = C.POINTER(C.c_float)
+
+
+def _vf_solve(lib, b, otris, face, q, xx, n, offset, x0=None, o_x0=None, o_x=None, inv_ns=0.1,
+              mu=0.0):
+    """orc_vf_solve on the tree's pred corners; friction off unless mu > 0."""
+    f32 = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)  # noqa: E731
+    o_x0, o_x = f32(o_x0), f32(o_x)
+    x0 = None if x0 is None else np.ascontiguousarray(x0, np.float64).reshape(-1)
+    lib.orc_vf_solve(lib.orc_bvh_pred(b.h), otris.ctypes.data_as(_IP), face.ctypes.data_as(_IP),
+                     q.ctypes.data_as(_D), xx.ctypes.data_as(_D), n, offset,
+                     None if x0 is None else x0.ctypes.data_as(_D),
+                     None if o_x0 is None else o_x0.ctypes.data_as(_FPt),
+                     None if o_x is None else o_x.ctypes.data_as(_FPt), inv_ns, mu)
 
 
 def test_vertex_in_the_contact_shell_is_projected_above_it():
@@ -147,8 +161,7 @@ def test_vertex_in_the_contact_shell_is_projected_above_it():
                             q.ctypes.data_as(_D))
     assert face[0] >= 0 and face[1] == -1
     xx = np.ascontiguousarray(x.reshape(-1))
-    lib.orc_vf_solve(lib.orc_bvh_pred(b.h), otris.ctypes.data_as(_IP), face.ctypes.data_as(_IP),
-                     q.ctypes.data_as(_D), xx.ctypes.data_as(_D), 2, 1e-3)
+    _vf_solve(lib, b, otris, face, q, xx, 2, 1e-3)
     assert xx[1] == pytest.approx(0.003 + 1e-3, abs=1e-9)
     assert xx[0] == pytest.approx(0.1) and xx[2] == pytest.approx(0.2)
     assert xx[4] == pytest.approx(-0.02)
@@ -168,9 +181,54 @@ def test_ccd_catches_a_vertex_that_tunnels_through():
     assert face[0] >= 0
     assert q[1] == pytest.approx(0.003, abs=1e-7)
     xx = np.ascontiguousarray(x.reshape(-1))
-    lib.orc_vf_solve(lib.orc_bvh_pred(b.h), otris.ctypes.data_as(_IP), face.ctypes.data_as(_IP),
-                     q.ctypes.data_as(_D), xx.ctypes.data_as(_D), 1, 0.0)
+    _vf_solve(lib, b, otris, face, q, xx, 1, 0.0)
     assert xx[1] == pytest.approx(0.003, abs=1e-7)
+
+
+@pytest.mark.parametrize("mu", [0.3, 10.0])
+def test_vf_contact_friction_closed_form(mu):
+    """Contact friction on a VF-DCD correction: a vertex that slid 1 cm along +x while
+    sinking into the shell of a static plane is pushed up by the correction c and pulled back
+    along -x by min(mu |c| / 1 cm, 1) of its slide (mu = 0.3: 0.75 mm; mu = 10: all of it,
+    static friction); the normal push is unchanged."""
+    lib = osc._lib()
+    b, otris = _plane_obstacle()
+    opos = np.array([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1]], np.float32)
+    x0 = np.array([[0.10, 0.0035, 0.2]], np.float64)
+    x = np.array([[0.11, 0.0015, 0.2]], np.float64)
+    face = np.zeros(1, np.int32)
+    q = np.zeros(3)
+    lib.orc_vf_dcd_generate(b.h, otris.ctypes.data_as(_IP), _d(x), 1, face.ctypes.data_as(_IP),
+                            q.ctypes.data_as(_D))
+    assert face[0] >= 0
+    xx = np.ascontiguousarray(x.reshape(-1))
+    _vf_solve(lib, b, otris, face, q, xx, 1, 1e-3, x0=x0, o_x0=opos, o_x=opos, mu=mu)
+    corr = 0.003 + 1e-3 - 0.0015                    # pushed to 1e-3 above the face at 0.003
+    slide_back = min(mu * corr / 0.01, 1.0) * 0.01
+    assert xx[1] == pytest.approx(0.004, abs=1e-9)
+    assert xx[0] == pytest.approx(0.11 - slide_back, abs=1e-9)
+    assert xx[2] == pytest.approx(0.2, abs=1e-12)
+
+
+def test_vf_contact_friction_follows_a_moving_obstacle():
+    """Friction acts on the displacement relative to the obstacle face: a vertex carried
+    along with an obstacle that moves 1 cm along +x per substep (10 cm over a frame of ten
+    substeps) keeps its slide; one that stays put is dragged mu |c| = 0.75 mm after it."""
+    lib = osc._lib()
+    b, otris = _plane_obstacle()
+    opos = np.array([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1]], np.float32)
+    moved = opos + np.array([0.1, 0.0, 0.0], np.float32)
+    x0 = np.array([[0.10, 0.0035, 0.2], [0.30, 0.0035, 0.2]], np.float64)
+    x = np.array([[0.11, 0.0015, 0.2], [0.30, 0.0015, 0.2]], np.float64)
+    face = np.zeros(2, np.int32)
+    q = np.zeros(6)
+    lib.orc_vf_dcd_generate(b.h, otris.ctypes.data_as(_IP), _d(x), 2, face.ctypes.data_as(_IP),
+                            q.ctypes.data_as(_D))
+    xx = np.ascontiguousarray(x.reshape(-1))
+    _vf_solve(lib, b, otris, face, q, xx, 2, 1e-3, x0=x0, o_x0=opos, o_x=moved, inv_ns=0.1, mu=0.3)
+    assert xx[0] == pytest.approx(0.11, abs=1e-7)   # moved with the obstacle: no friction
+    assert xx[3] == pytest.approx(0.30 + 0.3 * 0.0025, abs=1e-7)  # dragged along +x
+    assert xx[1] == pytest.approx(0.004, abs=1e-9) and xx[4] == pytest.approx(0.004, abs=1e-9)
 
 
 # ------------------------------------------------------------- PBD repulsion responses

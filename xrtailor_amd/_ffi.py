@@ -67,6 +67,19 @@ class XrtParams(C.Structure):
         ("fem_xy_poisson_ratio", C.c_float),
         ("fem_yx_poisson_ratio", C.c_float),
         ("fem_bending_stiffness", C.c_float),
+        ("contact_friction", C.c_float),
+    ]
+
+
+class XrtTuning(C.Structure):
+    _fields_ = [
+        ("persistent_block", C.c_int32),
+        ("persistent_unroll", C.c_int32),
+        ("wide_per_thread", C.c_int32),
+        ("path", C.c_int32),
+        ("best_ms", C.c_float),
+        ("default_ms", C.c_float),
+        ("candidates", C.c_int32),
     ]
 
 
@@ -173,6 +186,9 @@ SIGNATURES = {
     "xrt_bvh_query": [_P, _FP, _FP, _I, _IP, _IP],
     "xrt_bvh_traverse": [_P, _P, _F, _IP, C.c_int64, C.POINTER(C.c_int64)],
     "xrt_vertex_normals": [_P, _FP, _I, _IP, _I, _FP],
+    "xrt_autotune": [_P, _I, C.POINTER(XrtTuning)],
+    "xrt_get_tuning": [_P, C.POINTER(XrtTuning)],
+    "xrt_set_tuning": [_P, C.POINTER(XrtTuning)],
 }
 _RESTYPES = {"xrt_last_error": C.c_char_p, "xrt_default_params": None}
 

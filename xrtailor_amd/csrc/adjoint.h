@@ -40,13 +40,14 @@ struct WideArgs {
 // ------------------------------------------------------------------ forward
 // map (optional): this color's block map -- n_inst + 1 prefix sums of the instances' block
 // counts, then the instance of every block (`blocks` in total); without it the grid is
-// (blocks of the largest color, n_inst)
+// (blocks of the largest color, n_inst); per_thread 2 (untaped, with a map): two elements per
+// thread in half-size blocks (same results)
 hipError_t LaunchWideStretch(hipStream_t s, const WideArgs& a, int n_inst, int color, int max_cnt,
                              float inv_dt2, int iter, float4* tape, const int32_t* map = nullptr,
-                             int blocks = 0);
+                             int blocks = 0, int per_thread = 1);
 hipError_t LaunchWideBend(hipStream_t s, const WideArgs& a, int n_inst, int color, int max_cnt,
                           float inv_dt2, const BendMode& m, float4* tape,
-                          const int32_t* map = nullptr, int blocks = 0);
+                          const int32_t* map = nullptr, int blocks = 0, int per_thread = 1);
 hipError_t LaunchCopy4(hipStream_t s, float4* dst, const float4* src, int n);
 hipError_t LaunchZero4(hipStream_t s, float4* p, int n);
 hipError_t LaunchZero1(hipStream_t s, float* p, int n);

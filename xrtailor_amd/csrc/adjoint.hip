@@ -94,6 +94,37 @@ __global__ __launch_bounds__(kBs) void WideBendKernel(WideArgs a, int color, flo
   BendProject(X, id, a.bpar[k], inv_dt2, m);
 }
 
+// The same launches with two elements per thread (kBs / 2 threads per block, the block
+// maps unchanged): both elements' records and vertex gathers are issued before either is
+// solved.  Elements of one color share no vertex, so the order cannot change a bit.
+__global__ __launch_bounds__(kBs / 2) void WideStretch2Kernel(WideArgs a, int color,
+                                                              float inv_dt2, int iter,
+                                                              const int32_t* __restrict__ map,
+                                                              int n_inst) {
+  int inst, blk;
+  if (!WideBlock(map, n_inst, inst, blk)) return;
+  const InstDesc d = a.inst[inst];
+  if (color >= d.s_ncol) return;
+  const int beg = a.s_coloff[d.s_cbeg + color] + blk * kBs;
+  const int end = min(a.s_coloff[d.s_cbeg + color + 1], beg + kBs);
+  StretchColor<2>(a.x + d.vbeg, a.sc, a.slam, beg, end, inv_dt2, iter, static_cast<int>(threadIdx.x),
+                  kBs / 2);
+}
+
+__global__ __launch_bounds__(kBs / 2) void WideBend2Kernel(WideArgs a, int color, float inv_dt2,
+                                                           BendMode m,
+                                                           const int32_t* __restrict__ map,
+                                                           int n_inst) {
+  int inst, blk;
+  if (!WideBlock(map, n_inst, inst, blk)) return;
+  const InstDesc d = a.inst[inst];
+  if (color >= d.b_ncol) return;
+  const int beg = a.b_coloff[d.b_cbeg + color] + blk * kBs;
+  const int end = min(a.b_coloff[d.b_cbeg + color + 1], beg + kBs);
+  BendColor<2>(a.x + d.vbeg, a.bidx, a.bpar, beg, end, inv_dt2, m, static_cast<int>(threadIdx.x),
+               kBs / 2);
+}
+
 __global__ __launch_bounds__(kBs) void Copy4Kernel(float4* __restrict__ dst,
                                                    const float4* __restrict__ src, int n) {
   const int i = blockIdx.x * kBs + threadIdx.x;
@@ -311,9 +342,13 @@ __global__ __launch_bounds__(kBs) void WideBendAdjKernel(WideArgs a, int color, 
 // ------------------------------------------------------------------ launchers
 hipError_t LaunchWideStretch(hipStream_t s, const WideArgs& a, int n_inst, int color, int max_cnt,
                              float inv_dt2, int iter, float4* tape, const int32_t* map,
-                             int blocks) {
+                             int blocks, int per_thread) {
   if (max_cnt <= 0 || n_inst <= 0) return hipSuccess;
-  if (map) {
+  if (map && !tape && per_thread == 2) {
+    if (blocks > 0)
+      WideStretch2Kernel<<<(blocks + 7) / 8 * 8, kBs / 2, 0, s>>>(a, color, inv_dt2, iter, map,
+                                                                   n_inst);
+  } else if (map) {
     if (blocks > 0)
       WideStretchKernel<<<(blocks + 7) / 8 * 8, kBs, 0, s>>>(a, color, inv_dt2, iter, tape, map, n_inst);
   } else {
@@ -324,9 +359,12 @@ hipError_t LaunchWideStretch(hipStream_t s, const WideArgs& a, int n_inst, int c
 }
 hipError_t LaunchWideBend(hipStream_t s, const WideArgs& a, int n_inst, int color, int max_cnt,
                           float inv_dt2, const BendMode& m, float4* tape, const int32_t* map,
-                          int blocks) {
+                          int blocks, int per_thread) {
   if (max_cnt <= 0 || n_inst <= 0) return hipSuccess;
-  if (map) {
+  if (map && !tape && per_thread == 2) {
+    if (blocks > 0)
+      WideBend2Kernel<<<(blocks + 7) / 8 * 8, kBs / 2, 0, s>>>(a, color, inv_dt2, m, map, n_inst);
+  } else if (map) {
     if (blocks > 0)
       WideBendKernel<<<(blocks + 7) / 8 * 8, kBs, 0, s>>>(a, color, inv_dt2, m, tape, map, n_inst);
   } else {

@@ -51,6 +51,8 @@ int AdjSetup(xrt_ctx* c, int want_slots = 1) {
               "the adjoint does not support particle self-collision");
   XRT_REQUIRE(!BindingsOn(c), XRT_ERR_UNSUPPORTED, "the adjoint does not support bindings");
   XRT_REQUIRE(!FemOn(c), XRT_ERR_UNSUPPORTED, "the adjoint does not support the FEM constraints");
+  XRT_REQUIRE(!(c->params.enable_obstacle_contact && c->params.contact_friction > 0.0f),
+              XRT_ERR_UNSUPPORTED, "the adjoint does not support contact friction");
   const xrt_params& p = c->params;
   const int n_inst = static_cast<int>(c->inst.size());
   XRT_REQUIRE(n_inst > 0 && n_inst <= 65535, XRT_ERR_INVALID_ARG, "1..65535 instances");
@@ -228,13 +230,13 @@ int WideSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, AdjointData
       for (int col = 0; col < static_cast<int>(W.s_max.size()); ++col)
         XRT_CHECK_HIP(LaunchWideStretch(s, wa, n_inst, col, W.s_max[col], k.inv_dt2, it,
                                         TI ? TI + A->off_s : nullptr, W.maps.ptr + W.s_map[col],
-                                        W.s_blocks[col]));
+                                        W.s_blocks[col], c->tuning.wide_per_thread));
     }
     if (k.solve_bending && !FemOn(c))
       for (int col = 0; col < static_cast<int>(W.b_max.size()); ++col)
         XRT_CHECK_HIP(LaunchWideBend(s, wa, n_inst, col, W.b_max[col], k.inv_dt2, bm,
                                      TI ? TI + A->off_b : nullptr, W.maps.ptr + W.b_map[col],
-                                     W.b_blocks[col]));
+                                     W.b_blocks[col], c->tuning.wide_per_thread));
     if (c->total_slots > 0)  // never taped: the adjoint refuses long-range attachments
       XRT_CHECK_HIP(LaunchWideLra(s, c->desc.ptr, n_inst, c->max_slots, c->max_nv, c->x.ptr,
                                   c->slot_vert.ptr, c->slot_pos_all.ptr, c->lra_ptr.ptr,

@@ -193,8 +193,8 @@ int SubstepBatched(xrt_ctx* c) {
   if (ContactsActive(c) || ParticlesOn(c) || BindingsOn(c))
     return WideSubstep(c, k, ModeOf(k), nullptr, -1);
   XRT_CHECK_HIP(LaunchBatchedSubstep(c->stream, MakeBatchArgs(c), k,
-                                     static_cast<int>(c->inst.size()), c->batched_block,
-                                     c->batched_lds));
+                                     static_cast<int>(c->inst.size()), c->tuning.persistent_block,
+                                     c->batched_lds, c->tuning.persistent_unroll));
   return ParticleSubstepEnd(c, -1);
 }
 
@@ -212,6 +212,90 @@ int BuildGraph(xrt_ctx* c) {
   XRT_CHECK_HIP(e);
   c->graph = g;
   XRT_CHECK_HIP(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+  return XRT_OK;
+}
+
+// Everything a run of batched substeps changes outside the iterate, saved so that the
+// autotuner can time candidate launch shapes on the context's own workload and put the
+// context back where it was: the state (x0, v), each contact instance's frame position and
+// obstacle frame-start positions, the particle hash schedule.
+struct SavedRun {
+  DevBuf<float4> x0, v;
+  std::vector<std::unique_ptr<DevBuf<float4>>> o_x0;
+  std::vector<int> frame_substep, pass_cnt;
+  std::vector<char> moving;
+  std::vector<int> particle_substep;
+};
+
+int SaveRun(xrt_ctx* c, SavedRun& r) {
+  const size_t n = c->x0.n;
+  hipStream_t s = c->stream;
+  XRT_CHECK_HIP(r.x0.Alloc(n));
+  XRT_CHECK_HIP(r.v.Alloc(n));
+  XRT_CHECK_HIP(hipMemcpyAsync(r.x0.ptr, c->x0.ptr, sizeof(float4) * n, hipMemcpyDeviceToDevice, s));
+  XRT_CHECK_HIP(hipMemcpyAsync(r.v.ptr, c->v.ptr, sizeof(float4) * n, hipMemcpyDeviceToDevice, s));
+  for (auto& cd : c->contact) {
+    r.frame_substep.push_back(cd ? cd->frame_substep : 0);
+    r.pass_cnt.push_back(cd ? cd->pass_cnt : 0);
+    r.moving.push_back(cd ? cd->obstacle_moving : false);
+    auto b = std::make_unique<DevBuf<float4>>();
+    if (cd && cd->obs_nv > 0) {
+      XRT_CHECK_HIP(b->Alloc(cd->obs_nv));
+      XRT_CHECK_HIP(hipMemcpyAsync(b->ptr, cd->o_x0.ptr, sizeof(float4) * cd->obs_nv,
+                                   hipMemcpyDeviceToDevice, s));
+    }
+    r.o_x0.push_back(std::move(b));
+  }
+  r.particle_substep = c->particles.substep;
+  XRT_CHECK_HIP(hipStreamSynchronize(s));
+  return XRT_OK;
+}
+
+int RestoreRun(xrt_ctx* c, const SavedRun& r) {
+  const size_t n = c->x0.n;
+  hipStream_t s = c->stream;
+  XRT_CHECK_HIP(hipMemcpyAsync(c->x0.ptr, r.x0.ptr, sizeof(float4) * n, hipMemcpyDeviceToDevice, s));
+  XRT_CHECK_HIP(hipMemcpyAsync(c->x.ptr, r.x0.ptr, sizeof(float4) * n, hipMemcpyDeviceToDevice, s));
+  XRT_CHECK_HIP(hipMemcpyAsync(c->v.ptr, r.v.ptr, sizeof(float4) * n, hipMemcpyDeviceToDevice, s));
+  for (size_t k = 0; k < c->contact.size(); ++k) {
+    ContactData* cd = c->contact[k].get();
+    if (!cd) continue;
+    cd->frame_substep = r.frame_substep[k];
+    cd->pass_cnt = r.pass_cnt[k];
+    cd->obstacle_moving = r.moving[k] != 0;
+    if (cd->obs_nv > 0)
+      XRT_CHECK_HIP(hipMemcpyAsync(cd->o_x0.ptr, r.o_x0[k]->ptr, sizeof(float4) * cd->obs_nv,
+                                   hipMemcpyDeviceToDevice, s));
+  }
+  c->particles.substep = r.particle_substep;
+  XRT_CHECK_HIP(hipStreamSynchronize(s));
+  return XRT_OK;
+}
+
+// ms per batched substep with the context's current tuning, over n substeps (one untimed
+// substep first), the run saved in r restored before and after
+int TimeCandidate(xrt_ctx* c, const SavedRun& r, int n, float* ms) {
+  int rc = RestoreRun(c, r);
+  if (rc != XRT_OK) return rc;
+  if ((rc = SubstepBatched(c)) != XRT_OK) return rc;  // warm-up (code load, caches)
+  if ((rc = RestoreRun(c, r)) != XRT_OK) return rc;
+  XRT_CHECK_HIP(hipEventRecord(c->ev0, c->stream));
+  for (int i = 0; i < n; ++i)
+    if ((rc = SubstepBatched(c)) != XRT_OK) return rc;
+  XRT_CHECK_HIP(hipEventRecord(c->ev1, c->stream));
+  XRT_CHECK_HIP(hipEventSynchronize(c->ev1));
+  XRT_CHECK_HIP(hipEventElapsedTime(ms, c->ev0, c->ev1));
+  *ms /= static_cast<float>(n);
+  return RestoreRun(c, r);
+}
+
+int CheckTuning(const xrt_tuning& t) {
+  XRT_REQUIRE(t.persistent_block >= 64 && t.persistent_block <= 1024 && t.persistent_block % 64 == 0,
+              XRT_ERR_INVALID_ARG, "persistent_block must be a multiple of 64 in [64, 1024]");
+  XRT_REQUIRE(t.persistent_unroll >= 0 && t.persistent_unroll <= 2, XRT_ERR_INVALID_ARG,
+              "persistent_unroll must be 0, 1 or 2");
+  XRT_REQUIRE(t.wide_per_thread == 1 || t.wide_per_thread == 2, XRT_ERR_INVALID_ARG,
+              "wide_per_thread must be 1 or 2");
   return XRT_OK;
 }
 
@@ -269,6 +353,7 @@ void xrt_default_params(xrt_params* p) {
   p->fem_xy_poisson_ratio = 0.3f;
   p->fem_yx_poisson_ratio = 0.3f;
   p->fem_bending_stiffness = 1.0f;
+  p->contact_friction = 0.0f;
 }
 
 int xrt_create(xrt_ctx** out, int device) {
@@ -347,6 +432,8 @@ int xrt_set_params(xrt_ctx* ctx, const xrt_params* p) {
                 "fem_xy_poisson_ratio * fem_yx_poisson_ratio must differ from 1");
   }
   XRT_REQUIRE(p->pair_capacity_per_face >= 1, XRT_ERR_INVALID_ARG, "pair_capacity_per_face must be >= 1");
+  XRT_REQUIRE(std::isfinite(p->contact_friction) && p->contact_friction >= 0.0f, XRT_ERR_INVALID_ARG,
+              "contact_friction must be finite and >= 0");
   const bool lists = p->pair_capacity_per_face != ctx->params.pair_capacity_per_face ||
                      p->repulsion_vf_items_per_face != ctx->params.repulsion_vf_items_per_face ||
                      p->repulsion_ee_items_per_face != ctx->params.repulsion_ee_items_per_face;
@@ -892,6 +979,78 @@ int xrt_frames_batched(xrt_ctx* ctx, int32_t nframes) {
     if (rc != XRT_OK) return rc;
   }
   for (int f = 0; f < nframes; ++f) XRT_CHECK_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+  return XRT_OK;
+}
+
+int xrt_get_tuning(xrt_ctx* ctx, xrt_tuning* out) {
+  XRT_REQUIRE(ctx && out, XRT_ERR_INVALID_ARG, "null argument");
+  *out = ctx->tuning;
+  return XRT_OK;
+}
+
+int xrt_set_tuning(xrt_ctx* ctx, const xrt_tuning* t) {
+  XRT_REQUIRE(ctx && t, XRT_ERR_INVALID_ARG, "null argument");
+  const int rc = CheckTuning(*t);
+  if (rc != XRT_OK) return rc;
+  ctx->tuning.persistent_block = t->persistent_block;
+  ctx->tuning.persistent_unroll = t->persistent_unroll;
+  ctx->tuning.wide_per_thread = t->wide_per_thread;
+  DestroyGraph(ctx);  // the next batched frame captures the new shapes
+  return XRT_OK;
+}
+
+int xrt_autotune(xrt_ctx* ctx, int32_t substeps, xrt_tuning* out) {
+  XRT_REQUIRE(ctx && ctx->started, XRT_ERR_STATE, "xrt_start has not been called");
+  XRT_REQUIRE(substeps > 0, XRT_ERR_INVALID_ARG, "substeps must be > 0");
+  XRT_REQUIRE(!FemOn(ctx) || ctx->fem.ready, XRT_ERR_STATE, "FEM constraints are not set up");
+  int rc = CheckPairOverflow(ctx);  // also waits for queued work
+  if (rc != XRT_OK) return rc;
+  SavedRun saved;
+  if ((rc = SaveRun(ctx, saved)) != XRT_OK) return rc;
+  const bool wide = ContactsActive(ctx) || ParticlesOn(ctx) || BindingsOn(ctx);
+  std::vector<xrt_tuning> cand;
+  const xrt_tuning base{1024, 0, 1, wide ? 1 : 0, 0.0f, 0.0f, 0};
+  if (wide) {
+    for (int pt : {1, 2}) {
+      xrt_tuning t = base;
+      t.wide_per_thread = pt;
+      cand.push_back(t);
+    }
+  } else {
+    for (int block : {1024, 768, 512, 256})
+      for (int unroll : {0, 1, 2}) {
+        xrt_tuning t = base;
+        t.persistent_block = block;
+        t.persistent_unroll = unroll;
+        cand.push_back(t);
+      }
+  }
+  const xrt_tuning before = ctx->tuning;
+  xrt_tuning best = base;
+  float best_ms = 0.0f, default_ms = 0.0f;
+  for (size_t j = 0; j < cand.size(); ++j) {
+    ctx->tuning = cand[j];
+    float ms = 0.0f;
+    rc = TimeCandidate(ctx, saved, substeps, &ms);
+    if (rc != XRT_OK) break;
+    if (j == 0) default_ms = ms;
+    if (j == 0 || ms < best_ms) {
+      best_ms = ms;
+      best = cand[j];
+    }
+  }
+  if (rc != XRT_OK) {
+    ctx->tuning = before;
+    (void)RestoreRun(ctx, saved);
+    return rc;
+  }
+  best.best_ms = best_ms;
+  best.default_ms = default_ms;
+  best.candidates = static_cast<int32_t>(cand.size());
+  ctx->tuning = best;
+  DestroyGraph(ctx);
+  if ((rc = CheckPairOverflow(ctx)) != XRT_OK) return rc;
+  if (out) *out = best;
   return XRT_OK;
 }
 

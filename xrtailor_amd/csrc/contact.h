@@ -49,8 +49,18 @@ hipError_t LaunchVfCcdGenerate(hipStream_t s, const float4* x0, const float4* x,
                                const Lbvh& obs, int32_t* face, float4* q);
 hipError_t LaunchEeDcdGenerate(hipStream_t s, const float4* x, const int2* edges, int ne,
                                const Lbvh& obs, const uint32_t* obs_rtri, EeRec* rec);
+// friction of the VF solves (mu = 0: none, the reference's behaviour): x0 the cloth's
+// start-of-substep positions, o_x0 / o_x the obstacle's frame-start / frame-end positions,
+// inv_ns = 1 / substeps (the obstacle moves (o_x - o_x0) inv_ns per substep)
+struct VfFriction {
+  const float4* x0;
+  const float4* o_x0;
+  const float4* o_x;
+  float inv_ns;
+  float mu;
+};
 hipError_t LaunchVfSolve(hipStream_t s, float4* x, int n, const Lbvh& obs, const int32_t* face,
-                         const float4* q, float offset);
+                         const float4* q, float offset, const VfFriction& fr);
 hipError_t LaunchEeSolve(hipStream_t s, const float4* x, const int2* edges, int ne,
                          const EeRec* rec, long long* dq, int32_t* cnt);
 hipError_t LaunchApplyFixedDeltas(hipStream_t s, float4* x, int n, long long* dq, int32_t* cnt,
@@ -146,7 +156,8 @@ hipError_t MultiRepulsion(hipStream_t s, const InstContactDev* ic, const MultiMa
                           const float4* x0, float h, float relax, hipEvent_t ee_begin = nullptr,
                           hipEvent_t ee_end = nullptr);
 // VF-CCD (offset 0) then VF-DCD (offset 1e-3) solves
-hipError_t MultiVfSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x);
+hipError_t MultiVfSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
+                        const float4* x0, float mu, float inv_ns);
 // EE-DCD deltas applied x relax
 hipError_t MultiEeSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
                         float relax);

@@ -216,9 +216,16 @@ __device__ __forceinline__ void EeDcdGenerateBody(const float4* __restrict__ x,
 }
 
 // SolveVFDCD_Kernel / SolveVFCCD_Kernel (PBDCollision.cu:54-76,280-300)
+// VF solve: push along the pred face normal to the contact point (+offset), then, with
+// fr.mu > 0, ComputeFriction (ClothSolverHelper.cu:156-169) on that correction: the part of
+// the vertex's substep displacement (x - x0) relative to the obstacle face's (its vertices'
+// mean frame displacement / substeps) that is tangential to the normal is removed, at most
+// mu x |correction|.  The reference applies no friction on mesh contacts; mu = 0 is its
+// behaviour.
 __device__ __forceinline__ void VfSolveBody(float4* __restrict__ x, int i, const BvhView& t,
                                             const int32_t* __restrict__ face,
-                                            const float4* __restrict__ q, float offset) {
+                                            const float4* __restrict__ q, float offset,
+                                            const VfFriction& fr) {
   const int f = face[i];
   if (f < 0) return;
   const int4 tr = t.tris[f];
@@ -226,9 +233,28 @@ __device__ __forceinline__ void VfSolveBody(float4* __restrict__ x, int i, const
   float4 xi = x[i];
   const float lam = D(nrm, S(L3(xi), L3(q[i]))) - offset;
   if (lam < 0) {
-    xi.x += -lam * nrm.x;
-    xi.y += -lam * nrm.y;
-    xi.z += -lam * nrm.z;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    if (fr.mu > 0.0f) {
+      const float4 a0 = fr.o_x0[tr.x], b0 = fr.o_x0[tr.y], c0 = fr.o_x0[tr.z];
+      const float4 a1 = fr.o_x[tr.x], b1 = fr.o_x[tr.y], c1 = fr.o_x[tr.z];
+      const float ox = ((a1.x - a0.x) + (b1.x - b0.x) + (c1.x - c0.x)) / 3.0f * fr.inv_ns;
+      const float oy = ((a1.y - a0.y) + (b1.y - b0.y) + (c1.y - c0.y)) / 3.0f * fr.inv_ns;
+      const float oz = ((a1.z - a0.z) + (b1.z - b0.z) + (c1.z - c0.z)) / 3.0f * fr.inv_ns;
+      const float4 p0 = fr.x0[i];
+      const float rx = (xi.x - p0.x) - ox, ry = (xi.y - p0.y) - oy, rz = (xi.z - p0.z) - oz;
+      const float rn = rx * nrm.x + ry * nrm.y + rz * nrm.z;
+      const float tx = rx - nrm.x * rn, ty = ry - nrm.y * rn, tz = rz - nrm.z * rn;
+      const float tl = sqrtf(tx * tx + ty * ty + tz * tz);
+      if (tl > 0.0f) {
+        const float s = fminf(-lam * fr.mu / tl, 1.0f);
+        fx = -tx * s;
+        fy = -ty * s;
+        fz = -tz * s;
+      }
+    }
+    xi.x += -lam * nrm.x + fx;
+    xi.y += -lam * nrm.y + fy;
+    xi.z += -lam * nrm.z + fz;
     x[i] = xi;
   }
 }
@@ -593,9 +619,10 @@ __global__ __launch_bounds__(kBlock) void EeDcdGenerateKernel(const float4* __re
 
 __global__ __launch_bounds__(kBlock) void VfSolveKernel(float4* __restrict__ x, int n, BvhView t,
                                                         const int32_t* __restrict__ face,
-                                                        const float4* __restrict__ q, float offset) {
+                                                        const float4* __restrict__ q, float offset,
+                                                        VfFriction fr) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) VfSolveBody(x, i, t, face, q, offset);
+  if (i < n) VfSolveBody(x, i, t, face, q, offset, fr);
 }
 
 __global__ __launch_bounds__(kBlock) void EeSolveKernel(const float4* __restrict__ x,
@@ -1087,11 +1114,14 @@ __global__ __launch_bounds__(kBlock) void MultiApplyKernel(const InstContactDev*
 
 __global__ __launch_bounds__(kBlock) void MultiVfSolveKernel(const InstContactDev* ic,
                                                              const int32_t* off, int n,
-                                                             float4* __restrict__ x) {
+                                                             float4* __restrict__ x,
+                                                             const float4* __restrict__ x0,
+                                                             float mu, float inv_ns) {
   XRT_MULTI_INDEX(kVert, c.nv)
   if (!c.has_obs) return;
-  VfSolveBody(x + c.vbeg, i, c.obvh, c.rt_face, c.rt_q, 0.0f);
-  VfSolveBody(x + c.vbeg, i, c.obvh, c.vf_face, c.vf_q, 1e-3f);
+  const VfFriction fr{x0 + c.vbeg, c.o_x0, c.o_x, inv_ns, mu};
+  VfSolveBody(x + c.vbeg, i, c.obvh, c.rt_face, c.rt_q, 0.0f, fr);
+  VfSolveBody(x + c.vbeg, i, c.obvh, c.vf_face, c.vf_q, 1e-3f, fr);
 }
 
 __global__ __launch_bounds__(kBlock) void MultiEeSolveKernel(const InstContactDev* ic,
@@ -1135,9 +1165,9 @@ hipError_t LaunchEeDcdGenerate(hipStream_t s, const float4* x, const int2* edges
   return hipGetLastError();
 }
 hipError_t LaunchVfSolve(hipStream_t s, float4* x, int n, const Lbvh& obs, const int32_t* face,
-                         const float4* q, float offset) {
+                         const float4* q, float offset, const VfFriction& fr) {
   if (n <= 0 || !obs.active) return hipSuccess;
-  VfSolveKernel<<<Blocks(n), kBlock, 0, s>>>(x, n, obs.view(), face, q, offset);
+  VfSolveKernel<<<Blocks(n), kBlock, 0, s>>>(x, n, obs.view(), face, q, offset, fr);
   return hipGetLastError();
 }
 hipError_t LaunchEeSolve(hipStream_t s, const float4* x, const int2* edges, int ne,
@@ -1243,9 +1273,11 @@ hipError_t MultiRepulsion(hipStream_t s, const InstContactDev* ic, const MultiMa
   return hipGetLastError();
 }
 
-hipError_t MultiVfSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x) {
+hipError_t MultiVfSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
+                        const float4* x0, float mu, float inv_ns) {
   if (Empty(m, MK::kVert)) return hipSuccess;
-  MultiVfSolveKernel<<<XcdGrid(m.blocks[MK::kVert]), kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x);
+  MultiVfSolveKernel<<<XcdGrid(m.blocks[MK::kVert]), kBlock, 0, s>>>(ic, m.off[MK::kVert], m.n, x,
+                                                                     x0, mu, inv_ns);
   return hipGetLastError();
 }
 

@@ -88,6 +88,13 @@ MeshTopo ObsTopo(const ContactData& c) {
   return t;
 }
 
+// friction of instance inst's VF solves (contact_friction; 0: none)
+VfFriction Friction(const xrt_ctx* ctx, int inst) {
+  const ContactData& c = *ctx->contact[inst];
+  return VfFriction{ctx->x0.ptr + ctx->inst[inst].desc.vbeg, c.o_x0.ptr, c.o_x.ptr,
+                    1.0f / static_cast<float>(ctx->params.num_substeps), ctx->params.contact_friction};
+}
+
 std::vector<float4> ToF4(const std::vector<float>& p) {
   std::vector<float4> out(std::max<size_t>(p.size() / 3, 1));
   for (size_t i = 0; i < p.size() / 3; ++i) out[i] = make_float4(p[3 * i], p[3 * i + 1], p[3 * i + 2], 0.0f);
@@ -321,8 +328,9 @@ int ContactVfSolve(xrt_ctx* ctx, int inst) {
   if (!(c.obs_nf > 0 && ctx->params.enable_obstacle_contact)) return XRT_OK;
   const InstDesc& d = ctx->inst[inst].desc;
   float4* x = ctx->x.ptr + d.vbeg;
-  XRT_CHECK_HIP(LaunchVfSolve(ctx->stream, x, d.nv, c.obs_bvh, c.rt_face.ptr, c.rt_q.ptr, 0.0f));
-  XRT_CHECK_HIP(LaunchVfSolve(ctx->stream, x, d.nv, c.obs_bvh, c.vf_face.ptr, c.vf_q.ptr, 1e-3f));
+  const VfFriction fr = Friction(ctx, inst);
+  XRT_CHECK_HIP(LaunchVfSolve(ctx->stream, x, d.nv, c.obs_bvh, c.rt_face.ptr, c.rt_q.ptr, 0.0f, fr));
+  XRT_CHECK_HIP(LaunchVfSolve(ctx->stream, x, d.nv, c.obs_bvh, c.vf_face.ptr, c.vf_q.ptr, 1e-3f, fr));
   return XRT_OK;
 }
 
@@ -548,7 +556,9 @@ int MultiContactIterPre(xrt_ctx* ctx, int it, bool* pass_ran) {
 
 int MultiContactVfSolve(xrt_ctx* ctx) {
   if (!ctx->params.enable_obstacle_contact) return XRT_OK;
-  XRT_CHECK_HIP(MultiVfSolve(ctx->stream, ctx->ic.ptr, ctx->maps, ctx->x.ptr));
+  XRT_CHECK_HIP(MultiVfSolve(ctx->stream, ctx->ic.ptr, ctx->maps, ctx->x.ptr, ctx->x0.ptr,
+                             ctx->params.contact_friction,
+                             1.0f / static_cast<float>(ctx->params.num_substeps)));
   return XRT_OK;
 }
 
@@ -841,7 +851,7 @@ int xrt_vf_dcd_solve(xrt_ctx* ctx, int32_t inst) {
   ContactData& c = *ctx->contact[inst];
   const InstDesc& d = ctx->inst[inst].desc;
   XRT_CHECK_HIP(LaunchVfSolve(ctx->stream, ctx->x.ptr + d.vbeg, d.nv, c.obs_bvh, c.vf_face.ptr,
-                              c.vf_q.ptr, 1e-3f));
+                              c.vf_q.ptr, 1e-3f, Friction(ctx, inst)));
   return XRT_OK;
 }
 
@@ -851,7 +861,7 @@ int xrt_vf_ccd_solve(xrt_ctx* ctx, int32_t inst) {
   ContactData& c = *ctx->contact[inst];
   const InstDesc& d = ctx->inst[inst].desc;
   XRT_CHECK_HIP(LaunchVfSolve(ctx->stream, ctx->x.ptr + d.vbeg, d.nv, c.obs_bvh, c.rt_face.ptr,
-                              c.rt_q.ptr, 0.0f));
+                              c.rt_q.ptr, 0.0f, Friction(ctx, inst)));
   return XRT_OK;
 }
 

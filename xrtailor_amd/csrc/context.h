@@ -253,7 +253,8 @@ struct xrt_ctx {
   xrt::DevBuf<xrt::InstDesc> desc;
   xrt::DevBuf<int32_t> order;
   // batched launch configuration
-  int batched_block = 1024;
+  // launch shapes picked by xrt_autotune (defaults until it runs); none changes a result
+  xrt_tuning tuning{1024, 0, 1, 0, 0.0f, 0.0f, 0};
   size_t batched_lds = 0;
   int lds_verts = 0;
   int max_slots = 0;

@@ -22,8 +22,9 @@ hipError_t LaunchGround(hipStream_t s, float4* x, const float4* x0, int n, float
                         float mu);
 hipError_t LaunchFinalize(hipStream_t s, float4* x0, float4* v, const float4* x, int n,
                           const StepConsts& k);
+// unroll: 0 default, 1 light, 2 heavy (constraints per thread per round; same results)
 hipError_t LaunchBatchedSubstep(hipStream_t s, const BatchArgs& a, const StepConsts& k,
-                                int n_instances, int block, size_t lds_bytes);
+                                int n_instances, int block, size_t lds_bytes, int unroll);
 hipError_t ConfigureBatchedKernel(size_t lds_bytes);
 
 }  // namespace xrt

@@ -96,7 +96,18 @@ __global__ __launch_bounds__(256) void FinalizeKernel(float4* __restrict__ x0,
 }
 
 // ------------------------------------------------------------ batched kernel
-template <bool kLds>
+// Constraints per thread per round of the persistent kernel (kLds: iterate in LDS; global:
+// more gathers in flight), by unroll variant: 0 (default), 1 (light), 2 (heavy).  The variant
+// changes only how many independent constraints of one color a thread loads before solving
+// them, never the result.
+template <bool kLds, int kVariant>
+struct Unroll {
+  static constexpr int s = kVariant == 1 ? (kLds ? 1 : 2) : kVariant == 2 ? (kLds ? 4 : 8) : (kLds ? 2 : 4);
+  static constexpr int b = kVariant == 1 ? 1 : kVariant == 2 ? (kLds ? 2 : 4) : (kLds ? 1 : 2);
+  static constexpr int f = kVariant == 1 ? 2 : kVariant == 2 ? 8 : 4;
+};
+
+template <bool kLds, int kVariant>
 __device__ __forceinline__ void SubstepBody(const BatchArgs& a, const InstDesc& d, int inst,
                                             const StepConsts& k, float4* lds_x,
                                             float4* lds_slots) {
@@ -119,7 +130,8 @@ __device__ __forceinline__ void SubstepBody(const BatchArgs& a, const InstDesc& 
   const int32_t* bco = a.b_coloff + d.b_cbeg;
   const int32_t* lptr = a.lra_ptr + d.lra_vbeg;
   // constraints per thread per round: more gathers in flight where x lives in global memory
-  constexpr int kUs = kLds ? 2 : 4, kUb = kLds ? 1 : 2, kUf = 4;
+  constexpr int kUs = Unroll<kLds, kVariant>::s, kUb = Unroll<kLds, kVariant>::b,
+                kUf = Unroll<kLds, kVariant>::f;
   const int2 fc = a.fem ? a.fcol[inst] : make_int2(0, 0);
   for (int it = 0; it < k.iterations; ++it) {
     if (a.fem) {  // quality mode: FEM strain colors, then isometric bending colors
@@ -175,15 +187,16 @@ __device__ __forceinline__ void SubstepBody(const BatchArgs& a, const InstDesc& 
   }
 }
 
+template <int kVariant>
 __global__ __launch_bounds__(1024) void BatchedSubstepKernel(BatchArgs a, StepConsts k) {
   extern __shared__ float4 lds[];
   const int inst = a.order[blockIdx.x];
   const InstDesc d = a.inst[inst];
   float4* lds_slots = lds + a.lds_verts;
   if (d.nv <= a.lds_verts)
-    SubstepBody<true>(a, d, inst, k, lds, lds_slots);
+    SubstepBody<true, kVariant>(a, d, inst, k, lds, lds_slots);
   else
-    SubstepBody<false>(a, d, inst, k, lds, lds_slots);
+    SubstepBody<false, kVariant>(a, d, inst, k, lds, lds_slots);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -266,15 +279,28 @@ hipError_t LaunchFinalize(hipStream_t s, float4* x0, float4* v, const float4* x,
   return hipGetLastError();
 }
 hipError_t LaunchBatchedSubstep(hipStream_t s, const BatchArgs& a, const StepConsts& k,
-                                int n_instances, int block, size_t lds_bytes) {
+                                int n_instances, int block, size_t lds_bytes, int unroll) {
   if (n_instances <= 0) return hipSuccess;
-  BatchedSubstepKernel<<<n_instances, block, lds_bytes, s>>>(a, k);
+  if (block < 64 || block > 1024 || block % 64 != 0 || unroll < 0 || unroll > 2)
+    return hipErrorInvalidValue;
+  if (unroll == 1)
+    BatchedSubstepKernel<1><<<n_instances, block, lds_bytes, s>>>(a, k);
+  else if (unroll == 2)
+    BatchedSubstepKernel<2><<<n_instances, block, lds_bytes, s>>>(a, k);
+  else
+    BatchedSubstepKernel<0><<<n_instances, block, lds_bytes, s>>>(a, k);
   return hipGetLastError();
 }
 hipError_t ConfigureBatchedKernel(size_t lds_bytes) {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&BatchedSubstepKernel),
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             static_cast<int>(lds_bytes));
+  const void* fns[3] = {reinterpret_cast<const void*>(&BatchedSubstepKernel<0>),
+                        reinterpret_cast<const void*>(&BatchedSubstepKernel<1>),
+                        reinterpret_cast<const void*>(&BatchedSubstepKernel<2>)};
+  for (const void* f : fns) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             static_cast<int>(lds_bytes));
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace xrt

@@ -135,6 +135,52 @@ def capsule_y(radius: float, half_length: float, subdiv: int = 3, center=(0.0, 0
     return pos + np.asarray(center, float), tris
 
 
+def capsule_latlong_y(radius: float, half_length: float, ring: int = 56, rows: int = 123,
+                      center=(0.0, 0.0, 0.0)):
+    """Capsule along y as a latitude-longitude mesh: a pole vertex at each end and `rows`
+    rings of `ring` vertices spaced evenly in arc length along the profile (quarter circle,
+    straight side, quarter circle).  ring * rows + 2 vertices, 2 ring * rows triangles
+    (defaults: 6,890 vertices and 13,776 triangles, the vertex and face counts of an SMPL
+    body mesh), wound counter-clockwise seen from outside.  Vertex 0 is the bottom pole,
+    ring j (from the bottom) holds 1 + j ring + i, the last vertex is the top pole."""
+    arc = 0.5 * np.pi * radius
+    total = 2.0 * arc + 2.0 * half_length
+    s = total * np.arange(1, rows + 1) / (rows + 1)
+    rho = np.empty(rows)
+    y = np.empty(rows)
+    lo, hi = s < arc, s > arc + 2.0 * half_length
+    mid = ~(lo | hi)
+    rho[lo] = radius * np.sin(s[lo] / radius)
+    y[lo] = -half_length - radius * np.cos(s[lo] / radius)
+    rho[mid] = radius
+    y[mid] = -half_length + (s[mid] - arc)
+    rho[hi] = radius * np.sin((total - s[hi]) / radius)
+    y[hi] = half_length + radius * np.cos((total - s[hi]) / radius)
+    th = 2.0 * np.pi * np.arange(ring) / ring
+    pos = np.empty((ring * rows + 2, 3))
+    pos[0] = (0.0, -half_length - radius, 0.0)
+    pos[-1] = (0.0, half_length + radius, 0.0)
+    for j in range(rows):
+        pos[1 + j * ring:1 + (j + 1) * ring] = np.stack(
+            [rho[j] * np.cos(th), np.full(ring, y[j]), rho[j] * np.sin(th)], 1)
+    top = ring * rows + 1
+    tris = []
+    for i in range(ring):
+        i1 = (i + 1) % ring
+        tris.append((0, 1 + i, 1 + i1))
+    for j in range(rows - 1):
+        for i in range(ring):
+            i1 = (i + 1) % ring
+            a, b = 1 + j * ring + i, 1 + j * ring + i1
+            c, d = 1 + (j + 1) * ring + i1, 1 + (j + 1) * ring + i
+            tris += [(a, c, b), (a, d, c)]
+    for i in range(ring):
+        i1 = (i + 1) % ring
+        base = 1 + (rows - 1) * ring
+        tris.append((top, base + i1, base + i))
+    return pos + np.asarray(center, float), np.array(tris, np.int32)
+
+
 @dataclass
 class ChainBody:
     """A capsule-chain body: one capsule along y over `joints` segments of length `seg`
@@ -242,10 +288,16 @@ class ChainBody:
 
 
 def chain_body(seed: int, joints: int, radius: float, seg: float, y0: float,
-               subdiv: int = 3) -> ChainBody:
+               subdiv: int = 3, mesh: str = "ico") -> ChainBody:
+    """mesh "ico": icosphere-based capsule of subdivision `subdiv`; "latlong": the 6,890-
+    vertex latitude-longitude capsule (capsule_latlong_y)."""
     rng = Rng(seed)
     half = 0.5 * joints * seg - radius
-    pos, tris = capsule_y(radius, max(half, 0.0), subdiv, (0.0, y0 + 0.5 * joints * seg, 0.0))
+    ctr = (0.0, y0 + 0.5 * joints * seg, 0.0)
+    if mesh == "latlong":
+        pos, tris = capsule_latlong_y(radius, max(half, 0.0), center=ctr)
+    else:
+        pos, tris = capsule_y(radius, max(half, 0.0), subdiv, ctr)
     amp = rng.uniform(joints * 2 * 3).reshape(joints, 2, 3)
     amp = 0.6 * amp / amp.sum(-1, keepdims=True).sum(-2, keepdims=True)  # sum |a| = 0.6 / joint
     freq = 0.2 + 1.8 * rng.uniform(joints * 2 * 3).reshape(joints, 2, 3)
@@ -295,7 +347,8 @@ def differentiable_drop(seed: int = 5, n: int = 128, side: float = 1.0, height: 
 
 def garment(seed: int = 6, ring: int = 160, rows: int = 125, joints: int = 24,
             radius: float = 0.2, height: float = 0.6, gap: float = 0.01,
-            jitter: float = 1e-3, body_subdiv: int = 5) -> tuple[ClothSpec, ChainBody]:
+            jitter: float = 1e-3, body_subdiv: int = 5,
+            body_mesh: str = "ico") -> tuple[ClothSpec, ChainBody]:
     """Tube garment (ring x rows vertices, radius `radius`, top ring at y = height as LRA
     slots) around a capsule-chain body whose surface lies `gap` inside it; stretch 1e-8,
     bend 1e-4.  The body moves by ChainBody.at(t)."""
@@ -303,11 +356,20 @@ def garment(seed: int = 6, ring: int = 160, rows: int = 125, joints: int = 24,
     rng = Rng(seed)
     x0 = (pos + jitter * rng.normal(pos.size).reshape(-1, 3)).astype(np.float32)
     seg = (height + 0.2) / joints
-    body = chain_body(seed + 1, joints, radius - gap, seg, -0.1, body_subdiv)
+    body = chain_body(seed + 1, joints, radius - gap, seg, -0.1, body_subdiv, body_mesh)
     slots = np.arange((rows - 1) * ring, rows * ring, dtype=np.int32)
     spec = ClothSpec(pos, tris, np.zeros(0, np.int32), 1e-8, 1e-4, x0, (body.pos, body.tris),
                      slots)
     return spec, body
+
+
+def garment_on_body(seed: int = 6, gap: float = 0.01) -> tuple[ClothSpec, ChainBody]:
+    """The garment-on-animated-body workload at its stated shape: a 160 x 125 tube garment
+    (20,000 vertices, radius 0.2 m, 0.6 m tall) whose top (shoulder) ring holds the 160
+    long-range attachment slots (every slot to every vertex), around a capsule-chain body
+    of 6,890 vertices (latitude-longitude capsule) skinned by 24 joints, `gap` inside the
+    tube; joint angles are seeded sums of three sinusoids (0.2-2 Hz, sum |a| <= 0.6 rad)."""
+    return garment(seed=seed, ring=160, rows=125, joints=24, gap=gap, body_mesh="latlong")
 
 
 def garments(seed: int = 4, n_instances: int = 64, ring_lo: int = 64, ring_hi: int = 128,

@@ -542,6 +542,24 @@ class ClothSolver:
         check(self.lib.xrt_time_frames(self.ctx, int(batched), n, C.byref(ms)))
         return ms.value
 
+    # ---------------------------------------------------------------- tuning
+    def autotune(self, substeps: int = 3) -> dict:
+        """Times the candidate launch shapes of the batched substep on this context's
+        instances and keeps the fastest (state restored); returns the choice."""
+        t = _ffi.XrtTuning()
+        check(self.lib.xrt_autotune(self.ctx, substeps, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def tuning(self) -> dict:
+        t = _ffi.XrtTuning()
+        check(self.lib.xrt_get_tuning(self.ctx, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def set_tuning(self, persistent_block: int = 1024, persistent_unroll: int = 0,
+                   wide_per_thread: int = 1) -> None:
+        t = _ffi.XrtTuning(persistent_block, persistent_unroll, wide_per_thread, 0, 0.0, 0.0, 0)
+        check(self.lib.xrt_set_tuning(self.ctx, C.byref(t)))
+
     def time_batched_substep(self, n: int) -> float:
         ms = C.c_float()
         check(self.lib.xrt_time_batched_substep(self.ctx, n, C.byref(ms)))
