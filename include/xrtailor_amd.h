@@ -501,6 +501,15 @@ int xrt_bvh_query(xrt_bvh* bvh, const float* p0, const float* p1, int32_t n, int
  * *n_out receives the count; XRT_ERR_CAPACITY if it exceeds cap (first cap written) */
 int xrt_bvh_traverse(xrt_bvh* lhs, xrt_bvh* rhs, float thickness, int32_t* pairs, int64_t cap,
                      int64_t* n_out);
+/* No counterpart in the reference: xrt_bvh_traverse for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc).  Same arguments, results
+ * and errors; the device staging of the returned pairs is a fixed array of 2,097,152 pairs
+ * (16 MiB per device, held until the process ends) instead of a per-call allocation, and no
+ * hipMalloc / hipFree is called.  XRT_ERR_CAPACITY (with *n_out set) when min(count, cap)
+ * exceeds 2,097,152.  Every call on one device shares the array: calls that overlap on a
+ * device (other threads, other handles) race. */
+int xrt_bvh_traverse_noalloc(xrt_bvh* lhs, xrt_bvh* rhs, float thickness, int32_t* pairs,
+                             int64_t cap, int64_t* n_out);
 /* PhysicsMesh::UpdateNormals vertex normals (PhysicsMeshHelper.cu:196-210,282-295), summed
  * per vertex in ascending face order (deterministic) */
 int xrt_vertex_normals(xrt_bvh* bvh, const float* x, int32_t nv, const int32_t* tris, int32_t nt,

@@ -185,6 +185,7 @@ SIGNATURES = {
     "xrt_bvh_get": [_P, C.POINTER(C.c_uint32), _IP, _IP, _FP, _IP, _IP, _FP],
     "xrt_bvh_query": [_P, _FP, _FP, _I, _IP, _IP],
     "xrt_bvh_traverse": [_P, _P, _F, _IP, C.c_int64, C.POINTER(C.c_int64)],
+    "xrt_bvh_traverse_noalloc": [_P, _P, _F, _IP, C.c_int64, C.POINTER(C.c_int64)],
     "xrt_vertex_normals": [_P, _FP, _I, _IP, _I, _FP],
     "xrt_autotune": [_P, _I, C.POINTER(XrtTuning)],
     "xrt_get_tuning": [_P, C.POINTER(XrtTuning)],

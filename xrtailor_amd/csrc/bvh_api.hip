@@ -1,6 +1,7 @@
 // C ABI of the standalone LBVH (the reference's BVH class surface and Traverse).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "bvh.h"
@@ -14,8 +15,36 @@ struct xrt_bvh {
   xrt::DevBuf<int32_t> vptr, corner;
 };
 
+// Device staging of xrt_bvh_traverse_noalloc's pairs: a fixed array in place of the
+// per-call allocation of xrt_bvh_traverse.  Each device holds its own copy until the
+// process ends; every call on one device shares it (calls that overlap on a device race).
+// Nothing in it needs to start at zero: a call writes every slot it reads back.
+constexpr int64_t kNoAllocMaxPairs = int64_t{1} << 21;  // 2,097,152 pairs, 16 MiB
+static __device__ __align__(256) int2 scratch[kNoAllocMaxPairs];
+
 namespace {
 using xrt::DevBuf;
+
+// device address of `scratch` per device, resolved once per device by xrt_bvh_create with
+// that device current (the array's host-side name is not a device address)
+std::mutex g_scratch_mu;
+std::vector<int2*> g_scratch_ptr;
+
+int ResolveScratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (static_cast<int>(g_scratch_ptr.size()) <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (g_scratch_ptr[device]) return XRT_OK;
+  void* p = nullptr;
+  XRT_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  XRT_REQUIRE(p != nullptr, XRT_ERR_HIP, "no device address for the pair staging array");
+  g_scratch_ptr[device] = static_cast<int2*>(p);
+  return XRT_OK;
+}
+
+int2* ScratchOf(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device < static_cast<int>(g_scratch_ptr.size()) ? g_scratch_ptr[device] : nullptr;
+}
 
 int Stage(xrt_bvh* h, DevBuf<float4>& dst, const float* src, int n) {
   std::vector<float4> tmp(std::max(n, 1));
@@ -44,6 +73,8 @@ int xrt_bvh_create(xrt_bvh** out, int device) {
   XRT_CHECK_HIP(hipGetDeviceCount(&ndev));
   XRT_REQUIRE(device >= 0 && device < ndev, XRT_ERR_INVALID_ARG, "device index out of range");
   XRT_CHECK_HIP(hipSetDevice(device));
+  const int rc = ResolveScratch(device);  // device is current
+  if (rc != XRT_OK) return rc;
   auto* h = new xrt_bvh();
   h->device = device;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -191,6 +222,36 @@ int xrt_bvh_traverse(xrt_bvh* lhs, xrt_bvh* rhs, float thickness, int32_t* pairs
     XRT_CHECK_HIP(dp.Alloc(take));
     XRT_CHECK_HIP(xrt::LbvhFindPairs(lhs->tree, rhs->tree, thickness, dp.ptr, static_cast<int>(take), s));
     XRT_CHECK_HIP(hipMemcpyAsync(pairs, dp.ptr, sizeof(int2) * take, hipMemcpyDeviceToHost, s));
+    XRT_CHECK_HIP(hipStreamSynchronize(s));
+  }
+  XRT_REQUIRE(total <= cap, XRT_ERR_CAPACITY, "pair buffer smaller than the pair count");
+  return XRT_OK;
+}
+
+int xrt_bvh_traverse_noalloc(xrt_bvh* lhs, xrt_bvh* rhs, float thickness, int32_t* pairs,
+                             int64_t cap, int64_t* n_out) {
+  XRT_REQUIRE(lhs && rhs && n_out && cap >= 0 && (cap == 0 || pairs), XRT_ERR_INVALID_ARG,
+              "null argument");
+  XRT_REQUIRE(lhs->device == rhs->device, XRT_ERR_INVALID_ARG, "trees on different devices");
+  *n_out = 0;
+  if (!lhs->tree.active || !rhs->tree.active) return XRT_OK;
+  int2* stage = ScratchOf(lhs->device);
+  XRT_REQUIRE(stage != nullptr, XRT_ERR_STATE, "pair staging array not resolved for this device");
+  hipStream_t s = lhs->stream;
+  XRT_CHECK_HIP(hipStreamSynchronize(rhs->stream));
+  // count pass into the trees' own per-leaf counts / offsets (allocated by xrt_bvh_build)
+  XRT_CHECK_HIP(xrt::LbvhCountPairs(lhs->tree, rhs->tree, thickness, s));
+  int32_t total = 0;
+  XRT_CHECK_HIP(hipMemcpyAsync(&total, lhs->tree.total.ptr, 4, hipMemcpyDeviceToHost, s));
+  XRT_CHECK_HIP(hipStreamSynchronize(s));
+  *n_out = total;
+  const int64_t take = std::min<int64_t>(total, cap);
+  XRT_REQUIRE(take <= kNoAllocMaxPairs, XRT_ERR_CAPACITY,
+              "pairs to return (" + std::to_string(take) + ") exceed the no-allocation staging "
+              "maximum of 2,097,152 pairs; use xrt_bvh_traverse");
+  if (take > 0) {
+    XRT_CHECK_HIP(xrt::LbvhFindPairs(lhs->tree, rhs->tree, thickness, stage, static_cast<int>(take), s));
+    XRT_CHECK_HIP(hipMemcpyAsync(pairs, stage, sizeof(int2) * take, hipMemcpyDeviceToHost, s));
     XRT_CHECK_HIP(hipStreamSynchronize(s));
   }
   XRT_REQUIRE(total <= cap, XRT_ERR_CAPACITY, "pair buffer smaller than the pair count");
