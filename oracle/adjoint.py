@@ -10,7 +10,8 @@ reversed here:
                                                ->  gx = gx0 + gv k / dt,  gx0 = -gv k / dt
 
 Contact records (VF face and q, EE s, t, fn) are held fixed, as the library's adjoint
-does.  PBD repulsion and long-range attachments have no adjoint (ValueError).
+does.  Long-range attachments replay the slot snapshot of the pass (the pre-LRA state) and
+each target's slot sequence.  PBD repulsion has no adjoint here (ValueError).
 """
 from __future__ import annotations
 
@@ -35,10 +36,11 @@ def _lib():
         lib.orc_adj_stretch_color.argtypes = [Cl, C.c_int, C.c_double, C.c_int, _D, _D, _D, _D]
         lib.orc_adj_bend_color.argtypes = [Cl, P, C.c_int, C.c_double, _D, _D]
         lib.orc_adj_ground.argtypes = [Cl, P, _D, _D]
+        lib.orc_adj_lra.argtypes = [Cl, P, _D]
         lib.orc_adj_vf.argtypes = [_F, _I, _I, _D, _D, C.c_int, C.c_double, _D]
         lib.orc_adj_ee.argtypes = [_D, _D, _I, C.c_int, C.c_int, C.c_double, _D]
         for f in ("orc_adj_stretch_color", "orc_adj_bend_color", "orc_adj_ground", "orc_adj_vf",
-                  "orc_adj_ee"):
+                  "orc_adj_ee", "orc_adj_lra"):
             getattr(lib, f).restype = None
         _bound = True
     return lib
@@ -95,6 +97,10 @@ def backward(scene: osc.ContactScene, tape: list, gx_end, gv_end) -> dict:
                 _, k, xpre = e
                 c.x[:] = xpre
                 lib.orc_adj_bend_color(C.byref(c.s), C.byref(p), k, dt, _d(gx), _d(gb))
+            elif op == "lra":
+                _, xpre = e
+                c.x[:] = xpre
+                lib.orc_adj_lra(C.byref(c.s), C.byref(p), _d(gx))
             elif op == "ground":
                 _, xpre, x0s = e
                 c.x[:] = xpre

@@ -132,6 +132,71 @@ void orc_adj_stretch_color(const orc_cloth* c, int k, double dt, int iter, const
   }
 }
 
+/* ----------------------------------------------------------------- LRA */
+typedef struct { double target; } lra_ctx;
+
+/* in: x_t[3] s[3]; out: x_t'[3] (one slot's projection in orc_lra, active branch) */
+static void lra_elem(const dn* in, dn* out, const void* vctx) {
+  const lra_ctx* c = (const lra_ctx*)vctx;
+  dv3 xt = vin(in, 0), sp = vin(in, 3);
+  dv3 diff = vsub(xt, sp);
+  dn dist = vlen(diff);
+  vout(out, 0, vadd(vadd(xt, vscl(diff, K(-1.0))), vscl(diff, ddiv(K(c->target), dist))));
+}
+
+/* SolveGeodesicLRA (orc_lra) reversed: each target replays its slots in registration order
+ * from the pre-LRA state (c->x), then walks them backwards; the slot snapshot's adjoint is
+ * summed per slot and added to the slot vertex's (the snapshot is x at the pass start). */
+void orc_adj_lra(const orc_cloth* c, const orc_params* p, double* gx) {
+  if (c->nslots <= 0) return;
+  double* sp = (double*)malloc(sizeof(double) * 3 * (size_t)c->nslots);
+  double* gsl = (double*)calloc(3 * (size_t)c->nslots, sizeof(double));
+  int maxk = 0;
+  for (int t = 0; t < c->nv; t++)
+    if (c->lra_ptr[t + 1] - c->lra_ptr[t] > maxk) maxk = c->lra_ptr[t + 1] - c->lra_ptr[t];
+  double* xs = (double*)malloc(sizeof(double) * 3 * (size_t)(maxk + 1));
+  for (int s = 0; s < c->nslots; s++) ld3(c->x, c->lra_slot_vert[s], sp + 3 * s);
+  for (int t = 0; t < c->nv; t++) {
+    const int kb = c->lra_ptr[t], ke = c->lra_ptr[t + 1];
+    if (kb == ke) continue;
+    ld3(c->x, t, xs);
+    for (int k = kb; k < ke; k++) {  /* forward replay: xs[j] = x_t before slot j */
+      double* a = xs + 3 * (k - kb);
+      double* b = a + 3;
+      const double* sk = sp + 3 * c->lra_slot[k];
+      const double target = c->lra_rest[k] * p->long_range_stretchiness;
+      const double d[3] = {a[0] - sk[0], a[1] - sk[1], a[2] - sk[2]};
+      const double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      const int skip = (c->inv_mass[t] < ORC_EPS && target > 0) || !(dist > target);
+      for (int q = 0; q < 3; q++) b[q] = skip ? a[q] : a[q] - d[q] + d[q] * (target / dist);
+    }
+    double g[3];
+    ld3(gx, t, g);
+    for (int k = ke - 1; k >= kb; k--) {
+      const double* a = xs + 3 * (k - kb);
+      const double* sk = sp + 3 * c->lra_slot[k];
+      const double target = c->lra_rest[k] * p->long_range_stretchiness;
+      const double d[3] = {a[0] - sk[0], a[1] - sk[1], a[2] - sk[2]};
+      const double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      if ((c->inv_mass[t] < ORC_EPS && target > 0) || !(dist > target)) continue;
+      lra_ctx ctx = {target};
+      double in[6] = {a[0], a[1], a[2], sk[0], sk[1], sk[2]}, ib[6];
+      elem_vjp(lra_elem, &ctx, in, 6, g, 3, ib);
+      g[0] = ib[0]; g[1] = ib[1]; g[2] = ib[2];
+      double* gs = gsl + 3 * c->lra_slot[k];
+      gs[0] += ib[3]; gs[1] += ib[4]; gs[2] += ib[5];
+    }
+    st3(gx, t, g);
+  }
+  for (int s = 0; s < c->nslots; s++) {
+    double* gv = gx + 3 * (size_t)c->lra_slot_vert[s];
+    gv[0] += gsl[3 * s]; gv[1] += gsl[3 * s + 1]; gv[2] += gsl[3 * s + 2];
+  }
+  free(xs);
+  free(gsl);
+  free(sp);
+}
+
 /* ------------------------------------------------------------------- bend */
 typedef struct { double w[4], theta, dt; int model; } bend_ctx;
 

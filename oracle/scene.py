@@ -300,7 +300,7 @@ class ContactScene:
                     self._rec("bend", k, c.x.copy())
                     lib.orc_bend_color(C.byref(c.s), C.byref(p), k, dt)
             if len(c.slot_vert) > 0:
-                self._rec("lra")
+                self._rec("lra", c.x.copy())
             lib.orc_lra(C.byref(c.s), C.byref(p))
             if p.enable_ground:
                 self._rec("ground", c.x.copy(), x0s)

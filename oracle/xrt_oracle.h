@@ -196,6 +196,7 @@ void orc_adj_stretch_color(const orc_cloth* c, int k, double dt, int iter, const
 void orc_adj_bend_color(const orc_cloth* c, const orc_params* p, int k, double dt, double* gx,
                         double* gb);
 /* ground plane with friction; gx0 += adjoint through the friction's x0 */
+void orc_adj_lra(const orc_cloth* c, const orc_params* p, double* gx);
 void orc_adj_ground(const orc_cloth* c, const orc_params* p, double* gx, double* gx0);
 /* VF-DCD / VF-CCD solve with the records (face, q) held fixed */
 void orc_adj_vf(const float* obs_pred, const int32_t* obs_tris, const int32_t* face, const double* q,

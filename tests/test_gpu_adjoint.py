@@ -27,7 +27,7 @@ class Case:
     """One cloth instance on the GPU and its oracle twin (same topology, colors, state)."""
 
     def __init__(self, xp, n=12, side=0.4, height=0.0, vy=-3.0, pinned=True, obstacle=None,
-                 bend=1e-3, seed=3):
+                 bend=1e-3, seed=3, lra_slots=None, pins=None):
         self.xp = xp
         self.pos, self.tris = scenes.grid(n, side, height)
         rng = scenes.Rng(seed)
@@ -39,10 +39,14 @@ class Case:
         self.bend = bend
         s = ClothSolver(params=xp)
         self.i = s.AddCloth(self.pos, self.tris, 1e-6, bend)
-        if pinned:
+        if pins is not None:
+            s.Pin(self.i, np.asarray(pins, np.int32))
+        elif pinned:
             s.Pin(self.i, np.array([0, n - 1], np.int32))
         if obstacle is not None:
             s.SetObstacle(self.i, *obstacle)
+        if lra_slots is not None:
+            s.AddLraSlots(self.i, np.asarray(lra_slots, np.int32))
         s.Start(seed)
         self.s = s
         self.ne = s.counts(self.i).n_edges
@@ -236,16 +240,31 @@ def test_unsupported_scenes_are_refused():
     c = Case(xp, n=6)
     with pytest.raises(XrtError):
         c.s.AdjointBegin(1, 1)
-    xp = make_params(bend_model=1)
-    s = ClothSolver(params=xp)
-    pos, tris = scenes.grid(6, 0.3)
-    i = s.AddCloth(pos, tris, 1e-6, 1e-3)
-    s.AddLraSlots(i, np.array([0], np.int32))
-    s.Start(1)
+    xp = make_params(bend_model=1, enable_obstacle_contact=1, contact_friction=0.3)
+    c = Case(xp, n=6, obstacle=scenes.icosphere(2, 0.1, (0.0, -1.0, 0.0)))
     with pytest.raises(XrtError):
-        s.AdjointBegin(1, 1)
+        c.s.AdjointBegin(1, 1)
     with pytest.raises(XrtError):
-        s.AdjointBackward()
+        c.s.AdjointBackward()
+
+
+@pytest.mark.parametrize("frames,ckpt", [(1, 1), (2, 2)])
+def test_lra_gradient_matches_oracle(frames, ckpt):
+    """Long-range attachments from the top row of a sheet hanging from every third top
+    vertex and pulled down (soft stretch, so the attachments bind; stretchiness 1.05): the
+    reverse replays each target's slot sequence from the taped pre-LRA state and sums
+    every slot's adjoint in constraint order."""
+    print(f"case adjoint with geodesic LRA, 12x12, {frames} frame(s)", flush=True)
+    xp = make_params(bend_model=1, ground_height=-1.0, num_substeps=4, num_iterations=4,
+                     long_range_stretchiness=1.05)
+    n = 12
+    top = np.arange(n * (n - 1), n * n)
+    c = Case(xp, n=n, lra_slots=top, pins=top[::3], vy=-1.0)
+    c.sc = (c.sc * 1e2).astype(np.float32)
+    g = c.gpu_grad(frames, ckpt=ckpt)
+    ref, tape = c.oracle_grad(frames)
+    assert any(e[0] == "lra" for e in tape)
+    _compare(g, ref)
 
 
 def test_batch_of_rollouts_matches_single_rollouts():

@@ -34,8 +34,22 @@ def _sphere_case(n=9, seed=5):
     return pos, tris, x0, v0, np.zeros(0, np.int32), scenes.icosphere(3, 0.3)
 
 
+def _lra_csr(pos, edges, slots):
+    """Per-target LRA CSR of every slot (registration order) at its geodesic distance."""
+    dist = [orc.geodesic(pos, edges, int(s)) for s in slots]
+    ptr, slot, rest = [0], [], []
+    for t in range(len(pos)):
+        for k, d in enumerate(dist):
+            if np.isfinite(d[t]):
+                slot.append(k)
+                rest.append(np.float32(d[t]))
+        ptr.append(len(slot))
+    return (np.asarray(slots, np.int32), np.asarray(ptr, np.int32), np.asarray(slot, np.int32),
+            np.asarray(rest, np.float64))
+
+
 class Rollout:
-    def __init__(self, case, xp, seed=1):
+    def __init__(self, case, xp, seed=1, lra_slots=None):
         self.pos, self.tris, self.x0, self.v0, self.pinned, self.obs = case
         self.xp = xp
         self.op, _ = orc.params_from(xp)
@@ -50,11 +64,16 @@ class Rollout:
         self.bc = 10.0 ** (-4 + 2 * rng.uniform(len(self.bends)))
         self.w = np.ones(len(self.pos))
         self.w[self.pinned] = 0.0
+        self.lra = {}
+        if lra_slots is not None:
+            sv, ptr, sl, rs = _lra_csr(self.pos, self.edges, lra_slots)
+            self.lra = dict(slot_vert=sv, lra_ptr=ptr, lra_slot=sl, lra_rest=rs)
 
     def scene(self, sc=None, bc=None, v0=None, x0=None):
         cl = orc.Cloth(self.x0 if x0 is None else x0, self.v0 if v0 is None else v0, self.w,
                        self.edges, self.rest, self.sc if sc is None else sc, self.scol, self.sn,
-                       self.bends, self.theta, self.bc if bc is None else bc, self.bcol, self.bn)
+                       self.bends, self.theta, self.bc if bc is None else bc, self.bcol, self.bn,
+                       **self.lra)
         obs = None if self.obs is None else (np.asarray(self.obs[0], np.float32), self.obs[1])
         return osc.ContactScene(cl, self.pos, self.tris, self.edges, self.op, self.xp, obstacle=obs)
 
@@ -116,6 +135,32 @@ def test_reverse_matches_fd_dynamics(bend_model):
     for _ in range(4):
         s.substep()
     assert (s.c.state()[0][:, 1] <= -0.012 + 1e-9).any()
+
+
+def test_reverse_matches_fd_with_long_range_attachments():
+    """Geodesic LRA from the top row of a falling, stretching sheet (soft stretch, so the
+    attachments bind): the reverse through the LRA passes against central differences.
+    Stretchiness 1.05: at 1.0 an attachment to a mesh neighbour puts the vertex at exactly
+    the edge's rest length, where the reference's stretch solve skips the edge (l == r)
+    and drops its multiplier term -- a jump of the forward that no difference quotient
+    resolves."""
+    xp = make_params(bend_model=1, num_substeps=3, num_iterations=4, enable_ground=0,
+                     long_range_stretchiness=1.05)
+    n = 7
+    r = Rollout(_grid_case(n), xp, lra_slots=np.arange(n * (n - 1), n * n))
+    r.sc = r.sc * 1e3                      # soft stretch: the attachments take the load
+    g, tape = _check(r, n_sub=3, tol=1e-6)
+    assert any(e[0] == "lra" for e in tape)
+    # the attachments act: the rollout differs from one without them
+    s = r.scene()
+    keep = r.lra
+    r.lra = {}
+    s2 = r.scene()
+    r.lra = keep
+    for _ in range(3):
+        s.substep()
+        s2.substep()
+    assert np.abs(s.c.state()[0] - s2.c.state()[0]).max() > 1e-6
 
 
 def _fd_jacobian_check(fwd, x, gbar_out, rev, h=1e-7, tol=1e-6):

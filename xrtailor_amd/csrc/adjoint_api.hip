@@ -66,9 +66,6 @@ int AdjSetup(xrt_ctx* c, int want_slots = 1) {
   A.b_max = W.b_max;
   A.contacts = false;
   for (int i = 0; i < n_inst; ++i) {
-    const HostInstance& h = c->inst[i];
-    XRT_REQUIRE(h.desc.nslots == 0, XRT_ERR_UNSUPPORTED,
-                "the adjoint does not support long-range attachments");
     if (c->contact[i]) {
       XRT_REQUIRE(c->contact[i]->frame_substep == 0, XRT_ERR_STATE,
                   "the adjoint starts at a frame boundary");
@@ -79,9 +76,36 @@ int AdjSetup(xrt_ctx* c, int want_slots = 1) {
   }
   const size_t N = static_cast<size_t>(A.n_total);
   const bool bend = p.solve_bending != 0, ground = p.enable_ground != 0;
+  A.lra = c->total_slots > 0;
+  if (A.lra) {  // slot -> constraints (transpose of the per-target CSR), in constraint order
+    std::vector<int32_t> cnt(c->total_slots + 1, 0), sinst(c->total_slots), cons;
+    size_t total_k = 0;
+    for (int i = 0; i < n_inst; ++i) total_k += c->inst[i].lra_slot.size();
+    for (int i = 0; i < n_inst; ++i) {
+      const HostInstance& h = c->inst[i];
+      for (int j = 0; j < h.desc.nslots; ++j) sinst[h.desc.slot_beg + j] = i;
+      for (int32_t q : h.lra_slot) cnt[h.desc.slot_beg + q + 1]++;
+    }
+    for (int g = 0; g < c->total_slots; ++g) cnt[g + 1] += cnt[g];
+    cons.assign(std::max<size_t>(total_k, 1), 0);
+    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+    for (int i = 0; i < n_inst; ++i) {
+      const HostInstance& h = c->inst[i];
+      for (size_t q = 0; q < h.lra_slot.size(); ++q)
+        cons[fill[h.desc.slot_beg + h.lra_slot[q]]++] = h.kbeg + static_cast<int32_t>(q);
+    }
+    XRT_CHECK_HIP(A.slot_inst.Upload(sinst, c->stream));
+    XRT_CHECK_HIP(A.slot_ptr.Upload(cnt, c->stream));
+    XRT_CHECK_HIP(A.slot_cons.Upload(cons, c->stream));
+    if (A.lra_xs.n < cons.size()) {
+      XRT_CHECK_HIP(A.lra_xs.Alloc(cons.size()));
+      XRT_CHECK_HIP(A.lra_gslot.Alloc(cons.size()));
+    }
+  }
   A.off_s = 0;
   A.off_b = A.off_s + 2 * static_cast<size_t>(A.e_total);
-  A.off_g = A.off_b + (bend ? 3 * static_cast<size_t>(A.b_total) : 0);
+  A.off_l = A.off_b + (bend ? 3 * static_cast<size_t>(A.b_total) : 0);
+  A.off_g = A.off_l + (A.lra ? N : 0);
   A.off_c = A.off_g + (ground ? N : 0);
   A.off_e = A.off_c + (A.contacts ? N : 0);
   A.it_stride = A.off_e + (A.contacts ? N : 0);
@@ -151,10 +175,10 @@ int AdjSetup(xrt_ctx* c, int want_slots = 1) {
 
 int ZeroAdjoints(xrt_ctx* c) {
   AdjointData& A = *c->adj;
-  XRT_CHECK_HIP(LaunchZero4(c->stream, A.gx0.ptr, A.n_total));
-  XRT_CHECK_HIP(LaunchZero4(c->stream, A.gv.ptr, A.n_total));
-  XRT_CHECK_HIP(LaunchZero1(c->stream, A.gcomp.ptr, static_cast<int>(A.gcomp.n)));
-  XRT_CHECK_HIP(LaunchZero1(c->stream, A.gbcomp.ptr, static_cast<int>(A.gbcomp.n)));
+  XRT_CHECK_HIP(LaunchZeroD4(c->stream, A.gx0.ptr, A.n_total));
+  XRT_CHECK_HIP(LaunchZeroD4(c->stream, A.gv.ptr, A.n_total));
+  XRT_CHECK_HIP(LaunchZeroD1(c->stream, A.gcomp.ptr, static_cast<int>(A.gcomp.n)));
+  XRT_CHECK_HIP(LaunchZeroD1(c->stream, A.gbcomp.ptr, static_cast<int>(A.gbcomp.n)));
   return XRT_OK;
 }
 
@@ -237,10 +261,12 @@ int WideSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, AdjointData
         XRT_CHECK_HIP(LaunchWideBend(s, wa, n_inst, col, W.b_max[col], k.inv_dt2, bm,
                                      TI ? TI + A->off_b : nullptr, W.maps.ptr + W.b_map[col],
                                      W.b_blocks[col], c->tuning.wide_per_thread));
-    if (c->total_slots > 0)  // never taped: the adjoint refuses long-range attachments
+    if (c->total_slots > 0) {
+      if (TI) XRT_CHECK_HIP(LaunchCopy4(s, TI + A->off_l, c->x.ptr, N));
       XRT_CHECK_HIP(LaunchWideLra(s, c->desc.ptr, n_inst, c->max_slots, c->max_nv, c->x.ptr,
                                   c->slot_vert.ptr, c->slot_pos_all.ptr, c->lra_ptr.ptr,
                                   c->lra_slot.ptr, c->lra_rest.ptr, k.stretchiness));
+    }
     if (k.enable_ground) {
       if (TI) XRT_CHECK_HIP(LaunchCopy4(s, TI + A->off_g, c->x.ptr, N));
       XRT_CHECK_HIP(LaunchGround(s, c->x.ptr, c->x0.ptr, N, k.ground_y, k.friction));
@@ -279,8 +305,10 @@ int ReverseSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, int sub)
   const WideArgs wa = MakeWideArgs(c);
   const float4* T =
       A.tape.ptr + (static_cast<size_t>(A.slot) * c->params.num_substeps + sub) * A.sub_stride;
-  XRT_CHECK_HIP(LaunchFinalizeAdj(s, A.gx0.ptr, A.gv.ptr, A.gx.ptr, N, k.damp_factor / k.dt));
-  XRT_CHECK_HIP(LaunchZero1(s, A.glam.ptr, static_cast<int>(A.glam.n)));
+  // the forward's finalize: v = (x - x0) * (1 / dt) * damp_factor (FP32 factors)
+  const double fin = static_cast<double>(1.0f / k.dt) * static_cast<double>(k.damp_factor);
+  XRT_CHECK_HIP(LaunchFinalizeAdj(s, A.gx0.ptr, A.gv.ptr, A.gx.ptr, N, fin));
+  XRT_CHECK_HIP(LaunchZeroD1(s, A.glam.ptr, static_cast<int>(A.glam.n)));
   for (int it = k.iterations - 1; it >= 0; --it) {
     const float4* TI = T + N + static_cast<size_t>(it) * A.it_stride;
     if (A.contacts) {
@@ -305,6 +333,12 @@ int ReverseSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, int sub)
     }
     if (k.enable_ground)
       XRT_CHECK_HIP(LaunchGroundAdj(s, TI + A.off_g, T, A.gx.ptr, A.gx0.ptr, N, k.ground_y, k.friction));
+    if (A.lra)
+      XRT_CHECK_HIP(LaunchWideLraAdj(s, c->desc.ptr, n_inst, c->max_nv, c->total_slots,
+                                     TI + A.off_l, c->slot_vert.ptr, c->lra_ptr.ptr,
+                                     c->lra_slot.ptr, c->lra_rest.ptr, k.stretchiness,
+                                     A.lra_xs.ptr, A.lra_gslot.ptr, A.slot_inst.ptr,
+                                     A.slot_ptr.ptr, A.slot_cons.ptr, A.gx.ptr));
     if (k.solve_bending)
       for (int col = static_cast<int>(A.b_max.size()) - 1; col >= 0; --col)
         XRT_CHECK_HIP(LaunchWideBendAdj(s, wa, n_inst, col, A.b_max[col], k.inv_dt2, bm,
@@ -380,16 +414,16 @@ int xrt_adjoint_seed(xrt_ctx* ctx, int32_t inst, const float* grad_x, const floa
     if ((rc = ZeroAdjoints(ctx)) != XRT_OK) return rc;
     A.has_grad = false;
   }
-  std::vector<float4> buf(h.nv);
+  std::vector<double4> buf(h.nv);
   for (int which = 0; which < 2; ++which) {
     const float* g = which == 0 ? grad_x : grad_v;
     for (int i = 0; i < h.nv; ++i) {
-      buf[i] = g ? make_float4(g[3 * i], g[3 * i + 1], g[3 * i + 2], 0.0f) : make_float4(0, 0, 0, 0);
+      buf[i] = g ? make_double4(g[3 * i], g[3 * i + 1], g[3 * i + 2], 0.0) : make_double4(0, 0, 0, 0);
       XRT_REQUIRE(std::isfinite(buf[i].x) && std::isfinite(buf[i].y) && std::isfinite(buf[i].z),
                   XRT_ERR_NOT_FINITE, "non-finite adjoint seed");
     }
-    float4* dst = (which == 0 ? A.gx0.ptr : A.gv.ptr) + h.desc.vbeg;
-    XRT_CHECK_HIP(hipMemcpyAsync(dst, buf.data(), sizeof(float4) * h.nv, hipMemcpyHostToDevice,
+    double4* dst = (which == 0 ? A.gx0.ptr : A.gv.ptr) + h.desc.vbeg;
+    XRT_CHECK_HIP(hipMemcpyAsync(dst, buf.data(), sizeof(double4) * h.nv, hipMemcpyHostToDevice,
                                  ctx->stream));
     XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   }
@@ -411,8 +445,8 @@ int xrt_adjoint_backward(xrt_ctx* ctx) {
   const BendMode bm = ModeOf(k);
   const size_t N = static_cast<size_t>(A.n_total);
   const int K = A.ckpt_every, n_ckpt = (A.n_frames + K - 1) / K;
-  XRT_CHECK_HIP(LaunchZero1(ctx->stream, A.gcomp.ptr, static_cast<int>(A.gcomp.n)));
-  XRT_CHECK_HIP(LaunchZero1(ctx->stream, A.gbcomp.ptr, static_cast<int>(A.gbcomp.n)));
+  XRT_CHECK_HIP(LaunchZeroD1(ctx->stream, A.gcomp.ptr, static_cast<int>(A.gcomp.n)));
+  XRT_CHECK_HIP(LaunchZeroD1(ctx->stream, A.gbcomp.ptr, static_cast<int>(A.gbcomp.n)));
   for (int seg = n_ckpt - 1; seg >= 0; --seg) {
     const int f0 = seg * K, f1 = std::min(f0 + K, A.n_frames);
     if ((rc = LoadState(ctx, A.ckpt.ptr + static_cast<size_t>(seg) * 2 * N)) != XRT_OK) return rc;
@@ -461,12 +495,12 @@ int xrt_adjoint_get_grad_compliance(xrt_ctx* ctx, int32_t inst, float* grad_stre
     float* out = which == 0 ? grad_stretch : grad_bend;
     const int n = which == 0 ? h.ne() : h.nb();
     if (!out || n == 0) continue;
-    std::vector<float> g(n);
-    const float* src = (which == 0 ? A.gcomp.ptr + h.sbeg : A.gbcomp.ptr + h.bbeg);
-    XRT_CHECK_HIP(hipMemcpyAsync(g.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> g(n);
+    const double* src = (which == 0 ? A.gcomp.ptr + h.sbeg : A.gbcomp.ptr + h.bbeg);
+    XRT_CHECK_HIP(hipMemcpyAsync(g.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
     XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     const std::vector<int32_t>& order = which == 0 ? h.s_order : h.b_order;
-    for (int p = 0; p < n; ++p) out[order[p]] = g[p];
+    for (int p = 0; p < n; ++p) out[order[p]] = static_cast<float>(g[p]);
   }
   return XRT_OK;
 }
@@ -477,17 +511,17 @@ int xrt_adjoint_get_grad_state(xrt_ctx* ctx, int32_t inst, float* grad_x0, float
   XRT_REQUIRE(inst >= 0, XRT_ERR_INVALID_ARG, "instance index out of range");
   const HostInstance& h = ctx->inst[inst];
   AdjointData& A = *ctx->adj;
-  std::vector<float4> buf(h.nv);
+  std::vector<double4> buf(h.nv);
   for (int which = 0; which < 2; ++which) {
     float* out = which == 0 ? grad_x0 : grad_v0;
     if (!out) continue;
     XRT_CHECK_HIP(hipMemcpyAsync(buf.data(), (which == 0 ? A.gx0.ptr : A.gv.ptr) + h.desc.vbeg,
-                                 sizeof(float4) * h.nv, hipMemcpyDeviceToHost, ctx->stream));
+                                 sizeof(double4) * h.nv, hipMemcpyDeviceToHost, ctx->stream));
     XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     for (int i = 0; i < h.nv; ++i) {
-      out[3 * i] = buf[i].x;
-      out[3 * i + 1] = buf[i].y;
-      out[3 * i + 2] = buf[i].z;
+      out[3 * i] = static_cast<float>(buf[i].x);
+      out[3 * i + 1] = static_cast<float>(buf[i].y);
+      out[3 * i + 2] = static_cast<float>(buf[i].z);
     }
   }
   return XRT_OK;

@@ -8,6 +8,7 @@
 // EE: delta_i += -(1 - s) lam fn, delta_j += -s lam fn with lam = (c1 - c2).fn - 1e-3,
 //   c1 = (1 - s) x_i + s x_j, then x += relax delta / count.  The record (s, t, fn and the
 //   obstacle edge) is held fixed; counts are recounted from the taped state.
+// The adjoints are FP64 (double4 gx); the activity tests run in FP32 on the taped state.
 #include <hip/hip_runtime.h>
 
 #include "adjoint.h"
@@ -45,7 +46,7 @@ __global__ __launch_bounds__(kBs) void VfAdjKernel(const float4* __restrict__ x_
                                                    const float4* __restrict__ rt_q,
                                                    const int32_t* __restrict__ vf_face,
                                                    const float4* __restrict__ vf_q,
-                                                   float4* __restrict__ gx) {
+                                                   double4* __restrict__ gx) {
   const int i = blockIdx.x * kBs + threadIdx.x;
   if (i >= n) return;
   // forward replay: CCD solve, then DCD solve on its result
@@ -63,13 +64,13 @@ __global__ __launch_bounds__(kBs) void VfAdjKernel(const float4* __restrict__ x_
   const int fd = vf_face[i];
   if (fd >= 0) ad = VfActive(t, fd, vf_q[i], 1e-3f, L3(xi), nd, lam);
   if (!ac && !ad) return;
-  float4 g = gx[i];
+  double4 g = gx[i];
   if (ad) {
-    const float p = nd.x * g.x + nd.y * g.y + nd.z * g.z;
+    const double p = nd.x * g.x + nd.y * g.y + nd.z * g.z;
     g.x -= p * nd.x; g.y -= p * nd.y; g.z -= p * nd.z;
   }
   if (ac) {
-    const float p = nc.x * g.x + nc.y * g.y + nc.z * g.z;
+    const double p = nc.x * g.x + nc.y * g.y + nc.z * g.z;
     g.x -= p * nc.x; g.y -= p * nc.y; g.z -= p * nc.z;
   }
   gx[i] = g;
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(kBs) void EeAdjCountKernel(const float4* __restrict
 __global__ __launch_bounds__(kBs) void EeAdjKernel(const float4* __restrict__ x,
                                                    const int2* __restrict__ edges, int ne,
                                                    const EeRec* __restrict__ rec, float relax,
-                                                   const float4* __restrict__ gx,
+                                                   const double4* __restrict__ gx,
                                                    const int32_t* __restrict__ cnt,
                                                    double* __restrict__ gadd) {
   const int e = blockIdx.x * kBs + threadIdx.x;
@@ -98,34 +99,33 @@ __global__ __launch_bounds__(kBs) void EeAdjKernel(const float4* __restrict__ x,
   const int2 ed = edges[e];
   const EeRec r = rec[e];
   if (!EeActive(r, L3(x[ed.x]), L3(x[ed.y]))) return;
-  const float s = r.a.y;
-  const F3 fn = L3(r.fn);
-  const float4 gi = gx[ed.x], gj = gx[ed.y];
-  const float ci = relax / static_cast<float>(cnt[ed.x]);
-  const float cj = relax / static_cast<float>(cnt[ed.y]);
+  const double s = r.a.y;
+  const double fx = r.fn.x, fy = r.fn.y, fz = r.fn.z;
+  const double4 gi = gx[ed.x], gj = gx[ed.y];
+  const double ci = static_cast<double>(relax) / cnt[ed.x];
+  const double cj = static_cast<double>(relax) / cnt[ed.y];
   // adjoint of lam, then of c1 = (1 - s) x_i + s x_j
-  const float gl = -(1 - s) * ci * (fn.x * gi.x + fn.y * gi.y + fn.z * gi.z) -
-                   s * cj * (fn.x * gj.x + fn.y * gj.y + fn.z * gj.z);
-  const F3 gd = M(fn, gl);
-  const F3 ai = M(gd, 1 - s), aj = M(gd, s);
-  atomicAdd(gadd + 3 * ed.x, static_cast<double>(ai.x));
-  atomicAdd(gadd + 3 * ed.x + 1, static_cast<double>(ai.y));
-  atomicAdd(gadd + 3 * ed.x + 2, static_cast<double>(ai.z));
-  atomicAdd(gadd + 3 * ed.y, static_cast<double>(aj.x));
-  atomicAdd(gadd + 3 * ed.y + 1, static_cast<double>(aj.y));
-  atomicAdd(gadd + 3 * ed.y + 2, static_cast<double>(aj.z));
+  const double gl = -(1 - s) * ci * (fx * gi.x + fy * gi.y + fz * gi.z) -
+                    s * cj * (fx * gj.x + fy * gj.y + fz * gj.z);
+  const double gx_ = fx * gl, gy_ = fy * gl, gz_ = fz * gl;
+  atomicAdd(gadd + 3 * ed.x, (1 - s) * gx_);
+  atomicAdd(gadd + 3 * ed.x + 1, (1 - s) * gy_);
+  atomicAdd(gadd + 3 * ed.x + 2, (1 - s) * gz_);
+  atomicAdd(gadd + 3 * ed.y, s * gx_);
+  atomicAdd(gadd + 3 * ed.y + 1, s * gy_);
+  atomicAdd(gadd + 3 * ed.y + 2, s * gz_);
 }
 
-__global__ __launch_bounds__(kBs) void EeAdjApplyKernel(float4* __restrict__ gx, int n,
+__global__ __launch_bounds__(kBs) void EeAdjApplyKernel(double4* __restrict__ gx, int n,
                                                         int32_t* __restrict__ cnt,
                                                         double* __restrict__ gadd) {
   const int i = blockIdx.x * kBs + threadIdx.x;
   if (i >= n) return;
   if (cnt[i] > 0) {
-    float4 g = gx[i];
-    g.x += static_cast<float>(gadd[3 * i]);
-    g.y += static_cast<float>(gadd[3 * i + 1]);
-    g.z += static_cast<float>(gadd[3 * i + 2]);
+    double4 g = gx[i];
+    g.x += gadd[3 * i];
+    g.y += gadd[3 * i + 1];
+    g.z += gadd[3 * i + 2];
     gx[i] = g;
     gadd[3 * i] = gadd[3 * i + 1] = gadd[3 * i + 2] = 0.0;
     cnt[i] = 0;
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kBs) void EeAdjApplyKernel(float4* __restrict__ gx,
 
 hipError_t LaunchVfAdj(hipStream_t s, const float4* x_pc, int n, const int4* obs_tris,
                        const float4* obs_pred, const int32_t* rt_face, const float4* rt_q,
-                       const int32_t* vf_face, const float4* vf_q, float4* gx) {
+                       const int32_t* vf_face, const float4* vf_q, double4* gx) {
   if (n <= 0) return hipSuccess;
   BvhView t;
   t.tris = obs_tris;
@@ -144,7 +144,7 @@ hipError_t LaunchVfAdj(hipStream_t s, const float4* x_pc, int n, const int4* obs
 }
 
 hipError_t LaunchEeAdj(hipStream_t s, const float4* x_pe, const int2* edges, int ne, int nv,
-                       const EeRec* rec, float relax, float4* gx, int32_t* cnt, double* gadd) {
+                       const EeRec* rec, float relax, double4* gx, int32_t* cnt, double* gadd) {
   if (ne <= 0 || nv <= 0) return hipSuccess;
   EeAdjCountKernel<<<Blocks(ne), kBs, 0, s>>>(x_pe, edges, ne, rec, cnt);
   EeAdjKernel<<<Blocks(ne), kBs, 0, s>>>(x_pe, edges, ne, rec, relax, gx, cnt, gadd);

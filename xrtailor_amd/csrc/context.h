@@ -206,7 +206,8 @@ struct AdjointData {
   // tape of `slots` frames: frame slot f, substep s at (f * num_substeps + s) * sub_stride;
   // iteration it at n_total + it * it_stride within the substep.  `slot` is the frame slot
   // the taped forward writes and the reverse reads.
-  size_t it_stride = 0, sub_stride = 0, off_s = 0, off_b = 0, off_g = 0, off_c = 0, off_e = 0;
+  size_t it_stride = 0, sub_stride = 0, off_s = 0, off_b = 0, off_l = 0, off_g = 0, off_c = 0,
+         off_e = 0;
   int slots = 1, slot = 0;
   DevBuf<float4> tape;
   // contact records of each collision pass of each frame slot (slot * n_passes + pass), per
@@ -220,10 +221,16 @@ struct AdjointData {
   DevBuf<float4> ckpt;    // (x0, v) every ckpt_every frames: [n_ckpt][2][n_total]
   DevBuf<float4> fstart;  // (x0, v) at each frame start of a segment: [ckpt_every][2][n_total]
   DevBuf<float4> final_state;  // (x0, v) after the recorded frames
-  DevBuf<float4> gx0, gv, gx;
-  DevBuf<float> glam, gcomp, gbcomp;
+  DevBuf<double4> gx0, gv, gx;  // FP64 adjoints
+  DevBuf<double> glam, gcomp, gbcomp;
   DevBuf<int32_t> ee_cnt;
   DevBuf<double> ee_gadd;
+  // long-range attachments: per constraint, the target's position before it (replay) and
+  // its slot's adjoint; per global slot, its instance and its constraints (in order)
+  bool lra = false;
+  DevBuf<float4> lra_xs;
+  DevBuf<double4> lra_gslot;
+  DevBuf<int32_t> slot_inst, slot_ptr, slot_cons;
 };
 }  // namespace xrt
 
@@ -280,9 +287,11 @@ struct xrt_ctx {
   xrt::FemData fem;
   // differentiable rollout (allocated by xrt_adjoint_begin)
   std::unique_ptr<xrt::AdjointData> adj;
-  // graph of one batched frame
+  // graph of one batched frame, and the frame schedule it was captured at (each contact
+  // instance's substep of the frame and collision passes, each particle instance's substep)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  std::vector<int> graph_key;
   // timing events
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // xrt_time_repulsion: event pairs recorded around the EE repulsion launches (next free pair)
@@ -315,6 +324,9 @@ int ContactIterPre(xrt_ctx* ctx, int inst, int it, bool* pass_ran);
 int ContactVfSolve(xrt_ctx* ctx, int inst);
 int ContactEeSolve(xrt_ctx* ctx, int inst);
 int ContactSubstepEnd(xrt_ctx* ctx, int inst);
+// after xrt_set_state on a started context: obstacle contact records cleared, cloth tree
+// rebuilt from the new positions
+int ContactStateReset(xrt_ctx* ctx, int inst);
 // the same phases for every instance at once (multi-instance kernels)
 bool ContactsActive(const xrt_ctx* ctx);
 int MultiContactSetup(xrt_ctx* ctx);
@@ -332,6 +344,7 @@ int ParticleConfigure(xrt_ctx* ctx);
 int ParticleSubstepBegin(xrt_ctx* ctx, int inst);  // after predict: hash when due
 int ParticleIter(xrt_ctx* ctx, int inst);          // iteration start: collide + apply
 int ParticleSubstepEnd(xrt_ctx* ctx, int inst);    // after finalize
+int ParticleStateReset(xrt_ctx* ctx, int inst);    // xrt_set_state: neighbor lists rebuilt
 // binding_api.hip: nearest-face records, binding generation and skinning set-up at
 // xrt_start (after ContactStart); the binding solve at the end of every iteration (an
 // instance, or -1 for every instance)

@@ -12,6 +12,9 @@ namespace xrt {
 // this one function, so all of them round identically.
 __device__ __forceinline__ bool StretchSolve(float4& a, float4& b, const StretchC& c, float& lam,
                                              float inv_dt2) {
+  // no FMA contraction: inlined into different launch structures (and unroll variants) the
+  // compiler would otherwise fuse different products, and the paths would round differently
+#pragma clang fp contract(off)
   const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
   const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
   const float w1 = a.w, w2 = b.w;
@@ -39,10 +42,15 @@ __device__ __forceinline__ void StretchProject(P X, const StretchC c, float* __r
   *lam_ptr = lam;
 }
 
+// (no FMA contraction here either: these are inlined into the solve bodies)
 __device__ __forceinline__ float3 Cross(float3 a, float3 b) {
+#pragma clang fp contract(off)
   return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-__device__ __forceinline__ float Dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float Dot(float3 a, float3 b) {
+#pragma clang fp contract(off)
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
 __device__ __forceinline__ float3 Scale(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
 __device__ __forceinline__ float3 Add(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
 __device__ __forceinline__ float3 Sub(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
@@ -54,6 +62,7 @@ __device__ __forceinline__ float3 Sub(float3 a, float3 b) { return make_float3(a
 // On loaded vertex values; returns whether they changed.
 __device__ __forceinline__ bool BendSolve(float4& x1, float4& x2, float4& x3, float4& x4,
                                           const float2 par, float inv_dt2, const BendMode m) {
+#pragma clang fp contract(off)  // see StretchSolve
   const float3 p2 = make_float3(x2.x - x1.x, x2.y - x1.y, x2.z - x1.z);
   const float3 p3 = make_float3(x3.x - x1.x, x3.y - x1.y, x3.z - x1.z);
   const float3 p4 = make_float3(x4.x - x1.x, x4.y - x1.y, x4.z - x1.z);
@@ -168,6 +177,7 @@ template <typename PS>
 __device__ __forceinline__ float4 LraVertex(float4 xt, PS sp, const int32_t* __restrict__ slot,
                                             const float* __restrict__ rest, int kb, int ke,
                                             float stretchiness) {
+#pragma clang fp contract(off)  // see StretchSolve
   for (int k = kb; k < ke; ++k) {
     const float target = rest[k] * stretchiness;
     if (xt.w < kEps && target > 0.0f) continue;
@@ -184,6 +194,7 @@ __device__ __forceinline__ float4 LraVertex(float4 xt, PS sp, const int32_t* __r
 
 // plane y = ground_y with ComputeFriction (xrtailor/src/physics/ClothSolverHelper.cu:156-169)
 __device__ __forceinline__ float4 GroundVertex(float4 x, float4 x0, float ground_y, float mu) {
+#pragma clang fp contract(off)  // see StretchSolve
   if (x.w <= kEps || x.y >= ground_y) return x;
   const float cl = ground_y - x.y;
   float fx = 0.0f, fz = 0.0f;
