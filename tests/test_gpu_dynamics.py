@@ -226,3 +226,37 @@ def test_error_paths_on_device():
     s.Finalize(i, dt)
     s2.SubstepPlain(j)
     np.testing.assert_array_equal(s.GetState(i)[0], s2.GetState(j)[0])
+
+
+@pytest.mark.parametrize("path", ["plain", "batched"])
+def test_reference_bend_model_at_configured_compliance(path):
+    """The reference bend formula (bend_model 0) at the plain workload's configured bend
+    compliance 1e-4: one substep of the 64 x 64 hanging cloth against the FP64 oracle.  The
+    formula amplifies rounding there (a flat hinge measures pi/2 against a rest angle of 0,
+    and its gradients come from the cross products of nearly opposite normals): the FP64
+    oracle's own one-substep result moves by millimetres when its start state is perturbed at
+    FP32 resolution (6e-8 relative).  The GPU is held to 1e-5 relative or to twice the
+    largest such move over six perturbations."""
+    from oracle import oracle as orc
+    _print(f"case reference bend model at compliance 1e-4, {path}")
+    spec = scenes.hanging_cloth(seed=1, n=64, bend=1e-4)
+    params = make_params(bend_model=0)
+    s, (i,) = build([spec], params, seed=5)
+    op, dt = orc.params_from(params)
+    oc = oracle_of(s, i, spec.stretch_compliance, spec.bend_compliance)
+    x_start, v_start = s.GetState(i)
+    s.SubstepPlain(i) if path == "plain" else s.SubstepBatched()
+    x_gpu = s.GetState(i)[0]
+    oc.substep(op, dt)
+    x_ref = oc.state()[0].copy()
+    err = rel_err(x_gpu, x_ref)
+    dev = float(np.abs(x_gpu - x_ref).max())
+    sens = 0.0
+    for trial in range(1, 7):
+        z = np.random.default_rng(trial).standard_normal(x_start.shape)
+        xp = x_start.astype(np.float64) * (1.0 + 6e-8 * z)
+        o2 = oracle_of(s, i, spec.stretch_compliance, spec.bend_compliance, x0=xp, v=v_start)
+        o2.substep(op, dt)
+        sens = max(sens, float(np.abs(o2.state()[0] - x_ref).max()))
+    _print(f"  rel err {err:.3e}; deviation {dev:.3e} m, oracle self-sensitivity {sens:.3e} m")
+    assert err <= TOL or dev <= 2.0 * sens

@@ -288,9 +288,12 @@ class ChainBody:
 
 
 def chain_body(seed: int, joints: int, radius: float, seg: float, y0: float,
-               subdiv: int = 3, mesh: str = "ico") -> ChainBody:
+               subdiv: int = 3, mesh: str = "ico", amp_total: float | None = None) -> ChainBody:
     """mesh "ico": icosphere-based capsule of subdivision `subdiv`; "latlong": the 6,890-
-    vertex latitude-longitude capsule (capsule_latlong_y)."""
+    vertex latitude-longitude capsule (capsule_latlong_y).  Each joint's amplitudes sum to
+    0.6 rad, or, with amp_total, to amp_total / joints (the chain's compound bend stays
+    within amp_total: 24 joints of 0.6 rad each would swing the top of a 0.8 m chain by
+    metres)."""
     rng = Rng(seed)
     half = 0.5 * joints * seg - radius
     ctr = (0.0, y0 + 0.5 * joints * seg, 0.0)
@@ -299,7 +302,8 @@ def chain_body(seed: int, joints: int, radius: float, seg: float, y0: float,
     else:
         pos, tris = capsule_y(radius, max(half, 0.0), subdiv, ctr)
     amp = rng.uniform(joints * 2 * 3).reshape(joints, 2, 3)
-    amp = 0.6 * amp / amp.sum(-1, keepdims=True).sum(-2, keepdims=True)  # sum |a| = 0.6 / joint
+    per_joint = 0.6 if amp_total is None else min(0.6, amp_total / joints)
+    amp = per_joint * amp / amp.sum(-1, keepdims=True).sum(-2, keepdims=True)  # sum |a| per joint
     freq = 0.2 + 1.8 * rng.uniform(joints * 2 * 3).reshape(joints, 2, 3)
     phase = 2.0 * np.pi * rng.uniform(joints * 2 * 3).reshape(joints, 2, 3)
     return ChainBody(pos, tris, y0, seg, amp, freq, phase)
@@ -347,8 +351,8 @@ def differentiable_drop(seed: int = 5, n: int = 128, side: float = 1.0, height: 
 
 def garment(seed: int = 6, ring: int = 160, rows: int = 125, joints: int = 24,
             radius: float = 0.2, height: float = 0.6, gap: float = 0.01,
-            jitter: float = 1e-3, body_subdiv: int = 5,
-            body_mesh: str = "ico") -> tuple[ClothSpec, ChainBody]:
+            jitter: float = 1e-3, body_subdiv: int = 5, body_mesh: str = "ico",
+            amp_total: float | None = None) -> tuple[ClothSpec, ChainBody]:
     """Tube garment (ring x rows vertices, radius `radius`, top ring at y = height as LRA
     slots) around a capsule-chain body whose surface lies `gap` inside it; stretch 1e-8,
     bend 1e-4.  The body moves by ChainBody.at(t)."""
@@ -356,20 +360,26 @@ def garment(seed: int = 6, ring: int = 160, rows: int = 125, joints: int = 24,
     rng = Rng(seed)
     x0 = (pos + jitter * rng.normal(pos.size).reshape(-1, 3)).astype(np.float32)
     seg = (height + 0.2) / joints
-    body = chain_body(seed + 1, joints, radius - gap, seg, -0.1, body_subdiv, body_mesh)
+    body = chain_body(seed + 1, joints, radius - gap, seg, -0.1, body_subdiv, body_mesh, amp_total)
     slots = np.arange((rows - 1) * ring, rows * ring, dtype=np.int32)
     spec = ClothSpec(pos, tris, np.zeros(0, np.int32), 1e-8, 1e-4, x0, (body.pos, body.tris),
                      slots)
     return spec, body
 
 
-def garment_on_body(seed: int = 6, gap: float = 0.01) -> tuple[ClothSpec, ChainBody]:
+def garment_on_body(seed: int = 6, gap: float = 0.01):
     """The garment-on-animated-body workload at its stated shape: a 160 x 125 tube garment
     (20,000 vertices, radius 0.2 m, 0.6 m tall) whose top (shoulder) ring holds the 160
-    long-range attachment slots (every slot to every vertex), around a capsule-chain body
-    of 6,890 vertices (latitude-longitude capsule) skinned by 24 joints, `gap` inside the
-    tube; joint angles are seeded sums of three sinusoids (0.2-2 Hz, sum |a| <= 0.6 rad)."""
-    return garment(seed=seed, ring=160, rows=125, joints=24, gap=gap, body_mesh="latlong")
+    long-range attachment slots (every slot to every vertex) and is bound to the body
+    (stiffness 1, at its start distance), around a capsule-chain body of 6,890 vertices
+    (latitude-longitude capsule) skinned by 24 joints, `gap` inside the tube; joint angles
+    are seeded sums of three sinusoids (0.2-2 Hz) whose amplitudes sum to 0.6 rad over the
+    chain.  Returns (spec, body, bound vertices, stiffness, distance)."""
+    ring, rows = 160, 125
+    spec, body = garment(seed=seed, ring=ring, rows=rows, joints=24, gap=gap, body_mesh="latlong",
+                         amp_total=0.6)
+    top = np.arange((rows - 1) * ring, rows * ring, dtype=np.int32)
+    return spec, body, top, np.ones(ring, np.float32), np.zeros(ring, np.float32)
 
 
 def garments(seed: int = 4, n_instances: int = 64, ring_lo: int = 64, ring_hi: int = 128,
