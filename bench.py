@@ -22,7 +22,7 @@ cpu_baseline the serial FP64 oracle on one core, one substep per instance over a
              batched substep against it (parity at the measured configuration)
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
-                    [--workload batched|swift|quality|garment|hanging]
+                    [--workload batched|swift|quality|garment|garment_body|hanging]
                     (swift: the same 256 cloths with body contact and the particle
                      self-collision of the reference's swift mode instead of PBD repulsion;
                      roofline kernel ParticleCollideKernel, algorithmic bytes per launch
@@ -41,7 +41,15 @@ cpu_baseline the serial FP64 oracle on one core, one substep per instance over a
                      waist, LRA, body contact; a step is the skinning plus one frame, timed
                      by the host clock; roofline kernel LbsKernel, algorithmic bytes 36 per
                      body vertex + 8 per weight entry + 48 per joint)
+                    (garment_body: one 160 x 125 tube garment (20,000 vertices, the shoulder
+                     ring's 160 vertices as LRA slots) on a 6,890-vertex capsule-chain body
+                     skinned by 24 joints every frame, body and self contact; roofline kernel
+                     the EE repulsion kernel, as for the batched workload)
                     [--no-graph]   (un-captured substeps, for profiler runs)
+                    [--rollout-frames 120] [--plain-at 0,40,80]
+                                   (batched workload: the whole 120-frame rollout, per-frame
+                                    times, with the plain path run at the listed frames)
+                    [--no-tune]    (default launch shapes: no autotune before timing)
     python bench.py --copy-calibration   (5 device copies of 256 MiB: the known-size kernel
                                           that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)
 """
@@ -308,9 +316,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "swift", "quality", "garment", "hanging"])
+                    choices=["batched", "swift", "quality", "garment", "garment_body", "hanging"])
     ap.add_argument("--garments", type=int, default=64, help="instances of --workload garment")
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--pair-capacity", type=int, default=256,
+                    help="face pairs per cloth face of the self-contact pair lists")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--plain-frames", type=int, default=1)
     ap.add_argument("--rollout-frames", type=int, default=120,
@@ -366,6 +376,17 @@ def main() -> int:
             for i, g in zip(insts, gs):
                 s.SetObstacleSkinning(i, *g[1].skinning(), g[1].amp.shape[0])
                 s.AddBindings(i, g[2], g[3], g[4])
+    elif a.workload == "garment_body":
+        gs = [scenes.garment_on_body(seed=a.seed)]
+        specs = [gs[0][0]]
+        wname = ("garment on an animated body: 160 x 125 tube (20,000 vertices) with its shoulder "
+                 "ring as 160 LRA slots and bound to a 6,890-vertex capsule-chain body skinned by "
+                 f"24 joints on the GPU each frame, body and self contact, seed {a.seed}")
+
+        def before(s, insts):
+            for i, g in zip(insts, gs):
+                s.SetObstacleSkinning(i, *g[1].skinning(), g[1].amp.shape[0])
+                s.AddBindings(i, g[2], g[3], g[4])
     elif a.workload in ("batched", "swift", "quality"):
         specs = scenes.batched(seed=a.seed, n_instances=a.instances)
         wname = f"batched {a.instances} cloths, side round(32*6^u) in [32,192], seed {a.seed}"
@@ -374,24 +395,32 @@ def main() -> int:
         wname = "hanging cloth 64x64, 1 m, two corners pinned"
     swift = a.workload == "swift"
     quality = a.workload == "quality"
-    contacts = a.workload in ("batched", "swift", "quality", "garment") and not a.no_contacts
-    garment = a.workload == "garment"
+    contacts = (a.workload in ("batched", "swift", "quality", "garment", "garment_body")
+                and not a.no_contacts)
+    garment = a.workload in ("garment", "garment_body")
+    bound = garment
+    self_contact = contacts and not swift and a.workload != "garment"
     # quality: moduli large enough that the strain projection acts from small strains on
     # (it is skipped while sum_i w_i |grad_i|^2 <= 1e-6, which at the reference's default
     # moduli of 1 holds up to strains of 4-20 % on these meshes), bending stiffness of the
     # reference's quality example configuration
     fem = dict(constraint_model=1, fem_xx_stiffness=1e3, fem_yy_stiffness=1e3,
                fem_xy_stiffness=4e2, fem_bending_stiffness=1e-2) if quality else {}
+    # face-pair lists: the 256 cloths draped over their obstacles reach ~76 pairs per face
+    # within the first frames and 130 in the crumpled frames near the end of the 120-frame
+    # rollout; 256 per face holds every pair of the whole rollout (an overflow would make
+    # the timing calls fail with XRT_ERR_CAPACITY)
     params = make_params(bend_model=1, ground_height=-1.0, enable_obstacle_contact=int(contacts),
-                         enable_self_contact=int(contacts and not swift and not garment),
-                         enable_particle_collision=int(swift), **fem)
+                         enable_self_contact=int(self_contact),
+                         enable_particle_collision=int(swift),
+                         pair_capacity_per_face=a.pair_capacity, **fem)
     if quality:
         wname += (", sphere / capsule obstacle and self contact" if contacts else "") + \
             ", FEM strain (moduli 1e3/1e3/4e2) and isometric bending (quality mode)"
     elif swift:
         wname += (", sphere / capsule obstacle" if contacts else "") + \
             ", particle self-collision (swift mode)"
-    elif contacts and not garment:
+    elif contacts and a.workload == "batched":
         wname += ", sphere / capsule obstacle and self contact"
     print(f"[bench] building {len(specs)} instances", file=sys.stderr, flush=True)
     solver, insts = build_solver(specs, params, seed=a.seed, obstacles=contacts,
@@ -490,7 +519,7 @@ def main() -> int:
             kern_ms = solver.time_batched_substep(max(10, a.steps))
             lra = sum(solver.counts(i).lra_constraints for i in insts)
             bytes_kern = 64 * n_verts + 32 * faces + 44 * hinges + 12 * lra
-    elif garment:
+    elif a.workload == "garment":
         kernel = "LbsKernel"
         kern_ms, bind_ms = solver.time_binding(20)
         bodies = sum(len(g[1].pos) for g in gs)
@@ -514,7 +543,7 @@ def main() -> int:
                     "inter_leaved_hash": params.inter_leaved_hash}
     elif contacts:
         kernel = "MultiRepItemKernel<1>"
-        kern_ms, rep_counts = solver.time_repulsion(2)
+        kern_ms, rep_counts = solver.time_repulsion(params.num_substeps)  # a whole frame
         n_edges = sum(solver.counts(i).n_edges for i in insts)
         bytes_kern = 8 * rep_counts["ee_items"] + 16 * n_verts + 8 * n_edges
     else:
@@ -563,7 +592,7 @@ def main() -> int:
     cpu, cpu_par, parity, parity_n = None, None, None, 0
     if not a.no_cpu:
         tw_setup = None
-        if garment:
+        if bound:
             def tw_setup(k, twin):
                 twin.setup_bindings(gs[k][2], gs[k][3], gs[k][4], params.bvh_tolerance)
         elif quality:
@@ -607,8 +636,8 @@ def main() -> int:
         "config": {"workload": wname, "instances": len(specs), "vertices": n_verts,
                    "substeps": params.num_substeps, "iterations": params.num_iterations,
                    "bend_model": "consistent", "obstacle_contact": contacts,
-                   "self_contact": contacts and not swift and not garment,
-                   "particle_collision": swift, "bindings": garment,
+                   "self_contact": self_contact,
+                   "particle_collision": swift, "bindings": bound,
                    "constraints": "FEM strain + isometric bending" if quality else "XPBD",
                    "skinning": "GPU LBS every frame" if garment else None,
                    "step": "one frame of all instances",

@@ -276,6 +276,12 @@ int xrt_get_lra(xrt_ctx* ctx, int32_t inst, int32_t* slot_vert, int32_t* ptr, in
 int xrt_get_fem(xrt_ctx* ctx, int32_t inst, int32_t* face_colors, int32_t* n_face_colors,
                 float* area, float* inv, float* iso_k, float* iso_coeff);
 int xrt_get_inv_mass(xrt_ctx* ctx, int32_t inst, float* w);
+/* sets an instance's positions (x0, also the predicted x) and / or velocities; NULL keeps
+ * one.  Before xrt_start it replaces the initial state.  On a started context new positions
+ * also void what earlier substeps derived from the old ones: the obstacle contact records
+ * are cleared (none acts before the instance's next collision pass), the cloth tree and the
+ * particle neighbor lists are rebuilt from the new positions.  The frame schedule (substep
+ * of the frame, collision passes) is kept. */
 int xrt_set_state(xrt_ctx* ctx, int32_t inst, const float* x0, const float* v);
 int xrt_get_state(xrt_ctx* ctx, int32_t inst, float* x0, float* v);
 /* predicted / iterate positions x (PhysicsMesh nodes' x) */
@@ -382,7 +388,10 @@ int xrt_frames_plain(xrt_ctx* ctx, int32_t nframes);
  * bit for bit. */
 int xrt_substep_batched(xrt_ctx* ctx);
 /* `nframes` frames of all instances; the frame's substep launches are captured once in
- * a hipGraph and replayed */
+ * a hipGraph and replayed.  The capture holds the frame schedule it was recorded at (which
+ * substep starts the frame, when collision passes and particle hashes run); after single
+ * substeps have shifted the schedule the next call captures again, so batched frames equal
+ * plain frames after any sequence of substeps. */
 int xrt_frames_batched(xrt_ctx* ctx, int32_t nframes);
 /* Autotuner (no counterpart in the reference): times every candidate launch shape of the
  * path the context's batched substep takes (persistent: workgroup size 1024 / 768 / 512 /

@@ -222,3 +222,76 @@ def test_pair_overflow_reported_and_lists_regrown():
     grown.FramesBatched(2)
     grown.sync()
     assert _compare(ref, ib, grown, ic, "regrown lists, 2 frames") == 0.0
+
+
+@pytest.mark.parametrize("after", ["plain", "batched"])
+def test_state_reset_then_frame_equals_fresh_start(after):
+    """Frames of a context whose state is set back to the initial positions (at rest) after
+    a few batched frames equal the same frames of a fresh context: nothing of the earlier
+    frames (trees, pair lists, contact records, counters) leaks into the restart."""
+    print(f"case state reset after 4 batched frames, then a {after} frame", flush=True)
+    specs = scenes.batched(seed=4, n_instances=256)[::37][:7]
+    xp = make_params(bend_model=1, ground_height=-1.0, enable_obstacle_contact=1,
+                     enable_self_contact=1, pair_capacity_per_face=128)
+    a, ia = build_solver(specs, xp, seed=4)
+    a.FramesBatched(4)
+    a.sync()
+    for i, sp in zip(ia, specs):
+        a.SetState(i, sp.x0, np.zeros_like(sp.x0))
+    b, ib = build_solver(specs, xp, seed=4)
+    for s in (a, b):
+        s.FramesPlain(1) if after == "plain" else s.FramesBatched(1)
+        s.sync()
+    assert _compare(a, ia, b, ib, f"reset then {after} frame") == 0.0
+
+
+@pytest.mark.parametrize("after", ["plain", "batched"])
+def test_state_reset_mid_frame_equals_fresh_context_at_same_substep(after):
+    """A state set between the substeps of a frame (here after two single batched substeps)
+    clears the obstacle contact records and rebuilds the cloth tree from the new positions:
+    the next frame of a context with 4 frames behind it equals that of a fresh context taken
+    through the same two substeps and the same reset.  Without the reset of the records the
+    first context solves its obstacle contacts against records of the draped cloth until the
+    next collision pass."""
+    print(f"case state reset mid-frame (substep 2 of the frame), then a {after} frame",
+          flush=True)
+    specs = scenes.batched(seed=4, n_instances=256)[::37][:7]
+    xp = make_params(bend_model=1, ground_height=-1.0, enable_obstacle_contact=1,
+                     enable_self_contact=1, pair_capacity_per_face=128)
+    a, ia = build_solver(specs, xp, seed=4)
+    a.FramesBatched(4)
+    b, ib = build_solver(specs, xp, seed=4)
+    for s, ii in ((a, ia), (b, ib)):
+        s.SubstepBatched()
+        s.SubstepBatched()
+        s.sync()
+        for i, sp in zip(ii, specs):
+            s.SetState(i, sp.x0, np.zeros_like(sp.x0))
+    for s in (a, b):
+        s.FramesPlain(1) if after == "plain" else s.FramesBatched(1)
+        s.sync()
+    assert _compare(a, ia, b, ib, f"mid-frame reset then {after} frame") == 0.0
+
+
+def test_graph_frames_after_single_substeps_follow_the_frame_schedule():
+    """Single substeps shift the frame schedule (frame start, collision passes) off the one a
+    captured frame was recorded at; the next batched frames re-capture at the shifted
+    schedule, so batched frames after single substeps equal plain frames after the same
+    single substeps bit for bit."""
+    print("case captured frame, 2 single substeps, 2 more frames: batched vs plain",
+          flush=True)
+    kw = dict(bend_model=1, enable_obstacle_contact=1, enable_self_contact=1, ground_height=-1.0,
+              num_substeps=4, num_iterations=4, num_collision_passes=4, pair_capacity_per_face=64)
+    specs = _specs(seed=19)
+    a, ia, b, ib = _pair(specs, make_params(**kw))
+    a.FramesBatched(1)
+    b.FramesPlain(1)
+    for _ in range(2):
+        a.SubstepBatched()
+        for i in ib:
+            b.SubstepPlain(i)
+    a.FramesBatched(2)
+    b.FramesPlain(2)
+    a.sync()
+    b.sync()
+    assert _compare(a, ia, b, ib, "frame, 2 substeps, 2 frames") == 0.0

@@ -65,6 +65,14 @@ def _listed(nb):
     return (nb != SENT) & (np.cumsum(nb == SENT, axis=0) == 0)
 
 
+def _assert_lists_equal(got, want):
+    """Neighbor lists equal up to and including each list's sentinel.  Entries after it are
+    whatever an earlier hash of the same buffers left there (nothing reads them): the
+    state reset of xrt_set_state hashes once before the test's own hash."""
+    upto = (np.cumsum(want == SENT, axis=0) - (want == SENT)) == 0
+    np.testing.assert_array_equal(got[upto], want[upto])
+
+
 @pytest.mark.parametrize("scene,max_nb", [("layers", 64), ("layers", 4), ("crumpled", 16)])
 def test_hash_matches_oracle(scene, max_nb):
     print(f"case particle hash {scene} max_nb={max_nb}", flush=True)
@@ -79,7 +87,7 @@ def test_hash_matches_oracle(scene, max_nb):
     np.testing.assert_array_equal(g["cell_start"], want["cell_start"])
     set_ = want["cell_start"] != SENT
     np.testing.assert_array_equal(g["cell_end"][set_], want["cell_end"][set_])
-    np.testing.assert_array_equal(g["neighbors"], want["neighbors"])
+    _assert_lists_equal(g["neighbors"], want["neighbors"])
     lst = _listed(want["neighbors"])
     print(f"  {len(x)} particles, {lst.sum()} neighbor entries, {lst.all(axis=0).sum()} full "
           f"lists, {set_.sum()} cells", flush=True)
@@ -183,7 +191,7 @@ def test_particle_params_and_errors():
     g = sh.buffers()
     assert g["neighbors"].shape == (8, len(state))
     want = opart.neighbors(s.GetPredicted(i), spec.x0, g["diameter"], 1.5, 8)
-    np.testing.assert_array_equal(g["neighbors"], want["neighbors"])
+    _assert_lists_equal(g["neighbors"], want["neighbors"])
     with pytest.raises(XrtError):
         s.AdjointBegin(1, 1)  # the adjoint has no particle collision
     s.close()

@@ -80,3 +80,29 @@ def test_tuned_frames_equal_untuned_frames():
             s.close()
         _equal(runs[0], runs[1])
         _equal(runs[2], runs[1])
+
+
+def test_autotune_between_substeps_of_a_frame_changes_nothing():
+    """The autotuner run between the substeps of a frame (here after one frame and three
+    single substeps, with body and self contact: contact records and the frame schedule in
+    use) puts back everything later substeps read, so the following frames equal those of a
+    context that never ran it."""
+    print("case autotune mid-frame with contacts vs no autotune", flush=True)
+    specs = _specs(8)
+    xp = make_params(bend_model=1, ground_height=-1.0, enable_obstacle_contact=1,
+                     enable_self_contact=1, pair_capacity_per_face=128)
+    runs = []
+    for tune in (True, False):
+        s, ids = build_solver(specs, xp, seed=4)
+        s.FramesBatched(1)
+        for _ in range(3):
+            s.SubstepBatched()
+        if tune:
+            t = s.autotune(3)
+            print(f"  chose {t}", flush=True)
+            s.set_tuning(1024, 0, 1)     # the defaults, so that only the autotuner differs
+        s.FramesBatched(2)
+        s.sync()
+        runs.append(_states(s, ids))
+        s.close()
+    _equal(runs[0], runs[1])

@@ -198,8 +198,23 @@ int SubstepBatched(xrt_ctx* c) {
   return ParticleSubstepEnd(c, -1);
 }
 
+// The host-side frame schedule that decides which phases a batched frame launches (frame
+// start, collision passes, particle hashes).  A frame of substeps brings it back to the same
+// value, so a captured frame replays correctly exactly while the schedule equals the one at
+// capture; single substeps (xrt_substep_batched / xrt_substep_plain) shift it.
+std::vector<int> ScheduleKey(const xrt_ctx* c) {
+  std::vector<int> key;
+  for (const auto& cd : c->contact) {
+    key.push_back(cd ? cd->frame_substep : -1);
+    key.push_back(cd ? cd->pass_cnt : -1);
+  }
+  for (int s : c->particles.substep) key.push_back(s);
+  return key;
+}
+
 int BuildGraph(xrt_ctx* c) {
   DestroyGraph(c);
+  c->graph_key = ScheduleKey(c);
   XRT_CHECK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int rc = XRT_OK;
   for (int s = 0; s < c->params.num_substeps && rc == XRT_OK; ++s) rc = SubstepBatched(c);
@@ -217,15 +232,52 @@ int BuildGraph(xrt_ctx* c) {
 
 // Everything a run of batched substeps changes outside the iterate, saved so that the
 // autotuner can time candidate launch shapes on the context's own workload and put the
-// context back where it was: the state (x0, v), each contact instance's frame position and
-// obstacle frame-start positions, the particle hash schedule.
+// context back where it was: the state (x0, v), each contact instance's frame position,
+// obstacle frame-start positions and contact records (solved until the next collision
+// pass), the particle hash schedule and neighbor lists (used until the next hash).
+struct SavedRecords {
+  DevBuf<int32_t> vf_face, rt_face;
+  DevBuf<float4> vf_q, rt_q;
+  DevBuf<char> ee;
+};
+
 struct SavedRun {
   DevBuf<float4> x0, v;
   std::vector<std::unique_ptr<DevBuf<float4>>> o_x0;
+  std::vector<std::unique_ptr<SavedRecords>> rec;
   std::vector<int> frame_substep, pass_cnt;
   std::vector<char> moving;
   std::vector<int> particle_substep;
+  DevBuf<uint32_t> nb;
 };
+
+template <class T>
+hipError_t CopyBuf(DevBuf<T>& dst, const DevBuf<T>& src, hipStream_t s, bool alloc) {
+  if (alloc) {
+    const hipError_t e = dst.Alloc(src.n);
+    if (e != hipSuccess) return e;
+  }
+  if (src.n == 0) return hipSuccess;
+  return hipMemcpyAsync(dst.ptr, src.ptr, sizeof(T) * src.n, hipMemcpyDeviceToDevice, s);
+}
+
+int CopyRecords(SavedRecords& dst, const ContactData& cd, hipStream_t s) {
+  XRT_CHECK_HIP(CopyBuf(dst.vf_face, cd.vf_face, s, true));
+  XRT_CHECK_HIP(CopyBuf(dst.rt_face, cd.rt_face, s, true));
+  XRT_CHECK_HIP(CopyBuf(dst.vf_q, cd.vf_q, s, true));
+  XRT_CHECK_HIP(CopyBuf(dst.rt_q, cd.rt_q, s, true));
+  XRT_CHECK_HIP(CopyBuf(dst.ee, cd.ee, s, true));
+  return XRT_OK;
+}
+
+int RestoreRecords(ContactData& cd, const SavedRecords& src, hipStream_t s) {
+  XRT_CHECK_HIP(CopyBuf(cd.vf_face, src.vf_face, s, false));
+  XRT_CHECK_HIP(CopyBuf(cd.rt_face, src.rt_face, s, false));
+  XRT_CHECK_HIP(CopyBuf(cd.vf_q, src.vf_q, s, false));
+  XRT_CHECK_HIP(CopyBuf(cd.rt_q, src.rt_q, s, false));
+  XRT_CHECK_HIP(CopyBuf(cd.ee, src.ee, s, false));
+  return XRT_OK;
+}
 
 int SaveRun(xrt_ctx* c, SavedRun& r) {
   const size_t n = c->x0.n;
@@ -245,8 +297,15 @@ int SaveRun(xrt_ctx* c, SavedRun& r) {
                                    hipMemcpyDeviceToDevice, s));
     }
     r.o_x0.push_back(std::move(b));
+    auto rec = std::make_unique<SavedRecords>();
+    if (cd) {
+      const int rc = CopyRecords(*rec, *cd, s);
+      if (rc != XRT_OK) return rc;
+    }
+    r.rec.push_back(std::move(rec));
   }
   r.particle_substep = c->particles.substep;
+  if (c->particles.sized) XRT_CHECK_HIP(CopyBuf(r.nb, c->particles.nb, s, true));
   XRT_CHECK_HIP(hipStreamSynchronize(s));
   return XRT_OK;
 }
@@ -266,8 +325,12 @@ int RestoreRun(xrt_ctx* c, const SavedRun& r) {
     if (cd->obs_nv > 0)
       XRT_CHECK_HIP(hipMemcpyAsync(cd->o_x0.ptr, r.o_x0[k]->ptr, sizeof(float4) * cd->obs_nv,
                                    hipMemcpyDeviceToDevice, s));
+    const int rc = RestoreRecords(*cd, *r.rec[k], s);
+    if (rc != XRT_OK) return rc;
   }
   c->particles.substep = r.particle_substep;
+  if (c->particles.sized && r.nb.n == c->particles.nb.n)
+    XRT_CHECK_HIP(CopyBuf(c->particles.nb, r.nb, s, false));
   XRT_CHECK_HIP(hipStreamSynchronize(s));
   return XRT_OK;
 }
@@ -850,7 +913,10 @@ int xrt_set_state(xrt_ctx* ctx, int32_t inst, const float* x0, const float* v) {
     if (v) h.v_init.assign(v, v + 3 * h.nv);
     return XRT_OK;
   }
-  return UploadState(ctx, inst, x0, v);
+  rc = UploadState(ctx, inst, x0, v);
+  if (rc != XRT_OK || !x0) return rc;
+  if ((rc = ContactStateReset(ctx, inst)) != XRT_OK) return rc;
+  return ParticleStateReset(ctx, inst);
 }
 
 int xrt_get_state(xrt_ctx* ctx, int32_t inst, float* x0, float* v) {
@@ -974,7 +1040,7 @@ int xrt_substep_batched(xrt_ctx* ctx) {
 int xrt_frames_batched(xrt_ctx* ctx, int32_t nframes) {
   XRT_REQUIRE(ctx && ctx->started, XRT_ERR_STATE, "xrt_start has not been called");
   XRT_REQUIRE(nframes >= 0, XRT_ERR_INVALID_ARG, "nframes must be >= 0");
-  if (!ctx->graph_exec) {
+  if (!ctx->graph_exec || ScheduleKey(ctx) != ctx->graph_key) {
     int rc = BuildGraph(ctx);
     if (rc != XRT_OK) return rc;
   }
@@ -1075,7 +1141,7 @@ int xrt_time_batched_substep(xrt_ctx* ctx, int32_t n, float* avg_ms) {
 int xrt_time_frames(xrt_ctx* ctx, int32_t batched, int32_t nframes, float* ms) {
   XRT_REQUIRE(ctx && ctx->started, XRT_ERR_STATE, "xrt_start has not been called");
   XRT_REQUIRE(nframes > 0 && ms, XRT_ERR_INVALID_ARG, "nframes must be > 0");
-  if (batched && !ctx->graph_exec) {
+  if (batched && (!ctx->graph_exec || ScheduleKey(ctx) != ctx->graph_key)) {
     int rc = BuildGraph(ctx);
     if (rc != XRT_OK) return rc;
   }

@@ -197,6 +197,26 @@ int ContactStart(xrt_ctx* ctx) {
   return MultiContactSetup(ctx);
 }
 
+// A state set from outside (xrt_set_state on a started context) voids what earlier substeps
+// derived from the old state: the obstacle contact records (VF / RT / EE) are cleared, so no
+// record acts before the instance's next collision pass, and the cloth tree is rebuilt from
+// the new positions (a fresh start builds it from the initial state the same way).
+int ContactStateReset(xrt_ctx* ctx, int inst) {
+  if (inst < 0 || static_cast<size_t>(inst) >= ctx->contact.size() || !ctx->contact[inst])
+    return XRT_OK;
+  ContactData& c = *ctx->contact[inst];
+  const HostInstance& h = ctx->inst[inst];
+  hipStream_t s = ctx->stream;
+  XRT_CHECK_HIP(hipMemsetAsync(c.vf_face.ptr, 0xff, sizeof(int32_t) * h.nv, s));
+  XRT_CHECK_HIP(hipMemsetAsync(c.rt_face.ptr, 0xff, sizeof(int32_t) * h.nv, s));
+  XRT_CHECK_HIP(hipMemsetAsync(c.ee.ptr, 0, c.ee.n, s));
+  if (ctx->params.enable_self_contact)
+    XRT_CHECK_HIP(LbvhBuild(c.cloth_bvh, ctx->x0.ptr + h.desc.vbeg, nullptr, nullptr, 0.0f, 0,
+                            0.0f, s));
+  XRT_CHECK_HIP(hipStreamSynchronize(s));
+  return XRT_OK;
+}
+
 // Contact phases of one instance's substep, shared by the plain driver below and the
 // wide (all instances per launch) driver of the adjoint.
 int ContactSubstepBegin(xrt_ctx* ctx, int inst) {

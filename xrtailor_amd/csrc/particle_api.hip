@@ -214,6 +214,16 @@ int ParticleIter(xrt_ctx* ctx, int inst) {
   return Collide(ctx, inst);
 }
 
+int ParticleStateReset(xrt_ctx* ctx, int inst) {
+  if (!ParticlesOn(ctx) || !ctx->particles.sized) return XRT_OK;
+  XRT_REQUIRE(inst >= 0 && static_cast<size_t>(inst) < ctx->particles.one_v.size(), XRT_ERR_STATE,
+              "particle maps missing for the instance");
+  const int rc = Hash(ctx, inst);
+  if (rc != XRT_OK) return rc;
+  XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return XRT_OK;
+}
+
 int ParticleSubstepEnd(xrt_ctx* ctx, int inst) {
   ParticleData& P = ctx->particles;
   const int S = ctx->params.num_substeps;
