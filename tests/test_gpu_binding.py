@@ -80,17 +80,27 @@ def _wrap(body, off=0.02):
                             1e-4, pos.astype(np.float32), (body.pos, body.tris))
 
 
-@pytest.mark.parametrize("scene,tol", [("garment", 0.003), ("garment", 0.0), ("wrap", 0.003)])
+@pytest.mark.parametrize("scene,tol", [("garment", 0.003), ("garment", 0.0), ("wrap", 0.003),
+                                       ("far", 0.0)])
 def test_nearest_barycentric_matches_oracle(scene, tol):
     """Face, weights and distance bit for bit from the same FP32 inputs (ties between
-    faces sharing the nearest vertex or edge resolved in the same traversal order)."""
+    faces sharing the nearest vertex or edge resolved in the same traversal order).  "far":
+    the wrapped body 100 m from the origin with the cloth 0.1 mm off it, where the FP32
+    point-triangle distances carry errors of several percent and near ties abound."""
     print(f"case nearest barycentric, {scene}, bvh tolerance {tol}", flush=True)
     spec, body, *_ = _garment()
+    ox0 = np.asarray(body.pos, np.float32)
     if scene == "wrap":
         spec = _wrap(body)
+    elif scene == "far":
+        shift = np.array([100.0, 0.0, -100.0])
+        w = _wrap(body, off=1e-4)
+        ox0 = (np.asarray(body.pos) + shift).astype(np.float32)
+        pos = w.pos + shift
+        spec = scenes.ClothSpec(pos, w.tris, w.pinned, 1e-8, 1e-4, pos.astype(np.float32),
+                                (ox0.astype(np.float64), body.tris))
     s, (i,) = build_solver([spec], _xp(bvh_tolerance=tol), seed=1)
     idx, uvw = s.QueryNearestBarycentric(i)
-    ox0 = np.asarray(body.pos, np.float32)
     tree = orc.Bvh()
     tree.Build(ox0, body.tris, tol=tol, n=orc.vertex_normals(ox0, body.tris))
     faces, wo, do = ob.nearest(tree, s.GetState(i)[0])

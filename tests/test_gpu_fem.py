@@ -327,3 +327,44 @@ def test_fem_with_bindings_on_a_skinned_body_matches_oracle_twin():
     assert errs[0] <= TOL
     assert max(errs) <= 1e-4
     s.close()
+
+
+def test_wide_fem_two_faces_per_thread_on_large_colors():
+    """A 72 x 72 sheet (jittered rest shape, deformed state) with an obstacle below it (body
+    contact on: the batched substep is the wide one, but nothing touches) has face colors of
+    more than 512 faces, so the wide strain launch's second face per thread runs: a
+    substep against the FP64 twin, batched frames equal to plain frames bit for bit, and the
+    FEM timing hook over the same launches."""
+    print("case FEM wide launches with > 512 faces per color: 72 x 72 sheet, obstacle below",
+          flush=True)
+    ball = scenes.icosphere(2, 0.3)
+    spec = dataclasses.replace(_sheet(n=72), obstacle=(ball[0] - np.array([0.0, 3.0, 0.0]), ball[1]))
+    v = np.zeros_like(spec.x0)
+    xp = _params(enable_obstacle_contact=1, ground_height=-5.0, num_substeps=4, num_iterations=4)
+    s, (i,) = build([spec], xp, seed=3)
+    s.SetState(i, spec.x0, v)
+    f = s.fem(i)
+    per_color = np.bincount(f["colors"], minlength=f["n_colors"])
+    print(f"  {len(spec.tris)} faces in {f['n_colors']} colors, largest {per_color.max()}",
+          flush=True)
+    assert per_color.max() > 512
+    twin = _twin(spec, xp, s, i)
+    s.SubstepBatched()
+    twin.substep()
+    e = rel_err(s.GetState(i)[0], twin.c.state()[0])
+    print(f"  one wide substep vs twin: rel err {e:.2e}", flush=True)
+    assert e <= TOL
+    s.close()
+    runs = []
+    for batched in (True, False):
+        s, (i,) = build([spec], xp, seed=3)
+        s.SetState(i, spec.x0, v)
+        s.FramesBatched(2) if batched else s.FramesPlain(2)
+        runs.append(s.GetState(i)[0])
+        if batched:
+            strain_ms, iso_ms = s.time_fem(2)
+            print(f"  time_fem: strain {strain_ms * 1e3:.1f} us, isometric bending "
+                  f"{iso_ms * 1e3:.1f} us per launch", flush=True)
+            assert strain_ms > 0 and iso_ms > 0
+        s.close()
+    np.testing.assert_array_equal(runs[0], runs[1])

@@ -96,11 +96,10 @@ __device__ float TriNearest(float4 p, float4 a, float4 b, float4 c, float& u, fl
 // first with a 64-entry stack of (node, squared min distance); children tested left then
 // right and pushed in that order; a popped node whose squared min distance exceeds the best
 // (unsquared) distance is skipped; a child is kept when its min distance <= its sibling's
-// min-max distance; a leaf replaces the best when its distance is <= it.
-// A popped node is also skipped when its squared min distance exceeds best^2 (1 + 1e-5):
-// every face under it is then strictly farther than the best, so it could neither replace
-// it nor tie with it, and the result is the reference's; below 1 m this prunes what the
-// mixed squared / unsquared test keeps.
+// min-max distance; a leaf replaces the best when its distance is <= it.  No other pruning:
+// a tighter squared test would skip faces whose FP32 point-triangle distance can still come
+// out <= the best (its rounding error is ~ulp(|coordinates|) / distance, far above any fixed
+// relative margin at millimetre distances), and the chosen face would then differ.
 __global__ __launch_bounds__(kBlock) void NearestBaryKernel(BvhView t, const float4* __restrict__ q,
                                                             int n, int4* __restrict__ out_idx,
                                                             float4* __restrict__ out_uvw,
@@ -118,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void NearestBaryKernel(BvhView t, const flo
   bool overflow = false;
   do {
     const int node = st_node[--sp];
-    if (st_dist[sp] > best || st_dist[sp] > best * best * 1.00001f) continue;
+    if (st_dist[sp] > best) continue;
     const int4 nd = t.nodes[node];
     const int kid[2] = {nd.y, nd.z};
     const Box kb0 = t.box[kid[0]], kb1 = t.box[kid[1]];
