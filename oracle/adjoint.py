@@ -11,7 +11,9 @@ reversed here:
 
 Contact records (VF face and q, EE s, t, fn) are held fixed, as the library's adjoint
 does.  Long-range attachments replay the slot snapshot of the pass (the pre-LRA state) and
-each target's slot sequence.  PBD repulsion has no adjoint here (ValueError).
+each target's slot sequence.  PBD repulsion replays the pass's recorded responses (the
+orientation of each VF response and each EE normal's source are held fixed; the normals of
+free edge pairs follow the start-of-substep state) with the pass's Jacobi counts.
 """
 from __future__ import annotations
 
@@ -39,8 +41,11 @@ def _lib():
         lib.orc_adj_lra.argtypes = [Cl, P, _D]
         lib.orc_adj_vf.argtypes = [_F, _I, _I, _D, _D, C.c_int, C.c_double, _D]
         lib.orc_adj_ee.argtypes = [_D, _D, _I, C.c_int, C.c_int, C.c_double, _D]
+        lib.orc_adj_repulsion.argtypes = [C.POINTER(osc.RepItem), C.c_long,
+                                          C.POINTER(osc.RMeshIn), C.POINTER(osc.RMeshIn), _I,
+                                          C.c_int, C.c_double, C.c_double, _D, _D]
         for f in ("orc_adj_stretch_color", "orc_adj_bend_color", "orc_adj_ground", "orc_adj_vf",
-                  "orc_adj_ee", "orc_adj_lra"):
+                  "orc_adj_ee", "orc_adj_lra", "orc_adj_repulsion"):
             getattr(lib, f).restype = None
         _bound = True
     return lib
@@ -115,6 +120,16 @@ def backward(scene: osc.ContactScene, tape: list, gx_end, gv_end) -> dict:
                 _, rec, xpre, relax = e
                 lib.orc_adj_ee(_d(rec), _d(xpre), scene.edges.ctypes.data_as(_I), scene.ne, c.nv,
                                relax, _d(gx))
+            elif op == "repulsion":
+                _, items, xpre, x0s, count, h, relax = e
+                c.x[:] = xpre
+                c.x0[:] = x0s
+                arr = (osc.RepItem * max(len(items), 1))(*items)
+                cm = scene._rmesh(False)
+                om = scene._rmesh(True) if scene.has_obs else None
+                lib.orc_adj_repulsion(arr, len(items), C.byref(cm),
+                                      C.byref(om) if om is not None else None,
+                                      count.ctypes.data_as(_I), c.nv, h, relax, _d(gx), _d(gx0))
             else:
                 raise ValueError(f"no adjoint for operator {op!r}")
     finally:

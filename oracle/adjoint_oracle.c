@@ -25,7 +25,7 @@
 
 #define ORC_PI 3.14159265358979323846
 #define ORC_EPS 1e-6 /* EPSILON (xrtailor/include/xrtailor/core/Scalar.hpp:37) */
-#define MAXIO 16
+#define MAXIO 24
 
 /* ----------------------------------------------------------- dual numbers */
 typedef struct { double v, d; } dn;
@@ -399,4 +399,103 @@ void orc_adj_ee(const double* rec12, const double* x_pre, const int32_t* edges, 
   for (int k = 0; k < 3 * nv; k++) gx[k] += add[k];
   free(add);
   free(cnt);
+}
+
+/* ----------------------------------------------------------- PBD repulsion */
+typedef struct { double side, h, w[4]; } vfr_ctx;
+
+/* in: node[3] face corners y0 y1 y2 [3 each]; out: corrections of node, y0, y1, y2
+ * (EvaluateVFProximity's response with the recorded orientation, contact_oracle.c eval_vf) */
+static void vfr_elem(const dn* in, dn* out, const void* vctx) {
+  const vfr_ctx* c = (const vfr_ctx*)vctx;
+  dv3 p1 = vin(in, 3);
+  dv3 q = vsub(vin(in, 0), p1), p2 = vsub(vin(in, 6), p1), p3 = vsub(vin(in, 9), p1);
+  dv3 n = vscl(vcrs(p2, p3), K(c->side));
+  dn c23 = vlen(n);
+  n = vnrm(n);
+  dn C = dsub(vdot(q, n), K(c->h));
+  dn nq = vdot(n, q);
+  dv3 dc[4];
+  dc[0] = n;
+  dc[2] = vscl(vadd(vcrs(p3, q), vscl(vcrs(n, p3), nq)), ddiv(K(1.0), c23));
+  dc[3] = vscl(vadd(vcrs(p2, q), vscl(vcrs(n, p2), nq)), ddiv(K(-1.0), c23));
+  dc[1] = vsub(vsub(vscl(dc[0], K(-1.0)), dc[2]), dc[3]);
+  dn den = K(0.0);
+  for (int i = 0; i < 4; i++) den = dadd(den, dmul(K(c->w[i]), vdot(dc[i], dc[i])));
+  dn s = ddiv(C, den);
+  for (int i = 0; i < 4; i++) vout(out, 3 * i, vscl(dc[i], dmul(K(-c->w[i]), s)));
+}
+
+typedef struct { int pn_x0; double pn_sign, pn[3], h, w[4]; } eer_ctx;
+
+static inline dn dmixed(dv3 a, dv3 b, dv3 c) { return vdot(a, vcrs(b, c)); }
+
+/* in: x_a0 x_a1 x_b0 x_b1 (current) then the same four at the start of the substep [3
+ * each]; out: corrections of a0, a1, b0, b1 (EvaluateEEProximity's response, eval_ee) */
+static void eer_elem(const dn* in, dn* out, const void* vctx) {
+  const eer_ctx* c = (const eer_ctx*)vctx;
+  dv3 x0 = vin(in, 0), x1 = vin(in, 3), y0 = vin(in, 6), y1 = vin(in, 9);
+  dv3 pn;
+  if (c->pn_x0)
+    pn = vscl(vcrs(vnrm(vsub(vin(in, 15), vin(in, 12))), vnrm(vsub(vin(in, 21), vin(in, 18)))),
+              K(c->pn_sign));
+  else
+    pn = V3(K(c->pn[0]), K(c->pn[1]), K(c->pn[2]));
+  dv3 n = vcrs(vnrm(vsub(x1, x0)), vnrm(vsub(y1, y0)));
+  dn a0 = dmixed(vsub(y1, x1), vsub(y0, x1), n), a1 = dmixed(vsub(y0, x0), vsub(y1, x0), n);
+  dn b0 = dmixed(vsub(x0, y1), vsub(x1, y1), n), b1 = dmixed(vsub(x1, y0), vsub(x0, y0), n);
+  dn w[4] = {ddiv(a0, dadd(a0, a1)), ddiv(a1, dadd(a0, a1)), dneg(ddiv(b0, dadd(b0, b1))),
+             dneg(ddiv(b1, dadd(b0, b1)))};
+  dv3 dd = vadd(vadd(vscl(x0, w[0]), vscl(x1, w[1])), vadd(vscl(y0, w[2]), vscl(y1, w[3])));
+  dn C = dsub(vdot(dd, pn), K(c->h));
+  dn den = K(0.0);
+  for (int i = 0; i < 4; i++) den = dadd(den, dmul(K(c->w[i]), dmul(w[i], w[i])));
+  dn s = ddiv(C, den);
+  for (int i = 0; i < 4; i++) vout(out, 3 * i, vscl(pn, dmul(w[i], dmul(K(-c->w[i]), s))));
+}
+
+void orc_adj_repulsion(const orc_rep_item* items, long n, const orc_rmesh_in* cloth,
+                       const orc_rmesh_in* obstacle, const int32_t* count, int nv, double h,
+                       double relax, double* gx, double* gx0) {
+  double* add = (double*)calloc(3 * (size_t)nv, sizeof(double));
+  double* add0 = (double*)calloc(3 * (size_t)nv, sizeof(double));
+  for (long k = 0; k < n; k++) {
+    const orc_rep_item* it = items + k;
+    const orc_rmesh_in* other = it->other_cloth ? cloth : obstacle;
+    /* corner i's mesh: VF node (cloth) + face (other); EE edge a (cloth) + edge b (other) */
+    const orc_rmesh_in* ms[4] = {cloth, other, other, other};
+    if (it->kind == 1) ms[1] = cloth;
+    double x[24], yb[12], xb[24];
+    int nin = it->kind == 0 ? 12 : 24;
+    double w[4];
+    for (int i = 0; i < 4; i++) {
+      ld3(ms[i]->x, it->ids[i], x + 3 * i);
+      if (it->kind == 1) ld3(ms[i]->x0, it->ids[i], x + 12 + 3 * i);
+      w[i] = ms[i]->w[it->ids[i]];
+      for (int j = 0; j < 3; j++)
+        yb[3 * i + j] = ((it->contrib >> i) & 1) ? gx[3 * it->ids[i] + j] * relax / count[it->ids[i]]
+                                                 : 0.0;
+    }
+    if (it->kind == 0) {
+      vfr_ctx ctx = {it->side, h, {w[0], w[1], w[2], w[3]}};
+      elem_vjp(vfr_elem, &ctx, x, nin, yb, 12, xb);
+    } else {
+      eer_ctx ctx = {it->pn_x0, it->pn_sign, {it->pn[0], it->pn[1], it->pn[2]}, h,
+                     {w[0], w[1], w[2], w[3]}};
+      elem_vjp(eer_elem, &ctx, x, nin, yb, 12, xb);
+    }
+    for (int i = 0; i < 4; i++) {
+      if (ms[i] != cloth) continue;  /* obstacle nodes are constants */
+      for (int j = 0; j < 3; j++) {
+        add[3 * it->ids[i] + j] += xb[3 * i + j];
+        if (it->kind == 1) add0[3 * it->ids[i] + j] += xb[12 + 3 * i + j];
+      }
+    }
+  }
+  for (int k = 0; k < 3 * nv; k++) {
+    gx[k] += add[k];
+    gx0[k] += add0[k];
+  }
+  free(add);
+  free(add0);
 }

@@ -408,17 +408,21 @@ static int in_edge(double w, const rmesh* m0, int e0, const rmesh* m1, int e1) {
 
 /* accumulate (AccumulateRepulsions, PBDRepulsion.cu:264-277): skip tiny corrections and
  * non-free nodes (pinned cloth nodes and every obstacle node), so only cloth nodes move */
-static void accumulate(double* delta, int32_t* count, const rmesh* const* ms, const int* ids, const v3* corr) {
+static int accumulate(double* delta, int32_t* count, const rmesh* const* ms, const int* ids, const v3* corr) {
+  int m = 0; /* corners accumulated */
   for (int i = 0; i < 4; i++) {
     if (ln(corr[i]) < EPS) continue;
     if (!ms[i]->free_[ids[i]]) continue;
     std3(delta, ids[i], ad(ldd(delta, ids[i]), corr[i]));
     count[ids[i]]++;
+    m |= 1 << i;
   }
+  return m;
 }
 
 /* EvaluateVFProximity (PBDRepulsion.cu:329-409) */
-static void eval_vf(const rmesh* c, int node, const rmesh* o, int face, double h, double* delta, int32_t* count) {
+static void eval_vf(const rmesh* c, int node, const rmesh* o, int face, double h, double* delta,
+                    int32_t* count, orc_rep_item* rec) {
   const int32_t* t = o->tris + 3 * face;
   if (c == o && (node == t[0] || node == t[1] || node == t[2])) return;
   double w[4];
@@ -453,11 +457,20 @@ static void eval_vf(const rmesh* c, int node, const rmesh* o, int face, double h
   v3 corr[4] = {sc(dcq, -wq * s), sc(dcp1, -w1 * s), sc(dcp2, -w2 * s), sc(dcp3, -w3 * s)};
   const rmesh* ms[4] = {c, o, o, o};
   int ids[4] = {node, t[0], t[1], t[2]};
-  accumulate(delta, count, ms, ids, corr);
+  int m = accumulate(delta, count, ms, ids, corr);
+  if (rec && m) {
+    memset(rec, 0, sizeof(*rec));
+    rec->kind = 0;
+    rec->other_cloth = c == o;
+    rec->contrib = m;
+    for (int i = 0; i < 4; i++) rec->ids[i] = ids[i];
+    rec->side = side;
+  }
 }
 
 /* EvaluateEEProximity (PBDRepulsion.cu:411-468) */
-static void eval_ee(const rmesh* c, int e0, const rmesh* o, int e1, double h, double* delta, int32_t* count) {
+static void eval_ee(const rmesh* c, int e0, const rmesh* o, int e1, double h, double* delta,
+                    int32_t* count, orc_rep_item* rec) {
   int a0 = c->edges[2 * e0], a1 = c->edges[2 * e0 + 1], b0 = o->edges[2 * e1], b1 = o->edges[2 * e1 + 1];
   if (c == o && (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1)) return;
   double w[4];
@@ -489,31 +502,64 @@ static void eval_ee(const rmesh* c, int e0, const rmesh* o, int e1, double h, do
   for (int i = 0; i < 4; i++) corr[i] = sc(g[i], -s * wq[i]);
   const rmesh* ms[4] = {c, c, o, o};
   int ids[4] = {a0, a1, b0, b1};
-  accumulate(delta, count, ms, ids, corr);
+  int m = accumulate(delta, count, ms, ids, corr);
+  if (rec && m) {
+    memset(rec, 0, sizeof(*rec));
+    rec->kind = 1;
+    rec->other_cloth = c == o;
+    rec->contrib = m;
+    for (int i = 0; i < 4; i++) rec->ids[i] = ids[i];
+    rec->pn_x0 = e1_free;
+    rec->pn_sign = d < 0.0 ? -1.0 : 1.0;
+    rec->pn[0] = pn.x; rec->pn[1] = pn.y; rec->pn[2] = pn.z;
+  }
 }
 
 /* EvaluateRep_Kernel (PBDRepulsion.cu:470-501) over pairs (cloth face, other face);
  * kind[p] = 1: other face is a cloth face, 0: an obstacle face */
-void orc_repulsion(const orc_rmesh_in* cl_in, const orc_rmesh_in* ob_in, const int32_t* pairs,
-                   const int32_t* kind, long np, double h, double* delta, int32_t* count) {
+static long repulsion(const orc_rmesh_in* cl_in, const orc_rmesh_in* ob_in, const int32_t* pairs,
+                      const int32_t* kind, long np, double h, double* delta, int32_t* count,
+                      orc_rep_item* items, long cap) {
   rmesh c = {cl_in->x, cl_in->x0, cl_in->w, cl_in->free_, cl_in->tris, cl_in->edges, cl_in->fe,
              cl_in->ef_ptr, cl_in->ef, cl_in->face_n, cl_in->rtri};
   rmesh o = {0};
   if (ob_in)
     o = (rmesh){ob_in->x, ob_in->x0, ob_in->w, ob_in->free_, ob_in->tris, ob_in->edges, ob_in->fe,
                 ob_in->ef_ptr, ob_in->ef, ob_in->face_n, ob_in->rtri};
+  long n = 0;
+  orc_rep_item scratch;
   for (long p = 0; p < np; p++) {
     int f0 = pairs[2 * p], f1 = pairs[2 * p + 1];
     const rmesh* other = kind[p] ? &c : &o;
     uint32_t r0 = c.rtri[f0], r1 = other->rtri[f1];
-    for (int i = 0; i < 3; i++)
-      if ((r0 >> i) & 1u) eval_vf(&c, c.tris[3 * f0 + i], other, f1, h, delta, count);
+    for (int i = 0; i < 3; i++) {
+      if (!((r0 >> i) & 1u)) continue;
+      orc_rep_item* rec = items ? (n < cap ? items + n : &scratch) : NULL;
+      if (rec) rec->contrib = 0;
+      eval_vf(&c, c.tris[3 * f0 + i], other, f1, h, delta, count, rec);
+      if (rec && rec->contrib) n++;
+    }
     for (int i = 0; i < 3; i++) {
       if (!((r0 >> (3 + i)) & 1u)) continue;
       for (int j = 0; j < 3; j++) {
         if (!((r1 >> (3 + j)) & 1u)) continue;
-        eval_ee(&c, c.fe[3 * f0 + i], other, other->fe[3 * f1 + j], h, delta, count);
+        orc_rep_item* rec = items ? (n < cap ? items + n : &scratch) : NULL;
+        if (rec) rec->contrib = 0;
+        eval_ee(&c, c.fe[3 * f0 + i], other, other->fe[3 * f1 + j], h, delta, count, rec);
+        if (rec && rec->contrib) n++;
       }
     }
   }
+  return n;
+}
+
+void orc_repulsion(const orc_rmesh_in* cl_in, const orc_rmesh_in* ob_in, const int32_t* pairs,
+                   const int32_t* kind, long np, double h, double* delta, int32_t* count) {
+  repulsion(cl_in, ob_in, pairs, kind, np, h, delta, count, NULL, 0);
+}
+
+long orc_repulsion_record(const orc_rmesh_in* cl_in, const orc_rmesh_in* ob_in,
+                          const int32_t* pairs, const int32_t* kind, long np, double h,
+                          double* delta, int32_t* count, orc_rep_item* items, long cap) {
+  return repulsion(cl_in, ob_in, pairs, kind, np, h, delta, count, items, cap);
 }

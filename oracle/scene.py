@@ -44,6 +44,13 @@ class RMeshIn(C.Structure):
                 ("fe", _I), ("ef_ptr", _I), ("ef", _I), ("face_n", _D), ("rtri", _U)]
 
 
+class RepItem(C.Structure):
+    """One recorded repulsion response (orc_rep_item, xrt_oracle.h)."""
+    _fields_ = [("kind", C.c_int), ("other_cloth", C.c_int), ("contrib", C.c_int),
+                ("pn_x0", C.c_int), ("ids", C.c_int * 4), ("side", C.c_double),
+                ("pn_sign", C.c_double), ("pn", C.c_double * 3)]
+
+
 _bound = False
 
 
@@ -85,6 +92,10 @@ def _lib():
         lib.orc_repulsion.argtypes = [C.POINTER(RMeshIn), C.POINTER(RMeshIn), _I, _I, C.c_long,
                                       C.c_double, _D, _I]
         lib.orc_repulsion.restype = None
+        lib.orc_repulsion_record.argtypes = [C.POINTER(RMeshIn), C.POINTER(RMeshIn), _I, _I,
+                                             C.c_long, C.c_double, _D, _I, C.POINTER(RepItem),
+                                             C.c_long]
+        lib.orc_repulsion_record.restype = C.c_long
         _bound = True
     return lib
 
@@ -277,12 +288,23 @@ class ContactScene:
                 self.pairs = np.ascontiguousarray(np.concatenate([po, ps]), np.int32)
                 self.kind = np.ascontiguousarray(np.r_[np.zeros(len(po)), np.ones(len(ps))], np.int32)
             if self_c:
-                self._rec("repulsion")
                 cm = self._rmesh(False)
                 om = self._rmesh(True) if self.has_obs else None
-                lib.orc_repulsion(C.byref(cm), C.byref(om) if om is not None else None,
-                                  _i(self.pairs.reshape(-1)), _i(self.kind), len(self.pairs),
-                                  xp.pbd_thickness, _d(self.delta), _i(self.count))
+                args = (C.byref(cm), C.byref(om) if om is not None else None,
+                        _i(self.pairs.reshape(-1)), _i(self.kind), len(self.pairs),
+                        xp.pbd_thickness, _d(self.delta), _i(self.count))
+                if self.tape is not None:
+                    # the responses and counts of this pass, for the reverse (the same
+                    # evaluation as orc_repulsion)
+                    xpre = c.x.copy()
+                    cap = max(64, 12 * len(self.pairs))
+                    items = (RepItem * cap)()
+                    n = lib.orc_repulsion_record(*args, items, cap)
+                    assert n <= cap
+                    self._rec("repulsion", items[:n] if n else [], xpre, c.x0.copy(),
+                              self.count.copy(), xp.pbd_thickness, xp.repulsion_relaxation)
+                else:
+                    lib.orc_repulsion(*args)
                 lib.orc_apply_deltas(_d(c.x), _d(self.delta), _i(self.count), self.nv, xp.repulsion_relaxation)
             if self.fem:
                 for k in range(self.f_ncol):

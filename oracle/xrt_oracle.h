@@ -184,6 +184,24 @@ typedef struct orc_rmesh_in {
 } orc_rmesh_in;
 void orc_repulsion(const orc_rmesh_in* cloth, const orc_rmesh_in* obstacle, const int32_t* pairs,
                    const int32_t* kind, long np, double h, double* delta, int32_t* count);
+/* One repulsion response that moved a node (for the reverse pass).  kind 0 = VF: ids[0]
+ * the cloth node, ids[1..3] the face; kind 1 = EE: ids[0..1] the cloth edge, ids[2..3] the
+ * other edge.  other_cloth: the face / other edge is the cloth's own (self contact).
+ * contrib: bit i set when corner i's correction was accumulated.  VF: side, the orientation
+ * taken from the start-of-substep state.  EE: pn_x0 = 1 when the normal is the
+ * start-of-substep edge-edge normal times pn_sign (free other edge), else pn (the other
+ * edge's fixed normal). */
+typedef struct orc_rep_item {
+  int kind, other_cloth, contrib, pn_x0;
+  int ids[4];
+  double side, pn_sign;
+  double pn[3];
+} orc_rep_item;
+/* orc_repulsion that also records its responses with a contribution in items (at most cap;
+ * returns how many there were, which may exceed cap) */
+long orc_repulsion_record(const orc_rmesh_in* cloth, const orc_rmesh_in* obstacle,
+                          const int32_t* pairs, const int32_t* kind, long np, double h,
+                          double* delta, int32_t* count, orc_rep_item* items, long cap);
 
 /* ------------------------------------ adjoint products (FP64, adjoint_oracle.c) */
 /* Each reads the operator's input state (c->x before the operator, c->x0 = start of the
@@ -198,6 +216,13 @@ void orc_adj_bend_color(const orc_cloth* c, const orc_params* p, int k, double d
 /* ground plane with friction; gx0 += adjoint through the friction's x0 */
 void orc_adj_lra(const orc_cloth* c, const orc_params* p, double* gx);
 void orc_adj_ground(const orc_cloth* c, const orc_params* p, double* gx, double* gx0);
+/* PBD repulsion + orc_apply_deltas(relax), from the forward's recorded responses and
+ * counts (orc_repulsion_record) on the pre-repulsion state (cloth->x, cloth->x0): gx (cloth
+ * nodes) becomes the adjoint before the repulsion; gx0 += through the start-of-substep
+ * edge-edge normals.  Obstacle nodes are constants. */
+void orc_adj_repulsion(const orc_rep_item* items, long n, const orc_rmesh_in* cloth,
+                       const orc_rmesh_in* obstacle, const int32_t* count, int nv, double h,
+                       double relax, double* gx, double* gx0);
 /* VF-DCD / VF-CCD solve with the records (face, q) held fixed */
 void orc_adj_vf(const float* obs_pred, const int32_t* obs_tris, const int32_t* face, const double* q,
                 const double* x_pre, int nv, double offset, double* gx);

@@ -236,7 +236,7 @@ def test_torch_function_rollout_loss():
 
 
 def test_unsupported_scenes_are_refused():
-    xp = make_params(bend_model=1, enable_self_contact=1)
+    xp = make_params(bend_model=1, enable_particle_collision=1)
     c = Case(xp, n=6)
     with pytest.raises(XrtError):
         c.s.AdjointBegin(1, 1)
@@ -309,3 +309,96 @@ def test_batch_of_rollouts_matches_single_rollouts():
         alone = run([k])[k]
         for a, b in zip(together[k], alone):
             np.testing.assert_array_equal(a, b)
+
+
+class TwoLayers:
+    """Two 8 x 8 sheets 1.5 mm apart (inside the 2 mm repulsion thickness), the upper one
+    turned and wrinkled and moving down, the lower one pinned at two corners: vertex-face
+    and edge-edge repulsion responses (self contact) in every substep."""
+
+    def __init__(self, xp, n=8, seed=5):
+        from contact_helpers import two_layers
+        self.xp = xp
+        self.spec = two_layers(n=n, seed=seed)
+        nv = len(self.spec.pos)
+        v0 = np.zeros((nv, 3))
+        v0[nv // 2:, 1] = -0.05
+        v0 += 0.01 * scenes.Rng(seed + 1).normal(3 * nv).reshape(-1, 3)
+        self.v0 = v0.astype(np.float32)
+        s = ClothSolver(params=xp)
+        self.i = s.AddCloth(self.spec.pos, self.spec.tris, 1e-7, 1e-3)
+        s.Pin(self.i, np.array([0, n - 1], np.int32))
+        s.Start(seed)
+        self.s = s
+        self.ne = s.counts(self.i).n_edges
+        rng = scenes.Rng(seed + 2)
+        self.sc = (10.0 ** (-7 + 2 * rng.uniform(self.ne))).astype(np.float32)
+        self.a = rng.normal(3 * nv).reshape(-1, 3).astype(np.float32)
+        self.b = (0.01 * rng.normal(3 * nv)).reshape(-1, 3).astype(np.float32)
+
+    def gpu_grad(self, frames, ckpt, x0=None):
+        s, i = self.s, self.i
+        s.SetStretchCompliance(i, self.sc)
+        s.SetState(i, self.spec.x0 if x0 is None else x0, self.v0)
+        s.AdjointBegin(frames, ckpt)
+        s.AdjointSeed(i, self.a, self.b)
+        s.AdjointBackward()
+        gx0, gv0 = s.GradState(i)
+        return dict(gs=s.GradCompliance(i)[0], gx0=gx0, gv0=gv0)
+
+    def oracle_grad(self, frames):
+        cl = oracle_of(self.s, self.i, self.sc.astype(np.float64), 1e-3, x0=self.spec.x0,
+                       v=self.v0)
+        edges = self.s.edges(self.i, colors=False)[0]
+        op, _ = orc.params_from(self.xp)
+        sc = osc.ContactScene(cl, self.spec.pos, self.spec.tris, edges, op, self.xp)
+        tape = oadj.record(sc, frames * self.xp.num_substeps)
+        g = oadj.backward(sc, tape, self.a, self.b)
+        n_resp = sum(len(e[1]) for e in tape if e[0] == "repulsion")
+        kinds = {it.kind for e in tape if e[0] == "repulsion" for it in e[1]}
+        return dict(gs=g["gs_edge"], gx0=g["gx0"], gv0=g["gv0"]), n_resp, kinds
+
+
+def test_self_contact_reverse_matches_fp64_reverse():
+    """PBD repulsion reversed on the GPU from the taped forward's recorded responses (their
+    Jacobians by dual numbers, Jacobi counts replayed, sums in fixed point): a two-layer
+    cloth's gradients against the FP64 reverse of the oracle twin, within max(1e-3, 4 x the
+    GPU gradient's own move under a 1e-7 start perturbation)."""
+    print("case adjoint with self contact: two layers 2 x 8 x 8, 3 x 4", flush=True)
+    xp = make_params(bend_model=1, num_substeps=3, num_iterations=4, enable_ground=0,
+                     enable_self_contact=1, pair_update_interval=2)
+    c = TwoLayers(xp)
+    g = c.gpu_grad(1, 1)
+    ref, n_resp, kinds = c.oracle_grad(1)
+    print(f"  {n_resp} repulsion responses in the oracle's frame, kinds {sorted(kinds)}",
+          flush=True)
+    assert n_resp > 0 and kinds == {0, 1}
+    spread = {k: 0.0 for k in g}
+    for t in (1, 2):
+        rng = np.random.default_rng(t)
+        x0 = (c.spec.x0 * (1.0 + 1e-7 * rng.standard_normal(c.spec.x0.shape))).astype(np.float32)
+        gm = c.gpu_grad(1, 1, x0=x0)
+        for k in spread:
+            spread[k] = max(spread[k], _rel(gm[k], g[k]))
+    for k in ("gs", "gx0", "gv0"):
+        e = _rel(g[k], ref[k])
+        bar = max(TOL, 4.0 * spread[k])
+        print(f"  {k}: rel err {e:.3e}; GPU FP32 spread {spread[k]:.3e}; bar {bar:.3e}",
+              flush=True)
+        assert e <= bar
+
+
+def test_self_contact_reverse_is_reproducible_and_schedule_independent():
+    """The repulsion reverse sums in fixed point, so two backward passes and two checkpoint
+    schedules (one 2-frame segment, two 1-frame segments) give the same bits."""
+    print("case adjoint with self contact: bitwise reproducibility", flush=True)
+    xp = make_params(bend_model=1, num_substeps=3, num_iterations=4, enable_ground=0,
+                     enable_self_contact=1, pair_update_interval=2)
+    c = TwoLayers(xp)
+    g1 = c.gpu_grad(2, 2)
+    g2 = c.gpu_grad(2, 2)
+    g3 = c.gpu_grad(2, 1)
+    for k in g1:
+        assert np.isfinite(g1[k]).all() and np.abs(g1[k]).max() > 0
+        np.testing.assert_array_equal(g1[k], g2[k])
+        np.testing.assert_array_equal(g1[k], g3[k])

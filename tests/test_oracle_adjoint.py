@@ -264,14 +264,54 @@ def test_body_contact_rollout_reverse_runs():
         assert np.all(np.isfinite(g[k])), k
 
 
-def test_repulsion_has_no_reverse():
-    xp = make_params(bend_model=1, num_substeps=2, num_iterations=2, enable_ground=0,
-                     enable_self_contact=1)
-    r = Rollout(_grid_case(n=5), xp)
+def _two_layers_case(n=8, seed=5):
+    """Two sheets 1.5 mm apart (inside the 2 mm repulsion thickness), the upper one turned
+    by 0.45 rad and wrinkled, the lower one pinned at two corners, the upper one moving
+    down: vertex-face and edge-edge repulsion responses in every substep."""
+    from contact_helpers import two_layers
+
+    spec = two_layers(n=n, seed=seed)
+    nv = len(spec.pos)
+    v0 = np.zeros((nv, 3))
+    v0[nv // 2:, 1] = -0.05
+    v0 += 0.01 * scenes.Rng(seed + 1).normal(3 * nv).reshape(-1, 3)
+    return spec.pos, spec.tris, spec.x0, v0, np.array([0, n - 1], np.int32), None
+
+
+def test_repulsion_reverse_matches_fd():
+    """PBD repulsion (vertex-face and edge-edge proximity responses, self contact, Jacobi
+    apply): the reverse replays the pass's recorded responses and counts; gradients w.r.t.
+    compliances, v0 and x0 against central differences of whole rollouts."""
+    xp = make_params(bend_model=1, num_substeps=3, num_iterations=4, enable_ground=0,
+                     enable_self_contact=1, pair_update_interval=2)
+    r = Rollout(_two_layers_case(), xp)
+    g, tape = _check(r, n_sub=3, tol=1e-5)
+    reps = [e for e in tape if e[0] == "repulsion"]
+    kinds = {it.kind for e in reps for it in e[1]}
+    print(f"  {sum(len(e[1]) for e in reps)} recorded responses over {len(reps)} passes, "
+          f"kinds {sorted(kinds)}", flush=True)
+    assert kinds == {0, 1}   # both vertex-face and edge-edge responses were reversed
+
+
+def test_repulsion_reverse_planted_mistake_fails():
+    """The reverse pin is sensitive: with the responses' contributions scaled by 1.01 (a
+    wrong relaxation or count), the x0 and v0 gradients leave the 1e-5 bar."""
+    xp = make_params(bend_model=1, num_substeps=3, num_iterations=4, enable_ground=0,
+                     enable_self_contact=1, pair_update_interval=2)
+    r = Rollout(_two_layers_case(), xp)
     s = r.scene()
-    tape = oadj.record(s, 1)
-    with pytest.raises(ValueError):
-        oadj.backward(s, tape, np.zeros((25, 3)), np.zeros((25, 3)))
+    tape = oadj.record(s, 3)
+    bent = [(e[0], e[1], e[2], e[3], e[4], e[5], e[6] * 1.01) if e[0] == "repulsion" else e
+            for e in tape]
+    nv = len(r.pos)
+    rng = scenes.Rng(11)
+    a = rng.normal(3 * nv).reshape(-1, 3)
+    b = 0.01 * rng.normal(3 * nv).reshape(-1, 3)
+    g_ok = oadj.backward(s, tape, a, b)
+    g_bad = oadj.backward(r.scene(), bent, a, b)
+    rel = float(np.abs(g_bad["gv0"] - g_ok["gv0"]).max() / np.abs(g_ok["gv0"]).max())
+    print(f"  relaxation x 1.01 in the reverse moves gv0 by {rel:.2e}", flush=True)
+    assert rel > 1e-4
 
 
 def test_fp32_forward_moves_the_gradient_beyond_fp64_spread():

@@ -89,8 +89,12 @@ hipError_t LaunchWideBendAdj(hipStream_t s, const WideArgs& a, int n_inst, int c
 hipError_t LaunchVfAdj(hipStream_t s, const float4* x_pc, int n, const int4* obs_tris,
                        const float4* obs_pred, const int32_t* rt_face, const float4* rt_q,
                        const int32_t* vf_face, const float4* vf_q, double4* gx);
-// cnt (int[nv]) and gadd (double[3 nv]) are zero on entry and on return
+// cnt (int[nv]) is zero on entry and on return; gl (double[ne]) is scratch; ve_ptr / ve_edge:
+// the edges of each vertex (CSR, ascending edge id), over which each vertex gathers its
+// adjoint terms in a fixed order (no floating-point atomics: the reverse is bitwise
+// reproducible)
 hipError_t LaunchEeAdj(hipStream_t s, const float4* x_pe, const int2* edges, int ne, int nv,
-                       const EeRec* rec, float relax, double4* gx, int32_t* cnt, double* gadd);
+                       const EeRec* rec, float relax, double4* gx, int32_t* cnt, double* gl,
+                       const int32_t* ve_ptr, const int32_t* ve_edge);
 
 }  // namespace xrt

@@ -45,8 +45,6 @@ bool ObsContact(const xrt_ctx* c, size_t i) {
 // 60 % of the free device memory, else one
 int AdjSetup(xrt_ctx* c, int want_slots = 1) {
   XRT_REQUIRE(c->started, XRT_ERR_STATE, "xrt_start has not been called");
-  XRT_REQUIRE(!c->params.enable_self_contact, XRT_ERR_UNSUPPORTED,
-              "the adjoint does not support self contact (PBD repulsion)");
   XRT_REQUIRE(!c->params.enable_particle_collision, XRT_ERR_UNSUPPORTED,
               "the adjoint does not support particle self-collision");
   XRT_REQUIRE(!BindingsOn(c), XRT_ERR_UNSUPPORTED, "the adjoint does not support bindings");
@@ -108,7 +106,9 @@ int AdjSetup(xrt_ctx* c, int want_slots = 1) {
   A.off_g = A.off_l + (A.lra ? N : 0);
   A.off_c = A.off_g + (ground ? N : 0);
   A.off_e = A.off_c + (A.contacts ? N : 0);
-  A.it_stride = A.off_e + (A.contacts ? N : 0);
+  A.rep = p.enable_self_contact != 0;
+  A.off_r = A.off_e + (A.contacts ? N : 0);
+  A.it_stride = A.off_r + (A.rep ? N : 0);
   A.sub_stride = N + static_cast<size_t>(p.num_iterations) * A.it_stride;
   const size_t frame_n = static_cast<size_t>(p.num_substeps) * A.sub_stride;
   A.frequency = std::max(1, p.num_substeps * p.num_iterations / p.num_collision_passes);
@@ -121,7 +121,15 @@ int AdjSetup(xrt_ctx* c, int want_slots = 1) {
                  (static_cast<size_t>(h.nv) * 40 + 16 * static_cast<size_t>(c->contact[i]->obs_nv) +
                   sizeof(EeRec) * std::max(h.ne(), 1));
   }
-  const size_t frame_bytes = frame_n * sizeof(float4) + rec_bytes;
+  // repulsion responses of one frame: at most 8 per node and pass (at least 2^20, at most
+  // 2^27 entries of 16 B); more is reported as XRT_ERR_CAPACITY by the reverse pass
+  A.rep_cap = 0;
+  if (A.rep)
+    A.rep_cap = std::min<unsigned long long>(
+        1ull << 27, std::max<unsigned long long>(
+                        1ull << 20, 8ull * N * static_cast<unsigned long long>(p.num_substeps) *
+                                        static_cast<unsigned long long>(std::max(1, p.num_iterations))));
+  const size_t frame_bytes = frame_n * sizeof(float4) + rec_bytes + A.rep_cap * sizeof(int4);
   XRT_REQUIRE(frame_bytes <= kMaxTapeBytes, XRT_ERR_CAPACITY,
               "one frame of adjoint tape exceeds 96 GiB");
   size_t free_b = 0, total_b = 0;
@@ -153,6 +161,20 @@ int AdjSetup(xrt_ctx* c, int want_slots = 1) {
     XRT_CHECK_HIP(hipMemsetAsync(r->rt_q.ptr, 0, 16 * r->rt_q.n, c->stream));
     XRT_CHECK_HIP(hipMemsetAsync(r->pred.ptr, 0, 16 * r->pred.n, c->stream));
     XRT_CHECK_HIP(hipMemsetAsync(r->ee.ptr, 0, r->ee.n, c->stream));
+    {  // vertex -> edge CSR of the contact edges (ascending edge id per vertex)
+      const int ne = h.ne();
+      std::vector<int32_t> ptr(h.nv + 1, 0), adj(2 * static_cast<size_t>(ne));
+      for (int e = 0; e < 2 * ne; ++e) ptr[h.edges[e] + 1]++;
+      for (int v = 0; v < h.nv; ++v) ptr[v + 1] += ptr[v];
+      std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+      for (int e = 0; e < ne; ++e) {
+        adj[fill[h.edges[2 * e]]++] = e;
+        adj[fill[h.edges[2 * e + 1]]++] = e;
+      }
+      XRT_CHECK_HIP(r->ve_ptr.Upload(ptr, c->stream));
+      XRT_CHECK_HIP(r->ve_edge.Upload(adj, c->stream));
+      XRT_CHECK_HIP(r->ee_gl.Alloc(std::max(ne, 1)));
+    }
     A.rec[i] = std::move(r);
   }
   if (A.gx0.n != N) {
@@ -160,9 +182,7 @@ int AdjSetup(xrt_ctx* c, int want_slots = 1) {
     XRT_CHECK_HIP(A.gv.Alloc(N));
     XRT_CHECK_HIP(A.gx.Alloc(N));
     XRT_CHECK_HIP(A.ee_cnt.Alloc(N));
-    XRT_CHECK_HIP(A.ee_gadd.Alloc(3 * N));
     XRT_CHECK_HIP(hipMemsetAsync(A.ee_cnt.ptr, 0, sizeof(int32_t) * N, c->stream));
-    XRT_CHECK_HIP(hipMemsetAsync(A.ee_gadd.ptr, 0, sizeof(double) * 3 * N, c->stream));
   }
   if (A.glam.n != static_cast<size_t>(std::max(A.e_total, 1))) {
     XRT_CHECK_HIP(A.glam.Alloc(std::max(A.e_total, 1)));
@@ -170,6 +190,22 @@ int AdjSetup(xrt_ctx* c, int want_slots = 1) {
   }
   if (A.gbcomp.n != static_cast<size_t>(std::max(A.b_total, 1)))
     XRT_CHECK_HIP(A.gbcomp.Alloc(std::max(A.b_total, 1)));
+  if (A.rep) {
+    const size_t marks = 2 * static_cast<size_t>(A.slots) * p.num_substeps * std::max(1, p.num_iterations);
+    if (A.rep_pool.n != A.slots * A.rep_cap) XRT_CHECK_HIP(A.rep_pool.Alloc(A.slots * A.rep_cap));
+    if (A.rep_count.n != static_cast<size_t>(A.slots)) XRT_CHECK_HIP(A.rep_count.Alloc(A.slots));
+    if (A.rep_marks.n != marks) XRT_CHECK_HIP(A.rep_marks.Alloc(marks));
+    if (A.rep_flags.n != 1) XRT_CHECK_HIP(A.rep_flags.Alloc(1));
+    if (A.rep_cnt.n != N) {
+      XRT_CHECK_HIP(A.rep_cnt.Alloc(N));
+      XRT_CHECK_HIP(A.rep_add.Alloc(6 * N));
+    }
+    XRT_CHECK_HIP(hipMemsetAsync(A.rep_marks.ptr, 0, sizeof(unsigned long long) * marks, c->stream));
+    XRT_CHECK_HIP(hipMemsetAsync(A.rep_flags.ptr, 0, sizeof(int32_t), c->stream));
+    XRT_CHECK_HIP(hipMemsetAsync(A.rep_cnt.ptr, 0, sizeof(int32_t) * N, c->stream));
+    XRT_CHECK_HIP(hipMemsetAsync(A.rep_add.ptr, 0, sizeof(unsigned long long) * 6 * N, c->stream));
+    XRT_CHECK_HIP(hipStreamSynchronize(c->stream));
+  }
   return XRT_OK;
 }
 
@@ -237,13 +273,25 @@ int WideSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, AdjointData
   int rc;
   if ((rc = MultiContactSubstepBegin(c)) != XRT_OK) return rc;
   if (T) XRT_CHECK_HIP(LaunchCopy4(s, T, c->x0.ptr, N));
+  if (T && A->rep && sub == 0)  // a frame slot's response pool starts empty
+    XRT_CHECK_HIP(hipMemsetAsync(A->rep_count.ptr + A->slot, 0, sizeof(unsigned long long), s));
   XRT_CHECK_HIP(LaunchPredict(s, c->x0.ptr, c->v.ptr, c->x.ptr, N, k));
   if ((rc = ParticleSubstepBegin(c, -1)) != XRT_OK) return rc;
   for (int it = 0; it < k.iterations; ++it) {
     float4* TI = T ? T + N + static_cast<size_t>(it) * A->it_stride : nullptr;
     if ((rc = ParticleIter(c, -1)) != XRT_OK) return rc;
     bool ran = false;
-    if ((rc = MultiContactIterPre(c, it, &ran)) != XRT_OK) return rc;
+    RepTapeHook hook;
+    if (TI && A->rep) {
+      hook.x_out = TI + A->off_r;
+      hook.pool = A->rep_pool.ptr + static_cast<size_t>(A->slot) * A->rep_cap;
+      hook.count = A->rep_count.ptr + A->slot;
+      hook.cap = A->rep_cap;
+      hook.flags = A->rep_flags.ptr;
+      hook.marks = A->rep_marks.ptr +
+                   2 * ((static_cast<size_t>(A->slot) * c->params.num_substeps + sub) * k.iterations + it);
+    }
+    if ((rc = MultiContactIterPre(c, it, &ran, TI && A->rep ? &hook : nullptr)) != XRT_OK) return rc;
     if (TI && ran)
       for (int i = 0; i < n_inst; ++i)
         if (ObsContact(c, i) && (rc = SnapshotRecords(c, i, c->contact[i]->pass_cnt - 1)) != XRT_OK)
@@ -324,7 +372,7 @@ int ReverseSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, int sub)
                           static_cast<size_t>(pass) * std::max(h.ne(), 1);
         XRT_CHECK_HIP(LaunchEeAdj(s, TI + A.off_e + vb, cd.c_edges.ptr, h.ne(), nv, ee,
                                   c->params.relaxation_factor, A.gx.ptr + vb, A.ee_cnt.ptr + vb,
-                                  A.ee_gadd.ptr + 3 * static_cast<size_t>(vb)));
+                                  r.ee_gl.ptr, r.ve_ptr.ptr, r.ve_edge.ptr));
         XRT_CHECK_HIP(LaunchVfAdj(s, TI + A.off_c + vb, nv, cd.o_tris.ptr,
                                   r.pred.ptr + static_cast<size_t>(pass) * cd.obs_nv,
                                   r.rt_face.ptr + pv, r.rt_q.ptr + pv, r.vf_face.ptr + pv,
@@ -346,6 +394,14 @@ int ReverseSubstep(xrt_ctx* c, const StepConsts& k, const BendMode& bm, int sub)
     for (int col = static_cast<int>(A.s_max.size()) - 1; col >= 0; --col)
       XRT_CHECK_HIP(LaunchWideStretchAdj(s, wa, n_inst, col, A.s_max[col], k.inv_dt2, it,
                                          TI + A.off_s, A.gx.ptr, A.glam.ptr, A.gcomp.ptr));
+    if (A.rep) {  // the iteration's repulsion pass came first
+      const size_t mi = (static_cast<size_t>(A.slot) * c->params.num_substeps + sub) * k.iterations + it;
+      XRT_CHECK_HIP(RepulsionAdj(s, c->ic.ptr, N, A.rep_pool.ptr + static_cast<size_t>(A.slot) * A.rep_cap,
+                                 A.rep_marks.ptr + 2 * mi, A.rep_cap, TI + A.off_r, T,
+                                 c->params.pbd_thickness, c->params.repulsion_relaxation,
+                                 A.rep_cnt.ptr, A.gx.ptr, A.gx0.ptr, A.rep_add.ptr,
+                                 A.rep_add.ptr + 3 * static_cast<size_t>(N)));
+    }
   }
   XRT_CHECK_HIP(LaunchPredictAdj(s, A.gx0.ptr, A.gv.ptr, A.gx.ptr, N, k.dt));
   return XRT_OK;
@@ -430,16 +486,34 @@ int xrt_adjoint_seed(xrt_ctx* ctx, int32_t inst, const float* grad_x, const floa
   return XRT_OK;
 }
 
+// after a taped forward: the repulsion responses fitted their pool and every instance's
+// repulsion ran from its item lists (the per-pair fallback has no replay)
+int CheckRepTape(xrt_ctx* ctx) {
+  AdjointData& A = *ctx->adj;
+  if (!A.rep) return XRT_OK;
+  int32_t f = 0;
+  XRT_CHECK_HIP(hipMemcpyAsync(&f, A.rep_flags.ptr, sizeof(f), hipMemcpyDeviceToHost, ctx->stream));
+  XRT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  XRT_REQUIRE(!(f & 1), XRT_ERR_CAPACITY,
+              "the repulsion responses of a frame exceeded the adjoint's record pool ("
+              "8 per node and pass, at most 2^27)");
+  XRT_REQUIRE(!(f & 2), XRT_ERR_UNSUPPORTED,
+              "an instance's repulsion items exceeded its item lists (repulsion_vf_items_per_face / "
+              "repulsion_ee_items_per_face, or the pair capacity): the per-pair evaluation has no "
+              "reverse; raise the capacities");
+  return XRT_OK;
+}
+
 int xrt_adjoint_backward(xrt_ctx* ctx) {
   int rc = CheckAdj(ctx, -1);
   if (rc != XRT_OK) return rc;
   AdjointData& A = *ctx->adj;
   XRT_REQUIRE(!A.has_grad, XRT_ERR_STATE,
               "the seeds were consumed by the previous reverse pass: seed again first");
-  XRT_REQUIRE(!ctx->params.enable_particle_collision && !ctx->params.enable_self_contact &&
-                  !FemOn(ctx),
+  XRT_REQUIRE(!ctx->params.enable_particle_collision &&
+                  (ctx->params.enable_self_contact != 0) == A.rep && !FemOn(ctx),
               XRT_ERR_UNSUPPORTED,
-              "particle self-collision / self contact / FEM constraints were enabled after "
+              "particle self-collision / self contact / FEM constraints changed after "
               "xrt_adjoint_begin");
   const StepConsts k = MakeConsts(ctx->params);
   const BendMode bm = ModeOf(k);
@@ -447,6 +521,7 @@ int xrt_adjoint_backward(xrt_ctx* ctx) {
   const int K = A.ckpt_every, n_ckpt = (A.n_frames + K - 1) / K;
   XRT_CHECK_HIP(LaunchZeroD1(ctx->stream, A.gcomp.ptr, static_cast<int>(A.gcomp.n)));
   XRT_CHECK_HIP(LaunchZeroD1(ctx->stream, A.gbcomp.ptr, static_cast<int>(A.gbcomp.n)));
+  if (A.rep) XRT_CHECK_HIP(hipMemsetAsync(A.rep_flags.ptr, 0, sizeof(int32_t), ctx->stream));
   for (int seg = n_ckpt - 1; seg >= 0; --seg) {
     const int f0 = seg * K, f1 = std::min(f0 + K, A.n_frames);
     if ((rc = LoadState(ctx, A.ckpt.ptr + static_cast<size_t>(seg) * 2 * N)) != XRT_OK) return rc;
@@ -457,6 +532,7 @@ int xrt_adjoint_backward(xrt_ctx* ctx) {
         A.slot = f - f0;
         if ((rc = WideFrame(ctx, true)) != XRT_OK) return rc;
       }
+      if ((rc = CheckRepTape(ctx)) != XRT_OK) return rc;
       for (int f = f1 - 1; f >= f0; --f) {
         A.slot = f - f0;
         for (int sub = ctx->params.num_substeps - 1; sub >= 0; --sub)
@@ -474,6 +550,7 @@ int xrt_adjoint_backward(xrt_ctx* ctx) {
     for (int f = f1 - 1; f >= f0; --f) {
       if ((rc = LoadState(ctx, A.fstart.ptr + static_cast<size_t>(f - f0) * 2 * N)) != XRT_OK) return rc;
       if ((rc = WideFrame(ctx, true)) != XRT_OK) return rc;
+      if ((rc = CheckRepTape(ctx)) != XRT_OK) return rc;
       for (int sub = ctx->params.num_substeps - 1; sub >= 0; --sub)
         if ((rc = ReverseSubstep(ctx, k, bm, sub)) != XRT_OK) return rc;
     }

@@ -88,48 +88,61 @@ __global__ __launch_bounds__(kBs) void EeAdjCountKernel(const float4* __restrict
   atomicAdd(cnt + ed.y, 1);
 }
 
+// per edge: the adjoint of its multiplier lam (0 for an inactive edge)
 __global__ __launch_bounds__(kBs) void EeAdjKernel(const float4* __restrict__ x,
                                                    const int2* __restrict__ edges, int ne,
                                                    const EeRec* __restrict__ rec, float relax,
                                                    const double4* __restrict__ gx,
                                                    const int32_t* __restrict__ cnt,
-                                                   double* __restrict__ gadd) {
+                                                   double* __restrict__ gl_out) {
   const int e = blockIdx.x * kBs + threadIdx.x;
   if (e >= ne) return;
   const int2 ed = edges[e];
   const EeRec r = rec[e];
-  if (!EeActive(r, L3(x[ed.x]), L3(x[ed.y]))) return;
+  if (!EeActive(r, L3(x[ed.x]), L3(x[ed.y]))) {
+    gl_out[e] = 0.0;
+    return;
+  }
   const double s = r.a.y;
   const double fx = r.fn.x, fy = r.fn.y, fz = r.fn.z;
   const double4 gi = gx[ed.x], gj = gx[ed.y];
   const double ci = static_cast<double>(relax) / cnt[ed.x];
   const double cj = static_cast<double>(relax) / cnt[ed.y];
-  // adjoint of lam, then of c1 = (1 - s) x_i + s x_j
-  const double gl = -(1 - s) * ci * (fx * gi.x + fy * gi.y + fz * gi.z) -
-                    s * cj * (fx * gj.x + fy * gj.y + fz * gj.z);
-  const double gx_ = fx * gl, gy_ = fy * gl, gz_ = fz * gl;
-  atomicAdd(gadd + 3 * ed.x, (1 - s) * gx_);
-  atomicAdd(gadd + 3 * ed.x + 1, (1 - s) * gy_);
-  atomicAdd(gadd + 3 * ed.x + 2, (1 - s) * gz_);
-  atomicAdd(gadd + 3 * ed.y, s * gx_);
-  atomicAdd(gadd + 3 * ed.y + 1, s * gy_);
-  atomicAdd(gadd + 3 * ed.y + 2, s * gz_);
+  // adjoint of lam (the corrections are -(1 - s) lam fn and -s lam fn, each scaled by
+  // relax / count at its vertex)
+  gl_out[e] = -(1 - s) * ci * (fx * gi.x + fy * gi.y + fz * gi.z) -
+              s * cj * (fx * gj.x + fy * gj.y + fz * gj.z);
 }
 
+// per vertex with an active edge: the adjoint of c1 = (1 - s) x_i + s x_j through lam =
+// fn . c1 - offset, summed over the vertex's edges in ascending edge order
 __global__ __launch_bounds__(kBs) void EeAdjApplyKernel(double4* __restrict__ gx, int n,
                                                         int32_t* __restrict__ cnt,
-                                                        double* __restrict__ gadd) {
+                                                        const double* __restrict__ gl,
+                                                        const int2* __restrict__ edges,
+                                                        const EeRec* __restrict__ rec,
+                                                        const int32_t* __restrict__ ve_ptr,
+                                                        const int32_t* __restrict__ ve_edge) {
   const int i = blockIdx.x * kBs + threadIdx.x;
-  if (i >= n) return;
-  if (cnt[i] > 0) {
-    double4 g = gx[i];
-    g.x += gadd[3 * i];
-    g.y += gadd[3 * i + 1];
-    g.z += gadd[3 * i + 2];
-    gx[i] = g;
-    gadd[3 * i] = gadd[3 * i + 1] = gadd[3 * i + 2] = 0.0;
-    cnt[i] = 0;
+  if (i >= n || cnt[i] <= 0) return;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (int q = ve_ptr[i]; q < ve_ptr[i + 1]; ++q) {
+    const int e = ve_edge[q];
+    const double g = gl[e];
+    if (g == 0.0) continue;
+    const EeRec r = rec[e];
+    const double s = r.a.y;
+    const double w = edges[e].x == i ? 1.0 - s : s;
+    ax += w * r.fn.x * g;
+    ay += w * r.fn.y * g;
+    az += w * r.fn.z * g;
   }
+  double4 v = gx[i];
+  v.x += ax;
+  v.y += ay;
+  v.z += az;
+  gx[i] = v;
+  cnt[i] = 0;
 }
 
 hipError_t LaunchVfAdj(hipStream_t s, const float4* x_pc, int n, const int4* obs_tris,
@@ -144,11 +157,12 @@ hipError_t LaunchVfAdj(hipStream_t s, const float4* x_pc, int n, const int4* obs
 }
 
 hipError_t LaunchEeAdj(hipStream_t s, const float4* x_pe, const int2* edges, int ne, int nv,
-                       const EeRec* rec, float relax, double4* gx, int32_t* cnt, double* gadd) {
+                       const EeRec* rec, float relax, double4* gx, int32_t* cnt, double* gl,
+                       const int32_t* ve_ptr, const int32_t* ve_edge) {
   if (ne <= 0 || nv <= 0) return hipSuccess;
   EeAdjCountKernel<<<Blocks(ne), kBs, 0, s>>>(x_pe, edges, ne, rec, cnt);
-  EeAdjKernel<<<Blocks(ne), kBs, 0, s>>>(x_pe, edges, ne, rec, relax, gx, cnt, gadd);
-  EeAdjApplyKernel<<<Blocks(nv), kBs, 0, s>>>(gx, nv, cnt, gadd);
+  EeAdjKernel<<<Blocks(ne), kBs, 0, s>>>(x_pe, edges, ne, rec, relax, gx, cnt, gl);
+  EeAdjApplyKernel<<<Blocks(nv), kBs, 0, s>>>(gx, nv, cnt, gl, edges, rec, ve_ptr, ve_edge);
   return hipGetLastError();
 }
 

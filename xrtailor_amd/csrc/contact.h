@@ -41,6 +41,20 @@ struct EeRec {
   float4 a, p2, q2, fn;
 };
 
+// One PBD repulsion response as the adjoint replays it: the corners whose corrections were
+// accumulated (mask, bit i = corner i) and their node ids; VF: the orientation taken from
+// the start-of-substep state; EE: the normal used (pn), and whether it is the
+// start-of-substep edge-edge normal times pn_sign (pn_x0, a free other edge) or the other
+// edge's fixed normal.
+struct RepOut {
+  int mask = 0;
+  int ids[4] = {0, 0, 0, 0};
+  float side = 1.0f;
+  int pn_x0 = 0;
+  float pn_sign = 1.0f;
+  float3 pn = make_float3(0.0f, 0.0f, 0.0f);
+};
+
 hipError_t LaunchInterpolate(hipStream_t s, const float4* x0, const float4* x, float4* x1, int n,
                              float alpha);
 hipError_t LaunchVfDcdGenerate(hipStream_t s, const float4* x, int n, const Lbvh& obs,
@@ -155,6 +169,23 @@ hipError_t MultiPairs(hipStream_t s, const InstContactDev* ic, const MultiMaps& 
 hipError_t MultiRepulsion(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
                           const float4* x0, float h, float relax, hipEvent_t ee_begin = nullptr,
                           hipEvent_t ee_end = nullptr);
+// Adjoint tape of a repulsion pass on the state (x, x0), before MultiRepulsion: the items of
+// every instance that produce a response go to pool (instance, is_ee, item) at *count (at
+// most cap); *mark_begin / *mark_end receive the count before and after.  flags |= 1 when
+// the pool overflowed, |= 2 when an instance evaluated its pairs without item lists.
+hipError_t MultiRepRecord(hipStream_t s, const InstContactDev* ic, const MultiMaps& m,
+                          const float4* x, const float4* x0, float h, int4* pool,
+                          unsigned long long* count, unsigned long long cap, int32_t* flags,
+                          unsigned long long* mark_begin, unsigned long long* mark_end);
+// Reverse of a repulsion pass + Jacobi apply from its recorded responses pool[range[0],
+// range[1]) on the pass's input state (x, x0 = start of the substep): gx (every instance's
+// nodes, the adjoint after the apply) becomes the adjoint before the pass, gx0 += through
+// the start-of-substep edge-edge normals.  cnt (int[n_total]) and add_x / add_x0
+// (3 n_total each) are zero on entry and on return.
+hipError_t RepulsionAdj(hipStream_t s, const InstContactDev* ic, int n_total, const int4* pool,
+                        const unsigned long long* range, unsigned long long cap, const float4* x,
+                        const float4* x0, float h, float relax, int32_t* cnt, double4* gx,
+                        double4* gx0, unsigned long long* add_x, unsigned long long* add_x0);
 // VF-CCD (offset 0) then VF-DCD (offset 1e-3) solves
 hipError_t MultiVfSolve(hipStream_t s, const InstContactDev* ic, const MultiMaps& m, float4* x,
                         const float4* x0, float mu, float inv_ns);

@@ -330,23 +330,34 @@ __device__ bool InEdge(float w, const RMesh& m0, int e0, const RMesh& m1, int e1
 }
 
 // AccumulateRepulsions (PBDRepulsion.cu:264-277): nodes [0, kSplit) belong to m0, the
-// rest to m1
+// rest to m1.  With `out` (the adjoint's replay) the accumulated corners go to out->mask
+// and out->ids, and the sums only where dq / cnt are given (a null dq adds counts only).
 template <int kSplit>
 __device__ __forceinline__ void Accumulate(const RMesh& m0, const RMesh& m1, const int* ids,
-                                           const F3* corr, long long* dq, int32_t* cnt) {
+                                           const F3* corr, long long* dq, int32_t* cnt,
+                                           RepOut* out = nullptr) {
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const RMesh& m = i < kSplit ? m0 : m1;
     if (Len(corr[i]) < kEpsC) continue;
     if (!m.Free(ids[i])) continue;
-    AddFixed(dq, cnt, ids[i], corr[i]);
+    if (out) {
+      out->mask |= 1 << i;
+      if (cnt) atomicAdd(cnt + ids[i], 1);
+    } else {
+      AddFixed(dq, cnt, ids[i], corr[i]);
+    }
+  }
+  if (out) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out->ids[i] = ids[i];
   }
 }
 
 // EvaluateVFProximity (PBDRepulsion.cu:329-409) of vertex `node` against the face with
 // vertices tr (the face id itself is not needed)
 __device__ void EvalVFt(const RMesh& c, int node, const RMesh& o, int4 tr, bool self, float h,
-                        long long* dq, int32_t* cnt) {
+                        long long* dq, int32_t* cnt, RepOut* out = nullptr) {
   if (self && (node == tr.x || node == tr.y || node == tr.z)) return;
   // the inside test on the current state rejects most pairs; it is evaluated before the
   // start-of-substep side (both are pure, so the order does not change any result).  It is
@@ -389,7 +400,8 @@ __device__ void EvalVFt(const RMesh& c, int node, const RMesh& o, int4 tr, bool 
   const float s = Cv / den;
   const F3 corr[4] = {M(dcq, -wq * s), M(dcp1, -w1 * s), M(dcp2, -w2 * s), M(dcp3, -w3 * s)};
   const int ids[4] = {node, tr.x, tr.y, tr.z};
-  Accumulate<1>(c, o, ids, corr, dq, cnt);
+  if (out) out->side = side;
+  Accumulate<1>(c, o, ids, corr, dq, cnt, out);
 }
 
 __device__ __forceinline__ void EvalVF(const RMesh& c, int node, const RMesh& o, int face,
@@ -426,7 +438,7 @@ __device__ __forceinline__ bool EEInside(int2 a, int2 b, F3 x0, F3 x1, F3 y0, F3
 // InEdge tests, the start-of-substep side and the constraint, accumulated into dq / cnt
 __device__ void EEResponse(const RMesh& c, int2 a, const RMesh& o, int2 b, F3 x0, F3 x1, F3 y0,
                            F3 y1, int e0, int e1, float h, float4 w4, long long* dq,
-                           int32_t* cnt) {
+                           int32_t* cnt, RepOut* out = nullptr) {
   float bw[4] = {w4.x, w4.y, w4.z, w4.w};
   if (!(InEdge(bw[1], c, e0, o, e1) && InEdge(-bw[3], o, e1, c, e0))) return;
   float w[4];
@@ -449,7 +461,12 @@ __device__ void EEResponse(const RMesh& c, int2 a, const RMesh& o, int2 b, F3 x0
   F3 corr[4];
   for (int i = 0; i < 4; ++i) corr[i] = M(g[i], -s * wq[i]);
   const int ids4[4] = {a.x, a.y, b.x, b.y};
-  Accumulate<2>(c, o, ids4, corr, dq, cnt);
+  if (out) {
+    out->pn_x0 = e1_free ? 1 : 0;
+    out->pn_sign = d0 < 0.0f ? -1.0f : 1.0f;
+    out->pn = make_float3(pn.x, pn.y, pn.z);
+  }
+  Accumulate<2>(c, o, ids4, corr, dq, cnt, out);
 }
 
 __device__ __forceinline__ void EvalEE(const RMesh& c, int e0, const RMesh& o, int e1, bool self,
@@ -706,16 +723,140 @@ __device__ __forceinline__ bool EeItemInside(const RMesh& c, const RMesh& o, int
 // the response of a queued EE item (positions reloaded: the pass does not move them)
 __device__ __forceinline__ void EeItemRespond(const RMesh& c, const RMesh& o, const EeQueued& e,
                                               float h, long long* __restrict__ dq,
-                                              int32_t* __restrict__ cnt) {
+                                              int32_t* __restrict__ cnt, RepOut* out = nullptr) {
   const int e1 = e.e1 & 0x7fffffff;
   const int2 a = c.t.edges[e.e0];
   if (e.e1 < 0) {
     const int2 b = c.t.edges[e1];
-    EEResponse(c, a, c, b, c.X(a.x), c.X(a.y), c.X(b.x), c.X(b.y), e.e0, e1, h, e.bw, dq, cnt);
+    EEResponse(c, a, c, b, c.X(a.x), c.X(a.y), c.X(b.x), c.X(b.y), e.e0, e1, h, e.bw, dq, cnt,
+               out);
   } else {
     const int2 b = o.t.edges[e1];
-    EEResponse(c, a, o, b, c.X(a.x), c.X(a.y), o.X(b.x), o.X(b.y), e.e0, e1, h, e.bw, dq, cnt);
+    EEResponse(c, a, o, b, c.X(a.x), c.X(a.y), o.X(b.x), o.X(b.y), e.e0, e1, h, e.bw, dq, cnt,
+               out);
   }
+}
+
+// ------------------------------------------------ adjoint replay of the PBD repulsion
+// Repulsion item `it` (VF: is_ee = 0, EE: 1) of instance c on the state (x, x0): the forward's
+// tests and response, with the response's corners and decisions in `out` instead of sums;
+// cnt (indexed like x) receives the Jacobi counts when given.
+__device__ __forceinline__ void ReplayRepItem(const InstContactDev& c, const float4* x,
+                                              const float4* x0, int is_ee, int2 it, float h,
+                                              int32_t* cnt, RepOut& out) {
+  const RMesh cm{c.cloth, MeshState{x + c.vbeg, x0 + c.vbeg, -1.0f}};
+  const RMesh om{c.obs, MeshState{c.o_x, c.o_x0, 1e-3f}};  // obstacle nodes: inv mass 1e-3
+  int32_t* cn = cnt ? cnt + c.vbeg : nullptr;
+  if (!is_ee) {
+    const int f = it.y & 0x7fffffff;
+    if (it.x < 0 || it.x >= c.cloth.nv) return;
+    if (it.y < 0) {
+      if (f < c.cloth.nf) EvalVFt(cm, it.x, cm, c.cloth.tris[f], true, h, nullptr, cn, &out);
+    } else if (c.has_obs && f < c.obs.nf) {
+      EvalVFt(cm, it.x, om, c.obs.tris[f], false, h, nullptr, cn, &out);
+    }
+    return;
+  }
+  if (it.y >= 0 && !c.has_obs) return;
+  EeQueued ent;
+  if (EeItemInside(cm, om, it, ent)) EeItemRespond(cm, om, ent, h, nullptr, cn, &out);
+}
+
+// dual numbers (value, one tangent) in FP64 for the responses' Jacobians
+struct Dd {
+  double v, d;
+};
+__device__ __forceinline__ Dd DK(double v) { return Dd{v, 0.0}; }
+__device__ __forceinline__ Dd operator+(Dd a, Dd b) { return Dd{a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dd operator-(Dd a, Dd b) { return Dd{a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dd operator*(Dd a, Dd b) { return Dd{a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dd operator/(Dd a, Dd b) {
+  return Dd{a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)};
+}
+__device__ __forceinline__ Dd DSqrt(Dd a) {
+  const double s = sqrt(a.v);
+  return Dd{s, a.v > 0.0 ? a.d / (2.0 * s) : 0.0};
+}
+struct D3 {
+  Dd x, y, z;
+};
+__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3 operator*(D3 a, Dd s) { return D3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ Dd Dot3(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 Crs3(D3 a, D3 b) {
+  return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ D3 Nrm3(D3 a) { return a * (DK(1.0) / DSqrt(Dot3(a, a))); }
+__device__ __forceinline__ D3 In3(const double* x, int k, int dir) {
+  return D3{Dd{x[k], dir == k ? 1.0 : 0.0}, Dd{x[k + 1], dir == k + 1 ? 1.0 : 0.0},
+            Dd{x[k + 2], dir == k + 2 ? 1.0 : 0.0}};
+}
+
+// VF response (EvalVFt after its tests) on in = node, y0, y1, y2 (12 values), tangent
+// direction dir: the four corrections' tangents dot the output adjoint yb
+__device__ double VfRespDir(const double* in, int dir, const RepOut& r, const double* w, double h,
+                            const double* yb) {
+  const D3 p1 = In3(in, 3, dir);
+  const D3 q = In3(in, 0, dir) - p1, p2 = In3(in, 6, dir) - p1, p3 = In3(in, 9, dir) - p1;
+  D3 n = Crs3(p2, p3) * DK(r.side);
+  const Dd c23 = DSqrt(Dot3(n, n));
+  n = n * (DK(1.0) / c23);
+  const Dd C = Dot3(q, n) - DK(h);
+  const Dd nq = Dot3(n, q);
+  D3 dc[4];
+  dc[0] = n;
+  dc[2] = (Crs3(p3, q) + Crs3(n, p3) * nq) * (DK(1.0) / c23);
+  dc[3] = (Crs3(p2, q) + Crs3(n, p2) * nq) * (DK(-1.0) / c23);
+  dc[1] = dc[0] * DK(-1.0) - dc[2] - dc[3];
+  Dd den = DK(0.0);
+  for (int i = 0; i < 4; ++i) den = den + DK(w[i]) * Dot3(dc[i], dc[i]);
+  const Dd s = C / den;
+  double acc = 0.0;
+  for (int i = 0; i < 4; ++i) {
+    const D3 corr = dc[i] * (DK(-w[i]) * s);
+    acc += corr.x.d * yb[3 * i] + corr.y.d * yb[3 * i + 1] + corr.z.d * yb[3 * i + 2];
+  }
+  return acc;
+}
+
+__device__ __forceinline__ Dd Mixed3(D3 a, D3 b, D3 c) { return Dot3(a, Crs3(b, c)); }
+
+// EE response (EEResponse after its tests) on in = a0, a1, b0, b1 (current) then the same at
+// the start of the substep (24 values)
+__device__ double EeRespDir(const double* in, int dir, const RepOut& r, const double* w, double h,
+                            const double* yb) {
+  const D3 x0 = In3(in, 0, dir), x1 = In3(in, 3, dir), y0 = In3(in, 6, dir), y1 = In3(in, 9, dir);
+  D3 pn;
+  if (r.pn_x0)
+    pn = Crs3(Nrm3(In3(in, 15, dir) - In3(in, 12, dir)), Nrm3(In3(in, 21, dir) - In3(in, 18, dir))) *
+         DK(r.pn_sign);
+  else
+    pn = D3{DK(r.pn.x), DK(r.pn.y), DK(r.pn.z)};
+  const D3 n = Crs3(Nrm3(x1 - x0), Nrm3(y1 - y0));
+  const Dd a0 = Mixed3(y1 - x1, y0 - x1, n), a1 = Mixed3(y0 - x0, y1 - x0, n);
+  const Dd b0 = Mixed3(x0 - y1, x1 - y1, n), b1 = Mixed3(x1 - y0, x0 - y0, n);
+  const Dd wt[4] = {a0 / (a0 + a1), a1 / (a0 + a1), DK(0.0) - b0 / (b0 + b1),
+                    DK(0.0) - b1 / (b0 + b1)};
+  const D3 dd = x0 * wt[0] + x1 * wt[1] + y0 * wt[2] + y1 * wt[3];
+  const Dd C = Dot3(dd, pn) - DK(h);
+  Dd den = DK(0.0);
+  for (int i = 0; i < 4; ++i) den = den + DK(w[i]) * wt[i] * wt[i];
+  const Dd s = C / den;
+  double acc = 0.0;
+  for (int i = 0; i < 4; ++i) {
+    const D3 corr = pn * (wt[i] * (DK(-w[i]) * s));
+    acc += corr.x.d * yb[3 * i] + corr.y.d * yb[3 * i + 1] + corr.z.d * yb[3 * i + 2];
+  }
+  return acc;
+}
+
+constexpr double kAdjFix = 4294967296.0;  // 2^32: adjoint sums in 32.32 fixed point
+
+__device__ __forceinline__ void AddFix3(unsigned long long* p, double x, double y, double z) {
+  atomicAdd(p, static_cast<unsigned long long>(__double2ll_rn(x * kAdjFix)));
+  atomicAdd(p + 1, static_cast<unsigned long long>(__double2ll_rn(y * kAdjFix)));
+  atomicAdd(p + 2, static_cast<unsigned long long>(__double2ll_rn(z * kAdjFix)));
 }
 
 // -------------------------------------------------------- multi-instance kernels
@@ -1009,18 +1150,19 @@ __global__ __launch_bounds__(kBlock) void MultiEmitPairsKernel(const InstContact
     }
     leaf[threadIdx.x] = L;
   }
-  // the wave's leaves with hits, in lane order
-  const uint64_t work = __ballot(cnt > 0);
+  // the wave's staged leaves with hits, in lane order (leaves beyond the stage are
+  // traversed again below, each by its own lane)
+  const bool staged_own = cnt > 0 && c.stage && cnt <= static_cast<uint32_t>(kPairStage);
+  const uint64_t work = __ballot(staged_own);
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  if (cnt > 0) order[w][__popcll(work & below)] = lane;
+  if (staged_own) order[w][__popcll(work & below)] = lane;
   const int nwork = __popcll(work);
   __syncthreads();
   for (int r = 0; 2 * r < nwork; ++r) {
-    const int slot = 2 * r + (lane >> 5);  // this half-wave's leaf: the slot-th with hits
+    const int slot = 2 * r + (lane >> 5);  // this half-wave's leaf: the slot-th staged one
     const bool live = slot < nwork;
     const EmitLeaf& L = leaf[w * 64 + (live ? order[w][slot] : 0)];
-    const bool staged = live && c.stage && L.cnt <= static_cast<uint32_t>(kPairStage);
-    const bool have = staged && static_cast<uint32_t>(hl) < L.cnt;
+    const bool have = live && static_cast<uint32_t>(hl) < L.cnt;
     int2 e = make_int2(0, 0);
     if (have) {
       e = c.stage[static_cast<int64_t>(L.i) * kPairStage + hl];
@@ -1051,9 +1193,12 @@ __global__ __launch_bounds__(kBlock) void MultiEmitPairsKernel(const InstContact
           }
       }
     }
-    // leaf with more hits than the stage holds (or no stage: self contact came on after
-    // start): the half's first lane traverses again, in the count pass's hit order
-    if (live && !staged && hl == 0) {
+  }
+  // a leaf with more hits than the stage holds (or no stage: self contact came on after
+  // start): its own lane traverses again, in the count pass's hit order
+  if (cnt > 0 && !staged_own) {
+    const EmitLeaf& L = leaf[threadIdx.x];
+    {
       int64_t index = L.lo + L.cnt - 1, vo = L.vo, eo = L.eo;
       const int lobj = L.lobj;
       const int4 f0 = L.f0, fe0 = L.fe0;
@@ -1154,6 +1299,142 @@ __global__ __launch_bounds__(kBlock) void MultiRepItemKernel(const InstContactDe
     }
     if (!have && qn == 0) break;
   }
+}
+
+// Adjoint tape of the repulsion: the items of every instance that produced a response on
+// the state (x, x0), appended to pool as (instance, is_ee, item) at *count (cap entries;
+// flags bit 0: the pool overflowed; bit 1: an instance evaluated its pairs without item
+// lists, which the reverse cannot replay)
+template <int is_ee>
+__global__ __launch_bounds__(kBlock) void MultiRepRecordKernel(
+    const InstContactDev* ic, const int32_t* off, int n, const float4* __restrict__ x,
+    const float4* __restrict__ x0, float h, int4* __restrict__ pool,
+    unsigned long long* __restrict__ count, unsigned long long cap, int32_t* __restrict__ flags) {
+  const int lblk = XcdBlock(off, n);
+  if (lblk < 0) return;
+  const int k = InstOfBlock(off, n, lblk);
+  const InstContactDev& c = ic[k];
+  if (!c.scal) return;
+  if (!ItemsUsable(c, is_ee)) {
+    if (threadIdx.x == 0 && lblk == off[k] && c.scal[2] > 0) atomicOr(flags, 2);
+    return;
+  }
+  const int total = c.scal[is_ee ? 4 : 3];
+  const int nblk = off[k + 1] - off[k];
+  const int2* items = is_ee ? c.rep_ee : c.rep_vf;
+  for (int q = (lblk - off[k]) * kBlock + threadIdx.x; q < total; q += nblk * kBlock) {
+    const int2 it = items[q];
+    RepOut out;
+    ReplayRepItem(c, x, x0, is_ee, it, h, nullptr, out);
+    if (!out.mask) continue;
+    const unsigned long long slot = atomicAdd(count, 1ull);
+    if (slot < cap)
+      pool[slot] = make_int4(k, is_ee, it.x, it.y);
+    else
+      atomicOr(flags, 1);
+  }
+}
+
+__global__ void RepMarkKernel(const unsigned long long* __restrict__ count,
+                              unsigned long long* __restrict__ mark) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *mark = *count;
+}
+
+// reverse, step 1: the Jacobi counts of the recorded responses in [range[0], range[1])
+__global__ __launch_bounds__(kBlock) void RepAdjCountKernel(
+    const InstContactDev* ic, const int4* __restrict__ pool,
+    const unsigned long long* __restrict__ range, const float4* __restrict__ x,
+    const float4* __restrict__ x0, float h, unsigned long long cap, int32_t* __restrict__ cnt) {
+  const unsigned long long b = range[0], e = min(range[1], cap);
+  for (unsigned long long q = b + blockIdx.x * kBlock + threadIdx.x; q < e;
+       q += static_cast<unsigned long long>(gridDim.x) * kBlock) {
+    const int4 r = pool[q];
+    RepOut out;
+    ReplayRepItem(ic[r.x], x, x0, r.y, make_int2(r.z, r.w), h, cnt, out);
+  }
+}
+
+// reverse, step 2: each response's corrections reversed (Jacobian by dual numbers, one
+// input direction at a time) with output adjoint relax / count * gx at its accumulated
+// corners; cloth inputs add to add_x, the start-of-substep positions of EE normals to
+// add_x0 (32.32 fixed point, so the sums do not depend on their order)
+__global__ __launch_bounds__(kBlock) void RepAdjVjpKernel(
+    const InstContactDev* ic, const int4* __restrict__ pool,
+    const unsigned long long* __restrict__ range, const float4* __restrict__ x,
+    const float4* __restrict__ x0, float h, float relax, unsigned long long cap,
+    const int32_t* __restrict__ cnt, const double4* __restrict__ gx,
+    unsigned long long* __restrict__ add_x, unsigned long long* __restrict__ add_x0) {
+  const unsigned long long b = range[0], e = min(range[1], cap);
+  for (unsigned long long q = b + blockIdx.x * kBlock + threadIdx.x; q < e;
+       q += static_cast<unsigned long long>(gridDim.x) * kBlock) {
+    const int4 r = pool[q];
+    const InstContactDev& c = ic[r.x];
+    RepOut out;
+    ReplayRepItem(c, x, x0, r.y, make_int2(r.z, r.w), h, nullptr, out);
+    if (!out.mask) continue;
+    const bool self = r.w < 0;
+    // corner i on the cloth: VF node / EE edge a always; the other side when self contact
+    bool cl[4];
+    cl[0] = true;
+    cl[1] = r.y == 1 || self;
+    cl[2] = cl[3] = self;
+    double in[24], w[4], yb[12];
+    for (int i = 0; i < 4; ++i) {
+      const int id = out.ids[i];
+      const float4 p = cl[i] ? x[c.vbeg + id] : c.o_x[id];
+      const float4 p0 = cl[i] ? x0[c.vbeg + id] : c.o_x0[id];
+      in[3 * i] = p.x; in[3 * i + 1] = p.y; in[3 * i + 2] = p.z;
+      in[12 + 3 * i] = p0.x; in[12 + 3 * i + 1] = p0.y; in[12 + 3 * i + 2] = p0.z;
+      w[i] = cl[i] ? static_cast<double>(p0.w) : static_cast<double>(1e-3f);
+      const bool acc = (out.mask >> i) & 1;
+      const double f = acc ? static_cast<double>(relax) / cnt[c.vbeg + id] : 0.0;
+      const double4 g = acc ? gx[c.vbeg + id] : make_double4(0, 0, 0, 0);
+      yb[3 * i] = f * g.x; yb[3 * i + 1] = f * g.y; yb[3 * i + 2] = f * g.z;
+    }
+    const int nin = r.y ? 24 : 12;
+    double xb[24];
+    for (int d = 0; d < nin; ++d)
+      xb[d] = r.y ? EeRespDir(in, d, out, w, h, yb) : VfRespDir(in, d, out, w, h, yb);
+    for (int i = 0; i < 4; ++i) {
+      if (!cl[i]) continue;
+      const int v = c.vbeg + out.ids[i];
+      AddFix3(add_x + 3 * static_cast<int64_t>(v), xb[3 * i], xb[3 * i + 1], xb[3 * i + 2]);
+      if (r.y)
+        AddFix3(add_x0 + 3 * static_cast<int64_t>(v), xb[12 + 3 * i], xb[12 + 3 * i + 1],
+                xb[12 + 3 * i + 2]);
+    }
+  }
+}
+
+// reverse, step 3: gx += add_x, gx0 += add_x0; sums and counts cleared
+__global__ __launch_bounds__(kBlock) void RepAdjApplyKernel(int n, double4* __restrict__ gx,
+                                                           double4* __restrict__ gx0,
+                                                           unsigned long long* __restrict__ add_x,
+                                                           unsigned long long* __restrict__ add_x0,
+                                                           int32_t* __restrict__ cnt) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int64_t* ax = reinterpret_cast<const int64_t*>(add_x) + 3 * static_cast<int64_t>(i);
+  const int64_t* a0 = reinterpret_cast<const int64_t*>(add_x0) + 3 * static_cast<int64_t>(i);
+  if (ax[0] | ax[1] | ax[2]) {
+    double4 g = gx[i];
+    g.x += static_cast<double>(ax[0]) / kAdjFix;
+    g.y += static_cast<double>(ax[1]) / kAdjFix;
+    g.z += static_cast<double>(ax[2]) / kAdjFix;
+    gx[i] = g;
+  }
+  if (a0[0] | a0[1] | a0[2]) {
+    double4 g = gx0[i];
+    g.x += static_cast<double>(a0[0]) / kAdjFix;
+    g.y += static_cast<double>(a0[1]) / kAdjFix;
+    g.z += static_cast<double>(a0[2]) / kAdjFix;
+    gx0[i] = g;
+  }
+  for (int j = 0; j < 3; ++j) {
+    add_x[3 * static_cast<int64_t>(i) + j] = 0ull;
+    add_x0[3 * static_cast<int64_t>(i) + j] = 0ull;
+  }
+  cnt[i] = 0;
 }
 
 // the per-pair VF and / or EE halves of the instances whose item lists are not usable
@@ -1373,6 +1654,36 @@ hipError_t MultiBuildMerge(hipStream_t s, const InstContactDev* ic, const MultiM
 hipError_t MultiObstacleFrameEnd(hipStream_t s, const InstContactDev* ic, const MultiMaps& m) {
   if (Empty(m, MK::kObsVert)) return hipSuccess;
   MultiObsCopyKernel<<<XcdGrid(m.blocks[MK::kObsVert]), kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n);
+  return hipGetLastError();
+}
+
+
+hipError_t MultiRepRecord(hipStream_t s, const InstContactDev* ic, const MultiMaps& m,
+                          const float4* x, const float4* x0, float h, int4* pool,
+                          unsigned long long* count, unsigned long long cap, int32_t* flags,
+                          unsigned long long* mark_begin, unsigned long long* mark_end) {
+  RepMarkKernel<<<1, 64, 0, s>>>(count, mark_begin);
+  if (!Empty(m, MK::kPair)) {
+    MultiRepRecordKernel<0><<<XcdGrid(m.blocks[MK::kPair]), kBlock, 0, s>>>(
+        ic, m.off[MK::kPair], m.n, x, x0, h, pool, count, cap, flags);
+    MultiRepRecordKernel<1><<<XcdGrid(m.blocks[MK::kPair]), kBlock, 0, s>>>(
+        ic, m.off[MK::kPair], m.n, x, x0, h, pool, count, cap, flags);
+  }
+  RepMarkKernel<<<1, 64, 0, s>>>(count, mark_end);
+  return hipGetLastError();
+}
+
+hipError_t RepulsionAdj(hipStream_t s, const InstContactDev* ic, int n_total, const int4* pool,
+                        const unsigned long long* range, unsigned long long cap, const float4* x,
+                        const float4* x0, float h, float relax, int32_t* cnt, double4* gx,
+                        double4* gx0, unsigned long long* add_x, unsigned long long* add_x0) {
+  if (n_total <= 0) return hipSuccess;
+  constexpr int kGrid = 2048;  // grid-stride over the range (its length is on the device)
+  RepAdjCountKernel<<<kGrid, kBlock, 0, s>>>(ic, pool, range, x, x0, h, cap, cnt);
+  RepAdjVjpKernel<<<kGrid, kBlock, 0, s>>>(ic, pool, range, x, x0, h, relax, cap, cnt, gx, add_x,
+                                           add_x0);
+  RepAdjApplyKernel<<<(n_total + kBlock - 1) / kBlock, kBlock, 0, s>>>(n_total, gx, gx0, add_x,
+                                                                     add_x0, cnt);
   return hipGetLastError();
 }
 

@@ -536,7 +536,7 @@ int MultiContactSubstepBegin(xrt_ctx* ctx) {
   return XRT_OK;
 }
 
-int MultiContactIterPre(xrt_ctx* ctx, int it, bool* pass_ran) {
+int MultiContactIterPre(xrt_ctx* ctx, int it, bool* pass_ran, const RepTapeHook* rep_tape) {
   if (pass_ran) *pass_ran = false;
   ContactData* first = FirstContact(ctx);
   if (!first) return XRT_OK;
@@ -562,6 +562,14 @@ int MultiContactIterPre(xrt_ctx* ctx, int it, bool* pass_ran) {
   if (p.enable_self_contact) {
     if (it % interval == 0)
       XRT_CHECK_HIP(MultiPairs(s, ctx->ic.ptr, ctx->maps, ctx->x.ptr, ctx->x0.ptr, kPairThickness));
+    if (rep_tape) {
+      XRT_CHECK_HIP(hipMemcpyAsync(rep_tape->x_out, ctx->x.ptr, sizeof(float4) * ctx->x.n,
+                                   hipMemcpyDeviceToDevice, s));
+      XRT_CHECK_HIP(MultiRepRecord(s, ctx->ic.ptr, ctx->maps, ctx->x.ptr, ctx->x0.ptr,
+                                   p.pbd_thickness, rep_tape->pool, rep_tape->count,
+                                   rep_tape->cap, rep_tape->flags, rep_tape->marks,
+                                   rep_tape->marks + 1));
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ctx->rep_next + 1 < ctx->rep_events.size()) {
       e0 = ctx->rep_events[ctx->rep_next];

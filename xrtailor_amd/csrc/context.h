@@ -216,6 +216,9 @@ struct AdjointData {
     DevBuf<int32_t> vf_face, rt_face;
     DevBuf<float4> vf_q, rt_q, pred;
     DevBuf<char> ee;
+    // EE reverse: each vertex's edges (CSR, ascending) and a per-edge multiplier adjoint
+    DevBuf<int32_t> ve_ptr, ve_edge;
+    DevBuf<double> ee_gl;
   };
   std::vector<std::unique_ptr<Records>> rec;
   DevBuf<float4> ckpt;    // (x0, v) every ckpt_every frames: [n_ckpt][2][n_total]
@@ -224,7 +227,17 @@ struct AdjointData {
   DevBuf<double4> gx0, gv, gx;  // FP64 adjoints
   DevBuf<double> glam, gcomp, gbcomp;
   DevBuf<int32_t> ee_cnt;
-  DevBuf<double> ee_gadd;
+  // PBD repulsion (self contact): per frame slot a pool of the responses of every repulsion
+  // pass (instance, is_ee, item), per (slot, substep, iteration) their range (begin, end);
+  // the state before each pass is taped at off_r
+  bool rep = false;
+  size_t off_r = 0;
+  unsigned long long rep_cap = 0;        // pool entries per frame slot
+  DevBuf<int4> rep_pool;                 // slots x rep_cap
+  DevBuf<unsigned long long> rep_count;  // per slot
+  DevBuf<unsigned long long> rep_marks;  // 2 per (slot, substep, iteration)
+  DevBuf<int32_t> rep_flags, rep_cnt;    // 1; n_total
+  DevBuf<unsigned long long> rep_add;    // 6 n_total (32.32 fixed point)
   // long-range attachments: per constraint, the target's position before it (replay) and
   // its slot's adjoint; per global slot, its instance and its constraints (in order)
   bool lra = false;
@@ -331,7 +344,17 @@ int ContactStateReset(xrt_ctx* ctx, int inst);
 bool ContactsActive(const xrt_ctx* ctx);
 int MultiContactSetup(xrt_ctx* ctx);
 int MultiContactSubstepBegin(xrt_ctx* ctx);
-int MultiContactIterPre(xrt_ctx* ctx, int it, bool* pass_ran);
+// with `rep_tape` (the adjoint's taped forward): the state before the repulsion pass is
+// copied to rep_tape->x_out and the pass's responses recorded (MultiRepRecord)
+struct RepTapeHook {
+  float4* x_out = nullptr;
+  int4* pool = nullptr;
+  unsigned long long* count = nullptr;
+  unsigned long long cap = 0;
+  int32_t* flags = nullptr;
+  unsigned long long* marks = nullptr;  // (begin, end)
+};
+int MultiContactIterPre(xrt_ctx* ctx, int it, bool* pass_ran, const RepTapeHook* rep_tape = nullptr);
 int MultiContactVfSolve(xrt_ctx* ctx);
 int MultiContactEeSolve(xrt_ctx* ctx);
 int MultiContactSubstepEnd(xrt_ctx* ctx);
