@@ -648,6 +648,9 @@ def main() -> int:
         "parity_instances": parity_n,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": ("profiles/traffic.json: rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                                        "of the same launches (own runs; same algorithmic bytes)")
+                     if traffic else None,
                      "kernel": kernel, "kernel_ms": kern_ms, "bytes_per_launch": bytes_kern,
                      "repulsion_counts": rep_counts, "particle": particle,
                      "binding": binding, "fem": fem},
