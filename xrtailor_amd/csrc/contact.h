@@ -130,8 +130,8 @@ struct InstContactDev {
   uint32_t* ivf = nullptr;
   uint32_t* iee = nullptr;
   // pair refresh: the first kPairStage hits of each cloth leaf, hit j of leaf i at
-  // stage[i * kPairStage + j] as (rhs object, item mask) (null: the pairs are found by a
-  // second traversal)
+  // stage[j * nf + i] as (rhs object, item mask) (null: the pairs are found by a second
+  // traversal)
   int2* stage = nullptr;
   // pair refresh: per 256-leaf block of the count pass, its pair / VF / EE item sums
   // (3 x Blocks(nf)), then their exclusive prefix

@@ -975,9 +975,8 @@ __global__ __launch_bounds__(kBlock) void MultiPackObsKernel(const InstContactDe
 
 // pairs per cloth leaf against the obstacle (self = 0) or the cloth itself (self = 1),
 // traversing the packed records, with the repulsion item counts of those pairs when the
-// item lists are on; the first kPairStage hits of leaf i are kept at
-// stage[i * kPairStage + j] as (rhs object, item mask) (when the instance has a stage): a
-// leaf's hits are contiguous, so the emit pass reads them as one run per leaf
+// item lists are on; the first kPairStage hits of leaf i are kept at stage[j * nf + i] as
+// (rhs object, item mask) (when the instance has a stage)
 // exclusive scan of three counters over the 256 threads of a block; tot = the block sums
 __device__ __forceinline__ void BlockScan3(uint32_t v[3], uint32_t tot[3]) {
   __shared__ uint32_t wsum[kBlock / 64][3];
@@ -1023,7 +1022,8 @@ __global__ __launch_bounds__(kBlock) void MultiCountPairsKernel(const InstContac
     int4 f0 = make_int4(0, 0, 0, 0);
     if (items) f0 = c.cloth.trir[lobj];
     const uint32_t r0 = static_cast<uint32_t>(f0.w);
-    int2* st = c.stage + static_cast<int64_t>(i) * kPairStage;
+    int2* st = c.stage + i;
+    const int stride = c.nf;
     TraversePacked(self ? c.cbvh.packed : c.obvh.packed, self ? c.nf : c.obs_nf, lb, th,
                    [&](int robj) {
                      uint32_t m = 0;
@@ -1034,7 +1034,7 @@ __global__ __launch_bounds__(kBlock) void MultiCountPairsKernel(const InstContac
                        nee += __popc(m >> 3);
                      }
                      if (c.stage && v < static_cast<uint32_t>(kPairStage))
-                       st[v] = make_int2(robj, static_cast<int>(m));
+                       st[static_cast<int64_t>(v) * stride] = make_int2(robj, static_cast<int>(m));
                      ++v;
                    });
   }
@@ -1087,148 +1087,69 @@ __device__ __forceinline__ bool ItemsUsable(const InstContactDev& c, int is_ee) 
 // after all obstacle-pair items; only slots below the list capacities are written.  The
 // cloth-pair pass runs after the totals: when both lists will be used, nothing reads the
 // cloth pairs and they are not stored.
-// Each lane loads its own leaf's offsets and face data; then the wave emits its leaves with
-// hits two at a time: the 32 lanes of a half-wave take the staged hits of one leaf (one
-// contiguous read), so each leaf's pairs and items are written as contiguous runs instead
-// of one thread's scattered 8-byte stores.  Positions are those of the one-thread-per-leaf
-// order (hit j of the leaf at the same slot), so the lists are the same bit for bit.
-// Leaves beyond the stage are traversed again by the half's first lane.
-__device__ __forceinline__ uint32_t HalfExclusiveScan(uint32_t v, int hl) {
-  uint32_t inc = v;
-#pragma unroll
-  for (int d = 1; d < 32; d <<= 1) {
-    const uint32_t t = __shfl_up(inc, d, 32);
-    if (hl >= d) inc += t;
-  }
-  return inc - v;
-}
-
-// per-leaf emit data, loaded by the leaf's own lane and read by the half-wave that emits it
-struct EmitLeaf {
-  int64_t lo, vo, eo;
-  uint32_t cnt, r0;
-  int lobj, i;
-  int4 f0, fe0;
-};
-
 __global__ __launch_bounds__(kBlock) void MultiEmitPairsKernel(const InstContactDev* ic,
                                                                const int32_t* off, int n, float th,
                                                                int self) {
-  const int lblk = XcdBlock(off, n);
-  if (lblk < 0) return;
-  const int k = InstOfBlock(off, n, lblk);
-  const InstContactDev& c = ic[k];
+  XRT_MULTI_INDEX(kFace, c.nf)
   if (!self && !c.has_obs) return;
   const bool store = !self || !(ItemsUsable(c, 0) && ItemsUsable(c, 1));
   const int64_t cap = store ? c.pair_cap : 0;
+  const uint32_t cnt = c.cbvh.counts[i];
   const int nb = (c.nf + kBlock - 1) / kBlock, blk = lblk - off[k];
+  const int64_t lo = static_cast<int64_t>(c.scal[self ? 1 : 0]) + c.bsum[blk] + c.cbvh.offsets[i];
+  const Box lb = c.cbvh.packed[c.nf - 1 + i];
+  const int lobj = -__float_as_int(lb.lo.w) - 1;
   const bool items = c.ee_cap > 0;
   const MeshTopo& ot = self ? c.cloth : c.obs;
   const int tag = self ? kSelfBit : 0;
-  const int lane = threadIdx.x & 63, hl = lane & 31, w = threadIdx.x >> 6;
-  __shared__ EmitLeaf leaf[kBlock];
-  __shared__ int order[kBlock / 64][64];
-  // each lane loads its own leaf (as one thread per leaf would)
-  const int i = blk * kBlock + threadIdx.x;
-  uint32_t cnt = 0;
-  if (i < c.nf) {
-    cnt = c.cbvh.counts[i];
-    EmitLeaf L;
-    L.i = i;
-    L.cnt = cnt;
-    L.lo = static_cast<int64_t>(c.scal[self ? 1 : 0]) + c.bsum[blk] + c.cbvh.offsets[i];
-    L.lobj = -__float_as_int(c.cbvh.packed[c.nf - 1 + i].lo.w) - 1;
-    L.f0 = L.fe0 = make_int4(0, 0, 0, 0);
-    L.r0 = 0;
-    L.vo = L.eo = 0;
-    if (items && cnt) {
-      L.f0 = c.cloth.tris[L.lobj];
-      L.fe0 = c.cloth.face_edges[L.lobj];
-      L.r0 = c.cloth.rtri[L.lobj];
-      L.vo = (self ? static_cast<int64_t>(c.scal[5]) : 0) + c.bsum[nb + blk] + c.ivf[i];
-      L.eo = (self ? static_cast<int64_t>(c.scal[6]) : 0) + c.bsum[2 * nb + blk] + c.iee[i];
-    }
-    leaf[threadIdx.x] = L;
+  int4 f0 = make_int4(0, 0, 0, 0), fe0 = make_int4(0, 0, 0, 0);
+  uint32_t r0 = 0;
+  int64_t vo = 0, eo = 0;
+  if (items) {
+    f0 = c.cloth.tris[lobj];
+    fe0 = c.cloth.face_edges[lobj];
+    r0 = c.cloth.rtri[lobj];
+    vo = (self ? static_cast<int64_t>(c.scal[5]) : 0) + c.bsum[nb + blk] + c.ivf[i];
+    eo = (self ? static_cast<int64_t>(c.scal[6]) : 0) + c.bsum[2 * nb + blk] + c.iee[i];
   }
-  // the wave's staged leaves with hits, in lane order (leaves beyond the stage are
-  // traversed again below, each by its own lane)
-  const bool staged_own = cnt > 0 && c.stage && cnt <= static_cast<uint32_t>(kPairStage);
-  const uint64_t work = __ballot(staged_own);
-  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  if (staged_own) order[w][__popcll(work & below)] = lane;
-  const int nwork = __popcll(work);
-  __syncthreads();
-  for (int r = 0; 2 * r < nwork; ++r) {
-    const int slot = 2 * r + (lane >> 5);  // this half-wave's leaf: the slot-th staged one
-    const bool live = slot < nwork;
-    const EmitLeaf& L = leaf[w * 64 + (live ? order[w][slot] : 0)];
-    const bool have = live && static_cast<uint32_t>(hl) < L.cnt;
-    int2 e = make_int2(0, 0);
-    if (have) {
-      e = c.stage[static_cast<int64_t>(L.i) * kPairStage + hl];
-      const int64_t idx = L.lo + L.cnt - 1 - hl;
-      if (idx < cap) c.pairs[idx] = make_int2(L.lobj, e.x);
-    }
-    const uint32_t m = have && items ? static_cast<uint32_t>(e.y) : 0u;
-    const uint32_t pv = HalfExclusiveScan(__popc(m & 7u), hl);
-    const uint32_t pe = HalfExclusiveScan(__popc(m >> 3), hl);
-    if (m) {
-      const int v0[3] = {L.f0.x, L.f0.y, L.f0.z};
-      int64_t q0 = L.vo + pv;
+  auto emit_items = [&](int robj, uint32_t m) {
+    if (!m) return;
+    const int v0[3] = {f0.x, f0.y, f0.z};
 #pragma unroll
-      for (int q = 0; q < 3; ++q)
-        if ((m >> q) & 1u) {
-          if (q0 < c.vf_cap) c.rep_vf[q0] = make_int2(v0[q], e.x | tag);
-          ++q0;
-        }
-      if (m >> 3) {
-        const int4 fe1 = ot.face_edges[e.x];
-        const int ea[3] = {L.fe0.x, L.fe0.y, L.fe0.z}, eb[3] = {fe1.x, fe1.y, fe1.z};
-        int64_t q1 = L.eo + pe;
-#pragma unroll
-        for (int q = 0; q < 9; ++q)
-          if ((m >> (3 + q)) & 1u) {
-            if (q1 < c.ee_cap) c.rep_ee[q1] = make_int2(ea[q / 3], eb[q % 3] | tag);
-            ++q1;
-          }
+    for (int q = 0; q < 3; ++q)
+      if ((m >> q) & 1u) {
+        if (vo < c.vf_cap) c.rep_vf[vo] = make_int2(v0[q], robj | tag);
+        ++vo;
       }
-    }
-  }
-  // a leaf with more hits than the stage holds (or no stage: self contact came on after
-  // start): its own lane traverses again, in the count pass's hit order
-  if (cnt > 0 && !staged_own) {
-    const EmitLeaf& L = leaf[threadIdx.x];
-    {
-      int64_t index = L.lo + L.cnt - 1, vo = L.vo, eo = L.eo;
-      const int lobj = L.lobj;
-      const int4 f0 = L.f0, fe0 = L.fe0;
-      const uint32_t r0 = L.r0;
-      TraversePacked(self ? c.cbvh.packed : c.obvh.packed, self ? c.nf : c.obs_nf,
-                     c.cbvh.packed[c.nf - 1 + L.i], th, [&](int robj) {
-                       if (index >= L.lo && index < cap) c.pairs[index] = make_int2(lobj, robj);
-                       --index;
-                       if (!items) return;
+    if (!(m >> 3)) return;
+    const int4 fe1 = ot.face_edges[robj];
+    const int e0[3] = {fe0.x, fe0.y, fe0.z}, e1[3] = {fe1.x, fe1.y, fe1.z};
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+      if ((m >> (3 + q)) & 1u) {
+        if (eo < c.ee_cap) c.rep_ee[eo] = make_int2(e0[q / 3], e1[q % 3] | tag);
+        ++eo;
+      }
+  };
+  if (!c.stage || cnt > static_cast<uint32_t>(kPairStage)) {  // no stage when self contact
+    int64_t index = lo + cnt - 1;                              // came on after start
+    TraversePacked(self ? c.cbvh.packed : c.obvh.packed, self ? c.nf : c.obs_nf, lb, th,
+                   [&](int robj) {
+                     if (index >= lo && index < cap) c.pairs[index] = make_int2(lobj, robj);
+                     --index;
+                     if (items) {
                        const int4 f1 = ot.trir[robj];
-                       const uint32_t mm = PairItemMask(f0, r0, f1, static_cast<uint32_t>(f1.w),
-                                                        self != 0);
-                       const int v0[3] = {f0.x, f0.y, f0.z};
-#pragma unroll
-                       for (int q = 0; q < 3; ++q)
-                         if ((mm >> q) & 1u) {
-                           if (vo < c.vf_cap) c.rep_vf[vo] = make_int2(v0[q], robj | tag);
-                           ++vo;
-                         }
-                       if (!(mm >> 3)) return;
-                       const int4 fe1 = ot.face_edges[robj];
-                       const int ea[3] = {fe0.x, fe0.y, fe0.z}, eb[3] = {fe1.x, fe1.y, fe1.z};
-#pragma unroll
-                       for (int q = 0; q < 9; ++q)
-                         if ((mm >> (3 + q)) & 1u) {
-                           if (eo < c.ee_cap) c.rep_ee[eo] = make_int2(ea[q / 3], eb[q % 3] | tag);
-                           ++eo;
-                         }
-                     });
-    }
+                       emit_items(robj, PairItemMask(f0, r0, f1, static_cast<uint32_t>(f1.w),
+                                                     self != 0));
+                     }
+                   });
+    return;
+  }
+  for (uint32_t j = 0; j < cnt; ++j) {
+    const int2 e = c.stage[static_cast<int64_t>(j) * c.nf + i];
+    const int64_t idx = lo + cnt - 1 - j;
+    if (idx < cap) c.pairs[idx] = make_int2(lobj, e.x);
+    if (items) emit_items(e.x, static_cast<uint32_t>(e.y));
   }
 }
 
