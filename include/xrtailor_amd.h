@@ -131,7 +131,10 @@ typedef struct xrt_tuning {
                               * 4 FEM), 1 light (1 / 1, 2 / 1, 2), 2 heavy (4 / 2, 8 / 4, 8) */
   int32_t wide_per_thread;   /* stretch / bend elements per thread of the wide launches (the
                               * batched substep with contacts): 1 (default) or 2 */
-  int32_t path;              /* set by xrt_autotune: 0 persistent substep, 1 wide substep */
+  int32_t path;              /* 0 persistent substep, 1 wide substep (one launch per color of
+                              * every instance, each instance over as many blocks as the
+                              * color needs); contexts with contacts, particles or bindings
+                              * always take the wide substep */
   float best_ms;             /* set by xrt_autotune: ms per substep of the chosen shapes */
   float default_ms;          /* set by xrt_autotune: ms per substep of the defaults */
   int32_t candidates;        /* set by xrt_autotune: configurations timed */
@@ -394,8 +397,9 @@ int xrt_substep_batched(xrt_ctx* ctx);
  * plain frames after any sequence of substeps. */
 int xrt_frames_batched(xrt_ctx* ctx, int32_t nframes);
 /* Autotuner (no counterpart in the reference): times every candidate launch shape of the
- * path the context's batched substep takes (persistent: workgroup size 1024 / 768 / 512 /
- * 256 x unroll 0 / 1 / 2; wide: 1 or 2 elements per thread) over `substeps` batched
+ * path the context's batched substep takes (without contacts, particles or bindings: the
+ * persistent substep's workgroup size 1024 / 768 / 512 / 256 x unroll 0 / 1 / 2, and the
+ * wide substep; wide: 1 or 2 elements per thread) over `substeps` batched
  * substeps of the context's own instances, each from the current state, and keeps the
  * fastest.  The state (positions, velocities, frame position, obstacle frame-start
  * positions, hash schedule) is restored afterwards, so results are those of an untuned
@@ -403,7 +407,8 @@ int xrt_frames_batched(xrt_ctx* ctx, int32_t nframes);
  * choice and the timings. */
 int xrt_autotune(xrt_ctx* ctx, int32_t substeps, xrt_tuning* out);
 int xrt_get_tuning(xrt_ctx* ctx, xrt_tuning* out);
-/* applies persistent_block / persistent_unroll / wide_per_thread (validated) */
+/* applies persistent_block / persistent_unroll / wide_per_thread / path (validated; every
+ * choice gives the same results bit for bit) */
 int xrt_set_tuning(xrt_ctx* ctx, const xrt_tuning* t);
 /* `nframes` batched frames (as xrt_frames_batched) with the per-frame cache of
  * ClothSolverUniversal::SaveCache (xrtailor/src/pipeline/impl/universal/ClothSolver.cpp:

@@ -26,29 +26,31 @@ def _equal(a, b):
 
 
 def test_persistent_candidates_equal_default_and_autotune_restores_state():
-    print("case autotune: persistent substep candidates (block x unroll)", flush=True)
+    print("case autotune: persistent substep candidates (block x unroll) and the wide substep",
+          flush=True)
     specs = _specs()
     print(f"  instances of {sorted(len(sp.pos) for sp in specs)} vertices", flush=True)
     xp = make_params(bend_model=1, ground_height=-1.0)
     s, ids = build_solver(specs, xp, seed=4)
     start = [(sp.x0, np.zeros_like(sp.x0)) for sp in specs]
     ref = None
-    for block in (1024, 768, 512, 256):
-        for unroll in (0, 1, 2):
-            for i, (x0, v0) in zip(ids, start):
-                s.SetState(i, x0, v0)
-            s.set_tuning(block, unroll, 1)
-            s.FramesBatched(1)
-            s.sync()
-            got = _states(s, ids)
-            if ref is None:
-                ref = got
-            else:
-                _equal(got, ref)
+    shapes = [(block, unroll, 1, 0) for block in (1024, 768, 512, 256) for unroll in (0, 1, 2)]
+    shapes += [(1024, 0, 1, 1), (1024, 0, 2, 1)]  # wide substep, 1 / 2 elements per thread
+    for shape in shapes:
+        for i, (x0, v0) in zip(ids, start):
+            s.SetState(i, x0, v0)
+        s.set_tuning(*shape)
+        s.FramesBatched(1)
+        s.sync()
+        got = _states(s, ids)
+        if ref is None:
+            ref = got
+        else:
+            _equal(got, ref)
     before = _states(s, ids)
     t = s.autotune(2)
     print(f"  chose {t}", flush=True)
-    assert t["path"] == 0 and t["candidates"] == 12 and t["best_ms"] > 0.0
+    assert t["path"] in (0, 1) and t["candidates"] == 14 and t["best_ms"] > 0.0
     assert t["best_ms"] <= t["default_ms"]
     _equal(_states(s, ids), before)                 # state restored
     assert s.tuning()["persistent_block"] == t["persistent_block"]
@@ -56,6 +58,8 @@ def test_persistent_candidates_equal_default_and_autotune_restores_state():
         s.set_tuning(100, 0, 1)
     with pytest.raises(XrtError):
         s.set_tuning(1024, 3, 1)
+    with pytest.raises(XrtError):
+        s.set_tuning(1024, 0, 1, 2)
 
 
 def test_tuned_frames_equal_untuned_frames():
@@ -71,7 +75,8 @@ def test_tuned_frames_equal_untuned_frames():
             if mode == "tuned":
                 t = s.autotune(2)
                 print(f"  contacts={contacts}: {t}", flush=True)
-                assert t["path"] == contacts and t["candidates"] == (2 if contacts else 12)
+                assert t["candidates"] == (2 if contacts else 14)
+                assert t["path"] == 1 if contacts else t["path"] in (0, 1)
             elif mode == "two per thread":
                 s.set_tuning(1024, 0, 2)
             s.FramesBatched(2)

@@ -190,7 +190,10 @@ int SubstepPlain(xrt_ctx* c, int32_t inst) {
 int SubstepBatched(xrt_ctx* c) {
   const StepConsts k = MakeConsts(c->params);
   XRT_REQUIRE(!FemOn(c) || c->fem.ready, XRT_ERR_STATE, "FEM constraints are not set up");
-  if (ContactsActive(c) || ParticlesOn(c) || BindingsOn(c))
+  // contacts, particles and bindings need the wide substep; without them it is a tuning
+  // choice (every instance split over as many blocks as its colors need, one launch per
+  // color, against one workgroup per instance in the persistent kernel)
+  if (ContactsActive(c) || ParticlesOn(c) || BindingsOn(c) || c->tuning.path == 1)
     return WideSubstep(c, k, ModeOf(k), nullptr, -1);
   XRT_CHECK_HIP(LaunchBatchedSubstep(c->stream, MakeBatchArgs(c), k,
                                      static_cast<int>(c->inst.size()), c->tuning.persistent_block,
@@ -359,6 +362,7 @@ int CheckTuning(const xrt_tuning& t) {
               "persistent_unroll must be 0, 1 or 2");
   XRT_REQUIRE(t.wide_per_thread == 1 || t.wide_per_thread == 2, XRT_ERR_INVALID_ARG,
               "wide_per_thread must be 1 or 2");
+  XRT_REQUIRE(t.path == 0 || t.path == 1, XRT_ERR_INVALID_ARG, "path must be 0 or 1");
   return XRT_OK;
 }
 
@@ -1061,6 +1065,7 @@ int xrt_set_tuning(xrt_ctx* ctx, const xrt_tuning* t) {
   ctx->tuning.persistent_block = t->persistent_block;
   ctx->tuning.persistent_unroll = t->persistent_unroll;
   ctx->tuning.wide_per_thread = t->wide_per_thread;
+  ctx->tuning.path = t->path;
   DestroyGraph(ctx);  // the next batched frame captures the new shapes
   return XRT_OK;
 }
@@ -1090,6 +1095,12 @@ int xrt_autotune(xrt_ctx* ctx, int32_t substeps, xrt_tuning* out) {
         t.persistent_unroll = unroll;
         cand.push_back(t);
       }
+    for (int pt : {1, 2}) {  // the wide substep, which splits large instances over blocks
+      xrt_tuning t = base;
+      t.path = 1;
+      t.wide_per_thread = pt;
+      cand.push_back(t);
+    }
   }
   const xrt_tuning before = ctx->tuning;
   xrt_tuning best = base;

@@ -419,19 +419,36 @@ __device__ __forceinline__ void EvalVF(const RMesh& c, int node, const RMesh& o,
 // in bw, when the barycentric part passes; InEdge and the response follow in EEResponse.
 // The weights take early exits: an accepted pair has every weight > 0 (a NaN weight fails
 // InEdge or leaves another weight <= 0), which needs a0, a1 and b0, b1 each of one strict
-// sign, checked before the divisions on the same expressions.
-__device__ __forceinline__ bool EEInside(int2 a, int2 b, F3 x0, F3 x1, F3 y0, F3 y1, bool self,
-                                         F3 ua, F3 ub, float4& bw) {
+// sign, checked before the divisions on the same expressions.  EESigns is the part before
+// the divisions (num receives a0, a1, b0, b1), EEWeights the divisions and the lower bound.
+__device__ __forceinline__ bool EESigns(int2 a, int2 b, F3 x0, F3 x1, F3 y0, F3 y1, bool self,
+                                        F3 ua, F3 ub, float4& num) {
   if (self && (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y)) return false;
   const F3 ncur = C(ua, ub);
   if (D(ncur, ncur) < 1e-6f) return false;  // degenerate: weights 0
-  const float a0 = Mixed(S(y1, x1), S(y0, x1), ncur), a1 = Mixed(S(y0, x0), S(y1, x0), ncur);
+  const F3 d00 = S(y0, x0), d01 = S(y1, x0), d10 = S(y0, x1), d11 = S(y1, x1);
+  const float a0 = Mixed(d11, d10, ncur), a1 = Mixed(d00, d01, ncur);
   if (!((a0 > 0 && a1 > 0) || (a0 < 0 && a1 < 0))) return false;
-  const float b0 = Mixed(S(x0, y1), S(x1, y1), ncur), b1 = Mixed(S(x1, y0), S(x0, y0), ncur);
+  // b0 = Mixed(S(x0, y1), S(x1, y1), ncur) and b1 = Mixed(S(x1, y0), S(x0, y0), ncur): both
+  // differences negated, which leaves every nonzero result the same bits (rounding is
+  // symmetric; only the sign of an exact zero can differ, and zero fails the test either way)
+  const float b0 = Mixed(d01, d11, ncur), b1 = Mixed(d10, d00, ncur);
   if (!((b0 > 0 && b1 > 0) || (b0 < 0 && b1 < 0))) return false;
-  bw = make_float4(a0 / (a0 + a1), a1 / (a0 + a1), -b0 / (b0 + b1), -b1 / (b0 + b1));
+  num = make_float4(a0, a1, b0, b1);
+  return true;
+}
+
+__device__ __forceinline__ bool EEWeights(float4 num, float4& bw) {
+  bw = make_float4(num.x / (num.x + num.y), num.y / (num.x + num.y), -num.z / (num.z + num.w),
+                   -num.w / (num.z + num.w));
   const float mn = fminf(fminf(bw.x, bw.y), fminf(-bw.z, -bw.w));
   return mn >= 1e-6f;
+}
+
+__device__ __forceinline__ bool EEInside(int2 a, int2 b, F3 x0, F3 x1, F3 y0, F3 y1, bool self,
+                                         F3 ua, F3 ub, float4& bw) {
+  float4 num;
+  return EESigns(a, b, x0, x1, y0, y1, self, ua, ub, num) && EEWeights(num, bw);
 }
 
 // The rest of EvaluateEEProximity for a pair that passed EEInside with weights w: the
@@ -695,15 +712,30 @@ __global__ void AddTotalsKernel(const int32_t* a, const int32_t* b, int32_t* out
   }
 }
 
-// An EE item that passed the inside test, queued for its response: the cloth edge, the
-// other edge (bit 31: a cloth edge) and the weights
+// An EE item that passed the sign part of the inside test, queued for the rest (the weights'
+// divisions and lower bound) and its response: the cloth edge, the other edge (bit 31: a
+// cloth edge) and the weights' numerators.  The divisions run at response time, 64 queued
+// items at a time, instead of in every wave that holds one survivor.
 struct EeQueued {
   int e0, e1;
-  float4 bw;
+  float4 num;
 };
 
-// the inside test of EE item `it` (RepulsionEeItemBody up to its response); ent receives the
-// queued form when it passes
+// a load through the global address space: pointers read from instance records would
+// otherwise be loaded through flat addressing, which also waits on LDS traffic.  Device
+// allocations have the same address in the flat and the global aperture (the integer route
+// keeps the compiler from folding the cast away).
+template <class T>
+__device__ __forceinline__ T Gld(const T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *(const __attribute__((address_space(1))) T*)(reinterpret_cast<uintptr_t>(p));
+#else
+  return *p;  // the host pass of this device function (never executed)
+#endif
+}
+
+// the sign part of the inside test of EE item `it` (RepulsionEeItemBody up to its weights);
+// ent receives the queued form when it passes
 __device__ __forceinline__ bool EeItemInside(const RMesh& c, const RMesh& o, int2 it,
                                              EeQueued& ent) {
   if (it.x < 0 || it.x >= c.t.ne) return false;
@@ -712,27 +744,31 @@ __device__ __forceinline__ bool EeItemInside(const RMesh& c, const RMesh& o, int
   if (e1 >= (self ? c.t.ne : o.t.ne)) return false;
   const int2* oe = self ? c.t.edges : o.t.edges;  // pointers, not meshes, are selected
   const float4* ox = self ? c.s.x : o.s.x;
-  const int2 a = c.t.edges[it.x], b = oe[e1];
-  const F3 x0 = L3(c.s.x[a.x]), x1 = L3(c.s.x[a.y]), y0 = L3(ox[b.x]), y1 = L3(ox[b.y]);
+  const int2 a = Gld(c.t.edges + it.x), b = Gld(oe + e1);
+  const F3 x0 = L3(Gld(c.s.x + a.x)), x1 = L3(Gld(c.s.x + a.y)), y0 = L3(Gld(ox + b.x)),
+           y1 = L3(Gld(ox + b.y));
   const F3 ua = Nz(S(x1, x0)), ub = Nz(S(y1, y0));
   ent.e0 = it.x;
   ent.e1 = it.y;
-  return EEInside(a, b, x0, x1, y0, y1, self, ua, ub, ent.bw);
+  return EESigns(a, b, x0, x1, y0, y1, self, ua, ub, ent.num);
 }
 
-// the response of a queued EE item (positions reloaded: the pass does not move them)
+// the rest of the inside test and the response of a queued EE item (positions reloaded: the
+// pass does not move them)
 __device__ __forceinline__ void EeItemRespond(const RMesh& c, const RMesh& o, const EeQueued& e,
                                               float h, long long* __restrict__ dq,
                                               int32_t* __restrict__ cnt, RepOut* out = nullptr) {
+  float4 bw;
+  if (!EEWeights(e.num, bw)) return;
   const int e1 = e.e1 & 0x7fffffff;
   const int2 a = c.t.edges[e.e0];
   if (e.e1 < 0) {
     const int2 b = c.t.edges[e1];
-    EEResponse(c, a, c, b, c.X(a.x), c.X(a.y), c.X(b.x), c.X(b.y), e.e0, e1, h, e.bw, dq, cnt,
+    EEResponse(c, a, c, b, c.X(a.x), c.X(a.y), c.X(b.x), c.X(b.y), e.e0, e1, h, bw, dq, cnt,
                out);
   } else {
     const int2 b = o.t.edges[e1];
-    EEResponse(c, a, o, b, c.X(a.x), c.X(a.y), o.X(b.x), o.X(b.y), e.e0, e1, h, e.bw, dq, cnt,
+    EEResponse(c, a, o, b, c.X(a.x), c.X(a.y), o.X(b.x), o.X(b.y), e.e0, e1, h, bw, dq, cnt,
                out);
   }
 }
@@ -1205,7 +1241,7 @@ __global__ __launch_bounds__(kBlock) void MultiRepItemKernel(const InstContactDe
     if (have) {
       const int qi = base + lane;
       EeQueued ent;
-      const bool pass = qi < cnt && EeItemInside(cm, om, items[qi], ent);
+      const bool pass = qi < cnt && EeItemInside(cm, om, Gld(items + qi), ent);
       const uint64_t m = __ballot(pass);
       if (pass) qw[qn + __popcll(m & below)] = ent;
       qn += __popcll(m);

@@ -556,8 +556,9 @@ class ClothSolver:
         return {k: getattr(t, k) for k, _ in t._fields_}
 
     def set_tuning(self, persistent_block: int = 1024, persistent_unroll: int = 0,
-                   wide_per_thread: int = 1) -> None:
-        t = _ffi.XrtTuning(persistent_block, persistent_unroll, wide_per_thread, 0, 0.0, 0.0, 0)
+                   wide_per_thread: int = 1, path: int = 0) -> None:
+        t = _ffi.XrtTuning(persistent_block, persistent_unroll, wide_per_thread, path, 0.0, 0.0,
+                           0)
         check(self.lib.xrt_set_tuning(self.ctx, C.byref(t)))
 
     def time_batched_substep(self, n: int) -> float:
