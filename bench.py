@@ -7,14 +7,15 @@ in [32, 192]), per-instance stretch / bend compliance log-uniform in [1e-9, 1e-6
 synthetic from seed.  A step is one frame (10 substeps) of all instances.
 
 value        batched vertex-substeps/s (one hipGraph per frame; every contact phase and
-             constraint color of all instances per launch; without contacts one persistent
-             launch per substep), inputs resident in HBM
+             constraint color of all instances per launch; without contacts the same wide
+             launches or one persistent launch per substep, whichever the autotuner times
+             faster), inputs resident in HBM
 plain_value  the same workload one instance at a time through per-instance launches
 roofline     dominant kernel, per-launch device time from HIP events on the solver stream:
              with contacts the EE repulsion item kernel, algorithmic bytes per launch
              = 8 (EE items) + 16 N + 8 E (items, positions, edges read once); without
-             contacts the persistent substep kernel, sum_i (64 N_i + 16 E_i + 24 B_i +
-             12 K_i); HBM peak 8 TB/s.  traffic: HBM bytes per launch from the committed
+             contacts the persistent substep kernel (or the wide substep's launches
+             together), sum_i (64 N_i + 16 E_i + 24 B_i + 12 K_i); HBM peak 8 TB/s.  traffic: HBM bytes per launch from the committed
              rocprofv3 FETCH_SIZE / WRITE_SIZE summary of that kernel (profiles/traffic.json),
              null when absent
 cpu_baseline the serial FP64 oracle on one core, one substep per instance over a bounded
@@ -329,6 +330,10 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--pair-capacity", type=int, default=256,
                     help="face pairs per cloth face of the self-contact pair lists")
+    ap.add_argument("--vf-items", type=int, default=128,
+                    help="VF repulsion test items per cloth face of the batched item lists")
+    ap.add_argument("--ee-items", type=int, default=256,
+                    help="EE repulsion test items per cloth face of the batched item lists")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--plain-frames", type=int, default=1)
     ap.add_argument("--rollout-frames", type=int, default=120,
@@ -417,11 +422,15 @@ def main() -> int:
     # face-pair lists: the 256 cloths draped over their obstacles reach ~76 pairs per face
     # within the first frames and 130 in the crumpled frames near the end of the 120-frame
     # rollout; 256 per face holds every pair of the whole rollout (an overflow would make
-    # the timing calls fail with XRT_ERR_CAPACITY)
+    # the timing calls fail with XRT_ERR_CAPACITY).  Test-item lists of twice the default
+    # capacities keep the most crumpled cloth on the item path (with 64 / 128 one instance
+    # evaluated per pair); results are the same either way.
     params = make_params(bend_model=1, ground_height=-1.0, enable_obstacle_contact=int(contacts),
                          enable_self_contact=int(self_contact),
                          enable_particle_collision=int(swift),
-                         pair_capacity_per_face=a.pair_capacity, **fem)
+                         pair_capacity_per_face=a.pair_capacity,
+                         repulsion_vf_items_per_face=a.vf_items,
+                         repulsion_ee_items_per_face=a.ee_items, **fem)
     if quality:
         wname += (", sphere / capsule obstacle and self contact" if contacts else "") + \
             ", FEM strain (moduli 1e3/1e3/4e2) and isometric bending (quality mode)"

@@ -945,9 +945,10 @@ __global__ __launch_bounds__(kBlock) void MultiVfGenerateKernel(const InstContac
   VfCcdGenerateBody(x0 + c.vbeg, x + c.vbeg, i, c.obvh, c.rt_face, c.rt_q);
 }
 
-__global__ __launch_bounds__(kBlock) void MultiEeGenerateKernel(const InstContactDev* ic,
-                                                                const int32_t* off, int n,
-                                                                const float4* __restrict__ x) {
+// 8 waves per SIMD (64 VGPRs): the traversal waits on dependent loads
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void
+MultiEeGenerateKernel(const InstContactDev* ic, const int32_t* off, int n,
+                      const float4* __restrict__ x) {
   XRT_MULTI_INDEX(kEdge, c.ne)
   if (!c.has_obs) return;
   EeDcdGenerateBody(x + c.vbeg, c.cloth.edges, i, c.obvh, c.obs.rtri, c.ee);
@@ -1203,13 +1204,13 @@ __global__ __launch_bounds__(kBlock) void MultiAddTotalsKernel(const InstContact
 
 // repulsion tests of every instance, one VF (is_ee = 0) or EE item per thread; instances
 // whose list is not usable (off, overflowed, or pairs beyond the pair capacity) are left to
-// MultiRepPairsKernel
+// MultiRepPairsKernel.  Eight waves per SIMD (64 VGPRs; the spills fall in the response
+// path): the tests wait on dependent gathers (item -> edges -> positions), and the EE launch
+// measured 1.70 ms at 86 VGPRs (5 waves), 1.50 at 80, 1.37 at 72, 1.31 ms at 64.
 template <int is_ee>
-__global__ __launch_bounds__(kBlock) void MultiRepItemKernel(const InstContactDev* ic,
-                                                             const int32_t* off, int n,
-                                                             const float4* __restrict__ x,
-                                                             const float4* __restrict__ x0,
-                                                             float h) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void
+MultiRepItemKernel(const InstContactDev* ic, const int32_t* off, int n,
+                   const float4* __restrict__ x, const float4* __restrict__ x0, float h) {
   const int lblk = XcdBlock(off, n);
   if (lblk < 0) return;
   const int k = InstOfBlock(off, n, lblk);
