@@ -242,6 +242,7 @@ struct SavedRecords {
   DevBuf<int32_t> vf_face, rt_face;
   DevBuf<float4> vf_q, rt_q;
   DevBuf<char> ee;
+  DevBuf<uint8_t> ee_none;
 };
 
 struct SavedRun {
@@ -270,6 +271,7 @@ int CopyRecords(SavedRecords& dst, const ContactData& cd, hipStream_t s) {
   XRT_CHECK_HIP(CopyBuf(dst.vf_q, cd.vf_q, s, true));
   XRT_CHECK_HIP(CopyBuf(dst.rt_q, cd.rt_q, s, true));
   XRT_CHECK_HIP(CopyBuf(dst.ee, cd.ee, s, true));
+  XRT_CHECK_HIP(CopyBuf(dst.ee_none, cd.ee_none, s, true));
   return XRT_OK;
 }
 
@@ -279,6 +281,7 @@ int RestoreRecords(ContactData& cd, const SavedRecords& src, hipStream_t s) {
   XRT_CHECK_HIP(CopyBuf(cd.vf_q, src.vf_q, s, false));
   XRT_CHECK_HIP(CopyBuf(cd.rt_q, src.rt_q, s, false));
   XRT_CHECK_HIP(CopyBuf(cd.ee, src.ee, s, false));
+  XRT_CHECK_HIP(CopyBuf(cd.ee_none, src.ee_none, s, false));
   return XRT_OK;
 }
 

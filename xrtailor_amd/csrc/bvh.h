@@ -37,6 +37,7 @@ struct BvhView {
   uint32_t* offsets = nullptr;  //                 their exclusive prefix
   int32_t* total = nullptr;     //                 and their sum (1 int)
   Box* packed = nullptr;        // traversal records (PackNodeBody), batched pair refresh
+  int32_t* lvl = nullptr;       // LevelBottomUpWG's depth lists (2 nf - 1 ints)
 };
 
 // Owning storage of one LBVH plus its build scratch.
@@ -56,6 +57,7 @@ struct Lbvh {
   DevBuf<uint32_t> hist;  // radix histogram / offsets
   DevBuf<int32_t> counts, offsets, total;  // pair traversal scratch (per lhs leaf)
   DevBuf<Box> packed;                      // traversal records of the nodes
+  DevBuf<int32_t> lvl;                     // depth lists of the level-by-level merge
   BvhView view() const;
   // (re)allocate for a mesh; uploads faces (host int32[3*nf])
   hipError_t Setup(const int32_t* tris_host, int nf, int nv, hipStream_t s);

@@ -565,6 +565,7 @@ BvhView Lbvh::view() const {
   t.offsets = reinterpret_cast<uint32_t*>(offsets.ptr);
   t.total = total.ptr;
   t.packed = packed.ptr;
+  t.lvl = lvl.ptr;
   return t;
 }
 
@@ -592,6 +593,7 @@ hipError_t Lbvh::Setup(const int32_t* tris_host, int nfaces, int nverts, hipStre
   if ((e = escape.Alloc(nn)) != hipSuccess) return e;
   if ((e = packed.Alloc(nn)) != hipSuccess) return e;
   if ((e = flags.Alloc(std::max(nf, 1))) != hipSuccess) return e;
+  if ((e = lvl.Alloc(nn)) != hipSuccess) return e;
   if ((e = root.Alloc(6)) != hipSuccess) return e;
   if ((e = hist.Alloc(static_cast<size_t>(256) * nblocks)) != hipSuccess) return e;
   if ((e = counts.Alloc(std::max(nf, 1))) != hipSuccess) return e;

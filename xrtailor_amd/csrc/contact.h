@@ -61,8 +61,10 @@ hipError_t LaunchVfDcdGenerate(hipStream_t s, const float4* x, int n, const Lbvh
                                int32_t* face, float4* q);
 hipError_t LaunchVfCcdGenerate(hipStream_t s, const float4* x0, const float4* x, int n,
                                const Lbvh& obs, int32_t* face, float4* q);
+// none[e] = 1 where edge e got no record (rec[e].a.x > 1e6), read by the EE solve first
 hipError_t LaunchEeDcdGenerate(hipStream_t s, const float4* x, const int2* edges, int ne,
-                               const Lbvh& obs, const uint32_t* obs_rtri, EeRec* rec);
+                               const Lbvh& obs, const uint32_t* obs_rtri, EeRec* rec,
+                               uint8_t* none);
 // friction of the VF solves (mu = 0: none, the reference's behaviour): x0 the cloth's
 // start-of-substep positions, o_x0 / o_x the obstacle's frame-start / frame-end positions,
 // inv_ns = 1 / substeps (the obstacle moves (o_x - o_x0) inv_ns per substep)
@@ -76,7 +78,7 @@ struct VfFriction {
 hipError_t LaunchVfSolve(hipStream_t s, float4* x, int n, const Lbvh& obs, const int32_t* face,
                          const float4* q, float offset, const VfFriction& fr);
 hipError_t LaunchEeSolve(hipStream_t s, const float4* x, const int2* edges, int ne,
-                         const EeRec* rec, long long* dq, int32_t* cnt);
+                         const EeRec* rec, const uint8_t* none, long long* dq, int32_t* cnt);
 hipError_t LaunchApplyFixedDeltas(hipStream_t s, float4* x, int n, long long* dq, int32_t* cnt,
                                   float relax);
 // pairs of (cloth face, other face): entries < n_obs_pairs (read on device) are cloth vs
@@ -115,6 +117,7 @@ struct InstContactDev {
   float4* vf_q = nullptr;
   float4* rt_q = nullptr;
   EeRec* ee = nullptr;
+  uint8_t* ee_none = nullptr;
   long long* dq = nullptr;
   int32_t* cnt = nullptr;
   int2* pairs = nullptr;

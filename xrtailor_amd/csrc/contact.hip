@@ -186,7 +186,8 @@ __device__ __forceinline__ void EeDcdGenerateBody(const float4* __restrict__ x,
                                                   const int2* __restrict__ edges, int e,
                                                   const BvhView& t,
                                                   const uint32_t* __restrict__ rtri,
-                                                  EeRec* __restrict__ rec) {
+                                                  EeRec* __restrict__ rec,
+                                                  uint8_t* __restrict__ none) {
   const int2 ed = edges[e];
   const float4 pc4 = x[ed.x], qc4 = x[ed.y];
   const F3 pc = L3(pc4), qc = L3(qc4);
@@ -213,6 +214,7 @@ __device__ __forceinline__ void EeDcdGenerateBody(const float4* __restrict__ x,
     }
   });
   rec[e] = r;
+  none[e] = r.a.x > 1e6f ? 1 : 0;
 }
 
 // SolveVFDCD_Kernel / SolveVFCCD_Kernel (PBDCollision.cu:54-76,280-300)
@@ -260,10 +262,14 @@ __device__ __forceinline__ void VfSolveBody(float4* __restrict__ x, int i, const
 }
 
 // SolveEEDCD_Kernel (PBDCollision.cu:155-214)
+// none[e] mirrors `rec[e].a.x > 1e6f` (no record), one byte per edge, so that the edges
+// without a record, most of them, cost no read of their 64-byte record
 __device__ __forceinline__ void EeSolveBody(const float4* __restrict__ x,
                                             const int2* __restrict__ edges, int e,
                                             const EeRec* __restrict__ rec,
+                                            const uint8_t* __restrict__ none,
                                             long long* __restrict__ dq, int32_t* __restrict__ cnt) {
+  if (none[e]) return;
   const EeRec r = rec[e];
   if (r.a.x > 1e6f) return;
   const int2 ed = edges[e];
@@ -646,9 +652,10 @@ __global__ __launch_bounds__(kBlock) void EeDcdGenerateKernel(const float4* __re
                                                               const int2* __restrict__ edges, int ne,
                                                               BvhView t,
                                                               const uint32_t* __restrict__ rtri,
-                                                              EeRec* __restrict__ rec) {
+                                                              EeRec* __restrict__ rec,
+                                                              uint8_t* __restrict__ none) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < ne) EeDcdGenerateBody(x, edges, e, t, rtri, rec);
+  if (e < ne) EeDcdGenerateBody(x, edges, e, t, rtri, rec, none);
 }
 
 __global__ __launch_bounds__(kBlock) void VfSolveKernel(float4* __restrict__ x, int n, BvhView t,
@@ -662,10 +669,11 @@ __global__ __launch_bounds__(kBlock) void VfSolveKernel(float4* __restrict__ x, 
 __global__ __launch_bounds__(kBlock) void EeSolveKernel(const float4* __restrict__ x,
                                                         const int2* __restrict__ edges, int ne,
                                                         const EeRec* __restrict__ rec,
+                                                        const uint8_t* __restrict__ none,
                                                         long long* __restrict__ dq,
                                                         int32_t* __restrict__ cnt) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < ne) EeSolveBody(x, edges, e, rec, dq, cnt);
+  if (e < ne) EeSolveBody(x, edges, e, rec, none, dq, cnt);
 }
 
 __global__ __launch_bounds__(kBlock) void ApplyFixedKernel(float4* __restrict__ x, int n,
@@ -932,7 +940,7 @@ __global__ __launch_bounds__(kBlock) void MultiObsGrowKernel(const InstContactDe
 __global__ __launch_bounds__(1024) void MultiObsBottomUpKernel(const InstContactDev* ic) {
   const InstContactDev& c = ic[blockIdx.x];
   if (c.has_obs && c.obs_nf > 1)
-    BottomUpTreeWG(c.obvh.nodes, c.obvh.box, c.obvh.flags, c.obs_nf, 0);
+    LevelBottomUpWG(c.obvh.nodes, c.obvh.box, c.obvh.lvl, c.obs_nf, 0);
 }
 
 __global__ __launch_bounds__(kBlock) void MultiVfGenerateKernel(const InstContactDev* ic,
@@ -951,7 +959,7 @@ MultiEeGenerateKernel(const InstContactDev* ic, const int32_t* off, int n,
                       const float4* __restrict__ x) {
   XRT_MULTI_INDEX(kEdge, c.ne)
   if (!c.has_obs) return;
-  EeDcdGenerateBody(x + c.vbeg, c.cloth.edges, i, c.obvh, c.obs.rtri, c.ee);
+  EeDcdGenerateBody(x + c.vbeg, c.cloth.edges, i, c.obvh, c.obs.rtri, c.ee, c.ee_none);
 }
 
 // cloth LBVH Update(ccd): pred = x, then leaves grow by bounds(x, x0)
@@ -974,7 +982,7 @@ __global__ __launch_bounds__(kBlock) void MultiClothGrowKernel(const InstContact
 // one workgroup per instance merges its cloth tree (grow-only)
 __global__ __launch_bounds__(1024) void MultiClothBottomUpKernel(const InstContactDev* ic) {
   const InstContactDev& c = ic[blockIdx.x];
-  if (c.scal && c.nf > 1) BottomUpTreeWG(c.cbvh.nodes, c.cbvh.box, c.cbvh.flags, c.nf, 1);
+  if (c.scal && c.nf > 1) LevelBottomUpWG(c.cbvh.nodes, c.cbvh.box, c.cbvh.lvl, c.nf, 1);
 }
 
 // frame start: one workgroup per tree merges the freshly built trees of every instance
@@ -986,9 +994,9 @@ __global__ __launch_bounds__(1024) void MultiBuildMergeKernel(const InstContactD
   if (!c.scal) return;
   if (obs) {
     if (c.has_obs && c.obs_nf > 1)
-      BottomUpTreeWG(c.obvh.nodes, c.obvh.box, c.obvh.flags, c.obs_nf, 0);
+      LevelBottomUpWG(c.obvh.nodes, c.obvh.box, c.obvh.lvl, c.obs_nf, 0);
   } else if (cloth && c.nf > 1) {
-    BottomUpTreeWG(c.cbvh.nodes, c.cbvh.box, c.cbvh.flags, c.nf, 0);
+    LevelBottomUpWG(c.cbvh.nodes, c.cbvh.box, c.cbvh.lvl, c.nf, 0);
   }
 }
 
@@ -1443,7 +1451,7 @@ __global__ __launch_bounds__(kBlock) void MultiEeSolveKernel(const InstContactDe
                                                              const float4* __restrict__ x) {
   XRT_MULTI_INDEX(kEdge, c.ne)
   if (!c.has_obs) return;
-  EeSolveBody(x + c.vbeg, c.cloth.edges, i, c.ee, c.dq, c.cnt);
+  EeSolveBody(x + c.vbeg, c.cloth.edges, i, c.ee, c.ee_none, c.dq, c.cnt);
 }
 
 __global__ __launch_bounds__(kBlock) void MultiObsCopyKernel(const InstContactDev* ic,
@@ -1473,9 +1481,11 @@ hipError_t LaunchVfCcdGenerate(hipStream_t s, const float4* x0, const float4* x,
   return hipGetLastError();
 }
 hipError_t LaunchEeDcdGenerate(hipStream_t s, const float4* x, const int2* edges, int ne,
-                               const Lbvh& obs, const uint32_t* obs_rtri, EeRec* rec) {
+                               const Lbvh& obs, const uint32_t* obs_rtri, EeRec* rec,
+                               uint8_t* none) {
   if (ne <= 0 || !obs.active) return hipSuccess;
-  EeDcdGenerateKernel<<<Blocks(ne), kBlock, 0, s>>>(x, edges, ne, obs.view(), obs_rtri, rec);
+  EeDcdGenerateKernel<<<Blocks(ne), kBlock, 0, s>>>(x, edges, ne, obs.view(), obs_rtri, rec,
+                                                    none);
   return hipGetLastError();
 }
 hipError_t LaunchVfSolve(hipStream_t s, float4* x, int n, const Lbvh& obs, const int32_t* face,
@@ -1485,9 +1495,9 @@ hipError_t LaunchVfSolve(hipStream_t s, float4* x, int n, const Lbvh& obs, const
   return hipGetLastError();
 }
 hipError_t LaunchEeSolve(hipStream_t s, const float4* x, const int2* edges, int ne,
-                         const EeRec* rec, long long* dq, int32_t* cnt) {
+                         const EeRec* rec, const uint8_t* none, long long* dq, int32_t* cnt) {
   if (ne <= 0) return hipSuccess;
-  EeSolveKernel<<<Blocks(ne), kBlock, 0, s>>>(x, edges, ne, rec, dq, cnt);
+  EeSolveKernel<<<Blocks(ne), kBlock, 0, s>>>(x, edges, ne, rec, none, dq, cnt);
   return hipGetLastError();
 }
 hipError_t LaunchApplyFixedDeltas(hipStream_t s, float4* x, int n, long long* dq, int32_t* cnt,

@@ -178,6 +178,8 @@ int ContactStart(xrt_ctx* ctx) {
     XRT_CHECK_HIP(c->rt_q.Alloc(h.nv));
     XRT_CHECK_HIP(c->ee.Alloc(sizeof(EeRec) * std::max(h.ne(), 1)));
     XRT_CHECK_HIP(hipMemsetAsync(c->ee.ptr, 0, c->ee.n, s));  // no records before a pass
+    XRT_CHECK_HIP(c->ee_none.Alloc(std::max(h.ne(), 1)));      // (zeroed records: none = 0)
+    XRT_CHECK_HIP(hipMemsetAsync(c->ee_none.ptr, 0, c->ee_none.n, s));
     XRT_CHECK_HIP(c->dq.Alloc(3 * static_cast<size_t>(h.nv)));
     XRT_CHECK_HIP(c->cnt.Alloc(h.nv));
     XRT_CHECK_HIP(hipMemsetAsync(c->dq.ptr, 0, sizeof(long long) * 3 * h.nv, s));
@@ -210,6 +212,7 @@ int ContactStateReset(xrt_ctx* ctx, int inst) {
   XRT_CHECK_HIP(hipMemsetAsync(c.vf_face.ptr, 0xff, sizeof(int32_t) * h.nv, s));
   XRT_CHECK_HIP(hipMemsetAsync(c.rt_face.ptr, 0xff, sizeof(int32_t) * h.nv, s));
   XRT_CHECK_HIP(hipMemsetAsync(c.ee.ptr, 0, c.ee.n, s));
+  XRT_CHECK_HIP(hipMemsetAsync(c.ee_none.ptr, 0, c.ee_none.n, s));
   if (ctx->params.enable_self_contact)
     XRT_CHECK_HIP(LbvhBuild(c.cloth_bvh, ctx->x0.ptr + h.desc.vbeg, nullptr, nullptr, 0.0f, 0,
                             0.0f, s));
@@ -271,7 +274,7 @@ int PhaseGenerate(xrt_ctx* ctx, int inst, int kind) {
   if (kind == 1) XRT_CHECK_HIP(LaunchVfCcdGenerate(s, x0, x, h.nv, c.obs_bvh, c.rt_face.ptr, c.rt_q.ptr));
   if (kind == 2)
     XRT_CHECK_HIP(LaunchEeDcdGenerate(s, x, c.c_edges.ptr, h.ne(), c.obs_bvh, c.o_rtri.ptr,
-                                      reinterpret_cast<EeRec*>(c.ee.ptr)));
+                                      reinterpret_cast<EeRec*>(c.ee.ptr), c.ee_none.ptr));
   return XRT_OK;
 }
 
@@ -361,7 +364,8 @@ int ContactEeSolve(xrt_ctx* ctx, int inst) {
   const HostInstance& h = ctx->inst[inst];
   float4* x = ctx->x.ptr + h.desc.vbeg;
   EeRec* ee = reinterpret_cast<EeRec*>(c.ee.ptr);
-  XRT_CHECK_HIP(LaunchEeSolve(ctx->stream, x, c.c_edges.ptr, h.ne(), ee, c.dq.ptr, c.cnt.ptr));
+  XRT_CHECK_HIP(LaunchEeSolve(ctx->stream, x, c.c_edges.ptr, h.ne(), ee, c.ee_none.ptr, c.dq.ptr,
+                              c.cnt.ptr));
   XRT_CHECK_HIP(LaunchApplyFixedDeltas(ctx->stream, x, h.desc.nv, c.dq.ptr, c.cnt.ptr,
                                        ctx->params.relaxation_factor));
   return XRT_OK;
@@ -450,6 +454,7 @@ int MultiContactSetup(xrt_ctx* ctx) {
       d.vf_q = c.vf_q.ptr;
       d.rt_q = c.rt_q.ptr;
       d.ee = reinterpret_cast<EeRec*>(c.ee.ptr);
+      d.ee_none = c.ee_none.ptr;
       d.dq = c.dq.ptr;
       d.cnt = c.cnt.ptr;
       d.pairs = c.pairs.ptr;
@@ -899,7 +904,8 @@ int xrt_ee_dcd_solve(xrt_ctx* ctx, int32_t inst) {
   ContactData& c = *ctx->contact[inst];
   const HostInstance& h = ctx->inst[inst];
   XRT_CHECK_HIP(LaunchEeSolve(ctx->stream, ctx->x.ptr + h.desc.vbeg, c.c_edges.ptr, h.ne(),
-                              reinterpret_cast<EeRec*>(c.ee.ptr), c.dq.ptr, c.cnt.ptr));
+                              reinterpret_cast<EeRec*>(c.ee.ptr), c.ee_none.ptr, c.dq.ptr,
+                              c.cnt.ptr));
   return XRT_OK;
 }
 

@@ -86,6 +86,7 @@ struct ContactData {
   DevBuf<int32_t> vf_face, rt_face;
   DevBuf<float4> vf_q, rt_q;
   DevBuf<char> ee;  // EeRec[E]
+  DevBuf<uint8_t> ee_none;  // per edge: 1 when its EE record is empty (a.x > 1e6)
   DevBuf<long long> dq;
   DevBuf<int32_t> cnt;
   // face pairs of the repulsion (cloth face, other face)
