@@ -124,14 +124,16 @@ struct InstContactDev {
   // [1] obstacle pairs, [2] all pairs, [3] / [4] VF / EE items, [5] / [6] VF / EE items of
   // the obstacle pairs, [7] / [8] of the cloth pairs
   int32_t* scal = nullptr;
-  // repulsion test items rebuilt at each pair refresh (capacities 0: lists off), in leaf
-  // order; ivf / iee: per cloth leaf, the item counts of the current traversal pass and
-  // then their exclusive prefix
+  // repulsion test items rebuilt at each pair refresh (capacities 0: lists off): each wave of
+  // 64 cloth leaves owns the range its leaves' counts add up to, from its first leaf's
+  // offset; ivf / iee: per cloth leaf, the block-local exclusive prefix of the item counts of
+  // the current traversal pass; nitem: the leaf's own counts (VF, EE)
   int vf_cap = 0, ee_cap = 0;
   int2* rep_vf = nullptr;
   int2* rep_ee = nullptr;
   uint32_t* ivf = nullptr;
   uint32_t* iee = nullptr;
+  uint2* nitem = nullptr;
   // pair refresh: the first kPairStage hits of each cloth leaf, hit j of leaf i at
   // stage[j * nf + i] as (rhs object, item mask) (null: the pairs are found by a second
   // traversal)

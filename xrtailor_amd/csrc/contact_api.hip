@@ -129,6 +129,7 @@ int AllocPairLists(xrt_ctx* ctx, const HostInstance& h, ContactData& c) {
     XRT_CHECK_HIP(c.rep_ee.Alloc(c.ee_cap));
     if (!c.ivf.ptr) XRT_CHECK_HIP(c.ivf.Alloc(std::max(h.nt, 1)));
     if (!c.iee.ptr) XRT_CHECK_HIP(c.iee.Alloc(std::max(h.nt, 1)));
+    if (!c.nitem.ptr) XRT_CHECK_HIP(c.nitem.Alloc(std::max(h.nt, 1)));
   }
   return XRT_OK;
 }
@@ -468,6 +469,7 @@ int MultiContactSetup(xrt_ctx* ctx) {
       d.overflow = ctx->overflow.ptr;
       d.ivf = c.ivf.ptr;
       d.iee = c.iee.ptr;
+      d.nitem = c.nitem.ptr;
       cnt[MultiMaps::kVert] = blocks(h.nv);
       cnt[MultiMaps::kEdge] = blocks(h.ne());
       cnt[MultiMaps::kFace] = blocks(h.nt);

@@ -97,7 +97,8 @@ struct ContactData {
   // repulsion test items of the batched path (capacities 0: lists off)
   int vf_cap = 0, ee_cap = 0;
   DevBuf<int2> rep_vf, rep_ee;
-  DevBuf<uint32_t> ivf, iee;  // per cloth leaf item counts / offsets
+  DevBuf<uint32_t> ivf, iee;  // per cloth leaf item offsets (block-local prefix)
+  DevBuf<uint2> nitem;        // per cloth leaf item counts (VF, EE)
   DevBuf<int2> stage;  // kPairStage hits per cloth leaf of the batched pair refresh
   DevBuf<uint32_t> bsum;  // per-block sums of the batched pair refresh
   int frame_substep = 0, pass_cnt = 0;
