@@ -198,6 +198,8 @@ __device__ __forceinline__ void LeafNodesBody(const uint32_t* __restrict__ morto
   if (i < ni) {
     nodes[i] = make_int4(-1, -1, -1, -1);
     flags[i] = 0;
+  } else {
+    flags[ni] = 0;  // the topology changes: LevelBottomUpWG lists the depths again
   }
 }
 
@@ -593,7 +595,9 @@ hipError_t Lbvh::Setup(const int32_t* tris_host, int nfaces, int nverts, hipStre
   if ((e = escape.Alloc(nn)) != hipSuccess) return e;
   if ((e = packed.Alloc(nn)) != hipSuccess) return e;
   if ((e = flags.Alloc(std::max(nf, 1))) != hipSuccess) return e;
-  if ((e = lvl.Alloc(nn)) != hipSuccess) return e;
+  // flags[nf - 1]: the level lists of LevelBottomUpWG are not yet valid
+  if ((e = hipMemsetAsync(flags.ptr, 0, sizeof(int32_t) * flags.n, s)) != hipSuccess) return e;
+  if ((e = lvl.Alloc(nn + 1)) != hipSuccess) return e;
   if ((e = root.Alloc(6)) != hipSuccess) return e;
   if ((e = hist.Alloc(static_cast<size_t>(256) * nblocks)) != hipSuccess) return e;
   if ((e = counts.Alloc(std::max(nf, 1))) != hipSuccess) return e;

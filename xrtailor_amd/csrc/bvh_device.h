@@ -146,15 +146,33 @@ __device__ __forceinline__ void BottomUpBody(const int4* __restrict__ nodes, Box
 // merge reads finished boxes.  Each thread waits on a handful of loads per level instead of
 // a chain of atomics per climbed node (an arrival-counter climb by one workgroup took 1.5 ms
 // for the batched workload's largest trees), and the boxes are unions (min / max),
-// independent of the order, hence identical to BottomUpBody's.  lvl holds 2 nf - 1 ints;
-// node 0 is the root; a malformed tree cannot list more than nf - 1 nodes.
+// independent of the order, hence identical to BottomUpBody's.  lvl holds 2 nf ints (the
+// last one: the depth count); node 0 is the root; a malformed tree cannot list more than
+// nf - 1 nodes.  The lists depend on the topology only, so they are kept while flags[nf - 1]
+// (a slot the arrival counters do not use) is 1: every build clears it (LeafNodesBody).
 __device__ __forceinline__ void LevelBottomUpWG(const int4* __restrict__ nodes,
                                                 Box* __restrict__ box,
-                                                int32_t* __restrict__ lvl, int nf, int grow) {
+                                                int32_t* __restrict__ lvl,
+                                                int32_t* __restrict__ flags, int nf, int grow) {
   const int ni = nf - 1;
   if (ni <= 0) return;
   int32_t* order = lvl;
   int32_t* start = lvl + ni;  // start[d], d <= depth count <= ni
+  if (flags[ni] == 1) {       // the same value in every thread: written after the lists
+    const int depths = lvl[2 * ni + 1];
+    for (int d = depths - 1; d >= 0; --d) {
+      const int lo = start[d], hi = start[d + 1];
+      for (int t = lo + static_cast<int>(threadIdx.x); t < hi; t += blockDim.x) {
+        const int p = order[t];
+        const int4 pn = nodes[p];
+        Box m = MergeBox(box[pn.y], box[pn.z]);
+        if (grow) m = MergeBox(m, box[p]);
+        box[p] = m;
+      }
+      __syncthreads();
+    }
+    return;
+  }
   __shared__ int s_end;
   if (threadIdx.x == 0) {
     order[0] = 0;
@@ -185,6 +203,10 @@ __device__ __forceinline__ void LevelBottomUpWG(const int4* __restrict__ nodes,
       if (threadIdx.x == 0) start[depths + 1] = e;
       ++depths;
     }
+  }
+  if (threadIdx.x == 0) {
+    lvl[2 * ni + 1] = depths;
+    flags[ni] = 1;  // every thread read the old value before the first barrier above
   }
   __syncthreads();
   for (int d = depths - 1; d >= 0; --d) {

@@ -360,28 +360,36 @@ __device__ __forceinline__ void Accumulate(const RMesh& m0, const RMesh& m1, con
   }
 }
 
-// EvaluateVFProximity (PBDRepulsion.cu:329-409) of vertex `node` against the face with
-// vertices tr (the face id itself is not needed)
-__device__ void EvalVFt(const RMesh& c, int node, const RMesh& o, int4 tr, bool self, float h,
-                        long long* dq, int32_t* cnt, RepOut* out = nullptr) {
-  if (self && (node == tr.x || node == tr.y || node == tr.z)) return;
-  // the inside test on the current state rejects most pairs; it is evaluated before the
-  // start-of-substep side (both are pure, so the order does not change any result).  It is
-  // SignedVF's weights with an early exit: min(-w1, -w2, -w3) >= 1e-6 needs b0, b1, b2 of
-  // one strict sign (a NaN or a zero weight cannot pass either), so that is checked before
-  // the divisions, on the same expressions
-  {
-    const F3 x = c.X(node), y0 = o.X(tr.x), y1 = o.X(tr.y), y2 = o.X(tr.z);
-    F3 n = C(Nz(S(y1, y0)), Nz(S(y2, y0)));
-    if (D(n, n) < 1e-6f) return;  // degenerate: weights 0
-    n = Nz(n);
-    const float b0 = Mixed(S(y1, x), S(y2, x), n);
-    const float b1 = Mixed(S(y2, x), S(y0, x), n);
-    const float b2 = Mixed(S(y0, x), S(y1, x), n);
-    if (!((b0 > 0 && b1 > 0 && b2 > 0) || (b0 < 0 && b1 < 0 && b2 < 0))) return;
-    const float w1 = -b0 / (b0 + b1 + b2), w2 = -b1 / (b0 + b1 + b2), w3 = -b2 / (b0 + b1 + b2);
-    if (!(fminf(-w1, fminf(-w2, -w3)) >= 1e-6f)) return;
-  }
+// The inside test of EvaluateVFProximity on the current state, which rejects most pairs;
+// it is evaluated before the start-of-substep side (both are pure, so the order does not
+// change any result).  It is SignedVF's weights with an early exit: min(-w1, -w2, -w3) >=
+// 1e-6 needs b0, b1, b2 of one strict sign (a NaN or a zero weight cannot pass either), so
+// that is checked before the divisions, on the same expressions.  VfSigns is the part
+// before the divisions (num receives b0, b1, b2), VfWeights the divisions and the bound.
+__device__ __forceinline__ bool VfSigns(int node, int4 tr, bool self, F3 x, F3 y0, F3 y1, F3 y2,
+                                        float4& num) {
+  if (self && (node == tr.x || node == tr.y || node == tr.z)) return false;
+  F3 n = C(Nz(S(y1, y0)), Nz(S(y2, y0)));
+  if (D(n, n) < 1e-6f) return false;  // degenerate: weights 0
+  n = Nz(n);
+  const float b0 = Mixed(S(y1, x), S(y2, x), n);
+  const float b1 = Mixed(S(y2, x), S(y0, x), n);
+  const float b2 = Mixed(S(y0, x), S(y1, x), n);
+  if (!((b0 > 0 && b1 > 0 && b2 > 0) || (b0 < 0 && b1 < 0 && b2 < 0))) return false;
+  num = make_float4(b0, b1, b2, 0.0f);
+  return true;
+}
+
+__device__ __forceinline__ bool VfWeights(float4 num) {
+  const float b0 = num.x, b1 = num.y, b2 = num.z;
+  const float w1 = -b0 / (b0 + b1 + b2), w2 = -b1 / (b0 + b1 + b2), w3 = -b2 / (b0 + b1 + b2);
+  return fminf(-w1, fminf(-w2, -w3)) >= 1e-6f;
+}
+
+// The rest of EvaluateVFProximity (PBDRepulsion.cu:329-409) for a pair that passed the
+// inside test: the start-of-substep side and the constraint, accumulated into dq / cnt
+__device__ void VfResponse(const RMesh& c, int node, const RMesh& o, int4 tr, float h,
+                           long long* dq, int32_t* cnt, RepOut* out) {
   float w0[4];
   F3 pn;
   const float d0 = SignedVF(c.X0(node), o.X0(tr.x), o.X0(tr.y), o.X0(tr.z), pn, w0);
@@ -408,6 +416,16 @@ __device__ void EvalVFt(const RMesh& c, int node, const RMesh& o, int4 tr, bool 
   const int ids[4] = {node, tr.x, tr.y, tr.z};
   if (out) out->side = side;
   Accumulate<1>(c, o, ids, corr, dq, cnt, out);
+}
+
+// EvaluateVFProximity (PBDRepulsion.cu:329-409) of vertex `node` against the face with
+// vertices tr (the face id itself is not needed)
+__device__ void EvalVFt(const RMesh& c, int node, const RMesh& o, int4 tr, bool self, float h,
+                        long long* dq, int32_t* cnt, RepOut* out = nullptr) {
+  float4 num;
+  if (!VfSigns(node, tr, self, c.X(node), o.X(tr.x), o.X(tr.y), o.X(tr.z), num)) return;
+  if (!VfWeights(num)) return;
+  VfResponse(c, node, o, tr, h, dq, cnt, out);
 }
 
 __device__ __forceinline__ void EvalVF(const RMesh& c, int node, const RMesh& o, int face,
@@ -589,41 +607,6 @@ __device__ __forceinline__ uint32_t PairItemMask(int4 f0, uint32_t r0, int4 f1, 
 
 constexpr int kSelfBit = static_cast<int>(0x80000000u);  // item.y: cloth-vs-cloth item
 
-// one VF item: cloth vertex it.x against face it.y of the obstacle or the cloth
-// (the two meshes are separate inlined calls: selecting one of two structs by reference
-// would keep both in scratch memory)
-__device__ __forceinline__ void RepulsionVfItemBody(const RMesh& c, const RMesh& o, int2 it,
-                                                    float h, long long* __restrict__ dq,
-                                                    int32_t* __restrict__ cnt) {
-  const int f = it.y & 0x7fffffff;
-  if (it.x < 0 || it.x >= c.t.nv) return;
-  if (it.y < 0) {
-    if (f < c.t.nf) EvalVF(c, it.x, c, f, true, h, dq, cnt);
-  } else {
-    if (f < o.t.nf) EvalVF(c, it.x, o, f, false, h, dq, cnt);
-  }
-}
-
-template <bool kSelf>
-__device__ __forceinline__ void EeItem(const RMesh& c, const RMesh& other, int e0, int e1,
-                                       float h, long long* __restrict__ dq,
-                                       int32_t* __restrict__ cnt) {
-  if (e1 >= other.t.ne) return;
-  const int2 ea = c.t.edges[e0], eb = other.t.edges[e1];
-  const F3 ua = Nz(S(c.X(ea.y), c.X(ea.x))), ub = Nz(S(other.X(eb.y), other.X(eb.x)));
-  EvalEE(c, e0, other, e1, kSelf, h, ua, ub, dq, cnt);
-}
-
-// one EE item: cloth edge it.x against edge it.y of the obstacle or the cloth
-__device__ __forceinline__ void RepulsionEeItemBody(const RMesh& c, const RMesh& o, int2 it,
-                                                    float h, long long* __restrict__ dq,
-                                                    int32_t* __restrict__ cnt) {
-  if (it.x < 0 || it.x >= c.t.ne) return;
-  const int e1 = it.y & 0x7fffffff;
-  if (it.y < 0) EeItem<true>(c, c, it.x, e1, h, dq, cnt);
-  else EeItem<false>(c, o, it.x, e1, h, dq, cnt);
-}
-
 // ------------------------------------------------------- single-instance kernels
 __global__ __launch_bounds__(kBlock) void InterpolateKernel(const float4* __restrict__ x0,
                                                             const float4* __restrict__ x,
@@ -778,6 +761,76 @@ __device__ __forceinline__ void EeItemRespond(const RMesh& c, const RMesh& o, co
     const int2 b = o.t.edges[e1];
     EEResponse(c, a, o, b, c.X(a.x), c.X(a.y), o.X(b.x), o.X(b.y), e.e0, e1, h, bw, dq, cnt,
                out);
+  }
+}
+
+// A VF item that passed the sign part of the inside test, queued for the rest (the weights'
+// divisions and lower bound) and its response: the cloth vertex, the face (bit 31: a cloth
+// face) and the weights' numerators b0, b1, b2
+struct VfQueued {
+  int node, f;
+  float4 num;
+};
+
+// the sign part of the inside test of VF item `it` (cloth vertex it.x against face it.y of
+// the obstacle or the cloth); ent receives the queued form when it passes
+__device__ __forceinline__ bool VfItemInside(const RMesh& c, const RMesh& o, int2 it,
+                                             VfQueued& ent) {
+  if (it.x < 0 || it.x >= c.t.nv) return false;
+  const bool self = it.y < 0;
+  const int f = it.y & 0x7fffffff;
+  if (f >= (self ? c.t.nf : o.t.nf)) return false;
+  const int4* otris = self ? c.t.tris : o.t.tris;  // pointers, not meshes, are selected
+  const float4* ox = self ? c.s.x : o.s.x;
+  const int4 tr = Gld(otris + f);
+  ent.node = it.x;
+  ent.f = it.y;
+  return VfSigns(it.x, tr, self, L3(Gld(c.s.x + it.x)), L3(Gld(ox + tr.x)), L3(Gld(ox + tr.y)),
+                 L3(Gld(ox + tr.z)), ent.num);
+}
+
+// the rest of the inside test and the response of a queued VF item (the two meshes are
+// separate inlined calls: selecting one of two structs by reference would keep both in
+// scratch memory)
+__device__ __forceinline__ void VfItemRespond(const RMesh& c, const RMesh& o, const VfQueued& e,
+                                              float h, long long* __restrict__ dq,
+                                              int32_t* __restrict__ cnt) {
+  if (!VfWeights(e.num)) return;
+  const int f = e.f & 0x7fffffff;
+  if (e.f < 0) VfResponse(c, e.node, c, c.t.tris[f], h, dq, cnt, nullptr);
+  else VfResponse(c, e.node, o, o.t.tris[f], h, dq, cnt, nullptr);
+}
+
+// One wave's share of an item list: 64 inside tests per step (item q of the wave's
+// stride), the survivors queued in LDS (qw: 128 entries); every 64 queued (and at the
+// end) all lanes run responses together, so the rare response path does not idle most of
+// a wave.  Same tests on the same values; the sums are fixed point, so the order of the
+// responses changes nothing.
+template <class Q, class Inside, class Respond>
+__device__ __forceinline__ void QueuedItems(int cnt, int base, int stride, Q* qw, Inside&& inside,
+                                            Respond&& respond) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int qn = 0;  // queued entries (the same in every lane)
+  while (true) {
+    const bool have = base < cnt;  // uniform over the wave
+    if (have) {
+      const int qi = base + lane;
+      Q ent;
+      const bool pass = qi < cnt && inside(qi, ent);
+      const uint64_t m = __ballot(pass);
+      if (pass) qw[qn + __popcll(m & below)] = ent;
+      qn += __popcll(m);
+      base += stride;
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (qn >= 64 || (!have && qn > 0)) {
+      const int take = qn < 64 ? qn : 64;
+      if (lane < take) respond(qw[qn - take + lane]);
+      qn -= take;
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (!have && qn == 0) break;
   }
 }
 
@@ -940,7 +993,7 @@ __global__ __launch_bounds__(kBlock) void MultiObsGrowKernel(const InstContactDe
 __global__ __launch_bounds__(1024) void MultiObsBottomUpKernel(const InstContactDev* ic) {
   const InstContactDev& c = ic[blockIdx.x];
   if (c.has_obs && c.obs_nf > 1)
-    LevelBottomUpWG(c.obvh.nodes, c.obvh.box, c.obvh.lvl, c.obs_nf, 0);
+    LevelBottomUpWG(c.obvh.nodes, c.obvh.box, c.obvh.lvl, c.obvh.flags, c.obs_nf, 0);
 }
 
 __global__ __launch_bounds__(kBlock) void MultiVfGenerateKernel(const InstContactDev* ic,
@@ -982,7 +1035,8 @@ __global__ __launch_bounds__(kBlock) void MultiClothGrowKernel(const InstContact
 // one workgroup per instance merges its cloth tree (grow-only)
 __global__ __launch_bounds__(1024) void MultiClothBottomUpKernel(const InstContactDev* ic) {
   const InstContactDev& c = ic[blockIdx.x];
-  if (c.scal && c.nf > 1) LevelBottomUpWG(c.cbvh.nodes, c.cbvh.box, c.cbvh.lvl, c.nf, 1);
+  if (c.scal && c.nf > 1)
+    LevelBottomUpWG(c.cbvh.nodes, c.cbvh.box, c.cbvh.lvl, c.cbvh.flags, c.nf, 1);
 }
 
 // frame start: one workgroup per tree merges the freshly built trees of every instance
@@ -994,9 +1048,9 @@ __global__ __launch_bounds__(1024) void MultiBuildMergeKernel(const InstContactD
   if (!c.scal) return;
   if (obs) {
     if (c.has_obs && c.obs_nf > 1)
-      LevelBottomUpWG(c.obvh.nodes, c.obvh.box, c.obvh.lvl, c.obs_nf, 0);
+      LevelBottomUpWG(c.obvh.nodes, c.obvh.box, c.obvh.lvl, c.obvh.flags, c.obs_nf, 0);
   } else if (cloth && c.nf > 1) {
-    LevelBottomUpWG(c.cbvh.nodes, c.cbvh.box, c.cbvh.lvl, c.nf, 0);
+    LevelBottomUpWG(c.cbvh.nodes, c.cbvh.box, c.cbvh.lvl, c.cbvh.flags, c.nf, 0);
   }
 }
 
@@ -1304,22 +1358,22 @@ MultiRepItemKernel(const InstContactDev* ic, const int32_t* off, int n,
   const RMesh cm{c.cloth, MeshState{x + c.vbeg, x0 + c.vbeg, -1.0f}};
   const RMesh om{c.obs, MeshState{c.o_x, c.o_x0, 1e-3f}};  // obstacle nodes: inv mass 1e-3
   const int2* items = is_ee ? c.rep_ee : c.rep_vf;
+  const int w = threadIdx.x >> 6;
+  const int base0 = (lblk - off[k]) * kBlock + w * 64, stride = nblk * kBlock;
   if (!is_ee) {
-    for (int q = (lblk - off[k]) * kBlock + threadIdx.x; q < cnt; q += nblk * kBlock)
-      RepulsionVfItemBody(cm, om, items[q], h, c.dq, c.cnt);
+    __shared__ VfQueued vq[kBlock / 64][128];
+    QueuedItems(cnt, base0, stride, vq[w],
+                [&](int qi, VfQueued& e) { return VfItemInside(cm, om, Gld(items + qi), e); },
+                [&](const VfQueued& e) { VfItemRespond(cm, om, e, h, c.dq, c.cnt); });
     return;
   }
-  // EE: each wave runs the inside test on 64 items per step and queues the survivors in
-  // LDS; every 64 queued (and at the end) all lanes run responses together, so the rare
-  // response path does not idle most of a wave.  Same tests on the same values; the sums
-  // are fixed point, so the order of the responses changes nothing.
+  // EE: the same queue, written out (the shared helper compiled to a 13 % slower EE launch)
   __shared__ EeQueued queue[kBlock / 64][128];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   EeQueued* qw = queue[w];
   int qn = 0;  // queued entries (the same in every lane)
-  int base = (lblk - off[k]) * kBlock + w * 64;
-  const int stride = nblk * kBlock;
+  int base = base0;
   while (true) {
     const bool have = base < cnt;  // uniform over the wave
     if (have) {
