@@ -335,7 +335,9 @@ int xrt_set_obstacle_skinning(xrt_ctx* ctx, int32_t inst, const int32_t* ptr, co
                               const float* weight, int32_t n_joints);
 /* linear blend skinning (body.cu:39-58) of every skinned obstacle into its frame positions
  * (as xrt_update_obstacle does from host positions): transforms of every skinned instance
- * in instance order, n_joints x 12 floats each (3x4 row-major [R | t]) */
+ * in instance order, n_joints x 12 floats each (3x4 row-major [R | t]).  The transforms are
+ * copied into pinned staging before the call returns (the caller may reuse its array at
+ * once); the upload and the skinning are queued on the context's stream without waiting. */
 int xrt_skin_obstacles(xrt_ctx* ctx, const float* transforms);
 int xrt_skin_obstacle(xrt_ctx* ctx, int32_t inst, const float* transforms);
 /* QueryNearestBarycentricClothToObstacle (xrtailor/include/xrtailor/physics/

@@ -10,6 +10,7 @@
 // frame positions from joint transforms, like xrt_update_obstacle.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "binding.h"
@@ -60,22 +61,55 @@ int NearestRecords(xrt_ctx* ctx, int inst) {
   return XRT_OK;
 }
 
+// The transforms go through the pinned staging slots, so the call returns without waiting
+// for the stream: the host can queue the next frame while the current one runs.
 int Skin(xrt_ctx* ctx, const MapRef& r, const float* transforms, size_t count, int inst) {
   BindingData& B = ctx->binding;
   hipStream_t s = ctx->stream;
   for (size_t k = 0; k < count; ++k)
     XRT_REQUIRE(std::isfinite(transforms[k]), XRT_ERR_NOT_FINITE, "non-finite joint transform");
   const size_t off = inst < 0 ? 0 : static_cast<size_t>(B.joint_off[inst]) * 12;
-  XRT_CHECK_HIP(hipMemcpyAsync(B.transforms.ptr + off, transforms, sizeof(float) * count,
+  SkinStage& st = B.stage;
+  const size_t per = 12 * static_cast<size_t>(B.total_joints);
+  if (st.per < per) {
+    st.Release();
+    for (int k = 0; k < 2; ++k)
+      XRT_CHECK_HIP(hipEventCreateWithFlags(&st.ev[k], hipEventDisableTiming));
+    XRT_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&st.host), sizeof(float) * 2 * per,
+                                hipHostMallocDefault));
+    st.per = per;
+  }
+  const int slot = st.slot;
+  if (!st.fresh[slot]) XRT_CHECK_HIP(hipEventSynchronize(st.ev[slot]));
+  float* h = st.host + static_cast<size_t>(slot) * st.per + off;
+  std::memcpy(h, transforms, sizeof(float) * count);
+  XRT_CHECK_HIP(hipMemcpyAsync(B.transforms.ptr + off, h, sizeof(float) * count,
                                hipMemcpyHostToDevice, s));
+  XRT_CHECK_HIP(hipEventRecord(st.ev[slot], s));
+  st.fresh[slot] = false;
+  st.slot = slot ^ 1;
   XRT_CHECK_HIP(LaunchLbs(s, B.skin_dev.ptr, B.transforms.ptr, Map(B.skin_maps, r)));
   for (size_t k = 0; k < ctx->inst.size(); ++k)
     if (B.joint_off[k] >= 0 && (inst < 0 || static_cast<int>(k) == inst))
       ctx->contact[k]->obstacle_moving = true;
-  XRT_CHECK_HIP(hipStreamSynchronize(s));
   return XRT_OK;
 }
 }  // namespace
+
+void SkinStage::Release() {
+  for (int k = 0; k < 2; ++k) {
+    if (ev[k]) {
+      (void)hipEventSynchronize(ev[k]);
+      (void)hipEventDestroy(ev[k]);
+    }
+    ev[k] = nullptr;
+    fresh[k] = true;
+  }
+  if (host) (void)hipHostFree(host);
+  host = nullptr;
+  per = 0;
+  slot = 0;
+}
 
 bool BindingsOn(const xrt_ctx* ctx) { return ctx->binding.total > 0; }
 

@@ -135,9 +135,9 @@ struct InstContactDev {
   uint32_t* iee = nullptr;
   uint2* nitem = nullptr;
   // pair refresh: the first kPairStage hits of each cloth leaf, hit j of leaf i at
-  // stage[j * nf + i] as (rhs object, item mask) (null: the pairs are found by a second
-  // traversal)
-  int2* stage = nullptr;
+  // stage[j * nf + i] as rhs object | item mask << kStageObjBits (null: the pairs are found
+  // by a second traversal; also for meshes of 2^kStageObjBits faces or more)
+  uint32_t* stage = nullptr;
   // pair refresh: per 256-leaf block of the count pass, its pair / VF / EE item sums
   // (3 x Blocks(nf)), then their exclusive prefix
   uint32_t* bsum = nullptr;
@@ -147,6 +147,7 @@ struct InstContactDev {
 };
 
 constexpr int kPairStage = 32;
+constexpr int kStageObjBits = 20;  // the item mask (12 bits) takes the rest of a stage entry
 
 // block -> instance maps: per kind, the prefix sums (n + 1 entries, device) of the
 // 256-thread blocks each instance needs, followed by the instance of each block (device),

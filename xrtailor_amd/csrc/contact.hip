@@ -1075,7 +1075,7 @@ __global__ __launch_bounds__(kBlock) void MultiPackObsKernel(const InstContactDe
 // pairs per cloth leaf against the obstacle (self = 0) or the cloth itself (self = 1),
 // traversing the packed records, with the repulsion item counts of those pairs when the
 // item lists are on; the first kPairStage hits of leaf i are kept at stage[j * nf + i] as
-// (rhs object, item mask) (when the instance has a stage)
+// rhs object | item mask << kStageObjBits (when the instance has a stage)
 // exclusive scan of three counters over the 256 threads of a block; tot = the block sums
 __device__ __forceinline__ void BlockScan3(uint32_t v[3], uint32_t tot[3]) {
   __shared__ uint32_t wsum[kBlock / 64][3];
@@ -1121,7 +1121,7 @@ __global__ __launch_bounds__(kBlock) void MultiCountPairsKernel(const InstContac
     int4 f0 = make_int4(0, 0, 0, 0);
     if (items) f0 = c.cloth.trir[lobj];
     const uint32_t r0 = static_cast<uint32_t>(f0.w);
-    int2* st = c.stage + i;
+    uint32_t* st = c.stage + i;
     const int stride = c.nf;
     TraversePacked(self ? c.cbvh.packed : c.obvh.packed, self ? c.nf : c.obs_nf, lb, th,
                    [&](int robj) {
@@ -1133,7 +1133,8 @@ __global__ __launch_bounds__(kBlock) void MultiCountPairsKernel(const InstContac
                        nee += __popc(m >> 3);
                      }
                      if (c.stage && v < static_cast<uint32_t>(kPairStage))
-                       st[static_cast<int64_t>(v) * stride] = make_int2(robj, static_cast<int>(m));
+                       st[static_cast<int64_t>(v) * stride] =
+                           static_cast<uint32_t>(robj) | (m << kStageObjBits);
                      ++v;
                    });
   }
@@ -1267,14 +1268,15 @@ __global__ __launch_bounds__(kBlock) void MultiEmitPairsKernel(const InstContact
   const uint32_t jn = WaveMax(staged ? cnt : 0u);
   for (uint32_t j = 0; j < jn; ++j) {
     const bool have = staged && j < cnt;
-    int2 e = make_int2(0, 0);
+    uint32_t se = 0;
+    if (have) se = c.stage[static_cast<int64_t>(j) * c.nf + i];
+    const int robj = static_cast<int>(se & ((1u << kStageObjBits) - 1u));
     if (have) {
-      e = c.stage[static_cast<int64_t>(j) * c.nf + i];
       const int64_t idx = lo + cnt - 1 - j;
-      if (idx < cap) c.pairs[idx] = make_int2(lobj, e.x);
+      if (idx < cap) c.pairs[idx] = make_int2(lobj, robj);
     }
     if (!items) continue;
-    const uint32_t m = have ? static_cast<uint32_t>(e.y) : 0u;
+    const uint32_t m = se >> kStageObjBits;
     uint32_t tot;
     const uint32_t pre = WaveExclusiveSum(__popc(m & 7u) | (__popc(m >> 3) << 16), &tot);
     int64_t qv = pv + (pre & 0xffffu), qe = pe + (pre >> 16);
@@ -1284,11 +1286,11 @@ __global__ __launch_bounds__(kBlock) void MultiEmitPairsKernel(const InstContact
 #pragma unroll
     for (int q = 0; q < 3; ++q)
       if ((m >> q) & 1u) {
-        if (qv < c.vf_cap) c.rep_vf[qv] = make_int2(Pick3(f0, q), e.x | tag);
+        if (qv < c.vf_cap) c.rep_vf[qv] = make_int2(Pick3(f0, q), robj | tag);
         ++qv;
       }
     if (!(m >> 3)) continue;
-    const int4 fe1 = ot.face_edges[e.x];
+    const int4 fe1 = ot.face_edges[robj];
 #pragma unroll
     for (int q = 0; q < 9; ++q)
       if ((m >> (3 + q)) & 1u) {
@@ -1298,7 +1300,8 @@ __global__ __launch_bounds__(kBlock) void MultiEmitPairsKernel(const InstContact
   }
   if (!trav) return;
   // a leaf with more hits than the stage holds (or no stage: self contact came on after
-  // start) traverses again, in the count pass's hit order
+  // start, or a mesh too large for the stage's object field) traverses again, in the count
+  // pass's hit order
   int64_t index = lo + cnt - 1;
   const int v0[3] = {f0.x, f0.y, f0.z};
   TraversePacked(self ? c.cbvh.packed : c.obvh.packed, self ? c.nf : c.obs_nf,

@@ -191,7 +191,8 @@ int ContactStart(xrt_ctx* ctx) {
     XRT_CHECK_HIP(hipMemsetAsync(c->scal.ptr, 0, sizeof(int32_t) * 16, s));
     rc = AllocPairLists(ctx, h, *c);
     if (rc != XRT_OK) return rc;
-    if (ctx->params.enable_self_contact)
+    if (ctx->params.enable_self_contact && h.nt < (1 << kStageObjBits) &&
+        c->obs_nf < (1 << kStageObjBits))
       XRT_CHECK_HIP(c->stage.Alloc(static_cast<size_t>(kPairStage) * std::max(h.nt, 1)));
     XRT_CHECK_HIP(c->bsum.Alloc(3 * static_cast<size_t>(std::max(1, (h.nt + 255) / 256))));
     ctx->contact[ii] = std::move(c);

@@ -99,7 +99,7 @@ struct ContactData {
   DevBuf<int2> rep_vf, rep_ee;
   DevBuf<uint32_t> ivf, iee;  // per cloth leaf item offsets (block-local prefix)
   DevBuf<uint2> nitem;        // per cloth leaf item counts (VF, EE)
-  DevBuf<int2> stage;  // kPairStage hits per cloth leaf of the batched pair refresh
+  DevBuf<uint32_t> stage;  // kPairStage hits per cloth leaf of the batched pair refresh
   DevBuf<uint32_t> bsum;  // per-block sums of the batched pair refresh
   int frame_substep = 0, pass_cnt = 0;
   bool obstacle_moving = false;  // xrt_update_obstacle since the last frame end
@@ -125,6 +125,22 @@ struct ParticleData {
 };
 
 // Bindings, nearest-face records and obstacle skinning of every instance (binding.h).
+// Pinned two-slot staging of the joint transforms (xrt_skin_obstacle[s]): the upload is
+// queued on the stream without waiting for it; a slot is written again only after the copy
+// issued from it two calls earlier has completed.
+struct SkinStage {
+  float* host = nullptr;  // 2 slots of `per` floats
+  size_t per = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool fresh[2] = {true, true};
+  int slot = 0;
+  SkinStage() = default;
+  SkinStage(const SkinStage&) = delete;
+  SkinStage& operator=(const SkinStage&) = delete;
+  void Release();
+  ~SkinStage() { Release(); }
+};
+
 struct BindingData {
   int total = 0;                   // bindings over every instance
   std::vector<BindInstDev> inst;   // per instance (nb 0 when it has none)
@@ -147,6 +163,7 @@ struct BindingData {
   DevBuf<int32_t> skin_maps;       // obstacle-vertex blocks: every instance, each alone
   MapRef skin_all;
   std::vector<MapRef> skin_one;
+  SkinStage stage;
 };
 
 // Quality-mode FEM constraints of every instance (fem.h has the HBM layout).  Built at
