@@ -146,7 +146,7 @@ struct InstContactDev {
   int32_t* overflow = nullptr;
 };
 
-constexpr int kPairStage = 32;
+constexpr int kPairStage = 128;
 constexpr int kStageObjBits = 20;  // the item mask (12 bits) takes the rest of a stage entry
 
 // block -> instance maps: per kind, the prefix sums (n + 1 entries, device) of the
