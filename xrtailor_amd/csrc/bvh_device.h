@@ -455,4 +455,31 @@ __device__ __forceinline__ int StacklessQuery(const BvhView& t, float4 p, float4
   return found;
 }
 
+// StacklessQuery over the packed traversal records (PackNodeBody): one 32-byte load per
+// visit instead of the box, node and escape arrays; the same visits in the same order
+template <bool kSegment, class F>
+__device__ __forceinline__ int StacklessQueryPacked(const Box* __restrict__ pk, int nf, float4 p,
+                                                    float4 q, F&& f) {
+  int node = 0, found = 0;
+  const int max_steps = 2 * nf;
+  for (int step = 0; node != -1 && step < max_steps; ++step) {
+    if (found >= kMaxHits) break;
+    const Box bx = pk[node];
+    const int code = __float_as_int(bx.lo.w), esc = __float_as_int(bx.hi.w);
+    const bool hit = kSegment ? OverlapSegment(bx, p, q) : OverlapPoint(bx, p);
+    if (hit) {
+      if (code < 0) {
+        f(-code - 1);
+        ++found;
+        node = esc;
+      } else {
+        node = code;
+      }
+    } else {
+      node = esc;
+    }
+  }
+  return found;
+}
+
 }  // namespace xrt

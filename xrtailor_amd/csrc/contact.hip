@@ -137,6 +137,15 @@ __device__ __forceinline__ void InterpolateBody(const float4* __restrict__ x0,
 }
 
 // GenerateVF_DCD_Kernel (PBDCollision.cu:17-43)
+// the obstacle query of the generate bodies: over the packed records when they are current
+// (the batched collision pass packs them after its refit), else over the node arrays
+template <bool kSegment, bool kPk, class F>
+__device__ __forceinline__ void ObsQuery(const BvhView& t, float4 p, float4 q, F&& f) {
+  if constexpr (kPk) StacklessQueryPacked<kSegment>(t.packed, t.nf, p, q, f);
+  else StacklessQuery<kSegment>(t, p, q, f);
+}
+
+template <bool kPk = false>
 __device__ __forceinline__ void VfDcdGenerateBody(const float4* __restrict__ x, int i,
                                                   const BvhView& t, int32_t* __restrict__ face,
                                                   float4* __restrict__ qout) {
@@ -145,7 +154,7 @@ __device__ __forceinline__ void VfDcdGenerateBody(const float4* __restrict__ x, 
   float best = kScalarMax;
   int bi = -1;
   F3 bq{0, 0, 0};
-  StacklessQuery<false>(t, p4, p4, [&](int f) {
+  ObsQuery<false, kPk>(t, p4, p4, [&](int f) {
     const int4 tr = t.tris[f];
     F3 q;
     const float d = NearestPoint(p, L3(t.pred[tr.x]), L3(t.pred[tr.y]), L3(t.pred[tr.z]), q);
@@ -156,6 +165,7 @@ __device__ __forceinline__ void VfDcdGenerateBody(const float4* __restrict__ x, 
 }
 
 // GenerateVFCCD_Kernel (PBDCollision.cu:236-270)
+template <bool kPk = false>
 __device__ __forceinline__ void VfCcdGenerateBody(const float4* __restrict__ x0,
                                                   const float4* __restrict__ x, int i,
                                                   const BvhView& t, int32_t* __restrict__ face,
@@ -169,7 +179,7 @@ __device__ __forceinline__ void VfCcdGenerateBody(const float4* __restrict__ x0,
   if (len < kEpsC) return;
   d = M(d, 1.0f / len);
   float best = kScalarMax;
-  StacklessQuery<true>(t, a4, b4, [&](int f) {
+  ObsQuery<true, kPk>(t, a4, b4, [&](int f) {
     const int4 tr = t.tris[f];
     const float s = RayIntersect(p1, d, len, L3(t.pred[tr.x]), L3(t.pred[tr.y]), L3(t.pred[tr.z]));
     if (s < best) {
@@ -182,6 +192,7 @@ __device__ __forceinline__ void VfCcdGenerateBody(const float4* __restrict__ x0,
 }
 
 // GenerateEE_DCD_Kernel (PBDCollision.cu:100-150)
+template <bool kPk = false>
 __device__ __forceinline__ void EeDcdGenerateBody(const float4* __restrict__ x,
                                                   const int2* __restrict__ edges, int e,
                                                   const BvhView& t,
@@ -194,7 +205,7 @@ __device__ __forceinline__ void EeDcdGenerateBody(const float4* __restrict__ x,
   EeRec r;
   r.a = make_float4(kScalarMax, 0.0f, 0.0f, 0.0f);
   r.p2 = r.q2 = r.fn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  StacklessQuery<true>(t, pc4, qc4, [&](int f) {
+  ObsQuery<true, kPk>(t, pc4, qc4, [&](int f) {
     const int4 tr = t.tris[f];
     const uint32_t info = rtri[f];
     const F3 ov[3] = {L3(t.pred[tr.x]), L3(t.pred[tr.y]), L3(t.pred[tr.z])};
@@ -1002,8 +1013,8 @@ __global__ __launch_bounds__(kBlock) void MultiVfGenerateKernel(const InstContac
                                                                 const float4* __restrict__ x) {
   XRT_MULTI_INDEX(kVert, c.nv)
   if (!c.has_obs) return;
-  VfDcdGenerateBody(x + c.vbeg, i, c.obvh, c.vf_face, c.vf_q);
-  VfCcdGenerateBody(x0 + c.vbeg, x + c.vbeg, i, c.obvh, c.rt_face, c.rt_q);
+  VfDcdGenerateBody<true>(x + c.vbeg, i, c.obvh, c.vf_face, c.vf_q);
+  VfCcdGenerateBody<true>(x0 + c.vbeg, x + c.vbeg, i, c.obvh, c.rt_face, c.rt_q);
 }
 
 // 8 waves per SIMD (64 VGPRs): the traversal waits on dependent loads
@@ -1012,7 +1023,7 @@ MultiEeGenerateKernel(const InstContactDev* ic, const int32_t* off, int n,
                       const float4* __restrict__ x) {
   XRT_MULTI_INDEX(kEdge, c.ne)
   if (!c.has_obs) return;
-  EeDcdGenerateBody(x + c.vbeg, c.cloth.edges, i, c.obvh, c.obs.rtri, c.ee, c.ee_none);
+  EeDcdGenerateBody<true>(x + c.vbeg, c.cloth.edges, i, c.obvh, c.obs.rtri, c.ee, c.ee_none);
 }
 
 // cloth LBVH Update(ccd): pred = x, then leaves grow by bounds(x, x0)
@@ -1675,6 +1686,8 @@ hipError_t MultiObstacleRefit(hipStream_t s, const InstContactDev* ic, const Mul
   MultiNormalPredKernel<<<XcdGrid(m.blocks[MK::kObsVert]), kBlock, 0, s>>>(ic, m.off[MK::kObsVert], m.n, tol);
   MultiObsGrowKernel<<<XcdGrid(m.blocks[MK::kObsFace]), kBlock, 0, s>>>(ic, m.off[MK::kObsFace], m.n);
   MultiObsBottomUpKernel<<<m.n, 1024, 0, s>>>(ic);
+  // the generate pass that follows queries the packed records
+  MultiPackObsKernel<<<XcdGrid(m.blocks[MK::kObsFace]), kBlock, 0, s>>>(ic, m.off[MK::kObsFace], m.n);
   return hipGetLastError();
 }
 
