@@ -1017,9 +1017,7 @@ __global__ __launch_bounds__(kBlock) void MultiVfGenerateKernel(const InstContac
   VfCcdGenerateBody<true>(x0 + c.vbeg, x + c.vbeg, i, c.obvh, c.rt_face, c.rt_q);
 }
 
-// 8 waves per SIMD (64 VGPRs): the traversal waits on dependent loads
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void
-MultiEeGenerateKernel(const InstContactDev* ic, const int32_t* off, int n,
+__global__ __launch_bounds__(kBlock) void MultiEeGenerateKernel(const InstContactDev* ic, const int32_t* off, int n,
                       const float4* __restrict__ x) {
   XRT_MULTI_INDEX(kEdge, c.ne)
   if (!c.has_obs) return;
