@@ -1017,8 +1017,9 @@ __global__ __launch_bounds__(kBlock) void MultiVfGenerateKernel(const InstContac
   VfCcdGenerateBody<true>(x0 + c.vbeg, x + c.vbeg, i, c.obvh, c.rt_face, c.rt_q);
 }
 
-__global__ __launch_bounds__(kBlock) void MultiEeGenerateKernel(const InstContactDev* ic, const int32_t* off, int n,
-                      const float4* __restrict__ x) {
+__global__ __launch_bounds__(kBlock) void MultiEeGenerateKernel(const InstContactDev* ic,
+                                                                const int32_t* off, int n,
+                                                                const float4* __restrict__ x) {
   XRT_MULTI_INDEX(kEdge, c.ne)
   if (!c.has_obs) return;
   EeDcdGenerateBody<true>(x + c.vbeg, c.cloth.edges, i, c.obvh, c.obs.rtri, c.ee, c.ee_none);
