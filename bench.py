@@ -334,6 +334,11 @@ def main() -> int:
                     help="VF repulsion test items per cloth face of the batched item lists")
     ap.add_argument("--ee-items", type=int, default=256,
                     help="EE repulsion test items per cloth face of the batched item lists")
+    ap.add_argument("--bend-model", default="consistent", choices=["consistent", "reference"],
+                    help="bend projection: the consistent model (default) or the reference's "
+                         "formula verbatim (same cost; it gains energy at small compliance)")
+    ap.add_argument("--bend-compliance", type=float, default=None,
+                    help="one bend compliance for every instance (the reference ships 100)")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--plain-frames", type=int, default=1)
     ap.add_argument("--rollout-frames", type=int, default=120,
@@ -410,6 +415,10 @@ def main() -> int:
     quality = a.workload == "quality"
     contacts = (a.workload in ("batched", "swift", "quality", "garment", "garment_body")
                 and not a.no_contacts)
+    if a.bend_compliance is not None:
+        for sp in specs:
+            sp.bend_compliance = a.bend_compliance
+        wname += f", bend compliance {a.bend_compliance:g}"
     garment = a.workload in ("garment", "garment_body")
     bound = garment
     self_contact = contacts and not swift and a.workload != "garment"
@@ -425,7 +434,9 @@ def main() -> int:
     # the timing calls fail with XRT_ERR_CAPACITY).  Test-item lists of twice the default
     # capacities keep the most crumpled cloth on the item path (with 64 / 128 one instance
     # evaluated per pair); results are the same either way.
-    params = make_params(bend_model=1, ground_height=-1.0, enable_obstacle_contact=int(contacts),
+    bend_model = 1 if a.bend_model == "consistent" else 0
+    params = make_params(bend_model=bend_model, ground_height=-1.0,
+                         enable_obstacle_contact=int(contacts),
                          enable_self_contact=int(self_contact),
                          enable_particle_collision=int(swift),
                          pair_capacity_per_face=a.pair_capacity,
@@ -652,7 +663,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": wname, "instances": len(specs), "vertices": n_verts,
                    "substeps": params.num_substeps, "iterations": params.num_iterations,
-                   "bend_model": "consistent", "obstacle_contact": contacts,
+                   "bend_model": a.bend_model, "obstacle_contact": contacts,
                    "self_contact": self_contact,
                    "particle_collision": swift, "bindings": bound,
                    "constraints": "FEM strain + isometric bending" if quality else "XPBD",
