@@ -37,7 +37,7 @@ struct BvhView {
   uint32_t* offsets = nullptr;  //                 their exclusive prefix
   int32_t* total = nullptr;     //                 and their sum (1 int)
   Box* packed = nullptr;        // traversal records (PackNodeBody), batched pair refresh
-  int32_t* lvl = nullptr;       // LevelBottomUpWG's depth lists (2 nf ints)
+  int32_t* lvl = nullptr;       // LevelBottomUpWG's depth and child lists (4 nf ints)
 };
 
 // Owning storage of one LBVH plus its build scratch.

@@ -597,7 +597,7 @@ hipError_t Lbvh::Setup(const int32_t* tris_host, int nfaces, int nverts, hipStre
   if ((e = flags.Alloc(std::max(nf, 1))) != hipSuccess) return e;
   // flags[nf - 1]: the level lists of LevelBottomUpWG are not yet valid
   if ((e = hipMemsetAsync(flags.ptr, 0, sizeof(int32_t) * flags.n, s)) != hipSuccess) return e;
-  if ((e = lvl.Alloc(nn + 1)) != hipSuccess) return e;
+  if ((e = lvl.Alloc(4 * static_cast<size_t>(std::max(nf, 1)))) != hipSuccess) return e;
   if ((e = root.Alloc(6)) != hipSuccess) return e;
   if ((e = hist.Alloc(static_cast<size_t>(256) * nblocks)) != hipSuccess) return e;
   if ((e = counts.Alloc(std::max(nf, 1))) != hipSuccess) return e;

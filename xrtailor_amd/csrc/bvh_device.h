@@ -146,8 +146,9 @@ __device__ __forceinline__ void BottomUpBody(const int4* __restrict__ nodes, Box
 // merge reads finished boxes.  Each thread waits on a handful of loads per level instead of
 // a chain of atomics per climbed node (an arrival-counter climb by one workgroup took 1.5 ms
 // for the batched workload's largest trees), and the boxes are unions (min / max),
-// independent of the order, hence identical to BottomUpBody's.  lvl holds 2 nf ints (the
-// last one: the depth count); node 0 is the root; a malformed tree cannot list more than
+// independent of the order, hence identical to BottomUpBody's.  lvl holds 4 nf ints (the
+// order, the depth starts, the depth count at 2 nf - 1, then each listed node's two
+// children as int2 from 2 nf on); node 0 is the root; a malformed tree cannot list more than
 // nf - 1 nodes.  The lists depend on the topology only, so they are kept while flags[nf - 1]
 // (a slot the arrival counters do not use) is 1: every build clears it (LeafNodesBody).
 __device__ __forceinline__ void LevelBottomUpWG(const int4* __restrict__ nodes,
@@ -158,14 +159,15 @@ __device__ __forceinline__ void LevelBottomUpWG(const int4* __restrict__ nodes,
   if (ni <= 0) return;
   int32_t* order = lvl;
   int32_t* start = lvl + ni;  // start[d], d <= depth count <= ni
+  int2* kids = reinterpret_cast<int2*>(lvl + 2 * nf);  // (first, second) child of order[t]
   if (flags[ni] == 1) {       // the same value in every thread: written after the lists
     const int depths = lvl[2 * ni + 1];
     for (int d = depths - 1; d >= 0; --d) {
       const int lo = start[d], hi = start[d + 1];
       for (int t = lo + static_cast<int>(threadIdx.x); t < hi; t += blockDim.x) {
         const int p = order[t];
-        const int4 pn = nodes[p];
-        Box m = MergeBox(box[pn.y], box[pn.z]);
+        const int2 ch = kids[t];  // loaded beside p: one dependent load less per node
+        Box m = MergeBox(box[ch.x], box[ch.y]);
         if (grow) m = MergeBox(m, box[p]);
         box[p] = m;
       }
@@ -203,6 +205,10 @@ __device__ __forceinline__ void LevelBottomUpWG(const int4* __restrict__ nodes,
       if (threadIdx.x == 0) start[depths + 1] = e;
       ++depths;
     }
+  }
+  for (int t = static_cast<int>(threadIdx.x); t < min(e, ni); t += blockDim.x) {
+    const int4 nd = nodes[order[t]];
+    kids[t] = make_int2(nd.y, nd.z);
   }
   if (threadIdx.x == 0) {
     lvl[2 * ni + 1] = depths;
