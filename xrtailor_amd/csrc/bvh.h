@@ -38,6 +38,7 @@ struct BvhView {
   int32_t* total = nullptr;     //                 and their sum (1 int)
   Box* packed = nullptr;        // traversal records (PackNodeBody), batched pair refresh
   int32_t* lvl = nullptr;       // LevelBottomUpWG's depth and child lists (4 nf ints)
+  int4* leaf_face = nullptr;    // per leaf slot: its face's record (batched pair refresh)
 };
 
 // Owning storage of one LBVH plus its build scratch.
@@ -58,6 +59,7 @@ struct Lbvh {
   DevBuf<int32_t> counts, offsets, total;  // pair traversal scratch (per lhs leaf)
   DevBuf<Box> packed;                      // traversal records of the nodes
   DevBuf<int32_t> lvl;                     // depth lists of the level-by-level merge
+  DevBuf<int4> leaf_face;                  // per leaf slot: its face's record
   BvhView view() const;
   // (re)allocate for a mesh; uploads faces (host int32[3*nf])
   hipError_t Setup(const int32_t* tris_host, int nf, int nv, hipStream_t s);

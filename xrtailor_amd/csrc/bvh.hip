@@ -568,6 +568,7 @@ BvhView Lbvh::view() const {
   t.total = total.ptr;
   t.packed = packed.ptr;
   t.lvl = lvl.ptr;
+  t.leaf_face = leaf_face.ptr;
   return t;
 }
 
@@ -598,6 +599,7 @@ hipError_t Lbvh::Setup(const int32_t* tris_host, int nfaces, int nverts, hipStre
   // flags[nf - 1]: the level lists of LevelBottomUpWG are not yet valid
   if ((e = hipMemsetAsync(flags.ptr, 0, sizeof(int32_t) * flags.n, s)) != hipSuccess) return e;
   if ((e = lvl.Alloc(4 * static_cast<size_t>(std::max(nf, 1)))) != hipSuccess) return e;
+  if ((e = leaf_face.Alloc(std::max(nf, 1))) != hipSuccess) return e;
   if ((e = root.Alloc(6)) != hipSuccess) return e;
   if ((e = hist.Alloc(static_cast<size_t>(256) * nblocks)) != hipSuccess) return e;
   if ((e = counts.Alloc(std::max(nf, 1))) != hipSuccess) return e;

@@ -285,6 +285,33 @@ __device__ __forceinline__ void TraversePacked(const Box* __restrict__ pk, int n
   }
 }
 
+// TraversePacked with the face record of every leaf slot (lf[j], leaf node nf - 1 + j)
+// loaded beside the node's record, so a hit's face data costs no dependent load:
+// on_leaf(rhs object, its face record) in the same order
+template <class OnLeaf>
+__device__ __forceinline__ void TraversePackedFace(const Box* __restrict__ pk,
+                                                   const int4* __restrict__ lf, int nf,
+                                                   const Box& lb, float th, OnLeaf&& on_leaf) {
+  const int ni = nf - 1;
+  int node = 0;
+  const int max_steps = 2 * nf;
+  for (int step = 0; node != -1 && step < max_steps; ++step) {
+    const Box b = pk[node];
+    const int4 f = node >= ni ? lf[node - ni] : make_int4(0, 0, 0, 0);
+    const int code = __float_as_int(b.lo.w), esc = __float_as_int(b.hi.w);
+    if (OverlapDilated(b, lb, th)) {
+      if (code < 0) {
+        on_leaf(-code - 1, f);
+        node = esc;
+      } else {
+        node = code;
+      }
+    } else {
+      node = esc;
+    }
+  }
+}
+
 // FindPairs_Kernel (BVH.cu:1189-1227): leaf id's pairs fill [lo, lo + n) backwards; only
 // slots below cap are written
 __device__ __forceinline__ void FindPairsBody(const BvhView& l, const BvhView& r, float th, int id,

@@ -1071,6 +1071,7 @@ __global__ __launch_bounds__(kBlock) void MultiPackClothKernel(const InstContact
   XRT_MULTI_INDEX(kFace, c.nf)
   PackNodeBody(c.cbvh, c.nf - 1 + i);
   if (i < c.nf - 1) PackNodeBody(c.cbvh, i);
+  c.cbvh.leaf_face[i] = c.cloth.trir[c.cbvh.nodes[c.nf - 1 + i].w];
 }
 
 // traversal records of every obstacle tree
@@ -1080,6 +1081,7 @@ __global__ __launch_bounds__(kBlock) void MultiPackObsKernel(const InstContactDe
   if (!c.has_obs) return;
   PackNodeBody(c.obvh, c.obs_nf - 1 + i);
   if (i < c.obs_nf - 1) PackNodeBody(c.obvh, i);
+  c.obvh.leaf_face[i] = c.obs.trir[c.obvh.nodes[c.obs_nf - 1 + i].w];
 }
 
 // pairs per cloth leaf against the obstacle (self = 0) or the cloth itself (self = 1),
@@ -1127,17 +1129,16 @@ __global__ __launch_bounds__(kBlock) void MultiCountPairsKernel(const InstContac
     const Box lb = c.cbvh.packed[c.nf - 1 + i];
     const int lobj = -__float_as_int(lb.lo.w) - 1;
     const bool items = c.ee_cap > 0;
-    const MeshTopo& ot = self ? c.cloth : c.obs;
     int4 f0 = make_int4(0, 0, 0, 0);
     if (items) f0 = c.cloth.trir[lobj];
     const uint32_t r0 = static_cast<uint32_t>(f0.w);
     uint32_t* st = c.stage + i;
     const int stride = c.nf;
-    TraversePacked(self ? c.cbvh.packed : c.obvh.packed, self ? c.nf : c.obs_nf, lb, th,
-                   [&](int robj) {
+    TraversePackedFace(self ? c.cbvh.packed : c.obvh.packed,
+                       self ? c.cbvh.leaf_face : c.obvh.leaf_face, self ? c.nf : c.obs_nf, lb, th,
+                       [&](int robj, int4 f1) {
                      uint32_t m = 0;
                      if (items) {
-                       const int4 f1 = ot.trir[robj];
                        m = PairItemMask(f0, r0, f1, static_cast<uint32_t>(f1.w), self != 0);
                        nvf += __popc(m & 7u);
                        nee += __popc(m >> 3);
